@@ -1,0 +1,214 @@
+/*
+ * liboecl -- C ABI of the MI355X (gfx950) electron-cloud PIC step.
+ *
+ * Every entry point is extern "C", takes plain device pointers, sizes and a hipStream_t
+ * (passed as void*), returns 0 on success or an OECL_ERR_* code, and never allocates or
+ * frees caller memory unless its comment says it owns a workspace.  The text of the last
+ * failure is available from oecl_last_error().
+ *
+ * Particle storage is one structure-of-arrays block of 6 rows (x, y, vx, vy, vz, w), each
+ * row `stride` elements long, in FP64 (dtype OECL_F64) or FP32 (OECL_F32).  Node fields are
+ * FP64 and laid out like the reference: node index i + j*nxExt, E as [Ex | Ey] and B as
+ * [Bx | By | Bz], each block npExt long.  Charge on the grid is always FP64.
+ *
+ * Each declaration names the reference routine it replaces (file:line in openECLOUD).
+ */
+#ifndef OECL_H
+#define OECL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    OECL_OK = 0,
+    OECL_ERR_ARG = 1,        /* invalid argument / shape */
+    OECL_ERR_HIP = 2,        /* a HIP call or launch failed */
+    OECL_ERR_CAPACITY = 3,   /* output would exceed the capacity passed in */
+    OECL_ERR_NOCONV = 4,     /* iterative solve did not reach its tolerance */
+    OECL_ERR_RANGE = 5       /* particle outside the grid / loader gave up */
+};
+
+enum { OECL_F64 = 0, OECL_F32 = 1 };
+
+/* Wall kinds of particleBoundary.pyx: AbsorbRectangular :77, AbsorbElliptical :225,
+ * AbsorbArbitraryCutCell :302. */
+enum { OECL_WALL_RECT = 0, OECL_WALL_ELLIP = 1, OECL_WALL_CUTCELL = 2 };
+
+/* Grid.__init__ quantities (grid.pyx:57-68): extended node counts and sizes. */
+typedef struct {
+    uint32_t nx, ny, np;        /* nxExt, nyExt, npExt */
+    double dx, dy;
+    double lxHalfExt, lyHalfExt; /* 0.5*lxExt, 0.5*lyExt */
+} oecl_grid_t;
+
+/* Wall description (particleBoundary.pyx:21-36 + the grid arrays the cut-cell wall reads). */
+typedef struct {
+    int32_t kind;
+    double lx, ly;               /* un-extended chamber size (getLx/getLy) */
+    double dx, dy;
+    double lxHalfExt, lyHalfExt;
+    uint32_t nx;                 /* nxExt */
+    const uint16_t *insideFaces;          /* [npExt]          (cut cell only) */
+    const double *boundaryPoints;         /* [nb][2]          (cut cell only) */
+    const int32_t *cutCellPointsInd;      /* [npExt][2]       (cut cell only) */
+    const double *cutCellNormalVectors;   /* [npExt][2]       (cut cell only) */
+} oecl_wall_t;
+
+/* Furman-Pivi model parameters (particleEmitter.pyx:107-152). */
+typedef struct {
+    double p1EInf, p1Ehat, eEHat, w, p, e1, e2, sigmaE;
+    double p1RInf, eR, r, q, r1, r2;
+    double deltaTSHat, s, eHat0, t1, t2, t3, t4;
+    double theta0max, deltaERmax;
+    double epsN[10], pSmallN[10];
+    double particleMass;
+    int32_t scaleSigmaE;
+} oecl_furman_t;
+
+const char *oecl_last_error(void);
+int oecl_device_sync(void);
+
+/* ---- particles.pyx ------------------------------------------------------------------ */
+
+/* particles.pyx:52-56 -> _calcGridWeights :400-418.  w4 holds 4 rows (LL, UL, LU, UU) of
+ * wstride elements of the particle dtype. */
+int oecl_calc_grid_weights(const oecl_grid_t *g, int dtype, const void *data, uint64_t stride,
+                           uint32_t n, uint32_t *inCell, void *w4, uint64_t wstride,
+                           void *stream);
+
+/* particles.pyx:70-77 -> _chargeToGrid :336-351.  Zeroes rho[npExt] and deposits
+ * w_k * weight * q at the four corners (FP64 accumulation). */
+int oecl_charge_to_grid(const oecl_grid_t *g, int dtype, const void *data, uint64_t stride,
+                        uint32_t n, const uint32_t *inCell, const void *w4, uint64_t wstride,
+                        double q, double *rho, void *stream);
+
+/* particles.pyx:58-68 -> _fieldToParticles :354-372 (dims 2) / _fieldToParticles3Dims
+ * :375-397 (dims 3).  out is AoS [n][dims] in the particle dtype. */
+int oecl_field_to_particles(const oecl_grid_t *g, int dtype, const double *field,
+                            const uint32_t *inCell, const void *w4, uint64_t wstride, uint32_t n,
+                            int dims, void *out, void *stream);
+
+/* particles.pyx:79-92 -> _borisPushNoB :324-333 (typeB 0, full kick q/m*dt),
+ * _borisPushConstB :298-321 (typeB 1, bConst[3]) and _borisPush :272-295 (typeB 2, bAt
+ * AoS [n][3]).  eAt AoS [n][2].  All in the particle dtype except bConst (FP64). */
+int oecl_boris_push(int dtype, void *data, uint64_t stride, uint32_t n, const void *eAt,
+                    const void *bAt, const double *bConst, int typeB, double dt, double qm,
+                    void *stream);
+
+/* Fused form of one particle step (particles.pyx:52-92 + particleBoundary indexInside):
+ * bilinear weights in registers, E (and B: typeB 2 from bField[3*npExt]) gathered at the
+ * particle, Boris push, wall test.  Writes isInside[n] (1 = keep) and, when rhoNext is not
+ * NULL, deposits the charge of every particle that stays inside at its NEW position
+ * (the first half of the next step's _chargeToGrid; rhoNext is not zeroed here). */
+int oecl_push_fused(const oecl_grid_t *g, const oecl_wall_t *wall, int dtype, void *data,
+                    uint64_t stride, uint32_t n, const double *E, const double *bField,
+                    const double *bConst, int typeB, double dt, double qm, double q,
+                    uint8_t *isInside, double *rhoNext, void *stream);
+
+/* Deposit without zeroing (rho += ...) of `n` particles, weights in registers.  Used for
+ * the secondaries and after particle management. */
+int oecl_deposit_add(const oecl_grid_t *g, int dtype, const void *data, uint64_t stride,
+                     uint32_t n, double q, double *rho, void *stream);
+
+/* ---- particleBoundary.pyx ----------------------------------------------------------- */
+
+/* indexInside: AbsorbRectangular :84-98, AbsorbElliptical :227-242, AbsorbArbitraryCutCell
+ * :333-370 (which first clips to the grid, :485-508; *clipped is set to 1 if it did). */
+int oecl_index_inside(const oecl_wall_t *wall, int dtype, void *data, uint64_t stride,
+                      uint32_t n, uint8_t *isInside, int32_t *clipped, void *stream);
+
+/* ParticleBoundary.saveAbsorbed :38-65.  Counts absorbed particles (isInside == 0), and
+ * copies them in index order to absorbed (AoS [cap][6], particle dtype).  work must hold
+ * oecl_scan_work_bytes(n) bytes.  *nAbsorbed is written on the device (uint32). */
+uint64_t oecl_scan_work_bytes(uint32_t n);
+int oecl_save_absorbed(int dtype, const void *data, uint64_t stride, uint32_t n,
+                       const uint8_t *isInside, void *absorbed, uint32_t absCap,
+                       uint32_t *nAbsorbed, void *work, void *stream);
+
+/* calculateInteractionPoint + calculateNormalVectors of the three walls
+ * (Rect :108-221, Ellip :252-297, CutCell :373-481) over nAbs absorbed particles (AoS).
+ * remaining[nAbs] (particle dtype) and normals [nAbs][2] are outputs; for the cut cell wall
+ * dt is the last push's time step and *notFound counts particles whose crossing was not
+ * found (the reference prints a warning for each). */
+int oecl_wall_interaction(const oecl_wall_t *wall, int dtype, void *absorbed, uint32_t nAbs,
+                          double dt, void *remaining, void *normals, uint32_t *notFound,
+                          void *stream);
+
+/* ---- particleEmitter.pyx ------------------------------------------------------------ */
+
+/* FurmanEmitter.generateSecondaries :154-308, first loop (:178-229): nSec[nAbs] per impact
+ * plus what the second loop needs.  uniforms: NULL to draw from Philox4x32-10 under
+ * (seed0, seed1) with counter (impact, slot/2, step, 1), else an injected FP64 table
+ * [nAbs][35].  impact[] receives per-impact e0, deltaE, deltaR, pN1 (4 doubles each). */
+int oecl_furman_pass1(const oecl_furman_t *prm, int dtype, const void *absorbed,
+                      const void *normals, uint32_t nAbs, const double *uniforms,
+                      uint32_t seed0, uint32_t seed1, uint32_t step, uint32_t *nSec,
+                      double *impact, void *stream);
+
+/* Exclusive scan of nSec -> offsets[nAbs + 1] (offsets[nAbs] = total). */
+int oecl_scan_u32(const uint32_t *in, uint32_t n, uint32_t *offsetsOut, void *work,
+                  void *stream);
+
+/* Second loop (:231-305): writes the secondaries (AoS [total][6], particle dtype). */
+int oecl_furman_pass2(const oecl_furman_t *prm, int dtype, const void *absorbed,
+                      const void *normals, uint32_t nAbs, const uint32_t *nSec,
+                      const uint32_t *offsets, const double *impact, const double *uniforms,
+                      uint32_t seed0, uint32_t seed1, uint32_t step, void *secondaries,
+                      uint32_t secCap, void *stream);
+
+/* homoLoader (particleEmitter.pyx:41-72): n particles uniform in the chamber by rejection
+ * against the wall, velocities N(0, vsigma) per component, w = weightInit (times
+ * 0.5 + u when randomizeWeight).  Counter-based: particle i uses Philox counter
+ * (i, k, 0, 2).  *failed counts particles that found no inside point in 64 attempts. */
+int oecl_homo_loader(const oecl_wall_t *wall, int dtype, void *data, uint64_t stride,
+                     uint32_t n, double weightInit, double vsigma, int randomizeWeight,
+                     uint32_t seed0, uint32_t seed1, uint32_t *failed, void *stream);
+
+/* ---- Particles.addAndRemoveParticles (particles.pyx:94-121) ------------------------- */
+
+/* Applies the reference's three compaction rules on the SoA block:
+ *   mode 0: _removeParticles :254-269 (nAdd must be 0)
+ *   mode 1: _addAndRemoveParticles :222-252 (in place, holes first filled by additions)
+ *   mode 2: _addAndRemoveParticlesNewArray :201-219 (stable keep, then append)
+ *   mode 3: fast form -- additions fill the first holes, extra additions are appended,
+ *           remaining holes below the new count are filled from the tail (the rule of
+ *           _removeParticles); the same particle set as modes 1 and 2.
+ * add: AoS [nAdd][6] (particle dtype).  spare: a second SoA block of the same stride used
+ * by modes 1 and 2.  work: oecl_compact_work_bytes(n, nAdd) bytes.  The new count is
+ * written to *newCount on the device. */
+uint64_t oecl_compact_work_bytes(uint32_t n, uint32_t nAdd);
+int oecl_add_and_remove(int dtype, void *data, void *spare, uint64_t stride, uint32_t n,
+                        const uint8_t *keep, const void *add, uint32_t nAdd, int mode,
+                        uint32_t *newCount, void *work, void *stream);
+
+/* ---- poissonSolver.pyx -------------------------------------------------------------- */
+
+/* FIT operator of poissonSolver.pyx:44-55 as a 5-point stencil on the full extended grid:
+ * cx[p] = eps0*dsi[p]*dst[p+np] (x-edge p -> p+1), cy[p] = eps0*dsi[p+np]*dst[p]
+ * (y-edge p -> p+nx); mask[p] = insidePoints[p].  Solves A phi = q on inside nodes by
+ * Jacobi-preconditioned CG in FP64 from the initial guess in phi (non-inside nodes are
+ * forced to 0), to ||r|| <= tol*||q_inside||.  work: oecl_pcg_work_bytes(np) bytes.
+ * *iters and *relres are host outputs (this call synchronises the stream). */
+uint64_t oecl_pcg_work_bytes(uint32_t np);
+int oecl_pcg_solve(uint32_t nx, uint32_t np, const double *cx, const double *cy,
+                   const uint8_t *mask, const double *q, double *phi, double tol,
+                   uint32_t maxIter, void *work, uint32_t *iters, double *relres,
+                   void *stream);
+
+/* phiToEAtGridPoints.dot(phi) (poissonSolver.pyx:67,87) as a CSR SpMV with the rows' own
+ * column order: y[r] = sum_k val[k]*x[col[k]], k from rowPtr[r] to rowPtr[r+1]. */
+int oecl_csr_spmv(uint32_t nRows, const int32_t *rowPtr, const int32_t *col,
+                  const double *val, const double *x, double *y, void *stream);
+
+/* q_total = rho + scale * profile (beam.py:68-79 / :129-136 getCharge added to the
+ * electron charge, example01_basic.py:81). */
+int oecl_axpy_rhs(uint32_t np, const double *rho, double scale, const double *profile,
+                  double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OECL_H */

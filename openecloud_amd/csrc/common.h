@@ -1,0 +1,98 @@
+// Shared device/host helpers of liboecl (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/oecl.h"
+
+#define OECL_EXPORT extern "C" __attribute__((visibility("default")))
+
+// Records the last HIP error text for oecl_last_error().
+void oecl_set_error(const char *where, hipError_t e);
+void oecl_set_error_msg(const char *msg);
+
+#define OECL_CHECK_LAUNCH(where)                          \
+    do {                                                  \
+        hipError_t _e = hipGetLastError();                \
+        if (_e != hipSuccess) {                           \
+            oecl_set_error(where, _e);                    \
+            return OECL_ERR_HIP;                          \
+        }                                                 \
+    } while (0)
+
+#define OECL_HIP(call, where)                             \
+    do {                                                  \
+        hipError_t _e = (call);                           \
+        if (_e != hipSuccess) {                           \
+            oecl_set_error(where, _e);                    \
+            return OECL_ERR_HIP;                          \
+        }                                                 \
+    } while (0)
+
+static inline unsigned oecl_blocks(uint64_t n, unsigned per_block)
+{
+    uint64_t b = (n + per_block - 1) / per_block;
+    if (b == 0) b = 1;
+    return (unsigned)b;
+}
+
+// ----------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Bit-identical to the oracle's CPU version.
+// ----------------------------------------------------------------------------------------
+struct Philox4 {
+    uint32_t v[4];
+};
+
+__device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                  uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0;
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    Philox4 r;
+    r.v[0] = c0;
+    r.v[1] = c1;
+    r.v[2] = c2;
+    r.v[3] = c3;
+    return r;
+}
+
+// Uniform in [0, 1) from the 53 high bits of (hi:lo).
+__device__ __forceinline__ double u01_from_words(uint32_t lo, uint32_t hi)
+{
+    uint64_t x = ((uint64_t)hi << 32) | lo;
+    return (double)(x >> 11) * 0x1.0p-53;
+}
+
+// Uniform number k of the stream addressed by (item, purpose, step) under key (k0, k1).
+// Block index k/2 of item `item` holds uniforms 2*(k/2) and 2*(k/2)+1.
+__device__ __forceinline__ double philox_uniform(uint32_t item, uint32_t k, uint32_t step,
+                                                 uint32_t purpose, uint32_t k0, uint32_t k1)
+{
+    Philox4 r = philox4x32_10(item, k >> 1, step, purpose, k0, k1);
+    return (k & 1) ? u01_from_words(r.v[2], r.v[3]) : u01_from_words(r.v[0], r.v[1]);
+}
+
+// Particle component accessor for the (6, stride) SoA block: x, y, vx, vy, vz, w.
+template <typename T>
+struct Soa {
+    T *base;
+    uint64_t stride;
+    __device__ __forceinline__ T &operator()(int c, uint64_t i) const
+    {
+        return base[(uint64_t)c * stride + i];
+    }
+};

@@ -1,0 +1,105 @@
+// Device form of the three particle walls of particleBoundary.pyx.
+#pragma once
+#include "common.h"
+
+struct WallDev {
+    int32_t kind;
+    double lx, ly, dx, dy, lxHalfExt, lyHalfExt;
+    double lxHalf, lyHalf;      // rectangle half sizes
+    double ax, ay;              // 4/lx^2, 4/ly^2 (ellipse)
+    double clipX, clipY;        // cut-cell clip limits, 0.5*lxExt - 1e-6*dx
+    double dxi, dyi;
+    uint32_t nx;
+    const uint16_t *insideFaces;
+    const double *bp;
+    const int32_t *cpi;
+    const double *ccn;
+};
+
+static inline int wall_to_device(const oecl_wall_t *w, WallDev *d)
+{
+    if (!w) return OECL_ERR_ARG;
+    d->kind = w->kind;
+    d->lx = w->lx;
+    d->ly = w->ly;
+    d->dx = w->dx;
+    d->dy = w->dy;
+    d->lxHalfExt = w->lxHalfExt;
+    d->lyHalfExt = w->lyHalfExt;
+    d->lxHalf = w->lx * 0.5;                 // particleBoundary.pyx:88
+    d->lyHalf = w->ly * 0.5;
+    d->ax = 4. / (w->lx * w->lx);            // particleBoundary.pyx:233
+    d->ay = 4. / (w->ly * w->ly);
+    d->clipX = w->lxHalfExt - 1.e-6 * w->dx; // particleBoundary.pyx:349-351
+    d->clipY = w->lyHalfExt - 1.e-6 * w->dy;
+    d->dxi = 1. / w->dx;
+    d->dyi = 1. / w->dy;
+    d->nx = w->nx;
+    d->insideFaces = w->insideFaces;
+    d->bp = w->boundaryPoints;
+    d->cpi = w->cutCellPointsInd;
+    d->ccn = w->cutCellNormalVectors;
+    if (w->kind < 0 || w->kind > 2) return OECL_ERR_ARG;
+    if (w->kind == OECL_WALL_CUTCELL &&
+        (!w->insideFaces || !w->boundaryPoints || !w->cutCellPointsInd ||
+         !w->cutCellNormalVectors))
+        return OECL_ERR_ARG;
+    return OECL_OK;
+}
+
+// Cell of a point as particleBoundary.pyx:353-355 computes it.
+__device__ __forceinline__ uint32_t wall_cell(const WallDev &w, double x, double y)
+{
+    uint32_t ix = (uint32_t)((x + w.lxHalfExt) * w.dxi);
+    uint32_t iy = (uint32_t)((y + w.lyHalfExt) * w.dyi);
+    return ix + iy * w.nx;
+}
+
+// Point-in-cut-cell test of particleBoundary.pyx:357-370 for an already clipped point.
+__device__ __forceinline__ uint8_t cutcell_inside(const WallDev &w, double x, double y)
+{
+    uint32_t c = wall_cell(w, x, y);
+    uint16_t f = w.insideFaces[c];
+    if (f == 8) return 1;
+    if (f == 0) return 0;
+    int32_t a = w.cpi[2 * c], b = w.cpi[2 * c + 1];
+    return ((w.bp[2 * a] - x) * (w.bp[2 * b + 1] - y) >
+            (w.bp[2 * b] - x) * (w.bp[2 * a + 1] - y))
+               ? 1
+               : 0;
+}
+
+// indexInside for one particle.  The cut-cell wall first clips the position to the grid
+// (particleBoundary.pyx:349-351 via _clipParticleDataToGrid :485-508) and writes it back.
+template <typename T>
+__device__ __forceinline__ uint8_t wall_inside(const WallDev &w, T &x, T &y)
+{
+    if (w.kind == OECL_WALL_RECT) {
+        double xd = (double)x, yd = (double)y;
+        return (xd < -w.lxHalf || xd > w.lxHalf || yd < -w.lyHalf || yd > w.lyHalf) ? 0 : 1;
+    } else if (w.kind == OECL_WALL_ELLIP) {
+        double xd = (double)x, yd = (double)y;
+        return (xd * xd * w.ax + yd * yd * w.ay) >= 1. ? 0 : 1;
+    } else {
+        double xd = (double)x, yd = (double)y;
+        if (xd > w.clipX) xd = w.clipX;
+        else if (xd < -w.clipX) xd = -w.clipX;
+        if (yd > w.clipY) yd = w.clipY;
+        else if (yd < -w.clipY) yd = -w.clipY;
+        x = (T)xd;
+        y = (T)yd;
+        return cutcell_inside(w, xd, yd);
+    }
+}
+
+// The reference's isInside(x, y) used by the loader and the split (no clipping).
+__device__ __forceinline__ uint8_t wall_is_inside_point(const WallDev &w, double x, double y)
+{
+    if (w.kind == OECL_WALL_RECT)
+        return (x < -w.lxHalf || x > w.lxHalf || y < -w.lyHalf || y > w.lyHalf) ? 0 : 1;
+    if (w.kind == OECL_WALL_ELLIP) return (x * x * w.ax + y * y * w.ay) >= 1. ? 0 : 1;
+    // The reference indexes its cell arrays without a range check; outside the grid the
+    // point is reported outside here.
+    if (x <= -w.lxHalfExt || x >= w.lxHalfExt || y <= -w.lyHalfExt || y >= w.lyHalfExt) return 0;
+    return cutcell_inside(w, x, y);
+}
