@@ -1,0 +1,229 @@
+"""Buildup benchmark of the plain GPU PIC step on one MI355X.
+
+Workload (the elliptical-chamber dipole buildup): elliptical cut-cell chamber with semi-axes
+45 mm x 35 mm on a 512 x 384 grid, uniform dipole By = 0.5 T, 2e6 macro-electrons loaded
+uniformly (1 eV Maxwellian, 1e9 electrons per metre in total), Gaussian bunches with
+sigma 1.5 mm, 1.15e11 protons per bunch, 25 ns spacing, 4 bunches, dt = 5e-12 s, Furman
+copper wall with delta_max = 1.3.  A 'step' is one full time step of the buildup loop:
+Poisson solve with the beam, gather + Boris push + wall test + deposit, absorption, wall
+interaction, Furman emission and compaction.
+
+Prints ONE JSON line: particle-steps per second (sum over timed steps of the live macro-
+particle count divided by the wall time), with the fused push kernel's roofline and the
+CPU baseline beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--particles', type=float, default=2e6)
+    ap.add_argument('--dtype', default='f64', choices=['f64', 'f32'])
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    return ap.parse_args()
+
+
+def build_case(n, dtype):
+    import torch
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter, homoLoader
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    randomGen.seed(0)
+    dt = 5e-12
+    g = Grid(512, 384, 0.09, 0.07, 1)
+    p = Particles('electrons', 'variableWeight', dtype=torch.float64 if dtype == 'f64' else torch.float32,
+                  ordering='fast')
+    p.setBConst([0., 0.5, 0.])
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.3)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    beam = GaussianBunchTrain(g, dt, nParticles=1.15e11, tBunchSpacing=25e-9, radiusSigma=1.5e-3,
+                              nBunches=4)
+    homoLoader(g, p, pb, int(n), 1e9 / n, 4.194e5)
+    p.reserve(int(n * 1.6))
+    pb.absorbedParticles = torch.empty((int(n * 0.05) + 4096, 6), dtype=p.dtype, device='cuda')
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
+    return g, p, pb, em, ps, b
+
+
+def cpu_baseline(seconds):
+    """The oracle's C restatement of the reference loops (AoS, one core) on a bounded sample:
+    particle loop (weights, deposit, gather, constant-B Boris push, elliptical wall test) over
+    a 2e5-particle sample of the same grid, plus the reference's SuperLU back-substitution on
+    the same 512 x 384 system; extrapolated to 2e6 particles per step."""
+    import ctypes
+    import numpy as np
+    import oracle
+    from oracle.grid_oracle import OraclePoisson
+    from openecloud_amd.grid import Grid
+    g = Grid(512, 384, 0.09, 0.07, 1)
+    rng = np.random.default_rng(0)
+    ns = 200000
+    pd = np.empty((ns, 6))
+    k = 0
+    while k < ns:
+        x = (rng.random(ns) - 0.5) * 0.09
+        y = (rng.random(ns) - 0.5) * 0.07
+        ok = x * x * 4 / 0.09 ** 2 + y * y * 4 / 0.07 ** 2 < 1
+        m = min(ns - k, ok.sum())
+        pd[k:k + m, 0] = x[ok][:m]
+        pd[k:k + m, 1] = y[ok][:m]
+        k += m
+    pd[:, 2:5] = rng.standard_normal((ns, 3)) * 4.194e5
+    pd[:, 5] = 500.
+    npx = g.npExt
+    E = rng.standard_normal(2 * npx) * 1e3
+    lib = oracle.lib()
+    D = ctypes.POINTER(ctypes.c_double)
+    ic = np.empty(ns, np.uint32)
+    w4 = np.empty(4 * ns)
+    eat = np.empty(2 * ns)
+    bat = np.empty(3 * ns)
+    ins = np.empty(ns, np.uint16)
+    rho = np.empty(npx)
+    bc = np.array([0., 0.5, 0.])
+    qm = -1.602176565e-19 / 9.10938291e-31
+    args = (pd.ctypes.data_as(D), ctypes.c_uint32(ns), E.ctypes.data_as(D), ctypes.c_void_p(0),
+            ctypes.c_int(1), bc.ctypes.data_as(D), ctypes.c_double(1e-14), ctypes.c_double(qm),
+            ctypes.c_double(1 / g.dx), ctypes.c_double(1 / g.dy), ctypes.c_double(g.lxExt / 2),
+            ctypes.c_double(g.lyExt / 2), ctypes.c_uint32(g.nxExt), ctypes.c_uint32(npx),
+            ctypes.c_int(1), ctypes.c_double(0.09), ctypes.c_double(0.07),
+            ic.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), w4.ctypes.data_as(D),
+            eat.ctypes.data_as(D), bat.ctypes.data_as(D),
+            ins.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), rho.ctypes.data_as(D),
+            ctypes.c_double(-1.602176565e-19))
+    lib.orc_particle_step(*args)
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < seconds * 0.6:
+        lib.orc_particle_step(*args)
+        reps += 1
+    t_part = (time.perf_counter() - t0) / reps / ns          # s per particle-step
+    ps = OraclePoisson(g)
+    q = rng.standard_normal(npx) * 1e-12
+    ps.solve(q)
+    t0 = time.perf_counter()
+    nsol = 0
+    while time.perf_counter() - t0 < seconds * 0.4:
+        ps.solve(q)
+        nsol += 1
+    t_lu = (time.perf_counter() - t0) / nsol
+    n_full = 2e6
+    step_time = t_part * n_full + t_lu
+    return dict(value=n_full / step_time, unit='particle-steps/s', cores=1, kind='port',
+                sample='%d-particle sample x %d reps of the C particle loop (%.3g s/particle-step) '
+                       '+ %d SuperLU solves of the 512x384 system (%.3g s each), extrapolated '
+                       'to 2e6 particles per step' % (ns, reps, t_part, nsol, t_lu),
+                poisson_solves_per_s=1. / t_lu)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path '
+              '(see the sharding design note); got --gpus %d' % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from openecloud_amd import _lib as L
+    L.require_gpu()
+    n = int(a.particles)
+    g, p, pb, em, ps, b = build_case(n, a.dtype)
+    for _ in range(a.warmup):
+        b.step()
+    torch.cuda.synchronize()
+    # timed region: K full steps; the push kernel is bracketed by events on the same stream
+    push_events = []
+    import ctypes
+    orig = L.lib().oecl_push_fused
+
+    class Timed:
+        def __call__(self, *args):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            rc = orig(*args)
+            e1.record()
+            push_events.append((e0, e1, p.macroParticleCount))
+            return rc
+    L.lib().oecl_push_fused = Timed()
+    counts = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        counts.append(p.macroParticleCount)
+        b.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.lib().oecl_push_fused = orig
+    wall = t1 - t0
+    psteps = float(sum(counts))
+    push_ms = [e0.elapsed_time(e1) for e0, e1, _ in push_events]
+    s = 8 if a.dtype == 'f64' else 4
+    npx = g.getNpExt()
+    bytes_per = [11 * s * c + 24 * npx for _, _, c in push_events]
+    avg_ms = sum(push_ms) / len(push_ms)
+    avg_bytes = sum(bytes_per) / len(bytes_per)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    # Poisson solves/s of the direct device solve alone
+    q = b.rho.clone()
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    nsol = 50
+    for _ in range(nsol):
+        ps.solvePhi(q)
+    torch.cuda.synchronize()
+    poisson_rate = nsol / (time.perf_counter() - tp)
+    res = ps.residual(q)
+    out = {
+        'metric': 'buildup particle-steps/sec (plain GPU step)',
+        'value': psteps / wall,
+        'unit': 'particle-steps/s',
+        'n_gpus': 1,
+        'steps': a.steps,
+        'warmup': a.warmup,
+        'ms_per_step': wall / a.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': a.dtype,
+        'data': 'synthetic (seeded uniform load, Philox draws)',
+        'config': {'workload': 'elliptical 45x35 mm cut-cell chamber, 512x384 grid, By 0.5 T, '
+                               '%d macro-electrons, Gaussian bunches sigma 1.5 mm 1.15e11 ppb x4, '
+                               'dt 5e-12 s, Furman copper delta_max 1.3' % n,
+                   'grid': [512, 384], 'particles': n, 'dt': 5e-12,
+                   'macro_particles_start_end': [counts[0], p.macroParticleCount]},
+        'poisson_solves_per_s': poisson_rate,
+        'poisson_relres': res,
+        'poisson_capacitance_rows': ps.capacitanceRows,
+        'roofline': {'bound': 'hbm', 'kernel': 'k_push_fused', 'achieved': achieved,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': None, 'avg_launch_us': avg_ms * 1e3,
+                     'bytes_per_launch': avg_bytes},
+    }
+    if not a.no_cpu_baseline:
+        cb = cpu_baseline(a.cpu_seconds)
+        out['cpu_baseline'] = {k: cb[k] for k in ('value', 'unit', 'cores', 'kind', 'sample')}
+        out['cpu_baseline_poisson_solves_per_s'] = cb['poisson_solves_per_s']
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

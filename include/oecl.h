@@ -112,6 +112,9 @@ int oecl_push_fused(const oecl_grid_t *g, const oecl_wall_t *wall, int dtype, vo
  * the secondaries and after particle management. */
 int oecl_deposit_add(const oecl_grid_t *g, int dtype, const void *data, uint64_t stride,
                      uint32_t n, double q, double *rho, void *stream);
+/* The same for an AoS block [n][6] (the emitter's secondaries). */
+int oecl_deposit_add_aos(const oecl_grid_t *g, int dtype, const void *aos, uint32_t n, double q,
+                         double *rho, void *stream);
 
 /* ---- particleBoundary.pyx ----------------------------------------------------------- */
 
@@ -207,6 +210,23 @@ int oecl_csr_spmv(uint32_t nRows, const int32_t *rowPtr, const int32_t *col,
  * electron charge, example01_basic.py:81). */
 int oecl_axpy_rhs(uint32_t np, const double *rho, double scale, const double *profile,
                   double *out, void *stream);
+
+/* Direct solve of the same system (poissonSolver.pyx:59,85 -- the SuperLU factorisation
+ * and back-substitution) by the capacitance-matrix method: the box of possible inside nodes
+ * (bx = nxExt-4 by by = nyExt-4) is inverted exactly by type-I DSTs done as FP64 GEMMs, and
+ * the m inside rows next to the wall are corrected through the m x m capacitance matrix,
+ * built and inverted on the device at creation.  Sx [bx*bx], Sy [by*by]: orthonormal DST-I
+ * matrices; Linv [by][bx]: reciprocal box eigenvalues; sIdx [m]: box index of each corrected
+ * row; W (CSR over box columns): those rows of (M - R).  maskDev (inside flags, device) must
+ * outlive the handle.  *capMax receives max |C_aa| (a scale check). */
+typedef struct oecl_fastpoisson oecl_fastpoisson;
+int oecl_fastpoisson_create(uint32_t nxExt, uint32_t nyExt, const double *Sx, const double *Sy,
+                            const double *Linv, uint32_t m, const int32_t *sIdx,
+                            const int32_t *wPtr, const int32_t *wCol, const double *wVal,
+                            const uint8_t *maskDev, oecl_fastpoisson **out, double *capMax);
+/* phi (npExt, zero on non-inside nodes) = A^-1 q on inside nodes. */
+int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, double *phi, void *stream);
+int oecl_fastpoisson_destroy(oecl_fastpoisson *h);
 
 #ifdef __cplusplus
 }

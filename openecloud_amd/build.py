@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'liboecl.so')
-SOURCES = ['particles.hip', 'boundary.hip', 'emitter.hip', 'poisson.hip']
+SOURCES = ['particles.hip', 'boundary.hip', 'emitter.hip', 'poisson.hip', 'fastpoisson.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['-O3', '-std=c++17', '--offload-arch=gfx950', '-fPIC', '-shared',
          '-ffp-contract=off', '-fvisibility=hidden',
@@ -23,7 +23,8 @@ FLAGS = ['-O3', '-std=c++17', '--offload-arch=gfx950', '-fPIC', '-shared',
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + '.tmp'
-    cmd = [HIPCC] + FLAGS + ['-o', tmp] + srcs
+    cmd = [HIPCC] + FLAGS + ['-o', tmp] + srcs + ['-L/opt/rocm/lib', '-lrocblas', '-lrocsolver',
+                                                   '-Wl,-rpath,/opt/rocm/lib']
     if verbose:
         print(' '.join(cmd), flush=True)
     subprocess.check_call(cmd)

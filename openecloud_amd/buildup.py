@@ -1,0 +1,199 @@
+"""The buildup time loop (the loop the reference writes out in example01_basic.py:52-97 and
+its siblings) in two forms:
+
+  Buildup.step_reference()  calls the reference's public methods one by one, in the
+                            reference's order (calcGridWeights, chargeToGrid, solve,
+                            eFieldToParticles, [bFieldToParticles], borisPush, indexInside,
+                            saveAbsorbed, interaction point, normals, generateSecondaries,
+                            addAndRemoveParticles);
+  Buildup.step()            the plain fused GPU step: one kernel gathers E (and B), pushes,
+                            tests the wall and deposits the charge of the surviving particles
+                            for the next step; the absorbed particles are collected, moved to
+                            the wall, emitted from, and the compaction moves only particles
+                            that must move.  Same physics, same particle set.
+Both use the same device objects, so they can be mixed.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+class Buildup:
+
+    def __init__(self, gridObj, particlesObj, boundaryObj, emitterObj, poissonObj, beamObj=None,
+                 dt=1e-11, typeBField=0, bAtGridPoints=None, secondaryEmission=True):
+        self.g = gridObj
+        self.p = particlesObj
+        self.pb = boundaryObj
+        self.em = emitterObj
+        self.ps = poissonObj
+        self.beam = beamObj
+        self.dt = float(dt)
+        self.typeB = int(typeBField)
+        self.secondaryEmission = secondaryEmission
+        self.bField = None if bAtGridPoints is None else L.to_dev(np.asarray(bAtGridPoints, np.float64))
+        if self.typeB == 2 and self.bField is None:
+            raise ValueError('typeBField=2 needs bAtGridPoints')
+        self.it = 0
+        dev = L.device()
+        npx = gridObj.getNpExt()
+        self.gd = L.grid_desc(gridObj)
+        self.rho = torch.zeros(npx, dtype=torch.float64, device=dev)
+        self.q = torch.zeros(npx, dtype=torch.float64, device=dev)
+        self._rho_valid = False
+        self._cnt = torch.zeros(8, dtype=torch.int32, device=dev)
+        self._pin = torch.zeros(8, dtype=torch.int32).pin_memory()
+        self._work = None
+        self.history = {'macro': [], 'absorbed': [], 'emitted': [], 'charge': []}
+
+    # ------------------------------------------------------------------ helpers
+    def _work_buf(self, n, nadd=0):
+        need = int(max(L.lib().oecl_compact_work_bytes(n, nadd), L.lib().oecl_scan_work_bytes(n))) // 4 + 64
+        if self._work is None or self._work.numel() < need:
+            self._work = torch.empty(int(need * 1.25), dtype=torch.int32, device=L.device())
+        return self._work
+
+    def deposit_all(self):
+        """rho = charge of all current particles (the reference's chargeToGrid)."""
+        p = self.p
+        self.rho.zero_()
+        L.check(L.lib().oecl_deposit_add(ctypes.byref(self.gd), L.i32(p.code), L.ptr(p._data),
+                                         L.u64(p.capacity), L.u32(p.macroParticleCount),
+                                         L.f64(p.particleCharge), L.ptr(self.rho), L.stream()),
+                'deposit')
+        self._rho_valid = True
+
+    def invalidate_charge(self):
+        """Call after changing the particles outside step() (e.g. particle management)."""
+        self._rho_valid = False
+
+    def _rhs(self, t):
+        if self.beam is None:
+            return self.rho
+        s = self.beam.getScale(t)
+        if s == 0.:
+            return self.rho
+        L.check(L.lib().oecl_axpy_rhs(L.u32(self.g.getNpExt()), L.ptr(self.rho), L.f64(s),
+                                      L.ptr(self.beam.getProfile()), L.ptr(self.q), L.stream()),
+                'rhs')
+        return self.q
+
+    # ------------------------------------------------------------------ reference order
+    def step_reference(self):
+        g, p, pb, ps = self.g, self.p, self.pb, self.ps
+        t = self.it * self.dt
+        p.calcGridWeights(g)
+        p.chargeToGrid(g)
+        q = p.getChargeOnGrid()
+        if self.beam is not None:
+            s = self.beam.getScale(t)
+            if s != 0.:
+                q = q + s * self.beam.getProfile()
+        ps.solve(q)
+        p.eFieldToParticles(g, ps.getEAtGridPoints())
+        if self.typeB == 2:
+            p.bFieldToParticles(g, self.bField)
+        p.borisPush(self.dt, typeBField=self.typeB)
+        pb.indexInside()
+        pb.saveAbsorbed()
+        na = pb.getAbsorbedMacroParticleCount()
+        nsec = 0
+        if na > 0:
+            pb.calculateInteractionPoint()
+            pb.calculateNormalVectors()
+            if self.secondaryEmission:
+                sec = self.em.generateSecondaries(step=self.it)
+            else:
+                sec = torch.empty((0, 6), dtype=p.dtype, device=L.device())
+            nsec = sec.shape[0]
+            p.addAndRemoveParticles(sec, p.getIsInside())
+        self._rho_valid = False
+        self._record(na, nsec)
+        self.it += 1
+
+    # ------------------------------------------------------------------ fused plain step
+    def step(self):
+        g, p, pb, ps, em = self.g, self.p, self.pb, self.ps, self.em
+        t = self.it * self.dt
+        if not self._rho_valid:
+            self.deposit_all()
+        E = ps.solve(self._rhs(t))
+        n = p.macroParticleCount
+        s = L.stream()
+        self.rho.zero_()
+        b = (ctypes.c_double * 3)(*p.bConst.tolist())
+        L.check(L.lib().oecl_push_fused(
+            ctypes.byref(self.gd), ctypes.byref(pb.desc), L.i32(p.code), L.ptr(p._data),
+            L.u64(p.capacity), L.u32(n), L.ptr(E), L.ptr(self.bField), b, L.i32(self.typeB),
+            L.f64(self.dt), L.f64(p.particleCharge / p.particleMass), L.f64(p.particleCharge),
+            L.ptr(p.isInside), L.ptr(self.rho), s), 'push')
+        p.dt = self.dt
+        # absorbed particles, in index order
+        work = self._work_buf(n)
+        cap = pb.absorbedParticles.shape[0]
+        L.check(L.lib().oecl_save_absorbed(
+            L.i32(p.code), L.ptr(p._data), L.u64(p.capacity), L.u32(n), L.ptr(p.isInside),
+            L.ptr(pb.absorbedParticles), L.u32(cap), L.ptr(self._cnt), L.ptr(work), s), 'absorb')
+        na = int(self._cnt[0].item())
+        if na > cap:
+            pb.absorbedParticles = torch.empty((int(na * 1.5) + 1024, 6), dtype=p.dtype,
+                                               device=L.device())
+            L.check(L.lib().oecl_save_absorbed(
+                L.i32(p.code), L.ptr(p._data), L.u64(p.capacity), L.u32(n), L.ptr(p.isInside),
+                L.ptr(pb.absorbedParticles), L.u32(pb.absorbedParticles.shape[0]),
+                L.ptr(self._cnt), L.ptr(work), s), 'absorb')
+        pb.absorbedMacroParticleCount = na
+        nsec = 0
+        sec = None
+        if na > 0:
+            pb._interaction()
+            if self.secondaryEmission:
+                sec = em.generateSecondaries(step=self.it)
+                nsec = sec.shape[0]
+                if nsec:
+                    L.check(L.lib().oecl_deposit_add_aos(
+                        ctypes.byref(self.gd), L.i32(p.code), L.ptr(sec), L.u32(nsec),
+                        L.f64(p.particleCharge), L.ptr(self.rho), s), 'deposit secondaries')
+            new = n - na + nsec
+            if new > p.capacity:
+                keep = p.isInside[:n].clone()
+                p.reserve(int(new * 1.25) + 1024)
+            else:
+                keep = p.isInside
+            work = self._work_buf(n, nsec)
+            spare = p._spare_block() if p.ordering == 'reference' else None
+            mode = 3
+            if p.ordering == 'reference':
+                if nsec == 0:
+                    mode = 0
+                elif new > p._refCapacity:
+                    mode = 2
+                    p._refCapacity = int(new * 1.1)
+                else:
+                    mode = 1
+            L.check(L.lib().oecl_add_and_remove(
+                L.i32(p.code), L.ptr(p._data), L.ptr(spare), L.u64(p.capacity), L.u32(n),
+                L.ptr(keep), L.ptr(sec), L.u32(nsec), L.i32(mode), L.ptr(self._cnt[1:2]),
+                L.ptr(work), s), 'compact')
+            if mode == 2:
+                p._data, p._spare = p._spare, p._data
+            p.macroParticleCount = new
+        self._rho_valid = True
+        self._record(na, nsec)
+        self.it += 1
+
+    def _record(self, na, nsec):
+        h = self.history
+        h['macro'].append(self.p.macroParticleCount)
+        h['absorbed'].append(na)
+        h['emitted'].append(nsec)
+
+    def run(self, nSteps, fused=True):
+        for _ in range(int(nSteps)):
+            if fused:
+                self.step()
+            else:
+                self.step_reference()

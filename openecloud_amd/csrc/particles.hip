@@ -102,6 +102,23 @@ __global__ void k_deposit_add(oecl_grid_t g, Soa<T> p, uint32_t n, double q, dou
     unsafeAtomicAdd(rho + w.cell + g.nx + 1, (double)w.wUU * wt * q);
 }
 
+// Same deposit for an AoS block [n][6] (the emitter's secondaries).
+template <typename T>
+__global__ void k_deposit_add_aos(oecl_grid_t g, const T *__restrict__ a, uint32_t n, double q,
+                                  double *rho)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T *r = a + 6 * (uint64_t)i;
+    Weights<T> w = grid_weights<T>(r[0], r[1], (T)(1. / g.dx), (T)(1. / g.dy), (T)g.lxHalfExt,
+                                   (T)g.lyHalfExt, g.nx, g.ny);
+    double wt = (double)r[5];
+    unsafeAtomicAdd(rho + w.cell, (double)w.wLL * wt * q);
+    unsafeAtomicAdd(rho + w.cell + 1, (double)w.wUL * wt * q);
+    unsafeAtomicAdd(rho + w.cell + g.nx, (double)w.wLU * wt * q);
+    unsafeAtomicAdd(rho + w.cell + g.nx + 1, (double)w.wUU * wt * q);
+}
+
 // particles.pyx:354-397: the first corner assigns, the other three accumulate, in order.
 template <typename T, int DIMS>
 __global__ void k_field_to_particles(const double *f, const uint32_t *inCell, const T *w4,
@@ -296,6 +313,25 @@ OECL_EXPORT int oecl_deposit_add(const oecl_grid_t *g, int dtype, const void *da
     else
         return OECL_ERR_ARG;
     OECL_CHECK_LAUNCH("oecl_deposit_add");
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_deposit_add_aos(const oecl_grid_t *g, int dtype, const void *aos, uint32_t n,
+                                     double q, double *rho, void *stream)
+{
+    if (!g || !rho) return OECL_ERR_ARG;
+    if (n == 0) return OECL_OK;
+    if (!aos) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_deposit_add_aos<double>, dim3(oecl_blocks(n, 256)), dim3(256), 0, s, *g,
+                           (const double *)aos, n, q, rho);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_deposit_add_aos<float>, dim3(oecl_blocks(n, 256)), dim3(256), 0, s, *g,
+                           (const float *)aos, n, q, rho);
+    else
+        return OECL_ERR_ARG;
+    OECL_CHECK_LAUNCH("oecl_deposit_add_aos");
     return OECL_OK;
 }
 

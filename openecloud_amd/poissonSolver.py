@@ -5,8 +5,15 @@ Setup (host, once) builds from the grid the same operators as the reference:
     (cx on x edges, cy on y edges) of the 5-point stencil;
   - phiToEAtGridPoints = edgeToNode . diag(dsi) . S~^T, assembled with scipy.sparse exactly
     as the reference does and moved to the device as CSR.
-solve(q) runs FP64 conjugate gradients on the device to tol * ||q_inside|| (warm-started
-from the previous potential) and then the CSR phi -> E product, also on the device.
+solve(q) solves A phi = q on the device and then applies the CSR phi -> E product there.
+Two device methods:
+  'direct' (default)  capacitance-matrix direct solve (DST-I by FP64 GEMMs for the box of
+                      inside nodes plus a dense correction of the rows next to the wall),
+                      the counterpart of the reference's one-off LU factorisation and
+                      back-substitution; its residual is checked against tol every
+                      `checkEvery` solves and, if too large, refined by CG.
+  'pcg'               Jacobi-preconditioned CG to tol * ||q_inside||, warm-started from the
+                      previous potential.
 """
 import ctypes
 
@@ -18,14 +25,73 @@ from . import _lib as L
 from .constants import epsilon_0
 
 
+def dst1_matrix(n):
+    """Orthonormal type-I DST matrix S[j, k] = sqrt(2/(n+1)) sin(pi (j+1)(k+1)/(n+1))."""
+    k = np.arange(1, n + 1)
+    return np.sqrt(2. / (n + 1)) * np.sin(np.pi * np.outer(k, k) / (n + 1))
+
+
+def capacitance_rows(nx, ny, mask, cx, cy):
+    """Rows of (M - R) for the capacitance method (see csrc/fastpoisson.hip).
+
+    Returns (cxu, cyu, sIdx, wPtr, wCol, wVal) with box indices t = (i-2) + (j-2)*(nx-4)."""
+    bx, by = nx - 4, ny - 4
+    m2 = mask.reshape(ny, nx).astype(bool)
+    cx2 = cx.reshape(ny, nx)
+    cy2 = cy.reshape(ny, nx)
+    # coefficients of edges joining two inside nodes are the uniform ones
+    ex = m2[:, :-1] & m2[:, 1:]
+    ey = m2[:-1, :] & m2[1:, :]
+    vx = np.unique(cx2[:, :-1][ex])
+    vy = np.unique(cy2[:-1, :][ey])
+    if vx.size != 1 or vy.size != 1:
+        raise ValueError('inside-inside edges do not share one coefficient per direction')
+    cxu, cyu = float(vx[0]), float(vy[0])
+    # a neighbour outside the box is a Dirichlet zero for R as well, so it only needs the
+    # uniform edge coefficient; a neighbour inside the box must be an inside node
+    okn = m2.copy()
+    okn[:2, :] = True
+    okn[ny - 2:, :] = True
+    okn[:, :2] = True
+    okn[:, nx - 2:] = True
+    reg = np.zeros_like(m2)
+    c = (slice(1, ny - 1), slice(1, nx - 1))
+    reg[c] = (m2[c] & okn[1:-1, 2:] & okn[1:-1, :-2] & okn[2:, 1:-1] & okn[:-2, 1:-1] &
+              (cx2[c] == cxu) & (cx2[1:-1, :-2] == cxu) & (cy2[c] == cyu) & (cy2[:-2, 1:-1] == cyu))
+    J, I = np.nonzero(m2 & ~reg)
+    s_idx, ptr, cols, vals = [], [0], [], []
+    for j, i in zip(J, I):
+        nb = ((i + 1, j, cx2[j, i], cxu), (i - 1, j, cx2[j, i - 1], cxu),
+              (i, j + 1, cy2[j, i], cyu), (i, j - 1, cy2[j - 1, i], cyu))
+        d = cx2[j, i] + cx2[j, i - 1] + cy2[j, i] + cy2[j - 1, i]
+        t = (i - 2) + (j - 2) * bx
+        s_idx.append(t)
+        row = [(t, d - (2 * cxu + 2 * cyu))]
+        for ii, jj, c, u in nb:
+            if 2 <= ii < nx - 2 and 2 <= jj < ny - 2:
+                diff = (-c if m2[jj, ii] else 0.) + u
+                if diff != 0.:
+                    row.append(((ii - 2) + (jj - 2) * bx, diff))
+        row.sort()
+        cols += [r[0] for r in row]
+        vals += [r[1] for r in row]
+        ptr.append(len(cols))
+    return (cxu, cyu, np.asarray(s_idx, np.int32), np.asarray(ptr, np.int32),
+            np.asarray(cols, np.int32), np.asarray(vals, np.float64))
+
+
 class PoissonSolver:
 
-    def __init__(self, gridObj, tol=1e-12, maxIter=200000):
+    def __init__(self, gridObj, tol=1e-12, maxIter=200000, method='direct', checkEvery=1):
         L.require_gpu()
         nx = gridObj.getNxExt()
         npx = gridObj.getNpExt()
-        self.nx, self.npExt = nx, npx
+        self.nx, self.ny, self.npExt = nx, gridObj.getNyExt(), npx
         self.tol, self.maxIter = float(tol), int(maxIter)
+        if method not in ('direct', 'pcg'):
+            raise ValueError("method must be 'direct' or 'pcg'")
+        self.method = method
+        self.checkEvery = int(checkEvery)
         ip = np.asarray(gridObj.getInsidePoints())
         self.insidePointsInd = np.nonzero(ip)[0]
         dsi = np.asarray(gridObj.getDsi(), np.float64)
@@ -57,6 +123,51 @@ class PoissonSolver:
         self.lastIterations = 0
         self.lastRelRes = 0.
         self.solveCount = 0
+        self._fp = None
+        if method == 'direct':
+            self._setup_direct()
+
+    def _setup_direct(self):
+        nx, ny = self.nx, self.ny
+        cxu, cyu, sIdx, wPtr, wCol, wVal = capacitance_rows(
+            nx, ny, self.mask.cpu().numpy(), self.cx_host, self.cy_host)
+        bx, by = nx - 4, ny - 4
+        Sx = np.ascontiguousarray(dst1_matrix(bx))
+        Sy = np.ascontiguousarray(dst1_matrix(by))
+        lx = 2. - 2. * np.cos(np.pi * np.arange(1, bx + 1) / (bx + 1))
+        ly = 2. - 2. * np.cos(np.pi * np.arange(1, by + 1) / (by + 1))
+        Linv = np.ascontiguousarray(1. / (cxu * lx[None, :] + cyu * ly[:, None]))
+        self.capacitanceRows = int(sIdx.size)
+        self._fp_keep = (Sx, Sy, Linv, sIdx, wPtr, wCol, wVal)
+        h = ctypes.c_void_p(0)
+        cap = ctypes.c_double(0.)
+        D = ctypes.POINTER(ctypes.c_double)
+        I32 = ctypes.POINTER(ctypes.c_int32)
+        L.check(L.lib().oecl_fastpoisson_create(
+            L.u32(nx), L.u32(ny), Sx.ctypes.data_as(D), Sy.ctypes.data_as(D),
+            Linv.ctypes.data_as(D), L.u32(sIdx.size), sIdx.ctypes.data_as(I32),
+            wPtr.ctypes.data_as(I32), wCol.ctypes.data_as(I32), wVal.ctypes.data_as(D),
+            L.ptr(self.mask), ctypes.byref(h), ctypes.byref(cap)), 'PoissonSolver direct setup')
+        self._fp = h
+        self.capacitanceMax = cap.value
+
+    def __del__(self):
+        try:
+            if self._fp is not None and self._fp.value:
+                L.lib().oecl_fastpoisson_destroy(self._fp)
+        except Exception:
+            pass
+
+    def residual(self, q):
+        """||q - A phi|| / ||q|| over inside nodes for the current phi (device)."""
+        qd = self._rhs(q)
+        it = ctypes.c_uint32(0)
+        rr = ctypes.c_double(0.)
+        L.lib().oecl_pcg_solve(
+            L.u32(self.nx), L.u32(self.npExt), L.ptr(self.cx), L.ptr(self.cy), L.ptr(self.mask),
+            L.ptr(qd), L.ptr(self.phi), L.f64(np.inf), L.u32(0), L.ptr(self._work),
+            ctypes.byref(it), ctypes.byref(rr), L.stream())
+        return rr.value
 
     def _rhs(self, q):
         if isinstance(q, torch.Tensor) and q.is_cuda and q.dtype == torch.float64:
@@ -66,16 +177,28 @@ class PoissonSolver:
     def solvePhi(self, q):
         """phi[inside] = A^-1 q[inside] (poissonSolver.pyx:85) on the device."""
         qd = self._rhs(q)
+        if self.method == 'direct':
+            L.check(L.lib().oecl_fastpoisson_solve(self._fp, L.ptr(qd), L.ptr(self.phi),
+                                                   L.stream()), 'PoissonSolver.solve (direct)')
+            self.solveCount += 1
+            self.lastIterations = 0
+            if self.checkEvery > 0 and self.solveCount % self.checkEvery == 0:
+                # residual check; CG refinement from the direct solution if it falls short
+                self._pcg(qd, self.maxIter)
+            return self.phi
+        self._pcg(qd, self.maxIter)
+        self.solveCount += 1
+        return self.phi
+
+    def _pcg(self, qd, maxIter):
         it = ctypes.c_uint32(0)
         rr = ctypes.c_double(0.)
         L.check(L.lib().oecl_pcg_solve(
             L.u32(self.nx), L.u32(self.npExt), L.ptr(self.cx), L.ptr(self.cy), L.ptr(self.mask),
-            L.ptr(qd), L.ptr(self.phi), L.f64(self.tol), L.u32(self.maxIter), L.ptr(self._work),
+            L.ptr(qd), L.ptr(self.phi), L.f64(self.tol), L.u32(maxIter), L.ptr(self._work),
             ctypes.byref(it), ctypes.byref(rr), L.stream()), 'PoissonSolver.solve (PCG)')
         self.lastIterations = it.value
         self.lastRelRes = rr.value
-        self.solveCount += 1
-        return self.phi
 
     def phiToE(self):
         """eAtGridPoints = phiToEAtGridPoints . phi (poissonSolver.pyx:87)."""

@@ -1,0 +1,99 @@
+"""GPU parity of walls, absorption and add/remove compaction against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import random_particles
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def grids():
+    from openecloud_amd.grid import Grid
+    circ = lambda x, y: 1 if x * x + y * y < 0.018 ** 2 else 0
+    return {'rect': Grid(128, 64, 0.1, 0.05, 0), 'ellip': Grid(96, 72, 0.09, 0.07, 1),
+            'cutcell': Grid(60, 60, 0.04, 0.04, 3, boundFunc=circ)}
+
+
+def _setup(g, kind, n, seed, vscale=4e6, overshoot=1.08):
+    from openecloud_amd.particleBoundary import (AbsorbArbitraryCutCell, AbsorbElliptical,
+                                                 AbsorbRectangular)
+    from openecloud_amd.particles import Particles
+    rng = np.random.default_rng(seed)
+    pd = random_particles(g, n, seed=seed, vscale=vscale, margin=overshoot)
+    if kind == 'cutcell':
+        r = np.sqrt(pd[:, 0] ** 2 + pd[:, 1] ** 2)
+        pd[:, 0] *= 0.0185 / np.maximum(r, 1e-9) * rng.random(n) ** 0.5
+        pd[:, 1] *= 0.0185 / np.maximum(r, 1e-9) * rng.random(n) ** 0.5
+    p = Particles('electrons', 'variableWeight')
+    p.setParticleData(pd)
+    cls = {'rect': AbsorbRectangular, 'ellip': AbsorbElliptical, 'cutcell': AbsorbArbitraryCutCell}[kind]
+    return pd, p, cls(g, p)
+
+
+@pytest.mark.parametrize('kind', ['rect', 'ellip', 'cutcell'])
+def test_inside_absorb_interaction_normals(grids, orc, kind):
+    g = grids[kind]
+    pd, p, pb = _setup(g, kind, 20000, seed=3)
+    p.dt = 1e-11
+    pb.indexInside()
+    ins_o, pd_clip = orc.index_inside(pd, kind, g.getLx(), g.getLy(), grid=g)
+    assert np.array_equal(p.getIsInside().cpu().numpy().astype(np.uint16), ins_o)
+    assert np.array_equal(p.getParticleData().cpu().numpy(), pd_clip)
+    pb.saveAbsorbed()
+    abs_o = orc.save_absorbed(pd_clip, ins_o)
+    assert pb.getAbsorbedMacroParticleCount() == abs_o.shape[0] > 0
+    assert np.array_equal(pb.getAbsorbedParticles().cpu().numpy(), abs_o)
+    pb.calculateInteractionPoint()
+    pb.calculateNormalVectors()
+    moved, rts, nv, nf = orc.interaction_and_normals(abs_o, kind, g.getLx(), g.getLy(), g.getDx(),
+                                                     g.getDy(), grid=g, dt=1e-11)
+    got = pb.getAbsorbedParticles().cpu().numpy()
+    assert np.array_equal(got, moved, equal_nan=True)
+    assert np.array_equal(pb.getNormalVectors().cpu().numpy(), nv, equal_nan=True)
+    if kind != 'cutcell':
+        assert np.array_equal(pb.getRemainingTimeStep().cpu().numpy(), rts, equal_nan=True)
+
+
+def _compact_case(orc, n, nadd, seed, mode_rule):
+    from openecloud_amd.particles import Particles
+    rng = np.random.default_rng(seed)
+    pd = rng.standard_normal((n, 6))
+    keep = (rng.random(n) > 0.05).astype(np.uint16)
+    add = rng.standard_normal((nadd, 6))
+    p = Particles('electrons', 'variableWeight', ordering=mode_rule)
+    p.setParticleData(pd)
+    return pd, keep, add, p
+
+
+@pytest.mark.parametrize('n,nadd', [(10000, 0), (10000, 100), (10000, 3000), (9000, 480),
+                                    (70000, 1000), (5000, 5000)])
+def test_add_remove_reference_order(orc, n, nadd):
+    pd, keep, add, p = _compact_case(orc, n, nadd, seed=n + nadd, mode_rule='reference')
+    cap = n
+    # force the in-place rule once by giving the reference a roomy capacity
+    for roomy in (False, True):
+        p.setParticleData(pd)
+        cap = n
+        if roomy:
+            p._refCapacity = cap = n + nadd + 10
+        p.addAndRemoveParticles(add, keep)
+        ref, _ = orc.add_and_remove(pd, add, keep, cap)
+        got = p.getParticleData().cpu().numpy()
+        assert got.shape == ref.shape
+        assert np.array_equal(got, ref), (n, nadd, roomy)
+
+
+@pytest.mark.parametrize('n,nadd', [(10000, 0), (10000, 100), (10000, 3000), (70000, 1000)])
+def test_add_remove_fast_same_set(orc, n, nadd):
+    pd, keep, add, p = _compact_case(orc, n, nadd, seed=7 * n + nadd, mode_rule='fast')
+    p.addAndRemoveParticles(add, keep)
+    ref, _ = orc.add_and_remove(pd, add, keep, n)
+    got = p.getParticleData().cpu().numpy()
+    assert got.shape == ref.shape
+    key = lambda a: a[np.lexsort(a.T[::-1])]
+    assert np.array_equal(key(got), key(ref))
+    # the fast rule keeps every particle that did not have to move in place
+    kept_idx = np.nonzero(keep[: got.shape[0]])[0]
+    assert np.array_equal(got[kept_idx], pd[kept_idx])

@@ -1,0 +1,83 @@
+"""GPU buildup loop against the oracle loop: same initial particles, same Philox draws,
+several full time steps (deposit, Poisson, gather, push, wall, emission, add/remove)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import random_particles
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, key, cap):
+    from oracle.grid_oracle import OraclePoisson
+    ops = OraclePoisson(g)
+    Q = -1.602176565e-19
+    qm = Q / 9.10938291e-31
+    nx, npx = g.getNxExt(), g.getNpExt()
+    out = []
+    for it in range(nsteps):
+        ic, w = orc.calc_grid_weights(pd, g.getDx(), g.getDy(), g.getLxExt(), g.getLyExt(), nx)
+        rho = orc.charge_to_grid(pd, ic, w, Q, nx, npx)
+        s = beam_scale(it * dt)
+        E = ops.solve(rho + s * profile if s else rho)
+        eat = orc.field_to_particles(E, ic, w, nx, npx, 2)
+        pd = orc.boris_push(pd, eat, dt, qm, typeB, bconst)
+        ins, pd = orc.index_inside(pd, 'ellip', g.getLx(), g.getLy())
+        ab = orc.save_absorbed(pd, ins)
+        if ab.shape[0]:
+            moved, _, nv, _ = orc.interaction_and_normals(ab, 'ellip', g.getLx(), g.getLy(),
+                                                          g.getDx(), g.getDy())
+            na = ab.shape[0]
+            U = np.empty((na, 36))
+            for b in range(18):
+                U[:, 2 * b:2 * b + 2] = orc.words_to_uniforms(
+                    orc.philox_words(na, 0, b, it, 1, key[0], key[1]))
+            from oracle.furman_oracle import generate_secondaries
+            sec, _, _ = generate_secondaries(moved, nv, U[:, :35], P)
+            pd, cap = orc.add_and_remove(pd, sec, ins, cap)
+        out.append(pd.copy())
+    return out
+
+
+@pytest.mark.parametrize('fused', [False, True])
+def test_buildup_matches_oracle_fp64(orc, fused):
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.furman_oracle import material
+    g = Grid(64, 48, 0.09, 0.07, 1)
+    n = 20000
+    pd0 = random_particles(g, n, seed=11, wall='ellip', vscale=3e6)
+    pd0[:, 5] = 5e4
+    dt = 2e-11
+    randomGen.seed(2024)
+    key = randomGen.key()
+    p = Particles('electrons', 'variableWeight', ordering='reference')
+    p.setParticleData(pd0)
+    p.setBConst([0., 0.5, 0.])
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.6, eHat0=300.)
+    ps = PoissonSolver(g, method='direct', checkEvery=1, tol=1e-12)
+    beam = GaussianBunchTrain(g, dt, nParticles=1e11, tBunchSpacing=25e-9, radiusSigma=2e-3,
+                              nBunches=1, bunchLengthSigma=0.003)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
+    nsteps = 12
+    ref = _oracle_loop(orc, g, pd0.copy(), nsteps, dt, 1, np.array([0., 0.5, 0.]),
+                       beam.getTransversalProfile(), beam.getScale,
+                       material('copper', 1.6, eHat0=300.), key, n)
+    emitted = 0
+    for it in range(nsteps):
+        b.step() if fused else b.step_reference()
+        got = p.getParticleData().cpu().numpy()
+        assert got.shape == ref[it].shape, (it, got.shape, ref[it].shape)
+        scale = np.abs(ref[it]).max(axis=0)
+        err = (np.abs(got - ref[it]) / scale).max()
+        assert err < 1e-11, (it, err)
+        emitted += b.history['emitted'][-1]
+    assert emitted > 0 and sum(b.history['absorbed']) > 0

@@ -1,0 +1,72 @@
+"""CPU: the Poisson oracle pinned by the linear system, the DST closed form of the 5-point
+Laplacian, and exact gradients of linear potentials."""
+import numpy as np
+import pytest
+import scipy.fft as sfft
+
+from oracle.grid_oracle import EPS0, OraclePoisson
+from openecloud_amd.grid import Grid
+
+
+def test_rect_operator_is_scaled_5point_laplacian_and_dst_solution():
+    g = Grid(40, 24, 0.1, 0.05, 0)
+    ps = OraclePoisson(g)
+    nxi, nyi = g.nxExt - 4, g.nyExt - 4
+    assert ps.a.shape == (nxi * nyi, nxi * nyi)
+    cx = EPS0 * g.dy / g.dx
+    cy = EPS0 * g.dx / g.dy
+    rng = np.random.default_rng(0)
+    q = np.zeros(g.npExt)
+    q[ps.insidePointsInd] = rng.standard_normal(nxi * nyi) * 1e-12
+    ps.solve(q)
+    # closed form: A = cx*T_x (x) I + cy*I (x) T_y with T = tridiag(-1, 2, -1), DST-I diagonal
+    Q = q[ps.insidePointsInd].reshape(nyi, nxi)
+    lx = 2 - 2 * np.cos(np.pi * np.arange(1, nxi + 1) / (nxi + 1))
+    ly = 2 - 2 * np.cos(np.pi * np.arange(1, nyi + 1) / (nyi + 1))
+    Qh = sfft.dstn(Q, type=1)
+    Ph = Qh / (cx * lx[None, :] + cy * ly[:, None])
+    P = sfft.idstn(Ph, type=1)
+    assert np.allclose(ps.phi[ps.insidePointsInd].reshape(nyi, nxi), P, rtol=1e-10,
+                       atol=1e-12 * np.abs(P).max())
+
+
+@pytest.mark.parametrize('args', [(64, 48, 0.09, 0.07, 1), (50, 30, 0.1, 0.05, 0)])
+def test_residual_and_symmetry(args):
+    g = Grid(*args)
+    ps = OraclePoisson(g)
+    assert abs(ps.a - ps.a.T).max() <= 1e-12 * abs(ps.a).max()
+    rng = np.random.default_rng(1)
+    q = rng.standard_normal(g.npExt) * 1e-12
+    ps.solve(q)
+    r = ps.a.dot(ps.phi[ps.insidePointsInd]) - q[ps.insidePointsInd]
+    assert np.linalg.norm(r) <= 1e-12 * np.linalg.norm(q[ps.insidePointsInd])
+
+
+def test_linear_potential_gives_exact_interior_field():
+    g = Grid(40, 30, 0.09, 0.07, 1)
+    ps = OraclePoisson(g)
+    X, Y = np.meshgrid(g.xMesh, g.yMesh)
+    a, b = 2.5, -4.0
+    phi = np.where(g.insidePoints.reshape(X.shape) == 1, a * X + b * Y, 0.).ravel()
+    e = ps.phiToE.dot(phi)
+    nx, npx = g.nxExt, g.npExt
+    ip = g.insidePoints
+    # inside nodes whose 4 neighbours are inside: centred differences of a linear phi
+    idx = np.array([c for c in np.nonzero(ip)[0]
+                    if ip[c - 1] and ip[c + 1] and ip[c - nx] and ip[c + nx]])
+    assert np.allclose(e[idx], -a, rtol=1e-9)
+    assert np.allclose(e[idx + npx], -b, rtol=1e-9)
+
+
+def test_point_charge_symmetry():
+    g = Grid(41, 41, 0.04, 0.04, 1)
+    ps = OraclePoisson(g)
+    q = np.zeros(g.npExt)
+    c = (g.nxExt // 2) + (g.nyExt // 2) * g.nxExt
+    q[c] = 1e-12
+    ps.solve(q)
+    P = ps.phi.reshape(g.nyExt, g.nxExt)
+    assert np.allclose(P, P[::-1, :], rtol=1e-10, atol=1e-14 * np.abs(P).max())
+    assert np.allclose(P, P[:, ::-1], rtol=1e-10, atol=1e-14 * np.abs(P).max())
+    assert np.allclose(P, P.T, rtol=1e-10, atol=1e-14 * np.abs(P).max())
+    assert P.max() == P.flat[c] > 0
