@@ -58,8 +58,7 @@ def build_case(n, dtype):
     beam = GaussianBunchTrain(g, dt, nParticles=1.15e11, tBunchSpacing=25e-9, radiusSigma=1.5e-3,
                               nBunches=4)
     homoLoader(g, p, pb, int(n), 1e9 / n, 4.194e5)
-    p.reserve(int(n * 1.6))
-    pb.absorbedParticles = torch.empty((int(n * 0.05) + 4096, 6), dtype=p.dtype, device='cuda')
+    p.reserve(int(n * 2))
     b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
     return g, p, pb, em, ps, b
 
@@ -146,49 +145,58 @@ def main():
     L.require_gpu()
     n = int(a.particles)
     g, p, pb, em, ps, b = build_case(n, a.dtype)
-    for _ in range(a.warmup):
-        b.step()
-    torch.cuda.synchronize()
-    # timed region: K full steps; the push kernel is bracketed by events on the same stream
-    push_events = []
     import ctypes
-    orig = L.lib().oecl_push_fused
-
-    class Timed:
-        def __call__(self, *args):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            rc = orig(*args)
-            e1.record()
-            push_events.append((e0, e1, p.macroParticleCount))
-            return rc
-    L.lib().oecl_push_fused = Timed()
-    counts = []
+    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4),
+                   maxSteps=a.warmup + a.steps + 8)
+    for k in range(a.warmup):
+        b.step_native()
+        if k % 10 == 9:
+            b.sync()
+    torch.cuda.synchronize()
+    b.sync()
+    # timed region: K full steps; the push kernel of each step is bracketed by HIP events
+    # recorded on the stream the step runs on
+    evs = []
+    for _ in range(a.steps):
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        L.check(L.lib().oecl_event_create(ctypes.byref(e0)))
+        L.check(L.lib().oecl_event_create(ctypes.byref(e1)))
+        evs.append((e0, e1))
+    it0 = b.it
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
-        counts.append(p.macroParticleCount)
-        b.step()
+    for k in range(a.steps):
+        b._args.evPushBegin = evs[k][0].value
+        b._args.evPushEnd = evs[k][1].value
+        b.step_native()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    L.lib().oecl_push_fused = orig
+    b._args.evPushBegin = None
+    b._args.evPushEnd = None
+    st = b.sync()
     wall = t1 - t0
+    counts = b.counts(it0, it0 + a.steps)
     psteps = float(sum(counts))
-    push_ms = [e0.elapsed_time(e1) for e0, e1, _ in push_events]
+    push_ms = []
+    for e0, e1 in evs:
+        ms = ctypes.c_float()
+        L.check(L.lib().oecl_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+        push_ms.append(ms.value)
+        L.lib().oecl_event_destroy(e0)
+        L.lib().oecl_event_destroy(e1)
     s = 8 if a.dtype == 'f64' else 4
     npx = g.getNpExt()
-    bytes_per = [11 * s * c + 24 * npx for _, _, c in push_events]
+    bytes_per = [11 * s * c + 24 * npx for c in counts]
     avg_ms = sum(push_ms) / len(push_ms)
     avg_bytes = sum(bytes_per) / len(bytes_per)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    # Poisson solves/s of the direct device solve alone
+    # Poisson solves/s of the direct device solve (phi only, then phi -> E)
     q = b.rho.clone()
     torch.cuda.synchronize()
     tp = time.perf_counter()
-    nsol = 50
+    nsol = 100
     for _ in range(nsol):
-        ps.solvePhi(q)
+        ps.solve(q)
     torch.cuda.synchronize()
     poisson_rate = nsol / (time.perf_counter() - tp)
     res = ps.residual(q)
@@ -209,7 +217,8 @@ def main():
                                '%d macro-electrons, Gaussian bunches sigma 1.5 mm 1.15e11 ppb x4, '
                                'dt 5e-12 s, Furman copper delta_max 1.3' % n,
                    'grid': [512, 384], 'particles': n, 'dt': 5e-12,
-                   'macro_particles_start_end': [counts[0], p.macroParticleCount]},
+                   'macro_particles_start_end': [counts[0], p.macroParticleCount],
+                   'absorbed_total': st['absorbed_total'], 'emitted_total': st['emitted_total']},
         'poisson_solves_per_s': poisson_rate,
         'poisson_relres': res,
         'poisson_capacitance_rows': ps.capacitanceRows,

@@ -116,6 +116,13 @@ int oecl_deposit_add(const oecl_grid_t *g, int dtype, const void *data, uint64_t
 int oecl_deposit_add_aos(const oecl_grid_t *g, int dtype, const void *aos, uint32_t n, double q,
                          double *rho, void *stream);
 
+/* Moments of the particle set in FP64 with a fixed reduction order: out[6] = (sum w,
+ * sum w|v|^2, sum w^2|v|^2, sum w x, sum w y, count).  Replaces getMeanWeight's loop
+ * (particles.pyx:39-47), the driver's numpy.sum of the weights (example01_basic.py:56) and
+ * util.averageEnergy (util.pyx:8-20).  work: 512*6 doubles. */
+int oecl_particle_moments(int dtype, const void *data, uint64_t stride, uint32_t n, double *out,
+                          double *work, void *stream);
+
 /* ---- particleBoundary.pyx ----------------------------------------------------------- */
 
 /* indexInside: AbsorbRectangular :84-98, AbsorbElliptical :227-242, AbsorbArbitraryCutCell
@@ -187,6 +194,58 @@ int oecl_add_and_remove(int dtype, void *data, void *spare, uint64_t stride, uin
                         const uint8_t *keep, const void *add, uint32_t nAdd, int mode,
                         uint32_t *newCount, void *work, void *stream);
 
+/* ---- sorting (particles.pyx:431-459 _quicksortByColumn) ----------------------------- */
+
+/* Stable LSD radix sort of keys[n] ascending with 32-bit payloads (8 key bits per pass,
+ * ceil(bits/8) passes).  work: oecl_sort_work_bytes(n) bytes. */
+uint64_t oecl_sort_work_bytes(uint32_t n);
+int oecl_radix_sort_u64(uint64_t *keys, uint32_t *vals, uint32_t n, int bits, void *work,
+                        void *stream);
+/* Particles.sortByColumn (particles.pyx:49-50): the block sorted ascending by one column
+ * (stable) is written to spare; keys[n], perm[n] scratch; work: oecl_sort_work_bytes(n). */
+int oecl_sort_by_column(int dtype, const void *data, void *spare, uint64_t stride, uint32_t n,
+                        int column, uint64_t *keys, uint32_t *perm, void *work, void *stream);
+/* dst[:, i] = src[:, perm[i]], i < n. */
+int oecl_permute_particles(int dtype, const void *src, void *dst, uint64_t stride,
+                           const uint32_t *perm, uint32_t n, void *stream);
+
+/* ---- particleManagement.pyx --------------------------------------------------------- */
+
+/* w *= f for all n particles (particleManagement.pyx:45-46, :58-59). */
+int oecl_scale_weights(int dtype, void *data, uint64_t stride, uint32_t n, double f,
+                       void *stream);
+int oecl_fill_u8(uint8_t *a, uint32_t n, uint8_t v, void *stream);
+/* globalRandom's kill set (:47-50): keep[i] = 0 for a uniformly random subset of k of the n
+ * particles (the k smallest Philox keys).  keys[n], perm[n] scratch; work:
+ * oecl_sort_work_bytes(n). */
+int oecl_random_kill(uint32_t n, uint32_t k, uint32_t streamIndex, uint32_t seed0,
+                     uint32_t seed1, uint8_t *keep, uint64_t *keys, uint32_t *perm, void *work,
+                     void *stream);
+/* globalRandom's clones (:60-63): add[nAdd][6] = copies of uniformly random particles. */
+int oecl_random_clones(int dtype, const void *data, uint64_t stride, uint32_t n, uint32_t nAdd,
+                       uint32_t streamIndex, uint32_t seed0, uint32_t seed1, void *add,
+                       void *stream);
+/* _searchSorted (:244-261) of three values over the weight column sorted ascending:
+ * out[k] = first index whose weight exceeds v_k. */
+int oecl_search_sorted_weights(int dtype, const void *data, uint64_t stride, uint32_t n,
+                               double v0, double v1, double v2, uint32_t *out, void *stream);
+/* localRandom's split of particles [first, first+cnt) (:130-163) into add[cnt][6]. */
+int oecl_split_heaviest(const oecl_wall_t *wall, int dtype, void *data, uint64_t stride,
+                        uint32_t first, uint32_t cnt, double dx, double dy, double lx, double ly,
+                        uint32_t streamIndex, uint32_t seed0, uint32_t seed1, void *add,
+                        void *stream);
+/* localRandom's merge (:166-236) over particles [0, nc): Z-order (Morton) sort of the 5-D
+ * coordinates scaled by pw[5] (device), groups of G (2..8) Z-neighbours within squared
+ * distance disLimit of the group's first member, lightest groups first, until at least
+ * toMerge particles are removed.  keep[nc] written; *merged (device) = removed count;
+ * *groupsUsed (host) = merged groups.  work: oecl_merge_work_bytes(nc).  This call reads
+ * the sorted group list back to the host once (it synchronises the stream). */
+uint64_t oecl_merge_work_bytes(uint32_t nc);
+int oecl_morton_merge(int dtype, void *data, uint64_t stride, uint32_t nc, const double *pw,
+                      int G, double disLimit, uint32_t toMerge, int scheme, uint32_t streamIndex,
+                      uint32_t seed0, uint32_t seed1, uint8_t *keep, uint32_t *merged,
+                      uint32_t *groupsUsed, void *work, void *stream);
+
 /* ---- poissonSolver.pyx -------------------------------------------------------------- */
 
 /* FIT operator of poissonSolver.pyx:44-55 as a 5-point stencil on the full extended grid:
@@ -227,6 +286,56 @@ int oecl_fastpoisson_create(uint32_t nxExt, uint32_t nyExt, const double *Sx, co
 /* phi (npExt, zero on non-inside nodes) = A^-1 q on inside nodes. */
 int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, double *phi, void *stream);
 int oecl_fastpoisson_destroy(oecl_fastpoisson *h);
+
+/* ---- the whole plain step (example01_basic.py:52-97, example02/04 variants) ---------- */
+
+/* Arguments of oecl_step_run.  Counts live on the device: nDev (particles), stepDev (step
+ * index, also the Philox stream of the emitter and the index into beamScale).  stats[8]:
+ * 0 total absorbed, 1 total emitted, 2 status bits (1 more hits than absCap, 2 more
+ * secondaries than secCap, 4 capacity exceeded), 3 cut-cell crossings not found, 4 hits and
+ * 5 secondaries of the last step.  countHist[step] receives the particle count at the start
+ * of each step (step < histCap). */
+typedef struct {
+    int32_t dtype;
+    void *data;                 /* particle SoA block, 6 rows of `stride` (= capacity) */
+    uint64_t stride;
+    uint32_t *nDev;
+    uint8_t *isInside;          /* [stride] */
+    double *rho, *q;            /* [npExt]: charge of the current particles (in / out), scratch */
+    const double *profile;      /* [npExt] beam profile, or NULL */
+    const double *beamScale;    /* [nScale] beam scale per step, or NULL */
+    uint32_t nScale;
+    double *phi, *E;            /* [npExt], [2 npExt] */
+    const int32_t *e2nRowPtr, *e2nCol;   /* phiToEAtGridPoints as CSR */
+    const double *e2nVal;
+    const double *bField;       /* [3 npExt] for typeB 2 */
+    double b0, b1, b2;          /* constant B for typeB 1 */
+    int32_t typeB;
+    double dt, qm, qe;          /* time step, q/m, particle charge */
+    void *absorbed, *normals, *remaining;   /* [absCap][6], [absCap][2], [absCap] */
+    uint32_t absCap;
+    uint32_t *nSec, *secOff;    /* [absCap], [absCap + 1] */
+    double *impact;             /* [absCap][4] */
+    void *secondaries;          /* [secCap][6] */
+    uint32_t secCap;
+    int32_t emit;               /* 0: absorb without emission */
+    uint32_t seed0, seed1;
+    uint32_t *stepDev;
+    uint32_t *stats;
+    uint32_t *countHist;
+    uint32_t histCap;
+    void *work;                 /* oecl_step_work_bytes(stride, absCap) */
+    void *evPushBegin, *evPushEnd;   /* hipEvent_t recorded around the push kernel, or NULL */
+} oecl_step_args;
+
+uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap);
+/* hipEvent helpers for callers without a HIP binding. */
+int oecl_event_create(void **ev);
+int oecl_event_destroy(void *ev);
+int oecl_event_elapsed_ms(void *begin, void *end, float *ms);
+/* One full time step with the device-resident counts of `a`; no host synchronisation. */
+int oecl_step_run(const oecl_grid_t *g, const oecl_wall_t *wall, const oecl_furman_t *prm,
+                  oecl_fastpoisson *fp, const oecl_step_args *a, void *stream);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,28 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import randomGen
+
+_VP = ctypes.c_void_p
+
+
+class StepArgs(ctypes.Structure):
+    """Mirror of oecl_step_args (include/oecl.h)."""
+    _fields_ = [('dtype', ctypes.c_int32), ('data', _VP), ('stride', ctypes.c_uint64),
+                ('nDev', _VP), ('isInside', _VP), ('rho', _VP), ('q', _VP), ('profile', _VP),
+                ('beamScale', _VP), ('nScale', ctypes.c_uint32), ('phi', _VP), ('E', _VP),
+                ('e2nRowPtr', _VP), ('e2nCol', _VP), ('e2nVal', _VP), ('bField', _VP),
+                ('b0', ctypes.c_double), ('b1', ctypes.c_double), ('b2', ctypes.c_double),
+                ('typeB', ctypes.c_int32), ('dt', ctypes.c_double), ('qm', ctypes.c_double),
+                ('qe', ctypes.c_double), ('absorbed', _VP), ('normals', _VP), ('remaining', _VP),
+                ('absCap', ctypes.c_uint32), ('nSec', _VP), ('secOff', _VP), ('impact', _VP),
+                ('secondaries', _VP), ('secCap', ctypes.c_uint32), ('emit', ctypes.c_int32),
+                ('seed0', ctypes.c_uint32), ('seed1', ctypes.c_uint32), ('stepDev', _VP),
+                ('stats', _VP), ('countHist', _VP), ('histCap', ctypes.c_uint32), ('work', _VP),
+                ('evPushBegin', _VP), ('evPushEnd', _VP)]
+
+
+ST_ABS_OVERFLOW, ST_SEC_OVERFLOW, ST_CAP_OVERFLOW = 1, 2, 4
 
 
 class Buildup:
@@ -184,6 +206,112 @@ class Buildup:
         self._rho_valid = True
         self._record(na, nsec)
         self.it += 1
+
+    # ------------------------------------------------------------------ native step
+    def native_setup(self, absCap=None, secCap=None, maxSteps=100000):
+        """Allocates the device state of the one-call step (oecl_step_run).  The beam scale of
+        every step up to maxSteps is tabulated now (it depends on the step index only)."""
+        p, pb, ps = self.p, self.pb, self.ps
+        if ps.method != 'direct':
+            raise ValueError('the native step needs PoissonSolver(method="direct")')
+        dev = L.device()
+        n = p.macroParticleCount
+        cap = p.capacity
+        absCap = int(absCap or max(65536, cap // 8))
+        secCap = int(secCap or 2 * absCap)
+        self.absCap, self.secCap, self.maxSteps = absCap, secCap, int(maxSteps)
+        pb.absorbedParticles = torch.empty((absCap, 6), dtype=p.dtype, device=dev)
+        pb.normalVectors = torch.empty((absCap, 2), dtype=p.dtype, device=dev)
+        pb.remainingTimeStep = torch.empty(absCap, dtype=p.dtype, device=dev)
+        self._nsec = torch.empty(absCap, dtype=torch.int32, device=dev)
+        self._secoff = torch.empty(absCap + 1, dtype=torch.int32, device=dev)
+        self._impact = torch.empty(4 * absCap, dtype=torch.float64, device=dev)
+        self._sec = torch.empty((secCap, 6), dtype=p.dtype, device=dev)
+        self._nDev = torch.tensor([n], dtype=torch.int32, device=dev)
+        self._stepDev = torch.tensor([self.it], dtype=torch.int32, device=dev)
+        self._stats = torch.zeros(8, dtype=torch.int32, device=dev)
+        self._hist = torch.zeros(self.maxSteps, dtype=torch.int32, device=dev)
+        self._stepwork = torch.empty(int(L.lib().oecl_step_work_bytes(L.u64(cap), L.u32(absCap))) // 8 + 64,
+                                     dtype=torch.float64, device=dev)
+        if self.beam is not None:
+            sc = np.array([self.beam.getScale(k * self.dt) for k in range(self.maxSteps)])
+            self._scale = L.to_dev(sc)
+            prof = self.beam.getProfile()
+        else:
+            self._scale, prof = None, None
+        k0, k1 = randomGen.key()
+        a = StepArgs()
+        a.dtype = p.code
+        a.data = p._data.data_ptr()
+        a.stride = cap
+        a.nDev = self._nDev.data_ptr()
+        a.isInside = p.isInside.data_ptr()
+        a.rho = self.rho.data_ptr()
+        a.q = self.q.data_ptr()
+        a.profile = prof.data_ptr() if prof is not None else None
+        a.beamScale = self._scale.data_ptr() if self._scale is not None else None
+        a.nScale = self.maxSteps if self._scale is not None else 0
+        a.phi = ps.phi.data_ptr()
+        a.E = ps.eAtGridPoints.data_ptr()
+        a.e2nRowPtr = ps.rowPtr.data_ptr()
+        a.e2nCol = ps.col.data_ptr()
+        a.e2nVal = ps.val.data_ptr()
+        a.bField = self.bField.data_ptr() if self.bField is not None else None
+        a.b0, a.b1, a.b2 = [float(v) for v in p.bConst]
+        a.typeB = self.typeB
+        a.dt = self.dt
+        a.qm = p.particleCharge / p.particleMass
+        a.qe = p.particleCharge
+        a.absorbed = pb.absorbedParticles.data_ptr()
+        a.normals = pb.normalVectors.data_ptr()
+        a.remaining = pb.remainingTimeStep.data_ptr()
+        a.absCap = absCap
+        a.nSec = self._nsec.data_ptr()
+        a.secOff = self._secoff.data_ptr()
+        a.impact = self._impact.data_ptr()
+        a.secondaries = self._sec.data_ptr()
+        a.secCap = secCap
+        a.emit = 1 if self.secondaryEmission else 0
+        a.seed0, a.seed1 = k0, k1
+        a.stepDev = self._stepDev.data_ptr()
+        a.stats = self._stats.data_ptr()
+        a.countHist = self._hist.data_ptr()
+        a.histCap = self.maxSteps
+        a.work = self._stepwork.data_ptr()
+        self._args = a
+        self._native_it0 = self.it
+        if not self._rho_valid:
+            self.deposit_all()
+
+    def step_native(self):
+        """One full step in one library call; counts stay on the device (call sync())."""
+        if self.it - self._native_it0 >= self.maxSteps:
+            raise L.OeclError('native step: maxSteps exhausted; call native_setup again')
+        L.check(L.lib().oecl_step_run(ctypes.byref(self.gd), ctypes.byref(self.pb.desc),
+                                      ctypes.byref(self.em.desc), self.ps._fp,
+                                      ctypes.byref(self._args), L.stream()), 'step')
+        self.it += 1
+
+    def sync(self):
+        """Reads the device counts back, updates the Particles object and checks the status
+        word (raises on any overflow)."""
+        st = self._stats.cpu().tolist()
+        self.p.macroParticleCount = int(self._nDev.item())
+        self.pb.absorbedMacroParticleCount = min(st[4], self.absCap)
+        if st[2] & ST_ABS_OVERFLOW:
+            raise L.OeclError('more wall hits in one step than absCap=%d' % self.absCap)
+        if st[2] & ST_SEC_OVERFLOW:
+            raise L.OeclError('more secondaries in one step than secCap=%d' % self.secCap)
+        if st[2] & ST_CAP_OVERFLOW:
+            raise L.OeclError('particle count exceeded the capacity %d' % self.p.capacity)
+        return {'absorbed_total': st[0], 'emitted_total': st[1], 'not_found': st[3],
+                'absorbed_last': st[4], 'emitted_last': st[5]}
+
+    def counts(self, first, last):
+        """Particle counts at the start of steps first .. last-1 (native steps)."""
+        a = first - self._native_it0
+        b = last - self._native_it0
+        return self._hist[a:b].cpu().tolist()
 
     def _record(self, na, nsec):
         h = self.history

@@ -100,32 +100,6 @@ __global__ void __launch_bounds__(OECL_SCAN_THREADS)
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == OECL_SCAN_THREADS - 1) out[n] = run;
 }
 
-// Visits every element of the block's tile with its exclusive hole rank (holes before it).
-// F(i, isHole, holesBefore).
-template <typename F>
-__device__ __forceinline__ void for_each_ranked(const uint8_t *__restrict__ flags, uint32_t n,
-                                                const uint32_t *__restrict__ tileOff, F f)
-{
-    __shared__ uint32_t lds[4];
-    uint64_t base = (uint64_t)blockIdx.x * OECL_SCAN_TILE + threadIdx.x * OECL_SCAN_ITEMS;
-    uint8_t fl[OECL_SCAN_ITEMS];
-    uint32_t c = 0;
-#pragma unroll
-    for (int k = 0; k < OECL_SCAN_ITEMS; ++k) {
-        uint64_t i = base + k;
-        fl[k] = i < n ? flags[i] : 1;
-        c += fl[k] == 0;
-    }
-    uint32_t tot;
-    uint32_t run = tileOff[blockIdx.x] + block_excl_scan(c, lds, &tot);
-#pragma unroll
-    for (int k = 0; k < OECL_SCAN_ITEMS; ++k) {
-        uint64_t i = base + k;
-        if (i < n) f((uint32_t)i, fl[k] == 0, run);
-        run += fl[k] == 0;
-    }
-}
-
 // ---------------------------------------------------------------------------- inside
 template <typename T>
 __global__ void k_index_inside(WallDev w, Soa<T> p, uint32_t n, uint8_t *isInside, int32_t *clipped)

@@ -5,41 +5,10 @@
 // library is built with -ffp-contract=off, so the FP64 path is bit-identical to the
 // reference's own FP64 arithmetic for the same inputs.
 #include "common.h"
+#include "particles_dev.h"
 #include "wall.h"
 
 namespace {
-
-// particles.pyx:400-418 for one particle, with the cell index clamped into the grid so an
-// escaped particle can never address memory outside the node arrays (the reference has no
-// bounds check there; inside the grid the result is unchanged).
-template <typename T>
-struct Weights {
-    uint32_t cell;
-    T wLL, wUL, wLU, wUU;
-};
-
-template <typename T>
-__device__ __forceinline__ Weights<T> grid_weights(T x, T y, T dxi, T dyi, T lxh, T lyh,
-                                                   uint32_t nx, uint32_t ny)
-{
-    T fx = (x + lxh) * dxi;
-    T fy = (y + lyh) * dyi;
-    T fxc = fx < T(0) ? T(0) : fx;
-    T fyc = fy < T(0) ? T(0) : fy;
-    uint32_t ix = (uint32_t)fxc;
-    uint32_t iy = (uint32_t)fyc;
-    if (ix > nx - 2) ix = nx - 2;
-    if (iy > ny - 2) iy = ny - 2;
-    fx = fx - (T)ix;
-    fy = fy - (T)iy;
-    Weights<T> w;
-    w.cell = ix + iy * nx;
-    w.wLL = (1 - fx) * (1 - fy);
-    w.wUL = fx * (1 - fy);
-    w.wLU = (1 - fx) * fy;
-    w.wUU = fx * fy;
-    return w;
-}
 
 template <typename T>
 __global__ void k_calc_weights(oecl_grid_t g, Soa<T> p, uint32_t n, uint32_t *inCell, T *w4,
@@ -60,17 +29,6 @@ __global__ void k_zero(double *a, uint32_t n)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = 0.;
-}
-
-// Four FP64 atomic adds per particle: wk * weight * q as in particles.pyx:344-351.
-template <typename T>
-__device__ __forceinline__ void deposit4(double *rho, uint32_t c, uint32_t nx, double wLL,
-                                         double wUL, double wLU, double wUU, double wq)
-{
-    unsafeAtomicAdd(rho + c, wLL * wq);
-    unsafeAtomicAdd(rho + c + 1, wUL * wq);
-    unsafeAtomicAdd(rho + c + nx, wLU * wq);
-    unsafeAtomicAdd(rho + c + nx + 1, wUU * wq);
 }
 
 template <typename T>
@@ -139,22 +97,6 @@ __global__ void k_field_to_particles(const double *f, const uint32_t *inCell, co
     }
 }
 
-// Boris rotation + second half kick + drift of particles.pyx:272-321, from the velocity
-// after the first half kick.
-template <typename T>
-__device__ __forceinline__ void boris_rotate(T &vx, T &vy, T &vz, T t0, T t1, T t2, T kex,
-                                             T key)
-{
-    T vs0 = vx + vy * t2 - vz * t1;
-    T vs1 = vy + vz * t0 - vx * t2;
-    T vs2 = vz + vx * t1 - vy * t0;
-    T f = T(2) / (T(1) + t0 * t0 + t1 * t1 + t2 * t2);
-    T s0 = t0 * f, s1 = t1 * f, s2 = t2 * f;
-    vx += vs1 * s2 - vs2 * s1 + kex;
-    vy += vs2 * s0 - vs0 * s2 + key;
-    vz += vs0 * s1 - vs1 * s0;
-}
-
 template <typename T>
 __global__ void k_boris_push(Soa<T> p, uint32_t n, const T *eAt, const T *bAt, T b0, T b1,
                              T b2, int typeB, T dt, T k)
@@ -191,67 +133,99 @@ __global__ void k_boris_push(Soa<T> p, uint32_t n, const T *eAt, const T *bAt, T
 template <typename T, int TYPEB>
 __global__ void __launch_bounds__(256) k_push_fused(oecl_grid_t g, WallDev wall, Soa<T> p,
                                                     uint32_t n, const double *__restrict__ E,
-                                                    const double *__restrict__ B, T b0, T b1,
-                                                    T b2, T dt, T k, double q,
+                                                    const double *__restrict__ B, double b0,
+                                                    double b1, double b2, double dt, double k,
+                                                    double q,
                                                     uint8_t *__restrict__ isInside,
                                                     double *__restrict__ rhoNext)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const T dxi = (T)(1. / g.dx), dyi = (T)(1. / g.dy);
-    const T lxh = (T)g.lxHalfExt, lyh = (T)g.lyHalfExt;
-    T x = p(0, i), y = p(1, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
-    Weights<T> w = grid_weights<T>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
-    const uint32_t c = w.cell, nx = g.nx, np = g.np;
-    T ex = w.wLL * (T)E[c];
-    T ey = w.wLL * (T)E[c + np];
-    ex += w.wUL * (T)E[c + 1];
-    ey += w.wUL * (T)E[c + 1 + np];
-    ex += w.wLU * (T)E[c + nx];
-    ey += w.wLU * (T)E[c + nx + np];
-    ex += w.wUU * (T)E[c + nx + 1];
-    ey += w.wUU * (T)E[c + nx + 1 + np];
-    vx += k * ex;
-    vy += k * ey;
-    if (TYPEB != 0) {
-        T t0, t1, t2;
-        if (TYPEB == 1) {
-            t0 = k * b0; t1 = k * b1; t2 = k * b2;
-        } else {
-            T bx = w.wLL * (T)B[c], by = w.wLL * (T)B[c + np], bz = w.wLL * (T)B[c + 2 * np];
-            bx += w.wUL * (T)B[c + 1];
-            by += w.wUL * (T)B[c + 1 + np];
-            bz += w.wUL * (T)B[c + 1 + 2 * np];
-            bx += w.wLU * (T)B[c + nx];
-            by += w.wLU * (T)B[c + nx + np];
-            bz += w.wLU * (T)B[c + nx + 2 * np];
-            bx += w.wUU * (T)B[c + nx + 1];
-            by += w.wUU * (T)B[c + nx + 1 + np];
-            bz += w.wUU * (T)B[c + nx + 1 + 2 * np];
-            t0 = k * bx; t1 = k * by; t2 = k * bz;
-        }
-        boris_rotate<T>(vx, vy, vz, t0, t1, t2, k * ex, k * ey);
-        p(4, i) = vz;
-    }
-    x += dt * vx;
-    y += dt * vy;
-    uint8_t in = wall_inside<T>(wall, x, y);  // may clip x, y (cut-cell wall)
-    p(0, i) = x;
-    p(1, i) = y;
-    p(2, i) = vx;
-    p(3, i) = vy;
+    // FP32 storage is widened on load; the arithmetic runs in FP64 registers.
+    using C = double;
+    const C dxi = 1. / g.dx, dyi = 1. / g.dy, lxh = g.lxHalfExt, lyh = g.lyHalfExt;
+    C x = p(0, i), y = p(1, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
+    Weights<C> w = grid_weights<C>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
+    gather_push<C, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vx, vy, vz);
+    uint8_t in = wall_inside<C>(wall, x, y);  // may clip x, y (cut-cell wall)
+    p(0, i) = (T)x;
+    p(1, i) = (T)y;
+    p(2, i) = (T)vx;
+    p(3, i) = (T)vy;
+    if (TYPEB != 0) p(4, i) = (T)vz;
     isInside[i] = in;
     if (rhoNext != nullptr && in) {
-        Weights<T> w2 = grid_weights<T>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
-        double wt = (double)p(5, i);
-        unsafeAtomicAdd(rhoNext + w2.cell, (double)w2.wLL * wt * q);
-        unsafeAtomicAdd(rhoNext + w2.cell + 1, (double)w2.wUL * wt * q);
-        unsafeAtomicAdd(rhoNext + w2.cell + nx, (double)w2.wLU * wt * q);
-        unsafeAtomicAdd(rhoNext + w2.cell + nx + 1, (double)w2.wUU * wt * q);
+        Weights<C> w2 = grid_weights<C>((C)(T)x, (C)(T)y, dxi, dyi, lxh, lyh, g.nx, g.ny);
+        deposit_corners<C>(rhoNext, w2, g.nx, (double)p(5, i), q);
+    }
+}
+
+// Moments of the particle set, FP64, fixed reduction order (bit-reproducible):
+//   m[0] = sum w, m[1] = sum w |v|^2, m[2] = sum w^2 |v|^2, m[3] = sum w x, m[4] = sum w y,
+//   m[5] = count.  Block partials first, then one block sums them.
+constexpr int kMomThreads = 256;
+constexpr int kMom = 6;
+
+template <typename T>
+__global__ void __launch_bounds__(kMomThreads)
+    k_moments_partial(Soa<T> p, uint32_t n, double *__restrict__ partials)
+{
+    __shared__ double lds[kMomThreads / 64][kMom];
+    double v[kMom] = {0., 0., 0., 0., 0., 0.};
+    for (uint32_t i = blockIdx.x * kMomThreads + threadIdx.x; i < n; i += gridDim.x * kMomThreads) {
+        double w = p(5, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
+        double v2 = vx * vx + vy * vy + vz * vz;
+        v[0] += w;
+        v[1] += w * v2;
+        v[2] += w * w * v2;
+        v[3] += w * (double)p(0, i);
+        v[4] += w * (double)p(1, i);
+        v[5] += 1.;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kMom; ++k) {
+        double x = v[k];
+        for (int d = 32; d > 0; d >>= 1) x += __shfl_down(x, d, 64);
+        if (lane == 0) lds[wave][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kMom) {
+        double s = 0.;
+        for (int w = 0; w < kMomThreads / 64; ++w) s += lds[w][threadIdx.x];
+        partials[blockIdx.x * kMom + threadIdx.x] = s;
+    }
+}
+
+__global__ void k_moments_final(const double *__restrict__ partials, uint32_t nb, double *__restrict__ out)
+{
+    if (threadIdx.x < kMom) {
+        double s = 0.;
+        for (uint32_t b = 0; b < nb; ++b) s += partials[b * kMom + threadIdx.x];
+        out[threadIdx.x] = s;
     }
 }
 
 }  // namespace
+
+OECL_EXPORT int oecl_particle_moments(int dtype, const void *data, uint64_t stride, uint32_t n,
+                                      double *out, double *work, void *stream)
+{
+    if (!out || !work || (n && !data)) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t nb = 512;
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_moments_partial<double>, dim3(nb), dim3(kMomThreads), 0, s,
+                           Soa<double>{(double *)data, stride}, n, work);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_moments_partial<float>, dim3(nb), dim3(kMomThreads), 0, s,
+                           Soa<float>{(float *)data, stride}, n, work);
+    else
+        return OECL_ERR_ARG;
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(64), 0, s, work, nb, out);
+    OECL_CHECK_LAUNCH("oecl_particle_moments");
+    return OECL_OK;
+}
 
 OECL_EXPORT int oecl_calc_grid_weights(const oecl_grid_t *g, int dtype, const void *data,
                                        uint64_t stride, uint32_t n, uint32_t *inCell, void *w4,
@@ -399,19 +373,19 @@ static int launch_fused(const oecl_grid_t *g, const WallDev &wd, void *data, uin
                         double *rhoNext, hipStream_t s)
 {
     double k = typeB == 0 ? qm * dt : 0.5 * qm * dt;
-    T b0 = typeB == 1 ? (T)bConst[0] : T(0), b1 = typeB == 1 ? (T)bConst[1] : T(0),
-      b2 = typeB == 1 ? (T)bConst[2] : T(0);
+    double b0 = typeB == 1 ? bConst[0] : 0., b1 = typeB == 1 ? bConst[1] : 0.,
+           b2 = typeB == 1 ? bConst[2] : 0.;
     Soa<T> p{(T *)data, stride};
     dim3 gr(oecl_blocks(n, 256)), bl(256);
     if (typeB == 0)
         hipLaunchKernelGGL((k_push_fused<T, 0>), gr, bl, 0, s, *g, wd, p, n, E, B, b0, b1, b2,
-                           (T)dt, (T)k, q, isInside, rhoNext);
+                           dt, k, q, isInside, rhoNext);
     else if (typeB == 1)
         hipLaunchKernelGGL((k_push_fused<T, 1>), gr, bl, 0, s, *g, wd, p, n, E, B, b0, b1, b2,
-                           (T)dt, (T)k, q, isInside, rhoNext);
+                           dt, k, q, isInside, rhoNext);
     else
         hipLaunchKernelGGL((k_push_fused<T, 2>), gr, bl, 0, s, *g, wd, p, n, E, B, b0, b1, b2,
-                           (T)dt, (T)k, q, isInside, rhoNext);
+                           dt, k, q, isInside, rhoNext);
     OECL_CHECK_LAUNCH("oecl_push_fused");
     return OECL_OK;
 }

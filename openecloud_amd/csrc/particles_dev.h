@@ -1,0 +1,108 @@
+// Per-particle device helpers shared by the particle kernels and the fused step.
+#pragma once
+#include "common.h"
+
+// particles.pyx:400-418 for one particle, with the cell index clamped into the grid so an
+// escaped particle can never address memory outside the node arrays (the reference has no
+// bounds check there; inside the grid the result is unchanged).
+template <typename T>
+struct Weights {
+    uint32_t cell;
+    T wLL, wUL, wLU, wUU;
+};
+
+template <typename T>
+__device__ __forceinline__ Weights<T> grid_weights(T x, T y, T dxi, T dyi, T lxh, T lyh,
+                                                   uint32_t nx, uint32_t ny)
+{
+    T fx = (x + lxh) * dxi;
+    T fy = (y + lyh) * dyi;
+    T fxc = fx < T(0) ? T(0) : fx;
+    T fyc = fy < T(0) ? T(0) : fy;
+    uint32_t ix = (uint32_t)fxc;
+    uint32_t iy = (uint32_t)fyc;
+    if (ix > nx - 2) ix = nx - 2;
+    if (iy > ny - 2) iy = ny - 2;
+    fx = fx - (T)ix;
+    fy = fy - (T)iy;
+    Weights<T> w;
+    w.cell = ix + iy * nx;
+    w.wLL = (1 - fx) * (1 - fy);
+    w.wUL = fx * (1 - fy);
+    w.wLU = (1 - fx) * fy;
+    w.wUU = fx * fy;
+    return w;
+}
+
+// Four FP64 atomic adds: (w_k * weight) * q at the cell's corners (particles.pyx:344-351).
+template <typename T>
+__device__ __forceinline__ void deposit_corners(double *rho, const Weights<T> &w, uint32_t nx,
+                                                double wt, double q)
+{
+    unsafeAtomicAdd(rho + w.cell, (double)w.wLL * wt * q);
+    unsafeAtomicAdd(rho + w.cell + 1, (double)w.wUL * wt * q);
+    unsafeAtomicAdd(rho + w.cell + nx, (double)w.wLU * wt * q);
+    unsafeAtomicAdd(rho + w.cell + nx + 1, (double)w.wUU * wt * q);
+}
+
+// Boris rotation + second half kick of particles.pyx:272-321, from the velocity after the
+// first half kick.
+template <typename T>
+__device__ __forceinline__ void boris_rotate(T &vx, T &vy, T &vz, T t0, T t1, T t2, T kex,
+                                             T key)
+{
+    T vs0 = vx + vy * t2 - vz * t1;
+    T vs1 = vy + vz * t0 - vx * t2;
+    T vs2 = vz + vx * t1 - vy * t0;
+    T f = T(2) / (T(1) + t0 * t0 + t1 * t1 + t2 * t2);
+    T s0 = t0 * f, s1 = t1 * f, s2 = t2 * f;
+    vx += vs1 * s2 - vs2 * s1 + kex;
+    vy += vs2 * s0 - vs0 * s2 + key;
+    vz += vs0 * s1 - vs1 * s0;
+}
+
+// Gather E (and B for TYPEB 2) at the particle and apply the Boris push (particles.pyx:
+// 272-333 with _fieldToParticles's accumulation order).  Positions advance in x and y only.
+template <typename T, int TYPEB>
+__device__ __forceinline__ void gather_push(const Weights<T> &w, uint32_t nx, uint32_t np,
+                                            const double *__restrict__ E,
+                                            const double *__restrict__ B, T b0, T b1, T b2, T dt,
+                                            T k, T &x, T &y, T &vx, T &vy, T &vz)
+{
+    const uint32_t c = w.cell;
+    T ex = w.wLL * (T)E[c];
+    T ey = w.wLL * (T)E[c + np];
+    ex += w.wUL * (T)E[c + 1];
+    ey += w.wUL * (T)E[c + 1 + np];
+    ex += w.wLU * (T)E[c + nx];
+    ey += w.wLU * (T)E[c + nx + np];
+    ex += w.wUU * (T)E[c + nx + 1];
+    ey += w.wUU * (T)E[c + nx + 1 + np];
+    vx += k * ex;
+    vy += k * ey;
+    if (TYPEB != 0) {
+        T t0, t1, t2;
+        if (TYPEB == 1) {
+            t0 = k * b0;
+            t1 = k * b1;
+            t2 = k * b2;
+        } else {
+            T bx = w.wLL * (T)B[c], by = w.wLL * (T)B[c + np], bz = w.wLL * (T)B[c + 2 * np];
+            bx += w.wUL * (T)B[c + 1];
+            by += w.wUL * (T)B[c + 1 + np];
+            bz += w.wUL * (T)B[c + 1 + 2 * np];
+            bx += w.wLU * (T)B[c + nx];
+            by += w.wLU * (T)B[c + nx + np];
+            bz += w.wLU * (T)B[c + nx + 2 * np];
+            bx += w.wUU * (T)B[c + nx + 1];
+            by += w.wUU * (T)B[c + nx + 1 + np];
+            bz += w.wUU * (T)B[c + nx + 1 + 2 * np];
+            t0 = k * bx;
+            t1 = k * by;
+            t2 = k * bz;
+        }
+        boris_rotate<T>(vx, vy, vz, t0, t1, t2, k * ex, k * ey);
+    }
+    x += dt * vx;
+    y += dt * vy;
+}

@@ -33,6 +33,32 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *lds4, 
     return base + x - v;
 }
 
+// Visits every element of the block's tile with its exclusive hole rank (holes before it):
+// F(i, isHole, holesBefore).  Elements at or beyond n are skipped.
+template <typename F>
+__device__ __forceinline__ void for_each_ranked(const uint8_t *__restrict__ flags, uint32_t n,
+                                                const uint32_t *__restrict__ tileOff, F f)
+{
+    __shared__ uint32_t lds[4];
+    uint64_t base = (uint64_t)blockIdx.x * OECL_SCAN_TILE + threadIdx.x * OECL_SCAN_ITEMS;
+    uint8_t fl[OECL_SCAN_ITEMS];
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < OECL_SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        fl[k] = i < n ? flags[i] : 1;
+        c += fl[k] == 0;
+    }
+    uint32_t tot;
+    uint32_t run = tileOff[blockIdx.x] + block_excl_scan(c, lds, &tot);
+#pragma unroll
+    for (int k = 0; k < OECL_SCAN_ITEMS; ++k) {
+        uint64_t i = base + k;
+        if (i < n) f((uint32_t)i, fl[k] == 0, run);
+        run += fl[k] == 0;
+    }
+}
+
 // Host launchers (defined in boundary.hip): tile counts of zero flags and their exclusive
 // scan.  counts[nTiles], offsets[nTiles + 1]; offsets[nTiles] is the total.
 int oecl_launch_hole_scan(const uint8_t *flags, uint32_t n, uint32_t *counts,

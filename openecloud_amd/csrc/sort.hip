@@ -1,0 +1,210 @@
+// Stable LSD radix sort of 64-bit keys with 32-bit payloads, and the particle permutation.
+//
+// Each pass handles 8 key bits: a per-tile digit histogram (digit-major layout, so one
+// exclusive scan gives every tile's output offset per digit), then a stable scatter whose
+// in-tile ranks come from a wave64 multisplit: the lanes holding the same digit are found
+// with eight ballots, their order within the wave from a popcount of the lower-lane mask,
+// and the order across waves and sub-tiles from LDS counters.
+#include "common.h"
+#include "scan.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;
+constexpr int kRadix = 256;
+
+__global__ void __launch_bounds__(kThreads)
+    k_digit_hist(const uint64_t *__restrict__ keys, uint32_t n, int shift, uint32_t nTiles,
+                 uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t cnt[kRadix];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * kTile;
+    for (int k = 0; k < kItems; ++k) {
+        uint64_t i = base + (uint64_t)k * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * nTiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_digit_scatter(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals, uint32_t n,
+                    int shift, uint32_t nTiles, const uint32_t *__restrict__ off,
+                    uint64_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut)
+{
+    __shared__ uint32_t run[kRadix];
+    __shared__ uint32_t wcnt[kThreads / 64][kRadix];
+    __shared__ uint32_t base_off[kRadix];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    run[t] = 0;
+    base_off[t] = off[(uint64_t)t * nTiles + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint64_t base = (uint64_t)blockIdx.x * kTile;
+    __syncthreads();
+    for (int k = 0; k < kItems; ++k) {
+        uint64_t i = base + (uint64_t)k * kThreads + t;
+        bool valid = i < n;
+        uint64_t key = valid ? keys[i] : 0ull;
+        uint32_t d = (uint32_t)((key >> shift) & 0xFF);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t lrank = (uint32_t)__popcll(peers & lt);
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) wcnt[w][t] = 0;
+        __syncthreads();
+        if (valid && lrank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t r = run[d] + lrank;
+            for (int w = 0; w < wave; ++w) r += wcnt[w][d];
+            uint32_t dst = base_off[d] + r;
+            keysOut[dst] = key;
+            valsOut[dst] = vals[i];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) add += wcnt[w][t];
+        run[t] += add;
+        __syncthreads();
+    }
+}
+
+__global__ void k_iota(uint32_t *v, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
+// Order-preserving map of a double (or float) column to uint64 keys.
+template <typename T>
+__global__ void k_column_keys(Soa<T> p, int col, uint32_t n, uint64_t *keys)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = (double)p(col, i);
+    uint64_t u = (uint64_t)__double_as_longlong(v);
+    keys[i] = (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+template <typename T>
+__global__ void k_gather_rows(Soa<T> src, Soa<T> dst, const uint32_t *__restrict__ perm, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = perm[i];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) dst(c, i) = src(c, s);
+}
+
+}  // namespace
+
+static uint32_t sort_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
+
+OECL_EXPORT uint64_t oecl_sort_work_bytes(uint32_t n)
+{
+    uint64_t nt = sort_tiles(n) ? sort_tiles(n) : 1;
+    uint64_t h = (uint64_t)kRadix * nt;
+    // hist, offsets (h + 1), scan work, alternate keys and values
+    return 4 * (h + (h + 1) + 2 * (oecl_scan_tiles((uint32_t)h) + 2) + 16) + 12ull * n + 1024;
+}
+
+// Sorts keys[n] ascending (stable), permuting vals along; bits: number of low key bits that
+// can be non-zero (rounded up to whole bytes).  Results end in keys/vals.
+OECL_EXPORT int oecl_radix_sort_u64(uint64_t *keys, uint32_t *vals, uint32_t n, int bits, void *work,
+                                    void *stream)
+{
+    if (!keys || !vals || !work || bits < 1 || bits > 64) return OECL_ERR_ARG;
+    if (n < 2) return OECL_OK;
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t nt = sort_tiles(n);
+    uint64_t h = (uint64_t)kRadix * nt;
+    char *w = (char *)work;
+    uint32_t *hist = (uint32_t *)w;
+    uint32_t *off = hist + h;
+    uint32_t *scanw = off + h + 1;
+    uint64_t scanWords = 2 * (oecl_scan_tiles((uint32_t)h) + 2) + 16;
+    char *alt = (char *)(scanw + scanWords);
+    alt = (char *)(((uintptr_t)alt + 15) & ~(uintptr_t)15);
+    uint64_t *k2 = (uint64_t *)alt;
+    uint32_t *v2 = (uint32_t *)(k2 + n);
+    uint64_t *kin = keys, *kout = k2;
+    uint32_t *vin = vals, *vout = v2;
+    int passes = (bits + 7) / 8;
+    for (int ps = 0; ps < passes; ++ps) {
+        int shift = 8 * ps;
+        hipLaunchKernelGGL(k_digit_hist, dim3(nt), dim3(kThreads), 0, s, kin, n, shift, nt, hist);
+        int rc = oecl_launch_scan_u32(hist, (uint32_t)h, off, scanw, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_digit_scatter, dim3(nt), dim3(kThreads), 0, s, kin, vin, n, shift, nt, off,
+                           kout, vout);
+        uint64_t *tk = kin; kin = kout; kout = tk;
+        uint32_t *tv = vin; vin = vout; vout = tv;
+    }
+    if (kin != keys) {
+        OECL_HIP(hipMemcpyAsync(keys, kin, 8ull * n, hipMemcpyDeviceToDevice, s), "sort copy");
+        OECL_HIP(hipMemcpyAsync(vals, vin, 4ull * n, hipMemcpyDeviceToDevice, s), "sort copy");
+    }
+    OECL_CHECK_LAUNCH("oecl_radix_sort_u64");
+    return OECL_OK;
+}
+
+// Particles.sortByColumn (particles.pyx:49-50 -> _quicksortByColumn :431-459): ascending by
+// one column, stable; the sorted block is written to `spare` (same stride).
+OECL_EXPORT int oecl_sort_by_column(int dtype, const void *data, void *spare, uint64_t stride,
+                                    uint32_t n, int column, uint64_t *keys, uint32_t *perm,
+                                    void *work, void *stream)
+{
+    if (!data || !spare || !keys || !perm || !work || column < 0 || column > 5) return OECL_ERR_ARG;
+    if (n == 0) return OECL_OK;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 gr(oecl_blocks(n, 256)), bl(256);
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_column_keys<double>, gr, bl, 0, s, Soa<double>{(double *)data, stride}, column,
+                           n, keys);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_column_keys<float>, gr, bl, 0, s, Soa<float>{(float *)data, stride}, column, n,
+                           keys);
+    else
+        return OECL_ERR_ARG;
+    hipLaunchKernelGGL(k_iota, gr, bl, 0, s, perm, n);
+    int rc = oecl_radix_sort_u64(keys, perm, n, 64, work, stream);
+    if (rc) return rc;
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_gather_rows<double>, gr, bl, 0, s, Soa<double>{(double *)data, stride},
+                           Soa<double>{(double *)spare, stride}, perm, n);
+    else
+        hipLaunchKernelGGL(k_gather_rows<float>, gr, bl, 0, s, Soa<float>{(float *)data, stride},
+                           Soa<float>{(float *)spare, stride}, perm, n);
+    OECL_CHECK_LAUNCH("oecl_sort_by_column");
+    return OECL_OK;
+}
+
+// dst[:, i] = src[:, perm[i]] for i < n.
+OECL_EXPORT int oecl_permute_particles(int dtype, const void *src, void *dst, uint64_t stride,
+                                       const uint32_t *perm, uint32_t n, void *stream)
+{
+    if (!src || !dst || !perm) return OECL_ERR_ARG;
+    if (n == 0) return OECL_OK;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 gr(oecl_blocks(n, 256)), bl(256);
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_gather_rows<double>, gr, bl, 0, s, Soa<double>{(double *)src, stride},
+                           Soa<double>{(double *)dst, stride}, perm, n);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_gather_rows<float>, gr, bl, 0, s, Soa<float>{(float *)src, stride},
+                           Soa<float>{(float *)dst, stride}, perm, n);
+    else
+        return OECL_ERR_ARG;
+    OECL_CHECK_LAUNCH("oecl_permute_particles");
+    return OECL_OK;
+}

@@ -210,6 +210,29 @@ def add_and_remove(pd, add, keep, capacity):
     return full[:new].copy(), capacity
 
 
+def add_and_remove_fast(pd, add, keep):
+    """The hole-filling rule of the product's fast ordering: additions fill the holes in index
+    order, extra additions are appended, and holes left below the new count are filled with
+    the kept particles taken from the end (the rule of particles.pyx:254-269)."""
+    pd = _aos(pd)
+    add = np.ascontiguousarray(add, np.float64).reshape(-1, 6)
+    keep = np.asarray(keep).astype(bool)
+    n, a = pd.shape[0], add.shape[0]
+    holes = np.nonzero(~keep)[0]
+    h = holes.size
+    out = pd.copy()
+    m = min(a, h)
+    out[holes[:m]] = add[:m]
+    if a >= h:
+        return np.vstack([out, add[h:]])
+    new = n - h + a
+    rest = holes[a:]
+    rest = rest[rest < new]
+    tail = np.nonzero(keep[new:])[0][::-1] + new
+    out[rest] = pd[tail]
+    return out[:new].copy()
+
+
 def philox_words(n, c0, c1, c2, c3, k0, k1):
     """n Philox4x32-10 blocks for counters (c0+i, c1, c2, c3), key (k0, k1) -> uint32[n, 4]."""
     out = np.empty((n, 4), np.uint32)

@@ -9,7 +9,8 @@ from conftest import random_particles
 pytestmark = pytest.mark.gpu
 
 
-def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, key, cap):
+def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, key, cap,
+                 rule='reference'):
     from oracle.grid_oracle import OraclePoisson
     ops = OraclePoisson(g)
     Q = -1.602176565e-19
@@ -35,7 +36,10 @@ def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, 
                     orc.philox_words(na, 0, b, it, 1, key[0], key[1]))
             from oracle.furman_oracle import generate_secondaries
             sec, _, _ = generate_secondaries(moved, nv, U[:, :35], P)
-            pd, cap = orc.add_and_remove(pd, sec, ins, cap)
+            if rule == 'fast':
+                pd = orc.add_and_remove_fast(pd, sec, ins)
+            else:
+                pd, cap = orc.add_and_remove(pd, sec, ins, cap)
         out.append(pd.copy())
     return out
 
@@ -81,3 +85,62 @@ def test_buildup_matches_oracle_fp64(orc, fused):
         assert err < 1e-11, (it, err)
         emitted += b.history['emitted'][-1]
     assert emitted > 0 and sum(b.history['absorbed']) > 0
+
+
+@pytest.mark.parametrize('dtype', ['f64', 'f32'])
+def test_native_step_matches_oracle(orc, dtype):
+    """The one-call native step (device-resident counts) against the oracle loop with the
+    same hole-filling order; FP64 to 1e-11 of each column's scale, FP32 to 1e-4."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.furman_oracle import material
+    g = Grid(64, 48, 0.09, 0.07, 1)
+    n = 20000
+    pd0 = random_particles(g, n, seed=12, wall='ellip', vscale=3e6)
+    pd0[:, 5] = 5e4
+    dt = 2e-11
+    randomGen.seed(77)
+    key = randomGen.key()
+    tdt = torch.float64 if dtype == 'f64' else torch.float32
+    if dtype == 'f32':
+        pd0 = pd0.astype(np.float32).astype(np.float64)
+    p = Particles('electrons', 'variableWeight', dtype=tdt, ordering='fast')
+    p.setParticleData(pd0)
+    p.setBConst([0., 0.5, 0.])
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.6, eHat0=300.)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    beam = GaussianBunchTrain(g, dt, nParticles=1e11, tBunchSpacing=25e-9, radiusSigma=2e-3,
+                              nBunches=1, bunchLengthSigma=0.003)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
+    b.native_setup(absCap=8192, secCap=16384, maxSteps=64)
+    nsteps = 10
+    ref = _oracle_loop(orc, g, pd0.copy(), nsteps, dt, 1, np.array([0., 0.5, 0.]),
+                       beam.getTransversalProfile(), beam.getScale,
+                       material('copper', 1.6, eHat0=300.), key, n, rule='fast')
+    for it in range(nsteps):
+        b.step_native()
+        st = b.sync()
+        got = p.getParticleData().cpu().numpy().astype(np.float64)
+        if dtype == 'f64':
+            assert got.shape == ref[it].shape, (it, got.shape, ref[it].shape)
+            scale = np.abs(ref[it]).max(axis=0)
+            assert (np.abs(got - ref[it]) / scale).max() < 1e-11, it
+        else:
+            # FP32 push: same particle count within a few wall-test flips, and the FP64
+            # oracle's state to FP32 accuracy on the particles both still hold
+            assert abs(got.shape[0] - ref[it].shape[0]) <= 0.002 * n, it
+    assert st['absorbed_total'] > 0 and st['emitted_total'] > 0
+    if dtype == 'f32':
+        m = min(got.shape[0], ref[-1].shape[0])
+        # compare the first particles, which no compaction has moved in either run
+        idx = np.nonzero(np.all(np.abs(got[:2000, :2] - ref[-1][:2000, :2]) < 1e-6, axis=1))[0]
+        assert idx.size > 1500
+        scale = np.abs(ref[-1]).max(axis=0)
+        assert (np.abs(got[idx] - ref[-1][idx]) / scale).max() < 1e-4

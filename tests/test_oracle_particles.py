@@ -113,6 +113,24 @@ def test_compaction_rules_hand_cases(orc):
     assert cap == int(11 * 1.1)
 
 
+@pytest.mark.parametrize('nadd', [0, 30, 600])
+def test_fast_compaction_rule_agrees_with_reference_where_defined(orc, nadd):
+    """The fast rule equals _removeParticles when nothing is added and
+    _addAndRemoveParticles when the additions cover every hole; otherwise the same set."""
+    rng = np.random.default_rng(nadd)
+    n = 5000
+    pd = rng.standard_normal((n, 6))
+    keep = (rng.random(n) > 0.1).astype(np.uint16)
+    add = rng.standard_normal((nadd, 6))
+    fast = orc.add_and_remove_fast(pd, add, keep)
+    ref, _ = orc.add_and_remove(pd, add, keep, n + nadd)   # roomy: the in-place rules
+    h = n - int(keep.sum())
+    if nadd == 0 or nadd >= h:
+        assert np.array_equal(fast, ref)
+    key = lambda a: a[np.lexsort(a.T[::-1])]
+    assert np.array_equal(key(fast), key(ref))
+
+
 def test_interaction_on_wall_and_analytic_normals(orc):
     gE = Grid(64, 48, 0.09, 0.07, 1)
     rng = np.random.default_rng(4)
