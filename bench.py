@@ -33,6 +33,8 @@ def parse():
     ap.add_argument('--dtype', default='f64', choices=['f64', 'f32'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    ap.add_argument('--sort-every', type=int, default=20,
+                    help='re-order particles by cell every this many steps (0: never)')
     return ap.parse_args()
 
 
@@ -148,10 +150,12 @@ def main():
     import ctypes
     b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4),
                    maxSteps=a.warmup + a.steps + 8)
+    b.sort_particles()
     for k in range(a.warmup):
         b.step_native()
-        if k % 10 == 9:
+        if a.sort_every and k % a.sort_every == a.sort_every - 1:
             b.sync()
+            b.sort_particles()
     torch.cuda.synchronize()
     b.sync()
     # timed region: K full steps; the push kernel of each step is bracketed by HIP events
@@ -169,6 +173,9 @@ def main():
         b._args.evPushBegin = evs[k][0].value
         b._args.evPushEnd = evs[k][1].value
         b.step_native()
+        if a.sort_every and k % a.sort_every == a.sort_every - 1:
+            b.sync()              # host count for the sort launch, and the status check
+            b.sort_particles()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     b._args.evPushBegin = None
@@ -222,6 +229,7 @@ def main():
         'poisson_solves_per_s': poisson_rate,
         'poisson_relres': res,
         'poisson_capacitance_rows': ps.capacitanceRows,
+        'sort_every': a.sort_every,
         'roofline': {'bound': 'hbm', 'kernel': 'k_push_fused', 'achieved': achieved,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                      'traffic': None, 'avg_launch_us': avg_ms * 1e3,

@@ -205,6 +205,13 @@ int oecl_radix_sort_u64(uint64_t *keys, uint32_t *vals, uint32_t n, int bits, vo
  * (stable) is written to spare; keys[n], perm[n] scratch; work: oecl_sort_work_bytes(n). */
 int oecl_sort_by_column(int dtype, const void *data, void *spare, uint64_t stride, uint32_t n,
                         int column, uint64_t *keys, uint32_t *perm, void *work, void *stream);
+/* Orders the particles by grid cell (stable) into spare (same stride): consecutive particles
+ * then share cells, so the field gather stays in cache and the deposit sums in LDS.  No
+ * reference counterpart (the reference's loops are serial); keys[n], perm[n] scratch, work:
+ * oecl_sort_work_bytes(n). */
+int oecl_sort_by_cell(const oecl_grid_t *g, int dtype, const void *data, void *spare,
+                      uint64_t stride, uint32_t n, uint64_t *keys, uint32_t *perm, void *work,
+                      void *stream);
 /* dst[:, i] = src[:, perm[i]], i < n. */
 int oecl_permute_particles(int dtype, const void *src, void *dst, uint64_t stride,
                            const uint32_t *perm, uint32_t n, void *stream);

@@ -307,6 +307,28 @@ class Buildup:
         return {'absorbed_total': st[0], 'emitted_total': st[1], 'not_found': st[3],
                 'absorbed_last': st[4], 'emitted_last': st[5]}
 
+    def sort_particles(self):
+        """Orders the particles by grid cell (device radix sort + permutation).  Needs the host
+        count to be current (call sync() first after native steps)."""
+        p = self.p
+        n = p.macroParticleCount
+        if n < 2:
+            return
+        dev = L.device()
+        if getattr(self, '_sortkeys', None) is None or self._sortkeys.numel() < p.capacity:
+            self._sortkeys = torch.empty(p.capacity, dtype=torch.int64, device=dev)
+            self._sortperm = torch.empty(p.capacity, dtype=torch.int32, device=dev)
+            self._sortwork = torch.empty(int(L.lib().oecl_sort_work_bytes(p.capacity)) // 8 + 64,
+                                         dtype=torch.float64, device=dev)
+        spare = p._spare_block()
+        L.check(L.lib().oecl_sort_by_cell(
+            ctypes.byref(self.gd), L.i32(p.code), L.ptr(p._data), L.ptr(spare), L.u64(p.capacity),
+            L.u32(n), L.ptr(self._sortkeys), L.ptr(self._sortperm), L.ptr(self._sortwork),
+            L.stream()), 'sort by cell')
+        p._data, p._spare = spare, p._data
+        if getattr(self, '_args', None) is not None:
+            self._args.data = p._data.data_ptr()
+
     def counts(self, first, last):
         """Particle counts at the start of steps first .. last-1 (native steps)."""
         a = first - self._native_it0

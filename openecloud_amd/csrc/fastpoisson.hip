@@ -30,6 +30,8 @@ struct oecl_fastpoisson {
     double *Cinv;             // m*m, column-major
     double *b, *t1, *t2;      // nb each
     double *wy, *z;           // m each
+    double *part;             // [nChunks][m] partial products of the capacitance GEMV
+    uint32_t nChunks;
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
 };
 
@@ -115,6 +117,100 @@ __global__ void k_cap_columns(uint32_t m, uint32_t a0, uint32_t cnt, uint64_t nb
     C[(uint64_t)(a0 + c) * m + a] = s;
 }
 
+// ------------------------------------------------------------------ per-solve kernels
+// C = A * B (column-major, FP64) with 32 x 32 output tiles: 256 threads, 2 x 2 outputs per
+// thread, K staged through LDS 32 at a time.  The box is only ~510 x 382, so small tiles
+// are what fills the 256 CUs.  EPI: 0 store, 1 store times Linv (elementwise), 2 scatter
+// into the extended-grid potential (masked; zero elsewhere is written by the caller).
+constexpr int GT = 32;
+
+template <int EPI>
+__global__ void __launch_bounds__(256)
+    k_gemm_nn(int M, int N, int K, const double *__restrict__ A, int lda, const double *__restrict__ B, int ldb,
+              double *__restrict__ C, int ldc, const double *__restrict__ S, uint32_t nx,
+              const uint8_t *__restrict__ mask)
+{
+    __shared__ double As[GT][GT + 1];
+    __shared__ double Bs[GT][GT + 1];
+    const int tid = threadIdx.x;
+    const int tm = tid & 15, tn = tid >> 4;
+    const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
+    double c00 = 0., c01 = 0., c10 = 0., c11 = 0.;
+    for (int k0 = 0; k0 < K; k0 += GT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int idx = tid + r * 256;
+            int row = idx & (GT - 1), kk = idx >> 5;
+            int gm = m0 + row, gk = k0 + kk;
+            As[kk][row] = (gm < M && gk < K) ? A[gm + (size_t)gk * lda] : 0.;
+            int kb = idx & (GT - 1), col = idx >> 5;
+            int gkb = k0 + kb, gn = n0 + col;
+            Bs[kb][col] = (gkb < K && gn < N) ? B[gkb + (size_t)gn * ldb] : 0.;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < GT; ++kk) {
+            double a0 = As[kk][2 * tm], a1 = As[kk][2 * tm + 1];
+            double b0 = Bs[kk][2 * tn], b1 = Bs[kk][2 * tn + 1];
+            c00 = fma(a0, b0, c00);
+            c01 = fma(a0, b1, c01);
+            c10 = fma(a1, b0, c10);
+            c11 = fma(a1, b1, c11);
+        }
+        __syncthreads();
+    }
+    double cv[2][2] = {{c00, c01}, {c10, c11}};
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            int gm = m0 + 2 * tm + i, gn = n0 + 2 * tn + j;
+            if (gm >= M || gn >= N) continue;
+            size_t o = gm + (size_t)gn * ldc;
+            if (EPI == 0) {
+                C[o] = cv[i][j];
+            } else if (EPI == 1) {
+                C[o] = cv[i][j] * S[o];
+            } else {
+                uint32_t p = (uint32_t)(gm + 2) + (uint32_t)(gn + 2) * nx;
+                C[p] = mask[p] ? cv[i][j] : 0.;
+            }
+        }
+}
+
+// Partial products of y = Cinv w over column chunks of 32: part[chunk * m + a].
+__global__ void __launch_bounds__(256)
+    k_gemv_part(uint32_t m, const double *__restrict__ Cm, const double *__restrict__ w,
+                double *__restrict__ part)
+{
+    uint32_t a = blockIdx.x * 256 + threadIdx.x;
+    uint32_t b0 = blockIdx.y * 32;
+    if (a >= m) return;
+    double s = 0.;
+    uint32_t b1 = b0 + 32 < m ? b0 + 32 : m;
+    for (uint32_t b = b0; b < b1; ++b) s = fma(Cm[a + (size_t)b * m], w[b], s);
+    part[(size_t)blockIdx.y * m + a] = s;
+}
+
+// z[a] = sum of the partials; b[sIdx[a]] -= z[a].
+__global__ void k_gemv_finish_sub(uint32_t m, uint32_t nChunks, const double *__restrict__ part,
+                                  const int32_t *__restrict__ sIdx, double *__restrict__ b)
+{
+    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= m) return;
+    double s = 0.;
+    for (uint32_t c = 0; c < nChunks; ++c) s += part[(size_t)c * m + a];
+    b[sIdx[a]] -= s;
+}
+
+__global__ void k_zero_outside_box(uint32_t nx, uint32_t ny, double *__restrict__ phi)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nx * ny) return;
+    uint32_t i = p % nx, j = p / nx;
+    if (i < 2 || j < 2 || i >= nx - 2 || j >= ny - 2) phi[p] = 0.;
+}
+
 #define RB(call, where)                                         \
     do {                                                        \
         rocblas_status _s = (call);                             \
@@ -188,6 +284,9 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, const double *
     h->sIdx = (int32_t *)cur; cur += 4ull * m;
     h->wPtr = (int32_t *)cur; cur += 4ull * (m + 1);
     h->wCol = (int32_t *)cur; cur += 4ull * nnz;
+    h->nChunks = (m + 31) / 32;
+    h->part = nullptr;
+    if (m) OECL_HIP(hipMalloc(&h->part, 8ull * h->nChunks * m), "fastpoisson alloc");
     OECL_HIP(hipMemcpy(h->Sx, SxHost, 8ull * h->bx * h->bx, hipMemcpyHostToDevice), "copy Sx");
     OECL_HIP(hipMemcpy(h->Sy, SyHost, 8ull * h->by * h->by, hipMemcpyHostToDevice), "copy Sy");
     OECL_HIP(hipMemcpy(h->Linv, LinvHost, 8ull * h->nb, hipMemcpyHostToDevice), "copy L");
@@ -258,25 +357,38 @@ OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, dou
 {
     if (!h || !q || !phi) return OECL_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
-    RB(rocblas_set_stream(h->blas, s), "set stream");
+    const int bx = h->bx, by = h->by;
+    dim3 gg((bx + GT - 1) / GT, (by + GT - 1) / GT), gb(256);
     hipLaunchKernelGGL(k_gather_box, dim3(oecl_blocks(h->nb, 256)), dim3(256), 0, s, h->nx, h->bx, h->by,
                        h->mask, q, h->b);
+    hipLaunchKernelGGL(k_zero_outside_box, dim3(oecl_blocks(h->np, 256)), dim3(256), 0, s, h->nx, h->ny, phi);
     if (h->m) {
-        OECL_HIP(hipMemcpyAsync(h->t2, h->b, 8ull * h->nb, hipMemcpyDeviceToDevice, s), "copy b");
-        int rc = dst_solve(h, h->t2, h->t1, s);
-        if (rc) return rc;
+        // y = R^-1 b
+        hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, bx, h->Sx, bx, h->b, bx, h->t1, bx, nullptr, 0u,
+                           nullptr);
+        hipLaunchKernelGGL(k_gemm_nn<1>, gg, gb, 0, s, bx, by, by, h->t1, bx, h->Sy, by, h->t2, bx, h->Linv, 0u,
+                           nullptr);
+        hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, bx, h->Sx, bx, h->t2, bx, h->t1, bx, nullptr, 0u,
+                           nullptr);
+        hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, by, h->t1, bx, h->Sy, by, h->t2, bx, nullptr, 0u,
+                           nullptr);
+        // b -= U C^-1 W y
         hipLaunchKernelGGL(k_wmul, dim3(oecl_blocks(h->m, 256)), dim3(256), 0, s, h->m, h->wPtr, h->wCol,
                            h->wVal, h->t2, h->wy);
-        const double one = 1., zero = 0.;
-        RB(rocblas_dgemv(h->blas, rocblas_operation_none, h->m, h->m, &one, h->Cinv, h->m, h->wy, 1,
-                         &zero, h->z, 1), "dgemv");
-        hipLaunchKernelGGL(k_sub_scatter, dim3(oecl_blocks(h->m, 256)), dim3(256), 0, s, h->m, h->sIdx, h->z,
-                           h->b);
+        hipLaunchKernelGGL(k_gemv_part, dim3((h->m + 255) / 256, h->nChunks), dim3(256), 0, s, h->m, h->Cinv,
+                           h->wy, h->part);
+        hipLaunchKernelGGL(k_gemv_finish_sub, dim3(oecl_blocks(h->m, 256)), dim3(256), 0, s, h->m, h->nChunks,
+                           h->part, h->sIdx, h->b);
     }
-    int rc = dst_solve(h, h->b, h->t1, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_scatter_phi, dim3(oecl_blocks(h->np, 256)), dim3(256), 0, s, h->nx, h->ny, h->bx,
-                       h->by, h->mask, h->b, phi);
+    // phi = R^-1 b, scattered into the extended grid
+    hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, bx, h->Sx, bx, h->b, bx, h->t1, bx, nullptr, 0u,
+                       nullptr);
+    hipLaunchKernelGGL(k_gemm_nn<1>, gg, gb, 0, s, bx, by, by, h->t1, bx, h->Sy, by, h->b, bx, h->Linv, 0u,
+                       nullptr);
+    hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, bx, h->Sx, bx, h->b, bx, h->t1, bx, nullptr, 0u,
+                       nullptr);
+    hipLaunchKernelGGL(k_gemm_nn<2>, gg, gb, 0, s, bx, by, by, h->t1, bx, h->Sy, by, phi, 0, nullptr, h->nx,
+                       h->mask);
     OECL_CHECK_LAUNCH("oecl_fastpoisson_solve");
     return OECL_OK;
 }

@@ -140,24 +140,33 @@ __global__ void __launch_bounds__(256) k_push_fused(oecl_grid_t g, WallDev wall,
                                                     double *__restrict__ rhoNext)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const bool live = i < n;
     // FP32 storage is widened on load; the arithmetic runs in FP64 registers.
     using C = double;
     const C dxi = 1. / g.dx, dyi = 1. / g.dy, lxh = g.lxHalfExt, lyh = g.lyHalfExt;
-    C x = p(0, i), y = p(1, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
-    Weights<C> w = grid_weights<C>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
-    gather_push<C, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vx, vy, vz);
-    uint8_t in = wall_inside<C>(wall, x, y);  // may clip x, y (cut-cell wall)
-    p(0, i) = (T)x;
-    p(1, i) = (T)y;
-    p(2, i) = (T)vx;
-    p(3, i) = (T)vy;
-    if (TYPEB != 0) p(4, i) = (T)vz;
-    isInside[i] = in;
-    if (rhoNext != nullptr && in) {
-        Weights<C> w2 = grid_weights<C>((C)(T)x, (C)(T)y, dxi, dyi, lxh, lyh, g.nx, g.ny);
-        deposit_corners<C>(rhoNext, w2, g.nx, (double)p(5, i), q);
+    bool dep = false;
+    Weights<C> w2{};
+    double wq = 0.;
+    if (live) {
+        C x = p(0, i), y = p(1, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
+        Weights<C> w = grid_weights<C>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
+        gather_push<C, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vx, vy, vz);
+        uint8_t in = wall_inside<C>(wall, x, y);  // may clip x, y (cut-cell wall)
+        p(0, i) = (T)x;
+        p(1, i) = (T)y;
+        p(2, i) = (T)vx;
+        p(3, i) = (T)vy;
+        if (TYPEB != 0) p(4, i) = (T)vz;
+        isInside[i] = in;
+        if (rhoNext != nullptr && in) {
+            dep = true;
+            w2 = grid_weights<C>((C)(T)x, (C)(T)y, dxi, dyi, lxh, lyh, g.nx, g.ny);
+            wq = (double)p(5, i);
+        }
     }
+    if (rhoNext == nullptr) return;  // uniform across the block: no barrier is skipped
+    block_deposit<512>(rhoNext, g.nx, dep, w2.cell, w2.wLL * wq * q, w2.wUL * wq * q,
+                       w2.wLU * wq * q, w2.wUU * wq * q);
 }
 
 // Moments of the particle set, FP64, fixed reduction order (bit-reproducible):

@@ -45,6 +45,57 @@ __device__ __forceinline__ void deposit_corners(double *rho, const Weights<T> &w
     unsafeAtomicAdd(rho + w.cell + nx + 1, (double)w.wUU * wt * q);
 }
 
+// Block-cooperative deposit for 256-thread blocks; every thread of the block must call it
+// (valid = false when the thread has no particle).  The block's window is the node range
+// [base, base + W) of the lowest cell in the block and the same range one row up; corner
+// contributions inside it are summed in LDS (LDS FP64 atomics) and flushed with one global
+// atomic per touched node, the rest go to global atomics directly.  With particles sorted
+// by cell a block covers a few dozen cells, so global atomics drop by an order of magnitude.
+template <int W>
+__device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t nx, bool valid,
+                                              uint32_t cell, double q00, double q10, double q01,
+                                              double q11)
+{
+    __shared__ double lo[W], hi[W];
+    __shared__ uint32_t wmin[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t c = valid ? cell : 0xFFFFFFFFu;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        uint32_t o = __shfl_xor(c, d, 64);
+        c = o < c ? o : c;
+    }
+    if (lane == 0) wmin[wave] = c;
+    for (int k = tid; k < W; k += 256) {
+        lo[k] = 0.;
+        hi[k] = 0.;
+    }
+    __syncthreads();
+    uint32_t base = wmin[0];
+    for (int w = 1; w < 4; ++w) base = wmin[w] < base ? wmin[w] : base;
+    if (valid) {
+        uint32_t off = cell - base;
+        if (off + 1 < (uint32_t)W) {
+            atomicAdd(&lo[off], q00);
+            atomicAdd(&lo[off + 1], q10);
+            atomicAdd(&hi[off], q01);
+            atomicAdd(&hi[off + 1], q11);
+        } else {
+            unsafeAtomicAdd(rho + cell, q00);
+            unsafeAtomicAdd(rho + cell + 1, q10);
+            unsafeAtomicAdd(rho + cell + nx, q01);
+            unsafeAtomicAdd(rho + cell + nx + 1, q11);
+        }
+    }
+    __syncthreads();
+    if (base == 0xFFFFFFFFu) return;
+    for (int k = tid; k < W; k += 256) {
+        double a = lo[k], b = hi[k];
+        if (a != 0.) unsafeAtomicAdd(rho + base + k, a);
+        if (b != 0.) unsafeAtomicAdd(rho + base + nx + k, b);
+    }
+}
+
 // Boris rotation + second half kick of particles.pyx:272-321, from the velocity after the
 // first half kick.
 template <typename T>

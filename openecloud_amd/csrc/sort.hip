@@ -6,6 +6,7 @@
 // with eight ballots, their order within the wave from a popcount of the lower-lane mask,
 // and the order across waves and sub-tiles from LDS counters.
 #include "common.h"
+#include "particles_dev.h"
 #include "scan.h"
 
 namespace {
@@ -187,6 +188,42 @@ OECL_EXPORT int oecl_sort_by_column(int dtype, const void *data, void *spare, ui
                            Soa<float>{(float *)spare, stride}, perm, n);
     OECL_CHECK_LAUNCH("oecl_sort_by_column");
     return OECL_OK;
+}
+
+template <typename T>
+__global__ void k_cell_keys(oecl_grid_t g, Soa<T> p, uint32_t n, uint64_t *keys, uint32_t *vals)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Weights<double> w = grid_weights<double>((double)p(0, i), (double)p(1, i), 1. / g.dx, 1. / g.dy,
+                                             g.lxHalfExt, g.lyHalfExt, g.nx, g.ny);
+    keys[i] = w.cell;
+    vals[i] = i;
+}
+
+// Orders the particles by grid cell (stable) into `spare`: consecutive particles then share
+// cells, which keeps the field gather in cache and lets the deposit sum in LDS.
+OECL_EXPORT int oecl_sort_by_cell(const oecl_grid_t *g, int dtype, const void *data, void *spare,
+                                  uint64_t stride, uint32_t n, uint64_t *keys, uint32_t *perm, void *work,
+                                  void *stream)
+{
+    if (!g || !data || !spare || !keys || !perm || !work) return OECL_ERR_ARG;
+    if (n == 0) return OECL_OK;
+    hipStream_t s = (hipStream_t)stream;
+    dim3 gr(oecl_blocks(n, 256)), bl(256);
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_cell_keys<double>, gr, bl, 0, s, *g, Soa<double>{(double *)data, stride}, n, keys,
+                           perm);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_cell_keys<float>, gr, bl, 0, s, *g, Soa<float>{(float *)data, stride}, n, keys,
+                           perm);
+    else
+        return OECL_ERR_ARG;
+    int bits = 1;
+    while ((1ull << bits) < g->np) ++bits;
+    int rc = oecl_radix_sort_u64(keys, perm, n, bits, work, stream);
+    if (rc) return rc;
+    return oecl_permute_particles(dtype, data, spare, stride, perm, n, stream);
 }
 
 // dst[:, i] = src[:, perm[i]] for i < n.

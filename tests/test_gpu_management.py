@@ -105,3 +105,19 @@ def test_local_random_split_only_increases_count():
     got = p.getParticleData().cpu().numpy()
     # the split halves sit at the parent position (chamber half width < 1 m, see docstring)
     assert np.sum(got[:, 5] == 256.) == 2 * info['toSplit']
+
+
+def test_sort_by_cell_orders_cells_and_keeps_set():
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.particleEmitter import FurmanEmitter
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g, p, pb, pd = _parts(100000, 11)
+    b = Buildup(g, p, pb, FurmanEmitter(pb, p), PoissonSolver(g), None)
+    b.sort_particles()
+    got = p.getParticleData().cpu().numpy()
+    cx = ((got[:, 0] + g.getLxExt() / 2) / g.getDx()).astype(np.int64)
+    cy = ((got[:, 1] + g.getLyExt() / 2) / g.getDy()).astype(np.int64)
+    cell = cx + cy * g.getNxExt()
+    assert np.all(np.diff(cell) >= 0)
+    key = lambda a: a[np.lexsort(a.T[::-1])]
+    assert np.array_equal(key(got), key(pd))
