@@ -279,17 +279,18 @@ int oecl_axpy_rhs(uint32_t np, const double *rho, double scale, const double *pr
 
 /* Direct solve of the same system (poissonSolver.pyx:59,85 -- the SuperLU factorisation
  * and back-substitution) by the capacitance-matrix method: the box of possible inside nodes
- * (bx = nxExt-4 by by = nyExt-4) is inverted exactly by type-I DSTs done as FP64 GEMMs, and
- * the m inside rows next to the wall are corrected through the m x m capacitance matrix,
- * built and inverted on the device at creation.  Sx [bx*bx], Sy [by*by]: orthonormal DST-I
- * matrices; Linv [by][bx]: reciprocal box eigenvalues; sIdx [m]: box index of each corrected
- * row; W (CSR over box columns): those rows of (M - R).  maskDev (inside flags, device) must
- * outlive the handle.  *capMax receives max |C_aa| (a scale check). */
+ * (bx = nxExt-4 by by = nyExt-4) with the uniform edge coefficients cxu (x) and cyu (y) is
+ * inverted exactly by a type-I DST along x (FP64 MFMA products) and tridiagonal solves along
+ * y, and the m inside rows next to the wall are corrected through the m x m capacitance
+ * matrix, built and inverted on the device at creation.  sIdx [m]: box index
+ * (i-2) + (j-2)*bx of each corrected row; W (CSR over box indices): those rows of (M - R).
+ * maskDev (inside flags, device) must outlive the handle.  *capMax receives max |C_aa|
+ * (a scale check). */
 typedef struct oecl_fastpoisson oecl_fastpoisson;
-int oecl_fastpoisson_create(uint32_t nxExt, uint32_t nyExt, const double *Sx, const double *Sy,
-                            const double *Linv, uint32_t m, const int32_t *sIdx,
-                            const int32_t *wPtr, const int32_t *wCol, const double *wVal,
-                            const uint8_t *maskDev, oecl_fastpoisson **out, double *capMax);
+int oecl_fastpoisson_create(uint32_t nxExt, uint32_t nyExt, double cxu, double cyu, uint32_t m,
+                            const int32_t *sIdx, const int32_t *wPtr, const int32_t *wCol,
+                            const double *wVal, const uint8_t *maskDev, oecl_fastpoisson **out,
+                            double *capMax);
 /* phi (npExt, zero on non-inside nodes) = A^-1 q on inside nodes. */
 int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, double *phi, void *stream);
 int oecl_fastpoisson_destroy(oecl_fastpoisson *h);

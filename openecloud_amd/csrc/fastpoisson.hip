@@ -1,18 +1,34 @@
 // Direct FIT Poisson solve by the capacitance-matrix method.
 //
 // The operator A (poissonSolver.pyx:44-59) acts on the inside nodes, all of which lie in
-// the box B of nodes 2 <= i <= nxExt-3, 2 <= j <= nyExt-3.  Let R be the box's uniform
-// 5-point operator (x-edge coefficient cxu, y-edge cyu, zero Dirichlet data outside B) and M
-// the box operator whose rows are A's rows on inside nodes and R's rows elsewhere.  M x = b
-// with b = q on inside nodes reproduces A phi = q exactly on inside nodes (the inside rows
-// never reference a non-inside node).  M and R differ only in the m rows S of inside nodes
-// next to the wall: M = R + U W with U the unit columns of S.  By the Woodbury identity
+// the box B of nodes 2 <= i <= nxExt-3, 2 <= j <= nyExt-3 (bx x by nodes, column-major with
+// x fastest).  Let R be the box's uniform 5-point operator (x-edge coefficient cxu, y-edge
+// cyu, zero Dirichlet data outside B) and M the box operator whose rows are A's rows on
+// inside nodes and R's rows elsewhere.  M x = b with b = q on inside nodes reproduces
+// A phi = q exactly on inside nodes (the inside rows never reference a non-inside node).  M
+// and R differ only in the m rows S of inside nodes next to the wall: M = R + U W with U the
+// unit columns of S.  By the Woodbury identity
 //     x = R^-1 (b - U z),   z = C^-1 W R^-1 b,   C = I_m + W R^-1 U.
-// R^-1 is applied exactly by the type-I discrete sine transform, written as four FP64
-// GEMMs with the orthonormal DST matrices (MFMA through rocBLAS):
-//     R^-1 b = Sx ((Sx b Sy) / L) Sy     (column-major box, x fastest),
-// L[q, p] = cxu * lx_q + cyu * ly_p with the 1-D eigenvalues l_k = 2 - 2 cos(pi k / (n + 1)).
-// C is built once on the device by m batched DST solves and inverted with rocSOLVER.
+//
+// R = cxu Tx (x) I + cyu I (x) Ty with T = tridiag(-1, 2, -1).  The type-I DST Sx
+// (orthonormal, symmetric, Sx^2 = I) diagonalises Tx, so in the x-transformed space
+// (hat b = Sx b) R splits into bx independent tridiagonal systems along y, one per x mode q:
+//     (cxu lx_q I + cyu Ty) hat u[q, :] = hat b[q, :],   lx_q = 2 - 2 cos(pi (q+1)/(bx+1)),
+// solved exactly by Thomas elimination with precomputed pivots.  The whole solve stays in
+// the transformed space:
+//     hat b = Sx b                                        (FP64 MFMA product)
+//     hat y = T^-1 hat b                                  (Thomas, per x mode)
+//     (W y)_a = sum_k W[a,k] (Sx hat y)[k]               (k near the wall: short dot products)
+//     z = C^-1 W y ;  hat b -= Sx[:, i_a] z_a  for each a (column j_a of hat b)
+//     hat x = T^-1 hat b ;  x = Sx hat x                  (product fused with the scatter)
+// Two matrix products per solve.  C is built once on the device with the same kernels
+// (batched unit right-hand sides), so that the correction matches the operator applied,
+// and inverted with rocSOLVER.
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
@@ -22,193 +38,322 @@ struct oecl_fastpoisson {
     uint32_t nx, ny, np;      // extended grid
     uint32_t bx, by, nb;      // box sizes, nb = bx * by
     uint32_t m;               // irregular rows
+    uint32_t nJ;              // distinct box columns j among the irregular nodes
+    double cyu;
     rocblas_handle blas;
-    double *Sx, *Sy, *Linv;   // bx*bx, by*by, nb
+    double *Sx;               // bx * bx (symmetric)
+    double *Fx;               // [2][kx][kx] even/odd-row halves of Sx, kx = ceil(bx / 2)
+    double *invw;             // nb: Thomas pivots 1 / w[q, j] of mode q's y system
     int32_t *sIdx;            // [m] box index of row a
     int32_t *wPtr, *wCol;     // CSR of W over box columns, m rows
     double *wVal;
-    double *Cinv;             // m*m, column-major
+    int32_t *jCol, *jStart, *jA;  // distinct columns, their ranges in jA, rows a by column
+    double *CinvT;            // m*m, C^-1 row-major (row a contiguous)
     double *b, *t1, *t2;      // nb each
     double *wy, *z;           // m each
-    double *part;             // [nChunks][m] partial products of the capacitance GEMV
-    uint32_t nChunks;
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
 };
 
 namespace {
 
-__global__ void k_gather_box(uint32_t nx, uint32_t bx, uint32_t by, const uint8_t *__restrict__ mask,
-                             const double *__restrict__ q, double *__restrict__ b)
-{
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= bx * by) return;
-    uint32_t i = t % bx + 2, j = t / bx + 2;
-    uint32_t p = i + j * nx;
-    b[t] = mask[p] ? q[p] : 0.;
-}
-
-__global__ void k_scatter_phi(uint32_t nx, uint32_t ny, uint32_t bx, uint32_t by,
-                              const uint8_t *__restrict__ mask, const double *__restrict__ x,
-                              double *__restrict__ phi)
-{
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nx * ny) return;
-    uint32_t i = p % nx, j = p / nx;
-    double v = 0.;
-    if (mask[p] && i >= 2 && j >= 2 && i - 2 < bx && j - 2 < by) v = x[(i - 2) + (j - 2) * bx];
-    phi[p] = v;
-}
-
-__global__ void k_scale(uint32_t n, const double *__restrict__ a, double *__restrict__ x)
-{
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) x[t] *= a[t];
-}
-
-// wy[a] = sum_k W[a, k] y[k]
-__global__ void k_wmul(uint32_t m, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-                       const double *__restrict__ val, const double *__restrict__ y, double *__restrict__ wy)
-{
-    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= m) return;
-    double s = 0.;
-    for (int32_t k = ptr[a]; k < ptr[a + 1]; ++k) s += val[k] * y[col[k]];
-    wy[a] = s;
-}
-
-__global__ void k_sub_scatter(uint32_t m, const int32_t *__restrict__ sIdx, const double *__restrict__ z,
-                              double *__restrict__ b)
-{
-    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < m) b[sIdx[a]] -= z[a];
-}
-
-// Batch of unit right-hand sides after the first DST pass: column b of the batch holds
-// (Sx e_s Sy) / L = outer(Sx[:, i_s], Sy[j_s, :]) / L.
-__global__ void k_unit_spectra(uint32_t bx, uint32_t by, uint32_t a0, uint32_t cnt,
-                               const int32_t *__restrict__ sIdx, const double *__restrict__ Sx,
-                               const double *__restrict__ Sy, const double *__restrict__ Linv,
-                               double *__restrict__ out)
-{
-    uint64_t nb = (uint64_t)bx * by;
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nb * cnt) return;
-    uint32_t c = (uint32_t)(t / nb);
-    uint32_t e = (uint32_t)(t % nb);
-    uint32_t q = e % bx, p = e / bx;
-    int32_t s = sIdx[a0 + c];
-    uint32_t is = s % bx, js = s / bx;
-    out[t] = Sx[q + (uint64_t)is * bx] * Sy[js + (uint64_t)p * by] * Linv[e];
-}
-
-// C[a, a0 + c] = delta + sum_k W[a, k] G_c[k]   (column-major m x m)
-__global__ void k_cap_columns(uint32_t m, uint32_t a0, uint32_t cnt, uint64_t nb,
-                              const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-                              const double *__restrict__ val, const double *__restrict__ G,
-                              double *__restrict__ C)
-{
-    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)m * cnt) return;
-    uint32_t a = (uint32_t)(t % m), c = (uint32_t)(t / m);
-    const double *g = G + (uint64_t)c * nb;
-    double s = 0.;
-    for (int32_t k = ptr[a]; k < ptr[a + 1]; ++k) s += val[k] * g[col[k]];
-    if (a == a0 + c) s += 1.;
-    C[(uint64_t)(a0 + c) * m + a] = s;
-}
-
 // ------------------------------------------------------------------ per-solve kernels
-// C = A * B (column-major, FP64) with 32 x 32 output tiles: 256 threads, 2 x 2 outputs per
-// thread, K staged through LDS 32 at a time.  The box is only ~510 x 382, so small tiles
-// are what fills the 256 CUs.  EPI: 0 store, 1 store times Linv (elementwise), 2 scatter
-// into the extended-grid potential (masked; zero elsewhere is written by the caller).
-constexpr int GT = 32;
+// The DST-I matrix S (order n, symmetric) satisfies S[i][n-1-j] = (-1)^i S[i][j], so
+//     (S X)[2i+z, :] = sum_{j < K_z} F_z[i][j] P_z[j, :],   F_z[i][j] = S[2i+z][j],
+//     P_z[j, :] = X[j, :] + (-1)^z X[n-1-j, :]   (the middle row of odd n enters P_0 once),
+// with K_0 = ceil(n/2), K_1 = floor(n/2): one transform is two half-size products, half
+// the flops of S X.  k_dst_gemm computes parity z = blockIdx.z on FP64 MFMA
+// (v_mfma_f64_16x16x4f64): 32 x 32 output tiles, GNW waves splitting each GKC-deep chunk of
+// the contraction, 2 x 2 MFMA tiles per wave, a cross-wave sum through LDS at the end.  The
+// fold P_z is formed while X is staged into LDS; the next chunk's operands are loaded into
+// registers while the current one is multiplied.
+//   A = F_z (col-major K_z x K_z, lda = K_0, F_z at z * K_0 * K_0), X = n x N (ldx)
+// EPI 0 stores C (ldc), EPI 2 scatters into the extended-grid potential (masked; the rest
+// of the grid is written by the caller).
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int GT = 32, GNW = 8, GKC = 64, GNT = 64 * GNW, GPER = GT * GKC / GNT;
 
 template <int EPI>
-__global__ void __launch_bounds__(256)
-    k_gemm_nn(int M, int N, int K, const double *__restrict__ A, int lda, const double *__restrict__ B, int ldb,
-              double *__restrict__ C, int ldc, const double *__restrict__ S, uint32_t nx,
-              const uint8_t *__restrict__ mask)
+__global__ void __launch_bounds__(GNT)
+    k_dst_gemm(int N, int n, const double *__restrict__ A, const double *__restrict__ X, int ldx,
+               double *__restrict__ C, int ldc, uint32_t nx, const uint8_t *__restrict__ mask)
 {
-    __shared__ double As[GT][GT + 1];
-    __shared__ double Bs[GT][GT + 1];
-    const int tid = threadIdx.x;
-    const int tm = tid & 15, tn = tid >> 4;
+    constexpr int LDSN = (2 * GKC * 33 > GNW * GT * 33) ? 2 * GKC * 33 : GNW * GT * 33;
+    __shared__ double sm[LDSN];
+    double(*As)[33] = reinterpret_cast<double(*)[33]>(sm);
+    double(*Bs)[33] = reinterpret_cast<double(*)[33]>(sm + GKC * 33);
+    const int z = blockIdx.z;
+    const int K0 = (n + 1) / 2, Kz = z ? n / 2 : K0;
     const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
-    double c00 = 0., c01 = 0., c10 = 0., c11 = 0.;
-    for (int k0 = 0; k0 < K; k0 += GT) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            int idx = tid + r * 256;
-            int row = idx & (GT - 1), kk = idx >> 5;
-            int gm = m0 + row, gk = k0 + kk;
-            As[kk][row] = (gm < M && gk < K) ? A[gm + (size_t)gk * lda] : 0.;
-            int kb = idx & (GT - 1), col = idx >> 5;
-            int gkb = k0 + kb, gn = n0 + col;
-            Bs[kb][col] = (gkb < K && gn < N) ? B[gkb + (size_t)gn * ldb] : 0.;
-        }
-        __syncthreads();
-#pragma unroll 8
-        for (int kk = 0; kk < GT; ++kk) {
-            double a0 = As[kk][2 * tm], a1 = As[kk][2 * tm + 1];
-            double b0 = Bs[kk][2 * tn], b1 = Bs[kk][2 * tn + 1];
-            c00 = fma(a0, b0, c00);
-            c01 = fma(a0, b1, c01);
-            c10 = fma(a1, b0, c10);
-            c11 = fma(a1, b1, c11);
-        }
-        __syncthreads();
-    }
-    double cv[2][2] = {{c00, c01}, {c10, c11}};
+    if (m0 >= Kz || n0 >= N) return;  // whole block: no barrier is skipped by part of it
+    const double *Az = A + (size_t)z * K0 * K0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    v4d acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            int gm = m0 + 2 * tm + i, gn = n0 + 2 * tn + j;
-            if (gm >= M || gn >= N) continue;
-            size_t o = gm + (size_t)gn * ldc;
-            if (EPI == 0) {
-                C[o] = cv[i][j];
-            } else if (EPI == 1) {
-                C[o] = cv[i][j] * S[o];
-            } else {
-                uint32_t p = (uint32_t)(gm + 2) + (uint32_t)(gn + 2) * nx;
-                C[p] = mask[p] ? cv[i][j] : 0.;
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0., 0., 0., 0.};
+    double ra[GPER], rb[GPER];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < GPER; ++r) {
+            const int idx = tid + r * GNT;
+            const int row = idx & (GT - 1), kk = idx >> 5;
+            const int gm = m0 + row, gk = k0 + kk;
+            ra[r] = (gm < Kz && gk < Kz) ? Az[gm + (size_t)gk * K0] : 0.;
+            const int kb = idx % GKC, col = idx / GKC;
+            const int gkb = k0 + kb, gn = n0 + col;
+            double v = 0.;
+            if (gkb < Kz && gn < N) {
+                const double *xc = X + (size_t)gn * ldx;
+                const int h = n - 1 - gkb;
+                v = xc[gkb];
+                if (h != gkb) v = z ? v - xc[h] : v + xc[h];
             }
+            rb[r] = v;
         }
+    };
+    load(0);
+    constexpr int KW = GKC / GNW;
+    for (int k0 = 0; k0 < Kz; k0 += GKC) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < GPER; ++r) {
+            const int idx = tid + r * GNT;
+            As[idx >> 5][idx & (GT - 1)] = ra[r];
+            Bs[idx % GKC][idx / GKC] = rb[r];
+        }
+        __syncthreads();
+        if (k0 + GKC < Kz) load(k0 + GKC);
+#pragma unroll
+        for (int kk = wave * KW; kk < wave * KW + KW; kk += 4) {
+            const int k = kk + (lane >> 4), c = lane & 15;
+            const double a0 = As[k][c], a1 = As[k][16 + c];
+            const double b0 = Bs[k][c], b1 = Bs[k][16 + c];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+    // MFMA 16x16 result: register r of lane l holds row (l >> 4) + 4 r, column l & 15.
+    double(*Red)[GT][33] = reinterpret_cast<double(*)[GT][33]>(sm);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                Red[wave][16 * i + (lane >> 4) + 4 * r][16 * j + (lane & 15)] = acc[i][j][r];
+    __syncthreads();
+    for (int e = tid; e < GT * GT; e += GNT) {
+        const int mm = e & (GT - 1), nn = e >> 5;
+        const int gm = m0 + mm, gn = n0 + nn;
+        if (gm >= Kz || gn >= N) continue;
+        double s = Red[0][mm][nn];
+#pragma unroll
+        for (int w = 1; w < GNW; ++w) s += Red[w][mm][nn];
+        const int om = 2 * gm + z;
+        if (EPI == 0) {
+            C[om + (size_t)gn * ldc] = s;
+        } else {
+            const uint32_t p = (uint32_t)(om + 2) + (uint32_t)(gn + 2) * nx;
+            C[p] = mask[p] ? s : 0.;
+        }
+    }
 }
 
-// Partial products of y = Cinv w over column chunks of 32: part[chunk * m + a].
-__global__ void __launch_bounds__(256)
-    k_gemv_part(uint32_t m, const double *__restrict__ Cm, const double *__restrict__ w,
-                double *__restrict__ part)
-{
-    uint32_t a = blockIdx.x * 256 + threadIdx.x;
-    uint32_t b0 = blockIdx.y * 32;
-    if (a >= m) return;
-    double s = 0.;
-    uint32_t b1 = b0 + 32 < m ? b0 + 32 : m;
-    for (uint32_t b = b0; b < b1; ++b) s = fma(Cm[a + (size_t)b * m], w[b], s);
-    part[(size_t)blockIdx.y * m + a] = s;
-}
-
-// z[a] = sum of the partials; b[sIdx[a]] -= z[a].
-__global__ void k_gemv_finish_sub(uint32_t m, uint32_t nChunks, const double *__restrict__ part,
-                                  const int32_t *__restrict__ sIdx, double *__restrict__ b)
-{
-    uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= m) return;
-    double s = 0.;
-    for (uint32_t c = 0; c < nChunks; ++c) s += part[(size_t)c * m + a];
-    b[sIdx[a]] -= s;
-}
-
-__global__ void k_zero_outside_box(uint32_t nx, uint32_t ny, double *__restrict__ phi)
+// b = masked charge on the box; the potential outside the box is zero.
+__global__ void k_gather_box_zero(uint32_t nx, uint32_t ny, uint32_t bx, const uint8_t *__restrict__ mask,
+                                  const double *__restrict__ q, double *__restrict__ b, double *__restrict__ phi)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nx * ny) return;
     uint32_t i = p % nx, j = p / nx;
     if (i < 2 || j < 2 || i >= nx - 2 || j >= ny - 2) phi[p] = 0.;
+    else b[(i - 2) + (j - 2) * bx] = mask[p] ? q[p] : 0.;
+}
+
+// Thomas elimination along y for every x mode q, one wave per q:
+//     g_j = (r_j + cyu g_{j-1}) / w_j   (g_{-1} = 0),
+//     u_j = g_j + (cyu / w_j) u_{j+1}   (u_{by} = 0),   w_0 = d_q, w_j = d_q - cyu^2 / w_{j-1},
+// with the pivots 1 / w precomputed.  Both sweeps are affine recurrences x_j = a_j x_{j-1} + b_j,
+// evaluated as a prefix scan of affine maps: lane L holds the C consecutive rows
+// [C L, C L + C); it composes its rows' maps, a shuffle scan over the 64 lanes gives every
+// lane its incoming value, and the lane then runs its rows.  The order of operations is
+// fixed, so results are reproducible.  UNIT: the right-hand side is Sx e_s for the irregular
+// node s = sIdx[a0 + blockIdx.y] (row j_s = Sx[:, i_s], zero elsewhere), for building the
+// capacitance matrix with exactly the operator the solve applies; batch element blockIdx.y
+// is written at u + blockIdx.y * nb.
+template <int C, bool UNIT>
+__global__ void __launch_bounds__(256)
+    k_thomas_y(uint32_t bx, uint32_t by, double cyu, const double *__restrict__ invw,
+               const double *__restrict__ r, double *__restrict__ u, const int32_t *__restrict__ sIdx,
+               uint32_t a0, const double *__restrict__ Sx)
+{
+    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (q >= bx) return;
+    const size_t nb = (size_t)bx * by;
+    double *ub = u + (UNIT ? (size_t)blockIdx.y * nb : 0);
+    uint32_t js = 0;
+    double rs = 0.;
+    if (UNIT) {
+        const uint32_t s = (uint32_t)sIdx[a0 + blockIdx.y];
+        js = s / bx;
+        rs = Sx[q + (size_t)(s % bx) * bx];
+    }
+    const uint32_t j0 = lane * C;
+    double iw[C], g[C];
+#pragma unroll
+    for (int t = 0; t < C; ++t) {
+        const uint32_t j = j0 + t;
+        const size_t o = q + (size_t)j * bx;
+        iw[t] = j < by ? invw[o] : 0.;
+        double rv = 0.;
+        if (j < by) rv = UNIT ? (j == js ? rs : 0.) : r[o];
+        g[t] = rv;
+    }
+    // forward: g_j = a_j g_{j-1} + b_j with a_j = cyu iw_j, b_j = r_j iw_j
+    double A = 1., B = 0.;
+#pragma unroll
+    for (int t = 0; t < C; ++t) {
+        const double a = cyu * iw[t], b = g[t] * iw[t];
+        A *= a;
+        B = fma(a, B, b);
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double Ao = __shfl_up(A, d, 64), Bo = __shfl_up(B, d, 64);
+        if (lane >= (uint32_t)d) {
+            B = fma(A, Bo, B);
+            A *= Ao;
+        }
+    }
+    double x = __shfl_up(B, 1, 64);
+    if (lane == 0) x = 0.;
+#pragma unroll
+    for (int t = 0; t < C; ++t) {
+        x = fma(cyu * iw[t], x, g[t] * iw[t]);
+        g[t] = x;
+    }
+    // backward: u_j = e_j u_{j+1} + g_j with e_j = cyu iw_j (rows past by: e = g = 0)
+    A = 1.;
+    B = 0.;
+#pragma unroll
+    for (int t = C - 1; t >= 0; --t) {
+        const double e = cyu * iw[t];
+        B = fma(e, B, g[t]);
+        A *= e;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double Ao = __shfl_down(A, d, 64), Bo = __shfl_down(B, d, 64);
+        if (lane + d < 64) {
+            B = fma(A, Bo, B);
+            A *= Ao;
+        }
+    }
+    x = __shfl_down(B, 1, 64);
+    if (lane == 63) x = 0.;
+#pragma unroll
+    for (int t = C - 1; t >= 0; --t) {
+        x = fma(cyu * iw[t], x, g[t]);
+        const uint32_t j = j0 + t;
+        if (j < by) ub[q + (size_t)j * bx] = x;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double s)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    return s;
+}
+
+// wy[a] = sum_k W[a, k] y[k] with y = Sx hat y evaluated only where W needs it:
+// y[i + j bx] = sum_q Sx[q, i] hat y[q, j] (Sx symmetric: column i).  One wave per row a;
+// a row has at most WMAX entries (the node and its four neighbours), all of whose dot
+// products are accumulated together so that every load of the row is in flight at once.
+// CAP: batch element c = blockIdx.y holds R^-1 of the unit vector of row a0 + c, and the
+// result is column a0 + c of C = I + W R^-1 U, stored row-major (C^T column-major).
+constexpr int WMAX = 5;
+
+template <int CQ, bool CAP>
+__global__ void __launch_bounds__(256)
+    k_cap_wy(uint32_t m, uint32_t bx, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+             const double *__restrict__ val, const double *__restrict__ Sx, const double *__restrict__ yh,
+             double *__restrict__ wy, size_t nb, uint32_t a0)
+{
+    const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (a >= m) return;
+    const double *yb = yh + (CAP ? (size_t)blockIdx.y * nb : 0);
+    const int32_t k0 = ptr[a], nk = ptr[a + 1] - k0;
+    double acc[WMAX];
+#pragma unroll
+    for (int e = 0; e < WMAX; ++e) {
+        acc[e] = 0.;
+        if (e < nk) {
+            const uint32_t t = (uint32_t)col[k0 + e], i = t % bx, j = t / bx;
+            const double *sc = Sx + (size_t)i * bx, *yc = yb + (size_t)j * bx;
+            double d = 0.;
+#pragma unroll
+            for (int c = 0; c < CQ; ++c) {
+                const uint32_t q = lane + 64 * c;
+                if (q < bx) d = fma(sc[q], yc[q], d);
+            }
+            acc[e] = d * val[k0 + e];
+        }
+    }
+    double s = 0.;
+#pragma unroll
+    for (int e = 0; e < WMAX; ++e) s += acc[e];
+    s = wave_sum(s);
+    if (lane == 0) {
+        if (CAP) {
+            const uint32_t c = a0 + blockIdx.y;
+            wy[(size_t)a * m + c] = s + (a == c ? 1. : 0.);
+        } else {
+            wy[a] = s;
+        }
+    }
+}
+
+// z = C^-1 wy, C^-1 row-major: one wave per row a, 8 independent partial sums per lane.
+__global__ void __launch_bounds__(256)
+    k_cap_z(uint32_t m, const double *__restrict__ CinvT, const double *__restrict__ wy, double *__restrict__ z)
+{
+    const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (a >= m) return;
+    const double *row = CinvT + (size_t)a * m;
+    double p[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    uint32_t k = lane;
+    for (; k + 7 * 64 < m; k += 8 * 64) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = fma(row[k + 64 * u], wy[k + 64 * u], p[u]);
+    }
+    for (; k < m; k += 64) p[0] = fma(row[k], wy[k], p[0]);
+    double s = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
+    s = wave_sum(s);
+    if (lane == 0) z[a] = s;
+}
+
+// hat b[q, j] -= sum over rows a with j_a = j of z_a Sx[q, i_a]: one thread per (q, column
+// of the distinct-column list), rows taken in order (deterministic).
+__global__ void __launch_bounds__(256)
+    k_cap_update(uint32_t nJ, uint32_t bx, const int32_t *__restrict__ jCol, const int32_t *__restrict__ jStart,
+                 const int32_t *__restrict__ jA, const int32_t *__restrict__ sIdx, const double *__restrict__ z,
+                 const double *__restrict__ Sx, double *__restrict__ bh)
+{
+    const uint32_t q = blockIdx.x * 64 + (threadIdx.x & 63);
+    const uint32_t c = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (q >= bx || c >= nJ) return;
+    double s = 0.;
+    const int32_t e1 = jStart[c + 1];
+    for (int32_t e = jStart[c]; e < e1; ++e) {
+        const uint32_t a = (uint32_t)jA[e];
+        s = fma(z[a], Sx[q + (size_t)((uint32_t)sIdx[a] % bx) * bx], s);
+    }
+    bh[q + (size_t)jCol[c] * bx] -= s;
 }
 
 #define RB(call, where)                                         \
@@ -222,131 +367,218 @@ __global__ void k_zero_outside_box(uint32_t nx, uint32_t ny, double *__restrict_
         }                                                       \
     } while (0)
 
-// x (nb) <- R^-1 x via DST: t1 = Sx x ; x = t1 Sy ; x /= L ; t1 = Sx x ; x = t1 Sy
-int dst_solve(oecl_fastpoisson *h, double *x, double *t1, hipStream_t s)
+// Orthonormal DST-I matrix S[j, k] = sqrt(2/(n+1)) sin(pi (j+1)(k+1)/(n+1)), the argument
+// reduced modulo 2(n+1) first so that sin sees at most 2 pi.
+std::vector<double> dst1(uint32_t n)
 {
-    const double one = 1., zero = 0.;
-    int bx = h->bx, by = h->by;
-    RB(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, bx, by, bx, &one, h->Sx,
-                     bx, x, bx, &zero, t1, bx), "dgemm 1");
-    RB(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, bx, by, by, &one, t1,
-                     bx, h->Sy, by, &zero, x, bx), "dgemm 2");
-    hipLaunchKernelGGL(k_scale, dim3(oecl_blocks(h->nb, 256)), dim3(256), 0, s, h->nb, h->Linv, x);
-    RB(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, bx, by, bx, &one, h->Sx,
-                     bx, x, bx, &zero, t1, bx), "dgemm 3");
-    RB(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, bx, by, by, &one, t1,
-                     bx, h->Sy, by, &zero, x, bx), "dgemm 4");
+    std::vector<double> S((size_t)n * n);
+    const double f = sqrt(2. / (n + 1.)), w = M_PI / (n + 1.);
+    for (uint32_t j = 0; j < n; ++j)
+        for (uint32_t k = 0; k < n; ++k) {
+            uint64_t t = ((uint64_t)(j + 1) * (k + 1)) % (2ull * (n + 1));
+            S[j + (size_t)k * n] = f * sin(w * (double)t);
+        }
+    return S;
+}
+
+// Thomas scan over all x modes (unit: cnt batched unit right-hand sides from row a0 on).
+int launch_thomas(const oecl_fastpoisson *h, bool unit, const double *r, double *u, uint32_t a0, uint32_t cnt,
+                  hipStream_t s)
+{
+    const dim3 g((h->bx + 3) / 4, unit ? cnt : 1), b(256);
+    const uint32_t c = (h->by + 63) / 64;
+#define OECL_THOMAS(C)                                                                                   \
+    do {                                                                                                 \
+        if (unit)                                                                                        \
+            hipLaunchKernelGGL((k_thomas_y<C, true>), g, b, 0, s, h->bx, h->by, h->cyu, h->invw, r, u, \
+                               h->sIdx, a0, h->Sx);                                                      \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_thomas_y<C, false>), g, b, 0, s, h->bx, h->by, h->cyu, h->invw, r, u, \
+                               h->sIdx, a0, h->Sx);                                                      \
+    } while (0)
+    if (c <= 4) OECL_THOMAS(4);
+    else if (c <= 8) OECL_THOMAS(8);
+    else if (c <= 16) OECL_THOMAS(16);
+    else if (c <= 32) OECL_THOMAS(32);
+    else return OECL_ERR_ARG;
+#undef OECL_THOMAS
+    return OECL_OK;
+}
+
+// W Sx hat y over all rows (cap: cnt batched unit columns from row a0 on, into C).
+int launch_cap_wy(const oecl_fastpoisson *h, bool cap, const double *yh, double *out, uint32_t a0, uint32_t cnt,
+                  hipStream_t s)
+{
+    const dim3 g((h->m + 3) / 4, cap ? cnt : 1), b(256);
+    const uint32_t c = (h->bx + 63) / 64;
+#define OECL_WY(CQ)                                                                                        \
+    do {                                                                                                   \
+        if (cap)                                                                                           \
+            hipLaunchKernelGGL((k_cap_wy<CQ, true>), g, b, 0, s, h->m, h->bx, h->wPtr, h->wCol, h->wVal,  \
+                               h->Sx, yh, out, (size_t)h->nb, a0);                                         \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_cap_wy<CQ, false>), g, b, 0, s, h->m, h->bx, h->wPtr, h->wCol, h->wVal, \
+                               h->Sx, yh, out, (size_t)h->nb, a0);                                         \
+    } while (0)
+    if (c <= 4) OECL_WY(4);
+    else if (c <= 8) OECL_WY(8);
+    else if (c <= 16) OECL_WY(16);
+    else if (c <= 32) OECL_WY(32);
+    else return OECL_ERR_ARG;
+#undef OECL_WY
     return OECL_OK;
 }
 
 }  // namespace
 
-OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, const double *SxHost,
-                                        const double *SyHost, const double *LinvHost, uint32_t m,
+OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, double cyu, uint32_t m,
                                         const int32_t *sIdxHost, const int32_t *wPtrHost,
                                         const int32_t *wColHost, const double *wValHost,
-                                        const uint8_t *maskDev, oecl_fastpoisson **out,
-                                        double *capDiag)
+                                        const uint8_t *maskDev, oecl_fastpoisson **out, double *capDiag)
 {
-    if (!out || !SxHost || !SyHost || !LinvHost || !maskDev || nx < 5 || ny < 5) return OECL_ERR_ARG;
+    if (!out || !maskDev || nx < 5 || ny < 5 || !(cxu > 0.) || !(cyu > 0.)) return OECL_ERR_ARG;
+    if (ny - 4 > 64 * 32 || nx - 4 > 64 * 32) {
+        oecl_set_error_msg("fastpoisson: nxExt - 4 and nyExt - 4 must not exceed 2048");
+        return OECL_ERR_ARG;
+    }
     if (m > 0 && (!sIdxHost || !wPtrHost || !wColHost || !wValHost)) return OECL_ERR_ARG;
+    const uint32_t bx = nx - 4, by = ny - 4, nb = bx * by;
+    for (uint32_t a = 0; a < m; ++a)
+        if (sIdxHost[a] < 0 || (uint32_t)sIdxHost[a] >= nb) return OECL_ERR_ARG;
+    for (uint32_t a = 0; a < m; ++a)
+        if (wPtrHost[a + 1] < wPtrHost[a] || wPtrHost[a + 1] - wPtrHost[a] > WMAX) return OECL_ERR_ARG;
+    const uint32_t nnz = m ? (uint32_t)wPtrHost[m] : 0;
+    for (uint32_t k = 0; k < nnz; ++k)
+        if (wColHost[k] < 0 || (uint32_t)wColHost[k] >= nb) return OECL_ERR_ARG;
     oecl_fastpoisson *h = new oecl_fastpoisson();
     h->nx = nx;
     h->ny = ny;
     h->np = nx * ny;
-    h->bx = nx - 4;
-    h->by = ny - 4;
-    h->nb = h->bx * h->by;
+    h->bx = bx;
+    h->by = by;
+    h->nb = nb;
     h->m = m;
+    h->cyu = cyu;
     h->mask = maskDev;
-    uint32_t nnz = m ? (uint32_t)wPtrHost[m] : 0;
+    // rows a grouped by their box column j (stable), for the per-column update
+    std::vector<int32_t> jA(m), jCol, jStart;
+    for (uint32_t a = 0; a < m; ++a) jA[a] = (int32_t)a;
+    std::stable_sort(jA.begin(), jA.end(), [&](int32_t u, int32_t v) {
+        return sIdxHost[u] / (int32_t)bx < sIdxHost[v] / (int32_t)bx;
+    });
+    for (uint32_t e = 0; e < m; ++e) {
+        int32_t j = sIdxHost[jA[e]] / (int32_t)bx;
+        if (jCol.empty() || jCol.back() != j) {
+            jCol.push_back(j);
+            jStart.push_back((int32_t)e);
+        }
+    }
+    jStart.push_back((int32_t)m);
+    h->nJ = (uint32_t)jCol.size();
+
     if (rocblas_create_handle(&h->blas) != rocblas_status_success) {
         delete h;
         oecl_set_error_msg("rocblas_create_handle failed");
         return OECL_ERR_HIP;
     }
-    uint64_t bytes = 8ull * ((uint64_t)h->bx * h->bx + (uint64_t)h->by * h->by + 4ull * h->nb +
-                             2ull * m + (uint64_t)m * m + nnz) +
-                     4ull * (m + (m + 1) + nnz) + 1024;
+    const uint64_t kx = (bx + 1) / 2;
+    const uint64_t bytes = 8ull * ((uint64_t)bx * bx + 2ull * kx * kx + 4ull * nb + 2ull * m + (uint64_t)m * m + nnz) +
+                           4ull * (m + (m + 1) + nnz + h->nJ + (h->nJ + 1) + m) + 1024;
     char *mem = nullptr;
     OECL_HIP(hipMalloc(&mem, bytes), "fastpoisson alloc");
     char *cur = mem;
-    h->Sx = (double *)cur; cur += 8ull * h->bx * h->bx;
-    h->Sy = (double *)cur; cur += 8ull * h->by * h->by;
-    h->Linv = (double *)cur; cur += 8ull * h->nb;
-    h->b = (double *)cur; cur += 8ull * h->nb;
-    h->t1 = (double *)cur; cur += 8ull * h->nb;
-    h->t2 = (double *)cur; cur += 8ull * h->nb;
+    h->Sx = (double *)cur; cur += 8ull * bx * bx;
+    h->Fx = (double *)cur; cur += 16ull * kx * kx;
+    h->invw = (double *)cur; cur += 8ull * nb;
+    h->b = (double *)cur; cur += 8ull * nb;
+    h->t1 = (double *)cur; cur += 8ull * nb;
+    h->t2 = (double *)cur; cur += 8ull * nb;
     h->wy = (double *)cur; cur += 8ull * m;
     h->z = (double *)cur; cur += 8ull * m;
-    h->Cinv = (double *)cur; cur += 8ull * m * m;
+    h->CinvT = (double *)cur; cur += 8ull * m * m;
     h->wVal = (double *)cur; cur += 8ull * nnz;
     h->sIdx = (int32_t *)cur; cur += 4ull * m;
     h->wPtr = (int32_t *)cur; cur += 4ull * (m + 1);
     h->wCol = (int32_t *)cur; cur += 4ull * nnz;
-    h->nChunks = (m + 31) / 32;
-    h->part = nullptr;
-    if (m) OECL_HIP(hipMalloc(&h->part, 8ull * h->nChunks * m), "fastpoisson alloc");
-    OECL_HIP(hipMemcpy(h->Sx, SxHost, 8ull * h->bx * h->bx, hipMemcpyHostToDevice), "copy Sx");
-    OECL_HIP(hipMemcpy(h->Sy, SyHost, 8ull * h->by * h->by, hipMemcpyHostToDevice), "copy Sy");
-    OECL_HIP(hipMemcpy(h->Linv, LinvHost, 8ull * h->nb, hipMemcpyHostToDevice), "copy L");
+    h->jCol = (int32_t *)cur; cur += 4ull * h->nJ;
+    h->jStart = (int32_t *)cur; cur += 4ull * (h->nJ + 1);
+    h->jA = (int32_t *)cur; cur += 4ull * m;
+
+    const std::vector<double> Sx = dst1(bx);
+    std::vector<double> lx(bx);
+    for (uint32_t q = 0; q < bx; ++q) lx[q] = 2. - 2. * cos(M_PI * (q + 1.) / (bx + 1.));
+    {
+        // F_z[i][j] = Sx[2i + z][j], column-major kx x kx (odd bx: F_1's last row/column zero)
+        std::vector<double> F(2 * kx * kx, 0.);
+        for (uint32_t z = 0; z < 2; ++z)
+            for (uint32_t i = 0; 2 * i + z < bx; ++i)
+                for (uint32_t j = 0; j < kx; ++j) F[z * kx * kx + i + j * kx] = Sx[(2 * i + z) + (uint64_t)j * bx];
+        // Thomas pivots of (cxu lx_q + 2 cyu) on the diagonal, -cyu off it
+        std::vector<double> iw(nb);
+        for (uint32_t q = 0; q < bx; ++q) {
+            const double d = cxu * lx[q] + 2. * cyu;
+            double w = d;
+            for (uint32_t j = 0; j < by; ++j) {
+                if (j) w = d - cyu * cyu / w;
+                iw[q + (size_t)j * bx] = 1. / w;
+            }
+        }
+        OECL_HIP(hipMemcpy(h->Sx, Sx.data(), 8ull * bx * bx, hipMemcpyHostToDevice), "copy Sx");
+        OECL_HIP(hipMemcpy(h->Fx, F.data(), 16ull * kx * kx, hipMemcpyHostToDevice), "copy Fx");
+        OECL_HIP(hipMemcpy(h->invw, iw.data(), 8ull * nb, hipMemcpyHostToDevice), "copy pivots");
+    }
     if (m) {
         OECL_HIP(hipMemcpy(h->sIdx, sIdxHost, 4ull * m, hipMemcpyHostToDevice), "copy S");
         OECL_HIP(hipMemcpy(h->wPtr, wPtrHost, 4ull * (m + 1), hipMemcpyHostToDevice), "copy W");
         OECL_HIP(hipMemcpy(h->wCol, wColHost, 4ull * nnz, hipMemcpyHostToDevice), "copy W");
         OECL_HIP(hipMemcpy(h->wVal, wValHost, 8ull * nnz, hipMemcpyHostToDevice), "copy W");
-        // Capacitance matrix, built in column chunks of `chunk` batched DST solves.
-        uint32_t chunk = (uint32_t)((512ull << 20) / (16ull * h->nb));
+        OECL_HIP(hipMemcpy(h->jCol, jCol.data(), 4ull * h->nJ, hipMemcpyHostToDevice), "copy columns");
+        OECL_HIP(hipMemcpy(h->jStart, jStart.data(), 4ull * (h->nJ + 1), hipMemcpyHostToDevice), "copy columns");
+        OECL_HIP(hipMemcpy(h->jA, jA.data(), 4ull * m, hipMemcpyHostToDevice), "copy columns");
+        // Capacitance matrix C = I + W R^-1 U with exactly the operator the solve applies
+        // (Sx, the Thomas scan, the W dot products), in chunks of `chunk` unit columns.
+        uint32_t chunk = (uint32_t)((256ull << 20) / (8ull * nb));
         if (chunk < 1) chunk = 1;
         if (chunk > m) chunk = m;
-        double *G = nullptr, *T = nullptr;
-        OECL_HIP(hipMalloc(&G, 8ull * h->nb * chunk), "cap alloc");
-        OECL_HIP(hipMalloc(&T, 8ull * h->nb * chunk), "cap alloc");
-        const double one = 1., zero = 0.;
-        int bx = h->bx, by = h->by;
-        long long snb = h->nb;
+        if (chunk > 65535) chunk = 65535;
+        double *G = nullptr;
+        OECL_HIP(hipMalloc(&G, 8ull * nb * chunk), "cap alloc");
         for (uint32_t a0 = 0; a0 < m; a0 += chunk) {
-            uint32_t cnt = m - a0 < chunk ? m - a0 : chunk;
-            uint64_t tot = (uint64_t)h->nb * cnt;
-            hipLaunchKernelGGL(k_unit_spectra, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, h->bx,
-                               h->by, a0, cnt, h->sIdx, h->Sx, h->Sy, h->Linv, G);
-            RB(rocblas_dgemm_strided_batched(h->blas, rocblas_operation_none, rocblas_operation_none,
-                                             bx, by, bx, &one, h->Sx, bx, 0, G, bx, snb, &zero, T, bx,
-                                             snb, cnt), "cap dgemm 1");
-            RB(rocblas_dgemm_strided_batched(h->blas, rocblas_operation_none, rocblas_operation_none,
-                                             bx, by, by, &one, T, bx, snb, h->Sy, by, 0, &zero, G, bx,
-                                             snb, cnt), "cap dgemm 2");
-            uint64_t tc = (uint64_t)m * cnt;
-            hipLaunchKernelGGL(k_cap_columns, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0, 0, m, a0,
-                               cnt, (uint64_t)h->nb, h->wPtr, h->wCol, h->wVal, G, h->Cinv);
+            const uint32_t cnt = m - a0 < chunk ? m - a0 : chunk;
+            int rc = launch_thomas(h, true, nullptr, G, a0, cnt, 0);
+            if (rc == OECL_OK) rc = launch_cap_wy(h, true, G, h->CinvT, a0, cnt, 0);
+            if (rc != OECL_OK) {
+                (void)hipFree(G);
+                return rc;
+            }
         }
         OECL_CHECK_LAUNCH("capacitance build");
         OECL_HIP(hipDeviceSynchronize(), "capacitance build");
         if (capDiag) {
-            // report max |C| for a scale check
-            double *hc = (double *)malloc(8ull * m);
-            OECL_HIP(hipMemcpy2D(hc, 8, h->Cinv, 8ull * (m + 1), 8, m, hipMemcpyDeviceToHost), "diag");
+            // report max |C_aa| for a scale check
+            std::vector<double> hc(m);
+            OECL_HIP(hipMemcpy2D(hc.data(), 8, h->CinvT, 8ull * (m + 1), 8, m, hipMemcpyDeviceToHost), "diag");
             double mx = 0.;
             for (uint32_t a = 0; a < m; ++a) mx = fabs(hc[a]) > mx ? fabs(hc[a]) : mx;
             *capDiag = mx;
-            free(hc);
         }
-        (void)hipFree(T);
+        (void)hipFree(G);
         // In-place inverse: getrf then getri.
         rocblas_int *ipiv = nullptr, *info = nullptr;
         OECL_HIP(hipMalloc(&ipiv, 4ull * m), "ipiv");
         OECL_HIP(hipMalloc(&info, 4), "info");
-        RB(rocsolver_dgetrf(h->blas, m, m, h->Cinv, m, ipiv, info), "dgetrf");
-        RB(rocsolver_dgetri(h->blas, m, h->Cinv, m, ipiv, info), "dgetri");
+        RB(rocsolver_dgetrf(h->blas, m, m, h->CinvT, m, ipiv, info), "dgetrf");
+        RB(rocsolver_dgetri(h->blas, m, h->CinvT, m, ipiv, info), "dgetri");
         int hinfo = 0;
         OECL_HIP(hipMemcpy(&hinfo, info, 4, hipMemcpyDeviceToHost), "info");
         (void)hipFree(ipiv);
         (void)hipFree(info);
-        (void)hipFree(G);
         if (hinfo != 0) {
             oecl_set_error_msg("capacitance matrix is singular");
             return OECL_ERR_NOCONV;
         }
+    } else if (capDiag) {
+        *capDiag = 0.;
     }
     OECL_HIP(hipDeviceSynchronize(), "fastpoisson create");
     *out = h;
@@ -357,38 +589,26 @@ OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, dou
 {
     if (!h || !q || !phi) return OECL_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
-    const int bx = h->bx, by = h->by;
-    dim3 gg((bx + GT - 1) / GT, (by + GT - 1) / GT), gb(256);
-    hipLaunchKernelGGL(k_gather_box, dim3(oecl_blocks(h->nb, 256)), dim3(256), 0, s, h->nx, h->bx, h->by,
-                       h->mask, q, h->b);
-    hipLaunchKernelGGL(k_zero_outside_box, dim3(oecl_blocks(h->np, 256)), dim3(256), 0, s, h->nx, h->ny, phi);
+    const int bx = h->bx, by = h->by, kx = (bx + 1) / 2;
+    const dim3 gg((kx + GT - 1) / GT, (by + GT - 1) / GT, 2), gb(GNT);
+    hipLaunchKernelGGL(k_gather_box_zero, dim3(oecl_blocks(h->np, 256)), dim3(256), 0, s, h->nx, h->ny, h->bx,
+                       h->mask, q, h->b, phi);
+    // hat b = Sx b
+    hipLaunchKernelGGL(k_dst_gemm<0>, gg, gb, 0, s, by, bx, h->Fx, h->b, bx, h->t1, bx, 0u, nullptr);
     if (h->m) {
-        // y = R^-1 b
-        hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, bx, h->Sx, bx, h->b, bx, h->t1, bx, nullptr, 0u,
-                           nullptr);
-        hipLaunchKernelGGL(k_gemm_nn<1>, gg, gb, 0, s, bx, by, by, h->t1, bx, h->Sy, by, h->t2, bx, h->Linv, 0u,
-                           nullptr);
-        hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, bx, h->Sx, bx, h->t2, bx, h->t1, bx, nullptr, 0u,
-                           nullptr);
-        hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, by, h->t1, bx, h->Sy, by, h->t2, bx, nullptr, 0u,
-                           nullptr);
-        // b -= U C^-1 W y
-        hipLaunchKernelGGL(k_wmul, dim3(oecl_blocks(h->m, 256)), dim3(256), 0, s, h->m, h->wPtr, h->wCol,
-                           h->wVal, h->t2, h->wy);
-        hipLaunchKernelGGL(k_gemv_part, dim3((h->m + 255) / 256, h->nChunks), dim3(256), 0, s, h->m, h->Cinv,
-                           h->wy, h->part);
-        hipLaunchKernelGGL(k_gemv_finish_sub, dim3(oecl_blocks(h->m, 256)), dim3(256), 0, s, h->m, h->nChunks,
-                           h->part, h->sIdx, h->b);
+        // hat y = T^-1 hat b ; wy = W Sx hat y ; hat b -= (U C^-1 wy) in x-transformed form
+        int rc = launch_thomas(h, false, h->t1, h->t2, 0, 1, s);
+        if (rc != OECL_OK) return rc;
+        rc = launch_cap_wy(h, false, h->t2, h->wy, 0, 1, s);
+        if (rc != OECL_OK) return rc;
+        hipLaunchKernelGGL(k_cap_z, dim3((h->m + 3) / 4), dim3(256), 0, s, h->m, h->CinvT, h->wy, h->z);
+        hipLaunchKernelGGL(k_cap_update, dim3((h->bx + 63) / 64, (h->nJ + 3) / 4), dim3(256), 0, s, h->nJ, h->bx,
+                           h->jCol, h->jStart, h->jA, h->sIdx, h->z, h->Sx, h->t1);
     }
-    // phi = R^-1 b, scattered into the extended grid
-    hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, bx, h->Sx, bx, h->b, bx, h->t1, bx, nullptr, 0u,
-                       nullptr);
-    hipLaunchKernelGGL(k_gemm_nn<1>, gg, gb, 0, s, bx, by, by, h->t1, bx, h->Sy, by, h->b, bx, h->Linv, 0u,
-                       nullptr);
-    hipLaunchKernelGGL(k_gemm_nn<0>, gg, gb, 0, s, bx, by, bx, h->Sx, bx, h->b, bx, h->t1, bx, nullptr, 0u,
-                       nullptr);
-    hipLaunchKernelGGL(k_gemm_nn<2>, gg, gb, 0, s, bx, by, by, h->t1, bx, h->Sy, by, phi, 0, nullptr, h->nx,
-                       h->mask);
+    // hat x = T^-1 hat b ; phi = Sx hat x scattered into the extended grid
+    int rc = launch_thomas(h, false, h->t1, h->t2, 0, 1, s);
+    if (rc != OECL_OK) return rc;
+    hipLaunchKernelGGL(k_dst_gemm<2>, gg, gb, 0, s, by, bx, h->Fx, h->t2, bx, phi, 0, h->nx, h->mask);
     OECL_CHECK_LAUNCH("oecl_fastpoisson_solve");
     return OECL_OK;
 }

@@ -45,12 +45,43 @@ __device__ __forceinline__ void deposit_corners(double *rho, const Weights<T> &w
     unsafeAtomicAdd(rho + w.cell + nx + 1, (double)w.wUU * wt * q);
 }
 
+// Sums each run of equal cells among consecutive lanes of the wave (an inclusive suffix
+// scan that stops at run ends, log2(64) shuffle steps); the first lane of a run receives
+// the run's totals and returns true.  Lanes with c == 0xFFFFFFFF never return true.  Atomics
+// to one address from many lanes of a wave are serialised by the hardware, so summing the
+// runs first is what makes the deposit of cell-sorted particles cheap.
+__device__ __forceinline__ bool wave_run_sum(uint32_t c, double &q0, double &q1, double &q2, double &q3)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t cn = __shfl_down(c, 1, 64), cp = __shfl_up(c, 1, 64);
+    bool tail = lane == 63 || cn != c;
+    const bool head = lane == 0 || cp != c;
+    // After the step with offset d, a lane holds the sum over [lane, min(lane + 2d - 1, run end)]
+    // and tail says whether that range reaches the run end; a lane that has not reached it
+    // lies at least d below the run end, so lane + d is in the same run and in range.
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double o0 = __shfl_down(q0, d, 64), o1 = __shfl_down(q1, d, 64);
+        const double o2 = __shfl_down(q2, d, 64), o3 = __shfl_down(q3, d, 64);
+        const bool ot = __shfl_down((int)tail, d, 64) != 0;
+        if (!tail) {
+            q0 += o0;
+            q1 += o1;
+            q2 += o2;
+            q3 += o3;
+            tail = ot;
+        }
+    }
+    return head && c != 0xFFFFFFFFu;
+}
+
 // Block-cooperative deposit for 256-thread blocks; every thread of the block must call it
-// (valid = false when the thread has no particle).  The block's window is the node range
-// [base, base + W) of the lowest cell in the block and the same range one row up; corner
-// contributions inside it are summed in LDS (LDS FP64 atomics) and flushed with one global
-// atomic per touched node, the rest go to global atomics directly.  With particles sorted
-// by cell a block covers a few dozen cells, so global atomics drop by an order of magnitude.
+// (valid = false when the thread has no particle).  Runs of equal cells within a wave are
+// summed first (wave_run_sum).  The block's window is the node range [base, base + W) of
+// the lowest cell in the block and the same range one row up; run totals inside it are
+// summed in LDS (LDS FP64 atomics) and flushed with one global atomic per touched node, the
+// rest go to global atomics directly.  With particles sorted by cell a wave holds a few
+// runs and a block covers a few dozen cells, so both LDS and global atomics become rare.
 template <int W>
 __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t nx, bool valid,
                                               uint32_t cell, double q00, double q10, double q01,
@@ -60,6 +91,8 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
     __shared__ uint32_t wmin[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint32_t c = valid ? cell : 0xFFFFFFFFu;
+    if (!valid) q00 = q10 = q01 = q11 = 0.;
+    valid = wave_run_sum(c, q00, q10, q01, q11);
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
         uint32_t o = __shfl_xor(c, d, 64);

@@ -46,7 +46,7 @@ def _vectorise(bf):
 
 def _wn_pnpoly(px, py, vv):
     """Winding number of points (px, py) against the closed polygon vv (grid.pyx:1464-1480)."""
-    wn = np.zeros(np.shape(px), dtype=np.int64)
+    wn = np.zeros(np.broadcast(px, py).shape, dtype=np.int64)
     for i in range(vv.shape[0] - 1):
         x0, y0 = vv[i]
         x1, y1 = vv[i + 1]

@@ -7,8 +7,9 @@ Setup (host, once) builds from the grid the same operators as the reference:
     as the reference does and moved to the device as CSR.
 solve(q) solves A phi = q on the device and then applies the CSR phi -> E product there.
 Two device methods:
-  'direct' (default)  capacitance-matrix direct solve (DST-I by FP64 GEMMs for the box of
-                      inside nodes plus a dense correction of the rows next to the wall),
+  'direct' (default)  capacitance-matrix direct solve (DST-I along x by FP64 MFMA products
+                      and tridiagonal solves along y for the box of inside nodes, plus a
+                      dense correction of the rows next to the wall),
                       the counterpart of the reference's one-off LU factorisation and
                       back-substitution; its residual is checked against tol every
                       `checkEvery` solves and, if too large, refined by CG.
@@ -23,12 +24,6 @@ import torch
 
 from . import _lib as L
 from .constants import epsilon_0
-
-
-def dst1_matrix(n):
-    """Orthonormal type-I DST matrix S[j, k] = sqrt(2/(n+1)) sin(pi (j+1)(k+1)/(n+1))."""
-    k = np.arange(1, n + 1)
-    return np.sqrt(2. / (n + 1)) * np.sin(np.pi * np.outer(k, k) / (n + 1))
 
 
 def capacitance_rows(nx, ny, mask, cx, cy):
@@ -131,23 +126,17 @@ class PoissonSolver:
         nx, ny = self.nx, self.ny
         cxu, cyu, sIdx, wPtr, wCol, wVal = capacitance_rows(
             nx, ny, self.mask.cpu().numpy(), self.cx_host, self.cy_host)
-        bx, by = nx - 4, ny - 4
-        Sx = np.ascontiguousarray(dst1_matrix(bx))
-        Sy = np.ascontiguousarray(dst1_matrix(by))
-        lx = 2. - 2. * np.cos(np.pi * np.arange(1, bx + 1) / (bx + 1))
-        ly = 2. - 2. * np.cos(np.pi * np.arange(1, by + 1) / (by + 1))
-        Linv = np.ascontiguousarray(1. / (cxu * lx[None, :] + cyu * ly[:, None]))
         self.capacitanceRows = int(sIdx.size)
-        self._fp_keep = (Sx, Sy, Linv, sIdx, wPtr, wCol, wVal)
+        self._fp_keep = (sIdx, wPtr, wCol, wVal)
         h = ctypes.c_void_p(0)
         cap = ctypes.c_double(0.)
         D = ctypes.POINTER(ctypes.c_double)
         I32 = ctypes.POINTER(ctypes.c_int32)
         L.check(L.lib().oecl_fastpoisson_create(
-            L.u32(nx), L.u32(ny), Sx.ctypes.data_as(D), Sy.ctypes.data_as(D),
-            Linv.ctypes.data_as(D), L.u32(sIdx.size), sIdx.ctypes.data_as(I32),
-            wPtr.ctypes.data_as(I32), wCol.ctypes.data_as(I32), wVal.ctypes.data_as(D),
-            L.ptr(self.mask), ctypes.byref(h), ctypes.byref(cap)), 'PoissonSolver direct setup')
+            L.u32(nx), L.u32(ny), L.f64(cxu), L.f64(cyu), L.u32(sIdx.size),
+            sIdx.ctypes.data_as(I32), wPtr.ctypes.data_as(I32), wCol.ctypes.data_as(I32),
+            wVal.ctypes.data_as(D), L.ptr(self.mask), ctypes.byref(h), ctypes.byref(cap)),
+            'PoissonSolver direct setup')
         self._fp = h
         self.capacitanceMax = cap.value
 

@@ -15,14 +15,18 @@ def _rhs(g, seed):
     return q
 
 
+# Box sizes nxExt - 4 and nyExt - 4 of both parities: the direct solve folds each DST
+# product by the transform's symmetry, and odd orders have an unpaired middle entry.
 @pytest.mark.parametrize('args', [(128, 64, 0.1, 0.05, 0), (96, 72, 0.09, 0.07, 1),
-                                  (200, 150, 0.09, 0.07, 1)])
-def test_pcg_matches_lu(args):
+                                  (200, 150, 0.09, 0.07, 1), (101, 77, 0.09, 0.07, 1),
+                                  (65, 50, 0.1, 0.05, 0), (64, 47, 0.1, 0.05, 0)])
+@pytest.mark.parametrize('method', ['direct', 'pcg'])
+def test_solve_matches_lu(args, method):
     from openecloud_amd.grid import Grid
     from openecloud_amd.poissonSolver import PoissonSolver
     from oracle.grid_oracle import OraclePoisson
     g = Grid(*args)
-    ps = PoissonSolver(g, tol=1e-14)
+    ps = PoissonSolver(g, tol=1e-14, method=method)
     q = _rhs(g, 1)
     e = ps.solve(torch.as_tensor(q).cuda()).cpu().numpy()
     phi = ps.getPhi().cpu().numpy()
@@ -35,15 +39,40 @@ def test_pcg_matches_lu(args):
     assert np.array_equal(ps.phiToE().cpu().numpy(), ref.phiToE.dot(ref.phi))
 
 
+@pytest.mark.parametrize('args', [(128, 64, 0.1, 0.05, 0), (200, 150, 0.09, 0.07, 1),
+                                  (101, 77, 0.09, 0.07, 1), (65, 50, 0.1, 0.05, 0),
+                                  (64, 47, 0.1, 0.05, 0), (70, 70, 0.05, 0.05, 4)])
+def test_direct_alone_matches_lu(args):
+    """The capacitance solve without CG refinement (checkEvery=0)."""
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.grid_oracle import OraclePoisson
+    a = list(args)
+    if a[4] == 4:   # polygon chamber: a hexagon's first-quadrant points
+        g = Grid(*a, boundFunc=np.array([[0.024, 0.], [0.012, 0.0208], [0., 0.0208]]))
+    else:
+        g = Grid(*a)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    q = _rhs(g, 3)
+    ps.solvePhi(torch.as_tensor(q).cuda())
+    phi = ps.getPhi().cpu().numpy()
+    ref = OraclePoisson(g)
+    ref.solve(q)
+    assert ps.lastIterations == 0
+    assert np.linalg.norm(phi - ref.phi) <= 1e-10 * np.linalg.norm(ref.phi)
+    assert ps.residual(q) <= 1e-9
+
+
 def test_pcg_warm_start_and_reproducible():
     from openecloud_amd.grid import Grid
     from openecloud_amd.poissonSolver import PoissonSolver
     g = Grid(128, 96, 0.09, 0.07, 1)
-    ps = PoissonSolver(g, tol=1e-12)
+    ps = PoissonSolver(g, tol=1e-12, method='pcg')
     q = torch.as_tensor(_rhs(g, 2)).cuda()
     ps.solve(q)
     it_cold = ps.lastIterations
     phi1 = ps.getPhi().clone()
+    assert it_cold > 64
     ps.solve(q)   # warm start from the solution: converges at once
     assert ps.lastIterations <= 64
     ps.phi.zero_()
