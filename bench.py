@@ -35,7 +35,7 @@ def parse():
     ap.add_argument('--calib-copy', action='store_true',
                     help='run one plain 256 MiB device copy first (HBM counter calibration)')
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
-    ap.add_argument('--sort-every', type=int, default=20,
+    ap.add_argument('--sort-every', type=int, default=50,
                     help='re-order particles by cell every this many steps (0: never)')
     return ap.parse_args()
 
@@ -200,6 +200,13 @@ def main():
         push_ms.append(ms.value)
         L.lib().oecl_event_destroy(e0)
         L.lib().oecl_event_destroy(e1)
+    if a.sort_every:
+        by_age = {}
+        for k, ms in enumerate(push_ms):
+            by_age.setdefault(k % a.sort_every, []).append(ms * 1e3)
+        print('push time (us) by steps since the last sort: ' +
+              ' '.join('%d:%.1f' % (k, sum(v) / len(v)) for k, v in sorted(by_age.items())),
+              file=sys.stderr)
     s = 8 if a.dtype == 'f64' else 4
     npx = g.getNpExt()
     bytes_per = [11 * s * c + 24 * npx for c in counts]

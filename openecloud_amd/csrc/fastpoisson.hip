@@ -47,10 +47,12 @@ struct oecl_fastpoisson {
     int32_t *sIdx;            // [m] box index of row a
     int32_t *wPtr, *wCol;     // CSR of W over box columns, m rows
     double *wVal;
-    int32_t *jCol, *jStart, *jA;  // distinct columns, their ranges in jA, rows a by column
+    int32_t *jCol, *jStart;   // distinct columns j of the irregular nodes, their entry ranges
+    int32_t *ePos;            // [m] entry of row a in the by-column order
+    uint32_t *colOff;         // [m] i_a * bx of entry e (by-column order)
     double *CinvT;            // m*m, C^-1 row-major (row a contiguous)
     double *b, *t1, *t2;      // nb each
-    double *wy, *z;           // m each
+    double *wy, *zE;          // m each (zE in the by-column order)
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
 };
 
@@ -287,22 +289,23 @@ __global__ void __launch_bounds__(256)
     const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (a >= m) return;
     const double *yb = yh + (CAP ? (size_t)blockIdx.y * nb : 0);
-    const int32_t k0 = ptr[a], nk = ptr[a + 1] - k0;
+    const int32_t k0 = ptr[a], nk = ptr[a + 1] - k0;  // 1 <= nk <= WMAX (the node itself first)
     double acc[WMAX];
 #pragma unroll
     for (int e = 0; e < WMAX; ++e) {
-        acc[e] = 0.;
-        if (e < nk) {
-            const uint32_t t = (uint32_t)col[k0 + e], i = t % bx, j = t / bx;
-            const double *sc = Sx + (size_t)i * bx, *yc = yb + (size_t)j * bx;
-            double d = 0.;
+        // absent entries repeat the last one with weight zero, so that every load is issued
+        // unconditionally and all of them are in flight together
+        const int32_t k = k0 + (e < nk ? e : nk - 1);
+        const uint32_t t = (uint32_t)col[k], i = t % bx, j = t / bx;
+        const double *sc = Sx + (size_t)i * bx, *yc = yb + (size_t)j * bx;
+        double d = 0.;
 #pragma unroll
-            for (int c = 0; c < CQ; ++c) {
-                const uint32_t q = lane + 64 * c;
-                if (q < bx) d = fma(sc[q], yc[q], d);
-            }
-            acc[e] = d * val[k0 + e];
+        for (int c = 0; c < CQ; ++c) {
+            const uint32_t q = lane + 64 * c;
+            const uint32_t qq = q < bx ? q : bx - 1;
+            d = fma(sc[qq], q < bx ? yc[qq] : 0., d);
         }
+        acc[e] = e < nk ? d * val[k] : 0.;
     }
     double s = 0.;
 #pragma unroll
@@ -318,9 +321,11 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// z = C^-1 wy, C^-1 row-major: one wave per row a, 8 independent partial sums per lane.
+// z = C^-1 wy, C^-1 row-major: one wave per row a, 8 independent partial sums per lane;
+// z_a is stored at zE[ePos[a]], its place in the by-column order of the update.
 __global__ void __launch_bounds__(256)
-    k_cap_z(uint32_t m, const double *__restrict__ CinvT, const double *__restrict__ wy, double *__restrict__ z)
+    k_cap_z(uint32_t m, const double *__restrict__ CinvT, const double *__restrict__ wy,
+            const int32_t *__restrict__ ePos, double *__restrict__ zE)
 {
     const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (a >= m) return;
@@ -334,26 +339,30 @@ __global__ void __launch_bounds__(256)
     for (; k < m; k += 64) p[0] = fma(row[k], wy[k], p[0]);
     double s = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
     s = wave_sum(s);
-    if (lane == 0) z[a] = s;
+    if (lane == 0) zE[ePos[a]] = s;
 }
 
 // hat b[q, j] -= sum over rows a with j_a = j of z_a Sx[q, i_a]: one thread per (q, column
-// of the distinct-column list), rows taken in order (deterministic).
+// of the distinct-column list), rows taken in order (deterministic).  Entry e of the column
+// order carries zE[e] and the offset colOff[e] = i_a * bx of its Sx column, so the loads of
+// different entries are independent.
 __global__ void __launch_bounds__(256)
     k_cap_update(uint32_t nJ, uint32_t bx, const int32_t *__restrict__ jCol, const int32_t *__restrict__ jStart,
-                 const int32_t *__restrict__ jA, const int32_t *__restrict__ sIdx, const double *__restrict__ z,
-                 const double *__restrict__ Sx, double *__restrict__ bh)
+                 const uint32_t *__restrict__ colOff, const double *__restrict__ zE, const double *__restrict__ Sx,
+                 double *__restrict__ bh)
 {
     const uint32_t q = blockIdx.x * 64 + (threadIdx.x & 63);
     const uint32_t c = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (q >= bx || c >= nJ) return;
-    double s = 0.;
-    const int32_t e1 = jStart[c + 1];
-    for (int32_t e = jStart[c]; e < e1; ++e) {
-        const uint32_t a = (uint32_t)jA[e];
-        s = fma(z[a], Sx[q + (size_t)((uint32_t)sIdx[a] % bx) * bx], s);
+    const int32_t e0 = jStart[c], e1 = jStart[c + 1];
+    double s[4] = {0., 0., 0., 0.};
+    int32_t e = e0;
+    for (; e + 3 < e1; e += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[u] = fma(zE[e + u], Sx[q + colOff[e + u]], s[u]);
     }
-    bh[q + (size_t)jCol[c] * bx] -= s;
+    for (; e < e1; ++e) s[0] = fma(zE[e], Sx[q + colOff[e]], s[0]);
+    bh[q + (size_t)jCol[c] * bx] -= (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 #define RB(call, where)                                         \
@@ -483,7 +492,7 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     }
     const uint64_t kx = (bx + 1) / 2;
     const uint64_t bytes = 8ull * ((uint64_t)bx * bx + 2ull * kx * kx + 4ull * nb + 2ull * m + (uint64_t)m * m + nnz) +
-                           4ull * (m + (m + 1) + nnz + h->nJ + (h->nJ + 1) + m) + 1024;
+                           4ull * (m + (m + 1) + nnz + h->nJ + (h->nJ + 1) + 2ull * m) + 1024;
     char *mem = nullptr;
     OECL_HIP(hipMalloc(&mem, bytes), "fastpoisson alloc");
     char *cur = mem;
@@ -494,7 +503,7 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->t1 = (double *)cur; cur += 8ull * nb;
     h->t2 = (double *)cur; cur += 8ull * nb;
     h->wy = (double *)cur; cur += 8ull * m;
-    h->z = (double *)cur; cur += 8ull * m;
+    h->zE = (double *)cur; cur += 8ull * m;
     h->CinvT = (double *)cur; cur += 8ull * m * m;
     h->wVal = (double *)cur; cur += 8ull * nnz;
     h->sIdx = (int32_t *)cur; cur += 4ull * m;
@@ -502,7 +511,8 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->wCol = (int32_t *)cur; cur += 4ull * nnz;
     h->jCol = (int32_t *)cur; cur += 4ull * h->nJ;
     h->jStart = (int32_t *)cur; cur += 4ull * (h->nJ + 1);
-    h->jA = (int32_t *)cur; cur += 4ull * m;
+    h->ePos = (int32_t *)cur; cur += 4ull * m;
+    h->colOff = (uint32_t *)cur; cur += 4ull * m;
 
     const std::vector<double> Sx = dst1(bx);
     std::vector<double> lx(bx);
@@ -534,7 +544,14 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
         OECL_HIP(hipMemcpy(h->wVal, wValHost, 8ull * nnz, hipMemcpyHostToDevice), "copy W");
         OECL_HIP(hipMemcpy(h->jCol, jCol.data(), 4ull * h->nJ, hipMemcpyHostToDevice), "copy columns");
         OECL_HIP(hipMemcpy(h->jStart, jStart.data(), 4ull * (h->nJ + 1), hipMemcpyHostToDevice), "copy columns");
-        OECL_HIP(hipMemcpy(h->jA, jA.data(), 4ull * m, hipMemcpyHostToDevice), "copy columns");
+        std::vector<int32_t> ePos(m);
+        std::vector<uint32_t> colOff(m);
+        for (uint32_t e = 0; e < m; ++e) {
+            ePos[jA[e]] = (int32_t)e;
+            colOff[e] = ((uint32_t)sIdxHost[jA[e]] % bx) * bx;
+        }
+        OECL_HIP(hipMemcpy(h->ePos, ePos.data(), 4ull * m, hipMemcpyHostToDevice), "copy columns");
+        OECL_HIP(hipMemcpy(h->colOff, colOff.data(), 4ull * m, hipMemcpyHostToDevice), "copy columns");
         // Capacitance matrix C = I + W R^-1 U with exactly the operator the solve applies
         // (Sx, the Thomas scan, the W dot products), in chunks of `chunk` unit columns.
         uint32_t chunk = (uint32_t)((256ull << 20) / (8ull * nb));
@@ -601,9 +618,9 @@ OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, dou
         if (rc != OECL_OK) return rc;
         rc = launch_cap_wy(h, false, h->t2, h->wy, 0, 1, s);
         if (rc != OECL_OK) return rc;
-        hipLaunchKernelGGL(k_cap_z, dim3((h->m + 3) / 4), dim3(256), 0, s, h->m, h->CinvT, h->wy, h->z);
+        hipLaunchKernelGGL(k_cap_z, dim3((h->m + 3) / 4), dim3(256), 0, s, h->m, h->CinvT, h->wy, h->ePos, h->zE);
         hipLaunchKernelGGL(k_cap_update, dim3((h->bx + 63) / 64, (h->nJ + 3) / 4), dim3(256), 0, s, h->nJ, h->bx,
-                           h->jCol, h->jStart, h->jA, h->sIdx, h->z, h->Sx, h->t1);
+                           h->jCol, h->jStart, h->colOff, h->zE, h->Sx, h->t1);
     }
     // hat x = T^-1 hat b ; phi = Sx hat x scattered into the extended grid
     int rc = launch_thomas(h, false, h->t1, h->t2, 0, 1, s);

@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(256) k_push_fused(oecl_grid_t g, WallDev wall,
         }
     }
     if (rhoNext == nullptr) return;  // uniform across the block: no barrier is skipped
-    block_deposit<512>(rhoNext, g.nx, dep, w2.cell, w2.wLL * wq * q, w2.wUL * wq * q,
+    block_deposit<2048>(rhoNext, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q,
                        w2.wLU * wq * q, w2.wUU * wq * q);
 }
 

@@ -7,7 +7,7 @@
 // bounds check there; inside the grid the result is unchanged).
 template <typename T>
 struct Weights {
-    uint32_t cell;
+    uint32_t cell, ix, iy;
     T wLL, wUL, wLU, wUU;
 };
 
@@ -27,6 +27,8 @@ __device__ __forceinline__ Weights<T> grid_weights(T x, T y, T dxi, T dyi, T lxh
     fy = fy - (T)iy;
     Weights<T> w;
     w.cell = ix + iy * nx;
+    w.ix = ix;
+    w.iy = iy;
     w.wLL = (1 - fx) * (1 - fy);
     w.wUL = fx * (1 - fy);
     w.wLU = (1 - fx) * fy;
@@ -76,56 +78,76 @@ __device__ __forceinline__ bool wave_run_sum(uint32_t c, double &q0, double &q1,
 }
 
 // Block-cooperative deposit for 256-thread blocks; every thread of the block must call it
-// (valid = false when the thread has no particle).  Runs of equal cells within a wave are
-// summed first (wave_run_sum).  The block's window is the node range [base, base + W) of
-// the lowest cell in the block and the same range one row up; run totals inside it are
-// summed in LDS (LDS FP64 atomics) and flushed with one global atomic per touched node, the
-// rest go to global atomics directly.  With particles sorted by cell a wave holds a few
-// runs and a block covers a few dozen cells, so both LDS and global atomics become rare.
-template <int W>
-__device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t nx, bool valid,
-                                              uint32_t cell, double q00, double q10, double q01,
-                                              double q11)
+// (valid = false when the thread has no particle), with the particle's cell (ix, iy) and
+// its four corner charges.  Runs of equal cells within a wave are summed first
+// (wave_run_sum).  The run totals are then summed in an LDS tile covering the bounding box
+// of the block's cells (nodes [X0, X1+1] x [Y0, Y1+1]), and each touched node of the tile
+// is added to rho with one global atomic.  Particles sorted by cell, or only locally in
+// order (a few cells of drift since the last sort), keep the box small; a block whose box
+// exceeds WN nodes adds its run totals to rho directly.
+template <int WN>
+__device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t nx, bool valid, uint32_t ix,
+                                              uint32_t iy, double q00, double q10, double q01, double q11)
 {
-    __shared__ double lo[W], hi[W];
-    __shared__ uint32_t wmin[4];
+    __shared__ double win[WN];
+    __shared__ uint32_t bb[4][4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t c = valid ? cell : 0xFFFFFFFFu;
+    const uint32_t c = valid ? ix + iy * nx : 0xFFFFFFFFu;
     if (!valid) q00 = q10 = q01 = q11 = 0.;
-    valid = wave_run_sum(c, q00, q10, q01, q11);
+    const bool head = wave_run_sum(c, q00, q10, q01, q11);
+    uint32_t x0 = head ? ix : 0xFFFFFFFFu, x1 = head ? ix : 0u;
+    uint32_t y0 = head ? iy : 0xFFFFFFFFu, y1 = head ? iy : 0u;
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
-        uint32_t o = __shfl_xor(c, d, 64);
-        c = o < c ? o : c;
+        x0 = min(x0, (uint32_t)__shfl_xor(x0, d, 64));
+        x1 = max(x1, (uint32_t)__shfl_xor(x1, d, 64));
+        y0 = min(y0, (uint32_t)__shfl_xor(y0, d, 64));
+        y1 = max(y1, (uint32_t)__shfl_xor(y1, d, 64));
     }
-    if (lane == 0) wmin[wave] = c;
-    for (int k = tid; k < W; k += 256) {
-        lo[k] = 0.;
-        hi[k] = 0.;
+    if (lane == 0) {
+        bb[wave][0] = x0;
+        bb[wave][1] = x1;
+        bb[wave][2] = y0;
+        bb[wave][3] = y1;
     }
     __syncthreads();
-    uint32_t base = wmin[0];
-    for (int w = 1; w < 4; ++w) base = wmin[w] < base ? wmin[w] : base;
-    if (valid) {
-        uint32_t off = cell - base;
-        if (off + 1 < (uint32_t)W) {
-            atomicAdd(&lo[off], q00);
-            atomicAdd(&lo[off + 1], q10);
-            atomicAdd(&hi[off], q01);
-            atomicAdd(&hi[off + 1], q11);
-        } else {
-            unsafeAtomicAdd(rho + cell, q00);
-            unsafeAtomicAdd(rho + cell + 1, q10);
-            unsafeAtomicAdd(rho + cell + nx, q01);
-            unsafeAtomicAdd(rho + cell + nx + 1, q11);
+    uint32_t X0 = bb[0][0], X1 = bb[0][1], Y0 = bb[0][2], Y1 = bb[0][3];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+        X0 = min(X0, bb[w][0]);
+        X1 = max(X1, bb[w][1]);
+        Y0 = min(Y0, bb[w][2]);
+        Y1 = max(Y1, bb[w][3]);
+    }
+    if (X0 == 0xFFFFFFFFu) return;  // no particle to deposit in this block (block-uniform)
+    const uint32_t wx = X1 - X0 + 2, wy = Y1 - Y0 + 2;
+    if ((uint64_t)wx * wy > WN) {   // block-uniform
+        if (head) {
+            const uint32_t cc = ix + iy * nx;
+            unsafeAtomicAdd(rho + cc, q00);
+            unsafeAtomicAdd(rho + cc + 1, q10);
+            unsafeAtomicAdd(rho + cc + nx, q01);
+            unsafeAtomicAdd(rho + cc + nx + 1, q11);
         }
+        return;
+    }
+    const uint32_t nw = wx * wy;
+    for (uint32_t k = tid; k < nw; k += 256) win[k] = 0.;
+    __syncthreads();
+    if (head) {
+        const uint32_t o = (ix - X0) + (iy - Y0) * wx;
+        atomicAdd(&win[o], q00);
+        atomicAdd(&win[o + 1], q10);
+        atomicAdd(&win[o + wx], q01);
+        atomicAdd(&win[o + wx + 1], q11);
     }
     __syncthreads();
-    if (base == 0xFFFFFFFFu) return;
-    for (int k = tid; k < W; k += 256) {
-        double a = lo[k], b = hi[k];
-        if (a != 0.) unsafeAtomicAdd(rho + base + k, a);
-        if (b != 0.) unsafeAtomicAdd(rho + base + nx + k, b);
+    for (uint32_t k = tid; k < nw; k += 256) {
+        const double v = win[k];
+        if (v != 0.) {
+            const uint32_t r = k / wx;
+            unsafeAtomicAdd(rho + (X0 + (k - r * wx)) + (size_t)(Y0 + r) * nx, v);
+        }
     }
 }
 

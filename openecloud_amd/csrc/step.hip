@@ -12,9 +12,10 @@
 //   compaction: secondaries fill the first holes, the rest append; remaining holes below the
 //   new count are filled from the tail (particles.pyx:222-269 rules)
 //
-// Kernels are launched over the capacity (or the hit / secondary capacities) and read the
-// live counts from device memory.  Overflows set status bits instead of writing past a
-// buffer; the host checks the status word.
+// Kernels read the live counts from device memory: the push grid covers the capacity and
+// whole blocks past the count return at once; the per-hit and per-secondary kernels loop
+// over the live counts with small fixed grids.  Overflows set status bits instead of writing
+// past a buffer; the host checks the status word.
 #include <rocblas/rocblas.h>
 
 #include "common.h"
@@ -31,44 +32,78 @@ enum {
 
 namespace {
 
-__global__ void k_step_begin(const uint32_t *nDev, const uint32_t *stepDev, uint32_t *hist,
-                             uint32_t histCap)
-{
-    uint32_t s = *stepDev;
-    if (s < histCap) hist[s] = *nDev;
-}
+constexpr uint32_t PB = 256;      // particles per push block
+constexpr uint32_t CT = 1024;     // threads of the single-block kernels
+constexpr uint32_t NZMAX = 16384; // push blocks with holes sorted in LDS (more: full scan)
+constexpr uint32_t MAXSEC = 10;   // secondaries per impact (Furman M)
+// step scratch counters
+enum { M_HOLES = 0, M_NZ = 1, M_DONE = 2, M_COUNT = 16 };
 
-__global__ void k_rhs_dev(uint32_t np, const double *__restrict__ rho, const double *__restrict__ table,
-                          uint32_t nScale, const uint32_t *__restrict__ stepDev,
-                          const double *__restrict__ prof, double *__restrict__ q)
+// q = rho + s(step) * beam profile (example01_basic.py:81), then rho = 0 for the deposit of
+// this step; thread 0 records the particle count of the step.
+__global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double *__restrict__ table,
+                            uint32_t nScale, const uint32_t *__restrict__ stepDev,
+                            const double *__restrict__ prof, double *__restrict__ q,
+                            const uint32_t *__restrict__ nDev, uint32_t *__restrict__ hist, uint32_t histCap,
+                            uint32_t *__restrict__ misc)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = *stepDev;
+    if (i == 0) {
+        if (hist && s < histCap) hist[s] = *nDev;
+        misc[M_NZ] = 0;
+        misc[M_DONE] = 0;
+    }
     if (i >= np) return;
-    uint32_t s = *stepDev;
-    double sc = (prof && table && s < nScale) ? table[s] : 0.;
-    q[i] = sc != 0. ? rho[i] + sc * prof[i] : rho[i];
+    const double sc = (prof && table && s < nScale) ? table[s] : 0.;
+    const double r = rho[i];
+    q[i] = sc != 0. ? r + sc * prof[i] : r;
+    rho[i] = 0.;
 }
 
-__global__ void k_zero_d(double *a, uint32_t n)
+// Holes (absorbed particles) of one push block in index order: the block's count goes to
+// counts[block], the indices to list[block * PB + rank], and a block with holes appends its
+// index to nzList (in no particular order).  Every thread of the block calls it.
+__device__ __forceinline__ void block_holes(bool dead, uint32_t i, uint32_t *__restrict__ counts,
+                                            uint32_t *__restrict__ list, uint32_t *__restrict__ misc,
+                                            uint32_t *__restrict__ nzList)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = 0.;
+    __shared__ uint32_t wc[PB / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t bal = __ballot(dead);
+    if (lane == 0) wc[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < (int)(PB / 64); ++w) {
+        const uint32_t c = wc[w];
+        if (w < wave) off += c;
+        tot += c;
+    }
+    if (dead) list[(size_t)blockIdx.x * PB + off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
+    if (threadIdx.x == 0) {
+        counts[blockIdx.x] = tot;
+        if (tot) nzList[atomicAdd(misc + M_NZ, 1u)] = blockIdx.x;
+    }
 }
 
 template <typename T, int TYPEB>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(PB)
     k_push_dev(oecl_grid_t g, WallDev wall, Soa<T> p, const uint32_t *__restrict__ nDev,
                const double *__restrict__ E, const double *__restrict__ B, double b0, double b1,
                double b2, double dt, double k, double q, uint8_t *__restrict__ isInside,
-               double *__restrict__ rho)
+               double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
+               uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < *nDev;
+    const uint32_t n = *nDev;
+    if (blockIdx.x * PB >= n) return;  // whole block: no barrier below is skipped by part of it
+    const uint32_t i = blockIdx.x * PB + threadIdx.x;
+    const bool live = i < n;
     // FP32 storage is widened on load: weights, gather and the Boris rotation run in FP64
     // registers (the kernel is bound by memory traffic, not arithmetic).
     using C = double;
     const C dxi = 1. / g.dx, dyi = 1. / g.dy, lxh = g.lxHalfExt, lyh = g.lyHalfExt;
-    bool dep = false;
+    bool dep = false, dead = false;
     Weights<C> w2{};
     double wq = 0.;
     if (live) {
@@ -86,83 +121,117 @@ __global__ void __launch_bounds__(256)
             dep = true;
             w2 = grid_weights<C>((C)(T)x, (C)(T)y, dxi, dyi, lxh, lyh, g.nx, g.ny);
             wq = (double)p(5, i);
+        } else {
+            dead = true;
         }
     }
-    block_deposit<512>(rho, g.nx, dep, w2.cell, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
+    block_holes(dead, i, counts, list, misc, nzList);
+    block_deposit<2048>(rho, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
                        w2.wUU * wq * q);
 }
 
-__global__ void __launch_bounds__(OECL_SCAN_THREADS)
-    k_tile_holes_dev(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ nDev,
-                     uint32_t *__restrict__ counts)
+// Exclusive prefix of v over the CT threads of the block (wsum: CT/64 LDS words); *tot the sum.
+__device__ __forceinline__ uint32_t block_scan_ct(uint32_t v, uint32_t *wsum, uint32_t *tot)
 {
-    __shared__ uint32_t lds[4];
-    uint32_t n = *nDev;
-    uint64_t base = (uint64_t)blockIdx.x * OECL_SCAN_TILE + threadIdx.x * OECL_SCAN_ITEMS;
-    uint32_t c = 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
 #pragma unroll
-    for (int k = 0; k < OECL_SCAN_ITEMS; ++k) {
-        uint64_t i = base + k;
-        if (i < n && flags[i] == 0) ++c;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
     }
-    uint32_t tot;
-    block_excl_scan(c, lds, &tot);
-    if (threadIdx.x == 0) counts[blockIdx.x] = tot;
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t off = 0, t = 0;
+#pragma unroll
+    for (int w = 0; w < (int)(CT / 64); ++w) {
+        const uint32_t c = wsum[w];
+        if (w < wave) off += c;
+        t += c;
+    }
+    __syncthreads();
+    *tot = t;
+    return off + x - v;
 }
 
-__global__ void __launch_bounds__(OECL_SCAN_THREADS)
-    k_scan_tiles_dev(const uint32_t *__restrict__ counts, uint32_t nTiles, uint32_t *__restrict__ offsets)
-{
-    __shared__ uint32_t lds[4];
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < nTiles; b += OECL_SCAN_THREADS) {
-        uint32_t i = b + threadIdx.x;
-        uint32_t v = i < nTiles ? counts[i] : 0;
-        uint32_t tot;
-        uint32_t ex = block_excl_scan(v, lds, &tot);
-        if (i < nTiles) offsets[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) offsets[nTiles] = carry;
-}
-
-// Absorbed rows (hole of rank r -> row r) and the hole positions.
+// One block: all holes in ascending index order (holePos, the first absCap of them), their
+// count misc[M_HOLES], and the absorbed rows (hole r -> row r).  The push blocks that have
+// holes (nzList, usually a few dozen) are sorted in LDS and their lists concatenated in
+// block order; beyond NZMAX such blocks every block's count is scanned instead.
 template <typename T>
-__global__ void __launch_bounds__(OECL_SCAN_THREADS)
-    k_collect_dev(Soa<T> p, const uint32_t *__restrict__ nDev, const uint8_t *__restrict__ flags,
-                  const uint32_t *__restrict__ tileOff, T *__restrict__ absorbed, uint32_t absCap,
-                  uint32_t *__restrict__ holePos)
+__global__ void __launch_bounds__(CT)
+    k_holes_collect(Soa<T> p, const uint32_t *__restrict__ nDev, const uint32_t *__restrict__ counts,
+                    const uint32_t *__restrict__ list, const uint32_t *__restrict__ nzList,
+                    uint32_t *__restrict__ holePos, uint32_t absCap, T *__restrict__ absorbed,
+                    uint32_t *__restrict__ misc)
 {
-    uint32_t n = *nDev;
-    for_each_ranked(flags, n, tileOff, [&](uint32_t i, bool hole, uint32_t r) {
-        if (hole) {
-            holePos[r] = i;
-            if (r < absCap) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) absorbed[6 * (uint64_t)r + c] = p(c, i);
+    __shared__ uint32_t keys[NZMAX];
+    __shared__ uint32_t wsum[CT / 64];
+    const uint32_t n = *nDev, nb = (n + PB - 1) / PB, nz = misc[M_NZ];
+    const uint32_t tid = threadIdx.x;
+    const bool sorted = nz <= NZMAX;
+    uint32_t N = 1;
+    if (sorted) {
+        while (N < nz) N <<= 1;
+        for (uint32_t k = tid; k < N; k += CT) keys[k] = k < nz ? nzList[k] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= N; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = tid; i < N; i += CT) {
+                    const uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        const uint32_t a = keys[i], b = keys[ixj];
+                        if ((a > b) == ((i & k) == 0)) {
+                            keys[i] = b;
+                            keys[ixj] = a;
+                        }
+                    }
+                }
+                __syncthreads();
             }
         }
-    });
+    }
+    const uint32_t items = sorted ? nz : nb;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < items; b0 += CT) {
+        const uint32_t k = b0 + tid;
+        const uint32_t blk = k < items ? (sorted ? keys[k] : k) : 0;
+        const uint32_t c = k < items ? counts[blk] : 0;
+        uint32_t tot;
+        const uint32_t start = carry + block_scan_ct(c, wsum, &tot);
+        for (uint32_t r = 0; r < c; ++r) {
+            const uint32_t dst = start + r;
+            if (dst < absCap) holePos[dst] = list[(size_t)blk * PB + r];
+        }
+        carry += tot;
+    }
+    if (tid == 0) misc[M_HOLES] = carry;
+    __syncthreads();
+    const uint32_t na = carry < absCap ? carry : absCap;
+    for (uint32_t r = tid; r < na; r += CT) {
+        const uint32_t i = holePos[r];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) absorbed[6 * (uint64_t)r + c] = p(c, i);
+    }
 }
 
-// Wall interaction, normals and the first Furman loop for hit j (< min(holes, absCap)).
-template <typename T>
-__global__ void k_hits_dev(WallDev w, oecl_furman_t P, T *ap, T *nv, T *rts, double dt,
-                           const uint32_t *__restrict__ holes, uint32_t absCap, int emit, uint32_t s0,
-                           uint32_t s1, const uint32_t *__restrict__ stepDev, uint32_t *__restrict__ nSec,
-                           double *__restrict__ impact, uint32_t *stats)
+__device__ __forceinline__ uint32_t n_hits(const uint32_t *misc, uint32_t absCap)
 {
-    uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x;
-    if (jj >= absCap) return;
-    uint32_t h = *holes;
-    uint32_t na = h < absCap ? h : absCap;
-    if (jj >= na) {
-        nSec[jj] = 0;
-        return;
-    }
-    T *a = ap + 6 * (uint64_t)jj;
-    double x = a[0], y = a[1], vx = a[2], vy = a[3];
-    double r = 0., n0 = 0., n1 = 0.;
+    const uint32_t h = misc[M_HOLES];
+    return h < absCap ? h : absCap;
+}
+
+// Interaction point, remaining time and wall normal of an absorbed particle at (x, y) with
+// velocity (vx, vy) (particleBoundary.pyx: AbsorbRectangular/Elliptical/ArbitraryCutCell
+// calculateInteractionPoint and calculateNormalVectors).  found = false when the cut-cell
+// search misses (counted by the caller).
+__device__ __forceinline__ void hit_geom(const WallDev &w, double dt, double &x, double &y, double vx, double vy,
+                                         double &r, double &n0, double &n1, bool &found)
+{
+    r = 0.;
+    n0 = 0.;
+    n1 = 0.;
+    found = true;
     if (w.kind == OECL_WALL_RECT) {
         const double eps = 1e-6;
         double lxh = w.lx * 0.5 - w.dx * eps, lyh = w.ly * 0.5 - w.dy * eps;
@@ -239,136 +308,207 @@ __global__ void k_hits_dev(WallDev w, oecl_furman_t P, T *ap, T *nv, T *rts, dou
                 }
             }
         }
-        if (!stop) atomicAdd(stats + 3, 1u);
+        found = stop;
         uint32_t c2 = wall_cell(w, x, y);
         n0 = w.ccn[2 * c2];
         n1 = w.ccn[2 * c2 + 1];
     }
+}
+
+// Wall interaction, normals (written in place of the absorbed row's position) and the first
+// Furman loop for hit jj: nSec[jj] and impact[jj].
+template <typename T>
+__device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const oecl_furman_t &P, T *ap, T *nv,
+                                            T *rts, double dt, int emit, uint32_t s0, uint32_t s1, uint32_t step,
+                                            uint32_t *__restrict__ nSec, double *__restrict__ impact,
+                                            uint32_t *stats)
+{
+    T *a = ap + 6 * (uint64_t)jj;
+    double x = a[0], y = a[1], r, n0, n1;
+    bool found;
+    hit_geom(w, dt, x, y, a[2], a[3], r, n0, n1, found);
+    if (!found) atomicAdd(stats + 3, 1u);
     a[0] = (T)x;
     a[1] = (T)y;
     rts[jj] = (T)r;
     T *nn = nv + 2 * (uint64_t)jj;
     nn[0] = (T)n0;
     nn[1] = (T)n1;
-    if (!emit) {
-        nSec[jj] = 0;
-        return;
+    uint32_t ns = 0;
+    if (emit) {
+        furman::Rng rng{nullptr, jj, step, s0, s1};
+        ns = furman::pass1<T>(P, a, nn, rng, impact + 4 * (uint64_t)jj);
     }
-    furman::Rng rng{nullptr, jj, *stepDev, s0, s1};
-    nSec[jj] = furman::pass1<T>(P, a, nn, rng, impact + 4 * (uint64_t)jj);
+    nSec[jj] = ns;
+    return ns;
 }
 
 template <typename T>
-__global__ void k_emit_dev(oecl_furman_t P, const T *__restrict__ ap, const T *__restrict__ nv,
-                           const uint32_t *__restrict__ holes, uint32_t absCap,
-                           const uint32_t *__restrict__ nSec, const uint32_t *__restrict__ off,
-                           const double *__restrict__ impact, uint32_t s0, uint32_t s1,
-                           const uint32_t *__restrict__ stepDev, T *__restrict__ out, uint32_t secCap,
-                           uint32_t *stats)
+__global__ void __launch_bounds__(128)
+    k_hits_dev(WallDev w, oecl_furman_t P, T *ap, T *nv, T *rts, double dt, uint32_t absCap, int emit, uint32_t s0,
+               uint32_t s1, const uint32_t *__restrict__ stepDev, uint32_t *__restrict__ nSec,
+               double *__restrict__ impact, const uint32_t *misc, uint32_t *stats)
 {
-    uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t h = *holes;
-    uint32_t na = h < absCap ? h : absCap;
-    if (jj >= na) return;
-    uint32_t ns = nSec[jj];
-    if (ns == 0) return;
-    furman::Rng rng{nullptr, jj, *stepDev, s0, s1};
-    uint32_t base = off[jj];
-    furman::pass2<T>(P, ap + 6 * (uint64_t)jj, nv + 2 * (uint64_t)jj, ns, impact[4 * (uint64_t)jj],
-                     impact + 4 * (uint64_t)(na - 1), rng, [&](uint32_t ii) -> T * {
-                         uint32_t k = base + ii;
-                         if (k >= secCap) {
-                             atomicOr(stats + 2, (uint32_t)ST_SEC_OVERFLOW);
-                             return nullptr;
-                         }
-                         return out + 6 * (uint64_t)k;
-                     });
+    const uint32_t na = n_hits(misc, absCap), step = *stepDev;
+    for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x)
+        hit_one<T>(jj, w, P, ap, nv, rts, dt, emit, s0, s1, step, nSec, impact, stats);
+}
+
+// Second Furman loop, grid-stride over the hits with ns > 0, into the hit's MAXSEC slots of
+// `slots` (impact-major; k_place_dev makes the compact order).  The single-secondary branch
+// takes the LAST hit's first-loop values, as the reference does.
+template <typename T>
+__global__ void __launch_bounds__(64)
+    k_emit_dev(oecl_furman_t P, const T *__restrict__ ap, const T *__restrict__ nv, uint32_t absCap,
+               const uint32_t *__restrict__ nSec, const double *__restrict__ impact, uint32_t s0, uint32_t s1,
+               const uint32_t *__restrict__ stepDev, T *__restrict__ slots, const uint32_t *misc)
+{
+    const uint32_t na = n_hits(misc, absCap), step = *stepDev;
+    for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x) {
+        const uint32_t ns = nSec[jj];
+        if (ns == 0) continue;
+        furman::Rng rng{nullptr, jj, step, s0, s1};
+        T *o = slots + 6 * (uint64_t)MAXSEC * jj;
+        furman::pass2<T>(P, ap + 6 * (uint64_t)jj, nv + 2 * (uint64_t)jj, ns, impact[4 * (uint64_t)jj],
+                         impact + 4 * (uint64_t)(na - 1), rng,
+                         [&](uint32_t ii) -> T * { return o + 6 * (uint64_t)ii; });
+    }
+}
+
+// One block: secOff = exclusive prefix of nSec over the hits (total at secOff[absCap]);
+// secondary s (hit order, then emission order) is deposited, copied to secOut[s] and placed
+// in hole s, or appended after the live particles once the holes are used up.  At most
+// secCap secondaries are taken (more set ST_SEC_OVERFLOW); nothing is placed when the hits
+// overflowed absCap (ST_ABS_OVERFLOW, set at the end of the step).
+template <typename T>
+__global__ void __launch_bounds__(CT)
+    k_place_dev(oecl_grid_t g, Soa<T> p, const T *__restrict__ slots, T *__restrict__ secOut, uint32_t secCap,
+                const uint32_t *__restrict__ nSec, uint32_t *__restrict__ secOff, uint32_t absCap,
+                const uint32_t *__restrict__ nDev, const uint32_t *__restrict__ holePos, uint64_t capacity,
+                double q, double *rho, uint32_t *misc, uint32_t *stats)
+{
+    __shared__ uint32_t wsum[CT / 64];
+    const uint32_t na = n_hits(misc, absCap), h = misc[M_HOLES], n = *nDev;
+    const uint32_t tid = threadIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < na; b0 += CT) {
+        const uint32_t r = b0 + tid;
+        const uint32_t c = r < na ? nSec[r] : 0;
+        uint32_t tot;
+        const uint32_t o = block_scan_ct(c, wsum, &tot);
+        if (r < na) secOff[r] = carry + o;
+        carry += tot;
+    }
+    if (tid == 0) secOff[absCap] = carry;
+    __syncthreads();
+    for (uint32_t jj = tid; jj < na; jj += CT) {
+        const uint32_t ns = nSec[jj], base = secOff[jj];
+        for (uint32_t ii = 0; ii < ns; ++ii) {
+            const uint32_t s = base + ii;
+            if (s >= secCap) {
+                atomicOr(stats + 2, (uint32_t)ST_SEC_OVERFLOW);
+                break;
+            }
+            const T *r = slots + 6 * ((uint64_t)MAXSEC * jj + ii);
+            Weights<T> w = grid_weights<T>(r[0], r[1], (T)(1. / g.dx), (T)(1. / g.dy), (T)g.lxHalfExt,
+                                           (T)g.lyHalfExt, g.nx, g.ny);
+            deposit_corners<T>(rho, w, g.nx, (double)r[5], q);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) secOut[6 * (uint64_t)s + c] = r[c];
+            if (h > absCap) continue;
+            const uint64_t dst = s < h ? holePos[s] : (uint64_t)n + (s - h);
+            if (dst >= capacity) {
+                atomicOr(stats + 2, (uint32_t)ST_CAP_OVERFLOW);
+                continue;
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) p(c, dst) = r[c];
+        }
+    }
 }
 
 __device__ __forceinline__ uint32_t n_added(const uint32_t *secOff, uint32_t absCap, uint32_t secCap)
 {
-    uint32_t t = secOff[absCap];
+    const uint32_t t = secOff[absCap];
     return t < secCap ? t : secCap;
 }
 
+// Holes left below the new count newN = n - h + nAdd are filled from the tail: the t-th
+// position from the end (i = n-1-t >= newN), when it is not a hole, carries the j-th last
+// survivor, j = t - (holes above i), into holePos[nAdd + j] (holes above i counted by a
+// binary search in the ascending hole list).  The last block to finish then updates the
+// counts and the statistics (no block waits for another).
 template <typename T>
-__global__ void k_deposit_sec_dev(oecl_grid_t g, const T *__restrict__ a, const uint32_t *__restrict__ secOff,
-                                  uint32_t absCap, uint32_t secCap, double q, double *rho)
+__global__ void __launch_bounds__(256)
+    k_tail_end_dev(Soa<T> p, uint32_t *__restrict__ nDev, const uint32_t *__restrict__ secOff, uint32_t absCap,
+                   uint32_t secCap, const uint32_t *__restrict__ holePos, uint64_t capacity,
+                   uint32_t *__restrict__ stepDev, uint32_t *misc, uint32_t *stats)
 {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_added(secOff, absCap, secCap)) return;
-    const T *r = a + 6 * (uint64_t)k;
-    Weights<T> w = grid_weights<T>(r[0], r[1], (T)(1. / g.dx), (T)(1. / g.dy), (T)g.lxHalfExt,
-                                   (T)g.lyHalfExt, g.nx, g.ny);
-    deposit_corners<T>(rho, w, g.nx, (double)r[5], q);
-}
-
-template <typename T>
-__global__ void k_place_dev(Soa<T> p, const T *__restrict__ add, const uint32_t *__restrict__ secOff,
-                            uint32_t absCap, uint32_t secCap, const uint32_t *__restrict__ nDev,
-                            const uint32_t *__restrict__ holes, const uint32_t *__restrict__ holePos,
-                            uint64_t capacity, uint32_t *stats)
-{
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_added(secOff, absCap, secCap)) return;
-    uint32_t h = *holes, n = *nDev;
-    uint64_t dst = s < h ? holePos[s] : (uint64_t)n + (s - h);
-    if (dst >= capacity) {
-        atomicOr(stats + 2, (uint32_t)ST_CAP_OVERFLOW);
-        return;
-    }
-#pragma unroll
-    for (int c = 0; c < 6; ++c) p(c, dst) = add[6 * (uint64_t)s + c];
-}
-
-template <typename T>
-__global__ void __launch_bounds__(OECL_SCAN_THREADS)
-    k_tail_dev(Soa<T> p, const uint8_t *__restrict__ flags, const uint32_t *__restrict__ nDev,
-               const uint32_t *__restrict__ secOff, uint32_t absCap, uint32_t secCap,
-               const uint32_t *__restrict__ tileOff, const uint32_t *__restrict__ holes,
-               const uint32_t *__restrict__ holePos)
-{
-    uint32_t n = *nDev, h = *holes, nAdd = n_added(secOff, absCap, secCap);
-    if (nAdd >= h) return;
-    uint32_t newN = n - h + nAdd;
-    for_each_ranked(flags, n, tileOff, [&](uint32_t i, bool hole, uint32_t hb) {
-        if (!hole && i >= newN) {
-            uint32_t j = (n - 1 - i) - (h - hb);
-            uint32_t dst = holePos[nAdd + j];
+    __shared__ bool lastBlock;
+    const uint32_t n = *nDev, h = misc[M_HOLES], nAdd = n_added(secOff, absCap, secCap);
+    if (h <= absCap && nAdd < h) {
+        const uint32_t cnt = h - nAdd;
+        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
+            const uint32_t i = n - 1 - t;
+            uint32_t lo = 0, hi = h;  // first index with holePos > i
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (holePos[mid] <= i) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo > 0 && holePos[lo - 1] == i) continue;
+            const uint32_t j = t - (h - lo);
+            const uint32_t dst = holePos[nAdd + j];
 #pragma unroll
             for (int c = 0; c < 6; ++c) p(c, dst) = p(c, i);
         }
-    });
-}
-
-__global__ void k_step_end(uint32_t *nDev, const uint32_t *holes, const uint32_t *secOff, uint32_t absCap,
-                           uint32_t secCap, uint64_t capacity, uint32_t *stepDev, uint32_t *stats)
-{
-    uint32_t n = *nDev, h = *holes;
-    uint32_t tot = secOff[absCap];
-    uint32_t nAdd = tot < secCap ? tot : secCap;
-    uint64_t newN = (uint64_t)n - h + nAdd;
-    if (h > absCap) stats[2] |= ST_ABS_OVERFLOW;
-    if (tot > secCap) stats[2] |= ST_SEC_OVERFLOW;
-    if (newN > capacity) {
-        stats[2] |= ST_CAP_OVERFLOW;
-        newN = capacity;
     }
-    *nDev = (uint32_t)newN;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        lastBlock = atomicAdd(misc + M_DONE, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!lastBlock || threadIdx.x != 0) return;
+    const uint32_t tot = secOff[absCap];
+    if (tot > secCap) stats[2] |= ST_SEC_OVERFLOW;
+    if (h > absCap) {
+        stats[2] |= ST_ABS_OVERFLOW;  // compaction was skipped; the count stays
+    } else {
+        uint64_t newN = (uint64_t)n - h + nAdd;
+        if (newN > capacity) {
+            stats[2] |= ST_CAP_OVERFLOW;
+            newN = capacity;
+        }
+        *nDev = (uint32_t)newN;
+    }
     stats[0] += h;
     stats[1] += nAdd;
     stats[4] = h;
     stats[5] = nAdd;
     *stepDev += 1;
+    misc[M_DONE] = 0;
+}
+
+unsigned grid_cap(uint64_t items, unsigned block, unsigned maxBlocks)
+{
+    const uint64_t b = (items + block - 1) / block;
+    return (unsigned)(b < 1 ? 1 : (b > maxBlocks ? maxBlocks : b));
 }
 
 }  // namespace
 
+// Work buffer of oecl_step_run: per push block hole counts and lists, the list of blocks
+// with holes, the ascending hole list, counters, and MAXSEC secondary slots per hit.
+static uint64_t slots_offset(uint64_t capacity, uint32_t absCap)
+{
+    const uint64_t nBlk = (capacity + PB - 1) / PB;
+    return ((4 * (nBlk * (PB + 2) + absCap + M_COUNT)) + 255) / 256 * 256;
+}
+
 OECL_EXPORT uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap)
 {
-    uint64_t nt = (capacity + OECL_SCAN_TILE - 1) / OECL_SCAN_TILE + 1;
-    uint64_t ntA = oecl_scan_tiles(absCap) + 1;
-    return 4 * (2 * nt + 8 + capacity + 2 * ntA + 8) + 1024;
+    return slots_offset(capacity, absCap) + 8ull * 6 * MAXSEC * absCap + 1024;
 }
 
 template <typename T>
@@ -378,55 +518,46 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     const uint64_t cap = a->stride;
     const uint32_t np = g->np;
     Soa<T> p{(T *)a->data, a->stride};
-    uint32_t nT = (uint32_t)((cap + OECL_SCAN_TILE - 1) / OECL_SCAN_TILE);
+    const uint64_t nBlk = (cap + PB - 1) / PB;
     uint32_t *counts = (uint32_t *)a->work;
-    uint32_t *off = counts + nT + 1;
-    uint32_t *holes = off + nT;
-    uint32_t *holePos = off + nT + 8;
-    uint32_t *scanw = holePos + cap;
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, s, a->nDev, a->stepDev, a->countHist, a->histCap);
-    hipLaunchKernelGGL(k_rhs_dev, dim3(oecl_blocks(np, 256)), dim3(256), 0, s, np, a->rho, a->beamScale,
-                       a->nScale, a->stepDev, a->profile, a->q);
+    uint32_t *list = counts + nBlk;
+    uint32_t *nzList = list + nBlk * PB;
+    uint32_t *holePos = nzList + nBlk;
+    uint32_t *misc = holePos + a->absCap;
+    T *slots = (T *)((char *)a->work + slots_offset(cap, a->absCap));
+    hipLaunchKernelGGL(k_begin_dev, dim3(oecl_blocks(np, 256)), dim3(256), 0, s, np, a->rho, a->beamScale,
+                       a->nScale, a->stepDev, a->profile, a->q, a->nDev, a->countHist, a->histCap, misc);
     int rc = oecl_fastpoisson_solve(fp, a->q, a->phi, (void *)s);
     if (rc) return rc;
     rc = oecl_csr_spmv(2 * np, a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, a->E, (void *)s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_zero_d, dim3(oecl_blocks(np, 256)), dim3(256), 0, s, a->rho, np);
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
-    dim3 gp(oecl_blocks(cap, 256)), bp(256);
+    const dim3 gp((unsigned)nBlk), bp(PB);
     if (a->evPushBegin) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushBegin, s), "event");
     if (a->typeB == 0)
         hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, *g, wd, p, a->nDev, a->E, a->bField, 0., 0., 0.,
-                           a->dt, kk, a->qe, a->isInside, a->rho);
+                           a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     else if (a->typeB == 1)
         hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, *g, wd, p, a->nDev, a->E, a->bField, a->b0, a->b1,
-                           a->b2, a->dt, kk, a->qe, a->isInside, a->rho);
+                           a->b2, a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     else
         hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, *g, wd, p, a->nDev, a->E, a->bField, 0., 0., 0.,
-                           a->dt, kk, a->qe, a->isInside, a->rho);
+                           a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     if (a->evPushEnd) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushEnd, s), "event");
-    hipLaunchKernelGGL(k_tile_holes_dev, dim3(nT), dim3(OECL_SCAN_THREADS), 0, s, a->isInside, a->nDev, counts);
-    hipLaunchKernelGGL(k_scan_tiles_dev, dim3(1), dim3(OECL_SCAN_THREADS), 0, s, counts, nT, off);
-    hipLaunchKernelGGL(k_collect_dev<T>, dim3(nT), dim3(OECL_SCAN_THREADS), 0, s, p, a->nDev, a->isInside, off,
-                       (T *)a->absorbed, a->absCap, holePos);
-    hipLaunchKernelGGL(k_hits_dev<T>, dim3(oecl_blocks(a->absCap, 128)), dim3(128), 0, s, wd, *P,
-                       (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, holes, a->absCap, a->emit,
-                       a->seed0, a->seed1, a->stepDev, a->nSec, a->impact, a->stats);
-    rc = oecl_launch_scan_u32(a->nSec, a->absCap, a->secOff, scanw, s);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_holes_collect<T>, dim3(1), dim3(CT), 0, s, p, a->nDev, counts, list, nzList, holePos,
+                       a->absCap, (T *)a->absorbed, misc);
+    hipLaunchKernelGGL(k_hits_dev<T>, dim3(grid_cap(a->absCap, 128, 512)), dim3(128), 0, s, wd, *P,
+                       (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, a->absCap, a->emit, a->seed0,
+                       a->seed1, a->stepDev, a->nSec, a->impact, misc, a->stats);
     if (a->emit)
-        hipLaunchKernelGGL(k_emit_dev<T>, dim3(oecl_blocks(a->absCap, 64)), dim3(64), 0, s, *P,
-                           (const T *)a->absorbed, (const T *)a->normals, holes, a->absCap, a->nSec, a->secOff,
-                           a->impact, a->seed0, a->seed1, a->stepDev, (T *)a->secondaries, a->secCap, a->stats);
-    hipLaunchKernelGGL(k_deposit_sec_dev<T>, dim3(oecl_blocks(a->secCap, 256)), dim3(256), 0, s, *g,
-                       (const T *)a->secondaries, a->secOff, a->absCap, a->secCap, a->qe, a->rho);
-    hipLaunchKernelGGL(k_place_dev<T>, dim3(oecl_blocks(a->secCap, 256)), dim3(256), 0, s, p,
-                       (const T *)a->secondaries, a->secOff, a->absCap, a->secCap, a->nDev, holes, holePos, cap,
+        hipLaunchKernelGGL(k_emit_dev<T>, dim3(grid_cap(a->absCap, 64, 1024)), dim3(64), 0, s, *P,
+                           (const T *)a->absorbed, (const T *)a->normals, a->absCap, a->nSec, a->impact, a->seed0,
+                           a->seed1, a->stepDev, slots, misc);
+    hipLaunchKernelGGL(k_place_dev<T>, dim3(1), dim3(CT), 0, s, *g, p, (const T *)slots, (T *)a->secondaries,
+                       a->secCap, a->nSec, a->secOff, a->absCap, a->nDev, holePos, cap, a->qe, a->rho, misc,
                        a->stats);
-    hipLaunchKernelGGL(k_tail_dev<T>, dim3(nT), dim3(OECL_SCAN_THREADS), 0, s, p, a->isInside, a->nDev, a->secOff,
-                       a->absCap, a->secCap, off, holes, holePos);
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, s, a->nDev, holes, a->secOff, a->absCap, a->secCap, cap,
-                       a->stepDev, a->stats);
+    hipLaunchKernelGGL(k_tail_end_dev<T>, dim3(grid_cap(a->absCap, 256, 256)), dim3(256), 0, s, p, a->nDev,
+                       a->secOff, a->absCap, a->secCap, holePos, cap, a->stepDev, misc, a->stats);
     OECL_CHECK_LAUNCH("oecl_step_run");
     return OECL_OK;
 }
