@@ -33,7 +33,7 @@ def parse():
     ap.add_argument('--dtype', default='f64', choices=['f64', 'f32'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--calib-copy', action='store_true',
-                    help='run one plain 256 MiB device copy first (HBM counter calibration)')
+                    help='run one plain 1 GiB device copy first (HBM counter calibration)')
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--sort-every', type=int, default=50,
                     help='re-order particles by cell every this many steps (0: never)')
@@ -154,10 +154,12 @@ def main():
                    maxSteps=a.warmup + a.steps + 8)
     b.sort_particles()
     if a.calib_copy:
-        src_t = torch.ones(32 << 20, dtype=torch.float64, device='cuda')
+        # one plain 1 GiB device copy (beyond the 256 MB last-level cache) for the counters
+        src_t = torch.ones(1 << 27, dtype=torch.float64, device='cuda')
         dst_t = torch.empty_like(src_t)
         torch.cuda.synchronize()
-        dst_t.copy_(src_t)
+        L.check(L.lib().oecl_copy_bytes(L.ptr(dst_t), L.ptr(src_t), L.u64(src_t.numel() * 8), L.stream()),
+                'calibration copy')
         torch.cuda.synchronize()
         del src_t, dst_t
     for k in range(a.warmup):

@@ -70,6 +70,9 @@ typedef struct {
 
 const char *oecl_last_error(void);
 int oecl_device_sync(void);
+/* Plain device-to-device copy of `bytes` (pointers and size 16-byte aligned).  No reference
+ * counterpart: the known-size kernel against which HBM traffic counters are calibrated. */
+int oecl_copy_bytes(void *dst, const void *src, uint64_t bytes, void *stream);
 
 /* ---- particles.pyx ------------------------------------------------------------------ */
 

@@ -24,6 +24,29 @@ OECL_EXPORT int oecl_device_sync(void)
 }
 
 namespace {
+// Plain streaming copy, 16 bytes per lane per iteration (grid-stride).
+__global__ void __launch_bounds__(256)
+    k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, uint64_t n16)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+}  // namespace
+
+OECL_EXPORT int oecl_copy_bytes(void *dst, const void *src, uint64_t bytes, void *stream)
+{
+    if ((!dst || !src) && bytes) return OECL_ERR_ARG;
+    if (((uintptr_t)dst | (uintptr_t)src | bytes) & 15) return OECL_ERR_ARG;
+    const uint64_t n16 = bytes / 16;
+    if (n16 == 0) return OECL_OK;
+    const uint64_t blocks = (n16 + 255) / 256;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256), 0, (hipStream_t)stream,
+                       (uint4 *)dst, (const uint4 *)src, n16);
+    OECL_CHECK_LAUNCH("oecl_copy_bytes");
+    return OECL_OK;
+}
+
+namespace {
 
 // ---------------------------------------------------------------------------- scans
 __global__ void __launch_bounds__(OECL_SCAN_THREADS)
