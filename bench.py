@@ -22,6 +22,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'push_hbm_traffic_ellip512x384_2e6_f64.md')
+
+
+def measured_traffic(dtype, n):
+    """HBM bytes per push launch from the committed counter summary (FP64, 2e6 particles), or
+    None when the summary does not cover this run's particle type and count."""
+    if dtype != 'f64' or n != 2000000 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith('traffic_bytes_per_launch_f64:'):
+            return float(line.split(':')[1])
+    return None
 
 
 def parse():
@@ -250,7 +262,7 @@ def main():
         'sort_every': a.sort_every,
         'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
-                     'traffic': None, 'avg_launch_us': avg_ms * 1e3,
+                     'traffic': measured_traffic(a.dtype, n), 'avg_launch_us': avg_ms * 1e3,
                      'bytes_per_launch': avg_bytes},
     }
     if not a.no_cpu_baseline:
