@@ -59,7 +59,7 @@ def build_case(n, dtype):
     from openecloud_amd.buildup import Buildup
     from openecloud_amd.grid import Grid
     from openecloud_amd.particleBoundary import AbsorbElliptical
-    from openecloud_amd.particleEmitter import FurmanEmitter, homoLoader
+    from openecloud_amd.particleEmitter import FurmanEmitter, PhotoEmitter, homoLoader
     from openecloud_amd.particles import Particles
     from openecloud_amd.poissonSolver import PoissonSolver
     randomGen.seed(0)
@@ -75,7 +75,9 @@ def build_case(n, dtype):
                               nBunches=4)
     homoLoader(g, p, pb, int(n), 1e9 / n, 4.194e5)
     p.reserve(int(n * 2))
-    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
+    # primary photoelectrons, 1e-3 per proton per metre, 1e5 macro-particles per bunch
+    photo = PhotoEmitter(pb, p, beam, photoYield=1e-3, macroPerBunch=1e5, energyMean=7., energySigma=5.)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1, photoEmitter=photo)
     return g, p, pb, em, ps, b
 
 
@@ -252,10 +254,12 @@ def main():
         'data': 'synthetic (seeded uniform load, Philox draws)',
         'config': {'workload': 'elliptical 45x35 mm cut-cell chamber, 512x384 grid, By 0.5 T, '
                                '%d macro-electrons, Gaussian bunches sigma 1.5 mm 1.15e11 ppb x4, '
-                               'dt 5e-12 s, Furman copper delta_max 1.3' % n,
+                               'dt 5e-12 s, Furman copper delta_max 1.3, primary photoelectrons '
+                               '1e-3 per proton per m' % n,
                    'grid': [512, 384], 'particles': n, 'dt': 5e-12,
                    'macro_particles_start_end': [counts[0], p.macroParticleCount],
-                   'absorbed_total': st['absorbed_total'], 'emitted_total': st['emitted_total']},
+                   'absorbed_total': st['absorbed_total'], 'emitted_total': st['emitted_total'],
+                   'photoelectrons_total': st['photo_total']},
         'poisson_solves_per_s': poisson_rate,
         'poisson_relres': res,
         'poisson_capacitance_rows': ps.capacitanceRows,

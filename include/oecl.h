@@ -180,6 +180,16 @@ int oecl_homo_loader(const oecl_wall_t *wall, int dtype, void *data, uint64_t st
                      uint32_t n, double weightInit, double vsigma, int randomizeWeight,
                      uint32_t seed0, uint32_t seed1, uint32_t *failed, void *stream);
 
+/* Primary photoelectrons of time step `step` into out[n][6] (AoS).  No counterpart in the
+ * reference (its Readme.md lists photoemission as an outlook); the source of the elliptical
+ * dipole workload.  Rectangular or elliptical wall: points on the wall inset by 1e-3 cell,
+ * energy |e0 + sigmaE z| eV, direction about the inward normal with the reference emitter's
+ * secondary angles (particleEmitter.pyx:289-302), weight `weight`.  Philox counter
+ * (k, t, step, 6). */
+int oecl_photo_emit(const oecl_wall_t *wall, int dtype, void *out, uint32_t n, uint32_t step,
+                    uint32_t seed0, uint32_t seed1, double e0, double sigmaE, double weight,
+                    double particleMass, void *stream);
+
 /* ---- Particles.addAndRemoveParticles (particles.pyx:94-121) ------------------------- */
 
 /* Applies the reference's three compaction rules on the SoA block:
@@ -302,10 +312,14 @@ int oecl_fastpoisson_destroy(oecl_fastpoisson *h);
 
 /* Arguments of oecl_step_run.  Counts live on the device: nDev (particles), stepDev (step
  * index, also the Philox stream of the emitter and the index into beamScale).  stats[8]:
- * 0 total absorbed, 1 total emitted, 2 status bits (1 more hits than absCap, 2 more
- * secondaries than secCap, 4 capacity exceeded), 3 cut-cell crossings not found, 4 hits and
- * 5 secondaries of the last step.  countHist[step] receives the particle count at the start
- * of each step (step < histCap). */
+ * 0 total absorbed, 1 total emitted secondaries, 2 status bits (1 more hits than absCap,
+ * 2 more secondaries and photoelectrons than secCap, 4 capacity exceeded, 8 more
+ * photoelectrons in a step than photoCap), 3 cut-cell crossings not found, 4 hits and 5 added
+ * particles (secondaries + photoelectrons) of the last step, 6 total photoelectrons.
+ * countHist[step] receives the particle count at the start of each step (step < histCap).
+ * Photoelectrons (oecl_photo_emit's source, no counterpart in the reference) are generated
+ * when photoCount is not NULL: photoCount[step] macro-particles of weight photoWeight[step]
+ * (step < nPhoto), added after the step's secondaries. */
 typedef struct {
     int32_t dtype;
     void *data;                 /* particle SoA block, 6 rows of `stride` (= capacity) */
@@ -337,6 +351,12 @@ typedef struct {
     uint32_t histCap;
     void *work;                 /* oecl_step_work_bytes(stride, absCap) */
     void *evPushBegin, *evPushEnd;   /* hipEvent_t recorded around the push kernel, or NULL */
+    const uint32_t *photoCount; /* [nPhoto] photoelectron macro-particles per step, or NULL */
+    const double *photoWeight;  /* [nPhoto] their weight */
+    uint32_t nPhoto;
+    double photoE0, photoSigmaE;  /* eV */
+    void *photoSlots;           /* [photoCap][6] */
+    uint32_t photoCap;
 } oecl_step_args;
 
 uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap);

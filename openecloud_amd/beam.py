@@ -78,6 +78,14 @@ class _BeamBase:
         b = float(_norm_cdf(np.clip((tRed - self.dt / 2.) / s - lim, -lim, lim)))
         return (a - b) * self.nParticles / self.dt / self.beamVelocity * self.charge
 
+    def bunchFraction(self, t):
+        """Fraction of one bunch's protons that passes during the step starting at t (0 outside
+        a bunch window): getScale(t) without the charge-per-length factor."""
+        s = self.getScale(t)
+        if s == 0.:
+            return 0.
+        return s * self.dt * self.beamVelocity / (self.nParticles * self.charge)
+
     def getCharge(self, t):
         s = self.getScale(t)
         if s == 0.:

@@ -37,17 +37,21 @@ class StepArgs(ctypes.Structure):
                 ('secondaries', _VP), ('secCap', ctypes.c_uint32), ('emit', ctypes.c_int32),
                 ('seed0', ctypes.c_uint32), ('seed1', ctypes.c_uint32), ('stepDev', _VP),
                 ('stats', _VP), ('countHist', _VP), ('histCap', ctypes.c_uint32), ('work', _VP),
-                ('evPushBegin', _VP), ('evPushEnd', _VP)]
+                ('evPushBegin', _VP), ('evPushEnd', _VP), ('photoCount', _VP), ('photoWeight', _VP),
+                ('nPhoto', ctypes.c_uint32), ('photoE0', ctypes.c_double),
+                ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32)]
 
 
-ST_ABS_OVERFLOW, ST_SEC_OVERFLOW, ST_CAP_OVERFLOW = 1, 2, 4
+ST_ABS_OVERFLOW, ST_SEC_OVERFLOW, ST_CAP_OVERFLOW, ST_PHOTO_OVERFLOW = 1, 2, 4, 8
 
 
 class Buildup:
 
     def __init__(self, gridObj, particlesObj, boundaryObj, emitterObj, poissonObj, beamObj=None,
-                 dt=1e-11, typeBField=0, bAtGridPoints=None, secondaryEmission=True):
+                 dt=1e-11, typeBField=0, bAtGridPoints=None, secondaryEmission=True,
+                 photoEmitter=None):
         self.g = gridObj
+        self.photo = photoEmitter
         self.p = particlesObj
         self.pb = boundaryObj
         self.em = emitterObj
@@ -69,7 +73,7 @@ class Buildup:
         self._cnt = torch.zeros(8, dtype=torch.int32, device=dev)
         self._pin = torch.zeros(8, dtype=torch.int32).pin_memory()
         self._work = None
-        self.history = {'macro': [], 'absorbed': [], 'emitted': [], 'charge': []}
+        self.history = {'macro': [], 'absorbed': [], 'emitted': [], 'photo': [], 'charge': []}
 
     # ------------------------------------------------------------------ helpers
     def _work_buf(self, n, nadd=0):
@@ -123,17 +127,18 @@ class Buildup:
         pb.saveAbsorbed()
         na = pb.getAbsorbedMacroParticleCount()
         nsec = 0
+        sec = torch.empty((0, 6), dtype=p.dtype, device=L.device())
         if na > 0:
             pb.calculateInteractionPoint()
             pb.calculateNormalVectors()
             if self.secondaryEmission:
                 sec = self.em.generateSecondaries(step=self.it)
-            else:
-                sec = torch.empty((0, 6), dtype=p.dtype, device=L.device())
             nsec = sec.shape[0]
-            p.addAndRemoveParticles(sec, p.getIsInside())
+        ph = self.photo.generate(self.it) if self.photo is not None else sec[:0]
+        if na > 0 or ph.shape[0] > 0:
+            p.addAndRemoveParticles(torch.cat([sec, ph]) if ph.shape[0] else sec, p.getIsInside())
         self._rho_valid = False
-        self._record(na, nsec)
+        self._record(na, nsec, ph.shape[0])
         self.it += 1
 
     # ------------------------------------------------------------------ fused plain step
@@ -175,10 +180,19 @@ class Buildup:
             if self.secondaryEmission:
                 sec = em.generateSecondaries(step=self.it)
                 nsec = sec.shape[0]
-                if nsec:
-                    L.check(L.lib().oecl_deposit_add_aos(
-                        ctypes.byref(self.gd), L.i32(p.code), L.ptr(sec), L.u32(nsec),
-                        L.f64(p.particleCharge), L.ptr(self.rho), s), 'deposit secondaries')
+        nph = 0
+        if self.photo is not None:
+            ph = self.photo.generate(self.it)
+            nph = ph.shape[0]
+            if nph:
+                sec = ph if not nsec else torch.cat([sec, ph])
+        nadd = nsec + nph
+        if nadd:
+            L.check(L.lib().oecl_deposit_add_aos(
+                ctypes.byref(self.gd), L.i32(p.code), L.ptr(sec), L.u32(nadd),
+                L.f64(p.particleCharge), L.ptr(self.rho), s), 'deposit secondaries')
+        if na > 0 or nadd > 0:
+            nsec = nadd
             new = n - na + nsec
             if new > p.capacity:
                 keep = p.isInside[:n].clone()
@@ -204,7 +218,7 @@ class Buildup:
                 p._data, p._spare = p._spare, p._data
             p.macroParticleCount = new
         self._rho_valid = True
-        self._record(na, nsec)
+        self._record(na, nsec - nph, nph)
         self.it += 1
 
     # ------------------------------------------------------------------ native step
@@ -230,11 +244,12 @@ class Buildup:
         self._nDev = torch.tensor([n], dtype=torch.int32, device=dev)
         self._stepDev = torch.tensor([self.it], dtype=torch.int32, device=dev)
         self._stats = torch.zeros(8, dtype=torch.int32, device=dev)
-        self._hist = torch.zeros(self.maxSteps, dtype=torch.int32, device=dev)
+        nTab = self.it + self.maxSteps   # tables are indexed by the absolute step
+        self._hist = torch.zeros(nTab, dtype=torch.int32, device=dev)
         self._stepwork = torch.empty(int(L.lib().oecl_step_work_bytes(L.u64(cap), L.u32(absCap))) // 8 + 64,
                                      dtype=torch.float64, device=dev)
         if self.beam is not None:
-            sc = np.array([self.beam.getScale(k * self.dt) for k in range(self.maxSteps)])
+            sc = np.array([self.beam.getScale(k * self.dt) for k in range(nTab)])
             self._scale = L.to_dev(sc)
             prof = self.beam.getProfile()
         else:
@@ -250,7 +265,7 @@ class Buildup:
         a.q = self.q.data_ptr()
         a.profile = prof.data_ptr() if prof is not None else None
         a.beamScale = self._scale.data_ptr() if self._scale is not None else None
-        a.nScale = self.maxSteps if self._scale is not None else 0
+        a.nScale = nTab if self._scale is not None else 0
         a.phi = ps.phi.data_ptr()
         a.E = ps.eAtGridPoints.data_ptr()
         a.e2nRowPtr = ps.rowPtr.data_ptr()
@@ -276,7 +291,20 @@ class Buildup:
         a.stepDev = self._stepDev.data_ptr()
         a.stats = self._stats.data_ptr()
         a.countHist = self._hist.data_ptr()
-        a.histCap = self.maxSteps
+        a.histCap = nTab
+        if self.photo is not None:
+            cnt, wts = self.photo.tables(nTab)
+            self._photoCount = torch.as_tensor(cnt.astype(np.int32)).to(dev)
+            self._photoWeight = L.to_dev(wts)
+            self.photoCap = max(1, int(cnt.max()) if cnt.size else 1)
+            self._photoSlots = torch.empty((self.photoCap, 6), dtype=p.dtype, device=dev)
+            a.photoCount = self._photoCount.data_ptr()
+            a.photoWeight = self._photoWeight.data_ptr()
+            a.nPhoto = nTab
+            a.photoE0 = self.photo.energyMean
+            a.photoSigmaE = self.photo.energySigma
+            a.photoSlots = self._photoSlots.data_ptr()
+            a.photoCap = self.photoCap
         a.work = self._stepwork.data_ptr()
         self._args = a
         self._native_it0 = self.it
@@ -302,10 +330,12 @@ class Buildup:
             raise L.OeclError('more wall hits in one step than absCap=%d' % self.absCap)
         if st[2] & ST_SEC_OVERFLOW:
             raise L.OeclError('more secondaries in one step than secCap=%d' % self.secCap)
+        if st[2] & ST_PHOTO_OVERFLOW:
+            raise L.OeclError('more photoelectrons in one step than the tables allow')
         if st[2] & ST_CAP_OVERFLOW:
             raise L.OeclError('particle count exceeded the capacity %d' % self.p.capacity)
         return {'absorbed_total': st[0], 'emitted_total': st[1], 'not_found': st[3],
-                'absorbed_last': st[4], 'emitted_last': st[5]}
+                'absorbed_last': st[4], 'added_last': st[5], 'photo_total': st[6]}
 
     def sort_particles(self):
         """Orders the particles by grid cell (device radix sort + permutation).  Needs the host
@@ -335,11 +365,12 @@ class Buildup:
         b = last - self._native_it0
         return self._hist[a:b].cpu().tolist()
 
-    def _record(self, na, nsec):
+    def _record(self, na, nsec, nph=0):
         h = self.history
         h['macro'].append(self.p.macroParticleCount)
         h['absorbed'].append(na)
         h['emitted'].append(nsec)
+        h['photo'].append(nph)
 
     def run(self, nSteps, fused=True):
         for _ in range(int(nSteps)):

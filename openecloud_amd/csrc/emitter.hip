@@ -7,6 +7,7 @@
 //   5..13 thetaK, 14 y, 15+2i / 16+2i theta / phi of secondary i.
 #include "common.h"
 #include "furman_dev.h"
+#include "photo_dev.h"
 #include "scan.h"
 #include "wall.h"
 
@@ -89,7 +90,41 @@ __global__ void k_homo_loader(WallDev w, Soa<T> p, uint32_t n, double weightInit
     p(5, i) = (T)wt;
 }
 
+template <typename T>
+__global__ void k_photo_emit(WallDev w, T *out, uint32_t n, uint32_t step, uint32_t s0, uint32_t s1, double e0,
+                             double sigmaE, double weight, double mass)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) photo::emit_one<T>(w, k, step, s0, s1, e0, sigmaE, weight, mass, out + 6 * (uint64_t)k);
+}
+
 }  // namespace
+
+OECL_EXPORT int oecl_photo_emit(const oecl_wall_t *wall, int dtype, void *out, uint32_t n, uint32_t step,
+                                uint32_t seed0, uint32_t seed1, double e0, double sigmaE, double weight,
+                                double particleMass, void *stream)
+{
+    if (!wall || (n && !out)) return OECL_ERR_ARG;
+    if (wall->kind != OECL_WALL_RECT && wall->kind != OECL_WALL_ELLIP) {
+        oecl_set_error_msg("photoelectrons: rectangular or elliptical wall only");
+        return OECL_ERR_ARG;
+    }
+    if (n == 0) return OECL_OK;
+    WallDev wd;
+    int rc = wall_to_device(wall, &wd);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_photo_emit<double>, dim3(oecl_blocks(n, 128)), dim3(128), 0, s, wd, (double *)out, n,
+                           step, seed0, seed1, e0, sigmaE, weight, particleMass);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_photo_emit<float>, dim3(oecl_blocks(n, 128)), dim3(128), 0, s, wd, (float *)out, n,
+                           step, seed0, seed1, e0, sigmaE, weight, particleMass);
+    else
+        return OECL_ERR_ARG;
+    OECL_CHECK_LAUNCH("oecl_photo_emit");
+    return OECL_OK;
+}
 
 OECL_EXPORT int oecl_furman_pass1(const oecl_furman_t *prm, int dtype, const void *absorbed,
                                   const void *normals, uint32_t nAbs, const double *uniforms,

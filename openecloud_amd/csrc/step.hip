@@ -20,6 +20,7 @@
 
 #include "common.h"
 #include "furman_dev.h"
+#include "photo_dev.h"
 #include "particles_dev.h"
 #include "scan.h"
 #include "wall.h"
@@ -27,7 +28,8 @@
 enum {
     ST_ABS_OVERFLOW = 1u,   // more wall hits than absCap
     ST_SEC_OVERFLOW = 2u,   // more secondaries than secCap
-    ST_CAP_OVERFLOW = 4u    // particle count would exceed the capacity
+    ST_CAP_OVERFLOW = 4u,   // particle count would exceed the capacity
+    ST_PHOTO_OVERFLOW = 8u  // more photoelectrons in one step than photoCap
 };
 
 namespace {
@@ -37,7 +39,7 @@ constexpr uint32_t CT = 1024;     // threads of the single-block kernels
 constexpr uint32_t NZMAX = 16384; // push blocks with holes sorted in LDS (more: full scan)
 constexpr uint32_t MAXSEC = 10;   // secondaries per impact (Furman M)
 // step scratch counters
-enum { M_HOLES = 0, M_NZ = 1, M_DONE = 2, M_COUNT = 16 };
+enum { M_HOLES = 0, M_NZ = 1, M_DONE = 2, M_PHOTO = 3, M_PHOTO_PLACED = 4, M_COUNT = 16 };
 
 // q = rho + s(step) * beam profile (example01_basic.py:81), then rho = 0 for the deposit of
 // this step; thread 0 records the particle count of the step.
@@ -53,6 +55,8 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
         if (hist && s < histCap) hist[s] = *nDev;
         misc[M_NZ] = 0;
         misc[M_DONE] = 0;
+        misc[M_PHOTO] = 0;
+        misc[M_PHOTO_PLACED] = 0;
     }
     if (i >= np) return;
     const double sc = (prof && table && s < nScale) ? table[s] : 0.;
@@ -375,6 +379,25 @@ __global__ void __launch_bounds__(64)
     }
 }
 
+// Photoelectrons of this step into their slots (count from the per-step table).
+template <typename T>
+__global__ void __launch_bounds__(128)
+    k_photo_dev(WallDev w, const uint32_t *__restrict__ count, const double *__restrict__ weight, uint32_t nPhoto,
+                uint32_t photoCap, const uint32_t *__restrict__ stepDev, uint32_t s0, uint32_t s1, double e0,
+                double sigmaE, double mass, T *__restrict__ slots, uint32_t *misc, uint32_t *stats)
+{
+    const uint32_t step = *stepDev;
+    uint32_t n = step < nPhoto ? count[step] : 0;
+    if (n > photoCap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(stats + 2, (uint32_t)ST_PHOTO_OVERFLOW);
+        n = photoCap;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) misc[M_PHOTO] = n;
+    const double wt = n ? weight[step] : 0.;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        photo::emit_one<T>(w, k, step, s0, s1, e0, sigmaE, wt, mass, slots + 6 * (uint64_t)k);
+}
+
 // One block: secOff = exclusive prefix of nSec over the hits (total at secOff[absCap]);
 // secondary s (hit order, then emission order) is deposited, copied to secOut[s] and placed
 // in hole s, or appended after the live particles once the holes are used up.  At most
@@ -382,10 +405,11 @@ __global__ void __launch_bounds__(64)
 // overflowed absCap (ST_ABS_OVERFLOW, set at the end of the step).
 template <typename T>
 __global__ void __launch_bounds__(CT)
-    k_place_dev(oecl_grid_t g, Soa<T> p, const T *__restrict__ slots, T *__restrict__ secOut, uint32_t secCap,
-                const uint32_t *__restrict__ nSec, uint32_t *__restrict__ secOff, uint32_t absCap,
-                const uint32_t *__restrict__ nDev, const uint32_t *__restrict__ holePos, uint64_t capacity,
-                double q, double *rho, uint32_t *misc, uint32_t *stats)
+    k_place_dev(oecl_grid_t g, Soa<T> p, const T *__restrict__ slots, const T *__restrict__ photoSlots,
+                T *__restrict__ secOut, uint32_t secCap, const uint32_t *__restrict__ nSec,
+                uint32_t *__restrict__ secOff, uint32_t absCap, const uint32_t *__restrict__ nDev,
+                const uint32_t *__restrict__ holePos, uint64_t capacity, double q, double *rho, uint32_t *misc,
+                uint32_t *stats)
 {
     __shared__ uint32_t wsum[CT / 64];
     const uint32_t na = n_hits(misc, absCap), h = misc[M_HOLES], n = *nDev;
@@ -399,8 +423,28 @@ __global__ void __launch_bounds__(CT)
         if (r < na) secOff[r] = carry + o;
         carry += tot;
     }
-    if (tid == 0) secOff[absCap] = carry;
+    const uint32_t nPh = misc[M_PHOTO];
+    if (tid == 0) {
+        secOff[absCap] = carry + nPh;
+        misc[M_PHOTO_PLACED] = carry >= secCap ? 0u : min(nPh, secCap - carry);
+    }
     __syncthreads();
+    // secondary s (or photoelectron s - carry): deposit, copy out, place
+    auto place = [&](uint32_t s, const T *r) {
+        Weights<T> w = grid_weights<T>(r[0], r[1], (T)(1. / g.dx), (T)(1. / g.dy), (T)g.lxHalfExt,
+                                       (T)g.lyHalfExt, g.nx, g.ny);
+        deposit_corners<T>(rho, w, g.nx, (double)r[5], q);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) secOut[6 * (uint64_t)s + c] = r[c];
+        if (h > absCap) return;
+        const uint64_t dst = s < h ? holePos[s] : (uint64_t)n + (s - h);
+        if (dst >= capacity) {
+            atomicOr(stats + 2, (uint32_t)ST_CAP_OVERFLOW);
+            return;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) p(c, dst) = r[c];
+    };
     for (uint32_t jj = tid; jj < na; jj += CT) {
         const uint32_t ns = nSec[jj], base = secOff[jj];
         for (uint32_t ii = 0; ii < ns; ++ii) {
@@ -409,21 +453,16 @@ __global__ void __launch_bounds__(CT)
                 atomicOr(stats + 2, (uint32_t)ST_SEC_OVERFLOW);
                 break;
             }
-            const T *r = slots + 6 * ((uint64_t)MAXSEC * jj + ii);
-            Weights<T> w = grid_weights<T>(r[0], r[1], (T)(1. / g.dx), (T)(1. / g.dy), (T)g.lxHalfExt,
-                                           (T)g.lyHalfExt, g.nx, g.ny);
-            deposit_corners<T>(rho, w, g.nx, (double)r[5], q);
-#pragma unroll
-            for (int c = 0; c < 6; ++c) secOut[6 * (uint64_t)s + c] = r[c];
-            if (h > absCap) continue;
-            const uint64_t dst = s < h ? holePos[s] : (uint64_t)n + (s - h);
-            if (dst >= capacity) {
-                atomicOr(stats + 2, (uint32_t)ST_CAP_OVERFLOW);
-                continue;
-            }
-#pragma unroll
-            for (int c = 0; c < 6; ++c) p(c, dst) = r[c];
+            place(s, slots + 6 * ((uint64_t)MAXSEC * jj + ii));
         }
+    }
+    for (uint32_t k = tid; k < nPh; k += CT) {
+        const uint32_t s = carry + k;
+        if (s >= secCap) {
+            atomicOr(stats + 2, (uint32_t)ST_SEC_OVERFLOW);
+            break;
+        }
+        place(s, photoSlots + 6 * (uint64_t)k);
     }
 }
 
@@ -482,10 +521,12 @@ __global__ void __launch_bounds__(256)
         }
         *nDev = (uint32_t)newN;
     }
+    const uint32_t nPh = misc[M_PHOTO_PLACED];
     stats[0] += h;
-    stats[1] += nAdd;
+    stats[1] += nAdd - nPh;
     stats[4] = h;
     stats[5] = nAdd;
+    stats[6] += nPh;
     *stepDev += 1;
     misc[M_DONE] = 0;
 }
@@ -553,9 +594,13 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
         hipLaunchKernelGGL(k_emit_dev<T>, dim3(grid_cap(a->absCap, 64, 1024)), dim3(64), 0, s, *P,
                            (const T *)a->absorbed, (const T *)a->normals, a->absCap, a->nSec, a->impact, a->seed0,
                            a->seed1, a->stepDev, slots, misc);
-    hipLaunchKernelGGL(k_place_dev<T>, dim3(1), dim3(CT), 0, s, *g, p, (const T *)slots, (T *)a->secondaries,
-                       a->secCap, a->nSec, a->secOff, a->absCap, a->nDev, holePos, cap, a->qe, a->rho, misc,
-                       a->stats);
+    if (a->photoCount)
+        hipLaunchKernelGGL(k_photo_dev<T>, dim3(grid_cap(a->photoCap, 128, 256)), dim3(128), 0, s, wd, a->photoCount,
+                           a->photoWeight, a->nPhoto, a->photoCap, a->stepDev, a->seed0, a->seed1, a->photoE0,
+                           a->photoSigmaE, P->particleMass, (T *)a->photoSlots, misc, a->stats);
+    hipLaunchKernelGGL(k_place_dev<T>, dim3(1), dim3(CT), 0, s, *g, p, (const T *)slots, (const T *)a->photoSlots,
+                       (T *)a->secondaries, a->secCap, a->nSec, a->secOff, a->absCap, a->nDev, holePos, cap, a->qe,
+                       a->rho, misc, a->stats);
     hipLaunchKernelGGL(k_tail_end_dev<T>, dim3(grid_cap(a->absCap, 256, 256)), dim3(256), 0, s, p, a->nDev,
                        a->secOff, a->absCap, a->secCap, holePos, cap, a->stepDev, misc, a->stats);
     OECL_CHECK_LAUNCH("oecl_step_run");
@@ -571,6 +616,11 @@ OECL_EXPORT int oecl_step_run(const oecl_grid_t *g, const oecl_wall_t *wall, con
         !a->stats || !a->work || a->absCap == 0 || a->secCap == 0)
         return OECL_ERR_ARG;
     if (a->typeB == 2 && !a->bField) return OECL_ERR_ARG;
+    if (a->photoCount && (!a->photoWeight || !a->photoSlots || a->photoCap == 0)) return OECL_ERR_ARG;
+    if (a->photoCount && wall->kind != OECL_WALL_RECT && wall->kind != OECL_WALL_ELLIP) {
+        oecl_set_error_msg("photoelectrons: rectangular or elliptical wall only");
+        return OECL_ERR_ARG;
+    }
     WallDev wd;
     int rc = wall_to_device(wall, &wd);
     if (rc) return rc;

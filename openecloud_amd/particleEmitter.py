@@ -150,3 +150,72 @@ def homoLoader(gridObj, particlesObj, partBoundObj, macroParticleCount, weightIn
                           % int(failed.item()))
     p.macroParticleCount = n
     p._refCapacity = n
+
+
+class PhotoEmitter:
+    """Primary photoelectrons from the chamber wall while a bunch passes.
+
+    The reference has no photoelectron source (its Readme lists photoemission as an outlook);
+    this one is defined as follows.  During the step starting at t a fraction f(t) =
+    beam.bunchFraction(t) of a bunch passes, which frees photoYield * nParticles * f(t)
+    photoelectrons per metre.  They are carried by macro-particles at a rate of macroPerBunch
+    per bunch: the fractional macro-particle count and the charge are carried over from step
+    to step, so the total charge is exact and every macro-particle of a step has the same
+    weight.  Each photoelectron starts on the wall (inset by 1e-3 cell; rectangle uniform in
+    arc length, ellipse uniform in the parametric angle) with energy |energyMean +
+    energySigma z| eV and the reference emitter's secondary angles about the inward normal
+    (oecl_photo_emit).  Rectangular and elliptical walls only.
+    """
+
+    def __init__(self, partBoundObj, particlesObj, beamObj, photoYield=1e-3, macroPerBunch=10000,
+                 energyMean=7., energySigma=5.):
+        if partBoundObj.desc.kind not in (0, 1):
+            raise NotImplementedError('photoelectrons: rectangular or elliptical wall only')
+        self.pb = partBoundObj
+        self.p = particlesObj
+        self.beam = beamObj
+        self.photoYield = float(photoYield)
+        self.macroPerBunch = float(macroPerBunch)
+        self.energyMean = float(energyMean)
+        self.energySigma = float(energySigma)
+        self._counts = np.zeros(0, np.uint32)
+        self._weights = np.zeros(0, np.float64)
+        self._carry = (0., 0.)
+
+    def tables(self, nSteps):
+        """Macro-particle count and weight of the steps 0 .. nSteps-1 (numpy arrays)."""
+        k0 = self._counts.size
+        if nSteps > k0:
+            cnt = np.zeros(nSteps - k0, np.uint32)
+            wts = np.zeros(nSteps - k0, np.float64)
+            mAcc, qAcc = self._carry
+            dt = self.beam.dt
+            for k in range(k0, nSteps):
+                f = self.beam.bunchFraction(k * dt)
+                if f <= 0.:
+                    continue
+                mAcc += self.macroPerBunch * f
+                qAcc += self.photoYield * self.beam.nParticles * f
+                c = int(mAcc)
+                if c > 0:
+                    cnt[k - k0] = c
+                    wts[k - k0] = qAcc / c
+                    mAcc -= c
+                    qAcc = 0.
+            self._carry = (mAcc, qAcc)
+            self._counts = np.concatenate([self._counts, cnt])
+            self._weights = np.concatenate([self._weights, wts])
+        return self._counts[:nSteps], self._weights[:nSteps]
+
+    def generate(self, step):
+        """The photoelectrons of time step `step` as a device (n, 6) array."""
+        cnt, wts = self.tables(step + 1)
+        n = int(cnt[step])
+        out = torch.empty((n, 6), dtype=self.p.dtype, device=L.device())
+        if n:
+            k0, k1 = randomGen.key()
+            L.check(L.lib().oecl_photo_emit(
+                ctypes.byref(self.pb.desc), L.i32(self.p.code), L.ptr(out), L.u32(n), L.u32(step),
+                L.u32(k0), L.u32(k1), L.f64(self.energyMean), L.f64(self.energySigma),
+                L.f64(float(wts[step])), L.f64(self.p.particleMass), L.stream()), 'photoelectrons')
+        return out

@@ -10,7 +10,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, key, cap,
-                 rule='reference'):
+                 rule='reference', photo=None):
+    """photo: (counts, weights, e0, sigmaE) of a photoelectron source, added after each
+    step's secondaries (oracle.photo_oracle)."""
     from oracle.grid_oracle import OraclePoisson
     ops = OraclePoisson(g)
     Q = -1.602176565e-19
@@ -26,6 +28,7 @@ def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, 
         pd = orc.boris_push(pd, eat, dt, qm, typeB, bconst)
         ins, pd = orc.index_inside(pd, 'ellip', g.getLx(), g.getLy())
         ab = orc.save_absorbed(pd, ins)
+        sec = np.zeros((0, 6))
         if ab.shape[0]:
             moved, _, nv, _ = orc.interaction_and_normals(ab, 'ellip', g.getLx(), g.getLy(),
                                                           g.getDx(), g.getDy())
@@ -36,6 +39,12 @@ def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, 
                     orc.philox_words(na, 0, b, it, 1, key[0], key[1]))
             from oracle.furman_oracle import generate_secondaries
             sec, _, _ = generate_secondaries(moved, nv, U[:, :35], P)
+        if photo is not None and photo[0][it] > 0:
+            from oracle.photo_oracle import photo_emit
+            ph = photo_emit(1, g.getLx(), g.getLy(), g.getDx(), g.getDy(), int(photo[0][it]), it,
+                            key[0], key[1], photo[2], photo[3], float(photo[1][it]), 9.10938291e-31)
+            sec = np.concatenate([sec, ph])
+        if ab.shape[0] or sec.shape[0]:
             if rule == 'fast':
                 pd = orc.add_and_remove_fast(pd, sec, ins)
             else:
@@ -144,3 +153,62 @@ def test_native_step_matches_oracle(orc, dtype):
         assert idx.size > 1500
         scale = np.abs(ref[-1]).max(axis=0)
         assert (np.abs(got[idx] - ref[-1][idx]) / scale).max() < 1e-4
+
+
+@pytest.mark.parametrize('native', [False, True])
+def test_photoelectrons_in_the_step_match_oracle(orc, native):
+    """A bunch passage with the photoelectron source: the Python fused step and the one-call
+    native step against the oracle loop (photoelectrons after the secondaries, fast rule)."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter, PhotoEmitter
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.furman_oracle import material
+    g = Grid(64, 48, 0.09, 0.07, 1)
+    n = 20000
+    pd0 = random_particles(g, n, seed=13, wall='ellip', vscale=3e6)
+    pd0[:, 5] = 5e4
+    dt = 2e-11
+    randomGen.seed(78)
+    key = randomGen.key()
+    p = Particles('electrons', 'variableWeight', ordering='fast')
+    p.setParticleData(pd0)
+    p.setBConst([0., 0.5, 0.])
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.6, eHat0=300.)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    beam = GaussianBunchTrain(g, dt, nParticles=1e11, tBunchSpacing=25e-9, radiusSigma=2e-3,
+                              nBunches=1, bunchLengthSigma=0.003)
+    photo = PhotoEmitter(pb, p, beam, photoYield=1e-3, macroPerBunch=2000)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1, photoEmitter=photo)
+    nsteps = 8
+    cnt, wts = photo.tables(nsteps)
+    assert cnt.sum() > 1000
+    # the weights carry the exact photoelectron charge of the steps with a bunch
+    frac = np.array([beam.bunchFraction(k * dt) for k in range(nsteps)])
+    assert abs((cnt * wts).sum() - 1e-3 * 1e11 * frac.sum()) <= 1e-12 * (cnt * wts).sum() + \
+        1e-3 * 1e11 * max(photo._carry[1], 0.)
+    ref = _oracle_loop(orc, g, pd0.copy(), nsteps, dt, 1, np.array([0., 0.5, 0.]),
+                       beam.getTransversalProfile(), beam.getScale,
+                       material('copper', 1.6, eHat0=300.), key, n, rule='fast',
+                       photo=(cnt, wts, photo.energyMean, photo.energySigma))
+    if native:
+        b.native_setup(absCap=8192, secCap=16384, maxSteps=64)
+    for it in range(nsteps):
+        if native:
+            b.step_native()
+            st = b.sync()
+        else:
+            b.step()
+        got = p.getParticleData().cpu().numpy()
+        assert got.shape == ref[it].shape, (it, got.shape, ref[it].shape)
+        scale = np.abs(ref[it]).max(axis=0)
+        assert (np.abs(got - ref[it]) / scale).max() < 1e-11, it
+    if native:
+        assert st['photo_total'] == int(cnt.sum())
+    else:
+        assert sum(b.history['photo']) == int(cnt.sum())

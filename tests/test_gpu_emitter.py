@@ -113,3 +113,40 @@ def test_loader_matches_restatement(orc):
                   r1 * np.cos(2 * np.pi * b[1]) * vs, 1e4 * (0.5 + c[0])]
     assert np.array_equal(got[:, [0, 1, 5]], ref[:, [0, 1, 5]])
     assert np.allclose(got[:, 2:5], ref[:, 2:5], rtol=1e-13, atol=1e-8)
+
+
+@pytest.mark.parametrize('kind', ['rect', 'ellip'])
+@pytest.mark.parametrize('dtype', ['f64', 'f32'])
+def test_photo_emit_matches_oracle(kind, dtype):
+    """oecl_photo_emit against the photoelectron restatement with the same Philox draws."""
+    import ctypes
+    from openecloud_amd import _lib as L
+    from openecloud_amd import randomGen
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbElliptical, AbsorbRectangular
+    from openecloud_amd.particles import Particles
+    from oracle.photo_oracle import photo_emit
+    g = Grid(128, 96, 0.09, 0.07, 1 if kind == 'ellip' else 0)
+    tdt = torch.float64 if dtype == 'f64' else torch.float32
+    p = Particles('electrons', 'variableWeight', dtype=tdt)
+    p.setParticleData(np.zeros((1, 6)))
+    pb = (AbsorbElliptical if kind == 'ellip' else AbsorbRectangular)(g, p)
+    randomGen.seed(5)
+    k0, k1 = randomGen.key()
+    n, step = 30000, 17
+    out = torch.empty((n, 6), dtype=tdt, device='cuda')
+    L.check(L.lib().oecl_photo_emit(ctypes.byref(pb.desc), L.i32(p.code), L.ptr(out), L.u32(n),
+                                    L.u32(step), L.u32(k0), L.u32(k1), L.f64(7.), L.f64(5.),
+                                    L.f64(3e4), L.f64(9.10938291e-31), L.stream()))
+    got = out.cpu().numpy().astype(np.float64)
+    ref = photo_emit(1 if kind == 'ellip' else 0, g.getLx(), g.getLy(), g.getDx(), g.getDy(), n,
+                     step, k0, k1, 7., 5., 3e4, 9.10938291e-31)
+    tol = 1e-12 if dtype == 'f64' else 1e-6
+    scale = np.abs(ref).max(axis=0)
+    assert (np.abs(got - ref) / scale).max() < tol
+    # every photoelectron starts inside the chamber (the walls' inside tests)
+    x, y = got[:, 0], got[:, 1]
+    if kind == 'ellip':
+        assert np.all(x ** 2 * (4. / g.getLx() ** 2) + y ** 2 * (4. / g.getLy() ** 2) < 1.)
+    else:
+        assert np.all((np.abs(x) <= 0.5 * g.getLx()) & (np.abs(y) <= 0.5 * g.getLy()))
