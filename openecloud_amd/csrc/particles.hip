@@ -139,6 +139,7 @@ __global__ void __launch_bounds__(256) k_push_fused(oecl_grid_t g, WallDev wall,
                                                     uint8_t *__restrict__ isInside,
                                                     double *__restrict__ rhoNext)
 {
+    __shared__ double tiles[4][512];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < n;
     // FP32 storage is widened on load; the arithmetic runs in FP64 registers.
@@ -164,9 +165,9 @@ __global__ void __launch_bounds__(256) k_push_fused(oecl_grid_t g, WallDev wall,
             wq = (double)p(5, i);
         }
     }
-    if (rhoNext == nullptr) return;  // uniform across the block: no barrier is skipped
-    block_deposit<2048>(rhoNext, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q,
-                       w2.wLU * wq * q, w2.wUU * wq * q);
+    if (rhoNext == nullptr) return;  // uniform: every lane of a wave reaches the deposit
+    wave_deposit<512>(rhoNext, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
+                      w2.wUU * wq * q, tiles[threadIdx.x >> 6]);
 }
 
 // Moments of the particle set, FP64, fixed reduction order (bit-reproducible):

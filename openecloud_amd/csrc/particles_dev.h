@@ -47,81 +47,101 @@ __device__ __forceinline__ void deposit_corners(double *rho, const Weights<T> &w
     unsafeAtomicAdd(rho + w.cell + nx + 1, (double)w.wUU * wt * q);
 }
 
-// Sums each run of equal cells among consecutive lanes of the wave (an inclusive suffix
-// scan that stops at run ends, log2(64) shuffle steps); the first lane of a run receives
-// the run's totals and returns true.  Lanes with c == 0xFFFFFFFF never return true.  Atomics
-// to one address from many lanes of a wave are serialised by the hardware, so summing the
-// runs first is what makes the deposit of cell-sorted particles cheap.
+// ---- wave-level reductions through DPP (data-parallel primitives: VALU lane moves within
+// rows of 16 lanes, row_bcast across rows), which cost no LDS bandwidth.  All 64 lanes of
+// the wave must be active when these are called.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v, uint32_t ident)
+{
+    // lanes outside ROWMASK, or whose source lane is out of the row, receive `ident`
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)ident, (int)v, CTRL, ROWMASK, 0xF, false);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const uint32_t lo = (uint32_t)__double2loint(v), hi = (uint32_t)__double2hiint(v);
+    return __hiloint2double((int)dpp_u32<CTRL, ROWMASK>(hi, 0u), (int)dpp_u32<CTRL, ROWMASK>(lo, 0u));
+}
+
+// One step of a segmented inclusive scan: a lane that does not start a segment inside its
+// current range adds its predecessor's partial sums and inherits its start flag.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void seg_scan_step(uint32_t &f, double &q0, double &q1, double &q2, double &q3)
+{
+    const uint32_t fp = dpp_u32<CTRL, ROWMASK>(f, 0u);
+    const double p0 = dpp_f64<CTRL, ROWMASK>(q0), p1 = dpp_f64<CTRL, ROWMASK>(q1);
+    const double p2 = dpp_f64<CTRL, ROWMASK>(q2), p3 = dpp_f64<CTRL, ROWMASK>(q3);
+    if (!f) {
+        q0 += p0;
+        q1 += p1;
+        q2 += p2;
+        q3 += p3;
+        f = fp;
+    }
+}
+
+// Sums each run of equal cells among consecutive lanes of the wave: a segmented prefix scan
+// (row_shr 1, 2, 4, 8 within rows, then row_bcast 15 and 31 across rows).  The LAST lane of a
+// run receives the run's totals and returns true; lanes with c == 0xFFFFFFFF never do.
+// Atomics to one address from many lanes of a wave are serialised by the hardware, so summing
+// the runs first is what makes the deposit of cell-sorted particles cheap.
 __device__ __forceinline__ bool wave_run_sum(uint32_t c, double &q0, double &q1, double &q2, double &q3)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t cn = __shfl_down(c, 1, 64), cp = __shfl_up(c, 1, 64);
-    bool tail = lane == 63 || cn != c;
-    const bool head = lane == 0 || cp != c;
-    // After the step with offset d, a lane holds the sum over [lane, min(lane + 2d - 1, run end)]
-    // and tail says whether that range reaches the run end; a lane that has not reached it
-    // lies at least d below the run end, so lane + d is in the same run and in range.
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const double o0 = __shfl_down(q0, d, 64), o1 = __shfl_down(q1, d, 64);
-        const double o2 = __shfl_down(q2, d, 64), o3 = __shfl_down(q3, d, 64);
-        const bool ot = __shfl_down((int)tail, d, 64) != 0;
-        if (!tail) {
-            q0 += o0;
-            q1 += o1;
-            q2 += o2;
-            q3 += o3;
-            tail = ot;
-        }
-    }
-    return head && c != 0xFFFFFFFFu;
+    const uint32_t cp = __shfl_up(c, 1, 64), cn = __shfl_down(c, 1, 64);
+    uint32_t f = (lane == 0 || cp != c) ? 1u : 0u;
+    const bool last = lane == 63 || cn != c;
+    seg_scan_step<0x111, 0xF>(f, q0, q1, q2, q3);  // row_shr:1
+    seg_scan_step<0x112, 0xF>(f, q0, q1, q2, q3);  // row_shr:2
+    seg_scan_step<0x114, 0xF>(f, q0, q1, q2, q3);  // row_shr:4
+    seg_scan_step<0x118, 0xF>(f, q0, q1, q2, q3);  // row_shr:8
+    seg_scan_step<0x142, 0xA>(f, q0, q1, q2, q3);  // row_bcast:15 into rows 1 and 3
+    seg_scan_step<0x143, 0xC>(f, q0, q1, q2, q3);  // row_bcast:31 into rows 2 and 3
+    return last && c != 0xFFFFFFFFu;
 }
 
-// Block-cooperative deposit for 256-thread blocks; every thread of the block must call it
-// (valid = false when the thread has no particle), with the particle's cell (ix, iy) and
-// its four corner charges.  Runs of equal cells within a wave are summed first
-// (wave_run_sum).  The run totals are then summed in an LDS tile covering the bounding box
-// of the block's cells (nodes [X0, X1+1] x [Y0, Y1+1]), and each touched node of the tile
-// is added to rho with one global atomic.  Particles sorted by cell, or only locally in
-// order (a few cells of drift since the last sort), keep the box small; a block whose box
-// exceeds WN nodes adds its run totals to rho directly.
-template <int WN>
-__device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t nx, bool valid, uint32_t ix,
-                                              uint32_t iy, double q00, double q10, double q01, double q11)
+// Minimum (MIN) or maximum over the wave, uniform result.
+template <bool MIN>
+__device__ __forceinline__ uint32_t wave_minmax(uint32_t v)
 {
-    __shared__ double win[WN];
-    __shared__ uint32_t bb[4][4];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t id = MIN ? 0xFFFFFFFFu : 0u;
+    uint32_t o;
+    o = dpp_u32<0x111, 0xF>(v, id);
+    v = MIN ? min(v, o) : max(v, o);
+    o = dpp_u32<0x112, 0xF>(v, id);
+    v = MIN ? min(v, o) : max(v, o);
+    o = dpp_u32<0x114, 0xF>(v, id);
+    v = MIN ? min(v, o) : max(v, o);
+    o = dpp_u32<0x118, 0xF>(v, id);
+    v = MIN ? min(v, o) : max(v, o);
+    const uint32_t a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
+    const uint32_t c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
+    return MIN ? min(min(a, b), min(c, d)) : max(max(a, b), max(c, d));
+}
+
+// Wave-cooperative deposit (all 64 lanes call it; valid = false when the lane has no
+// particle), with the particle's cell (ix, iy) and its four corner charges.  Runs of equal
+// cells are summed first (wave_run_sum).  The run totals are then summed in the wave's own
+// LDS tile `win` covering the bounding box of the wave's cells (nodes [X0, X1+1] x
+// [Y0, Y1+1]), and each touched node is added to rho with one global FP64 atomic.  Particles
+// sorted by cell, or only locally in order (a few cells of drift since the last sort), keep
+// the box small; a wave whose box exceeds WN nodes adds its run totals to rho directly.  No
+// block barrier is involved: LDS accesses of one wave are ordered.
+template <int WN>
+__device__ __forceinline__ void wave_deposit(double *__restrict__ rho, uint32_t nx, bool valid, uint32_t ix,
+                                             uint32_t iy, double q00, double q10, double q01, double q11,
+                                             double *win)
+{
+    const int lane = threadIdx.x & 63;
     const uint32_t c = valid ? ix + iy * nx : 0xFFFFFFFFu;
     if (!valid) q00 = q10 = q01 = q11 = 0.;
     const bool head = wave_run_sum(c, q00, q10, q01, q11);
-    uint32_t x0 = head ? ix : 0xFFFFFFFFu, x1 = head ? ix : 0u;
-    uint32_t y0 = head ? iy : 0xFFFFFFFFu, y1 = head ? iy : 0u;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        x0 = min(x0, (uint32_t)__shfl_xor(x0, d, 64));
-        x1 = max(x1, (uint32_t)__shfl_xor(x1, d, 64));
-        y0 = min(y0, (uint32_t)__shfl_xor(y0, d, 64));
-        y1 = max(y1, (uint32_t)__shfl_xor(y1, d, 64));
-    }
-    if (lane == 0) {
-        bb[wave][0] = x0;
-        bb[wave][1] = x1;
-        bb[wave][2] = y0;
-        bb[wave][3] = y1;
-    }
-    __syncthreads();
-    uint32_t X0 = bb[0][0], X1 = bb[0][1], Y0 = bb[0][2], Y1 = bb[0][3];
-#pragma unroll
-    for (int w = 1; w < 4; ++w) {
-        X0 = min(X0, bb[w][0]);
-        X1 = max(X1, bb[w][1]);
-        Y0 = min(Y0, bb[w][2]);
-        Y1 = max(Y1, bb[w][3]);
-    }
-    if (X0 == 0xFFFFFFFFu) return;  // no particle to deposit in this block (block-uniform)
+    const uint32_t X0 = wave_minmax<true>(head ? ix : 0xFFFFFFFFu), X1 = wave_minmax<false>(head ? ix : 0u);
+    const uint32_t Y0 = wave_minmax<true>(head ? iy : 0xFFFFFFFFu), Y1 = wave_minmax<false>(head ? iy : 0u);
+    if (X0 == 0xFFFFFFFFu) return;  // nothing to deposit in this wave (wave-uniform)
     const uint32_t wx = X1 - X0 + 2, wy = Y1 - Y0 + 2;
-    if ((uint64_t)wx * wy > WN) {   // block-uniform
+    if ((uint64_t)wx * wy > WN) {  // wave-uniform
         if (head) {
             const uint32_t cc = ix + iy * nx;
             unsafeAtomicAdd(rho + cc, q00);
@@ -132,8 +152,9 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
         return;
     }
     const uint32_t nw = wx * wy;
-    for (uint32_t k = tid; k < nw; k += 256) win[k] = 0.;
-    __syncthreads();
+    for (uint32_t k = lane; k < nw; k += 64) win[k] = 0.;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     if (head) {
         const uint32_t o = (ix - X0) + (iy - Y0) * wx;
         atomicAdd(&win[o], q00);
@@ -141,8 +162,9 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
         atomicAdd(&win[o + wx], q01);
         atomicAdd(&win[o + wx + 1], q11);
     }
-    __syncthreads();
-    for (uint32_t k = tid; k < nw; k += 256) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = lane; k < nw; k += 64) {
         const double v = win[k];
         if (v != 0.) {
             const uint32_t r = k / wx;

@@ -36,7 +36,7 @@ namespace {
 
 constexpr uint32_t PB = 256;      // particles per push block
 constexpr uint32_t CT = 1024;     // threads of the single-block kernels
-constexpr uint32_t NZMAX = 16384; // push blocks with holes sorted in LDS (more: full scan)
+constexpr uint32_t NZMAX = 16384; // push waves with holes sorted in LDS (more: full scan)
 constexpr uint32_t MAXSEC = 10;   // secondaries per impact (Furman M)
 // step scratch counters
 enum { M_HOLES = 0, M_NZ = 1, M_DONE = 2, M_PHOTO = 3, M_PHOTO_PLACED = 4, M_COUNT = 16 };
@@ -65,29 +65,21 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
     rho[i] = 0.;
 }
 
-// Holes (absorbed particles) of one push block in index order: the block's count goes to
-// counts[block], the indices to list[block * PB + rank], and a block with holes appends its
-// index to nzList (in no particular order).  Every thread of the block calls it.
-__device__ __forceinline__ void block_holes(bool dead, uint32_t i, uint32_t *__restrict__ counts,
-                                            uint32_t *__restrict__ list, uint32_t *__restrict__ misc,
-                                            uint32_t *__restrict__ nzList)
+// Holes (absorbed particles) of one wave in index order: the wave's count goes to
+// counts[w], the indices to list[w * 64 + rank] (w = global wave index), and a wave with holes
+// appends w to nzList (in no particular order).  All 64 lanes of the wave call it.
+__device__ __forceinline__ void wave_holes(bool dead, uint32_t i, uint32_t *__restrict__ counts,
+                                           uint32_t *__restrict__ list, uint32_t *__restrict__ misc,
+                                           uint32_t *__restrict__ nzList)
 {
-    __shared__ uint32_t wc[PB / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = blockIdx.x * (PB / 64) + (threadIdx.x >> 6);
     const uint64_t bal = __ballot(dead);
-    if (lane == 0) wc[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t off = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < (int)(PB / 64); ++w) {
-        const uint32_t c = wc[w];
-        if (w < wave) off += c;
-        tot += c;
-    }
-    if (dead) list[(size_t)blockIdx.x * PB + off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
-    if (threadIdx.x == 0) {
-        counts[blockIdx.x] = tot;
-        if (tot) nzList[atomicAdd(misc + M_NZ, 1u)] = blockIdx.x;
+    if (dead) list[(size_t)w * 64 + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
+    if (lane == 0) {
+        const uint32_t tot = (uint32_t)__popcll(bal);
+        counts[w] = tot;
+        if (tot) nzList[atomicAdd(misc + M_NZ, 1u)] = w;
     }
 }
 
@@ -99,8 +91,9 @@ __global__ void __launch_bounds__(PB)
                double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
                uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
 {
+    __shared__ double tiles[PB / 64][512];
     const uint32_t n = *nDev;
-    if (blockIdx.x * PB >= n) return;  // whole block: no barrier below is skipped by part of it
+    if (blockIdx.x * PB >= n) return;  // whole block
     const uint32_t i = blockIdx.x * PB + threadIdx.x;
     const bool live = i < n;
     // FP32 storage is widened on load: weights, gather and the Boris rotation run in FP64
@@ -129,9 +122,9 @@ __global__ void __launch_bounds__(PB)
             dead = true;
         }
     }
-    block_holes(dead, i, counts, list, misc, nzList);
-    block_deposit<2048>(rho, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
-                       w2.wUU * wq * q);
+    wave_holes(dead, i, counts, list, misc, nzList);
+    wave_deposit<512>(rho, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
+                      w2.wUU * wq * q, tiles[threadIdx.x >> 6]);
 }
 
 // Exclusive prefix of v over the CT threads of the block (wsum: CT/64 LDS words); *tot the sum.
@@ -159,9 +152,9 @@ __device__ __forceinline__ uint32_t block_scan_ct(uint32_t v, uint32_t *wsum, ui
 }
 
 // One block: all holes in ascending index order (holePos, the first absCap of them), their
-// count misc[M_HOLES], and the absorbed rows (hole r -> row r).  The push blocks that have
+// count misc[M_HOLES], and the absorbed rows (hole r -> row r).  The push waves that have
 // holes (nzList, usually a few dozen) are sorted in LDS and their lists concatenated in
-// block order; beyond NZMAX such blocks every block's count is scanned instead.
+// wave order; beyond NZMAX such waves every wave's count is scanned instead.
 template <typename T>
 __global__ void __launch_bounds__(CT)
     k_holes_collect(Soa<T> p, const uint32_t *__restrict__ nDev, const uint32_t *__restrict__ counts,
@@ -171,7 +164,7 @@ __global__ void __launch_bounds__(CT)
 {
     __shared__ uint32_t keys[NZMAX];
     __shared__ uint32_t wsum[CT / 64];
-    const uint32_t n = *nDev, nb = (n + PB - 1) / PB, nz = misc[M_NZ];
+    const uint32_t n = *nDev, nb = (n + 63) / 64, nz = misc[M_NZ];
     const uint32_t tid = threadIdx.x;
     const bool sorted = nz <= NZMAX;
     uint32_t N = 1;
@@ -205,7 +198,7 @@ __global__ void __launch_bounds__(CT)
         const uint32_t start = carry + block_scan_ct(c, wsum, &tot);
         for (uint32_t r = 0; r < c; ++r) {
             const uint32_t dst = start + r;
-            if (dst < absCap) holePos[dst] = list[(size_t)blk * PB + r];
+            if (dst < absCap) holePos[dst] = list[(size_t)blk * 64 + r];
         }
         carry += tot;
     }
@@ -539,12 +532,12 @@ unsigned grid_cap(uint64_t items, unsigned block, unsigned maxBlocks)
 
 }  // namespace
 
-// Work buffer of oecl_step_run: per push block hole counts and lists, the list of blocks
-// with holes, the ascending hole list, counters, and MAXSEC secondary slots per hit.
+// Work buffer of oecl_step_run: per push wave hole counts and lists, the list of waves with
+// holes, the ascending hole list, counters, and MAXSEC secondary slots per hit.
 static uint64_t slots_offset(uint64_t capacity, uint32_t absCap)
 {
-    const uint64_t nBlk = (capacity + PB - 1) / PB;
-    return ((4 * (nBlk * (PB + 2) + absCap + M_COUNT)) + 255) / 256 * 256;
+    const uint64_t nW = (capacity + PB - 1) / PB * (PB / 64);
+    return ((4 * (nW * (64 + 2) + absCap + M_COUNT)) + 255) / 256 * 256;
 }
 
 OECL_EXPORT uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap)
@@ -559,11 +552,11 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     const uint64_t cap = a->stride;
     const uint32_t np = g->np;
     Soa<T> p{(T *)a->data, a->stride};
-    const uint64_t nBlk = (cap + PB - 1) / PB;
+    const uint64_t nBlk = (cap + PB - 1) / PB, nW = nBlk * (PB / 64);
     uint32_t *counts = (uint32_t *)a->work;
-    uint32_t *list = counts + nBlk;
-    uint32_t *nzList = list + nBlk * PB;
-    uint32_t *holePos = nzList + nBlk;
+    uint32_t *list = counts + nW;
+    uint32_t *nzList = list + nW * 64;
+    uint32_t *holePos = nzList + nW;
     uint32_t *misc = holePos + a->absCap;
     T *slots = (T *)((char *)a->work + slots_offset(cap, a->absCap));
     hipLaunchKernelGGL(k_begin_dev, dim3(oecl_blocks(np, 256)), dim3(256), 0, s, np, a->rho, a->beamScale,
