@@ -320,6 +320,32 @@ class Buildup:
                                       ctypes.byref(self._args), L.stream()), 'step')
         self.it += 1
 
+    def capture_native(self, nSteps):
+        """Captures nSteps native steps into one HIP graph (torch.cuda.CUDAGraph).  Every
+        replay() runs the next nSteps steps: the step index lives on the device.  The graph
+        holds the current device pointers, so sort_particles(keepBuffers=True) must be used
+        while the graph is in use.  Event timing is switched off inside the graph."""
+        self._args.evPushBegin = None
+        self._args.evPushEnd = None
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        it0 = self.it
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(int(nSteps)):
+                self.step_native()
+        torch.cuda.current_stream().wait_stream(s)
+        self.it = it0   # capturing runs nothing
+        self._graph, self._graphSteps = g, int(nSteps)
+        return g
+
+    def replay(self):
+        """Runs the captured steps (capture_native)."""
+        if self.it - self._native_it0 + self._graphSteps > self.maxSteps:
+            raise L.OeclError('native step: maxSteps exhausted; call native_setup again')
+        self._graph.replay()
+        self.it += self._graphSteps
+
     def sync(self):
         """Reads the device counts back, updates the Particles object and checks the status
         word (raises on any overflow)."""
@@ -337,9 +363,10 @@ class Buildup:
         return {'absorbed_total': st[0], 'emitted_total': st[1], 'not_found': st[3],
                 'absorbed_last': st[4], 'added_last': st[5], 'photo_total': st[6]}
 
-    def sort_particles(self):
+    def sort_particles(self, keepBuffers=False):
         """Orders the particles by grid cell (device radix sort + permutation).  Needs the host
-        count to be current (call sync() first after native steps)."""
+        count to be current (call sync() first after native steps).  keepBuffers: copy the
+        sorted block back so the particle storage keeps its address (captured graphs)."""
         p = self.p
         n = p.macroParticleCount
         if n < 2:
@@ -355,15 +382,16 @@ class Buildup:
             ctypes.byref(self.gd), L.i32(p.code), L.ptr(p._data), L.ptr(spare), L.u64(p.capacity),
             L.u32(n), L.ptr(self._sortkeys), L.ptr(self._sortperm), L.ptr(self._sortwork),
             L.stream()), 'sort by cell')
-        p._data, p._spare = spare, p._data
-        if getattr(self, '_args', None) is not None:
-            self._args.data = p._data.data_ptr()
+        if keepBuffers:
+            p._data[:, :n].copy_(spare[:, :n])
+        else:
+            p._data, p._spare = spare, p._data
+            if getattr(self, '_args', None) is not None:
+                self._args.data = p._data.data_ptr()
 
     def counts(self, first, last):
         """Particle counts at the start of steps first .. last-1 (native steps)."""
-        a = first - self._native_it0
-        b = last - self._native_it0
-        return self._hist[a:b].cpu().tolist()
+        return self._hist[first:last].cpu().tolist()
 
     def _record(self, na, nsec, nph=0):
         h = self.history
