@@ -33,6 +33,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include "common.h"
+#include "fastpoisson_internal.h"
 
 struct oecl_fastpoisson {
     uint32_t nx, ny, np;      // extended grid
@@ -602,14 +603,24 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     return OECL_OK;
 }
 
+FpBox fastpoisson_box(const oecl_fastpoisson *h)
+{
+    return FpBox{h->b, h->mask, h->nx, h->ny, h->bx};
+}
+
 OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, double *phi, void *stream)
 {
     if (!h || !q || !phi) return OECL_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
-    const int bx = h->bx, by = h->by, kx = (bx + 1) / 2;
-    const dim3 gg((kx + GT - 1) / GT, (by + GT - 1) / GT, 2), gb(GNT);
     hipLaunchKernelGGL(k_gather_box_zero, dim3(oecl_blocks(h->np, 256)), dim3(256), 0, s, h->nx, h->ny, h->bx,
                        h->mask, q, h->b, phi);
+    return fastpoisson_solve_box(h, phi, s);
+}
+
+int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, hipStream_t s)
+{
+    const int bx = h->bx, by = h->by, kx = (bx + 1) / 2;
+    const dim3 gg((kx + GT - 1) / GT, (by + GT - 1) / GT, 2), gb(GNT);
     // hat b = Sx b
     hipLaunchKernelGGL(k_dst_gemm<0>, gg, gb, 0, s, by, bx, h->Fx, h->b, bx, h->t1, bx, 0u, nullptr);
     if (h->m) {

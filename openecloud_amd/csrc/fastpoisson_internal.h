@@ -1,0 +1,18 @@
+// Library-internal access to the direct Poisson solver (not part of the C ABI): the one-call
+// step writes the masked right-hand side straight into the solver's box vector.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct oecl_fastpoisson;
+
+struct FpBox {
+    double *b;             // bx * by, column-major box right-hand side (x fastest)
+    const uint8_t *mask;   // inside flags of the extended grid
+    uint32_t nx, ny, bx;   // extended grid, box width (nx - 4)
+};
+
+FpBox fastpoisson_box(const oecl_fastpoisson *h);
+// Solve with b already holding q on inside box nodes and 0 elsewhere in the box; phi is
+// written on the box nodes only (nodes outside the box must be zero already).
+int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, hipStream_t s);

@@ -149,4 +149,79 @@ __device__ inline void pass2(const oecl_furman_t &P, const T *a, const T *nv, ui
     }
 }
 
+
+// Second loop with the secondaries of one impact spread over a group of MAXSEC lanes of one
+// wave: lane ii of the group draws the angle thetaK[ii] (ii < ns - 1) or the radius y
+// (ii = ns - 1) at the same time as the other lanes, the group exchanges them by shuffles,
+// and lane ii assembles secondary ii with exactly the arithmetic of pass2 (same values, same
+// order).  All 64 lanes of the wave call it; lanes without an impact pass ns = 0.  `base` is
+// the wave lane of the group's first lane; o points to this lane's output row (or nullptr).
+template <typename T, typename R>
+__device__ inline void pass2_lane(const oecl_furman_t &P, const T *a, const T *nv, uint32_t ns, double e0,
+                                  const double *last, const R &rng, uint32_t ii, int base, T *o)
+{
+    double th = 0., yv = 0., eS = 0.;
+    double en = 0.;
+    if (ns == 1) {
+        if (ii == 0) {
+            double aE = last[1] / last[3];
+            double aR = last[2] / last[3];
+            double u = rng.u(1);
+            if (u < aE) {
+                double z = sqrt(-2. * log(1. - rng.u(2))) * cos(2. * kPi * rng.u(3));
+                if (P.scaleSigmaE == 0) eS = e0 - fabs(P.sigmaE * z);
+                else eS = e0 - fabs(P.sigmaE * sqrt(e0 / 300) * z);
+                if (eS < 0.) eS = rng.u(4) * e0;
+            } else if (u < aE + aR) {
+                eS = e0 * pow(rng.u(4), 1. / (1. + P.q));
+            } else {
+                double ap = P.pSmallN[0];
+                eS = P.epsN[0] * sf::gammaincinv(ap, rng.u(4) * sf::gammainc(ap, e0 / P.epsN[0]));
+            }
+        }
+    } else if (ns >= 2) {
+        double pn = P.pSmallN[ns - 1];
+        en = P.epsN[ns - 1];
+        if (ii < ns - 1) {
+            th = asin(sqrt(sf::betaincinv(pn * (ns - ii - 1), pn, rng.u(5 + ii))));
+        } else if (ii == ns - 1) {
+            double p0 = sf::gammainc(ns * pn, e0 / en);
+            yv = sqrt(sf::gammaincinv(ns * pn, rng.u(14) * p0));
+        }
+    }
+    const double y = __shfl(yv, base + (ns >= 2 ? (int)ns - 1 : 0), 64);
+    double thK[9];
+#pragma unroll
+    for (int kk = 0; kk < 9; ++kk) thK[kk] = __shfl(th, base + kk, 64);
+    if (ns >= 2 && ii < ns) {
+        double yk;
+        if (ii == 0) {
+            yk = y * cos(thK[0]);
+        } else if (ii == ns - 1) {
+            yk = y;
+            for (uint32_t kk = 0; kk < ns - 1; ++kk) yk *= sin(thK[kk]);
+        } else {
+            yk = y * cos(thK[ii]);
+            for (uint32_t kk = 0; kk < ii; ++kk) yk *= sin(thK[kk]);
+        }
+        eS = en * (yk * yk);
+    }
+    if (ii >= ns || !o) return;
+    double eFactorInv = 1. / (0.5 * P.particleMass / kQe);
+    double n0x = nv[0], n0y = nv[1];
+    double vSec = sqrt(eFactorInv * eS);
+    double thv = asin(rng.u(15 + 2 * ii));
+    double ct = cos(thv), st = sin(thv);
+    double ph = rng.u(16 + 2 * ii) * kPi * 2.;
+    double cp = cos(ph), sp = sin(ph);
+    double t1 = ct * n0x - st * n0y;
+    double t2 = st * n0x + ct * n0y;
+    o[0] = a[0];
+    o[1] = a[1];
+    o[5] = a[5];
+    o[2] = (T)(vSec * ((cp + n0x * n0x * (1 - cp)) * t1 + n0x * n0y * (1 - cp) * t2));
+    o[3] = (T)(vSec * ((cp + n0y * n0y * (1 - cp)) * t2 + n0x * n0y * (1 - cp) * t1));
+    o[4] = (T)(vSec * (-sp * n0y * t1 + sp * n0x * t2));
+}
+
 }  // namespace furman

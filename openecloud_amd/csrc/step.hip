@@ -19,6 +19,7 @@
 #include <rocblas/rocblas.h>
 
 #include "common.h"
+#include "fastpoisson_internal.h"
 #include "furman_dev.h"
 #include "photo_dev.h"
 #include "particles_dev.h"
@@ -39,30 +40,47 @@ constexpr uint32_t CT = 1024;     // threads of the single-block kernels
 constexpr uint32_t NZMAX = 16384; // push waves with holes sorted in LDS (more: full scan)
 constexpr uint32_t MAXSEC = 10;   // secondaries per impact (Furman M)
 // step scratch counters
-enum { M_HOLES = 0, M_NZ = 1, M_DONE = 2, M_PHOTO = 3, M_PHOTO_PLACED = 4, M_COUNT = 16 };
+enum { M_HOLES = 0, M_NZ = 1, M_PHOTO = 2, M_COUNT = 16 };
 
-// q = rho + s(step) * beam profile (example01_basic.py:81), then rho = 0 for the deposit of
-// this step; thread 0 records the particle count of the step.
+// Start of a step, one thread per node:
+//   the Poisson right-hand side q = rho + s(step) * beam profile (example01_basic.py:81),
+//   masked and written straight into the direct solver's box vector (inside nodes keep q,
+//   other box nodes 0; the potential outside the box is set to 0);
+//   rho = 0 for this step's deposit; thread 0 records the particle count of the step;
+//   the step's photoelectrons (a per-step table) are generated into their slots.
+template <typename T>
 __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double *__restrict__ table,
                             uint32_t nScale, const uint32_t *__restrict__ stepDev,
-                            const double *__restrict__ prof, double *__restrict__ q,
+                            const double *__restrict__ prof, FpBox box, double *__restrict__ phi,
                             const uint32_t *__restrict__ nDev, uint32_t *__restrict__ hist, uint32_t histCap,
-                            uint32_t *__restrict__ misc)
+                            uint32_t *__restrict__ misc, WallDev w, const uint32_t *__restrict__ photoCount,
+                            const double *__restrict__ photoWeight, uint32_t nPhoto, uint32_t photoCap,
+                            uint32_t s0, uint32_t s1, double e0, double sigmaE, double mass,
+                            T *__restrict__ photoSlots, uint32_t *stats)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t s = *stepDev;
+    uint32_t nph = 0;
+    if (photoCount && s < nPhoto) {
+        nph = photoCount[s];
+        if (nph > photoCap) nph = photoCap;
+    }
     if (i == 0) {
         if (hist && s < histCap) hist[s] = *nDev;
         misc[M_NZ] = 0;
-        misc[M_DONE] = 0;
-        misc[M_PHOTO] = 0;
-        misc[M_PHOTO_PLACED] = 0;
+        misc[M_PHOTO] = nph;
+        if (photoCount && s < nPhoto && photoCount[s] > photoCap) atomicOr(stats + 2, (uint32_t)ST_PHOTO_OVERFLOW);
     }
+    for (uint32_t k = i; k < nph; k += gridDim.x * blockDim.x)
+        photo::emit_one<T>(w, k, s, s0, s1, e0, sigmaE, photoWeight[s], mass, photoSlots + 6 * (uint64_t)k);
     if (i >= np) return;
     const double sc = (prof && table && s < nScale) ? table[s] : 0.;
     const double r = rho[i];
-    q[i] = sc != 0. ? r + sc * prof[i] : r;
+    const double qv = sc != 0. ? r + sc * prof[i] : r;
     rho[i] = 0.;
+    const uint32_t ix = i % box.nx, iy = i / box.nx;
+    if (ix < 2 || iy < 2 || ix >= box.nx - 2 || iy >= box.ny - 2) phi[i] = 0.;
+    else box.b[(ix - 2) + (iy - 2) * box.bx] = box.mask[i] ? qv : 0.;
 }
 
 // Holes (absorbed particles) of one wave in index order: the wave's count goes to
@@ -351,44 +369,31 @@ __global__ void __launch_bounds__(128)
         hit_one<T>(jj, w, P, ap, nv, rts, dt, emit, s0, s1, step, nSec, impact, stats);
 }
 
-// Second Furman loop, grid-stride over the hits with ns > 0, into the hit's MAXSEC slots of
-// `slots` (impact-major; k_place_dev makes the compact order).  The single-secondary branch
-// takes the LAST hit's first-loop values, as the reference does.
+// Second Furman loop into the hit's MAXSEC slots of `slots` (impact-major; k_place_dev makes
+// the compact order).  Each wave takes 6 impacts at a time, one group of MAXSEC lanes per
+// impact (furman::pass2_lane), grid-stride over the hits.  The single-secondary branch takes
+// the LAST hit's first-loop values, as the reference does.
 template <typename T>
 __global__ void __launch_bounds__(64)
     k_emit_dev(oecl_furman_t P, const T *__restrict__ ap, const T *__restrict__ nv, uint32_t absCap,
                const uint32_t *__restrict__ nSec, const double *__restrict__ impact, uint32_t s0, uint32_t s1,
                const uint32_t *__restrict__ stepDev, T *__restrict__ slots, const uint32_t *misc)
 {
+    constexpr uint32_t GPW = 64 / MAXSEC;  // impact groups per wave
     const uint32_t na = n_hits(misc, absCap), step = *stepDev;
-    for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x) {
-        const uint32_t ns = nSec[jj];
-        if (ns == 0) continue;
-        furman::Rng rng{nullptr, jj, step, s0, s1};
-        T *o = slots + 6 * (uint64_t)MAXSEC * jj;
-        furman::pass2<T>(P, ap + 6 * (uint64_t)jj, nv + 2 * (uint64_t)jj, ns, impact[4 * (uint64_t)jj],
-                         impact + 4 * (uint64_t)(na - 1), rng,
-                         [&](uint32_t ii) -> T * { return o + 6 * (uint64_t)ii; });
+    const uint32_t lane = threadIdx.x & 63, grp = lane / MAXSEC, ii = lane % MAXSEC;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nWaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t j0 = wave * GPW; j0 < na; j0 += nWaves * GPW) {  // wave-uniform bound
+        const uint32_t jj = j0 + grp;
+        const bool ok = grp < GPW && jj < na;
+        const uint32_t ns = ok ? nSec[jj] : 0;
+        const uint32_t jr = ok ? jj : 0;
+        furman::Rng rng{nullptr, jr, step, s0, s1};
+        T *o = ok && ii < ns ? slots + 6 * ((uint64_t)MAXSEC * jj + ii) : nullptr;
+        furman::pass2_lane<T>(P, ap + 6 * (uint64_t)jr, nv + 2 * (uint64_t)jr, ns, ok ? impact[4 * (uint64_t)jj] : 0.,
+                              impact + 4 * (uint64_t)(na - 1), rng, ii, (int)(grp * MAXSEC), o);
     }
-}
-
-// Photoelectrons of this step into their slots (count from the per-step table).
-template <typename T>
-__global__ void __launch_bounds__(128)
-    k_photo_dev(WallDev w, const uint32_t *__restrict__ count, const double *__restrict__ weight, uint32_t nPhoto,
-                uint32_t photoCap, const uint32_t *__restrict__ stepDev, uint32_t s0, uint32_t s1, double e0,
-                double sigmaE, double mass, T *__restrict__ slots, uint32_t *misc, uint32_t *stats)
-{
-    const uint32_t step = *stepDev;
-    uint32_t n = step < nPhoto ? count[step] : 0;
-    if (n > photoCap) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(stats + 2, (uint32_t)ST_PHOTO_OVERFLOW);
-        n = photoCap;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) misc[M_PHOTO] = n;
-    const double wt = n ? weight[step] : 0.;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        photo::emit_one<T>(w, k, step, s0, s1, e0, sigmaE, wt, mass, slots + 6 * (uint64_t)k);
 }
 
 // One block: secOff = exclusive prefix of nSec over the hits (total at secOff[absCap]);
@@ -400,9 +405,9 @@ template <typename T>
 __global__ void __launch_bounds__(CT)
     k_place_dev(oecl_grid_t g, Soa<T> p, const T *__restrict__ slots, const T *__restrict__ photoSlots,
                 T *__restrict__ secOut, uint32_t secCap, const uint32_t *__restrict__ nSec,
-                uint32_t *__restrict__ secOff, uint32_t absCap, const uint32_t *__restrict__ nDev,
+                uint32_t *__restrict__ secOff, uint32_t absCap, uint32_t *__restrict__ nDev,
                 const uint32_t *__restrict__ holePos, uint64_t capacity, double q, double *rho, uint32_t *misc,
-                uint32_t *stats)
+                uint32_t *stepDev, uint32_t *stats)
 {
     __shared__ uint32_t wsum[CT / 64];
     const uint32_t na = n_hits(misc, absCap), h = misc[M_HOLES], n = *nDev;
@@ -417,10 +422,7 @@ __global__ void __launch_bounds__(CT)
         carry += tot;
     }
     const uint32_t nPh = misc[M_PHOTO];
-    if (tid == 0) {
-        secOff[absCap] = carry + nPh;
-        misc[M_PHOTO_PLACED] = carry >= secCap ? 0u : min(nPh, secCap - carry);
-    }
+    if (tid == 0) secOff[absCap] = carry + nPh;
     __syncthreads();
     // secondary s (or photoelectron s - carry): deposit, copy out, place
     auto place = [&](uint32_t s, const T *r) {
@@ -457,30 +459,12 @@ __global__ void __launch_bounds__(CT)
         }
         place(s, photoSlots + 6 * (uint64_t)k);
     }
-}
-
-__device__ __forceinline__ uint32_t n_added(const uint32_t *secOff, uint32_t absCap, uint32_t secCap)
-{
-    const uint32_t t = secOff[absCap];
-    return t < secCap ? t : secCap;
-}
-
-// Holes left below the new count newN = n - h + nAdd are filled from the tail: the t-th
-// position from the end (i = n-1-t >= newN), when it is not a hole, carries the j-th last
-// survivor, j = t - (holes above i), into holePos[nAdd + j] (holes above i counted by a
-// binary search in the ascending hole list).  The last block to finish then updates the
-// counts and the statistics (no block waits for another).
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_tail_end_dev(Soa<T> p, uint32_t *__restrict__ nDev, const uint32_t *__restrict__ secOff, uint32_t absCap,
-                   uint32_t secCap, const uint32_t *__restrict__ holePos, uint64_t capacity,
-                   uint32_t *__restrict__ stepDev, uint32_t *misc, uint32_t *stats)
-{
-    __shared__ bool lastBlock;
-    const uint32_t n = *nDev, h = misc[M_HOLES], nAdd = n_added(secOff, absCap, secCap);
+    const uint32_t tot = carry + nPh;
+    const uint32_t nAdd = tot < secCap ? tot : secCap;
+    __syncthreads();
     if (h <= absCap && nAdd < h) {
         const uint32_t cnt = h - nAdd;
-        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
+        for (uint32_t t = tid; t < cnt; t += CT) {
             const uint32_t i = n - 1 - t;
             uint32_t lo = 0, hi = h;  // first index with holePos > i
             while (lo < hi) {
@@ -495,14 +479,7 @@ __global__ void __launch_bounds__(256)
             for (int c = 0; c < 6; ++c) p(c, dst) = p(c, i);
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        lastBlock = atomicAdd(misc + M_DONE, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!lastBlock || threadIdx.x != 0) return;
-    const uint32_t tot = secOff[absCap];
+    if (tid != 0) return;
     if (tot > secCap) stats[2] |= ST_SEC_OVERFLOW;
     if (h > absCap) {
         stats[2] |= ST_ABS_OVERFLOW;  // compaction was skipped; the count stays
@@ -514,14 +491,13 @@ __global__ void __launch_bounds__(256)
         }
         *nDev = (uint32_t)newN;
     }
-    const uint32_t nPh = misc[M_PHOTO_PLACED];
+    const uint32_t nPhPlaced = carry >= secCap ? 0u : min(nPh, secCap - carry);
     stats[0] += h;
-    stats[1] += nAdd - nPh;
+    stats[1] += nAdd - nPhPlaced;
     stats[4] = h;
     stats[5] = nAdd;
-    stats[6] += nPh;
+    stats[6] += nPhPlaced;
     *stepDev += 1;
-    misc[M_DONE] = 0;
 }
 
 unsigned grid_cap(uint64_t items, unsigned block, unsigned maxBlocks)
@@ -559,9 +535,12 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     uint32_t *holePos = nzList + nW;
     uint32_t *misc = holePos + a->absCap;
     T *slots = (T *)((char *)a->work + slots_offset(cap, a->absCap));
-    hipLaunchKernelGGL(k_begin_dev, dim3(oecl_blocks(np, 256)), dim3(256), 0, s, np, a->rho, a->beamScale,
-                       a->nScale, a->stepDev, a->profile, a->q, a->nDev, a->countHist, a->histCap, misc);
-    int rc = oecl_fastpoisson_solve(fp, a->q, a->phi, (void *)s);
+    const FpBox box = fastpoisson_box(fp);
+    hipLaunchKernelGGL(k_begin_dev<T>, dim3(oecl_blocks(np, 256)), dim3(256), 0, s, np, a->rho, a->beamScale,
+                       a->nScale, a->stepDev, a->profile, box, a->phi, a->nDev, a->countHist, a->histCap, misc, wd,
+                       a->photoCount, a->photoWeight, a->nPhoto, a->photoCap, a->seed0, a->seed1, a->photoE0,
+                       a->photoSigmaE, P->particleMass, (T *)a->photoSlots, a->stats);
+    int rc = fastpoisson_solve_box(fp, a->phi, s);
     if (rc) return rc;
     rc = oecl_csr_spmv(2 * np, a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, a->E, (void *)s);
     if (rc) return rc;
@@ -584,18 +563,12 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
                        (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, a->absCap, a->emit, a->seed0,
                        a->seed1, a->stepDev, a->nSec, a->impact, misc, a->stats);
     if (a->emit)
-        hipLaunchKernelGGL(k_emit_dev<T>, dim3(grid_cap(a->absCap, 64, 1024)), dim3(64), 0, s, *P,
+        hipLaunchKernelGGL(k_emit_dev<T>, dim3(grid_cap(a->absCap, 6, 1024)), dim3(64), 0, s, *P,
                            (const T *)a->absorbed, (const T *)a->normals, a->absCap, a->nSec, a->impact, a->seed0,
                            a->seed1, a->stepDev, slots, misc);
-    if (a->photoCount)
-        hipLaunchKernelGGL(k_photo_dev<T>, dim3(grid_cap(a->photoCap, 128, 256)), dim3(128), 0, s, wd, a->photoCount,
-                           a->photoWeight, a->nPhoto, a->photoCap, a->stepDev, a->seed0, a->seed1, a->photoE0,
-                           a->photoSigmaE, P->particleMass, (T *)a->photoSlots, misc, a->stats);
     hipLaunchKernelGGL(k_place_dev<T>, dim3(1), dim3(CT), 0, s, *g, p, (const T *)slots, (const T *)a->photoSlots,
                        (T *)a->secondaries, a->secCap, a->nSec, a->secOff, a->absCap, a->nDev, holePos, cap, a->qe,
-                       a->rho, misc, a->stats);
-    hipLaunchKernelGGL(k_tail_end_dev<T>, dim3(grid_cap(a->absCap, 256, 256)), dim3(256), 0, s, p, a->nDev,
-                       a->secOff, a->absCap, a->secCap, holePos, cap, a->stepDev, misc, a->stats);
+                       a->rho, misc, a->stepDev, a->stats);
     OECL_CHECK_LAUNCH("oecl_step_run");
     return OECL_OK;
 }
