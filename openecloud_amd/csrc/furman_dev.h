@@ -67,8 +67,9 @@ __device__ inline uint32_t pass1(const oecl_furman_t &P, const T *a, const T *nv
     double u = rng.u(0);
     double cum = 0., pN1 = 0.;
     uint32_t ns = m;
+    const sf::Binom10 bin(pr);
     for (int ii = 0; ii < m; ++ii) {
-        double pn = sf::binom_pmf(ii, pr, m) * (1 - deltaE - deltaR);
+        double pn = bin.pmf(ii) * (1 - deltaE - deltaR);
         if (ii == 1) {
             pn += deltaE + deltaR;
             pN1 = pn;
@@ -79,7 +80,7 @@ __device__ inline uint32_t pass1(const oecl_furman_t &P, const T *a, const T *nv
             break;
         }
     }
-    if (ns == 0) pN1 = sf::binom_pmf(1, pr, m) * (1 - deltaE - deltaR) + deltaE + deltaR;
+    if (ns == 0) pN1 = bin.pmf(1) * (1 - deltaE - deltaR) + deltaE + deltaR;
     impact[0] = e0;
     impact[1] = deltaE;
     impact[2] = deltaR;

@@ -26,11 +26,11 @@ __device__ __forceinline__ double norm_quantile_rough(double p)
     return p < 0.5 ? -z : z;
 }
 
-// Regularized lower incomplete gamma P(a, x), a > 0.
-__device__ inline double gammainc(double a, double x)
+// Regularized lower incomplete gamma P(a, x), a > 0, given lga = lgamma(a).
+__device__ inline double gammainc_lg(double a, double x, double lga)
 {
     if (!(x > 0.)) return 0.;
-    double lpref = -x + a * log(x) - lgamma(a);
+    double lpref = -x + a * log(x) - lga;
     if (x < a + 1.) {
         // P = e^{-x} x^a / Gamma(a+1) * sum_n x^n / ((a+1)...(a+n))
         double term = 1. / a, sum = term, den = a;
@@ -62,6 +62,9 @@ __device__ inline double gammainc(double a, double x)
     return 1. - exp(lpref) * h;
 }
 
+// Regularized lower incomplete gamma P(a, x), a > 0.
+__device__ inline double gammainc(double a, double x) { return gammainc_lg(a, x, lgamma(a)); }
+
 // Inverse of P(a, .) at probability p.
 __device__ inline double gammaincinv(double a, double p)
 {
@@ -81,7 +84,7 @@ __device__ inline double gammaincinv(double a, double p)
     }
     double lo = 0., hi = INFINITY;
     for (int it = 0; it < 200; ++it) {
-        double f = gammainc(a, x) - p;
+        double f = gammainc_lg(a, x, lga) - p;
         if (f == 0.) return x;
         if (f > 0.) hi = x;
         else lo = x;
@@ -133,14 +136,20 @@ __device__ inline double betacf(double a, double b, double x)
     return h;
 }
 
-// Regularized incomplete beta I_x(a, b).
-__device__ inline double betainc(double a, double b, double x)
+// Regularized incomplete beta I_x(a, b), given lab = lgamma(a + b) - lgamma(a) - lgamma(b).
+__device__ inline double betainc_lg(double a, double b, double x, double lab)
 {
     if (x <= 0.) return 0.;
     if (x >= 1.) return 1.;
-    double lbt = lgamma(a + b) - lgamma(a) - lgamma(b) + a * log(x) + b * log1p(-x);
+    double lbt = lab + a * log(x) + b * log1p(-x);
     if (x < (a + 1.) / (a + b + 2.)) return exp(lbt) * betacf(a, b, x) / a;
     return 1. - exp(lbt) * betacf(b, a, 1. - x) / b;
+}
+
+// Regularized incomplete beta I_x(a, b).
+__device__ inline double betainc(double a, double b, double x)
+{
+    return betainc_lg(a, b, x, lgamma(a + b) - lgamma(a) - lgamma(b));
 }
 
 // Inverse of I_.(a, b) at probability p.
@@ -165,9 +174,10 @@ __device__ inline double betaincinv(double a, double b, double p)
     }
     if (!(x > 0. && x < 1.)) x = 0.5;
     double lbeta = lgamma(a) + lgamma(b) - lgamma(a + b);
+    const double lab = lgamma(a + b) - lgamma(a) - lgamma(b);
     double lo = 0., hi = 1.;
     for (int it = 0; it < 300; ++it) {
-        double f = betainc(a, b, x) - p;
+        double f = betainc_lg(a, b, x, lab) - p;
         if (f == 0.) return x;
         if (f > 0.) hi = x;
         else lo = x;
@@ -189,18 +199,27 @@ __device__ inline double betaincinv(double a, double b, double p)
     return x;
 }
 
-// Binomial probability C(n, k) p^k (1-p)^(n-k) for n <= 10 (gsl_ran_binomial_pdf's value).
-__device__ __forceinline__ double binom_pmf(int k, double p, int n)
-{
-    const double c10[11] = {1., 10., 45., 120., 210., 252., 210., 120., 45., 10., 1.};
-    double c;
-    if (n == 10) {
-        c = c10[k];
-    } else {
-        c = 1.;
-        for (int i = 1; i <= k; ++i) c = c * (n - k + i) / i;
+// Binomial probabilities C(10, k) p^k (1-p)^(10-k) (gsl_ran_binomial_pdf's value for the
+// emitter's m = 10), with p^k and (1-p)^j formed by repeated multiplication in a fixed order:
+// pk[k] = (...((1 * p) * p)...) * p, k factors.  Evaluated as (C * pk[k]) * qk[10 - k].
+struct Binom10 {
+    double pk[11], qk[11];
+    __device__ explicit Binom10(double p)
+    {
+        const double q = 1. - p;
+        pk[0] = 1.;
+        qk[0] = 1.;
+#pragma unroll
+        for (int i = 1; i <= 10; ++i) {
+            pk[i] = pk[i - 1] * p;
+            qk[i] = qk[i - 1] * q;
+        }
     }
-    return c * pow(p, (double)k) * pow(1. - p, (double)(n - k));
-}
+    __device__ double pmf(int k) const
+    {
+        const double c10[11] = {1., 10., 45., 120., 210., 252., 210., 120., 45., 10., 1.};
+        return c10[k] * pk[k] * qk[10 - k];
+    }
+};
 
 }  // namespace sf

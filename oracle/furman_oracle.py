@@ -56,8 +56,16 @@ def material(name='copper', seyMax=-1., reflec=-1., eHat0=None):
 
 
 def binom_pmf(k, pr, n):
-    """gsl_ran_binomial_pdf (specFun.pyx:34-35) evaluated directly."""
-    return math.comb(n, k) * pr ** k * (1. - pr) ** (n - k)
+    """gsl_ran_binomial_pdf (specFun.pyx:34-35): C(n, k) p^k (1-p)^(n-k), the powers formed by
+    repeated multiplication in a fixed order (the device code's order)."""
+    pk = 1.
+    for _ in range(k):
+        pk *= pr
+    qk = 1.
+    q = 1. - pr
+    for _ in range(n - k):
+        qk *= q
+    return math.comb(n, k) * pk * qk
 
 
 def yields(e0, theta0, P):

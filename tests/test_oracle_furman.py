@@ -68,3 +68,15 @@ def test_inverse_special_functions_inverse_property():
         p = rng.random(20)
         x = sps.betaincinv(a, b, p)
         assert np.allclose(sps.betainc(a, b, x), p, rtol=1e-12, atol=1e-14)
+
+
+def test_binomial_pmf_matches_library():
+    """The emitter's binomial probabilities (powers by repeated multiplication) against
+    scipy.stats.binom.pmf, and summing to 1."""
+    from scipy.stats import binom
+    from oracle.furman_oracle import binom_pmf
+    for p in (0., 1e-6, 0.013, 0.25, 0.5, 0.8, 0.999, 1.):
+        got = np.array([binom_pmf(k, p, 10) for k in range(11)])
+        ref = binom.pmf(np.arange(11), 10, p)
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-300)
+        assert abs(got.sum() - 1.) < 1e-14
