@@ -55,6 +55,10 @@ typedef struct {
     const double *boundaryPoints;         /* [nb][2]          (cut cell only) */
     const int32_t *cutCellPointsInd;      /* [npExt][2]       (cut cell only) */
     const double *cutCellNormalVectors;   /* [npExt][2]       (cut cell only) */
+    int32_t rectCornerFix;   /* rectangle only: 0 = the reference's normals where a particle
+                                left through a corner (particleBoundary.pyx:154-187 label four
+                                of those cases with the other wall's normal); 1 = the normal of
+                                the wall crossed first.  The interaction point is the same. */
 } oecl_wall_t;
 
 /* Furman-Pivi model parameters (particleEmitter.pyx:107-152). */
@@ -67,6 +71,19 @@ typedef struct {
     double particleMass;
     int32_t scaleSigmaE;
 } oecl_furman_t;
+
+/* The two simpler emitters of particleEmitter.pyx: SecElecEmitter (:316-482, parameters of
+ * stainlessMod :334-355) and SimpleSecEmitter (:489-596, whose values are fixed in its code,
+ * so it reads `model` only). */
+enum { OECL_SECEM_STAINLESS = 0, OECL_SECEM_SIMPLE = 1 };
+typedef struct {
+    int32_t model;
+    double p1EInf, p1Ehat, eEHat, w, p, e1, e2;
+    double p1RInf, eR, r, q, r1, r2;
+    double deltaTSHat, s, eHat0, t1, t2, t3, t4;
+    double sigmaETS, muETS, theta0max, deltaERmax;
+    double particleMass;
+} oecl_secem_t;
 
 const char *oecl_last_error(void);
 int oecl_device_sync(void);
@@ -171,6 +188,21 @@ int oecl_furman_pass2(const oecl_furman_t *prm, int dtype, const void *absorbed,
                       const uint32_t *offsets, const double *impact, const double *uniforms,
                       uint32_t seed0, uint32_t seed1, uint32_t step, void *secondaries,
                       uint32_t secCap, void *stream);
+
+/* SecElecEmitter / SimpleSecEmitter .generateSecondaries (particleEmitter.pyx:357-482 /
+ * :501-596), first loop: nSec[nAbs] (elastic + rediffused + true secondaries of each impact,
+ * at most 10) and impact[nAbs][4] = (e0, nE + 2 nR + 4 nTS, -, -).  uniforms: NULL for
+ * Philox4x32-10 under (seed0, seed1) with counter (impact, slot/2, step, 7), else an injected
+ * FP64 table [nAbs][44].  The true-secondary count must stay <= 8: the host rejects yields
+ * that could exceed it. */
+int oecl_secem_pass1(const oecl_secem_t *prm, int dtype, const void *absorbed, const void *normals,
+                     uint32_t nAbs, const double *uniforms, uint32_t seed0, uint32_t seed1, uint32_t step,
+                     uint32_t *nSec, double *impact, void *stream);
+/* Second loop: the secondaries at offsets[] (oecl_scan_u32 of nSec) into AoS [secCap][6]. */
+int oecl_secem_pass2(const oecl_secem_t *prm, int dtype, const void *absorbed, const void *normals,
+                     uint32_t nAbs, const uint32_t *offsets, const double *impact, const double *uniforms,
+                     uint32_t seed0, uint32_t seed1, uint32_t step, void *secondaries, uint32_t secCap,
+                     void *stream);
 
 /* homoLoader (particleEmitter.pyx:41-72): n particles uniform in the chamber by rejection
  * against the wall, velocities N(0, vsigma) per component, w = weightInit (times
@@ -357,6 +389,7 @@ typedef struct {
     double photoE0, photoSigmaE;  /* eV */
     void *photoSlots;           /* [photoCap][6] */
     uint32_t photoCap;
+    const oecl_secem_t *secem;  /* NULL: the Furman emitter `prm`; else this simpler emitter */
 } oecl_step_args;
 
 uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap);

@@ -33,7 +33,8 @@ class WallDesc(ctypes.Structure):
                 ('lxHalfExt', ctypes.c_double), ('lyHalfExt', ctypes.c_double),
                 ('nx', ctypes.c_uint32),
                 ('insideFaces', ctypes.c_void_p), ('boundaryPoints', ctypes.c_void_p),
-                ('cutCellPointsInd', ctypes.c_void_p), ('cutCellNormalVectors', ctypes.c_void_p)]
+                ('cutCellPointsInd', ctypes.c_void_p), ('cutCellNormalVectors', ctypes.c_void_p),
+                ('rectCornerFix', ctypes.c_int32)]
 
 
 class FurmanDesc(ctypes.Structure):
@@ -45,6 +46,17 @@ class FurmanDesc(ctypes.Structure):
                [('epsN', ctypes.c_double * 10), ('pSmallN', ctypes.c_double * 10),
                 ('particleMass', ctypes.c_double), ('scaleSigmaE', ctypes.c_int32)]
 
+
+class SecemDesc(ctypes.Structure):
+    _fields_ = [('model', ctypes.c_int32)] + [(n, ctypes.c_double) for n in
+                ('p1EInf', 'p1Ehat', 'eEHat', 'w', 'p', 'e1', 'e2',
+                 'p1RInf', 'eR', 'r', 'q', 'r1', 'r2',
+                 'deltaTSHat', 's', 'eHat0', 't1', 't2', 't3', 't4',
+                 'sigmaETS', 'muETS', 'theta0max', 'deltaERmax', 'particleMass')]
+
+
+SECEM_SLOTS = 44
+SECEM_STAINLESS, SECEM_SIMPLE = 0, 1
 
 _lib = None
 

@@ -39,7 +39,8 @@ class StepArgs(ctypes.Structure):
                 ('stats', _VP), ('countHist', _VP), ('histCap', ctypes.c_uint32), ('work', _VP),
                 ('evPushBegin', _VP), ('evPushEnd', _VP), ('photoCount', _VP), ('photoWeight', _VP),
                 ('nPhoto', ctypes.c_uint32), ('photoE0', ctypes.c_double),
-                ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32)]
+                ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32),
+                ('secem', _VP)]
 
 
 ST_ABS_OVERFLOW, ST_SEC_OVERFLOW, ST_CAP_OVERFLOW, ST_PHOTO_OVERFLOW = 1, 2, 4, 8
@@ -305,6 +306,13 @@ class Buildup:
             a.photoSigmaE = self.photo.energySigma
             a.photoSlots = self._photoSlots.data_ptr()
             a.photoCap = self.photoCap
+        secem = getattr(self.em, 'secemDesc', None)
+        if secem is not None:
+            a.secem = ctypes.addressof(secem)
+            self._prm = L.FurmanDesc()           # only its particleMass is read with secem set
+            self._prm.particleMass = p.particleMass
+        else:
+            self._prm = self.em.desc
         a.work = self._stepwork.data_ptr()
         self._args = a
         self._native_it0 = self.it
@@ -316,7 +324,7 @@ class Buildup:
         if self.it - self._native_it0 >= self.maxSteps:
             raise L.OeclError('native step: maxSteps exhausted; call native_setup again')
         L.check(L.lib().oecl_step_run(ctypes.byref(self.gd), ctypes.byref(self.pb.desc),
-                                      ctypes.byref(self.em.desc), self.ps._fp,
+                                      ctypes.byref(self._prm), self.ps._fp,
                                       ctypes.byref(self._args), L.stream()), 'step')
         self.it += 1
 

@@ -166,38 +166,7 @@ __global__ void k_wall_interaction(WallDev w, T *ap, uint32_t na, double dt, T *
     double x = p[0], y = p[1], vx = p[2], vy = p[3];
     double r = 0., n0 = 0., n1 = 0.;
     if (w.kind == OECL_WALL_RECT) {
-        // particleBoundary.pyx:108-192 and :195-221.
-        const double eps = 1e-6;
-        double lxh = w.lx * 0.5 - w.dx * eps, lyh = w.ly * 0.5 - w.dy * eps;
-        int xi = 1, yi = 1, lw = 0;
-        double tx, ty;
-        if (x > lxh) xi = 2;
-        else if (x < -lxh) xi = 0;
-        if (y > lyh) yi = 2;
-        else if (y < -lyh) yi = 0;
-        if (xi == 0 && yi == 1) { r = (x + lxh) / vx; lw = 0; }
-        else if (xi == 2 && yi == 1) { r = (x - lxh) / vx; lw = 1; }
-        else if (xi == 1 && yi == 0) { r = (y + lyh) / vy; lw = 2; }
-        else if (xi == 1 && yi == 2) { r = (y - lyh) / vy; lw = 3; }
-        else if (xi == 0 && yi == 0) {
-            tx = (x + lxh) / vx; ty = (y + lyh) / vy;
-            if (tx > ty) { r = tx; lw = 0; } else { r = ty; lw = 1; }
-        } else if (xi == 2 && yi == 0) {
-            tx = (x - lxh) / vx; ty = (y + lyh) / vy;
-            if (tx > ty) { r = tx; lw = 2; } else { r = ty; lw = 1; }
-        } else if (xi == 0 && yi == 2) {
-            tx = (x + lxh) / vx; ty = (y - lyh) / vy;
-            if (tx > ty) { r = tx; lw = 0; } else { r = ty; lw = 3; }
-        } else {
-            tx = (x - lxh) / vx; ty = (y - lyh) / vy;
-            if (tx > ty) { r = tx; lw = 2; } else { r = ty; lw = 3; }
-        }
-        x -= r * vx;
-        y -= r * vy;
-        if (lw == 0) { n0 = 1.; n1 = 0.; }
-        else if (lw == 1) { n0 = -1.; n1 = 0.; }
-        else if (lw == 2) { n0 = 0.; n1 = 1.; }
-        else { n0 = 0.; n1 = -1.; }
+        wall_rect_hit(w, x, y, vx, vy, r, n0, n1);
     } else if (w.kind == OECL_WALL_ELLIP) {
         // particleBoundary.pyx:252-274 and :277-297.
         const double eps = 1.e-6;

@@ -1,13 +1,15 @@
-// Furman-Pivi secondary emission (particleEmitter.pyx:154-308) and the uniform loader
-// (particleEmitter.pyx:41-72).
+// Furman-Pivi secondary emission (particleEmitter.pyx:154-308), the two simpler emitters
+// (SecElecEmitter, SimpleSecEmitter: particleEmitter.pyx:316-596, drawn as in secem_dev.h), the
+// uniform loader (particleEmitter.pyx:41-72) and the photoelectron source.
 //
-// Random numbers: either an injected FP64 table [nAbs][35] or Philox4x32-10 with counter
+// Furman random numbers: either an injected FP64 table [nAbs][35] or Philox4x32-10 with counter
 // (impact, slot/2, step, purpose).  The slot of every draw is fixed per impact:
 //   0 nSec pick, 1 branch pick, 2-3 Box-Muller pair, 4 the further nSec == 1 draw,
 //   5..13 thetaK, 14 y, 15+2i / 16+2i theta / phi of secondary i.
 #include "common.h"
 #include "furman_dev.h"
 #include "photo_dev.h"
+#include "secem_dev.h"
 #include "scan.h"
 #include "wall.h"
 
@@ -88,6 +90,32 @@ __global__ void k_homo_loader(WallDev w, Soa<T> p, uint32_t n, double weightInit
     p(3, i) = (T)(z1 * vsigma);
     p(4, i) = (T)(z2 * vsigma);
     p(5, i) = (T)wt;
+}
+
+template <typename T>
+__global__ void k_secem_pass1(oecl_secem_t P, const T *__restrict__ ap, const T *__restrict__ nv, uint32_t n0,
+                              const double *__restrict__ table, uint32_t s0, uint32_t s1, uint32_t step,
+                              uint32_t *__restrict__ nSec, double *__restrict__ impact)
+{
+    const uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= n0) return;
+    const secem::Rng rng{table, jj, step, s0, s1};
+    nSec[jj] = secem::pass1<T>(P, ap + 6 * (uint64_t)jj, nv + 2 * (uint64_t)jj, rng, impact + 4 * (uint64_t)jj);
+}
+
+template <typename T>
+__global__ void k_secem_pass2(oecl_secem_t P, const T *__restrict__ ap, const T *__restrict__ nv, uint32_t n0,
+                              const uint32_t *__restrict__ offs, const double *__restrict__ impact,
+                              const double *__restrict__ table, uint32_t s0, uint32_t s1, uint32_t step,
+                              T *__restrict__ out, uint32_t secCap)
+{
+    const uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= n0) return;
+    const uint32_t base = offs[jj];
+    if (offs[jj + 1] == base) return;
+    const secem::Rng rng{table, jj, step, s0, s1};
+    secem::pass2<T>(P, ap + 6 * (uint64_t)jj, nv + 2 * (uint64_t)jj, impact + 4 * (uint64_t)jj, rng,
+                    [&](uint32_t k) -> T * { return base + k < secCap ? out + 6 * (uint64_t)(base + k) : nullptr; });
 }
 
 template <typename T>
@@ -172,6 +200,53 @@ OECL_EXPORT int oecl_furman_pass2(const oecl_furman_t *prm, int dtype, const voi
     else
         return OECL_ERR_ARG;
     OECL_CHECK_LAUNCH("oecl_furman_pass2");
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_secem_pass1(const oecl_secem_t *prm, int dtype, const void *absorbed, const void *normals,
+                                 uint32_t nAbs, const double *uniforms, uint32_t seed0, uint32_t seed1,
+                                 uint32_t step, uint32_t *nSec, double *impact, void *stream)
+{
+    if (!prm || !nSec || !impact) return OECL_ERR_ARG;
+    if (prm->model != OECL_SECEM_STAINLESS && prm->model != OECL_SECEM_SIMPLE) return OECL_ERR_ARG;
+    if (nAbs == 0) return OECL_OK;
+    if (!absorbed || !normals) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 gr(oecl_blocks(nAbs, 128)), bl(128);
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_secem_pass1<double>, gr, bl, 0, s, *prm, (const double *)absorbed,
+                           (const double *)normals, nAbs, uniforms, seed0, seed1, step, nSec, impact);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_secem_pass1<float>, gr, bl, 0, s, *prm, (const float *)absorbed,
+                           (const float *)normals, nAbs, uniforms, seed0, seed1, step, nSec, impact);
+    else
+        return OECL_ERR_ARG;
+    OECL_CHECK_LAUNCH("oecl_secem_pass1");
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_secem_pass2(const oecl_secem_t *prm, int dtype, const void *absorbed, const void *normals,
+                                 uint32_t nAbs, const uint32_t *offsets, const double *impact,
+                                 const double *uniforms, uint32_t seed0, uint32_t seed1, uint32_t step,
+                                 void *secondaries, uint32_t secCap, void *stream)
+{
+    if (!prm || !offsets || !impact) return OECL_ERR_ARG;
+    if (prm->model != OECL_SECEM_STAINLESS && prm->model != OECL_SECEM_SIMPLE) return OECL_ERR_ARG;
+    if (nAbs == 0 || secCap == 0) return OECL_OK;
+    if (!absorbed || !normals || !secondaries) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 gr(oecl_blocks(nAbs, 64)), bl(64);
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_secem_pass2<double>, gr, bl, 0, s, *prm, (const double *)absorbed,
+                           (const double *)normals, nAbs, offsets, impact, uniforms, seed0, seed1, step,
+                           (double *)secondaries, secCap);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_secem_pass2<float>, gr, bl, 0, s, *prm, (const float *)absorbed,
+                           (const float *)normals, nAbs, offsets, impact, uniforms, seed0, seed1, step,
+                           (float *)secondaries, secCap);
+    else
+        return OECL_ERR_ARG;
+    OECL_CHECK_LAUNCH("oecl_secem_pass2");
     return OECL_OK;
 }
 

@@ -21,6 +21,7 @@
 #include "common.h"
 #include "fastpoisson_internal.h"
 #include "furman_dev.h"
+#include "secem_dev.h"
 #include "photo_dev.h"
 #include "particles_dev.h"
 #include "scan.h"
@@ -248,35 +249,7 @@ __device__ __forceinline__ void hit_geom(const WallDev &w, double dt, double &x,
     n1 = 0.;
     found = true;
     if (w.kind == OECL_WALL_RECT) {
-        const double eps = 1e-6;
-        double lxh = w.lx * 0.5 - w.dx * eps, lyh = w.ly * 0.5 - w.dy * eps;
-        int xi = 1, yi = 1, lw = 0;
-        double tx, ty;
-        if (x > lxh) xi = 2;
-        else if (x < -lxh) xi = 0;
-        if (y > lyh) yi = 2;
-        else if (y < -lyh) yi = 0;
-        if (xi == 0 && yi == 1) { r = (x + lxh) / vx; lw = 0; }
-        else if (xi == 2 && yi == 1) { r = (x - lxh) / vx; lw = 1; }
-        else if (xi == 1 && yi == 0) { r = (y + lyh) / vy; lw = 2; }
-        else if (xi == 1 && yi == 2) { r = (y - lyh) / vy; lw = 3; }
-        else if (xi == 0 && yi == 0) {
-            tx = (x + lxh) / vx; ty = (y + lyh) / vy;
-            if (tx > ty) { r = tx; lw = 0; } else { r = ty; lw = 1; }
-        } else if (xi == 2 && yi == 0) {
-            tx = (x - lxh) / vx; ty = (y + lyh) / vy;
-            if (tx > ty) { r = tx; lw = 2; } else { r = ty; lw = 1; }
-        } else if (xi == 0 && yi == 2) {
-            tx = (x + lxh) / vx; ty = (y - lyh) / vy;
-            if (tx > ty) { r = tx; lw = 0; } else { r = ty; lw = 3; }
-        } else {
-            tx = (x - lxh) / vx; ty = (y - lyh) / vy;
-            if (tx > ty) { r = tx; lw = 2; } else { r = ty; lw = 3; }
-        }
-        x -= r * vx;
-        y -= r * vy;
-        n0 = lw == 0 ? 1. : (lw == 1 ? -1. : 0.);
-        n1 = lw == 2 ? 1. : (lw == 3 ? -1. : 0.);
+        wall_rect_hit(w, x, y, vx, vy, r, n0, n1);
     } else if (w.kind == OECL_WALL_ELLIP) {
         const double eps = 1.e-6;
         double ax = 4. / ((w.lx - eps * w.dx) * (w.lx - eps * w.dx));
@@ -333,8 +306,9 @@ __device__ __forceinline__ void hit_geom(const WallDev &w, double dt, double &x,
 // Wall interaction, normals (written in place of the absorbed row's position) and the first
 // Furman loop for hit jj: nSec[jj] and impact[jj].
 template <typename T>
-__device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const oecl_furman_t &P, T *ap, T *nv,
-                                            T *rts, double dt, int emit, uint32_t s0, uint32_t s1, uint32_t step,
+__device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const oecl_furman_t &P,
+                                            const oecl_secem_t &S, int useSecem, T *ap, T *nv, T *rts, double dt,
+                                            int emit, uint32_t s0, uint32_t s1, uint32_t step,
                                             uint32_t *__restrict__ nSec, double *__restrict__ impact,
                                             uint32_t *stats)
 {
@@ -350,7 +324,10 @@ __device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const
     nn[0] = (T)n0;
     nn[1] = (T)n1;
     uint32_t ns = 0;
-    if (emit) {
+    if (emit && useSecem) {
+        const secem::Rng rng{nullptr, jj, step, s0, s1};
+        ns = secem::pass1<T>(S, a, nn, rng, impact + 4 * (uint64_t)jj);
+    } else if (emit) {
         furman::Rng rng{nullptr, jj, step, s0, s1};
         ns = furman::pass1<T>(P, a, nn, rng, impact + 4 * (uint64_t)jj);
     }
@@ -360,13 +337,32 @@ __device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const
 
 template <typename T>
 __global__ void __launch_bounds__(128)
-    k_hits_dev(WallDev w, oecl_furman_t P, T *ap, T *nv, T *rts, double dt, uint32_t absCap, int emit, uint32_t s0,
-               uint32_t s1, const uint32_t *__restrict__ stepDev, uint32_t *__restrict__ nSec,
-               double *__restrict__ impact, const uint32_t *misc, uint32_t *stats)
+    k_hits_dev(WallDev w, oecl_furman_t P, oecl_secem_t S, int useSecem, T *ap, T *nv, T *rts, double dt,
+               uint32_t absCap, int emit, uint32_t s0, uint32_t s1, const uint32_t *__restrict__ stepDev,
+               uint32_t *__restrict__ nSec, double *__restrict__ impact, const uint32_t *misc, uint32_t *stats)
 {
     const uint32_t na = n_hits(misc, absCap), step = *stepDev;
     for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x)
-        hit_one<T>(jj, w, P, ap, nv, rts, dt, emit, s0, s1, step, nSec, impact, stats);
+        hit_one<T>(jj, w, P, S, useSecem, ap, nv, rts, dt, emit, s0, s1, step, nSec, impact, stats);
+}
+
+// Second loop of the simpler emitters (secem::pass2) into the hit's MAXSEC slots, one thread
+// per hit, grid-stride.
+template <typename T>
+__global__ void __launch_bounds__(128)
+    k_secem_emit_dev(oecl_secem_t S, const T *__restrict__ ap, const T *__restrict__ nv, uint32_t absCap,
+                     const uint32_t *__restrict__ nSec, const double *__restrict__ impact, uint32_t s0, uint32_t s1,
+                     const uint32_t *__restrict__ stepDev, T *__restrict__ slots, const uint32_t *misc)
+{
+    const uint32_t na = n_hits(misc, absCap), step = *stepDev;
+    for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x) {
+        const uint32_t ns = nSec[jj];
+        if (ns == 0) continue;
+        const secem::Rng rng{nullptr, jj, step, s0, s1};
+        T *base = slots + 6 * (uint64_t)MAXSEC * jj;
+        secem::pass2<T>(S, ap + 6 * (uint64_t)jj, nv + 2 * (uint64_t)jj, impact + 4 * (uint64_t)jj, rng,
+                        [&](uint32_t k) -> T * { return k < ns && k < MAXSEC ? base + 6 * (uint64_t)k : nullptr; });
+    }
 }
 
 // Second Furman loop into the hit's MAXSEC slots of `slots` (impact-major; k_place_dev makes
@@ -559,10 +555,16 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     if (a->evPushEnd) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushEnd, s), "event");
     hipLaunchKernelGGL(k_holes_collect<T>, dim3(1), dim3(CT), 0, s, p, a->nDev, counts, list, nzList, holePos,
                        a->absCap, (T *)a->absorbed, misc);
-    hipLaunchKernelGGL(k_hits_dev<T>, dim3(grid_cap(a->absCap, 128, 512)), dim3(128), 0, s, wd, *P,
+    const oecl_secem_t S = a->secem ? *a->secem : oecl_secem_t{};
+    const int useSecem = a->secem ? 1 : 0;
+    hipLaunchKernelGGL(k_hits_dev<T>, dim3(grid_cap(a->absCap, 128, 512)), dim3(128), 0, s, wd, *P, S, useSecem,
                        (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, a->absCap, a->emit, a->seed0,
                        a->seed1, a->stepDev, a->nSec, a->impact, misc, a->stats);
-    if (a->emit)
+    if (a->emit && useSecem)
+        hipLaunchKernelGGL(k_secem_emit_dev<T>, dim3(grid_cap(a->absCap, 128, 512)), dim3(128), 0, s, S,
+                           (const T *)a->absorbed, (const T *)a->normals, a->absCap, a->nSec, a->impact, a->seed0,
+                           a->seed1, a->stepDev, slots, misc);
+    else if (a->emit)
         hipLaunchKernelGGL(k_emit_dev<T>, dim3(grid_cap(a->absCap, 6, 1024)), dim3(64), 0, s, *P,
                            (const T *)a->absorbed, (const T *)a->normals, a->absCap, a->nSec, a->impact, a->seed0,
                            a->seed1, a->stepDev, slots, misc);
@@ -582,6 +584,8 @@ OECL_EXPORT int oecl_step_run(const oecl_grid_t *g, const oecl_wall_t *wall, con
         !a->stats || !a->work || a->absCap == 0 || a->secCap == 0)
         return OECL_ERR_ARG;
     if (a->typeB == 2 && !a->bField) return OECL_ERR_ARG;
+    if (a->secem && a->secem->model != OECL_SECEM_STAINLESS && a->secem->model != OECL_SECEM_SIMPLE)
+        return OECL_ERR_ARG;
     if (a->photoCount && (!a->photoWeight || !a->photoSlots || a->photoCap == 0)) return OECL_ERR_ARG;
     if (a->photoCount && wall->kind != OECL_WALL_RECT && wall->kind != OECL_WALL_ELLIP) {
         oecl_set_error_msg("photoelectrons: rectangular or elliptical wall only");

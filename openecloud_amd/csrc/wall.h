@@ -14,6 +14,7 @@ struct WallDev {
     const double *bp;
     const int32_t *cpi;
     const double *ccn;
+    int32_t rectCornerFix;      // rectangle: normal of the wall crossed first at a corner
 };
 
 static inline int wall_to_device(const oecl_wall_t *w, WallDev *d)
@@ -39,6 +40,7 @@ static inline int wall_to_device(const oecl_wall_t *w, WallDev *d)
     d->bp = w->boundaryPoints;
     d->cpi = w->cutCellPointsInd;
     d->ccn = w->cutCellNormalVectors;
+    d->rectCornerFix = w->rectCornerFix;
     if (w->kind < 0 || w->kind > 2) return OECL_ERR_ARG;
     if (w->kind == OECL_WALL_CUTCELL &&
         (!w->insideFaces || !w->boundaryPoints || !w->cutCellPointsInd ||
@@ -90,6 +92,43 @@ __device__ __forceinline__ uint8_t wall_inside(const WallDev &w, T &x, T &y)
         y = (T)yd;
         return cutcell_inside(w, xd, yd);
     }
+}
+
+// Rectangle wall hit (particleBoundary.pyx:108-192 and :195-221): moves (x, y) back along
+// (vx, vy) by the remaining time r to the wall inset by 1e-6 of a cell and sets the inward
+// normal (n0, n1).  Where both coordinates are out, r is the larger of the two crossing
+// times, i.e. the wall crossed first.  The reference labels four of those corner cases with
+// the other wall's normal (:162-187); that is kept unless rectCornerFix is set, in which case
+// the normal is the crossed-first wall's.
+__device__ __forceinline__ void wall_rect_hit(const WallDev &w, double &x, double &y, double vx, double vy,
+                                              double &r, double &n0, double &n1)
+{
+    const double eps = 1e-6;
+    const double lxh = w.lx * 0.5 - w.dx * eps, lyh = w.ly * 0.5 - w.dy * eps;
+    int xi = 1, yi = 1, lw = 0;
+    if (x > lxh) xi = 2;
+    else if (x < -lxh) xi = 0;
+    if (y > lyh) yi = 2;
+    else if (y < -lyh) yi = 0;
+    if (yi == 1) {
+        r = (xi == 0 ? x + lxh : x - lxh) / vx;
+        lw = xi == 0 ? 0 : 1;
+    } else if (xi == 1) {
+        r = (yi == 0 ? y + lyh : y - lyh) / vy;
+        lw = yi == 0 ? 2 : 3;
+    } else {
+        const double tx = (xi == 0 ? x + lxh : x - lxh) / vx;
+        const double ty = (yi == 0 ? y + lyh : y - lyh) / vy;
+        const bool xFirst = tx > ty;
+        r = xFirst ? tx : ty;
+        if (w.rectCornerFix) lw = xFirst ? (xi == 0 ? 0 : 1) : (yi == 0 ? 2 : 3);
+        else if (xFirst) lw = xi == 0 ? 0 : 2;  // reference codes at a corner
+        else lw = yi == 0 ? 1 : 3;
+    }
+    x -= r * vx;
+    y -= r * vy;
+    n0 = lw == 0 ? 1. : (lw == 1 ? -1. : 0.);
+    n1 = lw == 2 ? 1. : (lw == 3 ? -1. : 0.);
 }
 
 // The reference's isInside(x, y) used by the loader and the split (no clipping).

@@ -146,8 +146,23 @@ class ParticleBoundary:
 
 
 class AbsorbRectangular(ParticleBoundary):
-    """particleBoundary.pyx:77-221."""
+    """particleBoundary.pyx:77-221.
+
+    cornerFix: where a particle left through a corner region (both coordinates out), the
+    reference gives four of the eight cases the normal of the other wall
+    (particleBoundary.pyx:154-187).  cornerFix=False keeps that, for parity; cornerFix=True
+    takes the normal of the wall crossed first (the one whose crossing time is the larger,
+    which is also the time the interaction point is moved back by)."""
     kind = L.WALL_RECT
+
+    def __init__(self, gridObj, particlesObj, cornerFix=False):
+        self.cornerFix = bool(cornerFix)
+        super().__init__(gridObj, particlesObj)
+
+    def _make_desc(self):
+        d = super()._make_desc()
+        d.rectCornerFix = 1 if self.cornerFix else 0
+        return d
 
     def isInside(self, x, y):
         lxh, lyh = self.lx * 0.5, self.ly * 0.5

@@ -1,9 +1,10 @@
-"""Secondary emission and loading (particleEmitter.pyx:41-311) on the GPU.
+"""Secondary emission and loading (particleEmitter.pyx:41-596) on the GPU.
 
-FurmanEmitter keeps the reference's constructor and generateSecondaries(); it additionally
-accepts eHat0 (the energy of maximum true-secondary yield, 'E_max'), which the reference
-hard-codes per material.  Random draws come from Philox (see randomGen) or, for testing, from
-an injected uniform table of shape (nAbs, 35).
+FurmanEmitter, SecElecEmitter and SimpleSecEmitter keep the reference's constructors and
+generateSecondaries(); FurmanEmitter additionally accepts eHat0 (the energy of maximum
+true-secondary yield, 'E_max'), which the reference hard-codes per material.  Random draws
+come from Philox (see randomGen) or, for testing, from an injected uniform table of shape
+(nAbs, 35) for FurmanEmitter and (nAbs, 44) for the other two.
 """
 import ctypes
 
@@ -128,6 +129,123 @@ class FurmanEmitter:
     def getNSec(self):
         na = self.particleBoundaryObj.getAbsorbedMacroParticleCount()
         return self._nsec[:na]
+
+
+class _SimpleModelEmitter:
+    """Shared driver of the two simpler emitters: first loop (counts), scan, second loop."""
+    model = None
+
+    def __init__(self, particleBoundaryObj, particlesObj):
+        self.particlesObj = particlesObj
+        self.particleBoundaryObj = particleBoundaryObj
+        self.particleMass = particlesObj.getParticleMass()
+        self.step = 0
+        dev = L.device()
+        self.secondaries = torch.empty((0, 6), dtype=particlesObj.dtype, device=dev)
+        self._nsec = torch.empty(1, dtype=torch.int32, device=dev)
+        self._off = torch.empty(2, dtype=torch.int32, device=dev)
+        self._impact = torch.empty(4, dtype=torch.float64, device=dev)
+
+    def _make_desc(self, P):
+        d = L.SecemDesc()
+        d.model = self.model
+        for k, v in P.items():
+            setattr(d, k, float(v))
+        d.particleMass = self.particleMass
+        return d
+
+    def generateSecondaries(self, uniforms=None, step=None):
+        """-> (nSecSum, 6) device tensor, the secondaries of each impact in the reference's
+        order (elastic, rediffused, true ones).  uniforms: optional injected table (nAbs, 44)
+        of FP64 uniforms; step: the Philox stream index (defaults to a call counter)."""
+        pb = self.particleBoundaryObj
+        p = self.particlesObj
+        na = pb.getAbsorbedMacroParticleCount()
+        if step is None:
+            step = self.step
+            self.step += 1
+        dev = L.device()
+        if na == 0:
+            self.secondaries = torch.empty((0, 6), dtype=p.dtype, device=dev)
+            return self.secondaries
+        if self._nsec.shape[0] < na:
+            cap = int(na * 1.25) + 64
+            self._nsec = torch.empty(cap, dtype=torch.int32, device=dev)
+            self._off = torch.empty(cap + 1, dtype=torch.int32, device=dev)
+            self._impact = torch.empty(4 * cap, dtype=torch.float64, device=dev)
+        u = None
+        if uniforms is not None:
+            u = L.to_dev(np.asarray(uniforms, np.float64).reshape(na, L.SECEM_SLOTS))
+        k0, k1 = randomGen.key()
+        ap, nv = pb.absorbedParticles, pb.normalVectors
+        L.check(L.lib().oecl_secem_pass1(
+            ctypes.byref(self.secemDesc), L.i32(p.code), L.ptr(ap), L.ptr(nv), L.u32(na), L.ptr(u),
+            L.u32(k0), L.u32(k1), L.u32(step), L.ptr(self._nsec), L.ptr(self._impact), L.stream()),
+            'secondary emission pass 1')
+        work = torch.empty(int(L.lib().oecl_scan_work_bytes(na)) // 4 + 8, dtype=torch.int32, device=dev)
+        L.check(L.lib().oecl_scan_u32(L.ptr(self._nsec), L.u32(na), L.ptr(self._off), L.ptr(work),
+                                      L.stream()), 'secondary emission scan')
+        total = int(self._off[na].item())
+        self.secondaries = torch.empty((total, 6), dtype=p.dtype, device=dev)
+        if total > 0:
+            L.check(L.lib().oecl_secem_pass2(
+                ctypes.byref(self.secemDesc), L.i32(p.code), L.ptr(ap), L.ptr(nv), L.u32(na),
+                L.ptr(self._off), L.ptr(self._impact), L.ptr(u), L.u32(k0), L.u32(k1), L.u32(step),
+                L.ptr(self.secondaries), L.u32(total), L.stream()), 'secondary emission pass 2')
+        return self.secondaries
+
+    def getEmittedParticles(self):
+        return self.secondaries
+
+    def getNSec(self):
+        na = self.particleBoundaryObj.getAbsorbedMacroParticleCount()
+        return self._nsec[:na]
+
+
+class SecElecEmitter(_SimpleModelEmitter):
+    """particleEmitter.pyx:316-482: the stainless-steel Furman-Pivi yields with Bernoulli elastic
+    and rediffused secondaries and floor(deltaTS) or floor(deltaTS) + 1 true secondaries of
+    log-normal energy (sigmaETS 1.041, muETS 2.456).  Device kernels oecl_secem_pass1/2.
+
+    The true-secondary count is limited to 8 per impact (10 secondaries in all): yields with
+    deltaTSHat * (1 + t1) >= 8, i.e. seyMax above about 5.6, are rejected."""
+    model = L.SECEM_STAINLESS
+
+    def __init__(self, particleBoundaryObj, particlesObj, seyMax=-1., reflec=-1.):
+        super().__init__(particleBoundaryObj, particlesObj)
+        self.params = self.stainlessMod(seyMax, reflec)
+        if self.params['deltaTSHat'] * (1. + self.params['t1']) >= 8.:
+            raise ValueError('SecElecEmitter: seyMax %g allows more than 8 true secondaries per impact'
+                             % seyMax)
+        self.secemDesc = self._make_desc(self.params)
+
+    @staticmethod
+    def stainlessMod(seyMax, reflec):
+        """particleEmitter.pyx:334-355."""
+        p = dict(p1EInf=0.07, p1Ehat=0.5, eEHat=0., w=100., p=0.9, e1=0.26, e2=2., q=0.4,
+                 p1RInf=0.74, eR=40., r=1., r1=0.26, r2=2., deltaTSHat=1.22, s=1.813, eHat0=310.,
+                 t3=0.7, t4=1., t1=0.66, t2=0.8, sigmaETS=1.041, muETS=2.456,
+                 theta0max=84. * pi / 180., deltaERmax=0.99)
+        if seyMax >= 0.5:
+            p['deltaTSHat'] = seyMax - p['p1RInf'] - p['p1EInf']
+            if p['deltaTSHat'] < 0.:
+                p['deltaTSHat'] = 1.e-6
+                print('WARNING: Maximum SEY too small. Setting true SEY to zero.')
+        if reflec >= 0.:
+            p['p1Ehat'] = reflec
+        return p
+
+
+class SimpleSecEmitter(_SimpleModelEmitter):
+    """particleEmitter.pyx:489-596 (marked there as for testing only): elastic yield 0.05,
+    rediffused 0.5, true-secondary yield 1.4 s x / (s - 1 + x^s) with x = e0 / 310 eV and
+    s = 1.813, true-secondary energies uniform in [0, 5) eV, and every secondary leaving along
+    the reversed incident velocity.  seyMax and reflec are accepted and ignored, as there."""
+    model = L.SECEM_SIMPLE
+
+    def __init__(self, particleBoundaryObj, particlesObj, seyMax=-1., reflec=-1.):
+        super().__init__(particleBoundaryObj, particlesObj)
+        self.secemDesc = self._make_desc({})
 
 
 def homoLoader(gridObj, particlesObj, partBoundObj, macroParticleCount, weightInit,

@@ -142,9 +142,10 @@ def _setup_restypes():
         lib_restype_set = True
 
 
-def interaction_and_normals(ap, kind, lx, ly, dx, dy, grid=None, dt=None):
+def interaction_and_normals(ap, kind, lx, ly, dx, dy, grid=None, dt=None, cornerFix=False):
     """particleBoundary.pyx calculateInteractionPoint + calculateNormalVectors.
-    Returns (moved absorbed copy, remainingTimeStep, normals, notFound)."""
+    Returns (moved absorbed copy, remainingTimeStep, normals, notFound).  cornerFix: the
+    rectangle's corner cases take the normal of the wall crossed first."""
     _setup_restypes()
     ap = _aos(ap).copy()
     na = ap.shape[0]
@@ -155,7 +156,7 @@ def interaction_and_normals(ap, kind, lx, ly, dx, dy, grid=None, dt=None):
         lw = np.zeros(max(na, 1), np.uint16)
         lib().orc_interaction_rect(_p(ap, _D), ctypes.c_uint32(na), ctypes.c_double(lx),
                                    ctypes.c_double(ly), ctypes.c_double(dx), ctypes.c_double(dy),
-                                   _p(rts, _D), _p(lw, _U16))
+                                   _p(rts, _D), _p(lw, _U16), ctypes.c_int(1 if cornerFix else 0))
         lib().orc_normals_rect(ctypes.c_uint32(na), _p(lw, _U16), _p(nv, _D))
     elif kind == 'ellip':
         lib().orc_interaction_ellip(_p(ap, _D), ctypes.c_uint32(na), ctypes.c_double(lx),

@@ -249,9 +249,11 @@ uint32_t orc_save_absorbed(const double *pd, uint32_t n, const uint16_t *isInsid
     return na;
 }
 
-/* particleBoundary.pyx:108-192 (AbsorbRectangular.calculateInteractionPoint) */
+/* particleBoundary.pyx:108-192 (AbsorbRectangular.calculateInteractionPoint).  cornerFix != 0
+ * relabels the corner cases with the wall crossed first (x wall when tx > ty, else y wall);
+ * cornerFix == 0 keeps the reference's labels. */
 void orc_interaction_rect(double *ap, uint32_t na, double lx, double ly, double dx, double dy,
-                          double *rts, uint16_t *leftWhere)
+                          double *rts, uint16_t *leftWhere, int cornerFix)
 {
     double eps = 1e-6;
     double lxHalf = lx * 0.5 - dx * eps, lyHalf = ly * 0.5 - dy * eps;
@@ -280,6 +282,8 @@ void orc_interaction_rect(double *ap, uint32_t na, double lx, double ly, double 
             tx = (p[0] - lxHalf) / p[2]; ty = (p[1] - lyHalf) / p[3];
             if (tx > ty) { rts[ii] = tx; leftWhere[ii] = 2; } else { rts[ii] = ty; leftWhere[ii] = 3; }
         }
+        if (cornerFix && xi != 1 && yi != 1)
+            leftWhere[ii] = (tx > ty) ? (xi == 0 ? 0 : 1) : (yi == 0 ? 2 : 3);
         p[0] -= rts[ii] * p[2];
         p[1] -= rts[ii] * p[3];
     }

@@ -10,9 +10,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, key, cap,
-                 rule='reference', photo=None):
+                 rule='reference', photo=None, secem=None):
     """photo: (counts, weights, e0, sigmaE) of a photoelectron source, added after each
-    step's secondaries (oracle.photo_oracle)."""
+    step's secondaries (oracle.photo_oracle).  secem: (model, parameters) of a simpler emitter
+    (oracle.secem_oracle) used in place of the Furman one."""
     from oracle.grid_oracle import OraclePoisson
     ops = OraclePoisson(g)
     Q = -1.602176565e-19
@@ -33,12 +34,20 @@ def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, 
             moved, _, nv, _ = orc.interaction_and_normals(ab, 'ellip', g.getLx(), g.getLy(),
                                                           g.getDx(), g.getDy())
             na = ab.shape[0]
-            U = np.empty((na, 36))
-            for b in range(18):
-                U[:, 2 * b:2 * b + 2] = orc.words_to_uniforms(
-                    orc.philox_words(na, 0, b, it, 1, key[0], key[1]))
-            from oracle.furman_oracle import generate_secondaries
-            sec, _, _ = generate_secondaries(moved, nv, U[:, :35], P)
+            if secem is not None:
+                from oracle import secem_oracle as so
+                U = np.empty((na, so.N_UNIFORMS))
+                for b in range(so.N_UNIFORMS // 2):
+                    U[:, 2 * b:2 * b + 2] = orc.words_to_uniforms(
+                        orc.philox_words(na, 0, b, it, 7, key[0], key[1]))
+                sec, _ = so.generate_secondaries(moved, nv, U, secem[0], secem[1])
+            else:
+                U = np.empty((na, 36))
+                for b in range(18):
+                    U[:, 2 * b:2 * b + 2] = orc.words_to_uniforms(
+                        orc.philox_words(na, 0, b, it, 1, key[0], key[1]))
+                from oracle.furman_oracle import generate_secondaries
+                sec, _, _ = generate_secondaries(moved, nv, U[:, :35], P)
         if photo is not None and photo[0][it] > 0:
             from oracle.photo_oracle import photo_emit
             ph = photo_emit(1, g.getLx(), g.getLy(), g.getDx(), g.getDy(), int(photo[0][it]), it,
@@ -212,3 +221,55 @@ def test_photoelectrons_in_the_step_match_oracle(orc, native):
         assert st['photo_total'] == int(cnt.sum())
     else:
         assert sum(b.history['photo']) == int(cnt.sum())
+
+
+@pytest.mark.parametrize('model', ['stainless', 'simple'])
+@pytest.mark.parametrize('form', ['reference', 'fused', 'native'])
+def test_simple_emitters_in_the_step_match_oracle(orc, model, form):
+    """SecElecEmitter / SimpleSecEmitter as the step's emitter in the three step forms."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import SecElecEmitter, SimpleSecEmitter
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle import secem_oracle as so
+    g = Grid(64, 48, 0.09, 0.07, 1)
+    n = 20000
+    pd0 = random_particles(g, n, seed=14, wall='ellip', vscale=3e6)
+    pd0[:, 5] = 5e4
+    dt = 2e-11
+    randomGen.seed(91)
+    key = randomGen.key()
+    rule = 'fast' if form == 'native' else 'reference'
+    p = Particles('electrons', 'variableWeight', ordering=rule)
+    p.setParticleData(pd0)
+    p.setBConst([0., 0.5, 0.])
+    pb = AbsorbElliptical(g, p)
+    em = (SecElecEmitter if model == 'stainless' else SimpleSecEmitter)(pb, p, seyMax=2.0)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    beam = GaussianBunchTrain(g, dt, nParticles=1e11, tBunchSpacing=25e-9, radiusSigma=2e-3,
+                              nBunches=1, bunchLengthSigma=0.003)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
+    if form == 'native':
+        b.native_setup(absCap=8192, secCap=16384, maxSteps=64)
+    nsteps = 8
+    ref = _oracle_loop(orc, g, pd0.copy(), nsteps, dt, 1, np.array([0., 0.5, 0.]),
+                       beam.getTransversalProfile(), beam.getScale, None, key, n, rule=rule,
+                       secem=(model, so.stainless(seyMax=2.0)))
+    for it in range(nsteps):
+        if form == 'native':
+            b.step_native()
+            b.sync()
+        elif form == 'fused':
+            b.step()
+        else:
+            b.step_reference()
+        got = p.getParticleData().cpu().numpy()
+        assert got.shape == ref[it].shape, (it, got.shape, ref[it].shape)
+        scale = np.abs(ref[it]).max(axis=0)
+        err = (np.abs(got - ref[it]) / scale).max()
+        assert err < 1e-11, (it, err)
+    assert ref[-1].shape[0] != n

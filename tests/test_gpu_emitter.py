@@ -62,6 +62,87 @@ def test_furman_injected_uniforms(orc, sey):
     assert np.allclose(sec[:, 2:5], ref[:, 2:5], rtol=1e-9, atol=1e-12 * scale)
 
 
+def _secem(ap, nv, model, sey=2.0, dtype=torch.float64):
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import SecElecEmitter, SimpleSecEmitter
+    from openecloud_amd.particles import Particles
+    g = Grid(32, 32, 0.09, 0.07, 1)
+    p = Particles('electrons', 'variableWeight', dtype=dtype)
+    pb = AbsorbElliptical(g, p)
+    pb.absorbedParticles = torch.as_tensor(ap).to(dtype).cuda()
+    pb.normalVectors = torch.as_tensor(nv).to(dtype).cuda()
+    pb.absorbedMacroParticleCount = ap.shape[0]
+    cls = SecElecEmitter if model == 'stainless' else SimpleSecEmitter
+    return cls(pb, p, seyMax=sey)
+
+
+@pytest.mark.parametrize('model', ['stainless', 'simple'])
+@pytest.mark.parametrize('dtype', ['f64', 'f32'])
+def test_secem_injected_uniforms(orc, model, dtype):
+    """SecElecEmitter / SimpleSecEmitter on the device against the oracle with the same table:
+    counts exact, positions and weights exact, velocities to 1e-12 (FP64) / 1e-6 (FP32) of
+    the largest.  Impacts include exact normal incidence (the reference's NaN angle)."""
+    from oracle import secem_oracle as so
+    n = 4000
+    ap, nv = _impacts(n, seed=41 if model == 'stainless' else 43, emax=1500.)
+    ap[:50, 2] = -np.sqrt(2 * 300. * so.ELEMENTARY_CHARGE / so.M_E) * nv[:50, 0]
+    ap[:50, 3] = -np.sqrt(2 * 300. * so.ELEMENTARY_CHARGE / so.M_E) * nv[:50, 1]
+    ap[:50, 4] = 0.
+    ap[50:100, 0:5] = ap[50:100, 0:5] * 1e-3      # slow impacts: the true-energy fallback
+    tdt = torch.float64 if dtype == 'f64' else torch.float32
+    if dtype == 'f32':
+        ap = ap.astype(np.float32).astype(np.float64)
+        nv = nv.astype(np.float32).astype(np.float64)
+    U = np.random.default_rng(5).random((n, so.N_UNIFORMS))
+    em = _secem(ap, nv, model, dtype=tdt)
+    sec = em.generateSecondaries(uniforms=U).cpu().numpy().astype(np.float64)
+    P = so.stainless(seyMax=2.0)
+    if dtype == 'f64':
+        ref, ns = so.generate_secondaries(ap, nv, U, model, P)
+        assert np.array_equal(em.getNSec().cpu().numpy(), ns)
+        assert sec.shape == ref.shape and sec.shape[0] > n // 2
+        assert np.array_equal(sec[:, [0, 1, 5]], ref[:, [0, 1, 5]])
+        scale = np.abs(ref[:, 2:5]).max()
+        assert np.allclose(sec[:, 2:5], ref[:, 2:5], rtol=1e-12, atol=1e-13 * scale)
+    else:
+        # FP32 storage: the device computes in FP64 from the FP32 values, as the oracle does
+        ref, ns = so.generate_secondaries(ap, nv, U, model, P)
+        assert np.array_equal(em.getNSec().cpu().numpy(), ns)
+        assert sec.shape == ref.shape
+        scale = np.abs(ref[:, 2:5]).max()
+        assert np.allclose(sec, ref.astype(np.float32).astype(np.float64), rtol=1e-6, atol=1e-6 * scale)
+
+
+@pytest.mark.parametrize('model', ['stainless', 'simple'])
+def test_secem_philox_stream_matches_oracle(orc, model):
+    """Device Philox draws (purpose 7) equal the oracle's words for the same (key, counter)."""
+    from oracle import secem_oracle as so
+    from openecloud_amd import randomGen
+    n = 2000
+    ap, nv = _impacts(n, seed=9, emax=1000.)
+    randomGen.seed(4321)
+    k0, k1 = randomGen.key()
+    em = _secem(ap, nv, model)
+    sec = em.generateSecondaries(step=23).cpu().numpy()
+    U = np.empty((n, so.N_UNIFORMS))
+    for b in range(so.N_UNIFORMS // 2):
+        U[:, 2 * b:2 * b + 2] = orc.words_to_uniforms(orc.philox_words(n, 0, b, 23, 7, k0, k1))
+    ref, ns = so.generate_secondaries(ap, nv, U, model, so.stainless(seyMax=2.0))
+    assert np.array_equal(em.getNSec().cpu().numpy(), ns)
+    scale = np.abs(ref[:, 2:5]).max()
+    assert np.allclose(sec[:, 2:5], ref[:, 2:5], rtol=1e-12, atol=1e-13 * scale)
+
+
+def test_secem_rejects_yields_beyond_eight_true_secondaries():
+    from openecloud_amd.particleEmitter import SecElecEmitter
+    ap, nv = _impacts(4, seed=1)
+    _secem(ap, nv, 'stainless', sey=5.5)
+    with pytest.raises(ValueError):
+        _secem(ap, nv, 'stainless', sey=6.0)
+    assert SecElecEmitter.stainlessMod(0.6, -1.)['deltaTSHat'] == 1e-6
+
+
 def test_furman_philox_stream_matches_oracle(orc):
     """Device Philox draws equal the oracle's words for the same (key, counter)."""
     from oracle.furman_oracle import generate_secondaries, material
