@@ -257,6 +257,13 @@ int oecl_sort_by_column(int dtype, const void *data, void *spare, uint64_t strid
 int oecl_sort_by_cell(const oecl_grid_t *g, int dtype, const void *data, void *spare,
                       uint64_t stride, uint32_t n, uint64_t *keys, uint32_t *perm, void *work,
                       void *stream);
+/* The cell sort of a batched particle block (oecl_runs_t layout): every slot of every run's
+ * segment is ordered by (run, bin) into `spare`, live particles first, so each run stays in
+ * its segment.  The bin is the cell (ix, iy) >> binShift in row-major (order 0) or Morton
+ * (order 1) order.  keys/perm hold stride entries; work is oecl_sort_work_bytes(stride). */
+int oecl_sort_runs_by_cell(const oecl_grid_t *g, int dtype, const void *data, void *spare, uint64_t stride,
+                           uint32_t R, const uint32_t *segOff, const uint32_t *blockRun, const uint32_t *nDev,
+                           int order, int binShift, uint64_t *keys, uint32_t *perm, void *work, void *stream);
 /* dst[:, i] = src[:, perm[i]], i < n. */
 int oecl_permute_particles(int dtype, const void *src, void *dst, uint64_t stride,
                            const uint32_t *perm, uint32_t n, void *stream);
@@ -336,6 +343,9 @@ int oecl_fastpoisson_create(uint32_t nxExt, uint32_t nyExt, double cxu, double c
                             const int32_t *sIdx, const int32_t *wPtr, const int32_t *wCol,
                             const double *wVal, const uint8_t *maskDev, oecl_fastpoisson **out,
                             double *capMax);
+/* Room in the handle for R right-hand sides solved together (the batched step,
+ * oecl_step_run_batch); allocates, so call it before capturing a batched step into a graph. */
+int oecl_fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R);
 /* phi (npExt, zero on non-inside nodes) = A^-1 q on inside nodes. */
 int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, double *phi, void *stream);
 int oecl_fastpoisson_destroy(oecl_fastpoisson *h);
@@ -393,6 +403,30 @@ typedef struct {
 } oecl_step_args;
 
 uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap);
+
+/* A batched step: R independent buildups (a parameter scan) on one grid, one wall and one
+ * direct Poisson operator, all advanced by one call (or one replay of a captured graph).  No
+ * counterpart in the reference, which runs one buildup per process.  Run r's particles live in
+ * [segOff[r], segOff[r] + nDev[r]) of the one SoA block of oecl_step_args (stride = segOff[R]);
+ * its segment [segOff[r], segOff[r + 1]) is a multiple of 256 particles.  blockRun[b] is the
+ * run of particles [256 b, 256 b + 256).  Run r's hits use rows [absOff[r], absOff[r + 1]) of
+ * absorbed, normals, remaining, nSec, impact (absCap = absOff[R]); secOff holds absOff[R] + R
+ * entries (run r's from absOff[r] + r).  Per run, one after another: nDev, stats[8],
+ * countHist[histCap], beamScale[nScale], rho, q, phi (npExt each), E (2 npExt).  Emitter: the
+ * Furman parameters furman[r] (device), or oecl_step_args.secem for all runs.  Philox keys:
+ * keys[2r], keys[2r + 1].  secondaries may be NULL (no copy of the emitted rows); the
+ * photoelectron source is not available here. */
+typedef struct {
+    uint32_t R;
+    const uint32_t *segOff;       /* device [R + 1] */
+    const uint32_t *blockRun;     /* device [segOff[R] / 256] */
+    const uint32_t *absOff;       /* device [R + 1] */
+    const oecl_furman_t *furman;  /* device [R] */
+    const uint32_t *keys;         /* device [2 R] */
+} oecl_runs_t;
+uint64_t oecl_step_batch_work_bytes(uint64_t capacity, uint32_t absTotal, uint32_t R);
+int oecl_step_run_batch(const oecl_grid_t *g, const oecl_wall_t *wall, const oecl_runs_t *runs,
+                        oecl_fastpoisson *fp, const oecl_step_args *a, void *stream);
 /* hipEvent helpers for callers without a HIP binding. */
 int oecl_event_create(void **ev);
 int oecl_event_destroy(void *ev);

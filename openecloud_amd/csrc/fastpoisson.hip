@@ -52,9 +52,11 @@ struct oecl_fastpoisson {
     int32_t *ePos;            // [m] entry of row a in the by-column order
     uint32_t *colOff;         // [m] i_a * bx of entry e (by-column order)
     double *CinvT;            // m*m, C^-1 row-major (row a contiguous)
-    double *b, *t1, *t2;      // nb each
-    double *wy, *zE;          // m each (zE in the by-column order)
+    double *b, *t1, *t2;      // nb each, per right-hand side
+    double *wy, *zE;          // m each (zE in the by-column order), per right-hand side
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
+    uint32_t rCap;            // right-hand sides the vectors above hold
+    char *batchMem;           // their own allocation once rCap > 1
 };
 
 namespace {
@@ -71,14 +73,16 @@ namespace {
 // registers while the current one is multiplied.
 //   A = F_z (col-major K_z x K_z, lda = K_0, F_z at z * K_0 * K_0), X = n x N (ldx)
 // EPI 0 stores C (ldc), EPI 2 scatters into the extended-grid potential (masked; the rest
-// of the grid is written by the caller).
+// of the grid is written by the caller).  Several right-hand sides are simply more columns:
+// for EPI 2, column gn is line gn % by of the potential gn / by (np nodes apart).
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int GT = 32, GNW = 8, GKC = 64, GNT = 64 * GNW, GPER = GT * GKC / GNT;
 
 template <int EPI>
 __global__ void __launch_bounds__(GNT)
     k_dst_gemm(int N, int n, const double *__restrict__ A, const double *__restrict__ X, int ldx,
-               double *__restrict__ C, int ldc, uint32_t nx, const uint8_t *__restrict__ mask)
+               double *__restrict__ C, int ldc, uint32_t nx, const uint8_t *__restrict__ mask, uint32_t by,
+               uint32_t np)
 {
     constexpr int LDSN = (2 * GKC * 33 > GNW * GT * 33) ? 2 * GKC * 33 : GNW * GT * 33;
     __shared__ double sm[LDSN];
@@ -160,8 +164,9 @@ __global__ void __launch_bounds__(GNT)
         if (EPI == 0) {
             C[om + (size_t)gn * ldc] = s;
         } else {
-            const uint32_t p = (uint32_t)(om + 2) + (uint32_t)(gn + 2) * nx;
-            C[p] = mask[p] ? s : 0.;
+            // column gn is line gn % by of right-hand side gn / by
+            const uint32_t p = (uint32_t)(om + 2) + ((uint32_t)gn % by + 2) * nx;
+            C[p + (size_t)((uint32_t)gn / by) * np] = mask[p] ? s : 0.;
         }
     }
 }
@@ -186,8 +191,9 @@ __global__ void k_gather_box_zero(uint32_t nx, uint32_t ny, uint32_t bx, const u
 // lane its incoming value, and the lane then runs its rows.  The order of operations is
 // fixed, so results are reproducible.  UNIT: the right-hand side is Sx e_s for the irregular
 // node s = sIdx[a0 + blockIdx.y] (row j_s = Sx[:, i_s], zero elsewhere), for building the
-// capacitance matrix with exactly the operator the solve applies; batch element blockIdx.y
-// is written at u + blockIdx.y * nb.
+// capacitance matrix with exactly the operator the solve applies.  Batch element blockIdx.y
+// (a unit column, or a right-hand side of a batched solve) is read from r + blockIdx.y * nb
+// and written at u + blockIdx.y * nb.
 template <int C, bool UNIT>
 __global__ void __launch_bounds__(256)
     k_thomas_y(uint32_t bx, uint32_t by, double cyu, const double *__restrict__ invw,
@@ -197,7 +203,8 @@ __global__ void __launch_bounds__(256)
     const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (q >= bx) return;
     const size_t nb = (size_t)bx * by;
-    double *ub = u + (UNIT ? (size_t)blockIdx.y * nb : 0);
+    double *ub = u + (size_t)blockIdx.y * nb;
+    const double *rb = UNIT ? r : r + (size_t)blockIdx.y * nb;
     uint32_t js = 0;
     double rs = 0.;
     if (UNIT) {
@@ -213,7 +220,7 @@ __global__ void __launch_bounds__(256)
         const size_t o = q + (size_t)j * bx;
         iw[t] = j < by ? invw[o] : 0.;
         double rv = 0.;
-        if (j < by) rv = UNIT ? (j == js ? rs : 0.) : r[o];
+        if (j < by) rv = UNIT ? (j == js ? rs : 0.) : rb[o];
         g[t] = rv;
     }
     // forward: g_j = a_j g_{j-1} + b_j with a_j = cyu iw_j, b_j = r_j iw_j
@@ -278,7 +285,8 @@ __device__ __forceinline__ double wave_sum(double s)
 // a row has at most WMAX entries (the node and its four neighbours), all of whose dot
 // products are accumulated together so that every load of the row is in flight at once.
 // CAP: batch element c = blockIdx.y holds R^-1 of the unit vector of row a0 + c, and the
-// result is column a0 + c of C = I + W R^-1 U, stored row-major (C^T column-major).
+// result is column a0 + c of C = I + W R^-1 U, stored row-major (C^T column-major).  Else
+// blockIdx.y is the right-hand side of a batched solve: wy[blockIdx.y * m + a].
 constexpr int WMAX = 5;
 
 template <int CQ, bool CAP>
@@ -289,7 +297,7 @@ __global__ void __launch_bounds__(256)
 {
     const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (a >= m) return;
-    const double *yb = yh + (CAP ? (size_t)blockIdx.y * nb : 0);
+    const double *yb = yh + (size_t)blockIdx.y * nb;
     const int32_t k0 = ptr[a], nk = ptr[a + 1] - k0;  // 1 <= nk <= WMAX (the node itself first)
     double acc[WMAX];
 #pragma unroll
@@ -317,19 +325,22 @@ __global__ void __launch_bounds__(256)
             const uint32_t c = a0 + blockIdx.y;
             wy[(size_t)a * m + c] = s + (a == c ? 1. : 0.);
         } else {
-            wy[a] = s;
+            wy[(size_t)blockIdx.y * m + a] = s;
         }
     }
 }
 
 // z = C^-1 wy, C^-1 row-major: one wave per row a, 8 independent partial sums per lane;
-// z_a is stored at zE[ePos[a]], its place in the by-column order of the update.
+// z_a is stored at zE[ePos[a]], its place in the by-column order of the update.  blockIdx.y:
+// the right-hand side (m apart in wy and zE).
 __global__ void __launch_bounds__(256)
     k_cap_z(uint32_t m, const double *__restrict__ CinvT, const double *__restrict__ wy,
             const int32_t *__restrict__ ePos, double *__restrict__ zE)
 {
     const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (a >= m) return;
+    wy += (size_t)blockIdx.y * m;
+    zE += (size_t)blockIdx.y * m;
     const double *row = CinvT + (size_t)a * m;
     double p[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
     uint32_t k = lane;
@@ -346,15 +357,17 @@ __global__ void __launch_bounds__(256)
 // hat b[q, j] -= sum over rows a with j_a = j of z_a Sx[q, i_a]: one thread per (q, column
 // of the distinct-column list), rows taken in order (deterministic).  Entry e of the column
 // order carries zE[e] and the offset colOff[e] = i_a * bx of its Sx column, so the loads of
-// different entries are independent.
+// different entries are independent.  blockIdx.z: the right-hand side (m apart in zE, nb in bh).
 __global__ void __launch_bounds__(256)
     k_cap_update(uint32_t nJ, uint32_t bx, const int32_t *__restrict__ jCol, const int32_t *__restrict__ jStart,
                  const uint32_t *__restrict__ colOff, const double *__restrict__ zE, const double *__restrict__ Sx,
-                 double *__restrict__ bh)
+                 double *__restrict__ bh, uint32_t m, size_t nb)
 {
     const uint32_t q = blockIdx.x * 64 + (threadIdx.x & 63);
     const uint32_t c = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (q >= bx || c >= nJ) return;
+    zE += (size_t)blockIdx.z * m;
+    bh += (size_t)blockIdx.z * nb;
     const int32_t e0 = jStart[c], e1 = jStart[c + 1];
     double s[4] = {0., 0., 0., 0.};
     int32_t e = e0;
@@ -391,11 +404,12 @@ std::vector<double> dst1(uint32_t n)
     return S;
 }
 
-// Thomas scan over all x modes (unit: cnt batched unit right-hand sides from row a0 on).
+// Thomas scan over all x modes (unit: cnt batched unit right-hand sides from row a0 on; else
+// cnt right-hand sides nb apart).
 int launch_thomas(const oecl_fastpoisson *h, bool unit, const double *r, double *u, uint32_t a0, uint32_t cnt,
                   hipStream_t s)
 {
-    const dim3 g((h->bx + 3) / 4, unit ? cnt : 1), b(256);
+    const dim3 g((h->bx + 3) / 4, cnt), b(256);
     const uint32_t c = (h->by + 63) / 64;
 #define OECL_THOMAS(C)                                                                                   \
     do {                                                                                                 \
@@ -415,11 +429,12 @@ int launch_thomas(const oecl_fastpoisson *h, bool unit, const double *r, double 
     return OECL_OK;
 }
 
-// W Sx hat y over all rows (cap: cnt batched unit columns from row a0 on, into C).
+// W Sx hat y over all rows (cap: cnt batched unit columns from row a0 on, into C; else cnt
+// right-hand sides).
 int launch_cap_wy(const oecl_fastpoisson *h, bool cap, const double *yh, double *out, uint32_t a0, uint32_t cnt,
                   hipStream_t s)
 {
-    const dim3 g((h->m + 3) / 4, cap ? cnt : 1), b(256);
+    const dim3 g((h->m + 3) / 4, cnt), b(256);
     const uint32_t c = (h->bx + 63) / 64;
 #define OECL_WY(CQ)                                                                                        \
     do {                                                                                                   \
@@ -470,6 +485,8 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->m = m;
     h->cyu = cyu;
     h->mask = maskDev;
+    h->rCap = 1;
+    h->batchMem = nullptr;
     // rows a grouped by their box column j (stable), for the per-column update
     std::vector<int32_t> jA(m), jCol, jStart;
     for (uint32_t a = 0; a < m; ++a) jA[a] = (int32_t)a;
@@ -608,35 +625,60 @@ FpBox fastpoisson_box(const oecl_fastpoisson *h)
     return FpBox{h->b, h->mask, h->nx, h->ny, h->bx};
 }
 
+int fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R)
+{
+    if (R <= h->rCap) return OECL_OK;
+    const uint64_t nb = h->nb, m = h->m;
+    char *mem = nullptr;
+    OECL_HIP(hipMalloc(&mem, 8ull * R * (3 * nb + 2 * m) + 1024), "fastpoisson batch alloc");
+    if (h->batchMem) (void)hipFree(h->batchMem);
+    h->batchMem = mem;
+    h->b = (double *)mem;
+    h->t1 = h->b + R * nb;
+    h->t2 = h->t1 + R * nb;
+    h->wy = h->t2 + R * nb;
+    h->zE = h->wy + R * m;
+    h->rCap = R;
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R)
+{
+    if (!h || R == 0) return OECL_ERR_ARG;
+    return fastpoisson_reserve(h, R);
+}
+
 OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, double *phi, void *stream)
 {
     if (!h || !q || !phi) return OECL_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_gather_box_zero, dim3(oecl_blocks(h->np, 256)), dim3(256), 0, s, h->nx, h->ny, h->bx,
                        h->mask, q, h->b, phi);
-    return fastpoisson_solve_box(h, phi, s);
+    return fastpoisson_solve_box(h, phi, 1, s);
 }
 
-int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, hipStream_t s)
+int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStream_t s)
 {
-    const int bx = h->bx, by = h->by, kx = (bx + 1) / 2;
-    const dim3 gg((kx + GT - 1) / GT, (by + GT - 1) / GT, 2), gb(GNT);
+    if (R < 1 || R > h->rCap || (uint64_t)h->by * R > 0x7FFFFFFFull) return OECL_ERR_ARG;
+    const int bx = h->bx, by = h->by, kx = (bx + 1) / 2, N = by * (int)R;
+    const dim3 gg((kx + GT - 1) / GT, (N + GT - 1) / GT, 2), gb(GNT);
     // hat b = Sx b
-    hipLaunchKernelGGL(k_dst_gemm<0>, gg, gb, 0, s, by, bx, h->Fx, h->b, bx, h->t1, bx, 0u, nullptr);
+    hipLaunchKernelGGL(k_dst_gemm<0>, gg, gb, 0, s, N, bx, h->Fx, h->b, bx, h->t1, bx, 0u, nullptr, 1u, 0u);
     if (h->m) {
         // hat y = T^-1 hat b ; wy = W Sx hat y ; hat b -= (U C^-1 wy) in x-transformed form
-        int rc = launch_thomas(h, false, h->t1, h->t2, 0, 1, s);
+        int rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
         if (rc != OECL_OK) return rc;
-        rc = launch_cap_wy(h, false, h->t2, h->wy, 0, 1, s);
+        rc = launch_cap_wy(h, false, h->t2, h->wy, 0, R, s);
         if (rc != OECL_OK) return rc;
-        hipLaunchKernelGGL(k_cap_z, dim3((h->m + 3) / 4), dim3(256), 0, s, h->m, h->CinvT, h->wy, h->ePos, h->zE);
-        hipLaunchKernelGGL(k_cap_update, dim3((h->bx + 63) / 64, (h->nJ + 3) / 4), dim3(256), 0, s, h->nJ, h->bx,
-                           h->jCol, h->jStart, h->colOff, h->zE, h->Sx, h->t1);
+        hipLaunchKernelGGL(k_cap_z, dim3((h->m + 3) / 4, R), dim3(256), 0, s, h->m, h->CinvT, h->wy, h->ePos, h->zE);
+        hipLaunchKernelGGL(k_cap_update, dim3((h->bx + 63) / 64, (h->nJ + 3) / 4, R), dim3(256), 0, s, h->nJ, h->bx,
+                           h->jCol, h->jStart, h->colOff, h->zE, h->Sx, h->t1, h->m, (size_t)h->nb);
     }
-    // hat x = T^-1 hat b ; phi = Sx hat x scattered into the extended grid
-    int rc = launch_thomas(h, false, h->t1, h->t2, 0, 1, s);
+    // hat x = T^-1 hat b ; phi = Sx hat x scattered into the extended grid (np apart per side)
+    int rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
     if (rc != OECL_OK) return rc;
-    hipLaunchKernelGGL(k_dst_gemm<2>, gg, gb, 0, s, by, bx, h->Fx, h->t2, bx, phi, 0, h->nx, h->mask);
+    hipLaunchKernelGGL(k_dst_gemm<2>, gg, gb, 0, s, N, bx, h->Fx, h->t2, bx, phi, 0, h->nx, h->mask,
+                       (uint32_t)by, h->np);
     OECL_CHECK_LAUNCH("oecl_fastpoisson_solve");
     return OECL_OK;
 }
@@ -645,6 +687,7 @@ OECL_EXPORT int oecl_fastpoisson_destroy(oecl_fastpoisson *h)
 {
     if (!h) return OECL_OK;
     (void)hipFree(h->Sx);
+    if (h->batchMem) (void)hipFree(h->batchMem);
     rocblas_destroy_handle(h->blas);
     delete h;
     return OECL_OK;

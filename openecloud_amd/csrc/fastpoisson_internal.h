@@ -7,12 +7,16 @@
 struct oecl_fastpoisson;
 
 struct FpBox {
-    double *b;             // bx * by, column-major box right-hand side (x fastest)
+    double *b;             // bx * by per right-hand side, column-major (x fastest), sides nb apart
     const uint8_t *mask;   // inside flags of the extended grid
     uint32_t nx, ny, bx;   // extended grid, box width (nx - 4)
 };
 
 FpBox fastpoisson_box(const oecl_fastpoisson *h);
-// Solve with b already holding q on inside box nodes and 0 elsewhere in the box; phi is
-// written on the box nodes only (nodes outside the box must be zero already).
-int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, hipStream_t s);
+// Room for R right-hand sides in the solver's box vectors (reallocates them when R grows, so
+// call it before fastpoisson_box and before any graph capture).
+int fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R);
+// Solve R systems with b already holding q on inside box nodes and 0 elsewhere in the box;
+// phi (R potentials np apart) is written on the box nodes only (nodes outside the box must be
+// zero already).
+int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStream_t s);

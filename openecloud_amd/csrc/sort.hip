@@ -226,6 +226,84 @@ OECL_EXPORT int oecl_sort_by_cell(const oecl_grid_t *g, int dtype, const void *d
     return oecl_permute_particles(dtype, data, spare, stride, perm, n, stream);
 }
 
+namespace {
+
+// 16 low bits of v spread to the even bit positions (Morton interleave).
+__device__ __forceinline__ uint32_t spread_bits(uint32_t v)
+{
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// Sort key of slot i of a batched particle block: (run << cellBits) | bin, where the bin of a
+// live particle is its cell (ix, iy) >> binShift in row-major (order 0) or Morton (order 1)
+// order, and a slot past the run's live count takes the largest bin, so that it sorts to the
+// end of its own run's segment.
+template <typename T>
+__global__ void k_run_cell_keys(oecl_grid_t g, Soa<T> p, uint32_t cap, const uint32_t *__restrict__ segOff,
+                                const uint32_t *__restrict__ blockRun, const uint32_t *__restrict__ nDev, int order,
+                                int binShift, int cellBits, uint64_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    const uint32_t r = blockRun[i / 256];
+    uint64_t bin = (1ull << cellBits) - 1;
+    if (i - segOff[r] < nDev[r]) {
+        const Weights<double> w = grid_weights<double>((double)p(0, i), (double)p(1, i), 1. / g.dx, 1. / g.dy,
+                                                       g.lxHalfExt, g.lyHalfExt, g.nx, g.ny);
+        const uint32_t bx = w.ix >> binShift, by = w.iy >> binShift;
+        bin = order ? (uint64_t)(spread_bits(bx) | (spread_bits(by) << 1))
+                    : (uint64_t)bx + (uint64_t)by * (((g.nx - 1) >> binShift) + 1);
+    }
+    keys[i] = ((uint64_t)r << cellBits) | bin;
+    vals[i] = i;
+}
+
+}  // namespace
+
+OECL_EXPORT int oecl_sort_runs_by_cell(const oecl_grid_t *g, int dtype, const void *data, void *spare, uint64_t stride,
+                                       uint32_t R, const uint32_t *segOff, const uint32_t *blockRun,
+                                       const uint32_t *nDev, int order, int binShift, uint64_t *keys, uint32_t *perm,
+                                       void *work, void *stream)
+{
+    if (!g || !data || !spare || !segOff || !blockRun || !nDev || !keys || !perm || !work || R == 0) return OECL_ERR_ARG;
+    if (order < 0 || order > 1 || binShift < 0 || binShift > 8 || stride % 256 || stride > 0xFFFFFFFFull)
+        return OECL_ERR_ARG;
+    if (g->nx > 65536 || g->ny > 65536) return OECL_ERR_ARG;
+    const uint32_t cap = (uint32_t)stride;
+    if (cap == 0) return OECL_OK;
+    // bits of the largest live bin + 1 (the dead key), and of the run index
+    const uint64_t bx = ((g->nx - 1) >> binShift) + 1, by = ((g->ny - 1) >> binShift) + 1;
+    uint64_t maxBin;
+    if (order) {
+        int b = 0;
+        while ((1ull << b) < (bx > by ? bx : by)) ++b;
+        maxBin = (1ull << (2 * b)) - 1;
+    } else {
+        maxBin = bx * by - 1;
+    }
+    int cellBits = 1, runBits = 0;
+    while ((1ull << cellBits) <= maxBin + 1) ++cellBits;
+    while ((1ull << runBits) < R) ++runBits;
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 gr(oecl_blocks(cap, 256)), bl(256);
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_run_cell_keys<double>, gr, bl, 0, s, *g, Soa<double>{(double *)data, stride}, cap, segOff,
+                           blockRun, nDev, order, binShift, cellBits, keys, perm);
+    else if (dtype == OECL_F32)
+        hipLaunchKernelGGL(k_run_cell_keys<float>, gr, bl, 0, s, *g, Soa<float>{(float *)data, stride}, cap, segOff,
+                           blockRun, nDev, order, binShift, cellBits, keys, perm);
+    else
+        return OECL_ERR_ARG;
+    int rc = oecl_radix_sort_u64(keys, perm, cap, cellBits + runBits, work, stream);
+    if (rc) return rc;
+    return oecl_permute_particles(dtype, data, spare, stride, perm, cap, stream);
+}
+
 // dst[:, i] = src[:, perm[i]] for i < n.
 OECL_EXPORT int oecl_permute_particles(int dtype, const void *src, void *dst, uint64_t stride,
                                        const uint32_t *perm, uint32_t n, void *stream)

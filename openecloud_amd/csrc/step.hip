@@ -43,12 +43,42 @@ constexpr uint32_t MAXSEC = 10;   // secondaries per impact (Furman M)
 // step scratch counters
 enum { M_HOLES = 0, M_NZ = 1, M_PHOTO = 2, M_COUNT = 16 };
 
-// Start of a step, one thread per node:
+// The runs of a step: R independent buildups sharing the grid, the wall and the Poisson
+// operator (R = 1 for the plain step).  Run r's particles live in [seg0(r), seg0(r) + nDev[r])
+// of one SoA block, inside its segment [seg0(r), seg1(r)), a multiple of PB, so that every
+// push block belongs to one run; its hits use rows [abs0(r), abs0(r) + absN(r)) of the
+// per-hit arrays.  Grid arrays (rho, q, phi: np nodes, E: 2 np) and per-run state (nDev,
+// stats[8], misc[M_COUNT], countHist[histCap], beam scale[nScale]) are stored run after run.
+// A single run uses no tables: segment [0, cap), hit rows [0, absCap), the emitter parameters
+// and Philox key passed by value.
+struct Runs {
+    uint32_t R;
+    uint32_t absCap;
+    uint64_t cap;
+    const uint32_t *segOff;       // [R + 1] or nullptr
+    const uint32_t *blockRun;     // [cap / PB] or nullptr
+    const uint32_t *absOff;       // [R + 1] or nullptr
+    const oecl_furman_t *furman;  // [R] or nullptr
+    const uint32_t *keys;         // [2 R] or nullptr
+    __device__ __forceinline__ uint64_t seg0(uint32_t r) const { return segOff ? segOff[r] : 0; }
+    __device__ __forceinline__ uint64_t seg1(uint32_t r) const { return segOff ? segOff[r + 1] : cap; }
+    __device__ __forceinline__ uint32_t abs0(uint32_t r) const { return absOff ? absOff[r] : 0; }
+    __device__ __forceinline__ uint32_t absN(uint32_t r) const
+    {
+        return absOff ? absOff[r + 1] - absOff[r] : absCap;
+    }
+    __device__ __forceinline__ uint32_t block_run(uint32_t b) const { return blockRun ? blockRun[b] : 0; }
+    __device__ __forceinline__ uint32_t key0(uint32_t r, uint32_t k) const { return keys ? keys[2 * r] : k; }
+    __device__ __forceinline__ uint32_t key1(uint32_t r, uint32_t k) const { return keys ? keys[2 * r + 1] : k; }
+};
+
+// Start of a step, one thread per node, blockIdx.y = run:
 //   the Poisson right-hand side q = rho + s(step) * beam profile (example01_basic.py:81),
 //   masked and written straight into the direct solver's box vector (inside nodes keep q,
 //   other box nodes 0; the potential outside the box is set to 0);
 //   rho = 0 for this step's deposit; thread 0 records the particle count of the step;
-//   the step's photoelectrons (a per-step table) are generated into their slots.
+//   the step's photoelectrons (a per-step table, single run only) are generated into their
+//   slots.
 template <typename T>
 __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double *__restrict__ table,
                             uint32_t nScale, const uint32_t *__restrict__ stepDev,
@@ -60,7 +90,16 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
                             T *__restrict__ photoSlots, uint32_t *stats)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t s = *stepDev;
+    const uint32_t s = *stepDev, run = blockIdx.y;
+    rho += (size_t)run * np;
+    phi += (size_t)run * np;
+    box.b += (size_t)run * box.bx * (box.ny - 4);
+    if (table) table += (size_t)run * nScale;
+    if (hist) hist += (size_t)run * histCap;
+    misc += run * M_COUNT;
+    nDev += run;
+    stats += 8 * run;
+    if (run) photoCount = nullptr;
     uint32_t nph = 0;
     if (photoCount && s < nPhoto) {
         nph = photoCount[s];
@@ -102,19 +141,28 @@ __device__ __forceinline__ void wave_holes(bool dead, uint32_t i, uint32_t *__re
     }
 }
 
+// Gather + Boris push + wall test + hole list + deposit of the survivors, one thread per
+// particle slot; the block's run gives its E, rho, counters and live count.
 template <typename T, int TYPEB>
 __global__ void __launch_bounds__(PB)
-    k_push_dev(oecl_grid_t g, WallDev wall, Soa<T> p, const uint32_t *__restrict__ nDev,
+    k_push_dev(Runs runs, oecl_grid_t g, WallDev wall, Soa<T> p, const uint32_t *__restrict__ nDev,
                const double *__restrict__ E, const double *__restrict__ B, double b0, double b1,
                double b2, double dt, double k, double q, uint8_t *__restrict__ isInside,
                double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
                uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
 {
     __shared__ double tiles[PB / 64][512];
-    const uint32_t n = *nDev;
-    if (blockIdx.x * PB >= n) return;  // whole block
-    const uint32_t i = blockIdx.x * PB + threadIdx.x;
-    const bool live = i < n;
+    const uint32_t run = runs.block_run(blockIdx.x);
+    const uint64_t sBeg = runs.seg0(run);
+    const uint32_t n = nDev[run];
+    const uint64_t first = (uint64_t)blockIdx.x * PB;
+    if (first - sBeg >= n) return;  // whole block past the run's live particles
+    const uint32_t i = (uint32_t)first + threadIdx.x;
+    const bool live = i - sBeg < n;
+    E += (size_t)run * 2 * g.np;
+    rho += (size_t)run * g.np;
+    misc += run * M_COUNT;
+    nzList += sBeg / 64;
     // FP32 storage is widened on load: weights, gather and the Boris rotation run in FP64
     // registers (the kernel is bound by memory traffic, not arithmetic).
     using C = double;
@@ -170,20 +218,25 @@ __device__ __forceinline__ uint32_t block_scan_ct(uint32_t v, uint32_t *wsum, ui
     return off + x - v;
 }
 
-// One block: all holes in ascending index order (holePos, the first absCap of them), their
-// count misc[M_HOLES], and the absorbed rows (hole r -> row r).  The push waves that have
-// holes (nzList, usually a few dozen) are sorted in LDS and their lists concatenated in
-// wave order; beyond NZMAX such waves every wave's count is scanned instead.
+// One block per run: all holes of the run in ascending index order (holePos, the first absCap
+// of them), their count misc[M_HOLES], and the absorbed rows (hole r -> row r).  The push
+// waves that have holes (nzList, usually a few dozen) are sorted in LDS and their lists
+// concatenated in wave order; beyond NZMAX such waves every wave's count is scanned instead.
 template <typename T>
 __global__ void __launch_bounds__(CT)
-    k_holes_collect(Soa<T> p, const uint32_t *__restrict__ nDev, const uint32_t *__restrict__ counts,
+    k_holes_collect(Runs runs, Soa<T> p, const uint32_t *__restrict__ nDev, const uint32_t *__restrict__ counts,
                     const uint32_t *__restrict__ list, const uint32_t *__restrict__ nzList,
-                    uint32_t *__restrict__ holePos, uint32_t absCap, T *__restrict__ absorbed,
-                    uint32_t *__restrict__ misc)
+                    uint32_t *__restrict__ holePos, T *__restrict__ absorbed, uint32_t *__restrict__ misc)
 {
     __shared__ uint32_t keys[NZMAX];
     __shared__ uint32_t wsum[CT / 64];
-    const uint32_t n = *nDev, nb = (n + 63) / 64, nz = misc[M_NZ];
+    const uint32_t run = blockIdx.x;
+    const uint32_t w0 = (uint32_t)(runs.seg0(run) / 64), a0 = runs.abs0(run), absCap = runs.absN(run);
+    misc += run * M_COUNT;
+    nzList += w0;
+    holePos += a0;
+    absorbed += 6 * (uint64_t)a0;
+    const uint32_t n = nDev[run], nb = (n + 63) / 64, nz = misc[M_NZ];
     const uint32_t tid = threadIdx.x;
     const bool sorted = nz <= NZMAX;
     uint32_t N = 1;
@@ -211,7 +264,7 @@ __global__ void __launch_bounds__(CT)
     uint32_t carry = 0;
     for (uint32_t b0 = 0; b0 < items; b0 += CT) {
         const uint32_t k = b0 + tid;
-        const uint32_t blk = k < items ? (sorted ? keys[k] : k) : 0;
+        const uint32_t blk = k < items ? (sorted ? keys[k] : w0 + k) : 0;
         const uint32_t c = k < items ? counts[blk] : 0;
         uint32_t tot;
         const uint32_t start = carry + block_scan_ct(c, wsum, &tot);
@@ -335,26 +388,44 @@ __device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const
     return ns;
 }
 
+// Wall hits of run blockIdx.y, grid-stride over its hits.
 template <typename T>
 __global__ void __launch_bounds__(128)
-    k_hits_dev(WallDev w, oecl_furman_t P, oecl_secem_t S, int useSecem, T *ap, T *nv, T *rts, double dt,
-               uint32_t absCap, int emit, uint32_t s0, uint32_t s1, const uint32_t *__restrict__ stepDev,
+    k_hits_dev(Runs runs, WallDev w, oecl_furman_t P, oecl_secem_t S, int useSecem, T *ap, T *nv, T *rts,
+               double dt, int emit, uint32_t s0, uint32_t s1, const uint32_t *__restrict__ stepDev,
                uint32_t *__restrict__ nSec, double *__restrict__ impact, const uint32_t *misc, uint32_t *stats)
 {
-    const uint32_t na = n_hits(misc, absCap), step = *stepDev;
+    const uint32_t run = blockIdx.y, a0 = runs.abs0(run);
+    const uint32_t na = n_hits(misc + run * M_COUNT, runs.absN(run)), step = *stepDev;
+    const oecl_furman_t &Pr = runs.furman ? runs.furman[run] : P;
+    const uint32_t k0 = runs.key0(run, s0), k1 = runs.key1(run, s1);
+    ap += 6 * (uint64_t)a0;
+    nv += 2 * (uint64_t)a0;
+    rts += a0;
+    nSec += a0;
+    impact += 4 * (uint64_t)a0;
+    stats += 8 * run;
     for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x)
-        hit_one<T>(jj, w, P, S, useSecem, ap, nv, rts, dt, emit, s0, s1, step, nSec, impact, stats);
+        hit_one<T>(jj, w, Pr, S, useSecem, ap, nv, rts, dt, emit, k0, k1, step, nSec, impact, stats);
 }
 
 // Second loop of the simpler emitters (secem::pass2) into the hit's MAXSEC slots, one thread
 // per hit, grid-stride.
 template <typename T>
 __global__ void __launch_bounds__(128)
-    k_secem_emit_dev(oecl_secem_t S, const T *__restrict__ ap, const T *__restrict__ nv, uint32_t absCap,
+    k_secem_emit_dev(Runs runs, oecl_secem_t S, const T *__restrict__ ap, const T *__restrict__ nv,
                      const uint32_t *__restrict__ nSec, const double *__restrict__ impact, uint32_t s0, uint32_t s1,
                      const uint32_t *__restrict__ stepDev, T *__restrict__ slots, const uint32_t *misc)
 {
-    const uint32_t na = n_hits(misc, absCap), step = *stepDev;
+    const uint32_t run = blockIdx.y, a0 = runs.abs0(run);
+    const uint32_t na = n_hits(misc + run * M_COUNT, runs.absN(run)), step = *stepDev;
+    s0 = runs.key0(run, s0);
+    s1 = runs.key1(run, s1);
+    ap += 6 * (uint64_t)a0;
+    nv += 2 * (uint64_t)a0;
+    nSec += a0;
+    impact += 4 * (uint64_t)a0;
+    slots += 6 * (uint64_t)MAXSEC * a0;
     for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x) {
         const uint32_t ns = nSec[jj];
         if (ns == 0) continue;
@@ -371,12 +442,21 @@ __global__ void __launch_bounds__(128)
 // the LAST hit's first-loop values, as the reference does.
 template <typename T>
 __global__ void __launch_bounds__(64)
-    k_emit_dev(oecl_furman_t P, const T *__restrict__ ap, const T *__restrict__ nv, uint32_t absCap,
+    k_emit_dev(Runs runs, oecl_furman_t P, const T *__restrict__ ap, const T *__restrict__ nv,
                const uint32_t *__restrict__ nSec, const double *__restrict__ impact, uint32_t s0, uint32_t s1,
                const uint32_t *__restrict__ stepDev, T *__restrict__ slots, const uint32_t *misc)
 {
     constexpr uint32_t GPW = 64 / MAXSEC;  // impact groups per wave
-    const uint32_t na = n_hits(misc, absCap), step = *stepDev;
+    const uint32_t run = blockIdx.y, a0 = runs.abs0(run);
+    const uint32_t na = n_hits(misc + run * M_COUNT, runs.absN(run)), step = *stepDev;
+    const oecl_furman_t &Pr = runs.furman ? runs.furman[run] : P;
+    s0 = runs.key0(run, s0);
+    s1 = runs.key1(run, s1);
+    ap += 6 * (uint64_t)a0;
+    nv += 2 * (uint64_t)a0;
+    nSec += a0;
+    impact += 4 * (uint64_t)a0;
+    slots += 6 * (uint64_t)MAXSEC * a0;
     const uint32_t lane = threadIdx.x & 63, grp = lane / MAXSEC, ii = lane % MAXSEC;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nWaves = (gridDim.x * blockDim.x) >> 6;
@@ -387,25 +467,36 @@ __global__ void __launch_bounds__(64)
         const uint32_t jr = ok ? jj : 0;
         furman::Rng rng{nullptr, jr, step, s0, s1};
         T *o = ok && ii < ns ? slots + 6 * ((uint64_t)MAXSEC * jj + ii) : nullptr;
-        furman::pass2_lane<T>(P, ap + 6 * (uint64_t)jr, nv + 2 * (uint64_t)jr, ns, ok ? impact[4 * (uint64_t)jj] : 0.,
+        furman::pass2_lane<T>(Pr, ap + 6 * (uint64_t)jr, nv + 2 * (uint64_t)jr, ns, ok ? impact[4 * (uint64_t)jj] : 0.,
                               impact + 4 * (uint64_t)(na - 1), rng, ii, (int)(grp * MAXSEC), o);
     }
 }
 
-// One block: secOff = exclusive prefix of nSec over the hits (total at secOff[absCap]);
-// secondary s (hit order, then emission order) is deposited, copied to secOut[s] and placed
-// in hole s, or appended after the live particles once the holes are used up.  At most
-// secCap secondaries are taken (more set ST_SEC_OVERFLOW); nothing is placed when the hits
-// overflowed absCap (ST_ABS_OVERFLOW, set at the end of the step).
+// One block per run: secOff = exclusive prefix of nSec over the hits (total at
+// secOff[absCap]); secondary s (hit order, then emission order) is deposited, copied to
+// secOut[s] (single run; secOut may be null) and placed in hole s, or appended after the live
+// particles once the holes are used up.  With secOut, at most secCap secondaries are taken
+// (more set ST_SEC_OVERFLOW); nothing is placed when the hits overflowed absCap
+// (ST_ABS_OVERFLOW, set at the end of the step).  Run 0 advances the step counter.
 template <typename T>
 __global__ void __launch_bounds__(CT)
-    k_place_dev(oecl_grid_t g, Soa<T> p, const T *__restrict__ slots, const T *__restrict__ photoSlots,
+    k_place_dev(Runs runs, oecl_grid_t g, Soa<T> p, const T *__restrict__ slots, const T *__restrict__ photoSlots,
                 T *__restrict__ secOut, uint32_t secCap, const uint32_t *__restrict__ nSec,
-                uint32_t *__restrict__ secOff, uint32_t absCap, uint32_t *__restrict__ nDev,
-                const uint32_t *__restrict__ holePos, uint64_t capacity, double q, double *rho, uint32_t *misc,
-                uint32_t *stepDev, uint32_t *stats)
+                uint32_t *__restrict__ secOff, uint32_t *__restrict__ nDev, const uint32_t *__restrict__ holePos,
+                double q, double *rho, uint32_t *misc, uint32_t *stepDev, uint32_t *stats)
 {
     __shared__ uint32_t wsum[CT / 64];
+    const uint32_t run = blockIdx.x, a0 = runs.abs0(run), absCap = runs.absN(run);
+    const uint64_t sBeg = runs.seg0(run), capacity = runs.seg1(run) - sBeg;
+    misc += run * M_COUNT;
+    stats += 8 * run;
+    nDev += run;
+    nSec += a0;
+    secOff += a0 + run;
+    holePos += a0;
+    slots += 6 * (uint64_t)MAXSEC * a0;
+    rho += (size_t)run * g.np;
+    if (!secOut) secCap = 0xFFFFFFFFu;
     const uint32_t na = n_hits(misc, absCap), h = misc[M_HOLES], n = *nDev;
     const uint32_t tid = threadIdx.x;
     uint32_t carry = 0;
@@ -425,14 +516,16 @@ __global__ void __launch_bounds__(CT)
         Weights<T> w = grid_weights<T>(r[0], r[1], (T)(1. / g.dx), (T)(1. / g.dy), (T)g.lxHalfExt,
                                        (T)g.lyHalfExt, g.nx, g.ny);
         deposit_corners<T>(rho, w, g.nx, (double)r[5], q);
+        if (secOut) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) secOut[6 * (uint64_t)s + c] = r[c];
+            for (int c = 0; c < 6; ++c) secOut[6 * (uint64_t)s + c] = r[c];
+        }
         if (h > absCap) return;
-        const uint64_t dst = s < h ? holePos[s] : (uint64_t)n + (s - h);
-        if (dst >= capacity) {
+        if (s >= h && (uint64_t)n + (s - h) >= capacity) {
             atomicOr(stats + 2, (uint32_t)ST_CAP_OVERFLOW);
             return;
         }
+        const uint64_t dst = s < h ? (uint64_t)holePos[s] : sBeg + n + (s - h);
 #pragma unroll
         for (int c = 0; c < 6; ++c) p(c, dst) = r[c];
     };
@@ -461,7 +554,7 @@ __global__ void __launch_bounds__(CT)
     if (h <= absCap && nAdd < h) {
         const uint32_t cnt = h - nAdd;
         for (uint32_t t = tid; t < cnt; t += CT) {
-            const uint32_t i = n - 1 - t;
+            const uint32_t i = (uint32_t)sBeg + n - 1 - t;
             uint32_t lo = 0, hi = h;  // first index with holePos > i
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
@@ -493,7 +586,7 @@ __global__ void __launch_bounds__(CT)
     stats[4] = h;
     stats[5] = nAdd;
     stats[6] += nPhPlaced;
-    *stepDev += 1;
+    if (run == 0) *stepDev += 1;
 }
 
 unsigned grid_cap(uint64_t items, unsigned block, unsigned maxBlocks)
@@ -504,84 +597,111 @@ unsigned grid_cap(uint64_t items, unsigned block, unsigned maxBlocks)
 
 }  // namespace
 
-// Work buffer of oecl_step_run: per push wave hole counts and lists, the list of waves with
-// holes, the ascending hole list, counters, and MAXSEC secondary slots per hit.
-static uint64_t slots_offset(uint64_t capacity, uint32_t absCap)
+// Work buffer of a step: per push wave hole counts and lists, the list of waves with holes,
+// the ascending hole lists (absTotal), counters (M_COUNT per run), and MAXSEC secondary slots
+// per hit.
+static uint64_t slots_offset(uint64_t capacity, uint32_t absTotal, uint32_t R)
 {
     const uint64_t nW = (capacity + PB - 1) / PB * (PB / 64);
-    return ((4 * (nW * (64 + 2) + absCap + M_COUNT)) + 255) / 256 * 256;
+    return ((4 * (nW * (64 + 2) + absTotal + (uint64_t)M_COUNT * R)) + 255) / 256 * 256;
 }
 
 OECL_EXPORT uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap)
 {
-    return slots_offset(capacity, absCap) + 8ull * 6 * MAXSEC * absCap + 1024;
+    return slots_offset(capacity, absCap, 1) + 8ull * 6 * MAXSEC * absCap + 1024;
 }
 
+OECL_EXPORT uint64_t oecl_step_batch_work_bytes(uint64_t capacity, uint32_t absTotal, uint32_t R)
+{
+    return slots_offset(capacity, absTotal, R) + 8ull * 6 * MAXSEC * absTotal + 1024;
+}
+
+namespace {
+
+// E_r = phiToE phi_r for every run r = blockIdx.y (rows in the CSR's own column order, the
+// arithmetic of oecl_csr_spmv).
+__global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, const int32_t *__restrict__ col,
+                            const double *__restrict__ val, const double *__restrict__ x, uint32_t xStride,
+                            double *__restrict__ y)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nRows) return;
+    x += (size_t)blockIdx.y * xStride;
+    y += (size_t)blockIdx.y * nRows;
+    double sum = 0.;
+    for (int32_t k = rowPtr[r]; k < rowPtr[r + 1]; ++k) sum += val[k] * x[col[k]];
+    y[r] = sum;
+}
+
+}  // namespace
+
+// Queues one step of every run.  R = runs.R; absTotal = rows of the per-hit arrays.
 template <typename T>
 static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman_t *P, oecl_fastpoisson *fp,
-                      const oecl_step_args *a, hipStream_t s)
+                      const oecl_step_args *a, const Runs &runs, hipStream_t s)
 {
     const uint64_t cap = a->stride;
-    const uint32_t np = g->np;
+    const uint32_t np = g->np, R = runs.R, absTotal = a->absCap;
     Soa<T> p{(T *)a->data, a->stride};
     const uint64_t nBlk = (cap + PB - 1) / PB, nW = nBlk * (PB / 64);
     uint32_t *counts = (uint32_t *)a->work;
     uint32_t *list = counts + nW;
     uint32_t *nzList = list + nW * 64;
     uint32_t *holePos = nzList + nW;
-    uint32_t *misc = holePos + a->absCap;
-    T *slots = (T *)((char *)a->work + slots_offset(cap, a->absCap));
+    uint32_t *misc = holePos + absTotal;
+    T *slots = (T *)((char *)a->work + slots_offset(cap, absTotal, R));
+    int rc = fastpoisson_reserve(fp, R);
+    if (rc) return rc;
     const FpBox box = fastpoisson_box(fp);
-    hipLaunchKernelGGL(k_begin_dev<T>, dim3(oecl_blocks(np, 256)), dim3(256), 0, s, np, a->rho, a->beamScale,
+    hipLaunchKernelGGL(k_begin_dev<T>, dim3(oecl_blocks(np, 256), R), dim3(256), 0, s, np, a->rho, a->beamScale,
                        a->nScale, a->stepDev, a->profile, box, a->phi, a->nDev, a->countHist, a->histCap, misc, wd,
                        a->photoCount, a->photoWeight, a->nPhoto, a->photoCap, a->seed0, a->seed1, a->photoE0,
                        a->photoSigmaE, P->particleMass, (T *)a->photoSlots, a->stats);
-    int rc = fastpoisson_solve_box(fp, a->phi, s);
+    rc = fastpoisson_solve_box(fp, a->phi, R, s);
     if (rc) return rc;
-    rc = oecl_csr_spmv(2 * np, a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, a->E, (void *)s);
-    if (rc) return rc;
+    hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
+                       a->e2nCol, a->e2nVal, a->phi, np, a->E);
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
     const dim3 gp((unsigned)nBlk), bp(PB);
     if (a->evPushBegin) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushBegin, s), "event");
     if (a->typeB == 0)
-        hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, *g, wd, p, a->nDev, a->E, a->bField, 0., 0., 0.,
+        hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->bField, 0., 0., 0.,
                            a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     else if (a->typeB == 1)
-        hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, *g, wd, p, a->nDev, a->E, a->bField, a->b0, a->b1,
-                           a->b2, a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
+        hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->bField, a->b0,
+                           a->b1, a->b2, a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     else
-        hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, *g, wd, p, a->nDev, a->E, a->bField, 0., 0., 0.,
+        hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->bField, 0., 0., 0.,
                            a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     if (a->evPushEnd) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushEnd, s), "event");
-    hipLaunchKernelGGL(k_holes_collect<T>, dim3(1), dim3(CT), 0, s, p, a->nDev, counts, list, nzList, holePos,
-                       a->absCap, (T *)a->absorbed, misc);
+    hipLaunchKernelGGL(k_holes_collect<T>, dim3(R), dim3(CT), 0, s, runs, p, a->nDev, counts, list, nzList, holePos,
+                       (T *)a->absorbed, misc);
     const oecl_secem_t S = a->secem ? *a->secem : oecl_secem_t{};
     const int useSecem = a->secem ? 1 : 0;
-    hipLaunchKernelGGL(k_hits_dev<T>, dim3(grid_cap(a->absCap, 128, 512)), dim3(128), 0, s, wd, *P, S, useSecem,
-                       (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, a->absCap, a->emit, a->seed0,
-                       a->seed1, a->stepDev, a->nSec, a->impact, misc, a->stats);
+    const unsigned hitBlocks = (grid_cap(absTotal, 128, 512) + R - 1) / R;
+    hipLaunchKernelGGL(k_hits_dev<T>, dim3(hitBlocks, R), dim3(128), 0, s, runs, wd, *P, S, useSecem,
+                       (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, a->emit, a->seed0, a->seed1,
+                       a->stepDev, a->nSec, a->impact, misc, a->stats);
     if (a->emit && useSecem)
-        hipLaunchKernelGGL(k_secem_emit_dev<T>, dim3(grid_cap(a->absCap, 128, 512)), dim3(128), 0, s, S,
-                           (const T *)a->absorbed, (const T *)a->normals, a->absCap, a->nSec, a->impact, a->seed0,
-                           a->seed1, a->stepDev, slots, misc);
+        hipLaunchKernelGGL(k_secem_emit_dev<T>, dim3(hitBlocks, R), dim3(128), 0, s, runs, S,
+                           (const T *)a->absorbed, (const T *)a->normals, a->nSec, a->impact, a->seed0, a->seed1,
+                           a->stepDev, slots, misc);
     else if (a->emit)
-        hipLaunchKernelGGL(k_emit_dev<T>, dim3(grid_cap(a->absCap, 6, 1024)), dim3(64), 0, s, *P,
-                           (const T *)a->absorbed, (const T *)a->normals, a->absCap, a->nSec, a->impact, a->seed0,
-                           a->seed1, a->stepDev, slots, misc);
-    hipLaunchKernelGGL(k_place_dev<T>, dim3(1), dim3(CT), 0, s, *g, p, (const T *)slots, (const T *)a->photoSlots,
-                       (T *)a->secondaries, a->secCap, a->nSec, a->secOff, a->absCap, a->nDev, holePos, cap, a->qe,
-                       a->rho, misc, a->stepDev, a->stats);
+        hipLaunchKernelGGL(k_emit_dev<T>, dim3((grid_cap(absTotal, 6, 1024) + R - 1) / R, R), dim3(64), 0, s, runs,
+                           *P, (const T *)a->absorbed, (const T *)a->normals, a->nSec, a->impact, a->seed0, a->seed1,
+                           a->stepDev, slots, misc);
+    hipLaunchKernelGGL(k_place_dev<T>, dim3(R), dim3(CT), 0, s, runs, *g, p, (const T *)slots,
+                       (const T *)a->photoSlots, (T *)a->secondaries, a->secCap, a->nSec, a->secOff, a->nDev, holePos,
+                       a->qe, a->rho, misc, a->stepDev, a->stats);
     OECL_CHECK_LAUNCH("oecl_step_run");
     return OECL_OK;
 }
 
-OECL_EXPORT int oecl_step_run(const oecl_grid_t *g, const oecl_wall_t *wall, const oecl_furman_t *P,
-                              oecl_fastpoisson *fp, const oecl_step_args *a, void *stream)
+static int step_check(const oecl_grid_t *g, const oecl_wall_t *wall, oecl_fastpoisson *fp, const oecl_step_args *a)
 {
-    if (!g || !wall || !P || !fp || !a || !a->data || !a->nDev || !a->isInside || !a->rho || !a->q ||
-        !a->phi || !a->E || !a->e2nRowPtr || !a->e2nCol || !a->e2nVal || !a->absorbed || !a->normals ||
-        !a->remaining || !a->nSec || !a->secOff || !a->impact || !a->secondaries || !a->stepDev ||
-        !a->stats || !a->work || a->absCap == 0 || a->secCap == 0)
+    if (!g || !wall || !fp || !a || !a->data || !a->nDev || !a->isInside || !a->rho || !a->q || !a->phi || !a->E ||
+        !a->e2nRowPtr || !a->e2nCol || !a->e2nVal || !a->absorbed || !a->normals || !a->remaining || !a->nSec ||
+        !a->secOff || !a->impact || !a->stepDev || !a->stats || !a->work || a->absCap == 0)
         return OECL_ERR_ARG;
     if (a->typeB == 2 && !a->bField) return OECL_ERR_ARG;
     if (a->secem && a->secem->model != OECL_SECEM_STAINLESS && a->secem->model != OECL_SECEM_SIMPLE)
@@ -591,13 +711,49 @@ OECL_EXPORT int oecl_step_run(const oecl_grid_t *g, const oecl_wall_t *wall, con
         oecl_set_error_msg("photoelectrons: rectangular or elliptical wall only");
         return OECL_ERR_ARG;
     }
-    WallDev wd;
-    int rc = wall_to_device(wall, &wd);
+    if (a->dtype != OECL_F64 && a->dtype != OECL_F32) return OECL_ERR_ARG;
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_step_run(const oecl_grid_t *g, const oecl_wall_t *wall, const oecl_furman_t *P,
+                              oecl_fastpoisson *fp, const oecl_step_args *a, void *stream)
+{
+    int rc = step_check(g, wall, fp, a);
     if (rc) return rc;
+    if (!P || !a->secondaries || a->secCap == 0) return OECL_ERR_ARG;
+    WallDev wd;
+    rc = wall_to_device(wall, &wd);
+    if (rc) return rc;
+    const Runs runs{1u, a->absCap, a->stride, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipStream_t s = (hipStream_t)stream;
-    if (a->dtype == OECL_F64) return step_typed<double>(g, wd, P, fp, a, s);
-    if (a->dtype == OECL_F32) return step_typed<float>(g, wd, P, fp, a, s);
-    return OECL_ERR_ARG;
+    if (a->dtype == OECL_F64) return step_typed<double>(g, wd, P, fp, a, runs, s);
+    return step_typed<float>(g, wd, P, fp, a, runs, s);
+}
+
+OECL_EXPORT int oecl_step_run_batch(const oecl_grid_t *g, const oecl_wall_t *wall, const oecl_runs_t *rs,
+                                    oecl_fastpoisson *fp, const oecl_step_args *a, void *stream)
+{
+    int rc = step_check(g, wall, fp, a);
+    if (rc) return rc;
+    if (!rs || rs->R == 0 || rs->R > 65535 || !rs->segOff || !rs->blockRun || !rs->absOff || !rs->keys)
+        return OECL_ERR_ARG;
+    if (a->emit && !a->secem && !rs->furman) return OECL_ERR_ARG;
+    if (a->photoCount) {
+        oecl_set_error_msg("step batch: no photoelectron source (use oecl_step_run)");
+        return OECL_ERR_ARG;
+    }
+    if (a->stride % PB) {
+        oecl_set_error_msg("step batch: the capacity must be a multiple of 256");
+        return OECL_ERR_ARG;
+    }
+    WallDev wd;
+    rc = wall_to_device(wall, &wd);
+    if (rc) return rc;
+    const oecl_furman_t P{};  // the runs' emitter parameters come from rs->furman
+    const Runs runs{rs->R, a->absCap, a->stride, rs->segOff, rs->blockRun, rs->absOff, rs->furman, rs->keys};
+    hipStream_t s = (hipStream_t)stream;
+    if (a->dtype == OECL_F64) return step_typed<double>(g, wd, &P, fp, a, runs, s);
+    return step_typed<float>(g, wd, &P, fp, a, runs, s);
 }
 
 OECL_EXPORT int oecl_event_create(void **ev)

@@ -1,0 +1,153 @@
+"""Batched parameter scans (BuildupScan, oecl_step_run_batch): every run of a scan against the
+same run stepped alone by the one-call native step, graph replays, and the batched sort."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import random_particles
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = (101, 202, 303)
+NS = (5000, 12000, 20000)
+SEY = (1.3, 1.6, 2.0)
+PPB = (5e10, 1e11, 2e11)
+
+
+def _shared(dtype=torch.float64):
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = Grid(64, 48, 0.09, 0.07, 1)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    return g, ps
+
+
+def _run(g, ps, r, dtype=torch.float64):
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter
+    from openecloud_amd.particles import Particles
+    pd = random_particles(g, NS[r], seed=40 + r, wall='ellip', vscale=3e6)
+    pd[:, 5] = 5e4
+    if dtype == torch.float32:
+        pd = pd.astype(np.float32).astype(np.float64)
+    dt = 2e-11
+    p = Particles('electrons', 'variableWeight', dtype=dtype, ordering='fast')
+    p.setParticleData(pd)
+    p.setBConst([0., 0.3, 0.])
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=SEY[r], eHat0=300.)
+    beam = GaussianBunchTrain(g, dt, nParticles=PPB[r], tBunchSpacing=25e-9, radiusSigma=2e-3,
+                              nBunches=1, bunchLengthSigma=0.003)
+    return Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
+
+
+def _alone(g, ps, nsteps, sort_at=(), dtype=torch.float64):
+    from openecloud_amd import randomGen
+    out = []
+    for r in range(3):
+        b = _run(g, ps, r, dtype)
+        randomGen.seed(SEEDS[r])
+        b.native_setup(absCap=4096, secCap=8192, maxSteps=64)
+        states = []
+        for it in range(nsteps):
+            if it in sort_at:
+                b.sync()
+                b.sort_particles(keepBuffers=True)
+            b.step_native()
+            b.sync()
+            states.append(b.p.getParticleData().cpu().numpy().astype(np.float64).copy())
+        out.append(states)
+    return out
+
+
+def _scan(g, ps, dtype=torch.float64):
+    from openecloud_amd.scan import BuildupScan
+    return BuildupScan([_run(g, ps, r, dtype) for r in range(3)], SEEDS, maxSteps=64)
+
+
+def _close(got, ref, tol):
+    assert got.shape == ref.shape, (got.shape, ref.shape)
+    scale = np.abs(ref).max(axis=0)
+    return (np.abs(got - ref) / scale).max() < tol
+
+
+def test_scan_runs_match_standalone_native_steps():
+    g, ps = _shared()
+    nsteps, sort_at = 10, (5,)
+    ref = _alone(g, ps, nsteps, sort_at)
+    sc = _scan(g, ps)
+    for it in range(nsteps):
+        if it in sort_at:
+            sc.sort()
+        sc.step()
+        st = sc.sync()
+        for r in range(3):
+            got = sc.run_particles(r).cpu().numpy()
+            assert _close(got, ref[r][it], 1e-11), (it, r)
+    assert all(a > 0 for a in st['absorbed_total']) and sum(st['emitted_total']) > 0
+    assert st['counts'] == [ref[r][-1].shape[0] for r in range(3)]
+    # the per-run histories hold each step's starting count
+    assert sc.history(2, 1, nsteps) == [ref[2][k].shape[0] for k in range(nsteps - 1)]
+
+
+def test_scan_graph_replays_match_eager_steps():
+    g, ps = _shared()
+    eager = _scan(g, ps)
+    for _ in range(8):
+        eager.step()
+    eager.sync()
+    sc = _scan(g, ps)
+    sc.step()             # first step eagerly (allocations happen outside the capture)
+    sc.capture(7)
+    sc.replay()
+    st = sc.sync()
+    assert sc.it == eager.it == 8
+    for r in range(3):
+        assert _close(sc.run_particles(r).cpu().numpy(), eager.run_particles(r).cpu().numpy(), 1e-11), r
+    assert st['counts'] == eager.counts()
+
+
+@pytest.mark.parametrize('order,binShift', [(0, 0), (1, 0), (0, 2), (1, 1)])
+def test_scan_sort_orders_each_run_in_its_segment(order, binShift):
+    g, ps = _shared()
+    sc = _scan(g, ps)
+    for _ in range(3):
+        sc.step()
+    sc.sync()
+    before = [sc.run_particles(r).cpu().numpy().copy() for r in range(3)]
+    sc.sort(order, binShift)
+    nx = g.getNxExt()
+    for r in range(3):
+        after = sc.run_particles(r).cpu().numpy()
+        # same particles (as a multiset of rows), now in bin order, stable within a bin
+        key = lambda a: np.lexsort(a.T[::-1])
+        assert np.array_equal(before[r][key(before[r])], after[key(after)])
+        ix = np.floor((after[:, 0] + 0.5 * g.getLxExt()) / g.getDx()).astype(np.int64) >> binShift
+        iy = np.floor((after[:, 1] + 0.5 * g.getLyExt()) / g.getDy()).astype(np.int64) >> binShift
+        if order == 0:
+            b = ix + iy * (((nx - 1) >> binShift) + 1)
+        else:
+            b = np.zeros_like(ix)
+            for k in range(16):
+                b |= ((ix >> k) & 1) << (2 * k) | ((iy >> k) & 1) << (2 * k + 1)
+        assert np.all(np.diff(b) >= 0), r
+
+
+def test_scan_fp32_runs_match_standalone():
+    g, ps = _shared()
+    nsteps = 5
+    ref = _alone(g, ps, nsteps, dtype=torch.float32)
+    sc = _scan(g, ps, dtype=torch.float32)
+    for it in range(nsteps):
+        sc.step()
+    sc.sync()
+    for r in range(3):
+        got = sc.run_particles(r).cpu().numpy().astype(np.float64)
+        assert abs(got.shape[0] - ref[r][-1].shape[0]) <= 2
+        m = min(got.shape[0], ref[r][-1].shape[0])
+        idx = np.nonzero(np.all(np.abs(got[:m, :2] - ref[r][-1][:m, :2]) < 1e-7, axis=1))[0]
+        assert idx.size > 0.95 * m
+        scale = np.abs(ref[r][-1]).max(axis=0)
+        assert (np.abs(got[idx] - ref[r][-1][idx]) / scale).max() < 1e-5
