@@ -51,6 +51,15 @@ def parse():
                     help='replay the steps between two sorts as one captured HIP graph')
     ap.add_argument('--sort-every', type=int, default=50,
                     help='re-order particles by cell every this many steps (0: never)')
+    ap.add_argument('--workload', default='dipole', choices=['dipole', 'scan'],
+                    help='dipole: the elliptical dipole buildup (default); scan: 64 buildups '
+                         'stepped together (delta_max and intensity scan)')
+    ap.add_argument('--runs', type=int, default=64, help='scan: number of buildups')
+    ap.add_argument('--tune-cache', default=None, help='scan: autotuner cache file (JSON)')
+    ap.add_argument('--scan-config', default=None,
+                    help='scan: sortEvery,order,binShift to use instead of autotuning')
+    ap.add_argument('--no-f32-compare', action='store_true',
+                    help='dipole: skip the FP32-storage run compared against the FP64 one')
     return ap.parse_args()
 
 
@@ -154,12 +163,194 @@ def cpu_baseline(seconds):
                 poisson_solves_per_s=1. / t_lu)
 
 
+def build_scan(R, dtype, maxSteps):
+    """The scan workload: R buildups on a shared 256 x 256 rectangular 100 x 50 mm chamber,
+    By = 0.3 T, dt = 1e-11 s, Gaussian bunches sigma 2 mm at 25 ns, 8 bunches, Furman copper
+    with E_max 300 eV.  delta_max runs over linspace(1.1, 2.0, R); bunch intensity and
+    macro-particle count are log-uniform over [5e10, 2e11] ppb and [1e4, 2e6] (seed 0); each
+    run carries 1e9 electrons per metre, loaded uniformly with a 1 eV Maxwellian."""
+    import numpy as np
+    import torch
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbRectangular
+    from openecloud_amd.particleEmitter import FurmanEmitter, homoLoader
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from openecloud_amd.scan import BuildupScan
+    rng = np.random.default_rng(0)
+    sey = np.linspace(1.1, 2.0, R)
+    ppb = np.exp(rng.uniform(np.log(5e10), np.log(2e11), R))
+    nmp = np.exp(rng.uniform(np.log(1e4), np.log(2e6), R)).astype(np.int64)
+    dt = 1e-11
+    g = Grid(256, 256, 0.1, 0.05, 0)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    tdt = torch.float64 if dtype == 'f64' else torch.float32
+    runs, seeds = [], []
+    for r in range(R):
+        seeds.append(1000 + r)
+        randomGen.seed(1000 + r)
+        p = Particles('electrons', 'variableWeight', dtype=tdt, ordering='fast')
+        p.setBConst([0., 0.3, 0.])
+        pb = AbsorbRectangular(g, p)
+        em = FurmanEmitter(pb, p, 'copper', seyMax=float(sey[r]), eHat0=300.)
+        beam = GaussianBunchTrain(g, dt, nParticles=float(ppb[r]), tBunchSpacing=25e-9, radiusSigma=2e-3,
+                                  nBunches=8)
+        homoLoader(g, p, pb, int(nmp[r]), 1e9 / float(nmp[r]), 4.194e5)
+        runs.append(Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1))
+    scan = BuildupScan(runs, seeds, growth=1.5, maxSteps=maxSteps)
+    return g, ps, runs, scan, dict(sey=sey, ppb=ppb, n=nmp)
+
+
+def compare_f32(a, n, b64, it0):
+    """The same seeded workload with FP32 particle storage (arithmetic stays FP64), stepped
+    exactly like the FP64 run (warm-up, the timed steps, sorts); its throughput and the
+    differences of the final state from the FP64 run: cloud density (charge on the grid,
+    relative L2), total charge and macro-particle count."""
+    import torch
+    del it0
+    g, p, pb, em, ps, b = build_case(n, 'f32')
+    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4), maxSteps=a.warmup + a.steps + 40)
+    b.sort_particles()
+    for k in range(a.warmup):
+        b.step_native()
+        if a.sort_every and k % a.sort_every == a.sort_every - 1:
+            b.sync()
+            b.sort_particles()
+    b.sync()
+    i0 = b.it
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        b.step_native()
+        if a.sort_every and k % a.sort_every == a.sort_every - 1:
+            b.sync()
+            b.sort_particles()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    b.sync()
+    rate = float(sum(b.counts(i0, i0 + a.steps))) / wall
+    b64.deposit_all()
+    b.deposit_all()
+    r64, r32 = b64.rho, b.rho
+    q64 = float(b64.p.getParticleData()[:, 5].to(torch.float64).sum())
+    q32 = float(b.p.getParticleData()[:, 5].to(torch.float64).sum())
+    return {'value': rate, 'unit': 'particle-steps/s',
+            'cloud_density_rel_l2': float(torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)),
+            'total_charge_rel': abs(q32 - q64) / abs(q64),
+            'macro_particles': [b64.p.macroParticleCount, b.p.macroParticleCount],
+            'note': 'FP32 particle storage, FP64 arithmetic, same seeds and steps; differences from the '
+                    'FP64 run after the timed steps'}
+
+
+def main_scan(a):
+    import ctypes
+    import torch
+    from openecloud_amd import _lib as L
+    from openecloud_amd import randomGen
+    from openecloud_amd.autotune import tune_scan
+    L.require_gpu()
+    R = a.runs
+    maxSteps = a.warmup + a.steps + 2000
+    g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps)
+    log = lambda m: print(m, file=sys.stderr)
+    scan.sort()
+    for _ in range(a.warmup):
+        scan.step()
+    scan.sync()
+    if a.scan_config:
+        se_, o_, b_ = [int(v) for v in a.scan_config.split(',')]
+        best, results = dict(sortEvery=se_, order=o_, binShift=b_, graphSteps=se_), []
+    else:
+        best, results = tune_scan(scan, cacheFile=a.tune_cache, log=log)
+    se = int(best['sortEvery'])
+    steps = (a.steps + se - 1) // se * se
+    scan.sync()
+    if scan._graph is None or scan._graphSteps != se:
+        scan.capture(se)
+    it0 = scan.it
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // se):
+        scan.sort(best['order'], best['binShift'])
+        scan.replay()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = scan.sync()
+    psteps = float(sum(sum(scan.history(r, it0, it0 + steps)) for r in range(R)))
+    rate = psteps / wall
+    # the same runs one at a time with the single-run one-call step (3 steps each)
+    seq_ps, seq_t = 0., 0.
+    for r, b in enumerate(runs):
+        randomGen.seed(1000 + r)
+        b.native_setup(absCap=max(4096, b.p.capacity // 16), secCap=max(8192, b.p.capacity // 8), maxSteps=8)
+        b.step_native()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            b.step_native()
+        torch.cuda.synchronize()
+        seq_t += time.perf_counter() - t
+        seq_ps += float(sum(b.counts(b.it - 3, b.it)))
+        b.sync()
+    # the reference-order method-by-method GPU step on 4 runs of the scan
+    ref_ps, ref_t = 0., 0.
+    for r in (0, R // 3, 2 * R // 3, R - 1):
+        b = runs[r]
+        b.step_reference()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(2):
+            n = b.p.macroParticleCount
+            b.step_reference()
+            ref_ps += n
+        torch.cuda.synchronize()
+        ref_t += time.perf_counter() - t
+    seq_rate, ref_rate = seq_ps / seq_t, ref_ps / ref_t
+    n0 = [int(x) for x in prm['n']]
+    out = {
+        'metric': 'batched scan buildup particle-steps/sec (%d buildups per step call)' % R,
+        'value': rate,
+        'unit': 'particle-steps/s',
+        'n_gpus': 1,
+        'steps': steps,
+        'warmup': a.warmup,
+        'ms_per_step': wall / steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': a.dtype,
+        'data': 'synthetic (seeded uniform loads, Philox draws)',
+        'config': {'workload': '%d buildups on a shared 256x256 rectangular 100x50 mm chamber, By 0.3 T, '
+                               'dt 1e-11 s, Gaussian bunches sigma 2 mm x8, delta_max linspace(1.1, 2.0), '
+                               'ppb log-uniform [5e10, 2e11], macro-particles log-uniform [1e4, 2e6]' % R,
+                   'runs': R, 'grid': [256, 256], 'macro_particles_total_start': int(sum(n0)),
+                   'macro_particles_total_end': int(sum(st['counts'])),
+                   'absorbed_total': int(sum(st['absorbed_total'])), 'emitted_total': int(sum(st['emitted_total']))},
+        'poisson_solves_per_s': R * steps / wall,
+        'autotune': {'best': best, 'candidates_us_per_step': [[c, 1e6 * t] for c, t in results]},
+        'plain_one_run_per_call': {'value': seq_rate, 'unit': 'particle-steps/s',
+                                   'note': 'the same runs stepped one at a time with the single-run one-call step'},
+        'plain_reference_order': {'value': ref_rate, 'unit': 'particle-steps/s',
+                                  'note': 'method-by-method GPU step in the reference order, 4 of the runs'},
+        'speedup_vs_one_run_per_call': rate / seq_rate,
+        'speedup_vs_reference_order': rate / ref_rate,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
               '(see the sharding design note); got --gpus %d' % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.workload == 'scan':
+        if a.dtype == 'f64' and '--dtype' not in sys.argv:
+            a.dtype = 'f32'   # the scan's particles are FP32 unless asked otherwise
+        return main_scan(a)
     import torch
     from openecloud_amd import _lib as L
     L.require_gpu()
@@ -296,6 +487,8 @@ def main():
                      'traffic': measured_traffic(a.dtype, n), 'avg_launch_us': avg_ms * 1e3,
                      'bytes_per_launch': avg_bytes},
     }
+    if not a.no_f32_compare and a.dtype == 'f64':
+        out['f32_vs_f64'] = compare_f32(a, n, b, it0)
     if not a.no_cpu_baseline:
         cb = cpu_baseline(a.cpu_seconds)
         out['cpu_baseline'] = {k: cb[k] for k in ('value', 'unit', 'cores', 'kind', 'sample')}

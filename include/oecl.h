@@ -350,6 +350,47 @@ int oecl_fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R);
 int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, double *phi, void *stream);
 int oecl_fastpoisson_destroy(oecl_fastpoisson *h);
 
+/* ---- learned preconditioner (no counterpart in the reference) ------------------------- */
+
+/* A 4-level U-Net (16 channels, 3x3 convolutions, ReLU, 2x2 average pooling, nearest
+ * upsampling with additive skips, 14 convolutions) used as the preconditioner of a flexible
+ * CG on the FIT Poisson system, and trained on the device.  Images are H x W (multiples of 8)
+ * NHWC bfloat16 with 16 channels; parameters are FP32 masters W [14][9][16][16] (tap
+ * 3 (dy + 1) + (dx + 1), out, in) and b [14][16] with bfloat16 copies for the products
+ * (bf16 MFMA, FP32 accumulation).  seed: the He initialisation of W (b = 0). */
+typedef struct oecl_unet oecl_unet;
+int oecl_unet_create(uint32_t H, uint32_t W, uint32_t seed, oecl_unet **out);
+int oecl_unet_destroy(oecl_unet *u);
+/* Host arrays in / out (any may be NULL in get). */
+int oecl_unet_set_params(oecl_unet *u, const float *W, const float *b);
+int oecl_unet_get_params(const oecl_unet *u, float *W, float *b, float *gW, float *gb);
+/* y[H][W] (FP32) = network(img), img H x W x 16 bf16 with the input in channel 0. */
+int oecl_unet_forward(oecl_unet *u, const uint16_t *img, float *y, void *stream);
+/* After a forward: gradients for the output gradient gout (H x W x 16 bf16, channel 0 =
+ * dL/dy), accumulated into the parameter gradients (zeroed first when zero != 0); gin
+ * (optional) receives the gradient of the input image. */
+int oecl_unet_backward(oecl_unet *u, const uint16_t *gout, uint16_t *gin, int zero, void *stream);
+/* One Adam step (Kingma & Ba) on the accumulated gradients. */
+int oecl_unet_adam(oecl_unet *u, float lr, float beta1, float beta2, float eps, void *stream);
+/* z = rms(r) * network(r / rms(r)) on the inside nodes (mask) of the nx x ny grid (FP64). */
+int oecl_unet_precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r, double *z,
+                           void *stream);
+/* One training sample (q, phi = A^-1 q): forward on q, the loss L = 1 - (z^T q)^2 /
+ * ((z^T A z)(phi^T q)) -- the A-norm error of the best multiple of the preconditioner's output
+ * z, relative to phi's A-norm (the flexible CG does not depend on z's scale) -- and backward.
+ * cx, cy, mask: the PCG stencil arrays of oecl_pcg_solve.  work: 2 nx ny doubles.  *loss
+ * (optional, synchronises) receives L. */
+int oecl_unet_train_sample(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                           const uint8_t *mask, const double *q, const double *phi, double *work, int zero,
+                           double *loss, void *stream);
+/* Flexible CG (Notay's truncated form) with the network as preconditioner, FP64 vectors and
+ * reductions: phi (initial guess in, solution out) until ||q - A phi|| <= tol ||q||, the norm
+ * read every `check` iterations.  Returns OECL_ERR_NOCONV when maxIter is reached. */
+uint64_t oecl_fcg_work_bytes(uint32_t np);
+int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy, const uint8_t *mask,
+                   const double *q, double *phi, double tol, uint32_t maxIter, uint32_t check, void *work,
+                   uint32_t *iters, double *relres, void *stream);
+
 /* ---- the whole plain step (example01_basic.py:52-97, example02/04 variants) ---------- */
 
 /* Arguments of oecl_step_run.  Counts live on the device: nDev (particles), stepDev (step

@@ -19,7 +19,7 @@ import time
 import torch
 
 DEFAULT_CANDIDATES = [dict(sortEvery=se, order=o, binShift=b)
-                      for se in (25, 50, 100) for o in (0, 1) for b in (0, 1)]
+                      for se in (10, 25, 50) for o in (0, 1) for b in (0, 1)]
 
 
 def scan_key(scan):
@@ -41,11 +41,14 @@ def _time_candidate(scan, c, intervals):
     return (time.perf_counter() - t0) / (intervals * se)
 
 
-def tune_scan(scan, candidates=None, intervals=1, cacheFile=None, log=None):
+def tune_scan(scan, candidates=None, intervals=1, rounds=2, cacheFile=None, log=None):
     """Picks the fastest candidate (a dict of sortEvery, order, binShift) for this scan.
-    Returns (best, results) with results a list of (candidate, seconds per step).  With
-    cacheFile, a stored choice for the same scan_key is returned without timing, and a new
-    choice is stored."""
+    The candidates are timed round-robin, `rounds` times each, in forward then reverse
+    order, and ranked by their mean: a state that drifts steadily (the cloud grows, the
+    particles heat up) then weighs on every candidate alike.  Returns
+    (best, results) with results a list of (candidate, seconds per step).  With cacheFile, a
+    stored choice for the same scan_key is returned without timing, and a new choice is
+    stored."""
     key = scan_key(scan)
     if cacheFile and os.path.exists(cacheFile):
         try:
@@ -55,12 +58,16 @@ def tune_scan(scan, candidates=None, intervals=1, cacheFile=None, log=None):
         if key in cache:
             return dict(cache[key]), []
     cands = list(candidates or DEFAULT_CANDIDATES)
-    results = []
-    for c in cands:
-        t = _time_candidate(scan, c, intervals)
-        results.append((dict(c), t))
-        if log:
-            log('autotune %s: %.1f us/step' % (c, 1e6 * t))
+    rounds = max(1, int(rounds))
+    tot = [0.] * len(cands)
+    for rd in range(rounds):
+        order = range(len(cands)) if rd % 2 == 0 else reversed(range(len(cands)))
+        for i in order:
+            t = _time_candidate(scan, cands[i], intervals)
+            tot[i] += t
+            if log:
+                log('autotune %s: %.1f us/step' % (cands[i], 1e6 * t))
+    results = [(dict(c), t / rounds) for c, t in zip(cands, tot)]
     best = min(results, key=lambda ct: ct[1])[0]
     best['graphSteps'] = best['sortEvery']
     if cacheFile:

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'liboecl.so')
 SOURCES = ['particles.hip', 'boundary.hip', 'emitter.hip', 'poisson.hip', 'fastpoisson.hip',
-           'sort.hip', 'manage.hip', 'step.hip']
+           'sort.hip', 'manage.hip', 'step.hip', 'unet.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['-O3', '-std=c++17', '--offload-arch=gfx950', '-fPIC', '-shared',
          '-ffp-contract=off', '-fvisibility=hidden',

@@ -1,0 +1,928 @@
+// Learned preconditioner for the FIT Poisson system (no counterpart in the reference, which
+// back-substitutes SuperLU): a 4-level U-Net with 16 channels trained on the device, and a
+// flexible conjugate-gradient solve that applies it.
+//
+// Network (C = 16, 3x3 convolutions with zero padding, ReLU, 2x2 average pooling, nearest
+// upsampling with additive skips; 14 convolutions, all 16 -> 16 channels):
+//     s0 = relu(c1(relu(c0(x))))            x: the input in channel 0
+//     s1 = relu(c3(relu(c2(pool(s0)))))
+//     s2 = relu(c5(relu(c4(pool(s1)))))
+//     u3 = relu(c7(relu(c6(pool(s2)))))
+//     u2 = relu(c9(relu(c8(up(u3) + s2))))
+//     u1 = relu(c11(relu(c10(up(u2) + s1))))
+//     y  = c13(relu(c12(up(u1) + s0)))[channel 0]
+// Activations are NHWC bfloat16 images (16 channels = 32 bytes per pixel); weights are FP32
+// masters with bfloat16 copies [conv][tap][16 out][16 in] (tap = 3 (dy + 1) + (dx + 1)); every
+// product runs on bf16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32 accumulation.  One
+// convolution wave computes 16 pixels of a row for all 16 output channels: 9 MFMAs, one per
+// tap, with K = the 16 input channels.  The weight gradient is the same contraction with K =
+// 16 pixels.  The solve keeps its vectors and reductions in FP64; the network sees the
+// residual scaled to unit RMS, so the preconditioner is homogeneous of degree one.
+#include <math.h>
+
+#include <vector>
+
+#include "common.h"
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct oecl_unet {
+    uint32_t H, W;            // level-0 image (multiples of 8)
+    float *Wm, *bm;           // FP32 masters [14][9][16][16], [14][16]
+    float *mW, *vW, *mb, *vb; // Adam moments
+    float *gW, *gb;           // gradients
+    uint16_t *Wb, *WbT;       // bf16 weights, and flipped/transposed for the data gradient
+    uint16_t *act[32];        // activations (see the layout in unet_layout)
+    uint16_t *grad[8];        // gradient scratch images
+    float *out;               // network output, H * W
+    double *scal;             // device scalars (scale, reductions)
+    double *part;             // per-block partial sums
+    char *mem;
+    uint32_t adamT;
+};
+
+namespace {
+
+constexpr int C = 16;
+constexpr int NCONV = 14;
+constexpr int WSZ = 9 * C * C;  // weights per convolution
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// float -> bfloat16, round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ v4s load4(const uint16_t *p) { return *reinterpret_cast<const v4s *>(p); }
+
+// out = conv3x3(in) + bias over an H x W image.  MODE 0: bf16 out with ReLU, 1: bf16 out
+// without ReLU, 2: FP32 channel 0 only.  Wave w: row w / segs, pixels 16 (w % segs) .. +15.
+// MFMA operand layouts (16x16x16): lane l holds A[l % 16][4 (l / 16) + j], B[4 (l / 16) + j][l % 16]
+// and D[4 (l / 16) + j][l % 16], j = 0..3; here A = weights (out x in), B = input (in x pixel).
+template <int MODE>
+__global__ void __launch_bounds__(kThreads)
+    k_conv(int H, int W, const uint16_t *__restrict__ in, const uint16_t *__restrict__ wt,
+           const float *__restrict__ bias, void *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4;
+    const int segs = (W + 15) >> 4;
+    const int wave = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    if (wave >= H * segs) return;  // whole wave
+    const int y = wave / segs, x = ((wave % segs) << 4) + px;
+    v4s a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = load4(wt + (k * C + px) * C + 4 * cq);
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int yy = y + k / 3 - 1, xx = x + k % 3 - 1;
+        v4s b = {0, 0, 0, 0};
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) b = load4(in + ((size_t)yy * W + xx) * C + 4 * cq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[k], b, acc, 0, 0, 0);
+    }
+    if (x >= W) return;
+    const size_t p = (size_t)y * W + x;
+    if (MODE == 2) {
+        if (cq == 0) static_cast<float *>(out)[p] = acc[0] + bias[0];
+        return;
+    }
+    v4s o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float v = acc[j] + bias[4 * cq + j];
+        if (MODE == 0) v = v > 0.f ? v : 0.f;
+        o[j] = (short)f2bf(v);
+    }
+    *reinterpret_cast<v4s *>(static_cast<uint16_t *>(out) + p * C + 4 * cq) = o;
+}
+
+// Weight and bias gradients of one convolution: gW[k][o][i] += sum_p g[p][o] x[p + off_k][i],
+// gb[o] += sum_p g[p][o].  MFMA with M = out channels, N = in channels, K = 16 pixels of a row
+// segment; waves stride over the segments, a block sums its waves in LDS and adds to gW / gb
+// with one atomic per entry.
+__global__ void __launch_bounds__(kThreads)
+    k_conv_wgrad(int H, int W, const uint16_t *__restrict__ x, const uint16_t *__restrict__ g,
+                 float *__restrict__ gW, float *__restrict__ gb)
+{
+    __shared__ float red[WSZ + C];
+    for (int t = threadIdx.x; t < WSZ + C; t += kThreads) red[t] = 0.f;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4;
+    const int segs = (W + 15) >> 4, nSeg = H * segs;
+    const int wave = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+    const int nWaves = gridDim.x * (kThreads / 64);
+    v4f acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k] = v4f{0.f, 0.f, 0.f, 0.f};
+    float bs = 0.f;
+    for (int s = wave; s < nSeg; s += nWaves) {
+        const int y = s / segs, x0 = (s % segs) << 4;
+        // A[o = px][p = 4 cq + j] = g[x0 + 4 cq + j][o]
+        v4s a;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int xx = x0 + 4 * cq + j;
+            const uint16_t v = xx < W ? g[((size_t)y * W + xx) * C + px] : (uint16_t)0;
+            a[j] = (short)v;
+            bs += bf2f(v);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int yy = y + k / 3 - 1;
+            v4s b;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int xo = x0 + 4 * cq + j, xx = xo + k % 3 - 1;
+                const bool ok = yy >= 0 && yy < H && xx >= 0 && xx < W && xo < W;
+                b[j] = ok ? (short)x[((size_t)yy * W + xx) * C + px] : (short)0;
+            }
+            acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[k], 0, 0, 0);
+        }
+    }
+    // D[o = 4 cq + j][i = px]
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) atomicAdd(&red[(k * C + 4 * cq + j) * C + px], acc[k][j]);
+    atomicAdd(&red[WSZ + px], bs);
+    __syncthreads();
+    for (int t = threadIdx.x; t < WSZ; t += kThreads) atomicAdd(gW + t, red[t]);
+    if (threadIdx.x < C) atomicAdd(gb + threadIdx.x, red[WSZ + threadIdx.x]);
+}
+
+// g *= (act > 0): the ReLU of a layer applied to the gradient of its output (8-byte groups).
+__global__ void k_relu_back(size_t n4, uint16_t *__restrict__ g, const uint16_t *__restrict__ act)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    v4s gv = load4(g + 4 * i), av = load4(act + 4 * i);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (!(bf2f((uint16_t)av[j]) > 0.f)) gv[j] = 0;
+    *reinterpret_cast<v4s *>(g + 4 * i) = gv;
+}
+
+// out (H x W) = 2x2 average of in (2H x 2W); one thread per 4 channels of an output pixel.
+__global__ void k_pool(int H, int W, const uint16_t *__restrict__ in, uint16_t *__restrict__ out)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)H * W * 4) return;
+    const int q = t & 3;
+    const size_t p = t >> 2;
+    const int y = (int)(p / W), x = (int)(p % W), W2 = 2 * W;
+    const v4s a = load4(in + ((size_t)(2 * y) * W2 + 2 * x) * C + 4 * q);
+    const v4s b = load4(in + ((size_t)(2 * y) * W2 + 2 * x + 1) * C + 4 * q);
+    const v4s c = load4(in + ((size_t)(2 * y + 1) * W2 + 2 * x) * C + 4 * q);
+    const v4s d = load4(in + ((size_t)(2 * y + 1) * W2 + 2 * x + 1) * C + 4 * q);
+    v4s o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        o[j] = (short)f2bf(0.25f * (((bf2f(a[j]) + bf2f(b[j])) + bf2f(c[j])) + bf2f(d[j])));
+    *reinterpret_cast<v4s *>(out + p * C + 4 * q) = o;
+}
+
+// out (H x W) = up(u) + s, u (H/2 x W/2).  BACK: out (H x W) += 0.25 up(u) instead (the pooling's
+// adjoint accumulated into a skip gradient).
+template <bool BACK>
+__global__ void k_up_add(int H, int W, const uint16_t *__restrict__ u, const uint16_t *__restrict__ s,
+                         uint16_t *__restrict__ out)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)H * W * 4) return;
+    const int q = t & 3;
+    const size_t p = t >> 2;
+    const int y = (int)(p / W), x = (int)(p % W);
+    const v4s uv = load4(u + ((size_t)(y >> 1) * (W >> 1) + (x >> 1)) * C + 4 * q);
+    const v4s sv = load4(s + p * C + 4 * q);
+    v4s o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        o[j] = (short)f2bf(BACK ? bf2f(sv[j]) + 0.25f * bf2f(uv[j]) : bf2f(uv[j]) + bf2f(sv[j]));
+    *reinterpret_cast<v4s *>(out + p * C + 4 * q) = o;
+}
+
+// out (H x W) = sum of the 2x2 block of g (2H x 2W): the adjoint of nearest upsampling.
+__global__ void k_up_back(int H, int W, const uint16_t *__restrict__ g, uint16_t *__restrict__ out)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)H * W * 4) return;
+    const int q = t & 3;
+    const size_t p = t >> 2;
+    const int y = (int)(p / W), x = (int)(p % W), W2 = 2 * W;
+    const v4s a = load4(g + ((size_t)(2 * y) * W2 + 2 * x) * C + 4 * q);
+    const v4s b = load4(g + ((size_t)(2 * y) * W2 + 2 * x + 1) * C + 4 * q);
+    const v4s c = load4(g + ((size_t)(2 * y + 1) * W2 + 2 * x) * C + 4 * q);
+    const v4s d = load4(g + ((size_t)(2 * y + 1) * W2 + 2 * x + 1) * C + 4 * q);
+    v4s o;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = (short)f2bf(((bf2f(a[j]) + bf2f(b[j])) + bf2f(c[j])) + bf2f(d[j]));
+    *reinterpret_cast<v4s *>(out + p * C + 4 * q) = o;
+}
+
+// bf16 weights, and the flipped/transposed copy WT[k][i][o] = W[8 - k][o][i] that turns the
+// data gradient into a forward convolution.
+__global__ void k_weights_bf16(const float *__restrict__ Wm, uint16_t *__restrict__ Wb, uint16_t *__restrict__ WbT)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= NCONV * WSZ) return;
+    const int c = t / WSZ, r = t % WSZ, k = r / (C * C), o = (r / C) % C, i = r % C;
+    const uint16_t v = f2bf(Wm[t]);
+    Wb[t] = v;
+    WbT[c * WSZ + ((8 - k) * C + i) * C + o] = v;
+}
+
+// Adam (Kingma & Ba, 2015) on the FP32 masters; bias corrections c1 = 1 - beta1^t, c2 = 1 - beta2^t.
+__global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    return v;
+}
+
+// Sum over the block (valid in thread 0) of K values.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K])
+{
+    __shared__ double lds[kThreads / 64][K];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double s = 0.;
+            for (int w = 0; w < kThreads / 64; ++w) s += lds[w][k];
+            v[k] = s;
+        }
+}
+
+// Final sum of K partials per block into s[off .. off + K), in a fixed order.
+template <int K>
+__global__ void __launch_bounds__(kThreads) k_finish(const double *__restrict__ part, uint32_t nb, double *__restrict__ s, int off)
+{
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = 0.;
+    for (uint32_t b = threadIdx.x; b < nb; b += kThreads)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += part[K * b + k];
+    block_sum<K>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) s[off + k] = v[k];
+}
+
+// Partial sums of r^2 and of the inside-node count (for the RMS scale of the network input).
+__global__ void __launch_bounds__(kThreads) k_rms_part(uint32_t np, const uint8_t *__restrict__ mask, const double *__restrict__ r, double *__restrict__ part)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v[2] = {0., 0.};
+    if (i < np && mask[i]) {
+        v[0] = r[i] * r[i];
+        v[1] = 1.;
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// Network input image (H x W x 16 bf16): channel 0 = r / rms on inside nodes of the grid (nx x
+// ny, row-major), zero elsewhere and in the padding; sc[0] = sum r^2, sc[1] = inside count.
+__global__ void k_net_input(uint32_t nx, uint32_t ny, int H, int W, const uint8_t *__restrict__ mask,
+                            const double *__restrict__ r, const double *__restrict__ sc, uint16_t *__restrict__ img)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)H * W * 4) return;
+    const int q = t & 3;
+    const size_t p = t >> 2;
+    const uint32_t y = (uint32_t)(p / W), x = (uint32_t)(p % W);
+    v4s o = {0, 0, 0, 0};
+    if (q == 0 && y < ny && x < nx) {
+        const uint32_t i = y * nx + x;
+        const double rms = sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.;
+        if (mask[i]) o[0] = (short)f2bf((float)(r[i] / rms));
+    }
+    *reinterpret_cast<v4s *>(img + p * C + 4 * q) = o;
+}
+
+// z = rms * y on inside nodes, 0 elsewhere.
+__global__ void k_net_output(uint32_t nx, uint32_t np, int W, const uint8_t *__restrict__ mask,
+                             const float *__restrict__ y, const double *__restrict__ sc, double *__restrict__ z)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const double rms = sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.;
+    z[i] = mask[i] ? rms * (double)y[(size_t)(i / nx) * W + i % nx] : 0.;
+}
+
+__device__ __forceinline__ double apply_a(const double *__restrict__ cx, const double *__restrict__ cy,
+                                          const double *__restrict__ x, uint32_t p, uint32_t nx)
+{
+    const double xp = x[p];
+    return cx[p] * (xp - x[p + 1]) - cx[p - 1] * (x[p - 1] - xp) + cy[p] * (xp - x[p + nx]) -
+           cy[p - nx] * (x[p - nx] - xp);
+}
+
+// Training loss: the A-norm error of the best multiple of the preconditioner's output z for
+// the right-hand side q (exact solution phi = A^-1 q), which does not depend on the scale of
+// z (the flexible CG does not either):
+//     L = min_c (c z - phi)^T A (c z - phi) / (phi^T A phi) = 1 - (z^T q)^2 / ((z^T A z)(phi^T q)).
+// With a = z^T q, b = z^T A z, c = phi^T q: dL/dz = (2 a / (b c)) ((a / b) A z - q), and
+// z = rms * y on inside nodes.  First pass: z.  Second: A z and the partials (b, c, a).
+// Third: dL/dy = rms dL/dz into channel 0 of the gradient image.
+__global__ void k_loss_z(uint32_t nx, uint32_t np, int W, const uint8_t *__restrict__ mask, const float *__restrict__ y,
+                         const double *__restrict__ sc, double *__restrict__ z)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const double rms = sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.;
+    z[i] = mask[i] ? rms * (double)y[(size_t)(i / nx) * W + i % nx] : 0.;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_loss_az(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+              const uint8_t *__restrict__ mask, const double *__restrict__ z, const double *__restrict__ phi,
+              const double *__restrict__ q, double *__restrict__ az, double *__restrict__ part)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v[3] = {0., 0., 0.};
+    if (i < np) {
+        double a = 0.;
+        if (mask[i]) {
+            a = apply_a(cx, cy, z, i, nx);
+            v[0] = z[i] * a;
+            v[1] = phi[i] * q[i];
+            v[2] = z[i] * q[i];
+        }
+        az[i] = a;
+    }
+    block_sum<3>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = v[k];
+}
+
+// sc[4] = z^T A z, sc[5] = phi^T q, sc[6] = z^T q
+__global__ void k_loss_grad(uint32_t nx, uint32_t ny, int H, int W, const uint8_t *__restrict__ mask,
+                            const double *__restrict__ az, const double *__restrict__ q, const double *__restrict__ sc,
+                            uint16_t *__restrict__ gimg)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)H * W * 4) return;
+    const int qd = t & 3;
+    const size_t p = t >> 2;
+    const uint32_t y = (uint32_t)(p / W), x = (uint32_t)(p % W);
+    v4s o = {0, 0, 0, 0};
+    if (qd == 0 && y < ny && x < nx && mask[y * nx + x]) {
+        const double rms = sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.;
+        const double b = sc[4], c = sc[5], a = sc[6];
+        const size_t i = (size_t)y * nx + x;
+        const double g = (b != 0. && c != 0.) ? (2. * a / (b * c)) * ((a / b) * az[i] - q[i]) : 0.;
+        o[0] = (short)f2bf((float)(rms * g));
+    }
+    *reinterpret_cast<v4s *>(gimg + p * C + 4 * qd) = o;
+}
+
+// ---- flexible conjugate gradients (Notay 2000, truncation to one direction) ----
+// scalars: s[0] rms part (r.r), s[1] inside count, s[2] q.q, s[3] alpha, s[4] beta, s[5] r.z, s[6] p.Ap,
+// s[7] z.Ap, s[8] r.r (after the update)
+__global__ void __launch_bounds__(kThreads)
+    k_fcg_init(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+               const uint8_t *__restrict__ mask, const double *__restrict__ q, double *__restrict__ phi,
+               double *__restrict__ r, double *__restrict__ part)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v[2] = {0., 0.};
+    if (i < np) {
+        double ri = 0.;
+        if (mask[i]) {
+            ri = q[i] - apply_a(cx, cy, phi, i, nx);
+            v[0] = ri * ri;
+            v[1] = q[i] * q[i];
+        }
+        r[i] = ri;
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+__global__ void k_mask_zero(uint32_t np, const uint8_t *__restrict__ mask, double *__restrict__ x)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np && !mask[i]) x[i] = 0.;
+}
+
+// partial r.z
+__global__ void __launch_bounds__(kThreads) k_dot(uint32_t np, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ part)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v[1] = {i < np ? a[i] * b[i] : 0.};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// p = z + beta p (beta = s[4]; first = 1: p = z)
+__global__ void k_fcg_dir(uint32_t np, const double *__restrict__ s, const double *__restrict__ z, double *__restrict__ p, int first)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) p[i] = first ? z[i] : z[i] + s[4] * p[i];
+}
+
+// Ap, partial (p.Ap, z.Ap) -- z.Ap with the previous z gives the next beta's denominator term
+__global__ void __launch_bounds__(kThreads)
+    k_fcg_ap(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+             const uint8_t *__restrict__ mask, const double *__restrict__ p, double *__restrict__ ap, double *__restrict__ part)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v[1] = {0.};
+    if (i < np) {
+        double a = 0.;
+        if (mask[i]) {
+            a = apply_a(cx, cy, p, i, nx);
+            v[0] = p[i] * a;
+        }
+        ap[i] = a;
+    }
+    block_sum<1>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// alpha = r.z / p.Ap
+__global__ void k_alpha(double *s)
+{
+    s[3] = s[6] != 0. ? s[5] / s[6] : 0.;
+}
+
+// phi += alpha p, r -= alpha Ap; partial r.r
+__global__ void __launch_bounds__(kThreads)
+    k_fcg_update(uint32_t np, const double *__restrict__ s, double *__restrict__ phi, double *__restrict__ r,
+                 const double *__restrict__ p, const double *__restrict__ ap, double *__restrict__ part)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v[2] = {0., 0.};
+    if (i < np) {
+        const double a = s[3];
+        const double ri = r[i] - a * ap[i];
+        phi[i] += a * p[i];
+        r[i] = ri;
+        v[0] = ri * ri;
+        v[1] = 1.;
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// After the new z: partial (r.z, z.Ap) -> beta = -alpha z.Ap / (r.z)_old (flexible CG)
+__global__ void __launch_bounds__(kThreads)
+    k_fcg_zdots(uint32_t np, const double *__restrict__ r, const double *__restrict__ z, const double *__restrict__ ap,
+                double *__restrict__ part)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v[2] = {0., 0.};
+    if (i < np) {
+        v[0] = r[i] * z[i];
+        v[1] = z[i] * ap[i];
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+__global__ void k_beta(double *s, const double *t)
+{
+    // t[0] = new r.z, t[1] = z.Ap; s[5] = old r.z
+    const double rzOld = s[5];
+    s[4] = rzOld != 0. ? -s[3] * t[1] / rzOld : 0.;
+    s[5] = t[0];
+}
+
+__global__ void k_copy_rz(double *s, const double *t) { s[5] = t[0]; }
+
+uint32_t nblk(size_t n, int b = kThreads) { return (uint32_t)((n + b - 1) / b); }
+
+// Activation images per level l (H_l = H >> l): x0 (input), a (first conv), s (second conv /
+// skip), p (pooled input of the level), t (up + skip input of an up level), u (up level's
+// first conv), v (up level's output).  Index: 7 l + slot.
+enum { IX = 0, IA = 1, IS = 2, IP = 3, IT = 4, IU = 5, IV = 6 };
+
+size_t img_elems(const oecl_unet *u, int l) { return (size_t)(u->H >> l) * (u->W >> l) * C; }
+
+int launch_conv(const oecl_unet *u, int l, int mode, const uint16_t *in, const uint16_t *w, const float *b, void *out,
+                hipStream_t s)
+{
+    const int H = u->H >> l, W = u->W >> l;
+    const uint32_t waves = (uint32_t)H * ((W + 15) / 16);
+    const dim3 g(nblk(waves, kThreads / 64)), bl(kThreads);
+    if (mode == 0) hipLaunchKernelGGL(k_conv<0>, g, bl, 0, s, H, W, in, w, b, out);
+    else if (mode == 1) hipLaunchKernelGGL(k_conv<1>, g, bl, 0, s, H, W, in, w, b, out);
+    else hipLaunchKernelGGL(k_conv<2>, g, bl, 0, s, H, W, in, w, b, out);
+    return OECL_OK;
+}
+
+const uint16_t *Wc(const oecl_unet *u, int c) { return u->Wb + (size_t)c * WSZ; }
+const uint16_t *WTc(const oecl_unet *u, int c) { return u->WbT + (size_t)c * WSZ; }
+const float *bc(const oecl_unet *u, int c) { return u->bm + (size_t)c * C; }
+
+// Forward pass from the input image act[IX] (level 0) to u->out.
+void unet_forward(oecl_unet *u, hipStream_t s)
+{
+    uint16_t **A = u->act;
+    auto pool = [&](int l, const uint16_t *in, uint16_t *out) {  // level l-1 -> l
+        const int H = u->H >> l, W = u->W >> l;
+        hipLaunchKernelGGL(k_pool, dim3(nblk((size_t)H * W * 4)), dim3(kThreads), 0, s, H, W, in, out);
+    };
+    auto upadd = [&](int l, const uint16_t *uu, const uint16_t *sk, uint16_t *out) {  // level l+1 -> l
+        const int H = u->H >> l, W = u->W >> l;
+        hipLaunchKernelGGL(k_up_add<false>, dim3(nblk((size_t)H * W * 4)), dim3(kThreads), 0, s, H, W, uu, sk, out);
+    };
+    launch_conv(u, 0, 0, A[IX], Wc(u, 0), bc(u, 0), A[IA], s);
+    launch_conv(u, 0, 0, A[IA], Wc(u, 1), bc(u, 1), A[IS], s);
+    for (int l = 1; l <= 3; ++l) {
+        pool(l, A[7 * (l - 1) + IS], A[7 * l + IP]);
+        launch_conv(u, l, 0, A[7 * l + IP], Wc(u, 2 * l), bc(u, 2 * l), A[7 * l + IA], s);
+        launch_conv(u, l, 0, A[7 * l + IA], Wc(u, 2 * l + 1), bc(u, 2 * l + 1), A[7 * l + IS], s);
+    }
+    // level 3's output is u3 = A[7*3 + IS]
+    for (int l = 2; l >= 0; --l) {
+        const uint16_t *below = l == 2 ? A[7 * 3 + IS] : A[7 * (l + 1) + IV];
+        upadd(l, below, A[7 * l + IS], A[7 * l + IT]);
+        const int c0 = 8 + 2 * (2 - l);
+        launch_conv(u, l, 0, A[7 * l + IT], Wc(u, c0), bc(u, c0), A[7 * l + IU], s);
+        if (l > 0) launch_conv(u, l, 0, A[7 * l + IU], Wc(u, c0 + 1), bc(u, c0 + 1), A[7 * l + IV], s);
+        else launch_conv(u, 0, 2, A[IU], Wc(u, 13), bc(u, 13), u->out, s);
+    }
+}
+
+// Backward pass: gout (level-0 image, channel 0 = dL/dy) -> gW, gb (accumulated) and, when gin
+// is not null, the gradient of the input image.  Scratch: P_l = grad[2 l], Q_l = grad[2 l + 1]
+// (level-l size).  The skip gradients (grad of t_l, which feeds s_l directly) stay in Q_0, P_1,
+// P_2 until the down path adds the pooling adjoint of the level below and consumes them.
+void unet_backward(oecl_unet *u, const uint16_t *gout, uint16_t *gin, hipStream_t s)
+{
+    uint16_t **A = u->act;
+    uint16_t *P[4], *Q[4];
+    for (int l = 0; l < 4; ++l) {
+        P[l] = u->grad[2 * l];
+        Q[l] = u->grad[2 * l + 1];
+    }
+    auto relu_back = [&](int l, uint16_t *g, const uint16_t *act) {
+        const size_t n4 = img_elems(u, l) / 4;
+        hipLaunchKernelGGL(k_relu_back, dim3(nblk(n4)), dim3(kThreads), 0, s, n4, g, act);
+    };
+    // conv c at level l: input x, gradient g of its (pre-activation) output -> wgrad, and the
+    // input gradient into gx (when not null)
+    auto convb = [&](int l, int c, const uint16_t *x, const uint16_t *g, uint16_t *gx) {
+        const int H = u->H >> l, W = u->W >> l;
+        const uint32_t segs = (uint32_t)H * ((W + 15) / 16);
+        uint32_t blocks = nblk(segs, kThreads / 64);
+        if (blocks > 1024) blocks = 1024;
+        hipLaunchKernelGGL(k_conv_wgrad, dim3(blocks), dim3(kThreads), 0, s, H, W, x, g, u->gW + (size_t)c * WSZ,
+                           u->gb + (size_t)c * C);
+        if (gx) launch_conv(u, l, 1, g, WTc(u, c), u->bm + (size_t)NCONV * C /* zero bias row */, gx, s);
+    };
+    auto up_back = [&](int l, const uint16_t *big, uint16_t *small) {  // level l-1 -> level l
+        const int H = u->H >> l, W = u->W >> l;
+        hipLaunchKernelGGL(k_up_back, dim3(nblk((size_t)H * W * 4)), dim3(kThreads), 0, s, H, W, big, small);
+    };
+    auto pool_back_add = [&](int l, const uint16_t *small, uint16_t *skip) {  // level l+1 -> level l
+        const int H = u->H >> l, W = u->W >> l;
+        hipLaunchKernelGGL(k_up_add<true>, dim3(nblk((size_t)H * W * 4)), dim3(kThreads), 0, s, H, W, small, skip,
+                           skip);
+    };
+    // up level 0: y = c13(relu(c12(t0)))
+    convb(0, 13, A[IU], gout, P[0]);
+    relu_back(0, P[0], A[IU]);
+    convb(0, 12, A[IT], P[0], Q[0]);              // Q0 = grad t0 (skip of s0)
+    // up level 1: u1 = relu(c11(relu(c10(t1))))
+    up_back(1, Q[0], P[1]);                        // grad u1
+    relu_back(1, P[1], A[7 + IV]);
+    convb(1, 11, A[7 + IU], P[1], Q[1]);
+    relu_back(1, Q[1], A[7 + IU]);
+    convb(1, 10, A[7 + IT], Q[1], P[1]);           // P1 = grad t1 (skip of s1)
+    // up level 2: u2 = relu(c9(relu(c8(t2))))
+    up_back(2, P[1], P[2]);                        // grad u2
+    relu_back(2, P[2], A[14 + IV]);
+    convb(2, 9, A[14 + IU], P[2], Q[2]);
+    relu_back(2, Q[2], A[14 + IU]);
+    convb(2, 8, A[14 + IT], Q[2], P[2]);           // P2 = grad t2 (skip of s2)
+    // bottom, level 3: u3 = relu(c7(relu(c6(pool(s2)))))
+    up_back(3, P[2], P[3]);                        // grad u3
+    relu_back(3, P[3], A[21 + IS]);
+    convb(3, 7, A[21 + IA], P[3], Q[3]);
+    relu_back(3, Q[3], A[21 + IA]);
+    convb(3, 6, A[21 + IP], Q[3], P[3]);           // P3 = grad pool(s2)
+    pool_back_add(2, P[3], P[2]);                  // P2 = grad s2
+    // down level 2: s2 = relu(c5(relu(c4(pool(s1)))))
+    relu_back(2, P[2], A[14 + IS]);
+    convb(2, 5, A[14 + IA], P[2], Q[2]);
+    relu_back(2, Q[2], A[14 + IA]);
+    convb(2, 4, A[14 + IP], Q[2], P[2]);           // P2 = grad pool(s1)
+    pool_back_add(1, P[2], P[1]);                  // P1 = grad s1
+    // down level 1
+    relu_back(1, P[1], A[7 + IS]);
+    convb(1, 3, A[7 + IA], P[1], Q[1]);
+    relu_back(1, Q[1], A[7 + IA]);
+    convb(1, 2, A[7 + IP], Q[1], P[1]);            // P1 = grad pool(s0)
+    pool_back_add(0, P[1], Q[0]);                  // Q0 = grad s0
+    // down level 0: s0 = relu(c1(relu(c0(x))))
+    relu_back(0, Q[0], A[IS]);
+    convb(0, 1, A[IA], Q[0], P[0]);
+    relu_back(0, P[0], A[IA]);
+    convb(0, 0, A[IX], P[0], gin);
+}
+
+}  // namespace
+
+OECL_EXPORT int oecl_unet_create(uint32_t H, uint32_t W, uint32_t seed, oecl_unet **out)
+{
+    if (!out || H < 8 || W < 8 || (H & 7) || (W & 7) || H > 16384 || W > 16384) return OECL_ERR_ARG;
+    oecl_unet *u = new oecl_unet();
+    u->H = H;
+    u->W = W;
+    const size_t nW = (size_t)NCONV * WSZ, nb = (size_t)(NCONV + 1) * C;  // + one zero bias row
+    // carve the buffers out of one allocation: a first pass sizes it, the second assigns
+    char *base = nullptr;
+    size_t used = 0;
+    auto take = [&](size_t b) {
+        char *r = base ? base + used : nullptr;
+        used += (b + 255) / 256 * 256;
+        return r;
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        used = 0;
+        u->Wm = (float *)take(4 * nW);
+        u->bm = (float *)take(4 * nb);
+        u->mW = (float *)take(4 * nW);
+        u->vW = (float *)take(4 * nW);
+        u->mb = (float *)take(4 * nb);
+        u->vb = (float *)take(4 * nb);
+        u->gW = (float *)take(4 * nW);
+        u->gb = (float *)take(4 * nb);
+        u->Wb = (uint16_t *)take(2 * nW);
+        u->WbT = (uint16_t *)take(2 * nW);
+        for (int l = 0; l < 4; ++l)
+            for (int k = 0; k < 7; ++k) u->act[7 * l + k] = (uint16_t *)take(2 * img_elems(u, l));
+        for (int l = 0; l < 4; ++l) {
+            u->grad[2 * l] = (uint16_t *)take(2 * img_elems(u, l));
+            u->grad[2 * l + 1] = (uint16_t *)take(2 * img_elems(u, l));
+        }
+        u->out = (float *)take(4ull * H * W);
+        u->scal = (double *)take(8 * 64);
+        u->part = (double *)take(8ull * 4 * (nblk((size_t)H * W) + 16));
+        if (pass == 0) {
+            if (hipMalloc(&base, used) != hipSuccess) {
+                delete u;
+                oecl_set_error_msg("unet alloc failed");
+                return OECL_ERR_HIP;
+            }
+            u->mem = base;
+        }
+    }
+    OECL_HIP(hipMemset(u->mem, 0, used), "unet clear");
+    // He initialisation from a fixed-seed host generator (xorshift + Box-Muller)
+    std::vector<float> w(nW);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ ((uint64_t)seed << 17 | seed);
+    auto next = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const double sd = sqrt(2. / (9. * C));
+    for (size_t i = 0; i < nW; ++i) {
+        const double a = next(), b = next();
+        w[i] = (float)(sd * sqrt(-2. * log(1. - a)) * cos(6.283185307179586 * b));
+    }
+    OECL_HIP(hipMemcpy(u->Wm, w.data(), 4 * nW, hipMemcpyHostToDevice), "unet weights");
+    hipLaunchKernelGGL(k_weights_bf16, dim3(nblk(nW)), dim3(kThreads), 0, 0, u->Wm, u->Wb, u->WbT);
+    OECL_CHECK_LAUNCH("oecl_unet_create");
+    OECL_HIP(hipDeviceSynchronize(), "unet create");
+    *out = u;
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_unet_destroy(oecl_unet *u)
+{
+    if (!u) return OECL_OK;
+    (void)hipFree(u->mem);
+    delete u;
+    return OECL_OK;
+}
+
+// Host copies of the FP32 parameters: W [14][9][16][16], b [14][16].
+OECL_EXPORT int oecl_unet_set_params(oecl_unet *u, const float *W, const float *b)
+{
+    if (!u || !W || !b) return OECL_ERR_ARG;
+    OECL_HIP(hipMemcpy(u->Wm, W, 4ull * NCONV * WSZ, hipMemcpyHostToDevice), "unet set W");
+    OECL_HIP(hipMemcpy(u->bm, b, 4ull * NCONV * C, hipMemcpyHostToDevice), "unet set b");
+    hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((size_t)NCONV * WSZ)), dim3(kThreads), 0, 0, u->Wm, u->Wb, u->WbT);
+    OECL_CHECK_LAUNCH("oecl_unet_set_params");
+    OECL_HIP(hipDeviceSynchronize(), "unet set");
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_unet_get_params(const oecl_unet *u, float *W, float *b, float *gW, float *gb)
+{
+    if (!u) return OECL_ERR_ARG;
+    OECL_HIP(hipDeviceSynchronize(), "unet get");
+    if (W) OECL_HIP(hipMemcpy(W, u->Wm, 4ull * NCONV * WSZ, hipMemcpyDeviceToHost), "unet get W");
+    if (b) OECL_HIP(hipMemcpy(b, u->bm, 4ull * NCONV * C, hipMemcpyDeviceToHost), "unet get b");
+    if (gW) OECL_HIP(hipMemcpy(gW, u->gW, 4ull * NCONV * WSZ, hipMemcpyDeviceToHost), "unet get gW");
+    if (gb) OECL_HIP(hipMemcpy(gb, u->gb, 4ull * NCONV * C, hipMemcpyDeviceToHost), "unet get gb");
+    return OECL_OK;
+}
+
+// Forward pass of an input image img (H x W x 16 bf16, channel 0 used) into y (H x W FP32).
+OECL_EXPORT int oecl_unet_forward(oecl_unet *u, const uint16_t *img, float *y, void *stream)
+{
+    if (!u || !img) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    OECL_HIP(hipMemcpyAsync(u->act[IX], img, img_elems(u, 0) * 2, hipMemcpyDeviceToDevice, s), "unet input");
+    unet_forward(u, s);
+    if (y) OECL_HIP(hipMemcpyAsync(y, u->out, 4ull * u->H * u->W, hipMemcpyDeviceToDevice, s), "unet output");
+    OECL_CHECK_LAUNCH("oecl_unet_forward");
+    return OECL_OK;
+}
+
+// Backward pass after oecl_unet_forward: gout (H x W x 16 bf16, channel 0 = dL/dy).  The weight
+// and bias gradients are zeroed first when zero != 0, else accumulated; gin (optional) receives
+// the gradient of the input image.
+OECL_EXPORT int oecl_unet_backward(oecl_unet *u, const uint16_t *gout, uint16_t *gin, int zero, void *stream)
+{
+    if (!u || !gout) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (zero) {
+        OECL_HIP(hipMemsetAsync(u->gW, 0, 4ull * NCONV * WSZ, s), "unet zero");
+        OECL_HIP(hipMemsetAsync(u->gb, 0, 4ull * NCONV * C, s), "unet zero");
+    }
+    unet_backward(u, gout, gin, s);
+    OECL_CHECK_LAUNCH("oecl_unet_backward");
+    return OECL_OK;
+}
+
+// One Adam step on the accumulated gradients, then the bf16 weight copies.
+OECL_EXPORT int oecl_unet_adam(oecl_unet *u, float lr, float beta1, float beta2, float eps, void *stream)
+{
+    if (!u) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    u->adamT += 1;
+    const float c1 = 1.f - powf(beta1, (float)u->adamT), c2 = 1.f - powf(beta2, (float)u->adamT);
+    const int nW = NCONV * WSZ, nb = NCONV * C;
+    hipLaunchKernelGGL(k_adam, dim3(nblk(nW)), dim3(kThreads), 0, s, nW, u->Wm, u->gW, u->mW, u->vW, lr, beta1, beta2,
+                       eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(nb)), dim3(kThreads), 0, s, nb, u->bm, u->gb, u->mb, u->vb, lr, beta1, beta2,
+                       eps, c1, c2);
+    hipLaunchKernelGGL(k_weights_bf16, dim3(nblk((size_t)nW)), dim3(kThreads), 0, s, u->Wm, u->Wb, u->WbT);
+    OECL_CHECK_LAUNCH("oecl_unet_adam");
+    return OECL_OK;
+}
+
+// z = rms(r) * Net(r / rms(r)) on the inside nodes of an nx x ny grid (row-major, mask = inside).
+static int precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r, double *z,
+                        hipStream_t s)
+{
+    const uint32_t np = nx * ny, nb = nblk(np);
+    hipLaunchKernelGGL(k_rms_part, dim3(nb), dim3(kThreads), 0, s, np, mask, r, u->part);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nb, u->scal, 0);
+    hipLaunchKernelGGL(k_net_input, dim3(nblk((size_t)u->H * u->W * 4)), dim3(kThreads), 0, s, nx, ny, (int)u->H,
+                       (int)u->W, mask, r, u->scal, u->act[IX]);
+    unet_forward(u, s);
+    hipLaunchKernelGGL(k_net_output, dim3(nb), dim3(kThreads), 0, s, nx, np, (int)u->W, mask, u->out, u->scal, z);
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_unet_precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r,
+                                       double *z, void *stream)
+{
+    if (!u || !mask || !r || !z || ny > u->H || nx > u->W) return OECL_ERR_ARG;
+    int rc = precondition(u, nx, ny, mask, r, z, (hipStream_t)stream);
+    OECL_CHECK_LAUNCH("oecl_unet_precondition");
+    return rc;
+}
+
+// One training sample: forward on q (the right-hand side, scaled to unit RMS), the
+// scale-free A-norm loss against the exact solution phi, backward (gradients accumulated;
+// zeroed when zero != 0).  work: 2 np doubles.  *loss (optional, synchronises) receives L.
+OECL_EXPORT int oecl_unet_train_sample(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                                       const uint8_t *mask, const double *q, const double *phi, double *work,
+                                       int zero, double *loss, void *stream)
+{
+    if (!u || !cx || !cy || !mask || !q || !phi || !work || ny > u->H || nx > u->W) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t np = nx * ny, nb = nblk(np);
+    double *z = work, *az = work + np;
+    int rc = precondition(u, nx, ny, mask, q, z, s);  // z, with the input RMS in scal[0..1]
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_loss_az, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, z, phi, q, az, u->part);
+    hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(kThreads), 0, s, u->part, nb, u->scal, 4);
+    // the output gradient goes into Q_0, which the backward first reads and only later reuses
+    hipLaunchKernelGGL(k_loss_grad, dim3(nblk((size_t)u->H * u->W * 4)), dim3(kThreads), 0, s, nx, ny, (int)u->H,
+                       (int)u->W, mask, az, q, u->scal, u->grad[1]);
+    if (zero) {
+        OECL_HIP(hipMemsetAsync(u->gW, 0, 4ull * NCONV * WSZ, s), "unet zero");
+        OECL_HIP(hipMemsetAsync(u->gb, 0, 4ull * NCONV * C, s), "unet zero");
+    }
+    unet_backward(u, u->grad[1], nullptr, s);
+    OECL_CHECK_LAUNCH("oecl_unet_train_sample");
+    if (loss) {
+        double h[3];
+        OECL_HIP(hipMemcpyAsync(h, u->scal + 4, 24, hipMemcpyDeviceToHost, s), "loss");
+        OECL_HIP(hipStreamSynchronize(s), "loss");
+        *loss = (h[0] != 0. && h[1] != 0.) ? 1. - h[2] * h[2] / (h[0] * h[1]) : 1.;
+    }
+    return OECL_OK;
+}
+
+OECL_EXPORT uint64_t oecl_fcg_work_bytes(uint32_t np) { return 8ull * (5ull * np + 4 * (nblk(np) + 1) + 64) + 256; }
+
+// Flexible CG on the FIT system with the network as preconditioner (u == NULL: Jacobi is not
+// offered here; use oecl_pcg_solve).  phi: initial guess in, solution out (zero off the
+// inside nodes).  Converged when ||q - A phi|| <= tol ||q||; the residual norm is read on the
+// host every `check` iterations.
+OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                               const uint8_t *mask, const double *q, double *phi, double tol, uint32_t maxIter,
+                               uint32_t check, void *work, uint32_t *iters, double *relres, void *stream)
+{
+    if (!u || !cx || !cy || !mask || !q || !phi || !work || ny > u->H || nx > u->W || nx < 3 || ny < 3)
+        return OECL_ERR_ARG;
+    if (check == 0) check = 1;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t np = nx * ny, nb = nblk(np);
+    double *r = (double *)work, *z = r + np, *p = z + np, *ap = p + np;
+    double *part = ap + np, *S = part + 4 * (nb + 1), *T = S + 16;
+    hipLaunchKernelGGL(k_mask_zero, dim3(nb), dim3(kThreads), 0, st, np, mask, phi);
+    hipLaunchKernelGGL(k_fcg_init, dim3(nb), dim3(kThreads), 0, st, nx, np, cx, cy, mask, q, phi, r, part);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 8);  // S[8] r.r, S[9] q.q
+    double h[2];
+    OECL_HIP(hipMemcpyAsync(h, S + 8, 16, hipMemcpyDeviceToHost, st), "fcg copy");
+    OECL_HIP(hipStreamSynchronize(st), "fcg sync");
+    const double qn = sqrt(h[1]) > 0. ? sqrt(h[1]) : 1.;
+    double rr = h[0];
+    uint32_t it = 0;
+    if (sqrt(rr) > tol * qn) {
+        int rc = precondition(u, nx, ny, mask, r, z, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kThreads), 0, st, np, r, z, part);
+        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kThreads), 0, st, part, nb, T, 0);
+        hipLaunchKernelGGL(k_copy_rz, dim3(1), dim3(1), 0, st, S, T);
+        hipLaunchKernelGGL(k_fcg_dir, dim3(nb), dim3(kThreads), 0, st, np, S, z, p, 1);
+    }
+    while (sqrt(rr) > tol * qn && it < maxIter) {
+        for (uint32_t k = 0; k < check && it < maxIter; ++k, ++it) {
+            hipLaunchKernelGGL(k_fcg_ap, dim3(nb), dim3(kThreads), 0, st, nx, np, cx, cy, mask, p, ap, part);
+            hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 6);
+            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1), 0, st, S);
+            hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kThreads), 0, st, np, S, phi, r, p, ap, part);
+            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 8);
+            int rc = precondition(u, nx, ny, mask, r, z, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_fcg_zdots, dim3(nb), dim3(kThreads), 0, st, np, r, z, ap, part);
+            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, T, 0);
+            hipLaunchKernelGGL(k_beta, dim3(1), dim3(1), 0, st, S, T);
+            hipLaunchKernelGGL(k_fcg_dir, dim3(nb), dim3(kThreads), 0, st, np, S, z, p, 0);
+        }
+        OECL_CHECK_LAUNCH("oecl_fcg_solve");
+        OECL_HIP(hipMemcpyAsync(h, S + 8, 8, hipMemcpyDeviceToHost, st), "fcg copy");
+        OECL_HIP(hipStreamSynchronize(st), "fcg sync");
+        rr = h[0];
+        if (!(rr == rr)) break;
+    }
+    if (iters) *iters = it;
+    const double rel = sqrt(rr) / qn;
+    if (relres) *relres = rel;
+    return rel <= tol ? OECL_OK : OECL_ERR_NOCONV;
+}

@@ -1,0 +1,172 @@
+"""Learned preconditioner for the Poisson solve (no counterpart in the reference, whose solve is
+a SuperLU back-substitution, poissonSolver.pyx:59,85).
+
+UNetPreconditioner wraps the device U-Net of oecl_unet_* (4 levels, 16 channels, 3x3
+convolutions, bf16 MFMA with FP32 accumulation).  It is trained on the device on pairs
+(q, phi = A^-1 q) of the grid's FIT system with the loss 1 - (z^T q)^2 / ((z^T A z)(phi^T q)):
+the A-norm error of the best multiple of the preconditioner's output z for q, relative to
+phi's A-norm (a preconditioner's scale does not matter to the flexible CG that applies it,
+so the loss ignores it too).  It is applied inside a flexible CG that
+keeps its vectors and reductions in FP64.  The network sees the residual scaled to unit RMS,
+so the preconditioner is homogeneous of degree one.  solve() falls back to the direct solve
+when the flexible CG does not reach its tolerance.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+NCONV, C = 14, 16
+
+
+def image_size(gridObj):
+    """(H, W): the extended grid rounded up to multiples of 8."""
+    ny, nx = gridObj.getNyExt(), gridObj.getNxExt()
+    return (ny + 7) // 8 * 8, (nx + 7) // 8 * 8
+
+
+class UNetPreconditioner:
+
+    def __init__(self, gridObj, seed=0):
+        self.g = gridObj
+        self.nx, self.ny = gridObj.getNxExt(), gridObj.getNyExt()
+        self.H, self.W = image_size(gridObj)
+        h = ctypes.c_void_p()
+        L.check(L.lib().oecl_unet_create(L.u32(self.H), L.u32(self.W), L.u32(seed), ctypes.byref(h)), 'unet create')
+        self._h = h
+        dev = L.device()
+        np_ = self.nx * self.ny
+        self._work = torch.empty(2 * np_, dtype=torch.float64, device=dev)
+        self._fcgwork = torch.empty(int(L.lib().oecl_fcg_work_bytes(np_)) // 8 + 8, dtype=torch.float64,
+                                    device=dev)
+        self.history = []
+        self.lastIterations = 0
+        self.lastRelres = None
+        self.fallbacks = 0
+
+    def __del__(self):
+        try:
+            if self._h is not None and self._h.value:
+                L.lib().oecl_unet_destroy(self._h)
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- parameters
+    def get_params(self):
+        """(W [14][9][16][16], b [14][16]) as float32 host arrays."""
+        W = np.zeros((NCONV, 9, C, C), np.float32)
+        b = np.zeros((NCONV, C), np.float32)
+        L.check(L.lib().oecl_unet_get_params(self._h, W.ctypes.data_as(ctypes.c_void_p),
+                                             b.ctypes.data_as(ctypes.c_void_p), None, None), 'unet get')
+        return W, b
+
+    def set_params(self, W, b):
+        W = np.ascontiguousarray(W, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        assert W.shape == (NCONV, 9, C, C) and b.shape == (NCONV, C)
+        L.check(L.lib().oecl_unet_set_params(self._h, W.ctypes.data_as(ctypes.c_void_p),
+                                             b.ctypes.data_as(ctypes.c_void_p)), 'unet set')
+
+    def get_grads(self):
+        gW = np.zeros((NCONV, 9, C, C), np.float32)
+        gb = np.zeros((NCONV, C), np.float32)
+        L.check(L.lib().oecl_unet_get_params(self._h, None, None, gW.ctypes.data_as(ctypes.c_void_p),
+                                             gb.ctypes.data_as(ctypes.c_void_p)), 'unet grads')
+        return gW, gb
+
+    def save(self, path):
+        W, b = self.get_params()
+        np.savez(path, W=W, b=b)
+
+    def load(self, path):
+        d = np.load(path)
+        self.set_params(d['W'], d['b'])
+
+    # ---------------------------------------------------------------- raw network
+    def forward_image(self, img):
+        """img: (H, W, 16) bfloat16 device tensor -> (H, W) float32 device tensor."""
+        assert img.dtype == torch.bfloat16 and tuple(img.shape) == (self.H, self.W, C)
+        y = torch.empty((self.H, self.W), dtype=torch.float32, device=L.device())
+        L.check(L.lib().oecl_unet_forward(self._h, L.ptr(img.contiguous()), L.ptr(y), L.stream()), 'unet forward')
+        return y
+
+    def backward_image(self, gout, zero=True):
+        """After forward_image: gout (H, W, 16) bf16 -> gradient of the input image (H, W, 16)
+        bf16; the parameter gradients are read with get_grads()."""
+        gin = torch.empty((self.H, self.W, C), dtype=torch.bfloat16, device=L.device())
+        L.check(L.lib().oecl_unet_backward(self._h, L.ptr(gout.contiguous()), L.ptr(gin), L.i32(1 if zero else 0),
+                                           L.stream()), 'unet backward')
+        return gin
+
+    # ---------------------------------------------------------------- on the grid
+    def precondition(self, ps, r):
+        """z = rms(r) * network(r / rms(r)) on the solver's inside nodes."""
+        z = torch.empty_like(r)
+        L.check(L.lib().oecl_unet_precondition(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.mask), L.ptr(r),
+                                               L.ptr(z), L.stream()), 'unet precondition')
+        return z
+
+    def train(self, ps, pairs, steps, lr=1e-3, batch=1, lrEnd=None, seed=0, log=None, logEvery=0):
+        """Adam on samples (q, phi) drawn from `pairs` (device FP64 grid vectors, phi = A^-1 q),
+        `batch` samples per update; the learning rate decays geometrically to lrEnd.  Returns
+        the list of (step, mean loss over the last logEvery steps) when logEvery > 0."""
+        rng = np.random.default_rng(seed)
+        lib = L.lib()
+        lrEnd = lr if lrEnd is None else lrEnd
+        out, acc, cnt = [], 0., 0
+        for k in range(int(steps)):
+            lk = lr * (lrEnd / lr) ** (k / max(1, steps - 1))
+            for j in range(batch):
+                q, phi = pairs[int(rng.integers(len(pairs)))]
+                want = logEvery and (k % logEvery == logEvery - 1 or k == steps - 1) and j == 0
+                loss = ctypes.c_double(0.)
+                L.check(lib.oecl_unet_train_sample(
+                    self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx), L.ptr(ps.cy), L.ptr(ps.mask), L.ptr(q),
+                    L.ptr(phi), L.ptr(self._work), L.i32(1 if j == 0 else 0),
+                    ctypes.byref(loss) if want else None, L.stream()), 'unet train')
+                if want:
+                    acc += loss.value
+                    cnt += 1
+            L.check(lib.oecl_unet_adam(self._h, ctypes.c_float(lk), ctypes.c_float(0.9), ctypes.c_float(0.999),
+                                       ctypes.c_float(1e-8), L.stream()), 'unet adam')
+            if logEvery and (k % logEvery == logEvery - 1 or k == steps - 1) and cnt:
+                out.append((k + 1, acc / cnt))
+                if log:
+                    log('train step %d: loss %.4g' % (k + 1, acc / cnt))
+                acc, cnt = 0., 0
+        self.history.extend(out)
+        return out
+
+    def loss(self, ps, q, phi):
+        """The training loss of one pair (does not change the parameters' values)."""
+        v = ctypes.c_double(0.)
+        L.check(L.lib().oecl_unet_train_sample(
+            self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx), L.ptr(ps.cy), L.ptr(ps.mask), L.ptr(q), L.ptr(phi),
+            L.ptr(self._work), L.i32(1), ctypes.byref(v), L.stream()), 'unet loss')
+        return v.value
+
+    def fcg(self, ps, q, phi=None, tol=1e-8, maxIter=2000, check=4):
+        """Flexible CG with this preconditioner -> (phi, iterations, relative residual, converged)."""
+        if phi is None:
+            phi = torch.zeros_like(q)
+        it = ctypes.c_uint32(0)
+        rr = ctypes.c_double(0.)
+        rc = L.lib().oecl_fcg_solve(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx), L.ptr(ps.cy),
+                                    L.ptr(ps.mask), L.ptr(q), L.ptr(phi), L.f64(tol), L.u32(maxIter), L.u32(check),
+                                    L.ptr(self._fcgwork), ctypes.byref(it), ctypes.byref(rr), L.stream())
+        if rc not in (0, 4):
+            L.check(rc, 'fcg solve')
+        self.lastIterations, self.lastRelres = it.value, rr.value
+        return phi, it.value, rr.value, rc == 0
+
+    def solve(self, ps, q, tol=1e-8, maxIter=2000):
+        """phi = A^-1 q by the learned flexible CG; the direct solve when it does not converge."""
+        phi, it, rr, ok = self.fcg(ps, q, tol=tol, maxIter=maxIter)
+        if ok:
+            return phi
+        self.fallbacks += 1
+        if ps.method != 'direct':
+            raise L.OeclError('learned PCG did not converge (relres %.3g after %d iterations)' % (rr, it))
+        return ps.solvePhi(q).clone()

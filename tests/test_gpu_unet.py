@@ -1,0 +1,120 @@
+"""The learned preconditioner: device U-Net forward and backward against the CPU oracle,
+training on the device, and the flexible CG it preconditions against Jacobi PCG, with the
+direct solve as judge and fallback."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_image(a):
+    """(H, W) float -> (H, W, 16) bf16 device tensor with a in channel 0."""
+    img = torch.zeros(a.shape + (16,), dtype=torch.float32)
+    img[..., 0] = torch.as_tensor(a, dtype=torch.float32)
+    return img.to(torch.bfloat16).cuda()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize('H,W', [(32, 32), (24, 40)])
+def test_forward_and_backward_match_oracle(H, W):
+    from oracle import unet_oracle as uo
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+
+    class _G:  # an image size without a grid: nxExt = W, nyExt = H
+        def getNxExt(self): return W
+        def getNyExt(self): return H
+    rng = np.random.default_rng(7)
+    Wt, b = uo.init_params(3)
+    b = rng.standard_normal(b.shape) * 0.1
+    Wt32, b32 = Wt.astype(np.float32), b.astype(np.float32)
+    net = UNetPreconditioner(_G(), seed=1)
+    net.set_params(Wt32, b32)
+    img = rng.standard_normal((H, W))
+    img_b = uo.bf16_round(img)
+    y = net.forward_image(_bf16_image(img_b)).cpu().numpy().astype(np.float64)
+    y_o, tape = uo.forward(img_b, Wt32.astype(np.float64), b32.astype(np.float64), bf16=True)
+    assert _rel(y, y_o) < 1e-2, _rel(y, y_o)
+    gout = uo.bf16_round(rng.standard_normal((H, W)))
+    gin = net.backward_image(_bf16_image(gout)).float().cpu().numpy()[..., 0].astype(np.float64)
+    gW, gb = net.get_grads()
+    gW_o, gb_o, gin_o = uo.backward(gout, tape, bf16=True)
+    for i in range(14):
+        assert _rel(gW[i], gW_o[i]) < 3e-2, (i, _rel(gW[i], gW_o[i]))
+        assert _rel(gb[i], gb_o[i]) < 3e-2, (i, _rel(gb[i], gb_o[i]))
+    assert _rel(gin, gin_o) < 3e-2
+
+
+def _chamber(n=64):
+    """Elliptical chamber (45 x 35 mm semi-axes) with a circular conductor of radius 2.5 mm
+    centred at x = 20 mm, as a cut-cell grid."""
+    from openecloud_amd.grid import Grid
+    a, bb, xc, rc = 0.045, 0.035, 0.02, 0.0025
+
+    def bf(x, y):
+        x = np.asarray(x, np.float64)
+        y = np.asarray(y, np.float64)
+        return ((x * x / (a * a) + y * y / (bb * bb) < 1.) & ((x - xc) ** 2 + y * y > rc * rc)).astype(np.int64)
+    return Grid(n, n, 2 * a, 2 * bb, 3, boundFunc=bf)
+
+
+def _pairs(g, ps, count, seed):
+    """Right-hand sides like the simulation's (Gaussian clouds and a beam-like spot plus
+    noise) and white-noise ones, with their exact solutions from the direct solve."""
+    rng = np.random.default_rng(seed)
+    nx, ny = g.getNxExt(), g.getNyExt()
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny))
+    mask = ps.mask.cpu().numpy().reshape(ny, nx).astype(bool)
+    out = []
+    for k in range(count):
+        if k % 2 == 0:
+            q = np.zeros((ny, nx))
+            for _ in range(rng.integers(1, 6)):
+                cx, cy = rng.uniform(0.2, 0.8) * nx, rng.uniform(0.2, 0.8) * ny
+                s = rng.uniform(1.5, 0.15 * nx)
+                q += rng.uniform(-1, 1) * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * s * s))
+            q += 0.05 * rng.standard_normal((ny, nx))
+        else:
+            q = rng.standard_normal((ny, nx))
+        q = np.where(mask, q, 0.) * 1e-9
+        qd = torch.as_tensor(q.ravel()).cuda()
+        out.append((qd, ps.solvePhi(qd).clone()))
+    return out
+
+
+def test_training_helps_flexible_cg_against_jacobi():
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = _chamber(64)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    pj = PoissonSolver(g, method='pcg', tol=1e-8, maxIter=20000)
+    train = _pairs(g, ps, 48, seed=1)
+    test = _pairs(g, ps, 4, seed=2)
+    net = UNetPreconditioner(g, seed=0)
+    l0 = np.mean([net.loss(ps, q, phi) for q, phi in test])
+    hist = net.train(ps, train, steps=600, lr=2e-3, lrEnd=2e-4, logEvery=100)
+    l1 = np.mean([net.loss(ps, q, phi) for q, phi in test])
+    assert l1 < 0.5 * l0, (l0, l1, hist)
+    for q, phi in test:
+        z, it, rr, ok = net.fcg(ps, q, tol=1e-8, maxIter=3000)
+        assert ok and rr <= 1e-8
+        # the direct solve is the judge
+        assert float(torch.linalg.norm(z - phi) / torch.linalg.norm(phi)) < 1e-6
+        pj.solvePhi(q)
+        jac = pj.lastIterations
+        assert it < 0.5 * jac, (it, jac)
+
+
+def test_learned_solve_falls_back_to_the_direct_solve():
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = _chamber(32)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    (q, phi), = _pairs(g, ps, 1, seed=5)
+    net = UNetPreconditioner(g, seed=0)
+    out = net.solve(ps, q, tol=1e-12, maxIter=1)
+    assert net.fallbacks == 1
+    assert float(torch.linalg.norm(out - phi) / torch.linalg.norm(phi)) < 1e-12
