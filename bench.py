@@ -51,7 +51,10 @@ def parse():
                     help='replay the steps between two sorts as one captured HIP graph')
     ap.add_argument('--sort-every', type=int, default=50,
                     help='re-order particles by cell every this many steps (0: never)')
-    ap.add_argument('--workload', default='dipole', choices=['dipole', 'scan'],
+    ap.add_argument('--train-steps', type=int, default=3000, help='learned: Adam steps')
+    ap.add_argument('--pairs', type=int, default=192, help='learned: training pairs (half clouds, half noise)')
+    ap.add_argument('--grid', type=int, default=1024, help='learned: grid cells per side')
+    ap.add_argument('--workload', default='dipole', choices=['dipole', 'scan', 'learned'],
                     help='dipole: the elliptical dipole buildup (default); scan: 64 buildups '
                          'stepped together (delta_max and intensity scan)')
     ap.add_argument('--runs', type=int, default=64, help='scan: number of buildups')
@@ -245,6 +248,147 @@ def compare_f32(a, n, b64, it0):
                     'FP64 run after the timed steps'}
 
 
+def main_learned(a):
+    """The learned-preconditioner workload: a cut-cell grid of the elliptical chamber (45 x 35
+    mm semi-axes) with a circular conductor of radius 2.5 mm centred at x = 20 mm; training
+    pairs (q, phi) from buildup clouds dumped every 50 steps (charge plus the beam term at that
+    time) and from white noise, phi from the direct solve; the U-Net trained on the device; then
+    Poisson solves to 1e-8 relative residual by the learned flexible CG, by Jacobi PCG and by
+    the direct solve."""
+    import numpy as np
+    import torch
+    from openecloud_amd import _lib as L
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.particleBoundary import AbsorbArbitraryCutCell
+    from openecloud_amd.particleEmitter import FurmanEmitter, homoLoader
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    L.require_gpu()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    ax, ay, xc, rc = 0.045, 0.035, 0.02, 0.0025
+
+    def bf(x, y):
+        x = np.asarray(x, np.float64)
+        y = np.asarray(y, np.float64)
+        return ((x * x / (ax * ax) + y * y / (ay * ay) < 1.) & ((x - xc) ** 2 + y * y > rc * rc)).astype(np.int64)
+    n = a.grid
+    t0 = time.perf_counter()
+    g = Grid(n, n, 2 * ax, 2 * ay, 3, boundFunc=bf)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    pj = PoissonSolver(g, method='pcg', tol=1e-8, maxIter=200000)
+    log('grid and direct solver: %.1f s, %d capacitance rows' % (time.perf_counter() - t0, ps.capacitanceRows))
+    # training pairs from a buildup on this grid: every 50 steps q = rho + beam(t)
+    randomGen.seed(0)
+    dt = 5e-12
+    p = Particles('electrons', 'variableWeight', dtype=torch.float32, ordering='fast')
+    p.setBConst([0., 0.5, 0.])
+    pb = AbsorbArbitraryCutCell(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.6, eHat0=300.)
+    beam = GaussianBunchTrain(g, dt, nParticles=1.15e11, tBunchSpacing=25e-9, radiusSigma=1.5e-3, nBunches=4)
+    npart = 200000
+    homoLoader(g, p, pb, npart, 1e9 / npart, 4.194e5)
+    p.reserve(3 * npart)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
+    ncl = a.pairs // 2
+    b.native_setup(absCap=65536, secCap=131072, maxSteps=50 * ncl + 10)
+    pairs = []
+    t0 = time.perf_counter()
+    for k in range(ncl):
+        for _ in range(50):
+            b.step_native()
+        b.sync()
+        b.deposit_all()
+        q = b._rhs(b.it * dt).clone()
+        pairs.append((q, ps.solvePhi(q).clone()))
+    rng = np.random.default_rng(1)
+    mask = ps.mask.to(torch.float64)
+    for k in range(a.pairs - ncl):
+        q = torch.as_tensor(rng.standard_normal(g.getNpExt())).cuda() * mask * 1e-9
+        pairs.append((q, ps.solvePhi(q).clone()))
+    log('%d pairs: %.1f s' % (len(pairs), time.perf_counter() - t0))
+    test = pairs[ncl - 4:ncl] + pairs[-4:]
+    train = pairs[:ncl - 4] + pairs[ncl:-4]
+    net = UNetPreconditioner(g, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.train(ps, train, steps=a.train_steps, lr=2e-3, lrEnd=1e-4, logEvery=max(1, a.train_steps // 10), log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # U-Net apply time (the preconditioner's cost per iteration)
+    r = test[0][0]
+    for _ in range(3):
+        net.precondition(ps, r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        net.precondition(ps, r)
+    torch.cuda.synchronize()
+    t_apply = (time.perf_counter() - t0) / 20
+    H, W = net.H, net.W
+    flops = 0
+    for lv in range(4):
+        convs = {0: 4, 1: 4, 2: 4, 3: 2}[lv]
+        flops += convs * 2 * 9 * 16 * 16 * (H >> lv) * (W >> lv)
+    res = {'learned': [], 'jacobi': [], 'direct': []}
+    for q, phi in test:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        z, it, rr, ok = net.fcg(ps, q, tol=1e-8, maxIter=5000)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        err = float(torch.linalg.norm(z - phi) / torch.linalg.norm(phi))
+        res['learned'].append((t1 - t0, it, rr, ok, err))
+        t0 = time.perf_counter()
+        pj.solvePhi(q)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res['jacobi'].append((t1 - t0, pj.lastIterations, None, True, None))
+        t0 = time.perf_counter()
+        for _ in range(10):
+            ps.solvePhi(q)
+        torch.cuda.synchronize()
+        res['direct'].append(((time.perf_counter() - t0) / 10, 0, None, True, None))
+    summ = {}
+    for k, v in res.items():
+        tm = [x[0] for x in v]
+        summ[k] = {'solves_per_s': len(tm) / sum(tm), 'iterations_mean': float(np.mean([x[1] for x in v]))}
+    summ['learned']['relres_max'] = max(x[2] for x in res['learned'])
+    summ['learned']['converged'] = all(x[3] for x in res['learned'])
+    summ['learned']['solution_rel_err_vs_direct_max'] = max(x[4] for x in res['learned'])
+    out = {
+        'metric': 'Poisson solves/sec to 1e-8 relative residual, learned-preconditioner flexible CG',
+        'value': summ['learned']['solves_per_s'],
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': len(test),
+        'warmup': 0,
+        'ms_per_step': 1e3 / summ['learned']['solves_per_s'],
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'bf16 network (FP32 accumulation), FP64 CG',
+        'data': 'synthetic: (q, phi) pairs from a seeded buildup on this grid and from white noise',
+        'config': {'workload': '%dx%d cut-cell elliptical chamber 45x35 mm with a conductor of radius 2.5 mm at '
+                               'x = 20 mm; U-Net 4 levels, 16 channels' % (n, n),
+                   'grid': [n, n], 'inside_nodes': int(ps.mask.sum().item()),
+                   'capacitance_rows': ps.capacitanceRows, 'pairs': len(pairs), 'train_steps': a.train_steps,
+                   'train_seconds': t_train, 'loss_history': hist},
+        'solvers': summ,
+        'unet_apply_ms': t_apply * 1e3,
+        'unet_apply_gflop': flops / 1e9,
+        'roofline': {'bound': 'mfma', 'kernel': 'U-Net apply (14 bf16 convolutions)',
+                     'achieved': flops / t_apply / 1e12, 'peak': 2500.0, 'unit': 'TFLOP/s',
+                     'frac': flops / t_apply / 1e12 / 2500.0, 'traffic': None},
+        'learned_vs_jacobi_time': summ['learned']['solves_per_s'] / summ['jacobi']['solves_per_s'],
+        'learned_vs_jacobi_iterations': summ['jacobi']['iterations_mean'] / max(1., summ['learned']['iterations_mean']),
+    }
+    print(json.dumps(out))
+
+
 def main_scan(a):
     import ctypes
     import torch
@@ -281,6 +425,33 @@ def main_scan(a):
     st = scan.sync()
     psteps = float(sum(sum(scan.history(r, it0, it0 + steps)) for r in range(R)))
     rate = psteps / wall
+    # push kernel timing: one more sort interval of eager steps with HIP events around each push
+    evs = []
+    scan.sort(best['order'], best['binShift'])
+    i1 = scan.it
+    for k in range(se):
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        L.check(L.lib().oecl_event_create(ctypes.byref(e0)))
+        L.check(L.lib().oecl_event_create(ctypes.byref(e1)))
+        evs.append((e0, e1))
+        scan._args.evPushBegin, scan._args.evPushEnd = e0.value, e1.value
+        scan.step()
+    scan._args.evPushBegin = scan._args.evPushEnd = None
+    torch.cuda.synchronize()
+    push_us = []
+    for e0, e1 in evs:
+        ms = ctypes.c_float()
+        L.check(L.lib().oecl_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+        push_us.append(ms.value * 1e3)
+        L.lib().oecl_event_destroy(e0)
+        L.lib().oecl_event_destroy(e1)
+    tot = [sum(scan.history(r, i1 + k, i1 + k + 1)[0] for r in range(R)) for k in range(se)]
+    s_b = 8 if a.dtype == 'f64' else 4
+    by = [11 * s_b * tot[k] + 24 * g.getNpExt() * R for k in range(se)]
+    avg_us = sum(push_us) / len(push_us)
+    achieved = (sum(by) / len(by)) / (avg_us * 1e-6) / 1e9
+    print('scan push time (us) by steps since the last sort: ' +
+          ' '.join('%d:%.0f' % (k, t) for k, t in enumerate(push_us)), file=sys.stderr)
     # the same runs one at a time with the single-run one-call step (3 steps each)
     seq_ps, seq_t = 0., 0.
     for r, b in enumerate(runs):
@@ -330,6 +501,9 @@ def main_scan(a):
                    'macro_particles_total_end': int(sum(st['counts'])),
                    'absorbed_total': int(sum(st['absorbed_total'])), 'emitted_total': int(sum(st['emitted_total']))},
         'poisson_solves_per_s': R * steps / wall,
+        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'avg_launch_us': avg_us,
+                     'bytes_per_launch': sum(by) / len(by)},
         'autotune': {'best': best, 'candidates_us_per_step': [[c, 1e6 * t] for c, t in results]},
         'plain_one_run_per_call': {'value': seq_rate, 'unit': 'particle-steps/s',
                                    'note': 'the same runs stepped one at a time with the single-run one-call step'},
@@ -347,6 +521,8 @@ def main():
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
               '(see the sharding design note); got --gpus %d' % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.workload == 'learned':
+        return main_learned(a)
     if a.workload == 'scan':
         if a.dtype == 'f64' and '--dtype' not in sys.argv:
             a.dtype = 'f32'   # the scan's particles are FP32 unless asked otherwise

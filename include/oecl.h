@@ -441,6 +441,8 @@ typedef struct {
     void *photoSlots;           /* [photoCap][6] */
     uint32_t photoCap;
     const oecl_secem_t *secem;  /* NULL: the Furman emitter `prm`; else this simpler emitter */
+    double *Eil;                /* [2 npExt] (per run) scratch: E interleaved (Ex, Ey) per node for
+                                   the push's gather, or NULL to gather from E */
 } oecl_step_args;
 
 uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap);

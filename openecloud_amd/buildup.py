@@ -40,7 +40,7 @@ class StepArgs(ctypes.Structure):
                 ('evPushBegin', _VP), ('evPushEnd', _VP), ('photoCount', _VP), ('photoWeight', _VP),
                 ('nPhoto', ctypes.c_uint32), ('photoE0', ctypes.c_double),
                 ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32),
-                ('secem', _VP)]
+                ('secem', _VP), ('Eil', _VP)]
 
 
 ST_ABS_OVERFLOW, ST_SEC_OVERFLOW, ST_CAP_OVERFLOW, ST_PHOTO_OVERFLOW = 1, 2, 4, 8
@@ -313,6 +313,8 @@ class Buildup:
             self._prm.particleMass = p.particleMass
         else:
             self._prm = self.em.desc
+        self._eil = torch.empty(2 * self.g.getNpExt(), dtype=torch.float64, device=dev)
+        a.Eil = self._eil.data_ptr()
         a.work = self._stepwork.data_ptr()
         self._args = a
         self._native_it0 = self.it

@@ -189,23 +189,56 @@ __device__ __forceinline__ void boris_rotate(T &vx, T &vy, T &vz, T t0, T t1, T 
     vz += vs0 * s1 - vs1 * s0;
 }
 
-// Gather E (and B for TYPEB 2) at the particle and apply the Boris push (particles.pyx:
-// 272-333 with _fieldToParticles's accumulation order).  Positions advance in x and y only.
-template <typename T, int TYPEB>
-__device__ __forceinline__ void gather_push(const Weights<T> &w, uint32_t nx, uint32_t np,
-                                            const double *__restrict__ E,
-                                            const double *__restrict__ B, T b0, T b1, T b2, T dt,
-                                            T k, T &x, T &y, T &vx, T &vy, T &vz)
+// E at the particle from the reference's layout [Ex | Ey] (particles.pyx:354-372 order).
+template <typename T>
+__device__ __forceinline__ void gather_e(const Weights<T> &w, uint32_t nx, uint32_t np, const double *__restrict__ E,
+                                         T &ex, T &ey)
 {
     const uint32_t c = w.cell;
-    T ex = w.wLL * (T)E[c];
-    T ey = w.wLL * (T)E[c + np];
+    ex = w.wLL * (T)E[c];
+    ey = w.wLL * (T)E[c + np];
     ex += w.wUL * (T)E[c + 1];
     ey += w.wUL * (T)E[c + 1 + np];
     ex += w.wLU * (T)E[c + nx];
     ey += w.wLU * (T)E[c + nx + np];
     ex += w.wUU * (T)E[c + nx + 1];
     ey += w.wUU * (T)E[c + nx + 1 + np];
+}
+
+// The same from an interleaved copy Eil[2 node + {0, 1}] = (Ex, Ey): one 16-byte load per node,
+// the same products in the same order.
+template <typename T>
+__device__ __forceinline__ void gather_e_il(const Weights<T> &w, uint32_t nx, const double *__restrict__ Eil, T &ex,
+                                            T &ey)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 *E2 = reinterpret_cast<const d2 *>(Eil);
+    const uint32_t c = w.cell;
+    const d2 a = E2[c], b = E2[c + 1], cc = E2[c + nx], d = E2[c + nx + 1];
+    ex = w.wLL * (T)a.x;
+    ey = w.wLL * (T)a.y;
+    ex += w.wUL * (T)b.x;
+    ey += w.wUL * (T)b.y;
+    ex += w.wLU * (T)cc.x;
+    ey += w.wLU * (T)cc.y;
+    ex += w.wUU * (T)d.x;
+    ey += w.wUU * (T)d.y;
+}
+
+// Gather E (and B for TYPEB 2) at the particle and apply the Boris push (particles.pyx:
+// 272-333 with _fieldToParticles's accumulation order).  Positions advance in x and y only.
+// With Eil (not null) E is read from the interleaved copy.
+template <typename T, int TYPEB>
+__device__ __forceinline__ void gather_push(const Weights<T> &w, uint32_t nx, uint32_t np,
+                                            const double *__restrict__ E,
+                                            const double *__restrict__ B, T b0, T b1, T b2, T dt,
+                                            T k, T &x, T &y, T &vx, T &vy, T &vz,
+                                            const double *__restrict__ Eil = nullptr)
+{
+    const uint32_t c = w.cell;
+    T ex, ey;
+    if (Eil) gather_e_il<T>(w, nx, Eil, ex, ey);
+    else gather_e<T>(w, nx, np, E, ex, ey);
     vx += k * ex;
     vy += k * ey;
     if (TYPEB != 0) {

@@ -123,15 +123,32 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
     else box.b[(ix - 2) + (iy - 2) * box.bx] = box.mask[i] ? qv : 0.;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8), each XCD with its
+// own L2.  The push maps hardware block b to logical block xcd_block(b) so that every XCD works
+// through whole chunks of XCD_CHUNK consecutive blocks (chunk c on XCD c % 8): a chunk's
+// particles share a few rows of cells after a cell sort, so its XCD's L2 holds that part of E
+// and rho, while the chunks of every XCD still spread over the whole block (any live prefix
+// keeps all XCDs busy).  Blocks past the last whole group of 8 chunks keep their index.  A
+// bijection on [0, n).
+constexpr uint32_t XCD_CHUNK = 32;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
+{
+    const uint32_t full = n / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
+    if (b >= full) return b;
+    const uint32_t x = b & 7u, k = b >> 3;
+    return ((k / XCD_CHUNK) * 8 + x) * XCD_CHUNK + k % XCD_CHUNK;
+}
+
 // Holes (absorbed particles) of one wave in index order: the wave's count goes to
-// counts[w], the indices to list[w * 64 + rank] (w = global wave index), and a wave with holes
-// appends w to nzList (in no particular order).  All 64 lanes of the wave call it.
-__device__ __forceinline__ void wave_holes(bool dead, uint32_t i, uint32_t *__restrict__ counts,
+// counts[w], the indices to list[w * 64 + rank] (w = global wave index of logical block blk),
+// and a wave with holes appends w to nzList (in no particular order).  All 64 lanes of the
+// wave call it.
+__device__ __forceinline__ void wave_holes(bool dead, uint32_t i, uint32_t blk, uint32_t *__restrict__ counts,
                                            uint32_t *__restrict__ list, uint32_t *__restrict__ misc,
                                            uint32_t *__restrict__ nzList)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t w = blockIdx.x * (PB / 64) + (threadIdx.x >> 6);
+    const uint32_t w = blk * (PB / 64) + (threadIdx.x >> 6);
     const uint64_t bal = __ballot(dead);
     if (dead) list[(size_t)w * 64 + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
     if (lane == 0) {
@@ -146,20 +163,22 @@ __device__ __forceinline__ void wave_holes(bool dead, uint32_t i, uint32_t *__re
 template <typename T, int TYPEB>
 __global__ void __launch_bounds__(PB)
     k_push_dev(Runs runs, oecl_grid_t g, WallDev wall, Soa<T> p, const uint32_t *__restrict__ nDev,
-               const double *__restrict__ E, const double *__restrict__ B, double b0, double b1,
+               const double *__restrict__ E, const double *__restrict__ Eil, const double *__restrict__ B, double b0, double b1,
                double b2, double dt, double k, double q, uint8_t *__restrict__ isInside,
                double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
                uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
 {
     __shared__ double tiles[PB / 64][512];
-    const uint32_t run = runs.block_run(blockIdx.x);
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t run = runs.block_run(blk);
     const uint64_t sBeg = runs.seg0(run);
     const uint32_t n = nDev[run];
-    const uint64_t first = (uint64_t)blockIdx.x * PB;
+    const uint64_t first = (uint64_t)blk * PB;
     if (first - sBeg >= n) return;  // whole block past the run's live particles
     const uint32_t i = (uint32_t)first + threadIdx.x;
     const bool live = i - sBeg < n;
     E += (size_t)run * 2 * g.np;
+    if (Eil) Eil += (size_t)run * 2 * g.np;
     rho += (size_t)run * g.np;
     misc += run * M_COUNT;
     nzList += sBeg / 64;
@@ -173,7 +192,7 @@ __global__ void __launch_bounds__(PB)
     if (live) {
         C x = p(0, i), y = p(1, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
         Weights<C> w = grid_weights<C>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
-        gather_push<C, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vx, vy, vz);
+        gather_push<C, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vx, vy, vz, Eil);
         uint8_t in = wall_inside<C>(wall, x, y);
         p(0, i) = (T)x;
         p(1, i) = (T)y;
@@ -189,7 +208,7 @@ __global__ void __launch_bounds__(PB)
             dead = true;
         }
     }
-    wave_holes(dead, i, counts, list, misc, nzList);
+    wave_holes(dead, i, blk, counts, list, misc, nzList);
     wave_deposit<512>(rho, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
                       w2.wUU * wq * q, tiles[threadIdx.x >> 6]);
 }
@@ -290,6 +309,19 @@ __device__ __forceinline__ uint32_t n_hits(const uint32_t *misc, uint32_t absCap
     return h < absCap ? h : absCap;
 }
 
+// The run of flattened hit t, hitOff[0 .. R] being the exclusive prefix of the runs' hit counts
+// (the largest r with hitOff[r] <= t; runs without hits are passed over).
+__device__ __forceinline__ uint32_t hit_run(const uint32_t *__restrict__ hitOff, uint32_t R, uint32_t t)
+{
+    uint32_t lo = 0, hi = R;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (hitOff[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Interaction point, remaining time and wall normal of an absorbed particle at (x, y) with
 // velocity (vx, vy) (particleBoundary.pyx: AbsorbRectangular/Elliptical/ArbitraryCutCell
 // calculateInteractionPoint and calculateNormalVectors).  found = false when the cut-cell
@@ -388,50 +420,60 @@ __device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const
     return ns;
 }
 
-// Wall hits of run blockIdx.y, grid-stride over its hits.
+// Wall hits of every run, grid-stride over the flattened hits (hitOff: the runs' hit offsets,
+// or null for a single run).
 template <typename T>
 __global__ void __launch_bounds__(128)
-    k_hits_dev(Runs runs, WallDev w, oecl_furman_t P, oecl_secem_t S, int useSecem, T *ap, T *nv, T *rts,
-               double dt, int emit, uint32_t s0, uint32_t s1, const uint32_t *__restrict__ stepDev,
-               uint32_t *__restrict__ nSec, double *__restrict__ impact, const uint32_t *misc, uint32_t *stats)
+    k_hits_dev(Runs runs, const uint32_t *__restrict__ hitOff, WallDev w, oecl_furman_t P, oecl_secem_t S,
+               int useSecem, T *ap, T *nv, T *rts, double dt, int emit, uint32_t s0, uint32_t s1,
+               const uint32_t *__restrict__ stepDev, uint32_t *__restrict__ nSec, double *__restrict__ impact,
+               const uint32_t *misc, uint32_t *stats)
 {
-    const uint32_t run = blockIdx.y, a0 = runs.abs0(run);
-    const uint32_t na = n_hits(misc + run * M_COUNT, runs.absN(run)), step = *stepDev;
-    const oecl_furman_t &Pr = runs.furman ? runs.furman[run] : P;
-    const uint32_t k0 = runs.key0(run, s0), k1 = runs.key1(run, s1);
-    ap += 6 * (uint64_t)a0;
-    nv += 2 * (uint64_t)a0;
-    rts += a0;
-    nSec += a0;
-    impact += 4 * (uint64_t)a0;
-    stats += 8 * run;
-    for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x)
-        hit_one<T>(jj, w, Pr, S, useSecem, ap, nv, rts, dt, emit, k0, k1, step, nSec, impact, stats);
+    const uint32_t total = hitOff ? hitOff[runs.R] : n_hits(misc, runs.absN(0)), step = *stepDev;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const uint32_t run = hitOff ? hit_run(hitOff, runs.R, t) : 0;
+        const uint32_t jj = hitOff ? t - hitOff[run] : t, a0 = runs.abs0(run);
+        const oecl_furman_t &Pr = runs.furman ? runs.furman[run] : P;
+        hit_one<T>(jj, w, Pr, S, useSecem, ap + 6 * (uint64_t)a0, nv + 2 * (uint64_t)a0, rts + a0, dt, emit,
+                   runs.key0(run, s0), runs.key1(run, s1), step, nSec + a0, impact + 4 * (uint64_t)a0,
+                   stats + 8 * run);
+    }
+}
+
+// hitOff[0 .. R]: exclusive prefix of the runs' hit counts (one block).
+__global__ void __launch_bounds__(CT) k_hit_offsets(Runs runs, const uint32_t *__restrict__ misc, uint32_t *hitOff)
+{
+    __shared__ uint32_t wsum[CT / 64];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < runs.R; b0 += CT) {
+        const uint32_t r = b0 + threadIdx.x;
+        const uint32_t c = r < runs.R ? n_hits(misc + r * M_COUNT, runs.absN(r)) : 0;
+        uint32_t tot;
+        const uint32_t o = block_scan_ct(c, wsum, &tot);
+        if (r < runs.R) hitOff[r] = carry + o;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) hitOff[runs.R] = carry;
 }
 
 // Second loop of the simpler emitters (secem::pass2) into the hit's MAXSEC slots, one thread
 // per hit, grid-stride.
 template <typename T>
 __global__ void __launch_bounds__(128)
-    k_secem_emit_dev(Runs runs, oecl_secem_t S, const T *__restrict__ ap, const T *__restrict__ nv,
-                     const uint32_t *__restrict__ nSec, const double *__restrict__ impact, uint32_t s0, uint32_t s1,
-                     const uint32_t *__restrict__ stepDev, T *__restrict__ slots, const uint32_t *misc)
+    k_secem_emit_dev(Runs runs, const uint32_t *__restrict__ hitOff, oecl_secem_t S, const T *__restrict__ ap,
+                     const T *__restrict__ nv, const uint32_t *__restrict__ nSec, const double *__restrict__ impact,
+                     uint32_t s0, uint32_t s1, const uint32_t *__restrict__ stepDev, T *__restrict__ slots,
+                     const uint32_t *misc)
 {
-    const uint32_t run = blockIdx.y, a0 = runs.abs0(run);
-    const uint32_t na = n_hits(misc + run * M_COUNT, runs.absN(run)), step = *stepDev;
-    s0 = runs.key0(run, s0);
-    s1 = runs.key1(run, s1);
-    ap += 6 * (uint64_t)a0;
-    nv += 2 * (uint64_t)a0;
-    nSec += a0;
-    impact += 4 * (uint64_t)a0;
-    slots += 6 * (uint64_t)MAXSEC * a0;
-    for (uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x; jj < na; jj += gridDim.x * blockDim.x) {
-        const uint32_t ns = nSec[jj];
+    const uint32_t total = hitOff ? hitOff[runs.R] : n_hits(misc, runs.absN(0)), step = *stepDev;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const uint32_t run = hitOff ? hit_run(hitOff, runs.R, t) : 0;
+        const uint32_t jj = hitOff ? t - hitOff[run] : t, r = runs.abs0(run) + jj;
+        const uint32_t ns = nSec[r];
         if (ns == 0) continue;
-        const secem::Rng rng{nullptr, jj, step, s0, s1};
-        T *base = slots + 6 * (uint64_t)MAXSEC * jj;
-        secem::pass2<T>(S, ap + 6 * (uint64_t)jj, nv + 2 * (uint64_t)jj, impact + 4 * (uint64_t)jj, rng,
+        const secem::Rng rng{nullptr, jj, step, runs.key0(run, s0), runs.key1(run, s1)};
+        T *base = slots + 6 * (uint64_t)MAXSEC * r;
+        secem::pass2<T>(S, ap + 6 * (uint64_t)r, nv + 2 * (uint64_t)r, impact + 4 * (uint64_t)r, rng,
                         [&](uint32_t k) -> T * { return k < ns && k < MAXSEC ? base + 6 * (uint64_t)k : nullptr; });
     }
 }
@@ -442,33 +484,29 @@ __global__ void __launch_bounds__(128)
 // the LAST hit's first-loop values, as the reference does.
 template <typename T>
 __global__ void __launch_bounds__(64)
-    k_emit_dev(Runs runs, oecl_furman_t P, const T *__restrict__ ap, const T *__restrict__ nv,
-               const uint32_t *__restrict__ nSec, const double *__restrict__ impact, uint32_t s0, uint32_t s1,
-               const uint32_t *__restrict__ stepDev, T *__restrict__ slots, const uint32_t *misc)
+    k_emit_dev(Runs runs, const uint32_t *__restrict__ hitOff, oecl_furman_t P, const T *__restrict__ ap,
+               const T *__restrict__ nv, const uint32_t *__restrict__ nSec, const double *__restrict__ impact,
+               uint32_t s0, uint32_t s1, const uint32_t *__restrict__ stepDev, T *__restrict__ slots,
+               const uint32_t *misc)
 {
     constexpr uint32_t GPW = 64 / MAXSEC;  // impact groups per wave
-    const uint32_t run = blockIdx.y, a0 = runs.abs0(run);
-    const uint32_t na = n_hits(misc + run * M_COUNT, runs.absN(run)), step = *stepDev;
-    const oecl_furman_t &Pr = runs.furman ? runs.furman[run] : P;
-    s0 = runs.key0(run, s0);
-    s1 = runs.key1(run, s1);
-    ap += 6 * (uint64_t)a0;
-    nv += 2 * (uint64_t)a0;
-    nSec += a0;
-    impact += 4 * (uint64_t)a0;
-    slots += 6 * (uint64_t)MAXSEC * a0;
+    const uint32_t total = hitOff ? hitOff[runs.R] : n_hits(misc, runs.absN(0)), step = *stepDev;
     const uint32_t lane = threadIdx.x & 63, grp = lane / MAXSEC, ii = lane % MAXSEC;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nWaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t j0 = wave * GPW; j0 < na; j0 += nWaves * GPW) {  // wave-uniform bound
-        const uint32_t jj = j0 + grp;
-        const bool ok = grp < GPW && jj < na;
-        const uint32_t ns = ok ? nSec[jj] : 0;
-        const uint32_t jr = ok ? jj : 0;
-        furman::Rng rng{nullptr, jr, step, s0, s1};
-        T *o = ok && ii < ns ? slots + 6 * ((uint64_t)MAXSEC * jj + ii) : nullptr;
-        furman::pass2_lane<T>(Pr, ap + 6 * (uint64_t)jr, nv + 2 * (uint64_t)jr, ns, ok ? impact[4 * (uint64_t)jj] : 0.,
-                              impact + 4 * (uint64_t)(na - 1), rng, ii, (int)(grp * MAXSEC), o);
+    for (uint32_t j0 = wave * GPW; j0 < total; j0 += nWaves * GPW) {  // wave-uniform bound
+        const uint32_t t = j0 + grp;
+        const bool ok = grp < GPW && t < total;
+        const uint32_t run = ok && hitOff ? hit_run(hitOff, runs.R, t) : 0;
+        const uint32_t jj = ok ? (hitOff ? t - hitOff[run] : t) : 0;
+        const uint32_t a0 = runs.abs0(run), na = n_hits(misc + run * M_COUNT, runs.absN(run));
+        const uint32_t r = a0 + jj;
+        const uint32_t ns = ok ? nSec[r] : 0;
+        const oecl_furman_t &Pr = runs.furman ? runs.furman[run] : P;
+        furman::Rng rng{nullptr, jj, step, runs.key0(run, s0), runs.key1(run, s1)};
+        T *o = ok && ii < ns ? slots + 6 * ((uint64_t)MAXSEC * r + ii) : nullptr;
+        furman::pass2_lane<T>(Pr, ap + 6 * (uint64_t)r, nv + 2 * (uint64_t)r, ns, ok ? impact[4 * (uint64_t)r] : 0.,
+                              impact + 4 * (uint64_t)(a0 + (na ? na - 1 : 0)), rng, ii, (int)(grp * MAXSEC), o);
     }
 }
 
@@ -603,7 +641,7 @@ unsigned grid_cap(uint64_t items, unsigned block, unsigned maxBlocks)
 static uint64_t slots_offset(uint64_t capacity, uint32_t absTotal, uint32_t R)
 {
     const uint64_t nW = (capacity + PB - 1) / PB * (PB / 64);
-    return ((4 * (nW * (64 + 2) + absTotal + (uint64_t)M_COUNT * R)) + 255) / 256 * 256;
+    return ((4 * (nW * (64 + 2) + absTotal + (uint64_t)M_COUNT * R + R + 1)) + 255) / 256 * 256;
 }
 
 OECL_EXPORT uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap)
@@ -619,10 +657,10 @@ OECL_EXPORT uint64_t oecl_step_batch_work_bytes(uint64_t capacity, uint32_t absT
 namespace {
 
 // E_r = phiToE phi_r for every run r = blockIdx.y (rows in the CSR's own column order, the
-// arithmetic of oecl_csr_spmv).
+// arithmetic of oecl_csr_spmv); with yil, also the interleaved copy yil[2 (r % xStride) + r / xStride].
 __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, const int32_t *__restrict__ col,
                             const double *__restrict__ val, const double *__restrict__ x, uint32_t xStride,
-                            double *__restrict__ y)
+                            double *__restrict__ y, double *__restrict__ yil)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nRows) return;
@@ -631,6 +669,7 @@ __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, 
     double sum = 0.;
     for (int32_t k = rowPtr[r]; k < rowPtr[r + 1]; ++k) sum += val[k] * x[col[k]];
     y[r] = sum;
+    if (yil) yil[(size_t)blockIdx.y * nRows + 2 * (r % xStride) + r / xStride] = sum;
 }
 
 }  // namespace
@@ -649,6 +688,7 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     uint32_t *nzList = list + nW * 64;
     uint32_t *holePos = nzList + nW;
     uint32_t *misc = holePos + absTotal;
+    uint32_t *hitOff = R > 1 ? misc + (size_t)M_COUNT * R : nullptr;
     T *slots = (T *)((char *)a->work + slots_offset(cap, absTotal, R));
     int rc = fastpoisson_reserve(fp, R);
     if (rc) return rc;
@@ -660,36 +700,37 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     rc = fastpoisson_solve_box(fp, a->phi, R, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
-                       a->e2nCol, a->e2nVal, a->phi, np, a->E);
+                       a->e2nCol, a->e2nVal, a->phi, np, a->E, a->Eil);
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
     const dim3 gp((unsigned)nBlk), bp(PB);
     if (a->evPushBegin) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushBegin, s), "event");
     if (a->typeB == 0)
-        hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->bField, 0., 0., 0.,
+        hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
                            a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     else if (a->typeB == 1)
-        hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->bField, a->b0,
+        hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, a->b0,
                            a->b1, a->b2, a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     else
-        hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->bField, 0., 0., 0.,
+        hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
                            a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     if (a->evPushEnd) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushEnd, s), "event");
     hipLaunchKernelGGL(k_holes_collect<T>, dim3(R), dim3(CT), 0, s, runs, p, a->nDev, counts, list, nzList, holePos,
                        (T *)a->absorbed, misc);
     const oecl_secem_t S = a->secem ? *a->secem : oecl_secem_t{};
     const int useSecem = a->secem ? 1 : 0;
-    const unsigned hitBlocks = (grid_cap(absTotal, 128, 512) + R - 1) / R;
-    hipLaunchKernelGGL(k_hits_dev<T>, dim3(hitBlocks, R), dim3(128), 0, s, runs, wd, *P, S, useSecem,
+    if (hitOff) hipLaunchKernelGGL(k_hit_offsets, dim3(1), dim3(CT), 0, s, runs, misc, hitOff);
+    const unsigned hitBlocks = grid_cap(absTotal, 128, R > 1 ? 2048 : 512);
+    hipLaunchKernelGGL(k_hits_dev<T>, dim3(hitBlocks), dim3(128), 0, s, runs, hitOff, wd, *P, S, useSecem,
                        (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, a->emit, a->seed0, a->seed1,
                        a->stepDev, a->nSec, a->impact, misc, a->stats);
     if (a->emit && useSecem)
-        hipLaunchKernelGGL(k_secem_emit_dev<T>, dim3(hitBlocks, R), dim3(128), 0, s, runs, S,
+        hipLaunchKernelGGL(k_secem_emit_dev<T>, dim3(hitBlocks), dim3(128), 0, s, runs, hitOff, S,
                            (const T *)a->absorbed, (const T *)a->normals, a->nSec, a->impact, a->seed0, a->seed1,
                            a->stepDev, slots, misc);
     else if (a->emit)
-        hipLaunchKernelGGL(k_emit_dev<T>, dim3((grid_cap(absTotal, 6, 1024) + R - 1) / R, R), dim3(64), 0, s, runs,
-                           *P, (const T *)a->absorbed, (const T *)a->normals, a->nSec, a->impact, a->seed0, a->seed1,
-                           a->stepDev, slots, misc);
+        hipLaunchKernelGGL(k_emit_dev<T>, dim3(grid_cap(absTotal, 6, R > 1 ? 4096 : 1024)), dim3(64), 0, s, runs,
+                           hitOff, *P, (const T *)a->absorbed, (const T *)a->normals, a->nSec, a->impact, a->seed0,
+                           a->seed1, a->stepDev, slots, misc);
     hipLaunchKernelGGL(k_place_dev<T>, dim3(R), dim3(CT), 0, s, runs, *g, p, (const T *)slots,
                        (const T *)a->photoSlots, (T *)a->secondaries, a->secCap, a->nSec, a->secOff, a->nDev, holePos,
                        a->qe, a->rho, misc, a->stepDev, a->stats);

@@ -124,6 +124,7 @@ class BuildupScan:
         self.q = torch.zeros(R * npx, dtype=torch.float64, device=dev)
         self.phi = torch.zeros(R * npx, dtype=torch.float64, device=dev)
         self.E = torch.zeros(R * 2 * npx, dtype=torch.float64, device=dev)
+        self._eil = torch.zeros(R * 2 * npx, dtype=torch.float64, device=dev)
         self.isInside = torch.zeros(cap, dtype=torch.uint8, device=dev)
         self.absorbed = torch.zeros((absTot, 6), dtype=self.dtype, device=dev)
         self.normals = torch.zeros((absTot, 2), dtype=self.dtype, device=dev)
@@ -195,6 +196,7 @@ class BuildupScan:
         a.countHist = self._hist.data_ptr()
         a.histCap = self._nTab
         a.work = self._work.data_ptr()
+        a.Eil = self._eil.data_ptr()
         return a
 
     # ------------------------------------------------------------------ state
