@@ -54,7 +54,7 @@ def parse():
     ap.add_argument('--train-steps', type=int, default=3000, help='learned: Adam steps')
     ap.add_argument('--pairs', type=int, default=192, help='learned: training pairs (half clouds, half noise)')
     ap.add_argument('--grid', type=int, default=1024, help='learned: grid cells per side')
-    ap.add_argument('--workload', default='dipole', choices=['dipole', 'scan', 'learned'],
+    ap.add_argument('--workload', default='dipole', choices=['dipole', 'scan', 'learned', 'full'],
                     help='dipole: the elliptical dipole buildup (default); scan: 64 buildups '
                          'stepped together (delta_max and intensity scan)')
     ap.add_argument('--runs', type=int, default=64, help='scan: number of buildups')
@@ -311,11 +311,14 @@ def main_learned(a):
         pairs.append((q, ps.solvePhi(q).clone()))
     log('%d pairs: %.1f s' % (len(pairs), time.perf_counter() - t0))
     test = pairs[ncl - 4:ncl] + pairs[-4:]
-    train = pairs[:ncl - 4] + pairs[ncl:-4]
     net = UNetPreconditioner(g, seed=0)
+    train = net.smoothed_pairs(ps, pairs[:ncl - 4] + pairs[ncl:-4])
+    # the same flexible CG with the untrained network (the ablation)
+    base = [net.fcg(ps, q, tol=1e-8, maxIter=5000)[1] for q, _ in test[:2]]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    hist = net.train(ps, train, steps=a.train_steps, lr=2e-3, lrEnd=1e-4, logEvery=max(1, a.train_steps // 10), log=log)
+    hist = net.train(ps, train, steps=a.train_steps, lr=5e-4, lrEnd=5e-5, batch=4,
+                     logEvery=max(1, a.train_steps // 10), log=log)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     # U-Net apply time (the preconditioner's cost per iteration)
@@ -359,6 +362,8 @@ def main_learned(a):
     summ['learned']['relres_max'] = max(x[2] for x in res['learned'])
     summ['learned']['converged'] = all(x[3] for x in res['learned'])
     summ['learned']['solution_rel_err_vs_direct_max'] = max(x[4] for x in res['learned'])
+    summ['learned']['iterations_untrained_network'] = float(np.mean(base))
+    summ['learned']['iterations_trained_same_pairs'] = float(np.mean([x[1] for x in res['learned'][:2]]))
     out = {
         'metric': 'Poisson solves/sec to 1e-8 relative residual, learned-preconditioner flexible CG',
         'value': summ['learned']['solves_per_s'],
@@ -370,7 +375,7 @@ def main_learned(a):
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'bf16 network (FP32 accumulation), FP64 CG',
+        'dtype': 'bf16 network (FP32 accumulation), FP64 CG, damped Jacobi (omega 2/3) around the network',
         'data': 'synthetic: (q, phi) pairs from a seeded buildup on this grid and from white noise',
         'config': {'workload': '%dx%d cut-cell elliptical chamber 45x35 mm with a conductor of radius 2.5 mm at '
                                'x = 20 mm; U-Net 4 levels, 16 channels' % (n, n),
@@ -385,6 +390,119 @@ def main_learned(a):
                      'frac': flops / t_apply / 1e12 / 2500.0, 'traffic': None},
         'learned_vs_jacobi_time': summ['learned']['solves_per_s'] / summ['jacobi']['solves_per_s'],
         'learned_vs_jacobi_iterations': summ['jacobi']['iterations_mean'] / max(1., summ['learned']['iterations_mean']),
+    }
+    print(json.dumps(out))
+
+
+def main_full(a):
+    """The combined workload: a 2048 x 2048 grid of the elliptical 90 x 70 mm chamber, a 50 T/m
+    quadrupole (the grid field path, typeBField 2), a.particles (default 2e7) FP32
+    macro-electrons, 20 bunches of 1.15e11 with sigma 1 mm, dt 2e-12 s, Furman copper with
+    delta_max 1.5, the one-call step with the direct Poisson solve.  After the timed steps one
+    merge event of localRandom reduces the count by the 1.5x trigger's factor (target = count
+    / 1.5), timed, with its charge change."""
+    import ctypes
+    import torch
+    from openecloud_amd import _lib as L
+    from openecloud_amd import particleManagement, randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.diagnostics import weight_sum
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.magneticField import multipoleExpansion
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter, homoLoader
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    L.require_gpu()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    n = int(a.particles) if '--particles' in sys.argv else 20000000
+    dt = 2e-12
+    t0 = time.perf_counter()
+    randomGen.seed(0)
+    g = Grid(2048, 2048, 0.09, 0.07, 1)
+    bq = multipoleExpansion(g, [0., 50.])
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    log('grid, field and direct solver: %.1f s (%d capacitance rows)' % (time.perf_counter() - t0,
+                                                                        ps.capacitanceRows))
+    p = Particles('electrons', 'variableWeight', dtype=torch.float32, ordering='fast')
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.5)
+    beam = GaussianBunchTrain(g, dt, nParticles=1.15e11, tBunchSpacing=25e-9, radiusSigma=1e-3, nBunches=20)
+    homoLoader(g, p, pb, n, 1e9 / n, 4.194e5)
+    p.reserve(int(1.6 * n))
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=2, bAtGridPoints=bq)
+    b.native_setup(absCap=max(65536, n // 32), secCap=max(131072, n // 16), maxSteps=a.warmup + a.steps + 20)
+    b.sort_particles()
+    se = a.sort_every
+    for k in range(a.warmup):
+        b.step_native()
+        if se and k % se == se - 1:
+            b.sync()
+            b.sort_particles()
+    b.sync()
+    evs = []
+    for _ in range(a.steps):
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        L.check(L.lib().oecl_event_create(ctypes.byref(e0)))
+        L.check(L.lib().oecl_event_create(ctypes.byref(e1)))
+        evs.append((e0, e1))
+    it0 = b.it
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        b._args.evPushBegin, b._args.evPushEnd = evs[k][0].value, evs[k][1].value
+        b.step_native()
+        if se and k % se == se - 1:
+            b.sync()
+            b.sort_particles()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    b._args.evPushBegin = b._args.evPushEnd = None
+    st = b.sync()
+    counts = b.counts(it0, it0 + a.steps)
+    push = []
+    for e0, e1 in evs:
+        ms = ctypes.c_float()
+        L.check(L.lib().oecl_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+        push.append(ms.value * 1e3)
+        L.lib().oecl_event_destroy(e0)
+        L.lib().oecl_event_destroy(e1)
+    avg_us = sum(push) / len(push)
+    bytes_per = sum(11 * 4 * c + 48 * g.getNpExt() for c in counts) / len(counts)
+    achieved = bytes_per / (avg_us * 1e-6) / 1e9
+    # one merge event of the 1.5x trigger
+    nBefore = p.macroParticleCount
+    q0 = weight_sum(p)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    mg = particleManagement.localRandom(g, p, pb, int(nBefore / 1.5))
+    torch.cuda.synchronize()
+    t_merge = time.perf_counter() - t1
+    q1 = weight_sum(p)
+    out = {
+        'metric': 'buildup particle-steps/sec (2048x2048 quadrupole, one-call step)',
+        'value': float(sum(counts)) / wall,
+        'unit': 'particle-steps/s',
+        'n_gpus': 1,
+        'steps': a.steps,
+        'warmup': a.warmup,
+        'ms_per_step': wall / a.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (seeded uniform load, Philox draws)',
+        'config': {'workload': '2048x2048 elliptical 90x70 mm chamber, quadrupole 50 T/m (grid field), '
+                               '%d FP32 macro-electrons, Gaussian bunches sigma 1 mm 1.15e11 ppb x20, dt 2e-12 s, '
+                               'Furman copper delta_max 1.5, direct Poisson solve' % n,
+                   'grid': [2048, 2048], 'particles': n, 'capacitance_rows': ps.capacitanceRows,
+                   'absorbed_total': st['absorbed_total'], 'emitted_total': st['emitted_total']},
+        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'avg_launch_us': avg_us,
+                     'bytes_per_launch': bytes_per},
+        'merge_event': {'count_before': nBefore, 'count_after': p.macroParticleCount, 'seconds': t_merge,
+                        'charge_rel_change': abs(q1 - q0) / abs(q0), 'ops': mg},
     }
     print(json.dumps(out))
 
@@ -523,6 +641,8 @@ def main():
         sys.exit(2)
     if a.workload == 'learned':
         return main_learned(a)
+    if a.workload == 'full':
+        return main_full(a)
     if a.workload == 'scan':
         if a.dtype == 'f64' and '--dtype' not in sys.argv:
             a.dtype = 'f32'   # the scan's particles are FP32 unless asked otherwise

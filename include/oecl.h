@@ -372,9 +372,18 @@ int oecl_unet_forward(oecl_unet *u, const uint16_t *img, float *y, void *stream)
 int oecl_unet_backward(oecl_unet *u, const uint16_t *gout, uint16_t *gin, int zero, void *stream);
 /* One Adam step (Kingma & Ba) on the accumulated gradients. */
 int oecl_unet_adam(oecl_unet *u, float lr, float beta1, float beta2, float eps, void *stream);
-/* z = rms(r) * network(r / rms(r)) on the inside nodes (mask) of the nx x ny grid (FP64). */
-int oecl_unet_precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r, double *z,
-                           void *stream);
+/* The preconditioner on the inside nodes (mask) of the nx x ny grid (FP64).  omega = 0: the
+ * network alone, z = rms(r) * network(r / rms(r)).  omega > 0: the network between damped
+ * Jacobi smoothings, z1 = omega D^-1 r, n = network applied to r1 = r - A z1,
+ * z = z1 + c n + omega D^-1 (r1 - c A n) with the A-norm line search c = n.r1 / n.A n (c = 0 when
+ * that is not positive); cx, cy: the stencil arrays of oecl_pcg_solve, D its diagonal. */
+int oecl_unet_precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                           const uint8_t *mask, const double *r, double *z, double omega, void *stream);
+/* The pair the network meets inside the smoothed preconditioner: for (q, phi = A^-1 q),
+ * r1 = q - A (omega D^-1 q) and e1 = phi - omega D^-1 q (so A e1 = r1). */
+int oecl_unet_smooth_pair(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                          const uint8_t *mask, const double *q, const double *phi, double omega, double *r1,
+                          double *e1, void *stream);
 /* One training sample (q, phi = A^-1 q): forward on q, the loss L = 1 - (z^T q)^2 /
  * ((z^T A z)(phi^T q)) -- the A-norm error of the best multiple of the preconditioner's output
  * z, relative to phi's A-norm (the flexible CG does not depend on z's scale) -- and backward.
@@ -383,13 +392,14 @@ int oecl_unet_precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t
 int oecl_unet_train_sample(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
                            const uint8_t *mask, const double *q, const double *phi, double *work, int zero,
                            double *loss, void *stream);
-/* Flexible CG (Notay's truncated form) with the network as preconditioner, FP64 vectors and
- * reductions: phi (initial guess in, solution out) until ||q - A phi|| <= tol ||q||, the norm
- * read every `check` iterations.  Returns OECL_ERR_NOCONV when maxIter is reached. */
+/* Flexible CG (Notay's truncated form) preconditioned as oecl_unet_precondition with omega,
+ * FP64 vectors and reductions: phi (initial guess in, solution out) until
+ * ||q - A phi|| <= tol ||q||, the norm read every `check` iterations.  Returns OECL_ERR_NOCONV
+ * when maxIter is reached. */
 uint64_t oecl_fcg_work_bytes(uint32_t np);
 int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy, const uint8_t *mask,
-                   const double *q, double *phi, double tol, uint32_t maxIter, uint32_t check, void *work,
-                   uint32_t *iters, double *relres, void *stream);
+                   const double *q, double *phi, double tol, uint32_t maxIter, uint32_t check, double omega,
+                   void *work, uint32_t *iters, double *relres, void *stream);
 
 /* ---- the whole plain step (example01_basic.py:52-97, example02/04 variants) ---------- */
 

@@ -38,6 +38,7 @@ struct oecl_unet {
     float *out;               // network output, H * W
     double *scal;             // device scalars (scale, reductions)
     double *part;             // per-block partial sums
+    double *sv[4];            // grid vectors (H W doubles each) of the smoothed preconditioner
     char *mem;
     uint32_t adamT;
 };
@@ -63,44 +64,67 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 __device__ __forceinline__ v4s load4(const uint16_t *p) { return *reinterpret_cast<const v4s *>(p); }
 
 // out = conv3x3(in) + bias over an H x W image.  MODE 0: bf16 out with ReLU, 1: bf16 out
-// without ReLU, 2: FP32 channel 0 only.  Wave w: row w / segs, pixels 16 (w % segs) .. +15.
+// without ReLU, 2: FP32 channel 0 only.  A wave computes RW rows x 16 pixels: it walks the RW + 2
+// input rows once, loads each row's three horizontally shifted operands, and feeds every output
+// row that uses that input row (the taps of each output keep the order 0..8).
 // MFMA operand layouts (16x16x16): lane l holds A[l % 16][4 (l / 16) + j], B[4 (l / 16) + j][l % 16]
 // and D[4 (l / 16) + j][l % 16], j = 0..3; here A = weights (out x in), B = input (in x pixel).
+constexpr int CONV_RW = 4;
 template <int MODE>
 __global__ void __launch_bounds__(kThreads)
     k_conv(int H, int W, const uint16_t *__restrict__ in, const uint16_t *__restrict__ wt,
            const float *__restrict__ bias, void *__restrict__ out)
 {
+    constexpr int RW = CONV_RW;
     const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4;
-    const int segs = (W + 15) >> 4;
+    const int segs = (W + 15) >> 4, groups = (H + RW - 1) / RW;
     const int wave = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-    if (wave >= H * segs) return;  // whole wave
-    const int y = wave / segs, x = ((wave % segs) << 4) + px;
+    if (wave >= groups * segs) return;  // whole wave
+    const int y0 = (wave / segs) * RW, x = ((wave % segs) << 4) + px;
     v4s a[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) a[k] = load4(wt + (k * C + px) * C + 4 * cq);
-    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    v4f acc[RW];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const int yy = y + k / 3 - 1, xx = x + k % 3 - 1;
-        v4s b = {0, 0, 0, 0};
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W) b = load4(in + ((size_t)yy * W + xx) * C + 4 * cq);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[k], b, acc, 0, 0, 0);
+    for (int r = 0; r < RW; ++r) acc[r] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int iy = 0; iy < RW + 2; ++iy) {
+        const int yy = y0 - 1 + iy;
+        v4s b[3];
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            const int xx = x + dx - 1;
+            b[dx] = v4s{0, 0, 0, 0};
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) b[dx] = load4(in + ((size_t)yy * W + xx) * C + 4 * cq);
+        }
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int ky = iy - r;  // tap row of output row y0 + r that reads input row yy
+            if (ky >= 0 && ky <= 2)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx)
+                    acc[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ky * 3 + dx], b[dx], acc[r], 0, 0, 0);
+        }
     }
     if (x >= W) return;
-    const size_t p = (size_t)y * W + x;
-    if (MODE == 2) {
-        if (cq == 0) static_cast<float *>(out)[p] = acc[0] + bias[0];
-        return;
-    }
-    v4s o;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        float v = acc[j] + bias[4 * cq + j];
-        if (MODE == 0) v = v > 0.f ? v : 0.f;
-        o[j] = (short)f2bf(v);
+    for (int r = 0; r < RW; ++r) {
+        const int y = y0 + r;
+        if (y >= H) break;
+        const size_t p = (size_t)y * W + x;
+        if (MODE == 2) {
+            if (cq == 0) static_cast<float *>(out)[p] = acc[r][0] + bias[0];
+            continue;
+        }
+        v4s o;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float v = acc[r][j] + bias[4 * cq + j];
+            if (MODE == 0) v = v > 0.f ? v : 0.f;
+            o[j] = (short)f2bf(v);
+        }
+        *reinterpret_cast<v4s *>(static_cast<uint16_t *>(out) + p * C + 4 * cq) = o;
     }
-    *reinterpret_cast<v4s *>(static_cast<uint16_t *>(out) + p * C + 4 * cq) = o;
 }
 
 // Weight and bias gradients of one convolution: gW[k][o][i] += sum_p g[p][o] x[p + off_k][i],
@@ -409,6 +433,72 @@ __global__ void k_loss_grad(uint32_t nx, uint32_t ny, int H, int W, const uint8_
     *reinterpret_cast<v4s *>(gimg + p * C + 4 * qd) = o;
 }
 
+// ---- damped Jacobi around the network ----
+// d = the FIT operator's diagonal at an inside node.
+__device__ __forceinline__ double diag_a(const double *__restrict__ cx, const double *__restrict__ cy, uint32_t p,
+                                         uint32_t nx)
+{
+    return cx[p] + cx[p - 1] + cy[p] + cy[p - nx];
+}
+
+// z = omega r / d on inside nodes, 0 elsewhere.
+__global__ void k_jacobi(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+                         const uint8_t *__restrict__ mask, const double *__restrict__ r, double omega,
+                         double *__restrict__ z)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) z[i] = mask[i] ? omega * r[i] / diag_a(cx, cy, i, nx) : 0.;
+}
+
+// out = r - A z on inside nodes, 0 elsewhere; e (optional) = phi - z.
+__global__ void k_resid_az(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+                           const uint8_t *__restrict__ mask, const double *__restrict__ r, const double *__restrict__ z,
+                           double *__restrict__ out, const double *__restrict__ phi, double *__restrict__ e)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const bool in = mask[i];
+    out[i] = in ? r[i] - apply_a(cx, cy, z, i, nx) : 0.;
+    if (e) e[i] = in ? phi[i] - z[i] : 0.;
+}
+
+// an = A n, partials (n.r1, n.an)
+__global__ void __launch_bounds__(kThreads)
+    k_an_dots(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+              const uint8_t *__restrict__ mask, const double *__restrict__ n, const double *__restrict__ r1,
+              double *__restrict__ an, double *__restrict__ part)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    double v[2] = {0., 0.};
+    if (i < np) {
+        double a = 0.;
+        if (mask[i]) {
+            a = apply_a(cx, cy, n, i, nx);
+            v[0] = n[i] * r1[i];
+            v[1] = n[i] * a;
+        }
+        an[i] = a;
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// z = z1 + c n + omega (r1 - c A n) / d with c = (n.r1) / (n.A n) (sc[8], sc[9]; c = 0 when
+// the network's direction does not lower the A-norm error).
+__global__ void k_combine(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+                          const uint8_t *__restrict__ mask, const double *__restrict__ z1, const double *__restrict__ n,
+                          const double *__restrict__ r1, const double *__restrict__ an, const double *__restrict__ sc,
+                          double omega, double *__restrict__ z)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const double c = (sc[9] > 0. && sc[8] > 0.) ? sc[8] / sc[9] : 0.;
+    z[i] = mask[i] ? z1[i] + c * n[i] + omega * (r1[i] - c * an[i]) / diag_a(cx, cy, i, nx) : 0.;
+}
+
 // ---- flexible conjugate gradients (Notay 2000, truncation to one direction) ----
 // scalars: s[0] rms part (r.r), s[1] inside count, s[2] q.q, s[3] alpha, s[4] beta, s[5] r.z, s[6] p.Ap,
 // s[7] z.Ap, s[8] r.r (after the update)
@@ -545,7 +635,7 @@ int launch_conv(const oecl_unet *u, int l, int mode, const uint16_t *in, const u
                 hipStream_t s)
 {
     const int H = u->H >> l, W = u->W >> l;
-    const uint32_t waves = (uint32_t)H * ((W + 15) / 16);
+    const uint32_t waves = (uint32_t)((H + CONV_RW - 1) / CONV_RW) * ((W + 15) / 16);
     const dim3 g(nblk(waves, kThreads / 64)), bl(kThreads);
     if (mode == 0) hipLaunchKernelGGL(k_conv<0>, g, bl, 0, s, H, W, in, w, b, out);
     else if (mode == 1) hipLaunchKernelGGL(k_conv<1>, g, bl, 0, s, H, W, in, w, b, out);
@@ -701,6 +791,7 @@ OECL_EXPORT int oecl_unet_create(uint32_t H, uint32_t W, uint32_t seed, oecl_une
             u->grad[2 * l + 1] = (uint16_t *)take(2 * img_elems(u, l));
         }
         u->out = (float *)take(4ull * H * W);
+        for (int k = 0; k < 4; ++k) u->sv[k] = (double *)take(8ull * H * W);
         u->scal = (double *)take(8 * 64);
         u->part = (double *)take(8ull * 4 * (nblk((size_t)H * W) + 16));
         if (pass == 0) {
@@ -825,13 +916,48 @@ static int precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *m
     return OECL_OK;
 }
 
-OECL_EXPORT int oecl_unet_precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r,
-                                       double *z, void *stream)
+// The network between damped Jacobi smoothings (omega > 0): z1 = omega D^-1 r; the network's
+// n for r1 = r - A z1, scaled by the A-norm line search c = n.r1 / n.A n; z = z1 + c n +
+// omega D^-1 (r1 - c A n).  omega = 0: the network alone.
+static int precondition_smoothed(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                                 const uint8_t *mask, const double *r, double *z, double omega, hipStream_t s)
+{
+    if (!(omega > 0.)) return precondition(u, nx, ny, mask, r, z, s);
+    const uint32_t np = nx * ny, nb = nblk(np);
+    double *z1 = u->sv[0], *r1 = u->sv[1], *n = u->sv[2], *an = u->sv[3];
+    hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, r, omega, z1);
+    hipLaunchKernelGGL(k_resid_az, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, r, z1, r1, nullptr, nullptr);
+    int rc = precondition(u, nx, ny, mask, r1, n, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_an_dots, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, n, r1, an, u->part);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nb, u->scal, 8);
+    hipLaunchKernelGGL(k_combine, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, z1, n, r1, an, u->scal, omega, z);
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_unet_precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                                       const uint8_t *mask, const double *r, double *z, double omega, void *stream)
 {
     if (!u || !mask || !r || !z || ny > u->H || nx > u->W) return OECL_ERR_ARG;
-    int rc = precondition(u, nx, ny, mask, r, z, (hipStream_t)stream);
+    if (omega > 0. && (!cx || !cy)) return OECL_ERR_ARG;
+    int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, (hipStream_t)stream);
     OECL_CHECK_LAUNCH("oecl_unet_precondition");
     return rc;
+}
+
+// The pair the network meets inside the smoothed preconditioner: for (q, phi = A^-1 q),
+// r1 = q - A (omega D^-1 q) and e1 = phi - omega D^-1 q, so that A e1 = r1.
+OECL_EXPORT int oecl_unet_smooth_pair(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                                      const uint8_t *mask, const double *q, const double *phi, double omega, double *r1,
+                                      double *e1, void *stream)
+{
+    if (!u || !cx || !cy || !mask || !q || !phi || !r1 || !e1 || ny > u->H || nx > u->W) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t np = nx * ny, nb = nblk(np);
+    hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, q, omega, u->sv[0]);
+    hipLaunchKernelGGL(k_resid_az, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, q, u->sv[0], r1, phi, e1);
+    OECL_CHECK_LAUNCH("oecl_unet_smooth_pair");
+    return OECL_OK;
 }
 
 // One training sample: forward on q (the right-hand side, scaled to unit RMS), the
@@ -875,7 +1001,8 @@ OECL_EXPORT uint64_t oecl_fcg_work_bytes(uint32_t np) { return 8ull * (5ull * np
 // host every `check` iterations.
 OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
                                const uint8_t *mask, const double *q, double *phi, double tol, uint32_t maxIter,
-                               uint32_t check, void *work, uint32_t *iters, double *relres, void *stream)
+                               uint32_t check, double omega, void *work, uint32_t *iters, double *relres,
+                               void *stream)
 {
     if (!u || !cx || !cy || !mask || !q || !phi || !work || ny > u->H || nx > u->W || nx < 3 || ny < 3)
         return OECL_ERR_ARG;
@@ -894,7 +1021,7 @@ OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const dou
     double rr = h[0];
     uint32_t it = 0;
     if (sqrt(rr) > tol * qn) {
-        int rc = precondition(u, nx, ny, mask, r, z, st);
+        int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, st);
         if (rc) return rc;
         hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kThreads), 0, st, np, r, z, part);
         hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kThreads), 0, st, part, nb, T, 0);
@@ -908,7 +1035,7 @@ OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const dou
             hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1), 0, st, S);
             hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kThreads), 0, st, np, S, phi, r, p, ap, part);
             hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 8);
-            int rc = precondition(u, nx, ny, mask, r, z, st);
+            int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, st);
             if (rc) return rc;
             hipLaunchKernelGGL(k_fcg_zdots, dim3(nb), dim3(kThreads), 0, st, np, r, z, ap, part);
             hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, T, 0);

@@ -8,8 +8,13 @@ the A-norm error of the best multiple of the preconditioner's output z for q, re
 phi's A-norm (a preconditioner's scale does not matter to the flexible CG that applies it,
 so the loss ignores it too).  It is applied inside a flexible CG that
 keeps its vectors and reductions in FP64.  The network sees the residual scaled to unit RMS,
-so the preconditioner is homogeneous of degree one.  solve() falls back to the direct solve
-when the flexible CG does not reach its tolerance.
+so the preconditioner is homogeneous of degree one.  With omega > 0 (the default) the network
+sits between two damped Jacobi smoothings, z1 = omega D^-1 r, and its output for r1 = r - A z1
+enters through an A-norm line search: the smoothing handles the high frequencies a network of
+3x3 convolutions hardly represents, and the line search keeps the network from ever making a
+step worse than the smoothing alone.  It is then trained on the pairs it meets there
+(smoothed_pairs).  solve() falls back to the direct solve when the flexible CG does not reach
+its tolerance.
 """
 import ctypes
 
@@ -29,8 +34,9 @@ def image_size(gridObj):
 
 class UNetPreconditioner:
 
-    def __init__(self, gridObj, seed=0):
+    def __init__(self, gridObj, seed=0, omega=2. / 3.):
         self.g = gridObj
+        self.omega = float(omega)
         self.nx, self.ny = gridObj.getNxExt(), gridObj.getNyExt()
         self.H, self.W = image_size(gridObj)
         h = ctypes.c_void_p()
@@ -101,12 +107,27 @@ class UNetPreconditioner:
         return gin
 
     # ---------------------------------------------------------------- on the grid
-    def precondition(self, ps, r):
-        """z = rms(r) * network(r / rms(r)) on the solver's inside nodes."""
+    def precondition(self, ps, r, omega=None):
+        """The preconditioner applied to r on the solver's inside nodes (oecl_unet_precondition)."""
         z = torch.empty_like(r)
-        L.check(L.lib().oecl_unet_precondition(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.mask), L.ptr(r),
-                                               L.ptr(z), L.stream()), 'unet precondition')
+        om = self.omega if omega is None else float(omega)
+        L.check(L.lib().oecl_unet_precondition(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx), L.ptr(ps.cy),
+                                               L.ptr(ps.mask), L.ptr(r), L.ptr(z), L.f64(om), L.stream()),
+                'unet precondition')
         return z
+
+    def smoothed_pairs(self, ps, pairs, omega=None):
+        """The pairs (r1, e1) the network meets inside the smoothed preconditioner, from pairs
+        (q, phi = A^-1 q): r1 = q - A omega D^-1 q, e1 = phi - omega D^-1 q (A e1 = r1)."""
+        om = self.omega if omega is None else float(omega)
+        out = []
+        for q, phi in pairs:
+            r1, e1 = torch.empty_like(q), torch.empty_like(q)
+            L.check(L.lib().oecl_unet_smooth_pair(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx),
+                                                  L.ptr(ps.cy), L.ptr(ps.mask), L.ptr(q), L.ptr(phi), L.f64(om),
+                                                  L.ptr(r1), L.ptr(e1), L.stream()), 'unet smooth pair')
+            out.append((r1, e1))
+        return out
 
     def train(self, ps, pairs, steps, lr=1e-3, batch=1, lrEnd=None, seed=0, log=None, logEvery=0):
         """Adam on samples (q, phi) drawn from `pairs` (device FP64 grid vectors, phi = A^-1 q),
@@ -147,15 +168,17 @@ class UNetPreconditioner:
             L.ptr(self._work), L.i32(1), ctypes.byref(v), L.stream()), 'unet loss')
         return v.value
 
-    def fcg(self, ps, q, phi=None, tol=1e-8, maxIter=2000, check=4):
+    def fcg(self, ps, q, phi=None, tol=1e-8, maxIter=2000, check=4, omega=None):
         """Flexible CG with this preconditioner -> (phi, iterations, relative residual, converged)."""
         if phi is None:
             phi = torch.zeros_like(q)
         it = ctypes.c_uint32(0)
         rr = ctypes.c_double(0.)
+        om = self.omega if omega is None else float(omega)
         rc = L.lib().oecl_fcg_solve(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx), L.ptr(ps.cy),
                                     L.ptr(ps.mask), L.ptr(q), L.ptr(phi), L.f64(tol), L.u32(maxIter), L.u32(check),
-                                    L.ptr(self._fcgwork), ctypes.byref(it), ctypes.byref(rr), L.stream())
+                                    L.f64(om), L.ptr(self._fcgwork), ctypes.byref(it), ctypes.byref(rr),
+                                    L.stream())
         if rc not in (0, 4):
             L.check(rc, 'fcg solve')
         self.lastIterations, self.lastRelres = it.value, rr.value

@@ -63,7 +63,7 @@ class BuildupScan:
     are copied into the scan's block at construction; read them back with run_particles(r).
     """
 
-    def __init__(self, buildups, seeds, growth=2.0, absFrac=1. / 16, maxSteps=100000):
+    def __init__(self, buildups, seeds, growth=2.0, absFrac=1. / 16, maxSteps=100000, slack=1024):
         if len(buildups) != len(seeds) or not buildups:
             raise ValueError('one seed per buildup')
         b0 = buildups[0]
@@ -91,7 +91,7 @@ class BuildupScan:
         npx = self.g.getNpExt()
         self.np = npx
         counts = [b.p.macroParticleCount for b in buildups]
-        segOff, absOff = plan_segments(counts, growth, absFrac=absFrac)
+        segOff, absOff = plan_segments(counts, growth, slack=slack, absFrac=absFrac)
         self.segOff, self.absOff = segOff, absOff
         cap = int(segOff[-1])
         absTot = int(absOff[-1])

@@ -38,3 +38,28 @@ def test_size_queries_without_gpu():
     lib.oecl_pcg_work_bytes.restype = ctypes.c_uint64
     assert lib.oecl_scan_work_bytes(ctypes.c_uint32(10 ** 6)) > 0
     assert lib.oecl_pcg_work_bytes(ctypes.c_uint32(8580)) >= 5 * 8 * 8580
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason='liboecl.so not built')
+def test_invalid_arguments_are_refused_before_any_device_call():
+    """Entry points check their arguments first and return OECL_ERR_ARG (1) without touching
+    the device, so these calls are safe on a machine without a GPU."""
+    lib = ctypes.CDLL(LIB)
+    ARG = 1
+    vp = ctypes.c_void_p
+    assert lib.oecl_step_run(None, None, None, None, None, None) == ARG
+    assert lib.oecl_step_run_batch(None, None, None, None, None, None) == ARG
+    h = vp()
+    assert lib.oecl_fastpoisson_create(ctypes.c_uint32(3), ctypes.c_uint32(3), ctypes.c_double(1.), ctypes.c_double(1.),
+                                       ctypes.c_uint32(0), None, None, None, None, None, ctypes.byref(h), None) == ARG
+    assert lib.oecl_unet_create(ctypes.c_uint32(7), ctypes.c_uint32(8), ctypes.c_uint32(0), ctypes.byref(h)) == ARG
+    assert lib.oecl_unet_create(ctypes.c_uint32(8), ctypes.c_uint32(12), ctypes.c_uint32(0), ctypes.byref(h)) == ARG
+    assert lib.oecl_secem_pass1(None, 0, None, None, ctypes.c_uint32(4), None, 0, 0, 0, None, None, None) == ARG
+    assert lib.oecl_furman_pass1(None, 0, None, None, ctypes.c_uint32(4), None, 0, 0, 0, None, None, None) == ARG
+    assert lib.oecl_photo_emit(None, 0, None, ctypes.c_uint32(4), 0, 0, 0, ctypes.c_double(7.), ctypes.c_double(5.),
+                               ctypes.c_double(1.), ctypes.c_double(9.1e-31), None) == ARG
+    assert lib.oecl_sort_runs_by_cell(None, 0, None, None, ctypes.c_uint64(256), ctypes.c_uint32(1), None, None,
+                                      None, 0, 0, None, None, None, None) == ARG
+    assert lib.oecl_fcg_solve(None, ctypes.c_uint32(8), ctypes.c_uint32(8), None, None, None, None, None,
+                              ctypes.c_double(1e-8), ctypes.c_uint32(10), ctypes.c_uint32(1), ctypes.c_double(0.),
+                              None, None, None, None) == ARG
