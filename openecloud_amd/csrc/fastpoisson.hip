@@ -57,7 +57,6 @@ struct oecl_fastpoisson {
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
     uint32_t rCap;            // right-hand sides the vectors above hold
     char *batchMem;           // their own allocation once rCap > 1
-    double *kPart;            // split-K partial products of a single solve, KSPLIT * bx * by
 };
 
 namespace {
@@ -78,34 +77,21 @@ namespace {
 // for EPI 2, column gn is line gn % by of the potential gn / by (np nodes apart).
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int GT = 32, GNW = 8, GKC = 64, GNT = 64 * GNW, GPER = GT * GKC / GNT;
-constexpr int KSPLIT = 4;  // contraction split of a single solve's products (few output tiles)
 
-// Split contraction (part != null): blockIdx.z = parity + 2 split, the block sums k in
-// [split kLen, (split + 1) kLen) and stores its partial tile at part + split n N (EPI 0 layout,
-// ld n); k_dst_reduce adds the splits in order and applies the epilogue.
 template <int EPI>
 __global__ void __launch_bounds__(GNT)
     k_dst_gemm(int N, int n, const double *__restrict__ A, const double *__restrict__ X, int ldx,
                double *__restrict__ C, int ldc, uint32_t nx, const uint8_t *__restrict__ mask, uint32_t by,
-               uint32_t np, double *__restrict__ part, int kLen)
+               uint32_t np)
 {
     constexpr int LDSN = (2 * GKC * 33 > GNW * GT * 33) ? 2 * GKC * 33 : GNW * GT * 33;
     __shared__ double sm[LDSN];
     double(*As)[33] = reinterpret_cast<double(*)[33]>(sm);
     double(*Bs)[33] = reinterpret_cast<double(*)[33]>(sm + GKC * 33);
-    const int z = blockIdx.z & 1, split = blockIdx.z >> 1;
+    const int z = blockIdx.z;
     const int K0 = (n + 1) / 2, Kz = z ? n / 2 : K0;
-    const int kBeg = part ? split * kLen : 0, kEnd = part ? min(Kz, kBeg + kLen) : Kz;
     const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
-    if (m0 >= Kz || n0 >= N || kBeg >= kEnd) {
-        // whole block (no barrier is skipped by part of it); an empty split still clears its tile
-        if (part && m0 < Kz && n0 < N)
-            for (int e = threadIdx.x; e < GT * GT; e += GNT) {
-                const int gm = m0 + (e & (GT - 1)), gn = n0 + (e >> 5);
-                if (gm < Kz && gn < N) part[(size_t)split * n * N + (2 * gm + z) + (size_t)gn * n] = 0.;
-            }
-        return;
-    }
+    if (m0 >= Kz || n0 >= N) return;  // whole block: no barrier is skipped by part of it
     const double *Az = A + (size_t)z * K0 * K0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     v4d acc[2][2];
@@ -120,11 +106,11 @@ __global__ void __launch_bounds__(GNT)
             const int idx = tid + r * GNT;
             const int row = idx & (GT - 1), kk = idx >> 5;
             const int gm = m0 + row, gk = k0 + kk;
-            ra[r] = (gm < Kz && gk < kEnd) ? Az[gm + (size_t)gk * K0] : 0.;
+            ra[r] = (gm < Kz && gk < Kz) ? Az[gm + (size_t)gk * K0] : 0.;
             const int kb = idx % GKC, col = idx / GKC;
             const int gkb = k0 + kb, gn = n0 + col;
             double v = 0.;
-            if (gkb < kEnd && gn < N) {
+            if (gkb < Kz && gn < N) {
                 const double *xc = X + (size_t)gn * ldx;
                 const int h = n - 1 - gkb;
                 v = xc[gkb];
@@ -133,9 +119,9 @@ __global__ void __launch_bounds__(GNT)
             rb[r] = v;
         }
     };
-    load(kBeg);
+    load(0);
     constexpr int KW = GKC / GNW;
-    for (int k0 = kBeg; k0 < kEnd; k0 += GKC) {
+    for (int k0 = 0; k0 < Kz; k0 += GKC) {
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < GPER; ++r) {
@@ -144,7 +130,7 @@ __global__ void __launch_bounds__(GNT)
             Bs[idx % GKC][idx / GKC] = rb[r];
         }
         __syncthreads();
-        if (k0 + GKC < kEnd) load(k0 + GKC);
+        if (k0 + GKC < Kz) load(k0 + GKC);
 #pragma unroll
         for (int kk = wave * KW; kk < wave * KW + KW; kk += 4) {
             const int k = kk + (lane >> 4), c = lane & 15;
@@ -175,34 +161,13 @@ __global__ void __launch_bounds__(GNT)
 #pragma unroll
         for (int w = 1; w < GNW; ++w) s += Red[w][mm][nn];
         const int om = 2 * gm + z;
-        if (part) {
-            part[(size_t)split * n * N + om + (size_t)gn * n] = s;
-        } else if (EPI == 0) {
+        if (EPI == 0) {
             C[om + (size_t)gn * ldc] = s;
         } else {
             // column gn is line gn % by of right-hand side gn / by
             const uint32_t p = (uint32_t)(om + 2) + ((uint32_t)gn % by + 2) * nx;
             C[p + (size_t)((uint32_t)gn / by) * np] = mask[p] ? s : 0.;
         }
-    }
-}
-
-// Sum of the split partial products in split order, then the epilogue of k_dst_gemm (one thread
-// per output element, n x N).
-template <int EPI>
-__global__ void k_dst_reduce(int N, int n, int nSplit, const double *__restrict__ part, double *__restrict__ C,
-                             int ldc, uint32_t nx, const uint8_t *__restrict__ mask, uint32_t by, uint32_t np)
-{
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)n * N) return;
-    const int om = (int)(e % n), gn = (int)(e / n);
-    double s = part[e];
-    for (int k = 1; k < nSplit; ++k) s += part[(size_t)k * n * N + e];
-    if (EPI == 0) {
-        C[om + (size_t)gn * ldc] = s;
-    } else {
-        const uint32_t p = (uint32_t)(om + 2) + ((uint32_t)gn % by + 2) * nx;
-        C[p + (size_t)((uint32_t)gn / by) * np] = mask[p] ? s : 0.;
     }
 }
 
@@ -544,8 +509,7 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
         return OECL_ERR_HIP;
     }
     const uint64_t kx = (bx + 1) / 2;
-    const uint64_t bytes = 8ull * ((uint64_t)bx * bx + 2ull * kx * kx + 4ull * nb + 2ull * m + (uint64_t)m * m + nnz +
-                                   (uint64_t)KSPLIT * nb) +
+    const uint64_t bytes = 8ull * ((uint64_t)bx * bx + 2ull * kx * kx + 4ull * nb + 2ull * m + (uint64_t)m * m + nnz) +
                            4ull * (m + (m + 1) + nnz + h->nJ + (h->nJ + 1) + 2ull * m) + 1024;
     char *mem = nullptr;
     OECL_HIP(hipMalloc(&mem, bytes), "fastpoisson alloc");
@@ -559,7 +523,6 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->wy = (double *)cur; cur += 8ull * m;
     h->zE = (double *)cur; cur += 8ull * m;
     h->CinvT = (double *)cur; cur += 8ull * m * m;
-    h->kPart = (double *)cur; cur += 8ull * KSPLIT * nb;
     h->wVal = (double *)cur; cur += 8ull * nnz;
     h->sIdx = (int32_t *)cur; cur += 4ull * m;
     h->wPtr = (int32_t *)cur; cur += 4ull * (m + 1);
@@ -699,23 +662,8 @@ int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStrea
     if (R < 1 || R > h->rCap || (uint64_t)h->by * R > 0x7FFFFFFFull) return OECL_ERR_ARG;
     const int bx = h->bx, by = h->by, kx = (bx + 1) / 2, N = by * (int)R;
     const dim3 gg((kx + GT - 1) / GT, (N + GT - 1) / GT, 2), gb(GNT);
-    // a single solve has few output tiles: split the contraction over KSPLIT blocks each
-    const int tiles = (int)(gg.x * gg.y * 2);
-    const int kLen = ((kx + KSPLIT - 1) / KSPLIT + GKC - 1) / GKC * GKC;
-    const bool split = R == 1 && tiles < 1024 && kx > GKC;
-    const int nSplit = split ? (kx + kLen - 1) / kLen : 1;
-    const dim3 gs(gg.x, gg.y, 2 * nSplit);
-    const unsigned rb = (unsigned)(((size_t)bx * N + 255) / 256);
     // hat b = Sx b
-    if (split) {
-        hipLaunchKernelGGL(k_dst_gemm<0>, gs, gb, 0, s, N, bx, h->Fx, h->b, bx, h->t1, bx, 0u, nullptr, 1u, 0u,
-                           h->kPart, kLen);
-        hipLaunchKernelGGL(k_dst_reduce<0>, dim3(rb), dim3(256), 0, s, N, bx, nSplit, h->kPart, h->t1, bx, 0u,
-                           nullptr, 1u, 0u);
-    } else {
-        hipLaunchKernelGGL(k_dst_gemm<0>, gg, gb, 0, s, N, bx, h->Fx, h->b, bx, h->t1, bx, 0u, nullptr, 1u, 0u,
-                           nullptr, 0);
-    }
+    hipLaunchKernelGGL(k_dst_gemm<0>, gg, gb, 0, s, N, bx, h->Fx, h->b, bx, h->t1, bx, 0u, nullptr, 1u, 0u);
     if (h->m) {
         // hat y = T^-1 hat b ; wy = W Sx hat y ; hat b -= (U C^-1 wy) in x-transformed form
         int rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
@@ -729,15 +677,8 @@ int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStrea
     // hat x = T^-1 hat b ; phi = Sx hat x scattered into the extended grid (np apart per side)
     int rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
     if (rc != OECL_OK) return rc;
-    if (split) {
-        hipLaunchKernelGGL(k_dst_gemm<2>, gs, gb, 0, s, N, bx, h->Fx, h->t2, bx, phi, 0, h->nx, h->mask,
-                           (uint32_t)by, h->np, h->kPart, kLen);
-        hipLaunchKernelGGL(k_dst_reduce<2>, dim3(rb), dim3(256), 0, s, N, bx, nSplit, h->kPart, phi, 0, h->nx,
-                           h->mask, (uint32_t)by, h->np);
-    } else {
-        hipLaunchKernelGGL(k_dst_gemm<2>, gg, gb, 0, s, N, bx, h->Fx, h->t2, bx, phi, 0, h->nx, h->mask,
-                           (uint32_t)by, h->np, nullptr, 0);
-    }
+    hipLaunchKernelGGL(k_dst_gemm<2>, gg, gb, 0, s, N, bx, h->Fx, h->t2, bx, phi, 0, h->nx, h->mask,
+                       (uint32_t)by, h->np);
     OECL_CHECK_LAUNCH("oecl_fastpoisson_solve");
     return OECL_OK;
 }
