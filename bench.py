@@ -52,7 +52,10 @@ def parse():
     ap.add_argument('--sort-every', type=int, default=50,
                     help='re-order particles by cell every this many steps (0: never)')
     ap.add_argument('--train-steps', type=int, default=3000, help='learned: Adam steps')
-    ap.add_argument('--pairs', type=int, default=192, help='learned: training pairs (half clouds, half noise)')
+    ap.add_argument('--pairs', type=int, default=192, help='learned: training pairs')
+    ap.add_argument('--train-pairs', choices=('operator', 'buildup'), default='operator',
+                    help='learned: train on (A phi, phi) with phi random at 0-64 cell scales, or on the '
+                         'buildup dumps (half) and white-noise charges (half) with phi from the direct solve')
     ap.add_argument('--grid', type=int, default=1024, help='learned: grid cells per side')
     ap.add_argument('--workload', default='dipole', choices=['dipole', 'scan', 'learned', 'full'],
                     help='dipole: the elliptical dipole buildup (default); scan: 64 buildups '
@@ -250,11 +253,13 @@ def compare_f32(a, n, b64, it0):
 
 def main_learned(a):
     """The learned-preconditioner workload: a cut-cell grid of the elliptical chamber (45 x 35
-    mm semi-axes) with a circular conductor of radius 2.5 mm centred at x = 20 mm; training
-    pairs (q, phi) from buildup clouds dumped every 50 steps (charge plus the beam term at that
-    time) and from white noise, phi from the direct solve; the U-Net trained on the device; then
-    Poisson solves to 1e-8 relative residual by the learned flexible CG, by Jacobi PCG and by
-    the direct solve."""
+    mm semi-axes) with a circular conductor of radius 2.5 mm centred at x = 20 mm.  Test pairs
+    (q, phi): buildup clouds dumped every 50 steps (charge plus the beam term at that time) and
+    white-noise charges, phi from the direct solve.  Training pairs: (A phi, phi) with phi
+    random at length scales of 0-64 cells (default), or more buildup dumps and white-noise
+    charges (--train-pairs buildup).  The U-Net is trained on the device; then Poisson solves
+    to 1e-8 relative residual by the learned flexible CG, by Jacobi PCG and by the direct
+    solve."""
     import numpy as np
     import torch
     from openecloud_amd import _lib as L
@@ -293,9 +298,12 @@ def main_learned(a):
     homoLoader(g, p, pb, npart, 1e9 / npart, 4.194e5)
     p.reserve(3 * npart)
     b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
-    ncl = a.pairs // 2
+    ntest = 4
+    fromBuildup = a.train_pairs == 'buildup'
+    ncl = (a.pairs // 2 if fromBuildup else 0) + ntest
+    nwh = (a.pairs - a.pairs // 2 if fromBuildup else 0) + ntest
     b.native_setup(absCap=65536, secCap=131072, maxSteps=50 * ncl + 10)
-    pairs = []
+    clouds, noise = [], []
     t0 = time.perf_counter()
     for k in range(ncl):
         for _ in range(50):
@@ -303,18 +311,22 @@ def main_learned(a):
         b.sync()
         b.deposit_all()
         q = b._rhs(b.it * dt).clone()
-        pairs.append((q, ps.solvePhi(q).clone()))
+        clouds.append((q, ps.solvePhi(q).clone()))
     rng = np.random.default_rng(1)
     mask = ps.mask.to(torch.float64)
-    for k in range(a.pairs - ncl):
+    for k in range(nwh):
         q = torch.as_tensor(rng.standard_normal(g.getNpExt())).cuda() * mask * 1e-9
-        pairs.append((q, ps.solvePhi(q).clone()))
-    log('%d pairs: %.1f s' % (len(pairs), time.perf_counter() - t0))
-    test = pairs[ncl - 4:ncl] + pairs[-4:]
+        noise.append((q, ps.solvePhi(q).clone()))
+    test = clouds[-ntest:] + noise[-ntest:]
     net = UNetPreconditioner(g, seed=0)
-    train = net.smoothed_pairs(ps, pairs[:ncl - 4] + pairs[ncl:-4])
+    raw = clouds[:-ntest] + noise[:-ntest] if fromBuildup else net.operator_pairs(ps, a.pairs, seed=1)
+    pairs = raw
+    log('%d training pairs (%s), %d test pairs: %.1f s' % (len(raw), a.train_pairs, len(test),
+                                                          time.perf_counter() - t0))
+    train = net.smoothed_pairs(ps, raw)
+    del clouds, noise
     # the same flexible CG with the untrained network (the ablation)
-    base = [net.fcg(ps, q, tol=1e-8, maxIter=5000)[1] for q, _ in test[:2]]
+    base = [net.fcg(ps, q, tol=1e-8, maxIter=5000)[1] for q, _ in test]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = net.train(ps, train, steps=a.train_steps, lr=5e-4, lrEnd=5e-5, batch=4,
@@ -363,7 +375,9 @@ def main_learned(a):
     summ['learned']['converged'] = all(x[3] for x in res['learned'])
     summ['learned']['solution_rel_err_vs_direct_max'] = max(x[4] for x in res['learned'])
     summ['learned']['iterations_untrained_network'] = float(np.mean(base))
-    summ['learned']['iterations_trained_same_pairs'] = float(np.mean([x[1] for x in res['learned'][:2]]))
+    summ['learned']['iterations_per_test'] = [int(x[1]) for x in res['learned']]
+    summ['learned']['iterations_untrained_per_test'] = [int(x) for x in base]
+    summ['jacobi']['iterations_per_test'] = [int(x[1]) for x in res['jacobi']]
     out = {
         'metric': 'Poisson solves/sec to 1e-8 relative residual, learned-preconditioner flexible CG',
         'value': summ['learned']['solves_per_s'],
@@ -376,11 +390,15 @@ def main_learned(a):
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'bf16 network (FP32 accumulation), FP64 CG, damped Jacobi (omega 2/3) around the network',
-        'data': 'synthetic: (q, phi) pairs from a seeded buildup on this grid and from white noise',
+        'data': 'synthetic: test pairs from a seeded buildup on this grid (4) and white-noise charges (4); '
+                'training pairs ' + ('(A phi, phi), phi random at 0-64 cell scales' if not fromBuildup else
+                                     'from the same buildup and white-noise charges'),
         'config': {'workload': '%dx%d cut-cell elliptical chamber 45x35 mm with a conductor of radius 2.5 mm at '
                                'x = 20 mm; U-Net 4 levels, 16 channels' % (n, n),
                    'grid': [n, n], 'inside_nodes': int(ps.mask.sum().item()),
-                   'capacitance_rows': ps.capacitanceRows, 'pairs': len(pairs), 'train_steps': a.train_steps,
+                   'capacitance_rows': ps.capacitanceRows, 'pairs': len(pairs), 'train_pairs': a.train_pairs,
+                   'fcg_directions': net.mDirs, 'smoothing_omega': net.omega,
+                   'train_steps': a.train_steps,
                    'train_seconds': t_train, 'loss_history': hist},
         'solvers': summ,
         'unet_apply_ms': t_apply * 1e3,

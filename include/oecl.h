@@ -319,6 +319,14 @@ int oecl_pcg_solve(uint32_t nx, uint32_t np, const double *cx, const double *cy,
                    uint32_t maxIter, void *work, uint32_t *iters, double *relres,
                    void *stream);
 
+/* y = A x with A the FIT operator of oecl_pcg_solve restricted to the inside nodes
+ * (poissonSolver.pyx:55: x is taken as zero on non-inside nodes, y is 0 there).  The
+ * reference forms A only inside its SuperLU factorisation; this is its product, used to make
+ * pairs (A phi, phi) for training the learned preconditioner.  Inside nodes lie off the
+ * extended grid's outer ring, as the reference's grids guarantee. */
+int oecl_fit_apply(uint32_t nx, uint32_t np, const double *cx, const double *cy,
+                   const uint8_t *mask, const double *x, double *y, void *stream);
+
 /* phiToEAtGridPoints.dot(phi) (poissonSolver.pyx:67,87) as a CSR SpMV with the rows' own
  * column order: y[r] = sum_k val[k]*x[col[k]], k from rowPtr[r] to rowPtr[r+1]. */
 int oecl_csr_spmv(uint32_t nRows, const int32_t *rowPtr, const int32_t *col,
@@ -375,8 +383,9 @@ int oecl_unet_adam(oecl_unet *u, float lr, float beta1, float beta2, float eps, 
 /* The preconditioner on the inside nodes (mask) of the nx x ny grid (FP64).  omega = 0: the
  * network alone, z = rms(r) * network(r / rms(r)).  omega > 0: the network between damped
  * Jacobi smoothings, z1 = omega D^-1 r, n = network applied to r1 = r - A z1,
- * z = z1 + c n + omega D^-1 (r1 - c A n) with the A-norm line search c = n.r1 / n.A n (c = 0 when
- * that is not positive); cx, cy: the stencil arrays of oecl_pcg_solve, D its diagonal. */
+ * z = z1 + c n + omega D^-1 (r1 - c A n) with the A-norm line search c = n.r1 / n.A n of either
+ * sign (c = 0 when n.A n is not positive); cx, cy: the stencil arrays of oecl_pcg_solve, D its
+ * diagonal. */
 int oecl_unet_precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
                            const uint8_t *mask, const double *r, double *z, double omega, void *stream);
 /* The pair the network meets inside the smoothed preconditioner: for (q, phi = A^-1 q),
@@ -392,14 +401,15 @@ int oecl_unet_smooth_pair(oecl_unet *u, uint32_t nx, uint32_t ny, const double *
 int oecl_unet_train_sample(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
                            const uint8_t *mask, const double *q, const double *phi, double *work, int zero,
                            double *loss, void *stream);
-/* Flexible CG (Notay's truncated form) preconditioned as oecl_unet_precondition with omega,
- * FP64 vectors and reductions: phi (initial guess in, solution out) until
+/* Flexible CG (Notay's FCG(m): each new direction is A-orthogonalised against the last mDirs,
+ * 1 <= mDirs <= 8; mDirs = 1 is the truncated form) preconditioned as oecl_unet_precondition
+ * with omega, FP64 vectors and reductions: phi (initial guess in, solution out) until
  * ||q - A phi|| <= tol ||q||, the norm read every `check` iterations.  Returns OECL_ERR_NOCONV
- * when maxIter is reached. */
-uint64_t oecl_fcg_work_bytes(uint32_t np);
+ * when maxIter is reached.  work: oecl_fcg_work_bytes(np, mDirs) bytes. */
+uint64_t oecl_fcg_work_bytes(uint32_t np, uint32_t mDirs);
 int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy, const uint8_t *mask,
                    const double *q, double *phi, double tol, uint32_t maxIter, uint32_t check, double omega,
-                   void *work, uint32_t *iters, double *relres, void *stream);
+                   uint32_t mDirs, void *work, uint32_t *iters, double *relres, void *stream);
 
 /* ---- the whole plain step (example01_basic.py:52-97, example02/04 variants) ---------- */
 

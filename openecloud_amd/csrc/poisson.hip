@@ -83,6 +83,24 @@ __global__ void __launch_bounds__(kThreads)
         for (int k = 0; k < 3; ++k) partials[3 * blockIdx.x + k] = v[k];
 }
 
+// y = A x on inside nodes with x taken as zero on the other nodes (the restriction of the
+// operator to the inside unknowns, poissonSolver.pyx:55), y = 0 elsewhere.
+__global__ void __launch_bounds__(kThreads)
+    k_fit_apply(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+                const uint8_t *__restrict__ mask, const double *__restrict__ x, double *__restrict__ y)
+{
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= np) return;
+    double a = 0.;
+    if (mask[i]) {
+        const double xp = x[i];
+        const double xe = mask[i + 1] ? x[i + 1] : 0., xw = mask[i - 1] ? x[i - 1] : 0.;
+        const double xn = mask[i + nx] ? x[i + nx] : 0., xs = mask[i - nx] ? x[i - nx] : 0.;
+        a = cx[i] * (xp - xe) - cx[i - 1] * (xw - xp) + cy[i] * (xp - xn) - cy[i - nx] * (xs - xp);
+    }
+    y[i] = a;
+}
+
 // Zero phi on non-inside nodes before the first residual (runs before k_pcg_init).
 __global__ void k_mask_phi(uint32_t np, const uint8_t *__restrict__ mask, double *__restrict__ phi)
 {
@@ -260,6 +278,16 @@ OECL_EXPORT int oecl_pcg_solve(uint32_t nx, uint32_t np, const double *cx, const
     double rel = sqrt(rr) / (qn > 0. ? qn : 1.);
     if (relres) *relres = rel;
     if (!(rel <= tol)) return OECL_ERR_NOCONV;
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_fit_apply(uint32_t nx, uint32_t np, const double *cx, const double *cy, const uint8_t *mask,
+                               const double *x, double *y, void *stream)
+{
+    if (!cx || !cy || !mask || !x || !y || nx < 3 || np < 3 * nx || np % nx) return OECL_ERR_ARG;
+    hipLaunchKernelGGL(k_fit_apply, dim3(oecl_blocks(np, kThreads)), dim3(kThreads), 0, (hipStream_t)stream, nx,
+                       np, cx, cy, mask, x, y);
+    OECL_CHECK_LAUNCH("oecl_fit_apply");
     return OECL_OK;
 }
 

@@ -486,8 +486,10 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
-// z = z1 + c n + omega (r1 - c A n) / d with c = (n.r1) / (n.A n) (sc[8], sc[9]; c = 0 when
-// the network's direction does not lower the A-norm error).
+// z = z1 + c n + omega (r1 - c A n) / d with c = (n.r1) / (n.A n) (sc[8], sc[9]), the A-norm
+// optimal multiple of the network's output, of either sign: the training loss does not fix the
+// output's sign, so a network may learn -A^-1 as well as A^-1.  c = 0 when n.A n is not
+// positive (n = 0, or not finite).
 __global__ void k_combine(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
                           const uint8_t *__restrict__ mask, const double *__restrict__ z1, const double *__restrict__ n,
                           const double *__restrict__ r1, const double *__restrict__ an, const double *__restrict__ sc,
@@ -495,13 +497,20 @@ __global__ void k_combine(uint32_t nx, uint32_t np, const double *__restrict__ c
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
-    const double c = (sc[9] > 0. && sc[8] > 0.) ? sc[8] / sc[9] : 0.;
+    const double c = sc[9] > 0. ? sc[8] / sc[9] : 0.;
     z[i] = mask[i] ? z1[i] + c * n[i] + omega * (r1[i] - c * an[i]) / diag_a(cx, cy, i, nx) : 0.;
 }
 
-// ---- flexible conjugate gradients (Notay 2000, truncation to one direction) ----
-// scalars: s[0] rms part (r.r), s[1] inside count, s[2] q.q, s[3] alpha, s[4] beta, s[5] r.z, s[6] p.Ap,
-// s[7] z.Ap, s[8] r.r (after the update)
+// ---- flexible conjugate gradients (Notay 2000: FCG with m_max kept directions) ----
+// Direction k is the preconditioned residual z_k A-orthogonalised against the last
+// min(k, m) directions, p_k = z_k - sum_j (z_k.Ap_j / p_j.Ap_j) p_j, held in a ring of m + 1
+// slots (slot k mod (m + 1)); alpha_k = p_k.r_k / p_k.Ap_k.  m = 1 is the truncated form
+// (beta = -z.Ap / p.Ap).  Keeping more directions restores what a preconditioner that varies
+// from one application to the next (a ReLU network and its line search) takes away from the
+// CG recurrence.
+// scalars: s[2] q.q, s[3] alpha, s[6] p.Ap, s[7] p.r, s[8] r.r (after the update), s[9] q.q (init)
+constexpr uint32_t FCG_MAXM = 8;
+
 __global__ void __launch_bounds__(kThreads)
     k_fcg_init(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
                const uint8_t *__restrict__ mask, const double *__restrict__ q, double *__restrict__ phi,
@@ -531,45 +540,73 @@ __global__ void k_mask_zero(uint32_t np, const uint8_t *__restrict__ mask, doubl
     if (i < np && !mask[i]) x[i] = 0.;
 }
 
-// partial r.z
-__global__ void __launch_bounds__(kThreads) k_dot(uint32_t np, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ part)
+// Partial z.Ap_j for the window's directions j = k - 1 - t, t < nUsed (FCG_MAXM per block).
+__global__ void __launch_bounds__(kThreads)
+    k_fcg_zdots(uint32_t np, const double *__restrict__ z, const double *__restrict__ AP, uint32_t k, uint32_t m1,
+                uint32_t nUsed, double *__restrict__ part)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    double v[1] = {i < np ? a[i] * b[i] : 0.};
-    block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    double v[FCG_MAXM];
+#pragma unroll
+    for (uint32_t t = 0; t < FCG_MAXM; ++t) v[t] = 0.;
+    if (i < np) {
+        const double zi = z[i];
+#pragma unroll
+        for (uint32_t t = 0; t < FCG_MAXM; ++t)
+            if (t < nUsed) v[t] = zi * AP[(size_t)((k - 1 - t) % m1) * np + i];
+    }
+    block_sum<FCG_MAXM>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (uint32_t t = 0; t < FCG_MAXM; ++t) part[FCG_MAXM * blockIdx.x + t] = v[t];
 }
 
-// p = z + beta p (beta = s[4]; first = 1: p = z)
-__global__ void k_fcg_dir(uint32_t np, const double *__restrict__ s, const double *__restrict__ z, double *__restrict__ p, int first)
+// p_k = z - sum_t (d_t / pAp[slot]) p_slot into slot k mod m1 (d = z.Ap_j from k_fcg_zdots; a
+// direction with p.Ap <= 0 is skipped).
+__global__ void k_fcg_newdir(uint32_t np, const double *__restrict__ z, double *__restrict__ P,
+                             const double *__restrict__ d, const double *__restrict__ pAp, uint32_t k, uint32_t m1,
+                             uint32_t nUsed)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < np) p[i] = first ? z[i] : z[i] + s[4] * p[i];
+    if (i >= np) return;
+    double v = z[i];
+    for (uint32_t t = 0; t < nUsed; ++t) {
+        const uint32_t sl = (k - 1 - t) % m1;
+        const double den = pAp[sl];
+        if (den > 0.) v -= (d[t] / den) * P[(size_t)sl * np + i];
+    }
+    P[(size_t)(k % m1) * np + i] = v;
 }
 
-// Ap, partial (p.Ap, z.Ap) -- z.Ap with the previous z gives the next beta's denominator term
+// Ap; partial (p.Ap, p.r)
 __global__ void __launch_bounds__(kThreads)
     k_fcg_ap(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
-             const uint8_t *__restrict__ mask, const double *__restrict__ p, double *__restrict__ ap, double *__restrict__ part)
+             const uint8_t *__restrict__ mask, const double *__restrict__ p, const double *__restrict__ r,
+             double *__restrict__ ap, double *__restrict__ part)
 {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    double v[1] = {0.};
+    double v[2] = {0., 0.};
     if (i < np) {
         double a = 0.;
         if (mask[i]) {
             a = apply_a(cx, cy, p, i, nx);
             v[0] = p[i] * a;
+            v[1] = p[i] * r[i];
         }
         ap[i] = a;
     }
-    block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
 }
 
-// alpha = r.z / p.Ap
-__global__ void k_alpha(double *s)
+// alpha = p.r / p.Ap (0 when p.Ap is not positive); p.Ap kept for the direction's slot.
+__global__ void k_alpha(double *s, double *pAp, uint32_t slot)
 {
-    s[3] = s[6] != 0. ? s[5] / s[6] : 0.;
+    s[3] = s[6] > 0. ? s[7] / s[6] : 0.;
+    pAp[slot] = s[6];
 }
 
 // phi += alpha p, r -= alpha Ap; partial r.r
@@ -593,34 +630,6 @@ __global__ void __launch_bounds__(kThreads)
         part[2 * blockIdx.x + 1] = v[1];
     }
 }
-
-// After the new z: partial (r.z, z.Ap) -> beta = -alpha z.Ap / (r.z)_old (flexible CG)
-__global__ void __launch_bounds__(kThreads)
-    k_fcg_zdots(uint32_t np, const double *__restrict__ r, const double *__restrict__ z, const double *__restrict__ ap,
-                double *__restrict__ part)
-{
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    double v[2] = {0., 0.};
-    if (i < np) {
-        v[0] = r[i] * z[i];
-        v[1] = z[i] * ap[i];
-    }
-    block_sum<2>(v);
-    if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = v[0];
-        part[2 * blockIdx.x + 1] = v[1];
-    }
-}
-
-__global__ void k_beta(double *s, const double *t)
-{
-    // t[0] = new r.z, t[1] = z.Ap; s[5] = old r.z
-    const double rzOld = s[5];
-    s[4] = rzOld != 0. ? -s[3] * t[1] / rzOld : 0.;
-    s[5] = t[0];
-}
-
-__global__ void k_copy_rz(double *s, const double *t) { s[5] = t[0]; }
 
 uint32_t nblk(size_t n, int b = kThreads) { return (uint32_t)((n + b - 1) / b); }
 
@@ -993,24 +1002,28 @@ OECL_EXPORT int oecl_unet_train_sample(oecl_unet *u, uint32_t nx, uint32_t ny, c
     return OECL_OK;
 }
 
-OECL_EXPORT uint64_t oecl_fcg_work_bytes(uint32_t np) { return 8ull * (5ull * np + 4 * (nblk(np) + 1) + 64) + 256; }
+OECL_EXPORT uint64_t oecl_fcg_work_bytes(uint32_t np, uint32_t mDirs)
+{
+    const uint64_t m1 = (mDirs < 1 ? 1 : mDirs > FCG_MAXM ? FCG_MAXM : mDirs) + 1;
+    return 8ull * ((2 + 2 * m1) * (uint64_t)np + FCG_MAXM * (nblk(np) + 1) + 2 * FCG_MAXM + 64) + 256;
+}
 
-// Flexible CG on the FIT system with the network as preconditioner (u == NULL: Jacobi is not
-// offered here; use oecl_pcg_solve).  phi: initial guess in, solution out (zero off the
-// inside nodes).  Converged when ||q - A phi|| <= tol ||q||; the residual norm is read on the
-// host every `check` iterations.
+// Flexible CG on the FIT system with the network as preconditioner, keeping mDirs directions
+// (1 .. 8).  phi: initial guess in, solution out (zero off the inside nodes).  Converged when
+// ||q - A phi|| <= tol ||q||; the residual norm is read on the host every `check` iterations.
 OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
                                const uint8_t *mask, const double *q, double *phi, double tol, uint32_t maxIter,
-                               uint32_t check, double omega, void *work, uint32_t *iters, double *relres,
-                               void *stream)
+                               uint32_t check, double omega, uint32_t mDirs, void *work, uint32_t *iters,
+                               double *relres, void *stream)
 {
-    if (!u || !cx || !cy || !mask || !q || !phi || !work || ny > u->H || nx > u->W || nx < 3 || ny < 3)
+    if (!u || !cx || !cy || !mask || !q || !phi || !work || ny > u->H || nx > u->W || nx < 3 || ny < 3 ||
+        mDirs < 1 || mDirs > FCG_MAXM)
         return OECL_ERR_ARG;
     if (check == 0) check = 1;
     hipStream_t st = (hipStream_t)stream;
-    const uint32_t np = nx * ny, nb = nblk(np);
-    double *r = (double *)work, *z = r + np, *p = z + np, *ap = p + np;
-    double *part = ap + np, *S = part + 4 * (nb + 1), *T = S + 16;
+    const uint32_t np = nx * ny, nb = nblk(np), m1 = mDirs + 1;
+    double *r = (double *)work, *z = r + np, *P = z + np, *AP = P + (size_t)m1 * np;
+    double *part = AP + (size_t)m1 * np, *S = part + FCG_MAXM * (nb + 1), *D = S + 16, *pAp = D + FCG_MAXM;
     hipLaunchKernelGGL(k_mask_zero, dim3(nb), dim3(kThreads), 0, st, np, mask, phi);
     hipLaunchKernelGGL(k_fcg_init, dim3(nb), dim3(kThreads), 0, st, nx, np, cx, cy, mask, q, phi, r, part);
     hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 8);  // S[8] r.r, S[9] q.q
@@ -1020,27 +1033,22 @@ OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const dou
     const double qn = sqrt(h[1]) > 0. ? sqrt(h[1]) : 1.;
     double rr = h[0];
     uint32_t it = 0;
-    if (sqrt(rr) > tol * qn) {
-        int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, st);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kThreads), 0, st, np, r, z, part);
-        hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kThreads), 0, st, part, nb, T, 0);
-        hipLaunchKernelGGL(k_copy_rz, dim3(1), dim3(1), 0, st, S, T);
-        hipLaunchKernelGGL(k_fcg_dir, dim3(nb), dim3(kThreads), 0, st, np, S, z, p, 1);
-    }
     while (sqrt(rr) > tol * qn && it < maxIter) {
         for (uint32_t k = 0; k < check && it < maxIter; ++k, ++it) {
-            hipLaunchKernelGGL(k_fcg_ap, dim3(nb), dim3(kThreads), 0, st, nx, np, cx, cy, mask, p, ap, part);
-            hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 6);
-            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1), 0, st, S);
-            hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kThreads), 0, st, np, S, phi, r, p, ap, part);
-            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 8);
             int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, st);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_fcg_zdots, dim3(nb), dim3(kThreads), 0, st, np, r, z, ap, part);
-            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, T, 0);
-            hipLaunchKernelGGL(k_beta, dim3(1), dim3(1), 0, st, S, T);
-            hipLaunchKernelGGL(k_fcg_dir, dim3(nb), dim3(kThreads), 0, st, np, S, z, p, 0);
+            const uint32_t nUsed = it < mDirs ? it : mDirs;
+            if (nUsed) {
+                hipLaunchKernelGGL(k_fcg_zdots, dim3(nb), dim3(kThreads), 0, st, np, z, AP, it, m1, nUsed, part);
+                hipLaunchKernelGGL(k_finish<FCG_MAXM>, dim3(1), dim3(kThreads), 0, st, part, nb, D, 0);
+            }
+            hipLaunchKernelGGL(k_fcg_newdir, dim3(nblk(np, 256)), dim3(256), 0, st, np, z, P, D, pAp, it, m1, nUsed);
+            double *p = P + (size_t)(it % m1) * np, *ap = AP + (size_t)(it % m1) * np;
+            hipLaunchKernelGGL(k_fcg_ap, dim3(nb), dim3(kThreads), 0, st, nx, np, cx, cy, mask, p, r, ap, part);
+            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 6);
+            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1), 0, st, S, pAp, it % m1);
+            hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kThreads), 0, st, np, S, phi, r, p, ap, part);
+            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 8);
         }
         OECL_CHECK_LAUNCH("oecl_fcg_solve");
         OECL_HIP(hipMemcpyAsync(h, S + 8, 8, hipMemcpyDeviceToHost, st), "fcg copy");

@@ -15,6 +15,11 @@ enters through an A-norm line search: the smoothing handles the high frequencies
 step worse than the smoothing alone.  It is then trained on the pairs it meets there
 (smoothed_pairs).  solve() falls back to the direct solve when the flexible CG does not reach
 its tolerance.
+
+Training pairs: operator_pairs() makes (A phi, phi) with phi random at length scales of 0 to 64
+cells.  Pairs (q, A^-1 q) for cloud-like charges also work on small grids, but on large ones
+their error is mostly a global smooth potential that a 4-level network of 3x3 convolutions
+cannot reach, and training on them stalls.
 """
 import ctypes
 
@@ -34,9 +39,10 @@ def image_size(gridObj):
 
 class UNetPreconditioner:
 
-    def __init__(self, gridObj, seed=0, omega=2. / 3.):
+    def __init__(self, gridObj, seed=0, omega=2. / 3., mDirs=8):
         self.g = gridObj
         self.omega = float(omega)
+        self.mDirs = int(mDirs)
         self.nx, self.ny = gridObj.getNxExt(), gridObj.getNyExt()
         self.H, self.W = image_size(gridObj)
         h = ctypes.c_void_p()
@@ -45,8 +51,7 @@ class UNetPreconditioner:
         dev = L.device()
         np_ = self.nx * self.ny
         self._work = torch.empty(2 * np_, dtype=torch.float64, device=dev)
-        self._fcgwork = torch.empty(int(L.lib().oecl_fcg_work_bytes(np_)) // 8 + 8, dtype=torch.float64,
-                                    device=dev)
+        self._fcgwork = None
         self.history = []
         self.lastIterations = 0
         self.lastRelres = None
@@ -129,6 +134,29 @@ class UNetPreconditioner:
             out.append((r1, e1))
         return out
 
+    def operator_pairs(self, ps, count, scales=(0, 1, 2, 4, 8, 16, 32, 64), seed=0):
+        """count pairs (q = A phi, phi) with phi white noise smoothed by a Gaussian of standard
+        deviation s cells (s cycling through `scales`, 0 = unsmoothed), zero outside the inside
+        nodes and of unit norm.  Their errors spread over every length scale up to the largest
+        s, which a 4-level network of 3x3 convolutions can represent; right-hand sides with
+        global solutions (a cloud's potential) leave most of their error beyond its reach."""
+        rng = np.random.default_rng(seed)
+        nx, ny = self.nx, self.ny
+        mask = ps.mask.cpu().numpy().astype(bool)
+        kx2 = np.fft.rfftfreq(nx)[None, :] ** 2
+        ky2 = np.fft.fftfreq(ny)[:, None] ** 2
+        out = []
+        for k in range(int(count)):
+            s = float(scales[k % len(scales)])
+            f = rng.standard_normal((ny, nx))
+            if s > 0:
+                f = np.fft.irfft2(np.fft.rfft2(f) * np.exp(-2. * (np.pi * s) ** 2 * (kx2 + ky2)), s=(ny, nx))
+            f = np.where(mask, f.ravel(), 0.)
+            f /= max(np.linalg.norm(f), 1e-300)
+            phi = torch.as_tensor(f).to(L.device())
+            out.append((ps.applyA(phi), phi))
+        return out
+
     def train(self, ps, pairs, steps, lr=1e-3, batch=1, lrEnd=None, seed=0, log=None, logEvery=0):
         """Adam on samples (q, phi) drawn from `pairs` (device FP64 grid vectors, phi = A^-1 q),
         `batch` samples per update; the learning rate decays geometrically to lrEnd.  Returns
@@ -168,17 +196,23 @@ class UNetPreconditioner:
             L.ptr(self._work), L.i32(1), ctypes.byref(v), L.stream()), 'unet loss')
         return v.value
 
-    def fcg(self, ps, q, phi=None, tol=1e-8, maxIter=2000, check=4, omega=None):
-        """Flexible CG with this preconditioner -> (phi, iterations, relative residual, converged)."""
+    def fcg(self, ps, q, phi=None, tol=1e-8, maxIter=2000, check=4, omega=None, mDirs=None):
+        """Flexible CG with this preconditioner, keeping mDirs directions (default self.mDirs)
+        -> (phi, iterations, relative residual, converged)."""
         if phi is None:
             phi = torch.zeros_like(q)
         it = ctypes.c_uint32(0)
         rr = ctypes.c_double(0.)
         om = self.omega if omega is None else float(omega)
-        rc = L.lib().oecl_fcg_solve(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx), L.ptr(ps.cy),
-                                    L.ptr(ps.mask), L.ptr(q), L.ptr(phi), L.f64(tol), L.u32(maxIter), L.u32(check),
-                                    L.f64(om), L.ptr(self._fcgwork), ctypes.byref(it), ctypes.byref(rr),
-                                    L.stream())
+        m = self.mDirs if mDirs is None else int(mDirs)
+        lib = L.lib()
+        need = int(lib.oecl_fcg_work_bytes(self.nx * self.ny, max(1, min(m, 8)))) // 8 + 8
+        if self._fcgwork is None or self._fcgwork.numel() < need:
+            self._fcgwork = torch.empty(need, dtype=torch.float64, device=L.device())
+        rc = lib.oecl_fcg_solve(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx), L.ptr(ps.cy),
+                                L.ptr(ps.mask), L.ptr(q), L.ptr(phi), L.f64(tol), L.u32(maxIter), L.u32(check),
+                                L.f64(om), L.u32(m), L.ptr(self._fcgwork), ctypes.byref(it), ctypes.byref(rr),
+                                L.stream())
         if rc not in (0, 4):
             L.check(rc, 'fcg solve')
         self.lastIterations, self.lastRelres = it.value, rr.value

@@ -189,6 +189,15 @@ class PoissonSolver:
         self.lastIterations = it.value
         self.lastRelRes = rr.value
 
+    def applyA(self, x, out=None):
+        """A x for the FIT operator restricted to the inside nodes (poissonSolver.pyx:55): x is
+        taken as zero on the other nodes, and the result is zero there (oecl_fit_apply)."""
+        xd = self._rhs(x)
+        y = torch.empty_like(xd) if out is None else out
+        L.check(L.lib().oecl_fit_apply(L.u32(self.nx), L.u32(self.npExt), L.ptr(self.cx), L.ptr(self.cy),
+                                       L.ptr(self.mask), L.ptr(xd), L.ptr(y), L.stream()), 'applyA')
+        return y
+
     def phiToE(self):
         """eAtGridPoints = phiToEAtGridPoints . phi (poissonSolver.pyx:87)."""
         L.check(L.lib().oecl_csr_spmv(

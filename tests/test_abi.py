@@ -62,4 +62,4 @@ def test_invalid_arguments_are_refused_before_any_device_call():
                                       None, 0, 0, None, None, None, None) == ARG
     assert lib.oecl_fcg_solve(None, ctypes.c_uint32(8), ctypes.c_uint32(8), None, None, None, None, None,
                               ctypes.c_double(1e-8), ctypes.c_uint32(10), ctypes.c_uint32(1), ctypes.c_double(0.),
-                              None, None, None, None) == ARG
+                              ctypes.c_uint32(4), None, None, None, None) == ARG

@@ -79,3 +79,30 @@ def test_pcg_warm_start_and_reproducible():
     ps.solve(q)
     assert ps.lastIterations == it_cold
     assert torch.equal(ps.getPhi(), phi1)
+
+
+@pytest.mark.parametrize('args', [(128, 64, 0.1, 0.05, 0), (96, 72, 0.09, 0.07, 1), (70, 70, 0.05, 0.05, 4)])
+def test_apply_matches_oracle_operator(args):
+    """A x on the inside nodes against the oracle's assembled FIT matrix (poissonSolver.pyx:55),
+    with x non-zero outside too (the restriction treats it as zero there); and the direct
+    solve inverts it."""
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.grid_oracle import OraclePoisson
+    if args[4] == 4:   # polygon chamber: a hexagon's first-quadrant points
+        g = Grid(*args, boundFunc=np.array([[0.024, 0.], [0.012, 0.0208], [0., 0.0208]]))
+    else:
+        g = Grid(*args)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    ref = OraclePoisson(g)
+    ind = ref.insidePointsInd
+    x = np.random.default_rng(3).standard_normal(g.getNpExt())
+    y = ps.applyA(torch.as_tensor(x).cuda()).cpu().numpy()
+    y_ref = np.zeros_like(x)
+    y_ref[ind] = ref.a.dot(x[ind])
+    assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
+    assert not np.any(np.delete(y, ind))
+    xin = np.zeros_like(x)
+    xin[ind] = x[ind]
+    phi = ps.solvePhi(torch.as_tensor(y).cuda()).cpu().numpy()
+    assert np.linalg.norm(phi - xin) <= 1e-9 * np.linalg.norm(xin)

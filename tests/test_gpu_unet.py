@@ -111,6 +111,7 @@ def test_training_helps_flexible_cg_against_jacobi():
         pj.solvePhi(q)
         rows.append((it, rr, ok, err, pj.lastIterations, b0))
     info = (l0, l1, hist, rows)
+    print('cloud and noise pairs, 64^2 (iterations, relres, converged, err, jacobi, untrained):', rows, l0, l1)
     assert l1 < 0.7 * l0, info
     for it, rr, ok, err, jac, b0 in rows:
         assert ok and rr <= 1e-8, info
@@ -129,3 +130,37 @@ def test_learned_solve_falls_back_to_the_direct_solve():
     out = net.solve(ps, q, tol=1e-12, maxIter=1)
     assert net.fallbacks == 1
     assert float(torch.linalg.norm(out - phi) / torch.linalg.norm(phi)) < 1e-12
+
+
+def test_operator_pairs_train_a_stronger_preconditioner():
+    """Trained on pairs (A phi, phi) with phi random at length scales from 0 to 64 cells, the
+    network cuts the flexible CG's iterations (8 kept directions) on cloud-like right-hand sides
+    below a quarter of Jacobi PCG's on a 128 x 128 chamber (measured: 36 against 410-430), and
+    below the untrained network's (300).  With one kept direction the same network needs about
+    250: the ReLU network varies between applications, which the truncated recurrence
+    tolerates poorly."""
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = _chamber(128)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    pj = PoissonSolver(g, method='pcg', tol=1e-8, maxIter=20000)
+    net = UNetPreconditioner(g, seed=0)
+    raw = net.operator_pairs(ps, 64, seed=1)
+    q0, phi0 = raw[3]
+    assert float(torch.linalg.norm(ps.solvePhi(q0) - phi0)) < 1e-9   # unit-norm phi, A phi = q
+    test = [p for k, p in enumerate(_pairs(g, ps, 6, seed=2)) if k % 2 == 0]
+    base = [net.fcg(ps, q, tol=1e-8, maxIter=4000)[1] for q, _ in test]
+    hist = net.train(ps, net.smoothed_pairs(ps, raw), steps=1000, lr=5e-4, lrEnd=5e-5, batch=4, logEvery=250)
+    rows = []
+    for (q, phi), b0 in zip(test, base):
+        z, it, rr, ok = net.fcg(ps, q, tol=1e-8, maxIter=4000)
+        err = float(torch.linalg.norm(z - phi) / torch.linalg.norm(phi))
+        pj.solvePhi(q)
+        rows.append((it, rr, ok, err, pj.lastIterations, b0))
+    print('operator pairs, 128^2 (iterations, relres, converged, err, jacobi, untrained):', rows, hist)
+    info = (hist, rows)
+    for it, rr, ok, err, jac, b0 in rows:
+        assert ok and rr <= 1e-8, info
+        assert err < 1e-6, info
+        assert it < 0.25 * jac, info
+        assert it < b0, info
