@@ -51,7 +51,7 @@ def parse():
                     help='replay the steps between two sorts as one captured HIP graph')
     ap.add_argument('--sort-every', type=int, default=50,
                     help='re-order particles by cell every this many steps (0: never)')
-    ap.add_argument('--train-steps', type=int, default=3000, help='learned: Adam steps')
+    ap.add_argument('--train-steps', type=int, default=8000, help='learned: Adam steps')
     ap.add_argument('--pairs', type=int, default=192, help='learned: training pairs')
     ap.add_argument('--train-pairs', choices=('operator', 'buildup'), default='operator',
                     help='learned: train on (A phi, phi) with phi random at 0-64 cell scales, or on the '
@@ -298,10 +298,10 @@ def main_learned(a):
     homoLoader(g, p, pb, npart, 1e9 / npart, 4.194e5)
     p.reserve(3 * npart)
     b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
-    ntest = 4
+    ntest = 4   # test pairs of each kind; one more of each is the validation set
     fromBuildup = a.train_pairs == 'buildup'
-    ncl = (a.pairs // 2 if fromBuildup else 0) + ntest
-    nwh = (a.pairs - a.pairs // 2 if fromBuildup else 0) + ntest
+    ncl = (a.pairs // 2 if fromBuildup else 0) + ntest + 1
+    nwh = (a.pairs - a.pairs // 2 if fromBuildup else 0) + ntest + 1
     b.native_setup(absCap=65536, secCap=131072, maxSteps=50 * ncl + 10)
     clouds, noise = [], []
     t0 = time.perf_counter()
@@ -318,8 +318,9 @@ def main_learned(a):
         q = torch.as_tensor(rng.standard_normal(g.getNpExt())).cuda() * mask * 1e-9
         noise.append((q, ps.solvePhi(q).clone()))
     test = clouds[-ntest:] + noise[-ntest:]
+    val = [clouds[-ntest - 1][0], noise[-ntest - 1][0]]
     net = UNetPreconditioner(g, seed=0)
-    raw = clouds[:-ntest] + noise[:-ntest] if fromBuildup else net.operator_pairs(ps, a.pairs, seed=1)
+    raw = (clouds[:-ntest - 1] + noise[:-ntest - 1]) if fromBuildup else net.operator_pairs(ps, a.pairs, seed=1)
     pairs = raw
     log('%d training pairs (%s), %d test pairs: %.1f s' % (len(raw), a.train_pairs, len(test),
                                                           time.perf_counter() - t0))
@@ -330,7 +331,8 @@ def main_learned(a):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = net.train(ps, train, steps=a.train_steps, lr=5e-4, lrEnd=5e-5, batch=4,
-                     logEvery=max(1, a.train_steps // 10), log=log)
+                     logEvery=max(1, a.train_steps // 10), log=log, validate=val,
+                     validateEvery=max(1, a.train_steps // 6), validateIter=4000)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     # U-Net apply time (the preconditioner's cost per iteration)
@@ -391,14 +393,15 @@ def main_learned(a):
         'vs_baseline': None,
         'dtype': 'bf16 network (FP32 accumulation), FP64 CG, damped Jacobi (omega 2/3) around the network',
         'data': 'synthetic: test pairs from a seeded buildup on this grid (4) and white-noise charges (4); '
-                'training pairs ' + ('(A phi, phi), phi random at 0-64 cell scales' if not fromBuildup else
-                                     'from the same buildup and white-noise charges'),
+                'validation right-hand sides: one more of each; training pairs ' +
+                ('(A phi, phi), phi random at 0-64 cell scales' if not fromBuildup else
+                 'from the same buildup and white-noise charges'),
         'config': {'workload': '%dx%d cut-cell elliptical chamber 45x35 mm with a conductor of radius 2.5 mm at '
                                'x = 20 mm; U-Net 4 levels, 16 channels' % (n, n),
                    'grid': [n, n], 'inside_nodes': int(ps.mask.sum().item()),
                    'capacitance_rows': ps.capacitanceRows, 'pairs': len(pairs), 'train_pairs': a.train_pairs,
                    'fcg_directions': net.mDirs, 'smoothing_omega': net.omega,
-                   'train_steps': a.train_steps,
+                   'train_steps': a.train_steps, 'validation_iterations': net.validation,
                    'train_seconds': t_train, 'loss_history': hist},
         'solvers': summ,
         'unet_apply_ms': t_apply * 1e3,

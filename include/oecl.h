@@ -369,6 +369,9 @@ int oecl_fastpoisson_destroy(oecl_fastpoisson *h);
 typedef struct oecl_unet oecl_unet;
 int oecl_unet_create(uint32_t H, uint32_t W, uint32_t seed, oecl_unet **out);
 int oecl_unet_destroy(oecl_unet *u);
+/* Output rows per convolution block: 1, 2, 4 or 8, or 0 (the default) to choose per image size
+ * (the tallest tile that leaves at least 1024 blocks).  The result does not depend on it. */
+int oecl_unet_set_conv_rows(oecl_unet *u, int rows);
 /* Host arrays in / out (any may be NULL in get). */
 int oecl_unet_set_params(oecl_unet *u, const float *W, const float *b);
 int oecl_unet_get_params(const oecl_unet *u, float *W, float *b, float *gW, float *gb);

@@ -96,3 +96,18 @@ struct Soa {
         return base[(uint64_t)c * stride + i];
     }
 };
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8), each XCD with its
+// own L2.  xcd_block maps hardware block b to a logical block so that every XCD works through
+// whole chunks of XCD_CHUNK consecutive logical blocks (chunk c on XCD c % 8): neighbouring
+// logical blocks then share their XCD's L2, while the chunks of every XCD still spread over the
+// whole grid (any live prefix keeps all XCDs busy).  Blocks past the last whole group of 8
+// chunks keep their index.  A bijection on [0, n).
+constexpr uint32_t XCD_CHUNK = 32;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
+{
+    const uint32_t full = n / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
+    if (b >= full) return b;
+    const uint32_t x = b & 7u, k = b >> 3;
+    return ((k / XCD_CHUNK) * 8 + x) * XCD_CHUNK + k % XCD_CHUNK;
+}

@@ -123,21 +123,9 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
     else box.b[(ix - 2) + (iy - 2) * box.bx] = box.mask[i] ? qv : 0.;
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8), each XCD with its
-// own L2.  The push maps hardware block b to logical block xcd_block(b) so that every XCD works
-// through whole chunks of XCD_CHUNK consecutive blocks (chunk c on XCD c % 8): a chunk's
-// particles share a few rows of cells after a cell sort, so its XCD's L2 holds that part of E
-// and rho, while the chunks of every XCD still spread over the whole block (any live prefix
-// keeps all XCDs busy).  Blocks past the last whole group of 8 chunks keep their index.  A
-// bijection on [0, n).
-constexpr uint32_t XCD_CHUNK = 32;
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
-{
-    const uint32_t full = n / (8 * XCD_CHUNK) * (8 * XCD_CHUNK);
-    if (b >= full) return b;
-    const uint32_t x = b & 7u, k = b >> 3;
-    return ((k / XCD_CHUNK) * 8 + x) * XCD_CHUNK + k % XCD_CHUNK;
-}
+// The push maps hardware block b to logical block xcd_block(b, n) (common.h): every XCD works
+// through whole chunks of consecutive blocks, whose particles share a few rows of cells after a
+// cell sort, so that XCD's L2 holds that part of E and rho.
 
 // Holes (absorbed particles) of one wave in index order: the wave's count goes to
 // counts[w], the indices to list[w * 64 + rank] (w = global wave index of logical block blk),

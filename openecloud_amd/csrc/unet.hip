@@ -20,6 +20,7 @@
 // residual scaled to unit RMS, so the preconditioner is homogeneous of degree one.
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -32,6 +33,7 @@ struct oecl_unet {
     float *Wm, *bm;           // FP32 masters [14][9][16][16], [14][16]
     float *mW, *vW, *mb, *vb; // Adam moments
     float *gW, *gb;           // gradients
+    float *wpart;             // per-block partial weight gradients of one convolution
     uint16_t *Wb, *WbT;       // bf16 weights, and flipped/transposed for the data gradient
     uint16_t *act[32];        // activations (see the layout in unet_layout)
     uint16_t *grad[8];        // gradient scratch images
@@ -41,6 +43,7 @@ struct oecl_unet {
     double *sv[4];            // grid vectors (H W doubles each) of the smoothed preconditioner
     char *mem;
     uint32_t adamT;
+    int convRows;             // output rows per convolution wave: 0 = chosen per image size
 };
 
 namespace {
@@ -64,23 +67,81 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 __device__ __forceinline__ v4s load4(const uint16_t *p) { return *reinterpret_cast<const v4s *>(p); }
 
 // out = conv3x3(in) + bias over an H x W image.  MODE 0: bf16 out with ReLU, 1: bf16 out
-// without ReLU, 2: FP32 channel 0 only.  A wave computes RW rows x 16 pixels: it walks the RW + 2
-// input rows once, loads each row's three horizontally shifted operands, and feeds every output
-// row that uses that input row (the taps of each output keep the order 0..8).
+// without ReLU, 2: FP32 channel 0 only.  A block computes RW rows x 64 pixels (its 4 waves take
+// 16 pixels each).  It first stages the RW + 2 input rows of its 66-pixel span (1 pixel of halo
+// each side, zero outside the image) in LDS with coalesced 16-byte loads, all issued before the
+// first use, so each wave keeps many rows in flight; a wave then walks the staged rows once,
+// reads each row's three horizontally shifted operands from LDS, and feeds every output row
+// that uses that input row (the taps of each output keep the order 0..8).  Tall tiles (RW 8)
+// re-read fewer halo rows on large images; short ones (RW 1, 2) give small images more blocks.
+// Blocks are numbered row group by row group and taken in XCD chunks (xcd_block), so a chunk's
+// vertically adjacent tiles share their halo rows in one L2.
 // MFMA operand layouts (16x16x16): lane l holds A[l % 16][4 (l / 16) + j], B[4 (l / 16) + j][l % 16]
 // and D[4 (l / 16) + j][l % 16], j = 0..3; here A = weights (out x in), B = input (in x pixel).
-constexpr int CONV_RW = 4;
-template <int MODE>
+// Grid vectors of the preconditioner (nx x ny, row-major; image pixel (y, x) is grid node
+// y nx + x for x < nx, y < ny).  sc[0] = sum of r^2 over inside nodes, sc[1] = their count.
+struct GridIO {
+    const double *r;      // network input: channel 0 = r / rms on inside nodes, 0 elsewhere
+    double *z;            // network output: z = rms * y on inside nodes, 0 elsewhere
+    const uint8_t *mask;  // inside nodes
+    const double *sc;
+    uint32_t nx, ny;
+};
+
+__device__ __forceinline__ double grid_rms(const double *sc) { return sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.; }
+
+// SRC 1 (the first convolution of an inference pass) stages its input straight from the FP64
+// grid vector g.r instead of an input image; MODE 3 (its last convolution) writes g.z.  Both give
+// bit for bit what the input-image and FP32-output passes give.
+constexpr int CONV_SPAN = 64 + 2;  // staged pixels per row
+template <int MODE, int RW, int SRC = 0>
 __global__ void __launch_bounds__(kThreads)
     k_conv(int H, int W, const uint16_t *__restrict__ in, const uint16_t *__restrict__ wt,
-           const float *__restrict__ bias, void *__restrict__ out)
+           const float *__restrict__ bias, void *__restrict__ out, GridIO g)
 {
-    constexpr int RW = CONV_RW;
-    const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4;
-    const int segs = (W + 15) >> 4, groups = (H + RW - 1) / RW;
-    const int wave = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-    if (wave >= groups * segs) return;  // whole wave
-    const int y0 = (wave / segs) * RW, x = ((wave % segs) << 4) + px;
+    __shared__ __align__(16) uint16_t tile[RW + 2][CONV_SPAN][C];
+    const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4, wv = threadIdx.x >> 6;
+    const int quads = (W + 63) >> 6;
+    const int blk = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int y0 = (blk / quads) * RW, xq = (blk % quads) << 6;  // grid = groups * quads exactly
+    // stage rows y0 - 1 .. y0 + RW, pixels xq - 1 .. xq + 64 (two 16-byte halves per pixel):
+    // every load of the thread is issued before the first LDS store
+    constexpr int NE = (RW + 2) * CONV_SPAN * 2, NL = (NE + kThreads - 1) / kThreads;
+    uint4 v[NL];
+    double rv[SRC ? NL : 1];
+    uint8_t mv[SRC ? NL : 1];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int e = threadIdx.x + k * kThreads;
+        const int row = e / (CONV_SPAN * 2), rem = e - row * (CONV_SPAN * 2), sp = rem >> 1, half = rem & 1;
+        const int gy = y0 - 1 + row, gx = xq - 1 + sp;
+        v[k] = make_uint4(0u, 0u, 0u, 0u);
+        if (SRC == 0) {
+            if (e < NE && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v[k] = *reinterpret_cast<const uint4 *>(in + ((size_t)gy * W + gx) * C + 8 * half);
+        } else {
+            // both loads issued unconditionally for in-range nodes; the mask selects afterwards
+            const bool ok = e < NE && half == 0 && gy >= 0 && gy < (int)g.ny && gx >= 0 && gx < (int)g.nx;
+            const uint32_t i = ok ? (uint32_t)gy * g.nx + (uint32_t)gx : 0u;
+            rv[k] = ok ? g.r[i] : 0.;
+            mv[k] = ok ? g.mask[i] : (uint8_t)0;
+        }
+    }
+    if (SRC == 1) {
+        const double rms = grid_rms(g.sc);
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+            if (mv[k]) v[k].x = f2bf((float)(rv[k] / rms));  // channel 0 in the low half
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int e = threadIdx.x + k * kThreads;
+        const int row = e / (CONV_SPAN * 2), rem = e - row * (CONV_SPAN * 2), sp = rem >> 1, half = rem & 1;
+        if (e < NE) *reinterpret_cast<uint4 *>(&tile[row][sp][8 * half]) = v[k];
+    }
+    __syncthreads();
+    const int x = xq + (wv << 4) + px;
+    if (xq + (wv << 4) >= W) return;  // whole wave, after the block's only barrier
     v4s a[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) a[k] = load4(wt + (k * C + px) * C + 4 * cq);
@@ -89,17 +150,12 @@ __global__ void __launch_bounds__(kThreads)
     for (int r = 0; r < RW; ++r) acc[r] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int iy = 0; iy < RW + 2; ++iy) {
-        const int yy = y0 - 1 + iy;
         v4s b[3];
 #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-            const int xx = x + dx - 1;
-            b[dx] = v4s{0, 0, 0, 0};
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W) b[dx] = load4(in + ((size_t)yy * W + xx) * C + 4 * cq);
-        }
+        for (int dx = 0; dx < 3; ++dx) b[dx] = *reinterpret_cast<const v4s *>(&tile[iy][(wv << 4) + px + dx][4 * cq]);
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
-            const int ky = iy - r;  // tap row of output row y0 + r that reads input row yy
+            const int ky = iy - r;  // tap row of output row y0 + r that reads input row y0 - 1 + iy
             if (ky >= 0 && ky <= 2)
 #pragma unroll
                 for (int dx = 0; dx < 3; ++dx)
@@ -107,6 +163,19 @@ __global__ void __launch_bounds__(kThreads)
         }
     }
     if (x >= W) return;
+    if (MODE == 3) {
+        if (cq != 0 || x >= (int)g.nx) return;
+        const double rms = grid_rms(g.sc);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int y = y0 + r;
+            if (y >= (int)g.ny) break;
+            const uint32_t i = (uint32_t)y * g.nx + (uint32_t)x;
+            const float yv = acc[r][0] + bias[0];
+            g.z[i] = g.mask[i] ? rms * (double)yv : 0.;
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
         const int y = y0 + r;
@@ -129,15 +198,17 @@ __global__ void __launch_bounds__(kThreads)
 
 // Weight and bias gradients of one convolution: gW[k][o][i] += sum_p g[p][o] x[p + off_k][i],
 // gb[o] += sum_p g[p][o].  MFMA with M = out channels, N = in channels, K = 16 pixels of a row
-// segment; waves stride over the segments, a block sums its waves in LDS and adds to gW / gb
-// with one atomic per entry.
+// segment; waves stride over the segments.  The sums are formed in a fixed order, so training is
+// reproducible: each wave stores its sums in its own LDS row, the block adds its 4 rows in order
+// and stores the block's partial sums (WG per block) in wpart; k_wgrad_sum adds the blocks.
+constexpr int WG = WSZ + C;            // weight and bias entries of one convolution
+constexpr uint32_t kWgradBlocks = 1024;
 __global__ void __launch_bounds__(kThreads)
     k_conv_wgrad(int H, int W, const uint16_t *__restrict__ x, const uint16_t *__restrict__ g,
-                 float *__restrict__ gW, float *__restrict__ gb)
+                 float *__restrict__ wpart)
 {
-    __shared__ float red[WSZ + C];
-    for (int t = threadIdx.x; t < WSZ + C; t += kThreads) red[t] = 0.f;
-    __syncthreads();
+    __shared__ float red[kThreads / 64][WSZ];
+    __shared__ float bl[kThreads / 64][64];
     const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4;
     const int segs = (W + 15) >> 4, nSeg = H * segs;
     const int wave = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
@@ -170,15 +241,43 @@ __global__ void __launch_bounds__(kThreads)
             acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[k], 0, 0, 0);
         }
     }
-    // D[o = 4 cq + j][i = px]
+    // D[o = 4 cq + j][i = px]: every entry of the wave's row is written once
+    const int wv = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < 9; ++k)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) atomicAdd(&red[(k * C + 4 * cq + j) * C + px], acc[k][j]);
-    atomicAdd(&red[WSZ + px], bs);
+        for (int j = 0; j < 4; ++j) red[wv][(k * C + 4 * cq + j) * C + px] = acc[k][j];
+    bl[wv][lane] = bs;  // channel px, pixels 4 cq .. 4 cq + 3 of each segment
     __syncthreads();
-    for (int t = threadIdx.x; t < WSZ; t += kThreads) atomicAdd(gW + t, red[t]);
-    if (threadIdx.x < C) atomicAdd(gb + threadIdx.x, red[WSZ + threadIdx.x]);
+    float *out = wpart + (size_t)blockIdx.x * WG;
+    for (int t = threadIdx.x; t < WSZ; t += kThreads) out[t] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
+    if (threadIdx.x < C) {
+        float sb = 0.f;
+        for (int w = 0; w < kThreads / 64; ++w)
+            for (int q = 0; q < 4; ++q) sb += bl[w][threadIdx.x + 16 * q];
+        out[WSZ + threadIdx.x] = sb;
+    }
+}
+
+// gW[e] += sum over the nb blocks of wpart[b][e] (e < WSZ; gb for the last C entries), in a fixed
+// order: 8 stripes each sum blocks s, s + 8, ... in turn, then the stripes are added in order.
+__global__ void __launch_bounds__(256)
+    k_wgrad_sum(uint32_t nb, const float *__restrict__ wpart, float *__restrict__ gW, float *__restrict__ gb)
+{
+    __shared__ float sh[8][32];
+    const int el = threadIdx.x & 31, st = threadIdx.x >> 5;
+    const int e = blockIdx.x * 32 + el;
+    float s = 0.f;
+    if (e < WG)
+        for (uint32_t b = st; b < nb; b += 8) s += wpart[(size_t)b * WG + e];
+    sh[st][el] = s;
+    __syncthreads();
+    if (threadIdx.x < 32 && e < WG) {
+        float t = 0.f;
+        for (int k = 0; k < 8; ++k) t += sh[k][el];
+        if (e < WSZ) gW[e] += t;
+        else gb[e - WSZ] += t;
+    }
 }
 
 // g *= (act > 0): the ReLU of a layer applied to the gradient of its output (8-byte groups).
@@ -320,15 +419,19 @@ __global__ void __launch_bounds__(kThreads) k_finish(const double *__restrict__ 
         for (int k = 0; k < K; ++k) s[off + k] = v[k];
 }
 
+// The grid-vector reductions below run on at most kRedBlocks blocks (grid-stride loops), so
+// that k_finish sums few partials.
+constexpr uint32_t kRedBlocks = 1024;
+
 // Partial sums of r^2 and of the inside-node count (for the RMS scale of the network input).
 __global__ void __launch_bounds__(kThreads) k_rms_part(uint32_t np, const uint8_t *__restrict__ mask, const double *__restrict__ r, double *__restrict__ part)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0., 0.};
-    if (i < np && mask[i]) {
-        v[0] = r[i] * r[i];
-        v[1] = 1.;
-    }
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads)
+        if (mask[i]) {
+            v[0] += r[i] * r[i];
+            v[1] += 1.;
+        }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
         part[2 * blockIdx.x] = v[0];
@@ -353,16 +456,6 @@ __global__ void k_net_input(uint32_t nx, uint32_t ny, int H, int W, const uint8_
         if (mask[i]) o[0] = (short)f2bf((float)(r[i] / rms));
     }
     *reinterpret_cast<v4s *>(img + p * C + 4 * q) = o;
-}
-
-// z = rms * y on inside nodes, 0 elsewhere.
-__global__ void k_net_output(uint32_t nx, uint32_t np, int W, const uint8_t *__restrict__ mask,
-                             const float *__restrict__ y, const double *__restrict__ sc, double *__restrict__ z)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const double rms = sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.;
-    z[i] = mask[i] ? rms * (double)y[(size_t)(i / nx) * W + i % nx] : 0.;
 }
 
 __device__ __forceinline__ double apply_a(const double *__restrict__ cx, const double *__restrict__ cy,
@@ -462,20 +555,50 @@ __global__ void k_resid_az(uint32_t nx, uint32_t np, const double *__restrict__ 
     if (e) e[i] = in ? phi[i] - z[i] : 0.;
 }
 
+// The pre-smoothing of the preconditioner in one pass: z1 = omega D^-1 r and r1 = r - A z1 on
+// inside nodes (0 elsewhere; z1 at the neighbours is recomputed, giving k_jacobi and
+// k_resid_az's values bit for bit); partials (r1.r1, inside count) for the network's input scale.
+// Inside nodes lie off the grid's outer ring, so every neighbour index is in range, and a
+// neighbour's own diagonal is only formed when it is inside.
+__global__ void __launch_bounds__(kThreads)
+    k_smooth1(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
+              const uint8_t *__restrict__ mask, const double *__restrict__ r, double omega, double *__restrict__ z1,
+              double *__restrict__ r1, double *__restrict__ part)
+{
+    double v[2] = {0., 0.};
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
+        double zi = 0., ri = 0.;
+        if (mask[i]) {
+            auto zj = [&](uint32_t j) { return mask[j] ? omega * r[j] / diag_a(cx, cy, j, nx) : 0.; };
+            zi = zj(i);
+            const double ze = zj(i + 1), zw = zj(i - 1), zn = zj(i + nx), zs = zj(i - nx);
+            ri = r[i] - (cx[i] * (zi - ze) - cx[i - 1] * (zw - zi) + cy[i] * (zi - zn) - cy[i - nx] * (zs - zi));
+            v[0] += ri * ri;
+            v[1] += 1.;
+        }
+        z1[i] = zi;
+        r1[i] = ri;
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = v[0];
+        part[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
 // an = A n, partials (n.r1, n.an)
 __global__ void __launch_bounds__(kThreads)
     k_an_dots(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
               const uint8_t *__restrict__ mask, const double *__restrict__ n, const double *__restrict__ r1,
               double *__restrict__ an, double *__restrict__ part)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0., 0.};
-    if (i < np) {
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         double a = 0.;
         if (mask[i]) {
             a = apply_a(cx, cy, n, i, nx);
-            v[0] = n[i] * r1[i];
-            v[1] = n[i] * a;
+            v[0] += n[i] * r1[i];
+            v[1] += n[i] * a;
         }
         an[i] = a;
     }
@@ -516,14 +639,13 @@ __global__ void __launch_bounds__(kThreads)
                const uint8_t *__restrict__ mask, const double *__restrict__ q, double *__restrict__ phi,
                double *__restrict__ r, double *__restrict__ part)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0., 0.};
-    if (i < np) {
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         double ri = 0.;
         if (mask[i]) {
             ri = q[i] - apply_a(cx, cy, phi, i, nx);
-            v[0] = ri * ri;
-            v[1] = q[i] * q[i];
+            v[0] += ri * ri;
+            v[1] += q[i] * q[i];
         }
         r[i] = ri;
     }
@@ -545,15 +667,14 @@ __global__ void __launch_bounds__(kThreads)
     k_fcg_zdots(uint32_t np, const double *__restrict__ z, const double *__restrict__ AP, uint32_t k, uint32_t m1,
                 uint32_t nUsed, double *__restrict__ part)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     double v[FCG_MAXM];
 #pragma unroll
     for (uint32_t t = 0; t < FCG_MAXM; ++t) v[t] = 0.;
-    if (i < np) {
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         const double zi = z[i];
 #pragma unroll
         for (uint32_t t = 0; t < FCG_MAXM; ++t)
-            if (t < nUsed) v[t] = zi * AP[(size_t)((k - 1 - t) % m1) * np + i];
+            if (t < nUsed) v[t] += zi * AP[(size_t)((k - 1 - t) % m1) * np + i];
     }
     block_sum<FCG_MAXM>(v);
     if (threadIdx.x == 0)
@@ -584,14 +705,13 @@ __global__ void __launch_bounds__(kThreads)
              const uint8_t *__restrict__ mask, const double *__restrict__ p, const double *__restrict__ r,
              double *__restrict__ ap, double *__restrict__ part)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0., 0.};
-    if (i < np) {
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         double a = 0.;
         if (mask[i]) {
             a = apply_a(cx, cy, p, i, nx);
-            v[0] = p[i] * a;
-            v[1] = p[i] * r[i];
+            v[0] += p[i] * a;
+            v[1] += p[i] * r[i];
         }
         ap[i] = a;
     }
@@ -614,15 +734,14 @@ __global__ void __launch_bounds__(kThreads)
     k_fcg_update(uint32_t np, const double *__restrict__ s, double *__restrict__ phi, double *__restrict__ r,
                  const double *__restrict__ p, const double *__restrict__ ap, double *__restrict__ part)
 {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     double v[2] = {0., 0.};
-    if (i < np) {
-        const double a = s[3];
+    const double a = s[3];
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         const double ri = r[i] - a * ap[i];
         phi[i] += a * p[i];
         r[i] = ri;
-        v[0] = ri * ri;
-        v[1] = 1.;
+        v[0] += ri * ri;
+        v[1] += 1.;
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
@@ -640,15 +759,40 @@ enum { IX = 0, IA = 1, IS = 2, IP = 3, IT = 4, IU = 5, IV = 6 };
 
 size_t img_elems(const oecl_unet *u, int l) { return (size_t)(u->H >> l) * (u->W >> l) * C; }
 
+// Rows per convolution block: the tallest tile that still leaves >= 1024 blocks (4 per CU).
+int conv_rows(const oecl_unet *u, int H, int W)
+{
+    if (u->convRows) return u->convRows;
+    const long quads = (W + 63) / 64;
+    for (int rw = 8; rw > 1; rw >>= 1)
+        if ((H + rw - 1) / rw * quads >= 1024) return rw;
+    return 1;
+}
+
+template <int MODE, int SRC = 0>
+void launch_conv_mode(int rw, dim3 g, hipStream_t s, int H, int W, const uint16_t *in, const uint16_t *w,
+                      const float *b, void *out, const GridIO &io)
+{
+    const dim3 bl(kThreads);
+    if (rw == 8) hipLaunchKernelGGL((k_conv<MODE, 8, SRC>), g, bl, 0, s, H, W, in, w, b, out, io);
+    else if (rw == 4) hipLaunchKernelGGL((k_conv<MODE, 4, SRC>), g, bl, 0, s, H, W, in, w, b, out, io);
+    else if (rw == 2) hipLaunchKernelGGL((k_conv<MODE, 2, SRC>), g, bl, 0, s, H, W, in, w, b, out, io);
+    else hipLaunchKernelGGL((k_conv<MODE, 1, SRC>), g, bl, 0, s, H, W, in, w, b, out, io);
+}
+
+// mode 0: ReLU, 1: linear, 2: FP32 channel 0; 3: the same into io.z; 4: ReLU with the input
+// staged from io.r (both on level 0 only).
 int launch_conv(const oecl_unet *u, int l, int mode, const uint16_t *in, const uint16_t *w, const float *b, void *out,
-                hipStream_t s)
+                hipStream_t s, const GridIO &io = GridIO{})
 {
     const int H = u->H >> l, W = u->W >> l;
-    const uint32_t waves = (uint32_t)((H + CONV_RW - 1) / CONV_RW) * ((W + 15) / 16);
-    const dim3 g(nblk(waves, kThreads / 64)), bl(kThreads);
-    if (mode == 0) hipLaunchKernelGGL(k_conv<0>, g, bl, 0, s, H, W, in, w, b, out);
-    else if (mode == 1) hipLaunchKernelGGL(k_conv<1>, g, bl, 0, s, H, W, in, w, b, out);
-    else hipLaunchKernelGGL(k_conv<2>, g, bl, 0, s, H, W, in, w, b, out);
+    const int rw = conv_rows(u, H, W);
+    const dim3 g((uint32_t)((H + rw - 1) / rw) * ((W + 63) / 64));  // one block per RW x 64 tile
+    if (mode == 0) launch_conv_mode<0>(rw, g, s, H, W, in, w, b, out, io);
+    else if (mode == 1) launch_conv_mode<1>(rw, g, s, H, W, in, w, b, out, io);
+    else if (mode == 2) launch_conv_mode<2>(rw, g, s, H, W, in, w, b, out, io);
+    else if (mode == 3) launch_conv_mode<3>(rw, g, s, H, W, in, w, b, out, io);
+    else launch_conv_mode<0, 1>(rw, g, s, H, W, in, w, b, out, io);
     return OECL_OK;
 }
 
@@ -657,7 +801,9 @@ const uint16_t *WTc(const oecl_unet *u, int c) { return u->WbT + (size_t)c * WSZ
 const float *bc(const oecl_unet *u, int c) { return u->bm + (size_t)c * C; }
 
 // Forward pass from the input image act[IX] (level 0) to u->out.
-void unet_forward(oecl_unet *u, hipStream_t s)
+// io != null: an inference pass from the grid vector io->r to io->z (no input image, no FP32
+// output image; the activations are still kept).
+void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr)
 {
     uint16_t **A = u->act;
     auto pool = [&](int l, const uint16_t *in, uint16_t *out) {  // level l-1 -> l
@@ -668,7 +814,8 @@ void unet_forward(oecl_unet *u, hipStream_t s)
         const int H = u->H >> l, W = u->W >> l;
         hipLaunchKernelGGL(k_up_add<false>, dim3(nblk((size_t)H * W * 4)), dim3(kThreads), 0, s, H, W, uu, sk, out);
     };
-    launch_conv(u, 0, 0, A[IX], Wc(u, 0), bc(u, 0), A[IA], s);
+    if (io) launch_conv(u, 0, 4, nullptr, Wc(u, 0), bc(u, 0), A[IA], s, *io);
+    else launch_conv(u, 0, 0, A[IX], Wc(u, 0), bc(u, 0), A[IA], s);
     launch_conv(u, 0, 0, A[IA], Wc(u, 1), bc(u, 1), A[IS], s);
     for (int l = 1; l <= 3; ++l) {
         pool(l, A[7 * (l - 1) + IS], A[7 * l + IP]);
@@ -682,6 +829,7 @@ void unet_forward(oecl_unet *u, hipStream_t s)
         const int c0 = 8 + 2 * (2 - l);
         launch_conv(u, l, 0, A[7 * l + IT], Wc(u, c0), bc(u, c0), A[7 * l + IU], s);
         if (l > 0) launch_conv(u, l, 0, A[7 * l + IU], Wc(u, c0 + 1), bc(u, c0 + 1), A[7 * l + IV], s);
+        else if (io) launch_conv(u, 0, 3, A[IU], Wc(u, 13), bc(u, 13), nullptr, s, *io);
         else launch_conv(u, 0, 2, A[IU], Wc(u, 13), bc(u, 13), u->out, s);
     }
 }
@@ -707,10 +855,10 @@ void unet_backward(oecl_unet *u, const uint16_t *gout, uint16_t *gin, hipStream_
     auto convb = [&](int l, int c, const uint16_t *x, const uint16_t *g, uint16_t *gx) {
         const int H = u->H >> l, W = u->W >> l;
         const uint32_t segs = (uint32_t)H * ((W + 15) / 16);
-        uint32_t blocks = nblk(segs, kThreads / 64);
-        if (blocks > 1024) blocks = 1024;
-        hipLaunchKernelGGL(k_conv_wgrad, dim3(blocks), dim3(kThreads), 0, s, H, W, x, g, u->gW + (size_t)c * WSZ,
-                           u->gb + (size_t)c * C);
+        const uint32_t blocks = std::min(nblk(segs, kThreads / 64), kWgradBlocks);
+        hipLaunchKernelGGL(k_conv_wgrad, dim3(blocks), dim3(kThreads), 0, s, H, W, x, g, u->wpart);
+        hipLaunchKernelGGL(k_wgrad_sum, dim3((WG + 31) / 32), dim3(256), 0, s, blocks, u->wpart,
+                           u->gW + (size_t)c * WSZ, u->gb + (size_t)c * C);
         if (gx) launch_conv(u, l, 1, g, WTc(u, c), u->bm + (size_t)NCONV * C /* zero bias row */, gx, s);
     };
     auto up_back = [&](int l, const uint16_t *big, uint16_t *small) {  // level l-1 -> level l
@@ -791,6 +939,7 @@ OECL_EXPORT int oecl_unet_create(uint32_t H, uint32_t W, uint32_t seed, oecl_une
         u->vb = (float *)take(4 * nb);
         u->gW = (float *)take(4 * nW);
         u->gb = (float *)take(4 * nb);
+        u->wpart = (float *)take(4ull * kWgradBlocks * WG);
         u->Wb = (uint16_t *)take(2 * nW);
         u->WbT = (uint16_t *)take(2 * nW);
         for (int l = 0; l < 4; ++l)
@@ -840,6 +989,13 @@ OECL_EXPORT int oecl_unet_destroy(oecl_unet *u)
     if (!u) return OECL_OK;
     (void)hipFree(u->mem);
     delete u;
+    return OECL_OK;
+}
+
+OECL_EXPORT int oecl_unet_set_conv_rows(oecl_unet *u, int rows)
+{
+    if (!u || !(rows == 0 || rows == 1 || rows == 2 || rows == 4 || rows == 8)) return OECL_ERR_ARG;
+    u->convRows = rows;
     return OECL_OK;
 }
 
@@ -915,13 +1071,11 @@ OECL_EXPORT int oecl_unet_adam(oecl_unet *u, float lr, float beta1, float beta2,
 static int precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r, double *z,
                         hipStream_t s)
 {
-    const uint32_t np = nx * ny, nb = nblk(np);
-    hipLaunchKernelGGL(k_rms_part, dim3(nb), dim3(kThreads), 0, s, np, mask, r, u->part);
-    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nb, u->scal, 0);
-    hipLaunchKernelGGL(k_net_input, dim3(nblk((size_t)u->H * u->W * 4)), dim3(kThreads), 0, s, nx, ny, (int)u->H,
-                       (int)u->W, mask, r, u->scal, u->act[IX]);
-    unet_forward(u, s);
-    hipLaunchKernelGGL(k_net_output, dim3(nb), dim3(kThreads), 0, s, nx, np, (int)u->W, mask, u->out, u->scal, z);
+    const uint32_t np = nx * ny, nbr = std::min(nblk(np), kRedBlocks);
+    hipLaunchKernelGGL(k_rms_part, dim3(nbr), dim3(kThreads), 0, s, np, mask, r, u->part);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0);
+    const GridIO io{r, z, mask, u->scal, nx, ny};
+    unet_forward(u, s, &io);
     return OECL_OK;
 }
 
@@ -932,14 +1086,14 @@ static int precondition_smoothed(oecl_unet *u, uint32_t nx, uint32_t ny, const d
                                  const uint8_t *mask, const double *r, double *z, double omega, hipStream_t s)
 {
     if (!(omega > 0.)) return precondition(u, nx, ny, mask, r, z, s);
-    const uint32_t np = nx * ny, nb = nblk(np);
+    const uint32_t np = nx * ny, nb = nblk(np), nbr = std::min(nb, kRedBlocks);
     double *z1 = u->sv[0], *r1 = u->sv[1], *n = u->sv[2], *an = u->sv[3];
-    hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, r, omega, z1);
-    hipLaunchKernelGGL(k_resid_az, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, r, z1, r1, nullptr, nullptr);
-    int rc = precondition(u, nx, ny, mask, r1, n, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_an_dots, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, n, r1, an, u->part);
-    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nb, u->scal, 8);
+    hipLaunchKernelGGL(k_smooth1, dim3(nbr), dim3(kThreads), 0, s, nx, np, cx, cy, mask, r, omega, z1, r1, u->part);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0);
+    const GridIO io{r1, n, mask, u->scal, nx, ny};
+    unet_forward(u, s, &io);
+    hipLaunchKernelGGL(k_an_dots, dim3(nbr), dim3(kThreads), 0, s, nx, np, cx, cy, mask, n, r1, an, u->part);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nbr, u->scal, 8);
     hipLaunchKernelGGL(k_combine, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, z1, n, r1, an, u->scal, omega, z);
     return OECL_OK;
 }
@@ -1021,12 +1175,12 @@ OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const dou
         return OECL_ERR_ARG;
     if (check == 0) check = 1;
     hipStream_t st = (hipStream_t)stream;
-    const uint32_t np = nx * ny, nb = nblk(np), m1 = mDirs + 1;
+    const uint32_t np = nx * ny, nb = nblk(np), nbr = std::min(nb, kRedBlocks), m1 = mDirs + 1;
     double *r = (double *)work, *z = r + np, *P = z + np, *AP = P + (size_t)m1 * np;
     double *part = AP + (size_t)m1 * np, *S = part + FCG_MAXM * (nb + 1), *D = S + 16, *pAp = D + FCG_MAXM;
     hipLaunchKernelGGL(k_mask_zero, dim3(nb), dim3(kThreads), 0, st, np, mask, phi);
-    hipLaunchKernelGGL(k_fcg_init, dim3(nb), dim3(kThreads), 0, st, nx, np, cx, cy, mask, q, phi, r, part);
-    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 8);  // S[8] r.r, S[9] q.q
+    hipLaunchKernelGGL(k_fcg_init, dim3(nbr), dim3(kThreads), 0, st, nx, np, cx, cy, mask, q, phi, r, part);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nbr, S, 8);  // S[8] r.r, S[9] q.q
     double h[2];
     OECL_HIP(hipMemcpyAsync(h, S + 8, 16, hipMemcpyDeviceToHost, st), "fcg copy");
     OECL_HIP(hipStreamSynchronize(st), "fcg sync");
@@ -1039,16 +1193,16 @@ OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const dou
             if (rc) return rc;
             const uint32_t nUsed = it < mDirs ? it : mDirs;
             if (nUsed) {
-                hipLaunchKernelGGL(k_fcg_zdots, dim3(nb), dim3(kThreads), 0, st, np, z, AP, it, m1, nUsed, part);
-                hipLaunchKernelGGL(k_finish<FCG_MAXM>, dim3(1), dim3(kThreads), 0, st, part, nb, D, 0);
+                hipLaunchKernelGGL(k_fcg_zdots, dim3(nbr), dim3(kThreads), 0, st, np, z, AP, it, m1, nUsed, part);
+                hipLaunchKernelGGL(k_finish<FCG_MAXM>, dim3(1), dim3(kThreads), 0, st, part, nbr, D, 0);
             }
             hipLaunchKernelGGL(k_fcg_newdir, dim3(nblk(np, 256)), dim3(256), 0, st, np, z, P, D, pAp, it, m1, nUsed);
             double *p = P + (size_t)(it % m1) * np, *ap = AP + (size_t)(it % m1) * np;
-            hipLaunchKernelGGL(k_fcg_ap, dim3(nb), dim3(kThreads), 0, st, nx, np, cx, cy, mask, p, r, ap, part);
-            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 6);
+            hipLaunchKernelGGL(k_fcg_ap, dim3(nbr), dim3(kThreads), 0, st, nx, np, cx, cy, mask, p, r, ap, part);
+            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nbr, S, 6);
             hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1), 0, st, S, pAp, it % m1);
-            hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kThreads), 0, st, np, S, phi, r, p, ap, part);
-            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nb, S, 8);
+            hipLaunchKernelGGL(k_fcg_update, dim3(nbr), dim3(kThreads), 0, st, np, S, phi, r, p, ap, part);
+            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nbr, S, 8);
         }
         OECL_CHECK_LAUNCH("oecl_fcg_solve");
         OECL_HIP(hipMemcpyAsync(h, S + 8, 8, hipMemcpyDeviceToHost, st), "fcg copy");

@@ -64,6 +64,10 @@ class UNetPreconditioner:
         except Exception:
             pass
 
+    def set_conv_rows(self, rows):
+        """Output rows per convolution block (1, 2, 4, 8; 0 = chosen per image size)."""
+        L.check(L.lib().oecl_unet_set_conv_rows(self._h, L.i32(rows)), 'unet conv rows')
+
     # ---------------------------------------------------------------- parameters
     def get_params(self):
         """(W [14][9][16][16], b [14][16]) as float32 host arrays."""
@@ -157,14 +161,36 @@ class UNetPreconditioner:
             out.append((ps.applyA(phi), phi))
         return out
 
-    def train(self, ps, pairs, steps, lr=1e-3, batch=1, lrEnd=None, seed=0, log=None, logEvery=0):
+    def train(self, ps, pairs, steps, lr=1e-3, batch=1, lrEnd=None, seed=0, log=None, logEvery=0,
+              validate=None, validateEvery=0, validateIter=4000):
         """Adam on samples (q, phi) drawn from `pairs` (device FP64 grid vectors, phi = A^-1 q),
         `batch` samples per update; the learning rate decays geometrically to lrEnd.  Returns
-        the list of (step, mean loss over the last logEvery steps) when logEvery > 0."""
+        the list of (step, mean loss over the last logEvery steps) when logEvery > 0.
+
+        validate: right-hand sides (device FP64 grid vectors) kept out of training.  Every
+        validateEvery steps and at the end the flexible CG solves them to 1e-8 (at most
+        validateIter iterations each, which a solve that does not converge counts), and the
+        parameters with the fewest total iterations are kept: single-sample losses are too noisy
+        to tell a good preconditioner from a poor one, the iterations are what it is for.  The
+        (step, iterations) record is in self.validation."""
         rng = np.random.default_rng(seed)
         lib = L.lib()
         lrEnd = lr if lrEnd is None else lrEnd
         out, acc, cnt = [], 0., 0
+        self.validation = []
+        best = None
+
+        def check(step):
+            nonlocal best
+            its = 0
+            for q in validate:
+                _, it, _, ok = self.fcg(ps, q, tol=1e-8, maxIter=validateIter)
+                its += it if ok else validateIter
+            self.validation.append((step, its))
+            if log:
+                log('train step %d: validation %d iterations' % (step, its))
+            if best is None or its < best[0]:
+                best = (its, self.get_params())
         for k in range(int(steps)):
             lk = lr * (lrEnd / lr) ** (k / max(1, steps - 1))
             for j in range(batch):
@@ -185,6 +211,12 @@ class UNetPreconditioner:
                 if log:
                     log('train step %d: loss %.4g' % (k + 1, acc / cnt))
                 acc, cnt = 0., 0
+            if validate and validateEvery and (k % validateEvery == validateEvery - 1) and k != steps - 1:
+                check(k + 1)
+        if validate:
+            check(int(steps))
+            if best[1] is not None:
+                self.set_params(*best[1])
         self.history.extend(out)
         return out
 

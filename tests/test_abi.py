@@ -54,6 +54,7 @@ def test_invalid_arguments_are_refused_before_any_device_call():
                                        ctypes.c_uint32(0), None, None, None, None, None, ctypes.byref(h), None) == ARG
     assert lib.oecl_unet_create(ctypes.c_uint32(7), ctypes.c_uint32(8), ctypes.c_uint32(0), ctypes.byref(h)) == ARG
     assert lib.oecl_unet_create(ctypes.c_uint32(8), ctypes.c_uint32(12), ctypes.c_uint32(0), ctypes.byref(h)) == ARG
+    assert lib.oecl_unet_set_conv_rows(None, 0) == ARG
     assert lib.oecl_secem_pass1(None, 0, None, None, ctypes.c_uint32(4), None, 0, 0, 0, None, None, None) == ARG
     assert lib.oecl_furman_pass1(None, 0, None, None, ctypes.c_uint32(4), None, 0, 0, 0, None, None, None) == ARG
     assert lib.oecl_photo_emit(None, 0, None, ctypes.c_uint32(4), 0, 0, 0, ctypes.c_double(7.), ctypes.c_double(5.),

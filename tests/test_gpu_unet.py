@@ -19,8 +19,9 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
+@pytest.mark.parametrize('rows', [0, 1, 2, 4, 8])
 @pytest.mark.parametrize('H,W', [(32, 32), (24, 40)])
-def test_forward_and_backward_match_oracle(H, W):
+def test_forward_and_backward_match_oracle(H, W, rows):
     from oracle import unet_oracle as uo
     from openecloud_amd.learnedPreconditioner import UNetPreconditioner
 
@@ -32,6 +33,7 @@ def test_forward_and_backward_match_oracle(H, W):
     b = rng.standard_normal(b.shape) * 0.1
     Wt32, b32 = Wt.astype(np.float32), b.astype(np.float32)
     net = UNetPreconditioner(_G(), seed=1)
+    net.set_conv_rows(rows)      # every tile height of the convolution kernel (0: the default)
     net.set_params(Wt32, b32)
     img = rng.standard_normal((H, W))
     img_b = uo.bf16_round(img)
@@ -164,3 +166,62 @@ def test_operator_pairs_train_a_stronger_preconditioner():
         assert err < 1e-6, info
         assert it < 0.25 * jac, info
         assert it < b0, info
+
+
+@pytest.mark.parametrize('rows', [0, 1, 8])
+def test_precondition_matches_image_path(rows):
+    """The preconditioner's inference pass stages r / rms straight from the grid vector into the
+    first convolution and writes z = rms * y from the last: it must give what the image path
+    (input image, FP32 output image) gives, with and without the Jacobi smoothings around the
+    network.  The tolerance covers rounding of rms, which is summed in another order here."""
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = _chamber(45)                     # an extended grid that is not a multiple of 8
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    net = UNetPreconditioner(g, seed=3)
+    net.set_conv_rows(rows)
+    nx, ny = g.getNxExt(), g.getNyExt()
+    mask = ps.mask.bool()
+    r = torch.as_tensor(np.random.default_rng(4).standard_normal(nx * ny)).cuda() * mask
+
+    def image_path(v):
+        rms = torch.sqrt((v[mask] ** 2).sum() / mask.sum())
+        img = torch.zeros((net.H, net.W, 16), dtype=torch.float32, device='cuda')
+        img[:ny, :nx, 0] = (v / rms).reshape(ny, nx).float()
+        img[..., 0] *= torch.nn.functional.pad(mask.reshape(ny, nx).float(), (0, net.W - nx, 0, net.H - ny))
+        y = net.forward_image(img.to(torch.bfloat16))[:ny, :nx].reshape(-1).double()
+        return torch.where(mask, rms * y, torch.zeros_like(y))
+
+    z0 = net.precondition(ps, r, omega=0.)
+    ref0 = image_path(r)
+    assert float(torch.linalg.norm(z0 - ref0) / torch.linalg.norm(ref0)) < 1e-3
+    assert not torch.any(z0[~mask])
+    # smoothed: z1 = omega D^-1 r, r1 = r - A z1, n = N(r1), c = n.r1 / n.An,
+    # z = z1 + c n + omega D^-1 (r1 - c A n)
+    om = 2. / 3.
+    z = net.precondition(ps, r, omega=om)
+    cx, cy = ps.cx, ps.cy
+    diag = torch.where(mask, cx + cx.roll(1) + cy + cy.roll(nx), torch.ones_like(cx))
+    z1 = torch.where(mask, om * r / diag, torch.zeros_like(r))
+    r1 = torch.where(mask, r - ps.applyA(z1), torch.zeros_like(r))
+    n = image_path(r1)
+    an = ps.applyA(n)
+    c = float((n * r1).sum() / (n * an).sum())
+    ref = torch.where(mask, z1 + c * n + om * (r1 - c * an) / diag, torch.zeros_like(r))
+    assert float(torch.linalg.norm(z - ref) / torch.linalg.norm(ref)) < 1e-3
+
+
+def test_training_is_reproducible():
+    """The weight gradients are summed in a fixed order (no atomics): two networks trained from the
+    same seed on the same pairs end with identical parameters."""
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = _chamber(96)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    params = []
+    for _ in range(2):
+        net = UNetPreconditioner(g, seed=5)
+        pairs = net.smoothed_pairs(ps, net.operator_pairs(ps, 8, seed=2))
+        net.train(ps, pairs, steps=40, lr=5e-4, batch=2, seed=1)
+        params.append(net.get_params())
+    assert np.array_equal(params[0][0], params[1][0]) and np.array_equal(params[0][1], params[1][1])
