@@ -369,6 +369,29 @@ def main_learned(a):
             ps.solvePhi(q)
         torch.cuda.synchronize()
         res['direct'].append(((time.perf_counter() - t0) / 10, 0, None, True, None))
+    # the buildup continued in the reference order with the learned solve (warm-started from
+    # the previous potential, the direct solve judging every 10th solve), against the same steps
+    # with the direct solve
+    psl = PoissonSolver(g, method='learned', tol=1e-8, maxIter=4000, checkEvery=10, preconditioner=net)
+    steps_b = 20
+    bsteps = {}
+    for name, solver in (('direct', ps), ('learned', psl)):
+        b.ps = solver
+        its, pst = [], 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps_b):
+            pst += p.getMacroParticleCount()
+            b.step_reference()
+            its.append(solver.lastIterations)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        bsteps[name] = {'ms_per_step': 1e3 * el / steps_b, 'particle_steps_per_s': pst / el,
+                        'iterations_mean': float(np.mean(its))}
+    b.ps = ps
+    bsteps['learned'].update(judge_max_rel_err=psl.judgeMaxErr, judge_count=psl.judgeCount,
+                             fallbacks=psl.fallbacks)
+    log('buildup steps: %s' % json.dumps(bsteps))
     summ = {}
     for k, v in res.items():
         tm = [x[0] for x in v]
@@ -411,6 +434,7 @@ def main_learned(a):
                      'frac': flops / t_apply / 1e12 / 2500.0, 'traffic': None},
         'learned_vs_jacobi_time': summ['learned']['solves_per_s'] / summ['jacobi']['solves_per_s'],
         'learned_vs_jacobi_iterations': summ['jacobi']['iterations_mean'] / max(1., summ['learned']['iterations_mean']),
+        'buildup_reference_steps': dict(bsteps, steps=steps_b, particles=p.getMacroParticleCount()),
     }
     print(json.dumps(out))
 

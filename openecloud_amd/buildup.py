@@ -228,7 +228,8 @@ class Buildup:
         every step up to maxSteps is tabulated now (it depends on the step index only)."""
         p, pb, ps = self.p, self.pb, self.ps
         if ps.method != 'direct':
-            raise ValueError('the native step needs PoissonSolver(method="direct")')
+            raise ValueError('the native step needs PoissonSolver(method="direct"); method "%s" runs through '
+                             'step() and step_reference()' % ps.method)
         dev = L.device()
         n = p.macroParticleCount
         cap = p.capacity

@@ -256,6 +256,8 @@ class UNetPreconditioner:
         if ok:
             return phi
         self.fallbacks += 1
-        if ps.method != 'direct':
+        if ps._fp is None:
             raise L.OeclError('learned PCG did not converge (relres %.3g after %d iterations)' % (rr, it))
-        return ps.solvePhi(q).clone()
+        out = torch.empty_like(phi)
+        ps._direct(ps._rhs(q), out)
+        return out

@@ -15,6 +15,12 @@ Two device methods:
                       `checkEvery` solves and, if too large, refined by CG.
   'pcg'               Jacobi-preconditioned CG to tol * ||q_inside||, warm-started from the
                       previous potential.
+  'learned'           flexible CG preconditioned by a trained UNetPreconditioner (passed as
+                      `preconditioner` or set with setPreconditioner), to tol * ||q_inside||,
+                      warm-started from the previous potential.  The direct solve is its
+                      fallback (when the CG does not converge within maxIter) and its judge:
+                      every `checkEvery` solves it solves the same q and the relative
+                      difference is recorded (judgeMaxErr, judgeCount).
 """
 import ctypes
 
@@ -77,14 +83,14 @@ def capacitance_rows(nx, ny, mask, cx, cy):
 
 class PoissonSolver:
 
-    def __init__(self, gridObj, tol=1e-12, maxIter=200000, method='direct', checkEvery=1):
+    def __init__(self, gridObj, tol=1e-12, maxIter=200000, method='direct', checkEvery=1, preconditioner=None):
         L.require_gpu()
         nx = gridObj.getNxExt()
         npx = gridObj.getNpExt()
         self.nx, self.ny, self.npExt = nx, gridObj.getNyExt(), npx
         self.tol, self.maxIter = float(tol), int(maxIter)
-        if method not in ('direct', 'pcg'):
-            raise ValueError("method must be 'direct' or 'pcg'")
+        if method not in ('direct', 'pcg', 'learned'):
+            raise ValueError("method must be 'direct', 'pcg' or 'learned'")
         self.method = method
         self.checkEvery = int(checkEvery)
         ip = np.asarray(gridObj.getInsidePoints())
@@ -119,7 +125,12 @@ class PoissonSolver:
         self.lastRelRes = 0.
         self.solveCount = 0
         self._fp = None
-        if method == 'direct':
+        self.preconditioner = preconditioner
+        self.fallbacks = 0
+        self.judgeMaxErr = 0.
+        self.judgeCount = 0
+        self._judge = None
+        if method in ('direct', 'learned'):
             self._setup_direct()
 
     def _setup_direct(self):
@@ -163,9 +174,38 @@ class PoissonSolver:
             return q.contiguous()
         return L.to_dev(np.asarray(q, np.float64))
 
+    def setPreconditioner(self, net):
+        """The trained UNetPreconditioner of method 'learned'."""
+        self.preconditioner = net
+
+    def _direct(self, qd, out):
+        L.check(L.lib().oecl_fastpoisson_solve(self._fp, L.ptr(qd), L.ptr(out), L.stream()),
+                'PoissonSolver.solve (direct)')
+
+    def _learned(self, qd):
+        if self.preconditioner is None:
+            raise L.OeclError("method 'learned' needs a trained preconditioner (setPreconditioner)")
+        _, it, rr, ok = self.preconditioner.fcg(self, qd, phi=self.phi, tol=self.tol, maxIter=self.maxIter)
+        self.lastIterations, self.lastRelRes = it, rr
+        if not ok:
+            self.fallbacks += 1
+            self._direct(qd, self.phi)
+        self.solveCount += 1
+        if self.checkEvery > 0 and self.solveCount % self.checkEvery == 0:
+            if self._judge is None:
+                self._judge = torch.empty_like(self.phi)
+            self._direct(qd, self._judge)
+            den = float(torch.linalg.norm(self._judge))
+            err = float(torch.linalg.norm(self.phi - self._judge)) / den if den > 0. else 0.
+            self.judgeMaxErr = max(self.judgeMaxErr, err)
+            self.judgeCount += 1
+        return self.phi
+
     def solvePhi(self, q):
         """phi[inside] = A^-1 q[inside] (poissonSolver.pyx:85) on the device."""
         qd = self._rhs(q)
+        if self.method == 'learned':
+            return self._learned(qd)
         if self.method == 'direct':
             L.check(L.lib().oecl_fastpoisson_solve(self._fp, L.ptr(qd), L.ptr(self.phi),
                                                    L.stream()), 'PoissonSolver.solve (direct)')

@@ -273,3 +273,56 @@ def test_simple_emitters_in_the_step_match_oracle(orc, model, form):
         err = (np.abs(got - ref[it]) / scale).max()
         assert err < 1e-11, (it, err)
     assert ref[-1].shape[0] != n
+
+
+def test_buildup_with_learned_solve_matches_oracle(orc):
+    """The reference-order step with PoissonSolver(method='learned'): the flexible CG with a
+    network trained on this grid, warm-started from the previous potential, to 1e-11; the
+    direct solve judges every solve.  The particle state follows the oracle's FP64 loop (whose
+    solve is exact) to the accuracy the tolerance allows, with no fallback.  (Measured: 52-92
+    iterations a step; the bunch changes the charge too much from one step to the next for the
+    warm start to save much here.)"""
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.furman_oracle import material
+    g = Grid(64, 48, 0.09, 0.07, 1)
+    n = 20000
+    pd0 = random_particles(g, n, seed=11, wall='ellip', vscale=3e6)
+    pd0[:, 5] = 5e4
+    dt = 2e-11
+    randomGen.seed(2024)
+    key = randomGen.key()
+    p = Particles('electrons', 'variableWeight', ordering='reference')
+    p.setParticleData(pd0)
+    p.setBConst([0., 0.5, 0.])
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.6, eHat0=300.)
+    ps = PoissonSolver(g, method='learned', tol=1e-11, maxIter=2000, checkEvery=1)
+    net = UNetPreconditioner(g, seed=0)
+    net.train(ps, net.smoothed_pairs(ps, net.operator_pairs(ps, 32, seed=1)), steps=300, lr=5e-4, batch=2)
+    ps.setPreconditioner(net)
+    beam = GaussianBunchTrain(g, dt, nParticles=1e11, tBunchSpacing=25e-9, radiusSigma=2e-3,
+                              nBunches=1, bunchLengthSigma=0.003)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1)
+    nsteps = 12
+    ref = _oracle_loop(orc, g, pd0.copy(), nsteps, dt, 1, np.array([0., 0.5, 0.]),
+                       beam.getTransversalProfile(), beam.getScale,
+                       material('copper', 1.6, eHat0=300.), key, n)
+    its = []
+    for it in range(nsteps):
+        b.step_reference()
+        its.append(ps.lastIterations)
+        got = p.getParticleData().cpu().numpy()
+        assert got.shape == ref[it].shape, (it, got.shape, ref[it].shape)
+        scale = np.abs(ref[it]).max(axis=0)
+        err = (np.abs(got - ref[it]) / scale).max()
+        assert err < 1e-8, (it, err, its)
+    print('learned solve in the buildup: iterations', its, 'judge max rel err', ps.judgeMaxErr)
+    assert ps.fallbacks == 0 and ps.judgeCount == nsteps and ps.judgeMaxErr < 1e-8, (its, ps.judgeMaxErr)
