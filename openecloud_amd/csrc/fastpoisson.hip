@@ -57,6 +57,8 @@ struct oecl_fastpoisson {
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
     uint32_t rCap;            // right-hand sides the vectors above hold
     char *batchMem;           // their own allocation once rCap > 1
+    double *gP, *gT;          // library-product path (large boxes): folded operands and products,
+    char *gemmMem;            //   2 kx x (by rCap) each, their own allocation
 };
 
 namespace {
@@ -168,6 +170,48 @@ __global__ void __launch_bounds__(GNT)
             const uint32_t p = (uint32_t)(om + 2) + ((uint32_t)gn % by + 2) * nx;
             C[p + (size_t)((uint32_t)gn / by) * np] = mask[p] ? s : 0.;
         }
+    }
+}
+
+// The same transform for large boxes through library GEMMs: k_dst_fold forms P_z (col-major
+// kx x N each, ld kx; row kx - 1 of P_1 is zero padding for odd n), rocBLAS multiplies
+// T_z = F_z P_z, and k_dst_unfold interleaves the rows of T_0, T_1 into C (EPI 0) or scatters
+// them into the potential (EPI 2) exactly as k_dst_gemm's epilogues do.  From kx >= kBlasMinK
+// the library's large tiles beat the 32 x 32 tiles above (at kx = 1022, N = 2044: 81 us a
+// product against about 200); below it the launch and the extra passes cost more.
+constexpr int kBlasMinK = 384;
+
+__global__ void k_dst_fold(int n, int N, const double *__restrict__ X, int ldx, int kx, double *__restrict__ P)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)kx * N) return;
+    const int j = (int)(t % kx), c = (int)(t / kx);
+    const double *xc = X + (size_t)c * ldx;
+    const int h = n - 1 - j;
+    const double a = xc[j];
+    double p0 = a, p1 = 0.;  // the middle row of odd n enters P_0 once
+    if (h != j) {
+        const double b = xc[h];
+        p0 = a + b;
+        p1 = a - b;
+    }
+    P[j + (size_t)c * kx] = p0;
+    P[(size_t)kx * N + j + (size_t)c * kx] = p1;
+}
+
+template <int EPI>
+__global__ void k_dst_unfold(int N, int n, int kx, const double *__restrict__ T, double *__restrict__ C, int ldc,
+                             uint32_t nx, const uint8_t *__restrict__ mask, uint32_t by, uint32_t np)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n * N) return;
+    const int om = (int)(t % n), gn = (int)(t / n);
+    const double s = T[(size_t)(om & 1) * kx * N + (om >> 1) + (size_t)gn * kx];
+    if (EPI == 0) {
+        C[om + (size_t)gn * ldc] = s;
+    } else {
+        const uint32_t p = (uint32_t)(om + 2) + ((uint32_t)gn % by + 2) * nx;
+        C[p + (size_t)((uint32_t)gn / by) * np] = mask[p] ? s : 0.;
     }
 }
 
@@ -456,6 +500,8 @@ int launch_cap_wy(const oecl_fastpoisson *h, bool cap, const double *yh, double 
 
 }  // namespace
 
+static int reserve_gemm(oecl_fastpoisson *h, uint32_t R);
+
 OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, double cyu, uint32_t m,
                                         const int32_t *sIdxHost, const int32_t *wPtrHost,
                                         const int32_t *wColHost, const double *wValHost,
@@ -487,6 +533,8 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->mask = maskDev;
     h->rCap = 1;
     h->batchMem = nullptr;
+    h->gemmMem = nullptr;
+    h->gP = h->gT = nullptr;
     // rows a grouped by their box column j (stable), for the per-column update
     std::vector<int32_t> jA(m), jCol, jStart;
     for (uint32_t a = 0; a < m; ++a) jA[a] = (int32_t)a;
@@ -615,6 +663,8 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     } else if (capDiag) {
         *capDiag = 0.;
     }
+    const int rcg = reserve_gemm(h, 1);
+    if (rcg != OECL_OK) return rcg;
     OECL_HIP(hipDeviceSynchronize(), "fastpoisson create");
     *out = h;
     return OECL_OK;
@@ -625,10 +675,27 @@ FpBox fastpoisson_box(const oecl_fastpoisson *h)
     return FpBox{h->b, h->mask, h->nx, h->ny, h->bx};
 }
 
+// Buffers of the library-product path for R right-hand sides (large boxes only).
+static int reserve_gemm(oecl_fastpoisson *h, uint32_t R)
+{
+    const uint64_t kx = (h->bx + 1) / 2;
+    if ((int)kx < kBlasMinK) return OECL_OK;
+    const uint64_t e = 2ull * kx * h->by * R;
+    char *mem = nullptr;
+    OECL_HIP(hipMalloc(&mem, 16ull * e + 512), "fastpoisson product buffers");
+    if (h->gemmMem) (void)hipFree(h->gemmMem);
+    h->gemmMem = mem;
+    h->gP = (double *)mem;
+    h->gT = h->gP + e;
+    return OECL_OK;
+}
+
 int fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R)
 {
     if (R <= h->rCap) return OECL_OK;
     const uint64_t nb = h->nb, m = h->m;
+    int rcg = reserve_gemm(h, R);
+    if (rcg != OECL_OK) return rcg;
     char *mem = nullptr;
     OECL_HIP(hipMalloc(&mem, 8ull * R * (3 * nb + 2 * m) + 1024), "fastpoisson batch alloc");
     if (h->batchMem) (void)hipFree(h->batchMem);
@@ -657,13 +724,47 @@ OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, dou
     return fastpoisson_solve_box(h, phi, 1, s);
 }
 
+// One DST transform of the N columns of X (bx rows, ld bx): EPI 0 into C (ld ldc), EPI 2 into the
+// potential C = phi.  Library products for large boxes, except while the stream is being
+// captured into a graph (the library's calls are kept out of graphs).
+template <int EPI>
+static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C, int ldc, hipStream_t s)
+{
+    const int bx = h->bx, kx = (bx + 1) / 2;
+    bool lib = kx >= kBlasMinK && h->gemmMem;
+    if (lib) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) lib = false;
+    }
+    if (!lib) {
+        const dim3 gg((kx + GT - 1) / GT, (N + GT - 1) / GT, 2), gb(GNT);
+        hipLaunchKernelGGL(k_dst_gemm<EPI>, gg, gb, 0, s, N, bx, h->Fx, X, bx, C, ldc, h->nx, h->mask,
+                           (uint32_t)h->by, h->np);
+        return OECL_OK;
+    }
+    hipLaunchKernelGGL(k_dst_fold, dim3((unsigned)(((size_t)kx * N + 255) / 256)), dim3(256), 0, s, bx, N, X, bx, kx,
+                       h->gP);
+    RB(rocblas_set_stream(h->blas, s), "fastpoisson set stream");
+    const double one = 1., zero = 0.;
+    for (int z = 0; z < 2; ++z) {
+        const int kz = z ? bx / 2 : kx;
+        RB(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, kz, N, kz, &one,
+                         h->Fx + (size_t)z * kx * kx, kx, h->gP + (size_t)z * kx * N, kx, &zero,
+                         h->gT + (size_t)z * kx * N, kx),
+           "fastpoisson dst product");
+    }
+    hipLaunchKernelGGL(k_dst_unfold<EPI>, dim3((unsigned)(((size_t)bx * N + 255) / 256)), dim3(256), 0, s, N, bx, kx,
+                       h->gT, C, ldc, h->nx, h->mask, (uint32_t)h->by, h->np);
+    return OECL_OK;
+}
+
 int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStream_t s)
 {
     if (R < 1 || R > h->rCap || (uint64_t)h->by * R > 0x7FFFFFFFull) return OECL_ERR_ARG;
-    const int bx = h->bx, by = h->by, kx = (bx + 1) / 2, N = by * (int)R;
-    const dim3 gg((kx + GT - 1) / GT, (N + GT - 1) / GT, 2), gb(GNT);
+    const int bx = h->bx, by = h->by, N = by * (int)R;
     // hat b = Sx b
-    hipLaunchKernelGGL(k_dst_gemm<0>, gg, gb, 0, s, N, bx, h->Fx, h->b, bx, h->t1, bx, 0u, nullptr, 1u, 0u);
+    int rct = dst_transform<0>(h, N, h->b, h->t1, bx, s);
+    if (rct != OECL_OK) return rct;
     if (h->m) {
         // hat y = T^-1 hat b ; wy = W Sx hat y ; hat b -= (U C^-1 wy) in x-transformed form
         int rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
@@ -677,8 +778,8 @@ int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStrea
     // hat x = T^-1 hat b ; phi = Sx hat x scattered into the extended grid (np apart per side)
     int rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
     if (rc != OECL_OK) return rc;
-    hipLaunchKernelGGL(k_dst_gemm<2>, gg, gb, 0, s, N, bx, h->Fx, h->t2, bx, phi, 0, h->nx, h->mask,
-                       (uint32_t)by, h->np);
+    rc = dst_transform<2>(h, N, h->t2, phi, 0, s);
+    if (rc != OECL_OK) return rc;
     OECL_CHECK_LAUNCH("oecl_fastpoisson_solve");
     return OECL_OK;
 }
@@ -688,6 +789,7 @@ OECL_EXPORT int oecl_fastpoisson_destroy(oecl_fastpoisson *h)
     if (!h) return OECL_OK;
     (void)hipFree(h->Sx);
     if (h->batchMem) (void)hipFree(h->batchMem);
+    if (h->gemmMem) (void)hipFree(h->gemmMem);
     rocblas_destroy_handle(h->blas);
     delete h;
     return OECL_OK;

@@ -106,3 +106,51 @@ def test_apply_matches_oracle_operator(args):
     xin[ind] = x[ind]
     phi = ps.solvePhi(torch.as_tensor(y).cuda()).cpu().numpy()
     assert np.linalg.norm(phi - xin) <= 1e-9 * np.linalg.norm(xin)
+
+
+# Boxes wide enough (half-width >= 384) for the transform's products to go through the library
+# GEMMs, with even and odd box widths, and irregular rows next to an elliptical wall.
+@pytest.mark.parametrize('args', [(800, 64, 0.1, 0.05, 0), (801, 64, 0.1, 0.05, 0), (790, 100, 0.09, 0.07, 1)])
+def test_direct_wide_box_matches_lu(args):
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.grid_oracle import OraclePoisson
+    g = Grid(*args)
+    assert (g.getNxExt() - 4 + 1) // 2 >= 384
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    q = _rhs(g, 5)
+    ps.solvePhi(torch.as_tensor(q).cuda())
+    phi = ps.getPhi().cpu().numpy()
+    ref = OraclePoisson(g)
+    ref.solve(q)
+    assert np.linalg.norm(phi - ref.phi) <= 1e-10 * np.linalg.norm(ref.phi)
+    assert ps.residual(q) <= 1e-9
+
+
+def test_direct_wide_box_in_a_graph_matches_eager():
+    """Under graph capture the wide-box transform keeps to the solver's own MFMA kernels (the
+    library's calls stay out of graphs): every replay must give the eager (library) result to
+    rounding."""
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = Grid(801, 64, 0.1, 0.05, 0)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    q = torch.as_tensor(_rhs(g, 6)).cuda()
+    eager = ps.solvePhi(q).clone()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        ps.solvePhi(q)                    # warm-up on the capture stream
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=side):
+            ps.solvePhi(q)
+    torch.cuda.current_stream().wait_stream(side)
+    for k in range(3):
+        ps.phi.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        err = float(torch.linalg.norm(ps.phi - eager) / torch.linalg.norm(eager))
+        assert err < 1e-12, (k, err)
+        ps.solvePhi(q * 0.5)              # eager work between the replays
