@@ -5,7 +5,11 @@
 //   inverses       safeguarded Halley iteration inside a shrinking bracket, started from the
 //                  Wilson-Hilferty approximation (gamma) and the A&S 26.5.22 approximation
 //                  (beta), each with the A&S 26.2.23 normal quantile.
-// All are FP64 and iterate to full double precision (the reference stops at 1e-9 / 1e-12).
+// All are FP64.  The inverses stop after a Halley (or Newton) step of at most kStepTol |x|:
+// the error left is then of order kStepTol^3 (kStepTol^2 for Newton), far below rounding.
+// The test comes before the bracket safeguard: at convergence the step rounds to zero and the
+// new iterate equals the bracket end just set, which the safeguard would replace by a bisection
+// point far from the root (the reference stops at 1e-9 / 1e-12).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -14,6 +18,7 @@ namespace sf {
 
 constexpr double kTiny = 1e-300;
 constexpr double kEps = 1.1102230246251565e-16;
+constexpr double kStepTol = 1e-8;
 
 // Normal quantile, A&S 26.2.23 (|error| < 4.5e-4): only used as a starting point.
 __device__ __forceinline__ double norm_quantile_rough(double p)
@@ -73,10 +78,20 @@ __device__ inline double gammaincinv(double a, double p)
     double lga = lgamma(a);
     double x;
     if (a > 1.) {
-        double z = norm_quantile_rough(p);
-        double s = 1. - 1. / (9. * a) + z / (3. * sqrt(a));
-        x = a * s * s * s;
-        if (x < 1e-3) x = 1e-3;
+        // Far lower tail (x << a): P(a, x) = x^a e^-x / Gamma(a + 1) (1 + x / (a + 1) + ...), so
+        // log x = (log p + lgamma(a + 1) + x - log(1 + x / (a + 1))) / a, two fixed-point passes
+        // from x = (p Gamma(a + 1))^(1/a).  Elsewhere Wilson-Hilferty.
+        const double lp = log(p) + lgamma(a + 1.);
+        x = exp(lp / a);
+        if (x < 0.2 * a) {
+            x = exp((lp + x - log1p(x / (a + 1.))) / a);
+            x = exp((lp + x - log1p(x / (a + 1.))) / a);
+        } else {
+            double z = norm_quantile_rough(p);
+            double s = 1. - 1. / (9. * a) + z / (3. * sqrt(a));
+            x = a * s * s * s;
+            if (x < 1e-3) x = 1e-3;
+        }
     } else {
         // P(a, x) ~ x^a / Gamma(a + 1) for small x.
         x = exp((log(p) + lgamma(a + 1.)) / a);
@@ -95,6 +110,8 @@ __device__ inline double gammaincinv(double a, double p)
             double corr = 1. - 0.5 * u * ((a - 1.) / x - 1.);
             double stepv = (corr > 0.5 && corr < 2.) ? u / corr : u;
             xn = x - stepv;
+            // converged; tested before the bracket, which a step rounding to zero lands on
+            if (fabs(stepv) <= kStepTol * fabs(x)) return xn;
         } else {
             xn = NAN;
         }
@@ -188,6 +205,8 @@ __device__ inline double betaincinv(double a, double b, double p)
             double corr = 1. - 0.5 * u * ((a - 1.) / x - (b - 1.) / (1. - x));
             double stepv = (corr > 0.5 && corr < 2.) ? u / corr : u;
             xn = x - stepv;
+            // converged; tested before the bracket, which a step rounding to zero lands on
+            if (fabs(stepv) <= kStepTol * fmax(fabs(x), 1e-300)) return xn;
         } else {
             xn = NAN;
         }

@@ -411,9 +411,12 @@ class Grid:
         a, b = cpi[c, 0], cpi[c, 1]
         n0 = bp[a, 1] - bp[b, 1]
         n1 = bp[b, 0] - bp[a, 0]
-        t = 1. / np.sqrt(n0 ** 2 + n1 ** 2)
-        self.cutCellNormalVectors[c, 0] = n0 * t
-        self.cutCellNormalVectors[c, 1] = n1 * t
+        # a cell whose two cut points coincide gets 0 * inf = NaN, as the reference does (its
+        # loop divides with cdivision=True, grid.pyx:4)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            t = 1. / np.sqrt(n0 ** 2 + n1 ** 2)
+            self.cutCellNormalVectors[c, 0] = n0 * t
+            self.cutCellNormalVectors[c, 1] = n1 * t
         # grid.pyx:808-849: face areas.
         daf = np.zeros(npx); daif = np.zeros(npx)
         J, I = np.meshgrid(np.arange(1, ny - 2), np.arange(1, nx - 2), indexing='ij')
