@@ -61,6 +61,9 @@ def parse():
                     help='dipole: the elliptical dipole buildup (default); scan: 64 buildups '
                          'stepped together (delta_max and intensity scan)')
     ap.add_argument('--runs', type=int, default=64, help='scan: number of buildups')
+    ap.add_argument('--scan-start', type=int, default=700,
+                    help='scan: the timed steps start at this step of the simulation (the first bunch passage, '
+                         'once multipacting has set in), whatever the autotuner ran before')
     ap.add_argument('--tune-cache', default=None, help='scan: autotuner cache file (JSON)')
     ap.add_argument('--scan-config', default=None,
                     help='scan: sortEvery,order,binShift to use instead of autotuning')
@@ -560,7 +563,8 @@ def main_scan(a):
     from openecloud_amd.autotune import tune_scan
     L.require_gpu()
     R = a.runs
-    maxSteps = a.warmup + a.steps + 2000
+    start = max(int(a.scan_start), a.warmup)
+    maxSteps = start + a.steps + 2000
     g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps)
     log = lambda m: print(m, file=sys.stderr)
     scan.sort()
@@ -574,9 +578,23 @@ def main_scan(a):
         best, results = tune_scan(scan, cacheFile=a.tune_cache, log=log)
     se = int(best['sortEvery'])
     steps = (a.steps + se - 1) // se * se
+    # the timed window is steps [start, start + steps) of a fresh scan from the same seeds, so
+    # that it does not depend on how many steps the autotuner ran (or whether it ran at all)
+    del scan, runs
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps)
+    scan.sort(best['order'], best['binShift'])
+    for _ in range(a.warmup):
+        scan.step()
     scan.sync()
-    if scan._graph is None or scan._graphSteps != se:
-        scan.capture(se)
+    scan.capture(se)
+    while scan.it + se <= start:
+        scan.sort(best['order'], best['binShift'])
+        scan.replay()
+    while scan.it < start:
+        scan.step()
+    st0 = scan.sync()
     it0 = scan.it
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -662,7 +680,10 @@ def main_scan(a):
                                'ppb log-uniform [5e10, 2e11], macro-particles log-uniform [1e4, 2e6]' % R,
                    'runs': R, 'grid': [256, 256], 'macro_particles_total_start': int(sum(n0)),
                    'macro_particles_total_end': int(sum(st['counts'])),
-                   'absorbed_total': int(sum(st['absorbed_total'])), 'emitted_total': int(sum(st['emitted_total']))},
+                   'absorbed_total': int(sum(st['absorbed_total'])), 'emitted_total': int(sum(st['emitted_total'])),
+                   'timed_steps': [it0, it0 + steps],
+                   'absorbed_in_timed_steps': int(sum(st['absorbed_total']) - sum(st0['absorbed_total'])),
+                   'emitted_in_timed_steps': int(sum(st['emitted_total']) - sum(st0['emitted_total']))},
         'poisson_solves_per_s': R * steps / wall,
         'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'avg_launch_us': avg_us,
