@@ -173,6 +173,96 @@ __device__ __forceinline__ void wave_deposit(double *__restrict__ rho, uint32_t 
     }
 }
 
+// Block-cooperative deposit: every thread of the block calls it, once.  As wave_deposit, but the
+// run totals of all NW waves go to one LDS tile covering the bounding box of the block's cells
+// when that box fits NW * WN nodes, so each touched node costs one global FP64 atomic per block
+// instead of one per wave: those atomics, on nodes that consecutive sorted blocks share, are what
+// bounds the push (adding the run totals to rho directly made it 3.4x slower).  A block whose box
+// is larger uses per-wave tiles in the same LDS (WN nodes each), as wave_deposit does.  tile:
+// NW * WN doubles of LDS; box: 4 words of LDS.
+template <int NW, int WN>
+__device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t nx, bool valid, uint32_t ix,
+                                              uint32_t iy, double q00, double q10, double q01, double q11,
+                                              double *tile, uint32_t *box)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t c = valid ? ix + iy * nx : 0xFFFFFFFFu;
+    if (!valid) q00 = q10 = q01 = q11 = 0.;
+    const bool head = wave_run_sum(c, q00, q10, q01, q11);
+    const uint32_t X0 = wave_minmax<true>(head ? ix : 0xFFFFFFFFu), X1 = wave_minmax<false>(head ? ix : 0u);
+    const uint32_t Y0 = wave_minmax<true>(head ? iy : 0xFFFFFFFFu), Y1 = wave_minmax<false>(head ? iy : 0u);
+    if (threadIdx.x == 0) {
+        box[0] = 0xFFFFFFFFu;
+        box[1] = 0u;
+        box[2] = 0xFFFFFFFFu;
+        box[3] = 0u;
+    }
+    __syncthreads();
+    if (lane == 0 && X0 != 0xFFFFFFFFu) {
+        atomicMin(&box[0], X0);
+        atomicMax(&box[1], X1);
+        atomicMin(&box[2], Y0);
+        atomicMax(&box[3], Y1);
+    }
+    __syncthreads();
+    const uint32_t B0 = box[0], B1 = box[1], C0 = box[2], C1 = box[3];
+    if (B0 == 0xFFFFFFFFu) return;  // nothing to deposit in this block (block-uniform)
+    const uint32_t bw = B1 - B0 + 2, bh = C1 - C0 + 2;
+    if ((uint64_t)bw * bh <= (uint64_t)NW * WN) {  // block-uniform
+        const uint32_t nb = bw * bh;
+        for (uint32_t k = threadIdx.x; k < nb; k += NW * 64) tile[k] = 0.;
+        __syncthreads();
+        if (head) {
+            const uint32_t o = (ix - B0) + (iy - C0) * bw;
+            atomicAdd(&tile[o], q00);
+            atomicAdd(&tile[o + 1], q10);
+            atomicAdd(&tile[o + bw], q01);
+            atomicAdd(&tile[o + bw + 1], q11);
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < nb; k += NW * 64) {
+            const double v = tile[k];
+            if (v != 0.) {
+                const uint32_t r = k / bw;
+                unsafeAtomicAdd(rho + (B0 + (k - r * bw)) + (size_t)(C0 + r) * nx, v);
+            }
+        }
+        return;
+    }
+    if (X0 == 0xFFFFFFFFu) return;  // wave-uniform
+    double *win = tile + wv * WN;
+    const uint32_t wx = X1 - X0 + 2, wy = Y1 - Y0 + 2;
+    if ((uint64_t)wx * wy > WN) {  // wave-uniform
+        if (head) {
+            unsafeAtomicAdd(rho + c, q00);
+            unsafeAtomicAdd(rho + c + 1, q10);
+            unsafeAtomicAdd(rho + c + nx, q01);
+            unsafeAtomicAdd(rho + c + nx + 1, q11);
+        }
+        return;
+    }
+    const uint32_t nw = wx * wy;
+    for (uint32_t k = lane; k < nw; k += 64) win[k] = 0.;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (head) {
+        const uint32_t o = (ix - X0) + (iy - Y0) * wx;
+        atomicAdd(&win[o], q00);
+        atomicAdd(&win[o + 1], q10);
+        atomicAdd(&win[o + wx], q01);
+        atomicAdd(&win[o + wx + 1], q11);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = lane; k < nw; k += 64) {
+        const double v = win[k];
+        if (v != 0.) {
+            const uint32_t r = k / wx;
+            unsafeAtomicAdd(rho + (X0 + (k - r * wx)) + (size_t)(Y0 + r) * nx, v);
+        }
+    }
+}
+
 // Boris rotation + second half kick of particles.pyx:272-321, from the velocity after the
 // first half kick.
 template <typename T>

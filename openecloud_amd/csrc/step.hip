@@ -156,7 +156,8 @@ __global__ void __launch_bounds__(PB)
                double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
                uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
 {
-    __shared__ double tiles[PB / 64][512];
+    __shared__ double tiles[PB / 64 * 512];
+    __shared__ uint32_t dbox[4];
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t run = runs.block_run(blk);
     const uint64_t sBeg = runs.seg0(run);
@@ -197,8 +198,8 @@ __global__ void __launch_bounds__(PB)
         }
     }
     wave_holes(dead, i, blk, counts, list, misc, nzList);
-    wave_deposit<512>(rho, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
-                      w2.wUU * wq * q, tiles[threadIdx.x >> 6]);
+    block_deposit<PB / 64, 512>(rho, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
+                                w2.wUU * wq * q, tiles, dbox);
 }
 
 // Exclusive prefix of v over the CT threads of the block (wsum: CT/64 LDS words); *tot the sum.
