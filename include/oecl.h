@@ -468,14 +468,19 @@ typedef struct {
                                    the push's gather, or NULL to gather from E */
 } oecl_step_args;
 
+/* Particles per push block (256): batched run segments are multiples of it, and the per-block
+ * run table blockRun of oecl_runs_t has one entry per that many slots.  No reference
+ * counterpart; callers that lay out runs check it against their own constant. */
+uint32_t oecl_push_block_size(void);
+
 uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap);
 
 /* A batched step: R independent buildups (a parameter scan) on one grid, one wall and one
  * direct Poisson operator, all advanced by one call (or one replay of a captured graph).  No
  * counterpart in the reference, which runs one buildup per process.  Run r's particles live in
  * [segOff[r], segOff[r] + nDev[r]) of the one SoA block of oecl_step_args (stride = segOff[R]);
- * its segment [segOff[r], segOff[r + 1]) is a multiple of 256 particles.  blockRun[b] is the
- * run of particles [256 b, 256 b + 256).  Run r's hits use rows [absOff[r], absOff[r + 1]) of
+ * its segment [segOff[r], segOff[r + 1]) is a multiple of oecl_push_block_size() = 256
+ * particles.  blockRun[b] is the run of particles [256 b, 256 b + 256).  Run r's hits use rows [absOff[r], absOff[r + 1]) of
  * absorbed, normals, remaining, nSec, impact (absCap = absOff[R]); secOff holds absOff[R] + R
  * entries (run r's from absOff[r] + r).  Per run, one after another: nDev, stats[8],
  * countHist[histCap], beamScale[nScale], rho, q, phi (npExt each), E (2 npExt).  Emitter: the

@@ -111,3 +111,8 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
     const uint32_t x = b & 7u, k = b >> 3;
     return ((k / XCD_CHUNK) * 8 + x) * XCD_CHUNK + k % XCD_CHUNK;
 }
+
+// Particles per push block.  Batched runs occupy segments that are multiples of it, and the
+// per-block run table (blockRun) has one entry per kPushBlock particle slots: every kernel that
+// reads that table divides slot indices by this constant.
+constexpr uint32_t kPushBlock = 256;

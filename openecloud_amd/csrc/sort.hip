@@ -250,7 +250,7 @@ __global__ void k_run_cell_keys(oecl_grid_t g, Soa<T> p, uint32_t cap, const uin
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cap) return;
-    const uint32_t r = blockRun[i / 256];
+    const uint32_t r = blockRun[i / kPushBlock];
     uint64_t bin = (1ull << cellBits) - 1;
     if (i - segOff[r] < nDev[r]) {
         const Weights<double> w = grid_weights<double>((double)p(0, i), (double)p(1, i), 1. / g.dx, 1. / g.dy,
@@ -271,7 +271,7 @@ OECL_EXPORT int oecl_sort_runs_by_cell(const oecl_grid_t *g, int dtype, const vo
                                        void *work, void *stream)
 {
     if (!g || !data || !spare || !segOff || !blockRun || !nDev || !keys || !perm || !work || R == 0) return OECL_ERR_ARG;
-    if (order < 0 || order > 1 || binShift < 0 || binShift > 8 || stride % 256 || stride > 0xFFFFFFFFull)
+    if (order < 0 || order > 1 || binShift < 0 || binShift > 8 || stride % kPushBlock || stride > 0xFFFFFFFFull)
         return OECL_ERR_ARG;
     if (g->nx > 65536 || g->ny > 65536) return OECL_ERR_ARG;
     const uint32_t cap = (uint32_t)stride;

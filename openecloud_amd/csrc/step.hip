@@ -36,7 +36,7 @@ enum {
 
 namespace {
 
-constexpr uint32_t PB = 256;      // particles per push block
+constexpr uint32_t PB = kPushBlock;  // particles per push block (one deposit tile per block)
 constexpr uint32_t CT = 1024;     // threads of the single-block kernels
 constexpr uint32_t NZMAX = 16384; // push waves with holes sorted in LDS (more: full scan)
 constexpr uint32_t MAXSEC = 10;   // secondaries per impact (Furman M)
@@ -633,6 +633,8 @@ static uint64_t slots_offset(uint64_t capacity, uint32_t absTotal, uint32_t R)
     return ((4 * (nW * (64 + 2) + absTotal + (uint64_t)M_COUNT * R + R + 1)) + 255) / 256 * 256;
 }
 
+OECL_EXPORT uint32_t oecl_push_block_size(void) { return kPushBlock; }
+
 OECL_EXPORT uint64_t oecl_step_work_bytes(uint64_t capacity, uint32_t absCap)
 {
     return slots_offset(capacity, absCap, 1) + 8ull * 6 * MAXSEC * absCap + 1024;
@@ -773,7 +775,7 @@ OECL_EXPORT int oecl_step_run_batch(const oecl_grid_t *g, const oecl_wall_t *wal
         return OECL_ERR_ARG;
     }
     if (a->stride % PB) {
-        oecl_set_error_msg("step batch: the capacity must be a multiple of 256");
+        oecl_set_error_msg("step batch: the capacity must be a multiple of the push block (256)");
         return OECL_ERR_ARG;
     }
     WallDev wd;

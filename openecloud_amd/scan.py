@@ -4,7 +4,7 @@ The reference runs one buildup per process (example01_basic.py); a scan over sec
 emission yield and bunch intensity is R separate runs.  BuildupScan advances R runs that share
 the grid, the wall and the Poisson operator with one library call per step for all of them
 (oecl_step_run_batch), so the device sees R times the work per launch:
-  - every run's particles live in its own segment of one SoA block (a multiple of 256 slots,
+  - every run's particles live in its own segment of one SoA block (a multiple of PB = 256 slots,
     so that each push block belongs to one run);
   - the R Poisson right-hand sides are solved together (the DST products get R times the
     columns);
@@ -77,6 +77,9 @@ class BuildupScan:
             raise ValueError('a scan needs PoissonSolver(method="direct")')
         if hasattr(b0.em, 'secemDesc'):
             raise ValueError('a scan runs the Furman emitter')
+        if int(L.lib().oecl_push_block_size()) != PB:
+            raise L.OeclError('the library pushes %d particles per block, the scan lays out %d'
+                              % (int(L.lib().oecl_push_block_size()), PB))
         self.runs = list(buildups)
         self.R = R = len(buildups)
         self.g, self.ps, self.pb = b0.g, b0.ps, b0.pb

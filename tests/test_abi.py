@@ -64,3 +64,13 @@ def test_invalid_arguments_are_refused_before_any_device_call():
     assert lib.oecl_fcg_solve(None, ctypes.c_uint32(8), ctypes.c_uint32(8), None, None, None, None, None,
                               ctypes.c_double(1e-8), ctypes.c_uint32(10), ctypes.c_uint32(1), ctypes.c_double(0.),
                               ctypes.c_uint32(4), None, None, None, None) == ARG
+
+
+def test_push_block_size_matches_the_scan_layout():
+    """Batched run segments and the per-block run table use the library's push block size; the
+    scan's host layout must use the same."""
+    from openecloud_amd import _lib as L
+    from openecloud_amd.scan import PB
+    lib = L.lib()
+    lib.oecl_push_block_size.restype = ctypes.c_uint32
+    assert lib.oecl_push_block_size() == PB

@@ -76,12 +76,13 @@ def test_capacity_overflow_is_flagged():
 
 def test_scan_segment_overflow_names_the_run():
     from openecloud_amd import _lib as L
-    from openecloud_amd.scan import BuildupScan
+    from openecloud_amd.scan import PB, BuildupScan
     b0 = _buildup(n=3000, seed=70, sey=0.5)
     b1 = _buildup(n=3000, seed=71, sey=3.0, g=b0.g, ps=b0.ps)
-    # segments with no room to grow beyond the rounding to 256: run 1 (many secondaries) overflows
+    # segments with no room to grow beyond the rounding to a push block: run 1 (many
+    # secondaries) overflows
     sc = BuildupScan([b0, b1], [70, 71], growth=1.0, slack=0, maxSteps=200)
-    assert int(sc.segOff[2] - sc.segOff[1]) - 3000 < 256
+    assert int(sc.segOff[2] - sc.segOff[1]) - 3000 < PB
     with pytest.raises(L.OeclError, match='run 1'):
         for _ in range(200):
             sc.step()
