@@ -326,3 +326,60 @@ def test_buildup_with_learned_solve_matches_oracle(orc):
         assert err < 1e-8, (it, err, its)
     print('learned solve in the buildup: iterations', its, 'judge max rel err', ps.judgeMaxErr)
     assert ps.fallbacks == 0 and ps.judgeCount == nsteps and ps.judgeMaxErr < 1e-8, (its, ps.judgeMaxErr)
+
+
+def test_native_step_deposit_paths_match_oracle(orc):
+    """The push's deposit merges run totals in one LDS tile per block when the block's cells are
+    close, in per-wave tiles when each wave's cells are close but the block's are not, and adds
+    them to rho directly otherwise.  Particles laid out to take each path (4 blocks of 256 each:
+    compact blocks, blocks of four far-apart compact waves, scattered blocks), one native step
+    with no wall hits, and the deposited charge against the oracle's deposit of the stepped
+    particles."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = Grid(128, 96, 0.09, 0.07, 1)
+    rng = np.random.default_rng(21)
+    dx, dy = g.getDx(), g.getDy()
+
+    def cloud(cx, cy, m):      # m particles within one cell of (cx, cy)
+        return np.column_stack([cx + rng.uniform(-dx, dx, m), cy + rng.uniform(-dy, dy, m)])
+    pos = []
+    for k in range(4):         # compact blocks: all 4 waves around one point
+        pos.append(cloud(-0.02 + 0.01 * k, 0.005 * k, 256))
+    for k in range(4):         # far-apart waves, each compact (a block box of ~76 x 42 cells)
+        for cx, cy in ((-0.025, -0.015), (0.025, 0.015), (-0.025, 0.015), (0.025, -0.015)):
+            pos.append(cloud(cx + 0.001 * k, cy, 64))
+    for k in range(4):         # scattered
+        r = np.sqrt(rng.uniform(0, 1, 256)) * 0.8
+        t = rng.uniform(0, 2 * np.pi, 256)
+        pos.append(np.column_stack([0.045 * r * np.cos(t), 0.035 * r * np.sin(t)]))
+    xy = np.concatenate(pos)
+    n = xy.shape[0]
+    pd0 = np.zeros((n, 6))
+    pd0[:, :2] = xy
+    pd0[:, 2:5] = rng.normal(0., 1e5, (n, 3))
+    pd0[:, 5] = rng.uniform(0.5, 1.5, n) * 1e5
+    randomGen.seed(5)
+    p = Particles('electrons', 'variableWeight', ordering='fast')
+    p.setParticleData(pd0)
+    p.setBConst([0., 0.5, 0.])
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.6, eHat0=300.)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    beam = GaussianBunchTrain(g, 1e-12, nParticles=1e11, tBunchSpacing=25e-9, radiusSigma=2e-3, nBunches=1)
+    b = Buildup(g, p, pb, em, ps, beam, dt=1e-12, typeBField=1)
+    b.native_setup(absCap=4096, secCap=8192, maxSteps=8)
+    b.step_native()
+    st = b.sync()
+    assert st['absorbed_total'] == 0 and p.getMacroParticleCount() == n
+    pd = p.getParticleData().cpu().numpy()
+    ic, w = orc.calc_grid_weights(pd, dx, dy, g.getLxExt(), g.getLyExt(), g.getNxExt())
+    rho_o = orc.charge_to_grid(pd, ic, w, p.getParticleCharge(), g.getNxExt(), g.getNpExt())
+    rho = b.rho.cpu().numpy()
+    assert np.allclose(rho, rho_o, rtol=1e-13, atol=1e-13 * np.abs(rho_o).max())
