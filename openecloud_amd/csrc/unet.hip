@@ -91,13 +91,23 @@ struct GridIO {
 __device__ __forceinline__ double grid_rms(const double *sc) { return sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.; }
 
 // SRC 1 (the first convolution of an inference pass) stages its input straight from the FP64
-// grid vector g.r instead of an input image; MODE 3 (its last convolution) writes g.z.  Both give
-// bit for bit what the input-image and FP32-output passes give.
+// grid vector g.r instead of an input image; MODE 3 (its last convolution) writes g.z.  SRC 2 (an
+// up level's first convolution in an inference pass) stages up(up) + in, the nearest upsampling
+// of the coarser level's output plus the skip image, instead of reading k_up_add's sum.  All give
+// bit for bit what the unfused passes give.
 constexpr int CONV_SPAN = 64 + 2;  // staged pixels per row
+__device__ __forceinline__ uint32_t add_bf16x2(uint32_t a, uint32_t b)
+{
+    const uint32_t lo = f2bf(bf2f((uint16_t)(a & 0xFFFFu)) + bf2f((uint16_t)(b & 0xFFFFu)));
+    const uint32_t hi = f2bf(bf2f((uint16_t)(a >> 16)) + bf2f((uint16_t)(b >> 16)));
+    return lo | (hi << 16);
+}
+
 template <int MODE, int RW, int SRC = 0>
 __global__ void __launch_bounds__(kThreads)
     k_conv(int H, int W, const uint16_t *__restrict__ in, const uint16_t *__restrict__ wt,
-           const float *__restrict__ bias, void *__restrict__ out, GridIO g)
+           const float *__restrict__ bias, void *__restrict__ out, GridIO g, const uint16_t *__restrict__ up,
+           uint16_t *__restrict__ pooled)
 {
     __shared__ __align__(16) uint16_t tile[RW + 2][CONV_SPAN][C];
     const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4, wv = threadIdx.x >> 6;
@@ -108,8 +118,8 @@ __global__ void __launch_bounds__(kThreads)
     // every load of the thread is issued before the first LDS store
     constexpr int NE = (RW + 2) * CONV_SPAN * 2, NL = (NE + kThreads - 1) / kThreads;
     uint4 v[NL];
-    double rv[SRC ? NL : 1];
-    uint8_t mv[SRC ? NL : 1];
+    double rv[SRC == 1 ? NL : 1];
+    uint8_t mv[SRC == 1 ? NL : 1];
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
         const int e = threadIdx.x + k * kThreads;
@@ -119,6 +129,14 @@ __global__ void __launch_bounds__(kThreads)
         if (SRC == 0) {
             if (e < NE && gy >= 0 && gy < H && gx >= 0 && gx < W)
                 v[k] = *reinterpret_cast<const uint4 *>(in + ((size_t)gy * W + gx) * C + 8 * half);
+        } else if (SRC == 2) {
+            if (e < NE && gy >= 0 && gy < H && gx >= 0 && gx < W) {
+                const uint4 sk = *reinterpret_cast<const uint4 *>(in + ((size_t)gy * W + gx) * C + 8 * half);
+                const uint4 cu =
+                    *reinterpret_cast<const uint4 *>(up + ((size_t)(gy >> 1) * (W >> 1) + (gx >> 1)) * C + 8 * half);
+                v[k] = make_uint4(add_bf16x2(cu.x, sk.x), add_bf16x2(cu.y, sk.y), add_bf16x2(cu.z, sk.z),
+                                  add_bf16x2(cu.w, sk.w));
+            }
         } else {
             // both loads issued unconditionally for in-range nodes; the mask selects afterwards
             const bool ok = e < NE && half == 0 && gy >= 0 && gy < (int)g.ny && gx >= 0 && gx < (int)g.nx;
@@ -127,7 +145,7 @@ __global__ void __launch_bounds__(kThreads)
             mv[k] = ok ? g.mask[i] : (uint8_t)0;
         }
     }
-    if (SRC == 1) {
+    if constexpr (SRC == 1) {
         const double rms = grid_rms(g.sc);
 #pragma unroll
         for (int k = 0; k < NL; ++k)
@@ -176,6 +194,7 @@ __global__ void __launch_bounds__(kThreads)
         }
         return;
     }
+    v4s pe{0, 0, 0, 0}, pen{0, 0, 0, 0};  // the even row's stored values at x and x + 1
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
         const int y = y0 + r;
@@ -193,6 +212,29 @@ __global__ void __launch_bounds__(kThreads)
             o[j] = (short)f2bf(v);
         }
         *reinterpret_cast<v4s *>(static_cast<uint16_t *>(out) + p * C + 4 * cq) = o;
+        if constexpr (RW >= 2) {
+            // pooled != null: also the 2x2 average of the stored values into the next level's
+            // image, as k_pool forms it ((a + b) + c) + d from rows y, y + 1 (this lane's r - 1,
+            // r) and columns x, x + 1 (this lane and the next; W and H are even, y0 too)
+            if (pooled) {
+                v4s on;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) on[j] = (short)__shfl((int)o[j], lane + 1, 64);
+                if (r & 1) {
+                    if (!(x & 1)) {
+                        v4s po;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            po[j] = (short)f2bf(0.25f * (((bf2f((uint16_t)pe[j]) + bf2f((uint16_t)pen[j])) +
+                                                          bf2f((uint16_t)o[j])) + bf2f((uint16_t)on[j])));
+                        *reinterpret_cast<v4s *>(pooled + ((size_t)(y >> 1) * (W >> 1) + (x >> 1)) * C + 4 * cq) = po;
+                    }
+                } else {
+                    pe = o;
+                    pen = on;
+                }
+            }
+        }
     }
 }
 
@@ -771,29 +813,33 @@ int conv_rows(const oecl_unet *u, int H, int W)
 
 template <int MODE, int SRC = 0>
 void launch_conv_mode(int rw, dim3 g, hipStream_t s, int H, int W, const uint16_t *in, const uint16_t *w,
-                      const float *b, void *out, const GridIO &io)
+                      const float *b, void *out, const GridIO &io, const uint16_t *up, uint16_t *pooled)
 {
     const dim3 bl(kThreads);
-    if (rw == 8) hipLaunchKernelGGL((k_conv<MODE, 8, SRC>), g, bl, 0, s, H, W, in, w, b, out, io);
-    else if (rw == 4) hipLaunchKernelGGL((k_conv<MODE, 4, SRC>), g, bl, 0, s, H, W, in, w, b, out, io);
-    else if (rw == 2) hipLaunchKernelGGL((k_conv<MODE, 2, SRC>), g, bl, 0, s, H, W, in, w, b, out, io);
-    else hipLaunchKernelGGL((k_conv<MODE, 1, SRC>), g, bl, 0, s, H, W, in, w, b, out, io);
+    if (rw == 8) hipLaunchKernelGGL((k_conv<MODE, 8, SRC>), g, bl, 0, s, H, W, in, w, b, out, io, up, pooled);
+    else if (rw == 4) hipLaunchKernelGGL((k_conv<MODE, 4, SRC>), g, bl, 0, s, H, W, in, w, b, out, io, up, pooled);
+    else if (rw == 2) hipLaunchKernelGGL((k_conv<MODE, 2, SRC>), g, bl, 0, s, H, W, in, w, b, out, io, up, pooled);
+    else hipLaunchKernelGGL((k_conv<MODE, 1, SRC>), g, bl, 0, s, H, W, in, w, b, out, io, up, nullptr);
 }
 
 // mode 0: ReLU, 1: linear, 2: FP32 channel 0; 3: the same into io.z; 4: ReLU with the input
-// staged from io.r (both on level 0 only).
+// staged from io.r (both on level 0 only); 5: ReLU with the input staged as up(up) + in.
+// pooled (modes 0, 1): also write the 2x2 average of the output there when the tile has two or
+// more rows; returns 1 when it did (the caller pools otherwise), 0 when not, < 0 on error.
 int launch_conv(const oecl_unet *u, int l, int mode, const uint16_t *in, const uint16_t *w, const float *b, void *out,
-                hipStream_t s, const GridIO &io = GridIO{})
+                hipStream_t s, const GridIO &io = GridIO{}, const uint16_t *up = nullptr, uint16_t *pooled = nullptr)
 {
     const int H = u->H >> l, W = u->W >> l;
     const int rw = conv_rows(u, H, W);
     const dim3 g((uint32_t)((H + rw - 1) / rw) * ((W + 63) / 64));  // one block per RW x 64 tile
-    if (mode == 0) launch_conv_mode<0>(rw, g, s, H, W, in, w, b, out, io);
-    else if (mode == 1) launch_conv_mode<1>(rw, g, s, H, W, in, w, b, out, io);
-    else if (mode == 2) launch_conv_mode<2>(rw, g, s, H, W, in, w, b, out, io);
-    else if (mode == 3) launch_conv_mode<3>(rw, g, s, H, W, in, w, b, out, io);
-    else launch_conv_mode<0, 1>(rw, g, s, H, W, in, w, b, out, io);
-    return OECL_OK;
+    uint16_t *pl = (mode == 0 || mode == 1) && rw >= 2 ? pooled : nullptr;
+    if (mode == 0) launch_conv_mode<0>(rw, g, s, H, W, in, w, b, out, io, up, pl);
+    else if (mode == 1) launch_conv_mode<1>(rw, g, s, H, W, in, w, b, out, io, up, pl);
+    else if (mode == 2) launch_conv_mode<2>(rw, g, s, H, W, in, w, b, out, io, up, nullptr);
+    else if (mode == 3) launch_conv_mode<3>(rw, g, s, H, W, in, w, b, out, io, up, nullptr);
+    else if (mode == 4) launch_conv_mode<0, 1>(rw, g, s, H, W, in, w, b, out, io, up, nullptr);
+    else launch_conv_mode<0, 2>(rw, g, s, H, W, in, w, b, out, io, up, nullptr);
+    return pl ? 1 : 0;
 }
 
 const uint16_t *Wc(const oecl_unet *u, int c) { return u->Wb + (size_t)c * WSZ; }
@@ -816,18 +862,24 @@ void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr)
     };
     if (io) launch_conv(u, 0, 4, nullptr, Wc(u, 0), bc(u, 0), A[IA], s, *io);
     else launch_conv(u, 0, 0, A[IX], Wc(u, 0), bc(u, 0), A[IA], s);
-    launch_conv(u, 0, 0, A[IA], Wc(u, 1), bc(u, 1), A[IS], s);
+    // each level's second convolution also pools its output for the next level when it can
+    int pooledNext = launch_conv(u, 0, 0, A[IA], Wc(u, 1), bc(u, 1), A[IS], s, GridIO{}, nullptr, A[7 + IP]);
     for (int l = 1; l <= 3; ++l) {
-        pool(l, A[7 * (l - 1) + IS], A[7 * l + IP]);
+        if (!pooledNext) pool(l, A[7 * (l - 1) + IS], A[7 * l + IP]);
         launch_conv(u, l, 0, A[7 * l + IP], Wc(u, 2 * l), bc(u, 2 * l), A[7 * l + IA], s);
-        launch_conv(u, l, 0, A[7 * l + IA], Wc(u, 2 * l + 1), bc(u, 2 * l + 1), A[7 * l + IS], s);
+        pooledNext = launch_conv(u, l, 0, A[7 * l + IA], Wc(u, 2 * l + 1), bc(u, 2 * l + 1), A[7 * l + IS], s,
+                                 GridIO{}, nullptr, l < 3 ? A[7 * (l + 1) + IP] : nullptr);
     }
     // level 3's output is u3 = A[7*3 + IS]
     for (int l = 2; l >= 0; --l) {
         const uint16_t *below = l == 2 ? A[7 * 3 + IS] : A[7 * (l + 1) + IV];
-        upadd(l, below, A[7 * l + IS], A[7 * l + IT]);
         const int c0 = 8 + 2 * (2 - l);
-        launch_conv(u, l, 0, A[7 * l + IT], Wc(u, c0), bc(u, c0), A[7 * l + IU], s);
+        if (io) {  // inference: the up level's sum is staged by its first convolution
+            launch_conv(u, l, 5, A[7 * l + IS], Wc(u, c0), bc(u, c0), A[7 * l + IU], s, *io, below);
+        } else {
+            upadd(l, below, A[7 * l + IS], A[7 * l + IT]);
+            launch_conv(u, l, 0, A[7 * l + IT], Wc(u, c0), bc(u, c0), A[7 * l + IU], s);
+        }
         if (l > 0) launch_conv(u, l, 0, A[7 * l + IU], Wc(u, c0 + 1), bc(u, c0 + 1), A[7 * l + IV], s);
         else if (io) launch_conv(u, 0, 3, A[IU], Wc(u, 13), bc(u, 13), nullptr, s, *io);
         else launch_conv(u, 0, 2, A[IU], Wc(u, 13), bc(u, 13), u->out, s);
