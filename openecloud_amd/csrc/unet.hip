@@ -847,9 +847,10 @@ const uint16_t *WTc(const oecl_unet *u, int c) { return u->WbT + (size_t)c * WSZ
 const float *bc(const oecl_unet *u, int c) { return u->bm + (size_t)c * C; }
 
 // Forward pass from the input image act[IX] (level 0) to u->out.
-// io != null: an inference pass from the grid vector io->r to io->z (no input image, no FP32
-// output image; the activations are still kept).
-void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr)
+// io != null: a pass from the grid vector io->r to io->z (no FP32 output image).  Without tape
+// it stages the input and the up levels' sums inside the convolutions (inference); with tape
+// (training) it writes the input image and those sums, which the backward pass reads.
+void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr, bool tape = false)
 {
     uint16_t **A = u->act;
     auto pool = [&](int l, const uint16_t *in, uint16_t *out) {  // level l-1 -> l
@@ -860,8 +861,15 @@ void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr)
         const int H = u->H >> l, W = u->W >> l;
         hipLaunchKernelGGL(k_up_add<false>, dim3(nblk((size_t)H * W * 4)), dim3(kThreads), 0, s, H, W, uu, sk, out);
     };
-    if (io) launch_conv(u, 0, 4, nullptr, Wc(u, 0), bc(u, 0), A[IA], s, *io);
-    else launch_conv(u, 0, 0, A[IX], Wc(u, 0), bc(u, 0), A[IA], s);
+    const bool fused = io && !tape;
+    if (fused) {
+        launch_conv(u, 0, 4, nullptr, Wc(u, 0), bc(u, 0), A[IA], s, *io);
+    } else {
+        if (io)
+            hipLaunchKernelGGL(k_net_input, dim3(nblk((size_t)u->H * u->W * 4)), dim3(kThreads), 0, s, io->nx, io->ny,
+                               (int)u->H, (int)u->W, io->mask, io->r, io->sc, A[IX]);
+        launch_conv(u, 0, 0, A[IX], Wc(u, 0), bc(u, 0), A[IA], s);
+    }
     // each level's second convolution also pools its output for the next level when it can
     int pooledNext = launch_conv(u, 0, 0, A[IA], Wc(u, 1), bc(u, 1), A[IS], s, GridIO{}, nullptr, A[7 + IP]);
     for (int l = 1; l <= 3; ++l) {
@@ -874,7 +882,7 @@ void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr)
     for (int l = 2; l >= 0; --l) {
         const uint16_t *below = l == 2 ? A[7 * 3 + IS] : A[7 * (l + 1) + IV];
         const int c0 = 8 + 2 * (2 - l);
-        if (io) {  // inference: the up level's sum is staged by its first convolution
+        if (fused) {  // inference: the up level's sum is staged by its first convolution
             launch_conv(u, l, 5, A[7 * l + IS], Wc(u, c0), bc(u, c0), A[7 * l + IU], s, *io, below);
         } else {
             upadd(l, below, A[7 * l + IS], A[7 * l + IT]);
@@ -1121,13 +1129,13 @@ OECL_EXPORT int oecl_unet_adam(oecl_unet *u, float lr, float beta1, float beta2,
 
 // z = rms(r) * Net(r / rms(r)) on the inside nodes of an nx x ny grid (row-major, mask = inside).
 static int precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r, double *z,
-                        hipStream_t s)
+                        hipStream_t s, bool tape = false)
 {
     const uint32_t np = nx * ny, nbr = std::min(nblk(np), kRedBlocks);
     hipLaunchKernelGGL(k_rms_part, dim3(nbr), dim3(kThreads), 0, s, np, mask, r, u->part);
     hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0);
     const GridIO io{r, z, mask, u->scal, nx, ny};
-    unet_forward(u, s, &io);
+    unet_forward(u, s, &io, tape);
     return OECL_OK;
 }
 
@@ -1186,7 +1194,7 @@ OECL_EXPORT int oecl_unet_train_sample(oecl_unet *u, uint32_t nx, uint32_t ny, c
     hipStream_t s = (hipStream_t)stream;
     const uint32_t np = nx * ny, nb = nblk(np);
     double *z = work, *az = work + np;
-    int rc = precondition(u, nx, ny, mask, q, z, s);  // z, with the input RMS in scal[0..1]
+    int rc = precondition(u, nx, ny, mask, q, z, s, true);  // z, input RMS in scal[0..1], the tape
     if (rc) return rc;
     hipLaunchKernelGGL(k_loss_az, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, z, phi, q, az, u->part);
     hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(kThreads), 0, s, u->part, nb, u->scal, 4);

@@ -225,3 +225,39 @@ def test_training_is_reproducible():
         net.train(ps, pairs, steps=40, lr=5e-4, batch=2, seed=1)
         params.append(net.get_params())
     assert np.array_equal(params[0][0], params[1][0]) and np.array_equal(params[0][1], params[1][1])
+
+
+def test_train_sample_gradients_match_the_image_path():
+    """oecl_unet_train_sample runs the network from the grid vector and keeps the tape (the input
+    image and the up levels' sums) for its backward pass: the gradients of every layer must equal
+    those of the image path (forward_image, then backward_image of the loss gradient formed here
+    as the device forms it), and none may vanish.  The tolerance covers sums taken in another
+    order (rms, the loss's dot products)."""
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = _chamber(45)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    net = UNetPreconditioner(g, seed=2)
+    nx, ny = g.getNxExt(), g.getNyExt()
+    mask = ps.mask.bool()
+    (q, phi), = _pairs(g, ps, 1, seed=8)
+    net.loss(ps, q, phi)                       # one train_sample: zeroed, then this sample's gradients
+    gW1, gb1 = net.get_grads()
+    rms = torch.sqrt((q[mask] ** 2).sum() / mask.sum())
+    pad = (0, net.W - nx, 0, net.H - ny)
+    m2 = torch.nn.functional.pad(mask.reshape(ny, nx).double(), pad)
+    img = torch.zeros((net.H, net.W, 16), dtype=torch.float32, device='cuda')
+    img[..., 0] = (torch.nn.functional.pad((q / rms).reshape(ny, nx), pad) * m2).float()
+    y = net.forward_image(img.to(torch.bfloat16))[:ny, :nx].reshape(-1).double()
+    z = torch.where(mask, rms * y, torch.zeros_like(y))
+    az = ps.applyA(z)
+    a, b, c = float((z * q).sum()), float((z * az).sum()), float((phi * q).sum())
+    dz = torch.where(mask, (2. * a / (b * c)) * ((a / b) * az - q), torch.zeros_like(q))
+    gout = torch.zeros((net.H, net.W, 16), dtype=torch.float32, device='cuda')
+    gout[..., 0] = torch.nn.functional.pad((rms * dz).reshape(ny, nx), pad).float()
+    net.backward_image(gout.to(torch.bfloat16))
+    gW2, gb2 = net.get_grads()
+    for k in range(14):
+        assert np.linalg.norm(gW1[k]) > 0 and np.linalg.norm(gb1[k]) > 0, k
+        assert _rel(gW1[k].astype(np.float64), gW2[k].astype(np.float64)) < 1e-2, (k, _rel(gW1[k], gW2[k]))
+        assert _rel(gb1[k].astype(np.float64), gb2[k].astype(np.float64)) < 1e-2, (k, _rel(gb1[k], gb2[k]))
