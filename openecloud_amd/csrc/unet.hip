@@ -513,16 +513,8 @@ __device__ __forceinline__ double apply_a(const double *__restrict__ cx, const d
 // z (the flexible CG does not either):
 //     L = min_c (c z - phi)^T A (c z - phi) / (phi^T A phi) = 1 - (z^T q)^2 / ((z^T A z)(phi^T q)).
 // With a = z^T q, b = z^T A z, c = phi^T q: dL/dz = (2 a / (b c)) ((a / b) A z - q), and
-// z = rms * y on inside nodes.  First pass: z.  Second: A z and the partials (b, c, a).
-// Third: dL/dy = rms dL/dz into channel 0 of the gradient image.
-__global__ void k_loss_z(uint32_t nx, uint32_t np, int W, const uint8_t *__restrict__ mask, const float *__restrict__ y,
-                         const double *__restrict__ sc, double *__restrict__ z)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const double rms = sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.;
-    z[i] = mask[i] ? rms * (double)y[(size_t)(i / nx) * W + i % nx] : 0.;
-}
+// z = rms * y on inside nodes (the forward pass writes z).  First pass: A z and the partials
+// (b, c, a).  Second: dL/dy = rms dL/dz into channel 0 of the gradient image.
 
 __global__ void __launch_bounds__(kThreads)
     k_loss_az(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
