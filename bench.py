@@ -47,8 +47,9 @@ def parse():
     ap.add_argument('--calib-copy', action='store_true',
                     help='run one plain 1 GiB device copy first (HBM counter calibration)')
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
-    ap.add_argument('--graph', action='store_true',
-                    help='replay the steps between two sorts as one captured HIP graph')
+    ap.add_argument('--eager', action='store_true',
+                    help='dipole: launch every step on its own (default: replay the steps between two '
+                         'sorts as one captured HIP graph)')
     ap.add_argument('--sort-every', type=int, default=50,
                     help='re-order particles by cell every this many steps (0: never)')
     ap.add_argument('--train-steps', type=int, default=8000, help='learned: Adam steps')
@@ -213,15 +214,16 @@ def build_scan(R, dtype, maxSteps):
     return g, ps, runs, scan, dict(sey=sey, ppb=ppb, n=nmp)
 
 
-def compare_f32(a, n, b64, it0):
+def compare_f32(a, n, b64, extra):
     """The same seeded workload with FP32 particle storage (arithmetic stays FP64), stepped
-    exactly like the FP64 run (warm-up, the timed steps, sorts); its throughput and the
-    differences of the final state from the FP64 run: cloud density (charge on the grid,
-    relative L2), total charge and macro-particle count."""
+    exactly like the FP64 run (warm-up, the timed steps as graph replays or eager launches,
+    sorts, then `extra` eager steps without a sort); its throughput and the differences of the
+    final state from the FP64 run: cloud density (charge on the grid, relative L2), total charge
+    and macro-particle count."""
     import torch
-    del it0
     g, p, pb, em, ps, b = build_case(n, 'f32')
-    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4), maxSteps=a.warmup + a.steps + 40)
+    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4),
+                   maxSteps=a.warmup + a.steps + extra + 40)
     b.sort_particles()
     for k in range(a.warmup):
         b.step_native()
@@ -230,17 +232,30 @@ def compare_f32(a, n, b64, it0):
             b.sort_particles()
     b.sync()
     i0 = b.it
+    if not a.eager:
+        per = a.sort_every if a.sort_every else a.steps
+        b.capture_native(per)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(a.steps):
-        b.step_native()
-        if a.sort_every and k % a.sort_every == a.sort_every - 1:
-            b.sync()
-            b.sort_particles()
+    if not a.eager:
+        for k in range(a.steps // per):
+            b.replay()
+            if a.sort_every:
+                b.sync()
+                b.sort_particles(keepBuffers=True)
+    else:
+        for k in range(a.steps):
+            b.step_native()
+            if a.sort_every and k % a.sort_every == a.sort_every - 1:
+                b.sync()
+                b.sort_particles()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     b.sync()
     rate = float(sum(b.counts(i0, i0 + a.steps))) / wall
+    for k in range(extra):
+        b.step_native()
+    b.sync()
     b64.deposit_all()
     b.deposit_all()
     r64, r32 = b64.rho, b.rho
@@ -747,11 +762,11 @@ def main():
         L.check(L.lib().oecl_event_create(ctypes.byref(e1)))
         evs.append((e0, e1))
     it0 = b.it
-    if a.graph:
+    if not a.eager:
         # steps between two sorts as one HIP graph; the sort keeps the particle buffers
         per = a.sort_every if a.sort_every else a.steps
         if a.steps % per:
-            print('--graph needs --steps to be a multiple of --sort-every', file=sys.stderr)
+            print('graph replay needs --steps to be a multiple of --sort-every (or --eager)', file=sys.stderr)
             sys.exit(2)
         b.capture_native(per)
         torch.cuda.synchronize()
@@ -765,7 +780,6 @@ def main():
         t1 = time.perf_counter()
         # push timing of the same kernel: a few eager steps with events after the timed region
         nev = min(20, len(evs))
-        b.maxSteps += nev
         for k in range(nev):
             b._args.evPushBegin = evs[k][0].value
             b._args.evPushEnd = evs[k][1].value
@@ -850,7 +864,7 @@ def main():
                      'bytes_per_launch': avg_bytes},
     }
     if not a.no_f32_compare and a.dtype == 'f64':
-        out['f32_vs_f64'] = compare_f32(a, n, b, it0)
+        out['f32_vs_f64'] = compare_f32(a, n, b, b.it - it0 - a.steps)
     if not a.no_cpu_baseline:
         cb = cpu_baseline(a.cpu_seconds)
         out['cpu_baseline'] = {k: cb[k] for k in ('value', 'unit', 'cores', 'kind', 'sample')}

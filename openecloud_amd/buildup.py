@@ -247,6 +247,7 @@ class Buildup:
         self._stepDev = torch.tensor([self.it], dtype=torch.int32, device=dev)
         self._stats = torch.zeros(8, dtype=torch.int32, device=dev)
         nTab = self.it + self.maxSteps   # tables are indexed by the absolute step
+        self._tabEnd = nTab              # steps at or past this index would read past the tables
         self._hist = torch.zeros(nTab, dtype=torch.int32, device=dev)
         self._stepwork = torch.empty(int(L.lib().oecl_step_work_bytes(L.u64(cap), L.u32(absCap))) // 8 + 64,
                                      dtype=torch.float64, device=dev)
@@ -318,13 +319,12 @@ class Buildup:
         a.Eil = self._eil.data_ptr()
         a.work = self._stepwork.data_ptr()
         self._args = a
-        self._native_it0 = self.it
         if not self._rho_valid:
             self.deposit_all()
 
     def step_native(self):
         """One full step in one library call; counts stay on the device (call sync())."""
-        if self.it - self._native_it0 >= self.maxSteps:
+        if self.it >= self._tabEnd:
             raise L.OeclError('native step: maxSteps exhausted; call native_setup again')
         L.check(L.lib().oecl_step_run(ctypes.byref(self.gd), ctypes.byref(self.pb.desc),
                                       ctypes.byref(self._prm), self.ps._fp,
@@ -352,7 +352,7 @@ class Buildup:
 
     def replay(self):
         """Runs the captured steps (capture_native)."""
-        if self.it - self._native_it0 + self._graphSteps > self.maxSteps:
+        if self.it + self._graphSteps > self._tabEnd:
             raise L.OeclError('native step: maxSteps exhausted; call native_setup again')
         self._graph.replay()
         self.it += self._graphSteps
