@@ -317,6 +317,105 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// The same Thomas solve for batched right-hand sides with coalesced loads: a block takes QB
+// consecutive x modes (lanes run over the modes, so a row's QB entries are one load) and
+// splits the by rows into S contiguous segments of L rows, one per thread.  Each thread
+// composes its segment's affine recurrence, an LDS scan over the S segments of each mode
+// gives every segment its entry value, and the segment is then applied from registers; the
+// backward sweep is done the same way from the other end.  Block QB * S threads (QB, S
+// powers of two, S * L >= by).  Deterministic (fixed segments and scan order).
+template <int L>
+__global__ void __launch_bounds__(1024)
+    k_thomas_seg(uint32_t bx, uint32_t by, uint32_t lqb, uint32_t S, double cyu, const double *__restrict__ invw,
+                 const double *__restrict__ r, double *__restrict__ u)
+{
+    __shared__ double sA[1024], sB[1024];
+    const uint32_t t = threadIdx.x, QB = 1u << lqb, qq = t & (QB - 1), sg = t >> lqb;
+    const uint32_t q = blockIdx.x * QB + qq;
+    const bool live = q < bx;
+    const size_t nb = (size_t)bx * by;
+    const double *rb = r + (size_t)blockIdx.y * nb;
+    double *ub = u + (size_t)blockIdx.y * nb;
+    const uint32_t j0 = sg * L;
+    double iw[L], g[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const uint32_t j = j0 + k;
+        const size_t o = q + (size_t)j * bx;
+        const bool in = live && j < by;
+        iw[k] = in ? invw[o] : 0.;
+        g[k] = in ? rb[o] : 0.;
+    }
+    // forward: g_j = a_j g_{j-1} + b_j with a_j = cyu iw_j, b_j = r_j iw_j; a segment maps its
+    // entry value x to A x + B
+    double A = 1., B = 0.;
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const double a = cyu * iw[k];
+        A *= a;
+        B = fma(a, B, g[k] * iw[k]);
+    }
+    sA[t] = A;
+    sB[t] = B;
+    __syncthreads();
+    for (uint32_t d = 1; d < S; d <<= 1) {
+        double Ao = 1., Bo = 0.;
+        if (sg >= d) {
+            Ao = sA[t - (d << lqb)];
+            Bo = sB[t - (d << lqb)];
+        }
+        __syncthreads();
+        if (sg >= d) {
+            B = fma(A, Bo, B);
+            A *= Ao;
+            sA[t] = A;
+            sB[t] = B;
+        }
+        __syncthreads();
+    }
+    double x = sg ? sB[t - QB] : 0.;   // g before the first row is zero
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        x = fma(cyu * iw[k], x, g[k] * iw[k]);
+        g[k] = x;
+    }
+    // backward: u_j = e_j u_{j+1} + g_j with e_j = cyu iw_j (rows past by: e = g = 0)
+    A = 1.;
+    B = 0.;
+#pragma unroll
+    for (int k = L - 1; k >= 0; --k) {
+        const double e = cyu * iw[k];
+        B = fma(e, B, g[k]);
+        A *= e;
+    }
+    sA[t] = A;
+    sB[t] = B;
+    __syncthreads();
+    for (uint32_t d = 1; d < S; d <<= 1) {
+        double Ao = 1., Bo = 0.;
+        if (sg + d < S) {
+            Ao = sA[t + (d << lqb)];
+            Bo = sB[t + (d << lqb)];
+        }
+        __syncthreads();
+        if (sg + d < S) {
+            B = fma(A, Bo, B);
+            A *= Ao;
+            sA[t] = A;
+            sB[t] = B;
+        }
+        __syncthreads();
+    }
+    x = sg + 1 < S ? sB[t + QB] : 0.;   // u past the last row is zero
+#pragma unroll
+    for (int k = L - 1; k >= 0; --k) {
+        x = fma(cyu * iw[k], x, g[k]);
+        const uint32_t j = j0 + k;
+        if (live && j < by) ub[q + (size_t)j * bx] = x;
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double s)
 {
 #pragma unroll
@@ -453,6 +552,25 @@ std::vector<double> dst1(uint32_t n)
 int launch_thomas(const oecl_fastpoisson *h, bool unit, const double *r, double *u, uint32_t a0, uint32_t cnt,
                   hipStream_t s)
 {
+    if (!unit && h->by <= 2048) {
+        // segmented solve: S segments of L <= 16 rows, QB modes a block, at least 4 modes (32-byte
+        // row pieces) and 256 blocks where the box allows
+        uint32_t S = 1;
+        while (S * 16 < h->by) S <<= 1;
+        const uint32_t Lr = (h->by + S - 1) / S;
+        uint32_t lqb = 2;
+        while ((S << (lqb + 1)) <= 256 && lqb < 4) ++lqb;
+        while (lqb > 2 && (size_t)((h->bx + (1u << lqb) - 1) >> lqb) * cnt < 256) --lqb;
+        const dim3 g((h->bx + (1u << lqb) - 1) >> lqb, cnt), b(S << lqb);
+#define OECL_THOMAS_SEG(LL) \
+    hipLaunchKernelGGL((k_thomas_seg<LL>), g, b, 0, s, h->bx, h->by, lqb, S, h->cyu, h->invw, r, u)
+        if (Lr <= 4) OECL_THOMAS_SEG(4);
+        else if (Lr <= 8) OECL_THOMAS_SEG(8);
+        else if (Lr <= 12) OECL_THOMAS_SEG(12);
+        else OECL_THOMAS_SEG(16);
+#undef OECL_THOMAS_SEG
+        return OECL_OK;
+    }
     const dim3 g((h->bx + 3) / 4, cnt), b(256);
     const uint32_t c = (h->by + 63) / 64;
 #define OECL_THOMAS(C)                                                                                   \
