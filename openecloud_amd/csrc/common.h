@@ -112,6 +112,16 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n)
     return ((k / XCD_CHUNK) * 8 + x) * XCD_CHUNK + k % XCD_CHUNK;
 }
 
+// The same mapping with chunks of ch blocks (ch >= 1), for kernels whose neighbouring blocks
+// share cache lines in small groups.
+__device__ __forceinline__ uint32_t xcd_block_chunk(uint32_t b, uint32_t n, uint32_t ch)
+{
+    const uint32_t full = n / (8 * ch) * (8 * ch);
+    if (b >= full) return b;
+    const uint32_t x = b & 7u, k = b >> 3;
+    return ((k / ch) * 8 + x) * ch + k % ch;
+}
+
 // Particles per push block.  Batched runs occupy segments that are multiples of it, and the
 // per-block run table (blockRun) has one entry per kPushBlock particle slots: every kernel that
 // reads that table divides slot indices by this constant.

@@ -331,7 +331,9 @@ __global__ void __launch_bounds__(1024)
 {
     __shared__ double sA[1024], sB[1024];
     const uint32_t t = threadIdx.x, QB = 1u << lqb, qq = t & (QB - 1), sg = t >> lqb;
-    const uint32_t q = blockIdx.x * QB + qq;
+    // blocks whose modes share a 128-byte row piece (16 modes) run on one XCD, so its L2 holds
+    // the line for all of them
+    const uint32_t q = xcd_block_chunk(blockIdx.x, gridDim.x, QB < 16 ? 16 / QB : 1) * QB + qq;
     const bool live = q < bx;
     const size_t nb = (size_t)bx * by;
     const double *rb = r + (size_t)blockIdx.y * nb;
@@ -553,13 +555,13 @@ int launch_thomas(const oecl_fastpoisson *h, bool unit, const double *r, double 
                   hipStream_t s)
 {
     if (!unit && h->by <= 2048) {
-        // segmented solve: S segments of L <= 16 rows, QB modes a block, at least 4 modes (32-byte
-        // row pieces) and 256 blocks where the box allows
+        // segmented solve: S segments of L <= 16 rows, QB modes a block (4 to 16: 32- to 128-byte
+        // row pieces), at most 1024 threads, and at least 256 blocks where the box allows
         uint32_t S = 1;
         while (S * 16 < h->by) S <<= 1;
         const uint32_t Lr = (h->by + S - 1) / S;
         uint32_t lqb = 2;
-        while ((S << (lqb + 1)) <= 256 && lqb < 4) ++lqb;
+        while ((S << (lqb + 1)) <= 1024 && lqb < 4) ++lqb;
         while (lqb > 2 && (size_t)((h->bx + (1u << lqb) - 1) >> lqb) * cnt < 256) --lqb;
         const dim3 g((h->bx + (1u << lqb) - 1) >> lqb, cnt), b(S << lqb);
 #define OECL_THOMAS_SEG(LL) \
