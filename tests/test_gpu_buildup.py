@@ -26,7 +26,9 @@ def _oracle_loop(orc, g, pd, nsteps, dt, typeB, bconst, profile, beam_scale, P, 
         s = beam_scale(it * dt)
         E = ops.solve(rho + s * profile if s else rho)
         eat = orc.field_to_particles(E, ic, w, nx, npx, 2)
-        pd = orc.boris_push(pd, eat, dt, qm, typeB, bconst)
+        # typeB 2: bconst is the field on the grid, [Bx | By | Bz] (3 npExt), gathered here
+        bat = orc.field_to_particles(bconst, ic, w, nx, npx, 3) if typeB == 2 else bconst
+        pd = orc.boris_push(pd, eat, dt, qm, typeB, bat)
         ins, pd = orc.index_inside(pd, 'ellip', g.getLx(), g.getLy())
         ab = orc.save_absorbed(pd, ins)
         sec = np.zeros((0, 6))
@@ -383,3 +385,48 @@ def test_native_step_deposit_paths_match_oracle(orc):
     rho_o = orc.charge_to_grid(pd, ic, w, p.getParticleCharge(), g.getNxExt(), g.getNpExt())
     rho = b.rho.cpu().numpy()
     assert np.allclose(rho, rho_o, rtol=1e-13, atol=1e-13 * np.abs(rho_o).max())
+
+
+def test_native_step_grid_b_matches_oracle(orc):
+    """B on the grid (quadrupole plus a skew term, typeBField 2, gathered at each particle like
+    E): the one-call native step against the oracle loop, FP64 to 1e-11 of each column's
+    scale (a swapped or dropped B component, with fields of ~2 T, would differ at order one)."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.magneticField import multipoleExpansion
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.furman_oracle import material
+    g = Grid(64, 48, 0.09, 0.07, 1)
+    bgrid = np.asarray(multipoleExpansion(g, [0., 50.], [0., 3.]), np.float64)
+    n = 20000
+    pd0 = random_particles(g, n, seed=21, wall='ellip', vscale=3e6)
+    pd0[:, 5] = 5e4
+    dt = 2e-11
+    randomGen.seed(91)
+    key = randomGen.key()
+    p = Particles('electrons', 'variableWeight', ordering='fast')
+    p.setParticleData(pd0)
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.6, eHat0=300.)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    beam = GaussianBunchTrain(g, dt, nParticles=1e11, tBunchSpacing=25e-9, radiusSigma=2e-3,
+                              nBunches=1, bunchLengthSigma=0.003)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=2, bAtGridPoints=bgrid)
+    b.native_setup(absCap=8192, secCap=16384, maxSteps=64)
+    nsteps = 10
+    ref = _oracle_loop(orc, g, pd0.copy(), nsteps, dt, 2, bgrid, beam.getTransversalProfile(),
+                       beam.getScale, material('copper', 1.6, eHat0=300.), key, n, rule='fast')
+    for it in range(nsteps):
+        b.step_native()
+        st = b.sync()
+        got = p.getParticleData().cpu().numpy()
+        assert got.shape == ref[it].shape, (it, got.shape, ref[it].shape)
+        scale = np.abs(ref[it]).max(axis=0)
+        err = (np.abs(got - ref[it]) / scale).max()
+        assert err < 1e-11, (it, err)
+    assert st['absorbed_total'] > 0 and st['emitted_total'] > 0
