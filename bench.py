@@ -1,20 +1,25 @@
-"""Buildup benchmark of the plain GPU PIC step on one MI355X.
+"""Buildup benchmark of the GPU PIC step on one MI355X.
 
-Workload (the elliptical-chamber dipole buildup): elliptical cut-cell chamber with semi-axes
-45 mm x 35 mm on a 512 x 384 grid, uniform dipole By = 0.5 T, 2e6 macro-electrons loaded
-uniformly (1 eV Maxwellian, 1e9 electrons per metre in total), Gaussian bunches with
-sigma 1.5 mm, 1.15e11 protons per bunch, 25 ns spacing, 4 bunches, dt = 5e-12 s, Furman
-copper wall with delta_max = 1.3.  A 'step' is one full time step of the buildup loop:
-Poisson solve with the beam, gather + Boris push + wall test + deposit, absorption, wall
-interaction, Furman emission and compaction.
+Default workload (the largest single-GPU configuration): a 2048 x 2048 grid of an elliptical
+90 x 70 mm chamber with a 50 T/m quadrupole on the grid (typeBField 2), 2e7 FP32
+macro-electrons (1 eV Maxwellian, 1e9 electrons per metre), 20 Gaussian bunches of 1.15e11
+protons with sigma 1 mm at 25 ns, dt = 2e-12 s, Furman copper wall with delta_max = 1.5, and
+localRandom particle management whenever the count leaves [0.95, 1.5] x 2e7 (checked every
+sort interval, as the reference checks every 50 steps).  A 'step' is one full time step of the
+buildup loop: Poisson solve with the beam, gather + Boris push + wall test + deposit,
+absorption, wall interaction, Furman emission and compaction.
 
-Prints ONE JSON line: particle-steps per second (sum over timed steps of the live macro-
-particle count divided by the wall time), with the fused push kernel's roofline and the
-CPU baseline beside it.
+Beside it, the same line reports the plain GPU step on the elliptical dipole buildup (512 x 384
+grid, 2e6 FP64 macro-electrons, By = 0.5 T) and the Poisson solves/s of both grids.
+
+Prints ONE JSON line: particle-steps per second (sum over the timed steps of the live
+macro-particle count divided by the wall time), with the push kernel's roofline and the CPU
+baseline beside it.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -22,45 +27,57 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'push_hbm_traffic_ellip512x384_2e6_f64.md')
+WORKLOADS = ('full', 'dipole', 'scan', 'learned')
 
 
-def measured_traffic(dtype, n):
-    """HBM bytes per push launch from the committed counter summary (FP64, 2e6 particles), or
-    None when the summary does not cover this run's particle type and count."""
-    if dtype != 'f64' or n != 2000000 or not os.path.exists(TRAFFIC_FILE):
+def profile_traffic(name):
+    """HBM bytes per push launch from a committed counter summary under profiles/ (the line
+    'traffic_bytes_per_launch: <bytes>'), or None when there is none for this workload."""
+    path = os.path.join(ROOT, 'profiles', name)
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
-        if line.startswith('traffic_bytes_per_launch_f64:'):
+    for line in open(path):
+        if line.startswith('traffic_bytes_per_launch:'):
             return float(line.split(':')[1])
     return None
 
 
-def parse():
+TRAFFIC_PROFILES = {('dipole', 'f64', 2000000): 'push_hbm_traffic_ellip512x384_2e6_f64.md',
+                    ('scan', 'f32', 64): 'push_hbm_traffic_scan_64runs_256sq_f32.md'}
+
+
+def parse(argv=None):
+    """Command line, checked completely before anything touches the GPU."""
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=200)
-    ap.add_argument('--warmup', type=int, default=20)
-    ap.add_argument('--particles', type=float, default=2e6)
-    ap.add_argument('--dtype', default='f64', choices=['f64', 'f32'])
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--particles', type=float, default=None,
+                    help='macro-particles (full: 2e7, dipole: 2e6)')
+    ap.add_argument('--dtype', default=None, choices=['f64', 'f32'],
+                    help='particle storage (full and scan: f32, dipole: f64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-dipole', action='store_true', help='full: skip the dipole line beside it')
     ap.add_argument('--calib-copy', action='store_true',
                     help='run one plain 1 GiB device copy first (HBM counter calibration)')
-    ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
     ap.add_argument('--eager', action='store_true',
                     help='dipole: launch every step on its own (default: replay the steps between two '
                          'sorts as one captured HIP graph)')
     ap.add_argument('--sort-every', type=int, default=50,
                     help='re-order particles by cell every this many steps (0: never)')
+    ap.add_argument('--target', type=float, default=None,
+                    help='full: particle-management target count (default: the loaded count)')
     ap.add_argument('--train-steps', type=int, default=8000, help='learned: Adam steps')
     ap.add_argument('--pairs', type=int, default=192, help='learned: training pairs')
     ap.add_argument('--train-pairs', choices=('operator', 'buildup'), default='operator',
                     help='learned: train on (A phi, phi) with phi random at 0-64 cell scales, or on the '
                          'buildup dumps (half) and white-noise charges (half) with phi from the direct solve')
     ap.add_argument('--grid', type=int, default=1024, help='learned: grid cells per side')
-    ap.add_argument('--workload', default='dipole', choices=['dipole', 'scan', 'learned', 'full'],
-                    help='dipole: the elliptical dipole buildup (default); scan: 64 buildups '
-                         'stepped together (delta_max and intensity scan)')
+    ap.add_argument('--workload', default='full', choices=WORKLOADS,
+                    help='full: the 2048^2 quadrupole buildup with particle management (default); '
+                         'dipole: the 512x384 dipole buildup; scan: 64 buildups stepped together; '
+                         'learned: the learned-preconditioner Poisson solve on a 1024^2 cut-cell grid')
     ap.add_argument('--runs', type=int, default=64, help='scan: number of buildups')
     ap.add_argument('--scan-start', type=int, default=700,
                     help='scan: the timed steps start at this step of the simulation (the first bunch passage, '
@@ -70,10 +87,133 @@ def parse():
                     help='scan: sortEvery,order,binShift to use instead of autotuning')
     ap.add_argument('--no-f32-compare', action='store_true',
                     help='dipole: skip the FP32-storage run compared against the FP64 one')
-    return ap.parse_args()
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        ap.exit(2, 'bench.py runs on one GPU only: this build has no multi-GPU path (the sharding '
+                   'design is described with the build); got --gpus %d\n' % a.gpus)
+    if a.steps < 1:
+        ap.error('--steps must be at least 1')
+    if a.warmup < 0 or a.sort_every < 0:
+        ap.error('--warmup and --sort-every must not be negative')
+    if a.particles is not None and a.particles < 1:
+        ap.error('--particles must be positive')
+    if a.scan_config is not None:
+        try:
+            v = [int(x) for x in a.scan_config.split(',')]
+        except ValueError:
+            v = []
+        if len(v) != 3 or v[0] < 1 or v[1] not in (0, 1) or not 0 <= v[2] <= 8:
+            ap.error('--scan-config is sortEvery,order,binShift (sortEvery >= 1, order 0 or 1, binShift 0-8)')
+    if a.dtype is None:
+        a.dtype = 'f64' if a.workload == 'dipole' else 'f32'
+    if a.particles is None:
+        a.particles = 2e6 if a.workload == 'dipole' else 2e7
+    return a
 
 
-def build_case(n, dtype):
+def step_plan(steps, sortEvery, graph):
+    """The order of work for `steps` buildup steps with a cell sort after every sortEvery-th
+    step (0: never).  Entries: ('graph', k) replays a captured graph of k steps, ('step',) runs
+    one step on its own, ('sort',) synchronises and sorts.  With graph, whole sort intervals
+    are graph replays and the remaining steps run on their own; the sorts fall after the same
+    steps either way, so both plans advance the buildup identically."""
+    steps, sortEvery = int(steps), int(sortEvery)
+    plan = []
+    if graph and sortEvery and steps >= sortEvery:
+        for _ in range(steps // sortEvery):
+            plan += [('graph', sortEvery), ('sort',)]
+        plan += [('step',)] * (steps % sortEvery)
+    elif graph and not sortEvery:
+        plan = [('graph', steps)]
+    else:
+        for k in range(steps):
+            plan.append(('step',))
+            if sortEvery and k % sortEvery == sortEvery - 1:
+                plan.append(('sort',))
+    return plan
+
+
+def plan_graph_steps(plan):
+    """Steps per captured graph in a plan (0 when it replays none)."""
+    return max([e[1] for e in plan if e[0] == 'graph'], default=0)
+
+
+def plan_steps(plan):
+    return sum(e[1] if e[0] == 'graph' else 1 for e in plan if e[0] != 'sort')
+
+
+def run_plan(b, plan, events=None, onSort=None):
+    """Runs a step plan on a native Buildup.  events: a list of (begin, end) HIP events, one
+    pair per eager step, recorded around that step's push kernel.  onSort: called after each
+    synchronisation, before the sort (particle management).  Returns the eager steps run."""
+    keep = plan_graph_steps(plan) > 0
+    k = 0
+    for e in plan:
+        if e[0] == 'graph':
+            b.replay()
+        elif e[0] == 'step':
+            if events is not None and k < len(events):
+                b._args.evPushBegin, b._args.evPushEnd = events[k][0].value, events[k][1].value
+            b.step_native()
+            b._args.evPushBegin = b._args.evPushEnd = None
+            k += 1
+        else:
+            b.sync()
+            if onSort is not None:
+                onSort()
+            b.sort_particles(keepBuffers=keep)
+    return k
+
+
+def make_events(n):
+    import ctypes
+    from openecloud_amd import _lib as L
+    evs = []
+    for _ in range(n):
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        L.check(L.lib().oecl_event_create(ctypes.byref(e0)))
+        L.check(L.lib().oecl_event_create(ctypes.byref(e1)))
+        evs.append((e0, e1))
+    return evs
+
+
+def event_us(evs):
+    """Elapsed microseconds of each event pair (and releases them)."""
+    import ctypes
+    from openecloud_amd import _lib as L
+    out = []
+    for e0, e1 in evs:
+        ms = ctypes.c_float()
+        L.check(L.lib().oecl_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+        out.append(ms.value * 1e3)
+        L.lib().oecl_event_destroy(e0)
+        L.lib().oecl_event_destroy(e1)
+    return out
+
+
+def host_info():
+    model = platform.processor() or ''
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = os.cpu_count()
+    return {'nproc': cpus, 'cpu_model': model}
+
+
+# ---------------------------------------------------------------------------------------------
+# workloads
+
+def build_dipole(n, dtype):
+    """The elliptical dipole buildup: 45 x 35 mm semi-axes on a 512 x 384 grid, By = 0.5 T,
+    n macro-electrons, Gaussian bunches sigma 1.5 mm, 1.15e11 ppb, 4 bunches, dt 5e-12 s,
+    Furman copper delta_max 1.3, primary photoelectrons 1e-3 per proton per metre."""
     import torch
     from openecloud_amd import randomGen
     from openecloud_amd.beam import GaussianBunchTrain
@@ -96,81 +236,396 @@ def build_case(n, dtype):
                               nBunches=4)
     homoLoader(g, p, pb, int(n), 1e9 / n, 4.194e5)
     p.reserve(int(n * 2))
-    # primary photoelectrons, 1e-3 per proton per metre, 1e5 macro-particles per bunch
     photo = PhotoEmitter(pb, p, beam, photoYield=1e-3, macroPerBunch=1e5, energyMean=7., energySigma=5.)
     b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1, photoEmitter=photo)
     return g, p, pb, em, ps, b
 
 
-def cpu_baseline(seconds):
-    """The oracle's C restatement of the reference loops (AoS, one core) on a bounded sample:
-    particle loop (weights, deposit, gather, constant-B Boris push, elliptical wall test) over
-    a 2e5-particle sample of the same grid, plus the reference's SuperLU back-substitution on
-    the same 512 x 384 system; extrapolated to 2e6 particles per step."""
-    import ctypes
-    import numpy as np
-    import oracle
-    from oracle.grid_oracle import OraclePoisson
+def build_full(n, log):
+    """The 2048 x 2048 quadrupole buildup (see the module docstring)."""
+    import torch
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
     from openecloud_amd.grid import Grid
-    g = Grid(512, 384, 0.09, 0.07, 1)
-    rng = np.random.default_rng(0)
-    ns = 200000
+    from openecloud_amd.magneticField import multipoleExpansion
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter, homoLoader
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    dt = 2e-12
+    t0 = time.perf_counter()
+    randomGen.seed(0)
+    g = Grid(2048, 2048, 0.09, 0.07, 1)
+    bq = multipoleExpansion(g, [0., 50.])
+    log('full: grid and field %.1f s' % (time.perf_counter() - t0))
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    log('full: direct solver %.1f s (%d capacitance rows)' % (time.perf_counter() - t0, ps.capacitanceRows))
+    p = Particles('electrons', 'variableWeight', dtype=torch.float32, ordering='fast')
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.5)
+    beam = GaussianBunchTrain(g, dt, nParticles=1.15e11, tBunchSpacing=25e-9, radiusSigma=1e-3, nBunches=20)
+    homoLoader(g, p, pb, n, 1e9 / n, 4.194e5)
+    p.reserve(int(1.6 * n) + 4096)
+    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=2, bAtGridPoints=bq)
+    return g, bq, p, pb, em, ps, b
+
+
+def poisson_rate(ps, q, nsol):
+    import torch
+    ps.solve(q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsol):
+        ps.solve(q)
+    torch.cuda.synchronize()
+    return nsol / (time.perf_counter() - t0)
+
+
+def measure_full(a, log):
+    """Times a.steps steps of the 2048^2 quadrupole buildup (see the module docstring) after
+    a.warmup steps; HIP events around every push kernel."""
+    import torch
+    from openecloud_amd import _lib as L
+    from openecloud_amd.diagnostics import weight_sum
+    n = int(a.particles)
+    target = int(a.target) if a.target else n
+    g, bq, p, pb, em, ps, b = build_full(n, log)
+    b.native_setup(absCap=max(65536, n // 32), secCap=max(131072, n // 16),
+                   maxSteps=a.warmup + a.steps + 20)
+    b.sort_particles()
+    mgmt = []
+
+    def manage():
+        ops = b.manage(target, upper=1.5, lower=0.95)
+        if ops is not None:
+            mgmt.append(dict(ops, step=b.it, count_after=p.macroParticleCount))
+
+    run_plan(b, step_plan(a.warmup, a.sort_every, False), onSort=manage)
+    b.sync()
+    plan = step_plan(a.steps, a.sort_every, False)
+    evs = make_events(a.steps)
+    it0 = b.it
+    st0 = b.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_plan(b, plan, events=evs, onSort=manage)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = b.sync()
+    counts = b.counts(it0, it0 + a.steps)
+    push = event_us(evs)
+    avg_us = sum(push) / len(push)
+    npx = g.getNpExt()
+    bytes_per = sum(11 * 4 * c + 48 * npx for c in counts) / len(counts)
+    achieved = bytes_per / (avg_us * 1e-6) / 1e9
+    hits = (st['absorbed_total'] - st0['absorbed_total']) / a.steps
+    # one forced management event (the 1.5x trigger's reduction from the current count), timed
+    nBefore = p.macroParticleCount
+    q0 = weight_sum(p)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ops = b.manage(int(nBefore / 1.5), upper=1.0, lower=0.)
+    torch.cuda.synchronize()
+    t_merge = time.perf_counter() - t1
+    q1 = weight_sum(p)
+    solves = poisson_rate(ps, b.rho.clone(), 20)
+    res = ps.residual(b.rho.clone())
+    out = {
+        'value': float(sum(counts)) / wall,
+        'ms_per_step': wall / a.steps * 1e3,
+        'config': {'workload': '2048x2048 elliptical 90x70 mm chamber, quadrupole 50 T/m on the grid, '
+                               '%d FP32 macro-electrons, Gaussian bunches sigma 1 mm 1.15e11 ppb x20, dt 2e-12 s, '
+                               'Furman copper delta_max 1.5, direct Poisson solve, localRandom management '
+                               'outside [0.95, 1.5] x %d' % (n, target),
+                   'grid': [2048, 2048], 'particles': n, 'management_target': target,
+                   'capacitance_rows': ps.capacitanceRows, 'sort_every': a.sort_every,
+                   'macro_particles_start_end': [counts[0], p.macroParticleCount if not ops else nBefore],
+                   'hits_per_step': hits, 'absorbed_total': st['absorbed_total'],
+                   'emitted_total': st['emitted_total']},
+        'poisson_solves_per_s': solves,
+        'poisson_relres': res,
+        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'traffic_source': 'not measured for this workload', 'avg_launch_us': avg_us,
+                     'bytes_per_launch': bytes_per,
+                     'bytes_model': '44 B per FP32 macro-particle + 48 B per grid node (E, rho, B)'},
+        'management_in_timed_steps': mgmt,
+        'management_event': {'count_before': nBefore, 'count_after': p.macroParticleCount, 'seconds': t_merge,
+                             'charge_rel_change': abs(q1 - q0) / abs(q0), 'ops': ops},
+    }
+    return out, g, bq, hits, counts
+
+
+def measure_dipole(a, log, steps, warmup, n, dtype, eager, compare=False):
+    """Times `steps` steps of the dipole buildup (graph replays between sorts unless eager);
+    push kernel times from eager steps with HIP events after the timed region."""
+    import torch
+    g, p, pb, em, ps, b = build_dipole(n, dtype)
+    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4),
+                   maxSteps=warmup + steps + 40)
+    b.sort_particles()
+    if a.calib_copy:
+        from openecloud_amd import _lib as L
+        # one plain 1 GiB device copy (beyond the 256 MB last-level cache) for the counters
+        src_t = torch.ones(1 << 27, dtype=torch.float64, device='cuda')
+        dst_t = torch.empty_like(src_t)
+        torch.cuda.synchronize()
+        L.check(L.lib().oecl_copy_bytes(L.ptr(dst_t), L.ptr(src_t), L.u64(src_t.numel() * 8), L.stream()),
+                'calibration copy')
+        torch.cuda.synchronize()
+        del src_t, dst_t
+    run_plan(b, step_plan(warmup, a.sort_every, False))
+    b.sync()
+    plan = step_plan(steps, a.sort_every, not eager)
+    gs = plan_graph_steps(plan)
+    if gs:
+        b.capture_native(gs)
+    it0 = b.it
+    evs = make_events(steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ran = run_plan(b, plan, events=evs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    evs_used, evs_left = evs[:ran], evs[ran:]
+    # push kernel timing: a few more eager steps with events when the timed steps were replays
+    extra = 0
+    if len(evs_used) < min(20, steps):
+        extra = min(20, steps) - len(evs_used)
+        run_plan(b, [('step',)] * extra, events=evs_left[:extra])
+        evs_used = evs_used + evs_left[:extra]
+        evs_left = evs_left[extra:]
+    st = b.sync()
+    counts = b.counts(it0, it0 + steps)
+    push_us = event_us(evs_used)
+    event_us(evs_left)
+    s = 8 if dtype == 'f64' else 4
+    npx = g.getNpExt()
+    avg_bytes = sum(11 * s * c + 24 * npx for c in counts) / len(counts)
+    avg_us = sum(push_us) / len(push_us)
+    achieved = avg_bytes / (avg_us * 1e-6) / 1e9
+    q = b.rho.clone()
+    tr = profile_traffic(TRAFFIC_PROFILES.get(('dipole', dtype, n), ''))
+    out = {
+        'value': float(sum(counts)) / wall,
+        'unit': 'particle-steps/s',
+        'steps': steps,
+        'warmup': warmup,
+        'ms_per_step': wall / steps * 1e3,
+        'dtype': dtype,
+        'config': {'workload': 'elliptical 45x35 mm cut-cell chamber, 512x384 grid, By 0.5 T, '
+                               '%d macro-electrons, Gaussian bunches sigma 1.5 mm 1.15e11 ppb x4, '
+                               'dt 5e-12 s, Furman copper delta_max 1.3, primary photoelectrons '
+                               '1e-3 per proton per m' % n,
+                   'grid': [512, 384], 'particles': n, 'dt': 5e-12, 'graph_steps': gs,
+                   'macro_particles_start_end': [counts[0], p.macroParticleCount],
+                   'absorbed_total': st['absorbed_total'], 'emitted_total': st['emitted_total'],
+                   'photoelectrons_total': st['photo_total']},
+        'poisson_solves_per_s': poisson_rate(ps, q, 100),
+        'poisson_relres': ps.residual(q),
+        'poisson_capacitance_rows': ps.capacitanceRows,
+        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': tr,
+                     'traffic_source': ('profiles/' + TRAFFIC_PROFILES[('dipole', dtype, n)]) if tr else
+                     'not measured for this workload',
+                     'avg_launch_us': avg_us, 'bytes_per_launch': avg_bytes,
+                     'note': 'the 96 MB FP64 particle state fits the 256 MB last-level cache'},
+    }
+    if compare:
+        out['f32_vs_f64'] = compare_f32(a, n, b, warmup, steps, eager, extra)
+    return out
+
+
+def compare_f32(a, n, b64, warmup, steps, eager, extra):
+    """The same seeded workload with FP32 particle storage, stepped exactly like the FP64 run
+    (warm-up, the timed steps, `extra` eager steps); its throughput and the differences of the
+    final state from the FP64 run: cloud density (charge on the grid, relative L2), total charge
+    and macro-particle count."""
+    import torch
+    g, p, pb, em, ps, b = build_dipole(n, 'f32')
+    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4), maxSteps=warmup + steps + 40)
+    b.sort_particles()
+    run_plan(b, step_plan(warmup, a.sort_every, False))
+    b.sync()
+    plan = step_plan(steps, a.sort_every, not eager)
+    gs = plan_graph_steps(plan)
+    if gs:
+        b.capture_native(gs)
+    i0 = b.it
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_plan(b, plan)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    b.sync()
+    rate = float(sum(b.counts(i0, i0 + steps))) / wall
+    run_plan(b, [('step',)] * extra)
+    b.sync()
+    b64.deposit_all()
+    b.deposit_all()
+    r64, r32 = b64.rho, b.rho
+    q64 = float(b64.p.getParticleData()[:, 5].to(torch.float64).sum())
+    q32 = float(b.p.getParticleData()[:, 5].to(torch.float64).sum())
+    return {'value': rate, 'unit': 'particle-steps/s',
+            'cloud_density_rel_l2': float(torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)),
+            'total_charge_rel': abs(q32 - q64) / abs(q64),
+            'macro_particles': [b64.p.macroParticleCount, b.p.macroParticleCount],
+            'note': 'FP32 particle storage, same seeds and steps; differences from the FP64 run after '
+                    'the timed steps'}
+
+
+# ---------------------------------------------------------------------------------------------
+# CPU baseline: the oracle's C restatement of the reference loops, one core
+
+def _sample_in_ellipse(rng, ns, lx, ly):
+    import numpy as np
     pd = np.empty((ns, 6))
     k = 0
     while k < ns:
-        x = (rng.random(ns) - 0.5) * 0.09
-        y = (rng.random(ns) - 0.5) * 0.07
-        ok = x * x * 4 / 0.09 ** 2 + y * y * 4 / 0.07 ** 2 < 1
-        m = min(ns - k, ok.sum())
+        x = (rng.random(ns) - 0.5) * lx
+        y = (rng.random(ns) - 0.5) * ly
+        ok = x * x * 4 / lx ** 2 + y * y * 4 / ly ** 2 < 1
+        m = min(ns - k, int(ok.sum()))
         pd[k:k + m, 0] = x[ok][:m]
         pd[k:k + m, 1] = y[ok][:m]
         k += m
     pd[:, 2:5] = rng.standard_normal((ns, 3)) * 4.194e5
     pd[:, 5] = 500.
-    npx = g.npExt
+    return pd
+
+
+def cpu_particle_rate(g, seconds, typeB, bConst=None, B=None, wall=1, ns=200000):
+    """Seconds per particle-step of the oracle's C particle loop (weights, deposit, E gather,
+    [B gather], Boris push, wall test, saveAbsorbed, compaction) on a sample of ns particles
+    uniform in the elliptical chamber of grid g, with a random E on the grid."""
+    import ctypes
+    import numpy as np
+    import oracle
+    rng = np.random.default_rng(0)
+    pd = _sample_in_ellipse(rng, ns, g.getLx(), g.getLy())
+    npx = g.getNpExt()
     E = rng.standard_normal(2 * npx) * 1e3
     lib = oracle.lib()
     D = ctypes.POINTER(ctypes.c_double)
+    U16 = ctypes.POINTER(ctypes.c_uint16)
     ic = np.empty(ns, np.uint32)
     w4 = np.empty(4 * ns)
     eat = np.empty(2 * ns)
     bat = np.empty(3 * ns)
     ins = np.empty(ns, np.uint16)
     rho = np.empty(npx)
-    bc = np.array([0., 0.5, 0.])
+    absb = np.empty((ns, 6))
+    bc = np.asarray(bConst if bConst is not None else [0., 0., 0.], np.float64)
+    Bp = np.ascontiguousarray(B, np.float64) if B is not None else None
     qm = -1.602176565e-19 / 9.10938291e-31
-    args = (pd.ctypes.data_as(D), ctypes.c_uint32(ns), E.ctypes.data_as(D), ctypes.c_void_p(0),
-            ctypes.c_int(1), bc.ctypes.data_as(D), ctypes.c_double(1e-14), ctypes.c_double(qm),
+    args = (pd.ctypes.data_as(D), ctypes.c_uint32(ns), E.ctypes.data_as(D),
+            Bp.ctypes.data_as(D) if Bp is not None else ctypes.c_void_p(0),
+            ctypes.c_int(typeB), bc.ctypes.data_as(D), ctypes.c_double(2e-12), ctypes.c_double(qm),
             ctypes.c_double(1 / g.dx), ctypes.c_double(1 / g.dy), ctypes.c_double(g.lxExt / 2),
             ctypes.c_double(g.lyExt / 2), ctypes.c_uint32(g.nxExt), ctypes.c_uint32(npx),
-            ctypes.c_int(1), ctypes.c_double(0.09), ctypes.c_double(0.07),
+            ctypes.c_int(wall), ctypes.c_double(g.getLx()), ctypes.c_double(g.getLy()),
             ic.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), w4.ctypes.data_as(D),
-            eat.ctypes.data_as(D), bat.ctypes.data_as(D),
-            ins.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), rho.ctypes.data_as(D),
+            eat.ctypes.data_as(D), bat.ctypes.data_as(D), ins.ctypes.data_as(U16), rho.ctypes.data_as(D),
             ctypes.c_double(-1.602176565e-19))
-    lib.orc_particle_step(*args)
+    lib.orc_save_absorbed.restype = ctypes.c_uint32
+
+    def once():
+        lib.orc_particle_step(*args)
+        na = lib.orc_save_absorbed(pd.ctypes.data_as(D), ctypes.c_uint32(ns), ins.ctypes.data_as(U16),
+                                   absb.ctypes.data_as(D))
+        if na:
+            ins[:] = 1   # keep the sample's size: the compaction below moves nothing
+        lib.orc_remove(pd.ctypes.data_as(D), ins.ctypes.data_as(U16), ctypes.c_uint32(ns), ctypes.c_uint32(ns))
+
+    once()
     t0 = time.perf_counter()
     reps = 0
-    while time.perf_counter() - t0 < seconds * 0.6:
-        lib.orc_particle_step(*args)
+    while time.perf_counter() - t0 < seconds:
+        once()
         reps += 1
-    t_part = (time.perf_counter() - t0) / reps / ns          # s per particle-step
+    return (time.perf_counter() - t0) / reps / ns, ns, reps
+
+
+def cpu_emission_rate(seconds, seyMax, eHat0=None):
+    """Seconds per wall impact of the Furman emission restatement (particleEmitter.pyx:154-308,
+    Python with scipy.special) on impacts at 1-400 eV and random incidence on an ellipse."""
+    import numpy as np
+    from oracle import furman_oracle as F
+    rng = np.random.default_rng(1)
+    P = F.material('copper', seyMax=seyMax, eHat0=eHat0)
+    n0 = 256
+    th = rng.random(n0) * 2 * np.pi
+    nrm = np.stack([-np.cos(th), -np.sin(th)], 1)
+    e = rng.uniform(1., 400., n0)
+    v = np.sqrt(2 * e * 1.602176565e-19 / 9.10938291e-31)
+    ang = rng.uniform(-1.4, 1.4, n0)
+    ab = np.zeros((n0, 6))
+    ab[:, 2] = -v * (np.cos(ang) * nrm[:, 0] - np.sin(ang) * nrm[:, 1])
+    ab[:, 3] = -v * (np.sin(ang) * nrm[:, 0] + np.cos(ang) * nrm[:, 1])
+    ab[:, 5] = 500.
+    t0 = time.perf_counter()
+    done = 0
+    while time.perf_counter() - t0 < seconds or done == 0:
+        U = rng.random((n0, F.N_UNIFORMS))
+        F.generate_secondaries(ab, nrm, U, P)
+        done += n0
+    return (time.perf_counter() - t0) / done, done
+
+
+def cpu_superlu_rate(g, seconds):
+    import numpy as np
+    from oracle.grid_oracle import OraclePoisson
+    rng = np.random.default_rng(2)
     ps = OraclePoisson(g)
-    q = rng.standard_normal(npx) * 1e-12
+    q = rng.standard_normal(g.getNpExt()) * 1e-12
     ps.solve(q)
     t0 = time.perf_counter()
     nsol = 0
-    while time.perf_counter() - t0 < seconds * 0.4:
+    while time.perf_counter() - t0 < seconds or nsol == 0:
         ps.solve(q)
         nsol += 1
-    t_lu = (time.perf_counter() - t0) / nsol
-    n_full = 2e6
-    step_time = t_part * n_full + t_lu
-    return dict(value=n_full / step_time, unit='particle-steps/s', cores=1, kind='port',
-                sample='%d-particle sample x %d reps of the C particle loop (%.3g s/particle-step) '
-                       '+ %d SuperLU solves of the 512x384 system (%.3g s each), extrapolated '
-                       'to 2e6 particles per step' % (ns, reps, t_part, nsol, t_lu),
-                poisson_solves_per_s=1. / t_lu)
+    return (time.perf_counter() - t0) / nsol, nsol
+
+
+def cpu_baseline_full(g, bq, n_avg, hits, seconds):
+    """The 2048^2 quadrupole step on one core: the C particle loop with B on the grid on a
+    2e5-particle sample, plus the Furman emission per impact times the GPU run's impacts per
+    step.  The reference's SuperLU factorisation of this 3.3e6-unknown system does not fit a
+    bounded sample, so the Poisson solve is left out of this figure (which favours the CPU);
+    SuperLU solves/s of the 512 x 384 system are reported beside it."""
+    from openecloud_amd.grid import Grid
+    t_part, ns, reps = cpu_particle_rate(g, seconds * 0.55, 2, B=bq)
+    t_imp, nimp = cpu_emission_rate(seconds * 0.2, 1.5)
+    g2 = Grid(512, 384, 0.09, 0.07, 1)
+    t_lu, nsol = cpu_superlu_rate(g2, seconds * 0.25)
+    step = t_part * n_avg + t_imp * hits
+    info = host_info()
+    return {'value': n_avg / step, 'unit': 'particle-steps/s', 'cores': 1, 'kind': 'port',
+            'sample': '%d-particle sample x %d reps of the C particle loop with the quadrupole on the 2048^2 grid '
+                      '(%.3g s/particle-step) plus %d Furman impacts of the Python/scipy restatement (%.3g s each) '
+                      'x %.0f impacts per step, scaled to %.3g particles per step; Poisson solve not included '
+                      '(SuperLU at 2048^2 exceeds the sample bound)' % (ns, reps, t_part, nimp, t_imp, hits, n_avg),
+            'nproc': info['nproc'], 'cpu_model': info['cpu_model'],
+            'superlu_solves_per_s_512x384': 1. / t_lu}
+
+
+def cpu_baseline_dipole(g, n, hits, seconds):
+    """The dipole step on one core: the C particle loop (constant By) on a 2e5-particle sample
+    of the 512 x 384 chamber, the Furman emission per impact times the GPU run's impacts per
+    step, and the reference's SuperLU back-substitution of the same system."""
+    t_part, ns, reps = cpu_particle_rate(g, seconds * 0.5, 1, bConst=[0., 0.5, 0.])
+    t_imp, nimp = cpu_emission_rate(seconds * 0.2, 1.3)
+    t_lu, nsol = cpu_superlu_rate(g, seconds * 0.3)
+    step = t_part * n + t_imp * hits + t_lu
+    info = host_info()
+    return {'value': n / step, 'unit': 'particle-steps/s', 'cores': 1, 'kind': 'port',
+            'sample': '%d-particle sample x %d reps of the C particle loop (%.3g s/particle-step) + %d Furman '
+                      'impacts (Python/scipy restatement, %.3g s each) x %.0f per step + %d SuperLU solves of the '
+                      '512x384 system (%.3g s each), scaled to %d particles per step'
+                      % (ns, reps, t_part, nimp, t_imp, hits, nsol, t_lu, n),
+            'nproc': info['nproc'], 'cpu_model': info['cpu_model'], 'poisson_solves_per_s': 1. / t_lu}
 
 
 def build_scan(R, dtype, maxSteps):
@@ -212,61 +667,6 @@ def build_scan(R, dtype, maxSteps):
         runs.append(Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1))
     scan = BuildupScan(runs, seeds, growth=1.5, maxSteps=maxSteps)
     return g, ps, runs, scan, dict(sey=sey, ppb=ppb, n=nmp)
-
-
-def compare_f32(a, n, b64, extra):
-    """The same seeded workload with FP32 particle storage (arithmetic stays FP64), stepped
-    exactly like the FP64 run (warm-up, the timed steps as graph replays or eager launches,
-    sorts, then `extra` eager steps without a sort); its throughput and the differences of the
-    final state from the FP64 run: cloud density (charge on the grid, relative L2), total charge
-    and macro-particle count."""
-    import torch
-    g, p, pb, em, ps, b = build_case(n, 'f32')
-    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4),
-                   maxSteps=a.warmup + a.steps + extra + 40)
-    b.sort_particles()
-    for k in range(a.warmup):
-        b.step_native()
-        if a.sort_every and k % a.sort_every == a.sort_every - 1:
-            b.sync()
-            b.sort_particles()
-    b.sync()
-    i0 = b.it
-    if not a.eager:
-        per = a.sort_every if a.sort_every else a.steps
-        b.capture_native(per)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    if not a.eager:
-        for k in range(a.steps // per):
-            b.replay()
-            if a.sort_every:
-                b.sync()
-                b.sort_particles(keepBuffers=True)
-    else:
-        for k in range(a.steps):
-            b.step_native()
-            if a.sort_every and k % a.sort_every == a.sort_every - 1:
-                b.sync()
-                b.sort_particles()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    b.sync()
-    rate = float(sum(b.counts(i0, i0 + a.steps))) / wall
-    for k in range(extra):
-        b.step_native()
-    b.sync()
-    b64.deposit_all()
-    b.deposit_all()
-    r64, r32 = b64.rho, b.rho
-    q64 = float(b64.p.getParticleData()[:, 5].to(torch.float64).sum())
-    q32 = float(b.p.getParticleData()[:, 5].to(torch.float64).sum())
-    return {'value': rate, 'unit': 'particle-steps/s',
-            'cloud_density_rel_l2': float(torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)),
-            'total_charge_rel': abs(q32 - q64) / abs(q64),
-            'macro_particles': [b64.p.macroParticleCount, b.p.macroParticleCount],
-            'note': 'FP32 particle storage, FP64 arithmetic, same seeds and steps; differences from the '
-                    'FP64 run after the timed steps'}
 
 
 def main_learned(a):
@@ -457,119 +857,6 @@ def main_learned(a):
     print(json.dumps(out))
 
 
-def main_full(a):
-    """The combined workload: a 2048 x 2048 grid of the elliptical 90 x 70 mm chamber, a 50 T/m
-    quadrupole (the grid field path, typeBField 2), a.particles (default 2e7) FP32
-    macro-electrons, 20 bunches of 1.15e11 with sigma 1 mm, dt 2e-12 s, Furman copper with
-    delta_max 1.5, the one-call step with the direct Poisson solve.  After the timed steps one
-    merge event of localRandom reduces the count by the 1.5x trigger's factor (target = count
-    / 1.5), timed, with its charge change."""
-    import ctypes
-    import torch
-    from openecloud_amd import _lib as L
-    from openecloud_amd import particleManagement, randomGen
-    from openecloud_amd.beam import GaussianBunchTrain
-    from openecloud_amd.buildup import Buildup
-    from openecloud_amd.diagnostics import weight_sum
-    from openecloud_amd.grid import Grid
-    from openecloud_amd.magneticField import multipoleExpansion
-    from openecloud_amd.particleBoundary import AbsorbElliptical
-    from openecloud_amd.particleEmitter import FurmanEmitter, homoLoader
-    from openecloud_amd.particles import Particles
-    from openecloud_amd.poissonSolver import PoissonSolver
-    L.require_gpu()
-    log = lambda m: print(m, file=sys.stderr, flush=True)
-    n = int(a.particles) if '--particles' in sys.argv else 20000000
-    dt = 2e-12
-    t0 = time.perf_counter()
-    randomGen.seed(0)
-    g = Grid(2048, 2048, 0.09, 0.07, 1)
-    bq = multipoleExpansion(g, [0., 50.])
-    ps = PoissonSolver(g, method='direct', checkEvery=0)
-    log('grid, field and direct solver: %.1f s (%d capacitance rows)' % (time.perf_counter() - t0,
-                                                                        ps.capacitanceRows))
-    p = Particles('electrons', 'variableWeight', dtype=torch.float32, ordering='fast')
-    pb = AbsorbElliptical(g, p)
-    em = FurmanEmitter(pb, p, 'copper', seyMax=1.5)
-    beam = GaussianBunchTrain(g, dt, nParticles=1.15e11, tBunchSpacing=25e-9, radiusSigma=1e-3, nBunches=20)
-    homoLoader(g, p, pb, n, 1e9 / n, 4.194e5)
-    p.reserve(int(1.6 * n))
-    b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=2, bAtGridPoints=bq)
-    b.native_setup(absCap=max(65536, n // 32), secCap=max(131072, n // 16), maxSteps=a.warmup + a.steps + 20)
-    b.sort_particles()
-    se = a.sort_every
-    for k in range(a.warmup):
-        b.step_native()
-        if se and k % se == se - 1:
-            b.sync()
-            b.sort_particles()
-    b.sync()
-    evs = []
-    for _ in range(a.steps):
-        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-        L.check(L.lib().oecl_event_create(ctypes.byref(e0)))
-        L.check(L.lib().oecl_event_create(ctypes.byref(e1)))
-        evs.append((e0, e1))
-    it0 = b.it
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        b._args.evPushBegin, b._args.evPushEnd = evs[k][0].value, evs[k][1].value
-        b.step_native()
-        if se and k % se == se - 1:
-            b.sync()
-            b.sort_particles()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    b._args.evPushBegin = b._args.evPushEnd = None
-    st = b.sync()
-    counts = b.counts(it0, it0 + a.steps)
-    push = []
-    for e0, e1 in evs:
-        ms = ctypes.c_float()
-        L.check(L.lib().oecl_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
-        push.append(ms.value * 1e3)
-        L.lib().oecl_event_destroy(e0)
-        L.lib().oecl_event_destroy(e1)
-    avg_us = sum(push) / len(push)
-    bytes_per = sum(11 * 4 * c + 48 * g.getNpExt() for c in counts) / len(counts)
-    achieved = bytes_per / (avg_us * 1e-6) / 1e9
-    # one merge event of the 1.5x trigger
-    nBefore = p.macroParticleCount
-    q0 = weight_sum(p)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    mg = particleManagement.localRandom(g, p, pb, int(nBefore / 1.5))
-    torch.cuda.synchronize()
-    t_merge = time.perf_counter() - t1
-    q1 = weight_sum(p)
-    out = {
-        'metric': 'buildup particle-steps/sec (2048x2048 quadrupole, one-call step)',
-        'value': float(sum(counts)) / wall,
-        'unit': 'particle-steps/s',
-        'n_gpus': 1,
-        'steps': a.steps,
-        'warmup': a.warmup,
-        'ms_per_step': wall / a.steps * 1e3,
-        'higher_is_better': True,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': 'f32',
-        'data': 'synthetic (seeded uniform load, Philox draws)',
-        'config': {'workload': '2048x2048 elliptical 90x70 mm chamber, quadrupole 50 T/m (grid field), '
-                               '%d FP32 macro-electrons, Gaussian bunches sigma 1 mm 1.15e11 ppb x20, dt 2e-12 s, '
-                               'Furman copper delta_max 1.5, direct Poisson solve' % n,
-                   'grid': [2048, 2048], 'particles': n, 'capacitance_rows': ps.capacitanceRows,
-                   'absorbed_total': st['absorbed_total'], 'emitted_total': st['emitted_total']},
-        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'avg_launch_us': avg_us,
-                     'bytes_per_launch': bytes_per},
-        'merge_event': {'count_before': nBefore, 'count_after': p.macroParticleCount, 'seconds': t_merge,
-                        'charge_rel_change': abs(q1 - q0) / abs(q0), 'ops': mg},
-    }
-    print(json.dumps(out))
-
-
 def main_scan(a):
     import ctypes
     import torch
@@ -578,10 +865,10 @@ def main_scan(a):
     from openecloud_amd.autotune import tune_scan
     L.require_gpu()
     R = a.runs
+    log = lambda m: print(m, file=sys.stderr, flush=True)
     start = max(int(a.scan_start), a.warmup)
     maxSteps = start + a.steps + 2000
     g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps)
-    log = lambda m: print(m, file=sys.stderr)
     scan.sort()
     for _ in range(a.warmup):
         scan.step()
@@ -592,7 +879,7 @@ def main_scan(a):
     else:
         best, results = tune_scan(scan, cacheFile=a.tune_cache, log=log)
     se = int(best['sortEvery'])
-    steps = (a.steps + se - 1) // se * se
+    steps = a.steps
     # the timed window is steps [start, start + steps) of a fresh scan from the same seeds, so
     # that it does not depend on how many steps the autotuner ran (or whether it ran at all)
     del scan, runs
@@ -613,9 +900,14 @@ def main_scan(a):
     it0 = scan.it
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    # whole sort intervals as graph replays, the remaining steps of a last interval one by one
     for _ in range(steps // se):
         scan.sort(best['order'], best['binShift'])
         scan.replay()
+    if steps % se:
+        scan.sort(best['order'], best['binShift'])
+        for _ in range(steps % se):
+            scan.step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     st = scan.sync()
@@ -677,6 +969,7 @@ def main_scan(a):
         ref_t += time.perf_counter() - t
     seq_rate, ref_rate = seq_ps / seq_t, ref_ps / ref_t
     n0 = [int(x) for x in prm['n']]
+    tr = profile_traffic(TRAFFIC_PROFILES.get(('scan', a.dtype, R), ''))
     out = {
         'metric': 'batched scan buildup particle-steps/sec (%d buildups per step call)' % R,
         'value': rate,
@@ -701,8 +994,10 @@ def main_scan(a):
                    'emitted_in_timed_steps': int(sum(st['emitted_total']) - sum(st0['emitted_total']))},
         'poisson_solves_per_s': R * steps / wall,
         'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'avg_launch_us': avg_us,
-                     'bytes_per_launch': sum(by) / len(by)},
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': tr,
+                     'traffic_source': ('profiles/' + TRAFFIC_PROFILES[('scan', a.dtype, R)]) if tr else
+                     'not measured for this workload',
+                     'avg_launch_us': avg_us, 'bytes_per_launch': sum(by) / len(by)},
         'autotune': {'best': best, 'candidates_us_per_step': [[c, 1e6 * t] for c, t in results]},
         'plain_one_run_per_call': {'value': seq_rate, 'unit': 'particle-steps/s',
                                    'note': 'the same runs stepped one at a time with the single-run one-call step'},
@@ -714,162 +1009,77 @@ def main_scan(a):
     print(json.dumps(out))
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print('bench.py runs on one GPU only: this build has no multi-GPU path '
-              '(see the sharding design note); got --gpus %d' % a.gpus, file=sys.stderr)
-        sys.exit(2)
-    if a.workload == 'learned':
-        return main_learned(a)
-    if a.workload == 'full':
-        return main_full(a)
-    if a.workload == 'scan':
-        if a.dtype == 'f64' and '--dtype' not in sys.argv:
-            a.dtype = 'f32'   # the scan's particles are FP32 unless asked otherwise
-        return main_scan(a)
+
+def main_full(a):
+    """The default line: the 2048^2 quadrupole buildup, with the dipole buildup and the CPU
+    baseline beside it."""
     import torch
     from openecloud_amd import _lib as L
     L.require_gpu()
-    n = int(a.particles)
-    g, p, pb, em, ps, b = build_case(n, a.dtype)
-    import ctypes
-    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4),
-                   maxSteps=a.warmup + a.steps + 40)
-    b.sort_particles()
-    if a.calib_copy:
-        # one plain 1 GiB device copy (beyond the 256 MB last-level cache) for the counters
-        src_t = torch.ones(1 << 27, dtype=torch.float64, device='cuda')
-        dst_t = torch.empty_like(src_t)
-        torch.cuda.synchronize()
-        L.check(L.lib().oecl_copy_bytes(L.ptr(dst_t), L.ptr(src_t), L.u64(src_t.numel() * 8), L.stream()),
-                'calibration copy')
-        torch.cuda.synchronize()
-        del src_t, dst_t
-    for k in range(a.warmup):
-        b.step_native()
-        if a.sort_every and k % a.sort_every == a.sort_every - 1:
-            b.sync()
-            b.sort_particles()
-    torch.cuda.synchronize()
-    b.sync()
-    # timed region: K full steps; the push kernel of each step is bracketed by HIP events
-    # recorded on the stream the step runs on
-    evs = []
-    for _ in range(a.steps):
-        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-        L.check(L.lib().oecl_event_create(ctypes.byref(e0)))
-        L.check(L.lib().oecl_event_create(ctypes.byref(e1)))
-        evs.append((e0, e1))
-    it0 = b.it
-    if not a.eager:
-        # steps between two sorts as one HIP graph; the sort keeps the particle buffers
-        per = a.sort_every if a.sort_every else a.steps
-        if a.steps % per:
-            print('graph replay needs --steps to be a multiple of --sort-every (or --eager)', file=sys.stderr)
-            sys.exit(2)
-        b.capture_native(per)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(a.steps // per):
-            b.replay()
-            if a.sort_every:
-                b.sync()
-                b.sort_particles(keepBuffers=True)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        # push timing of the same kernel: a few eager steps with events after the timed region
-        nev = min(20, len(evs))
-        for k in range(nev):
-            b._args.evPushBegin = evs[k][0].value
-            b._args.evPushEnd = evs[k][1].value
-            b.step_native()
-        evs = evs[:nev]
-    else:
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(a.steps):
-            b._args.evPushBegin = evs[k][0].value
-            b._args.evPushEnd = evs[k][1].value
-            b.step_native()
-            if a.sort_every and k % a.sort_every == a.sort_every - 1:
-                b.sync()              # host count for the sort launch, and the status check
-                b.sort_particles()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-    b._args.evPushBegin = None
-    b._args.evPushEnd = None
-    st = b.sync()
-    wall = t1 - t0
-    counts = b.counts(it0, it0 + a.steps)
-    psteps = float(sum(counts))
-    push_ms = []
-    for e0, e1 in evs:
-        ms = ctypes.c_float()
-        L.check(L.lib().oecl_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
-        push_ms.append(ms.value)
-        L.lib().oecl_event_destroy(e0)
-        L.lib().oecl_event_destroy(e1)
-    if a.sort_every:
-        by_age = {}
-        for k, ms in enumerate(push_ms):
-            by_age.setdefault(k % a.sort_every, []).append(ms * 1e3)
-        print('push time (us) by steps since the last sort: ' +
-              ' '.join('%d:%.1f' % (k, sum(v) / len(v)) for k, v in sorted(by_age.items())),
-              file=sys.stderr)
-    s = 8 if a.dtype == 'f64' else 4
-    npx = g.getNpExt()
-    bytes_per = [11 * s * c + 24 * npx for c in counts]
-    avg_ms = sum(push_ms) / len(push_ms)
-    avg_bytes = sum(bytes_per) / len(bytes_per)
-    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    # Poisson solves/s of the direct device solve (phi only, then phi -> E)
-    q = b.rho.clone()
-    torch.cuda.synchronize()
-    tp = time.perf_counter()
-    nsol = 100
-    for _ in range(nsol):
-        ps.solve(q)
-    torch.cuda.synchronize()
-    poisson_rate = nsol / (time.perf_counter() - tp)
-    res = ps.residual(q)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    res, g, bq, hits, counts = measure_full(a, log)
+    log('full: %.4g particle-steps/s, %.3f ms/step' % (res['value'], res['ms_per_step']))
     out = {
-        'metric': 'buildup particle-steps/sec (plain GPU step)',
-        'value': psteps / wall,
+        'metric': 'buildup particle-steps/sec and Poisson solves/sec, learned-PCG batched vs plain',
+        'value': res['value'],
         'unit': 'particle-steps/s',
         'n_gpus': 1,
         'steps': a.steps,
         'warmup': a.warmup,
-        'ms_per_step': wall / a.steps * 1e3,
+        'ms_per_step': res['ms_per_step'],
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': a.dtype,
+        'dtype': 'f32',
         'data': 'synthetic (seeded uniform load, Philox draws)',
-        'config': {'workload': 'elliptical 45x35 mm cut-cell chamber, 512x384 grid, By 0.5 T, '
-                               '%d macro-electrons, Gaussian bunches sigma 1.5 mm 1.15e11 ppb x4, '
-                               'dt 5e-12 s, Furman copper delta_max 1.3, primary photoelectrons '
-                               '1e-3 per proton per m' % n,
-                   'grid': [512, 384], 'particles': n, 'dt': 5e-12,
-                   'macro_particles_start_end': [counts[0], p.macroParticleCount],
-                   'absorbed_total': st['absorbed_total'], 'emitted_total': st['emitted_total'],
-                   'photoelectrons_total': st['photo_total']},
-        'poisson_solves_per_s': poisson_rate,
-        'poisson_relres': res,
-        'poisson_capacitance_rows': ps.capacitanceRows,
-        'sort_every': a.sort_every,
-        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved,
-                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
-                     'traffic': measured_traffic(a.dtype, n), 'avg_launch_us': avg_ms * 1e3,
-                     'bytes_per_launch': avg_bytes},
+        'config': res['config'],
+        'solver': 'direct (capacitance-matrix DST solve, the reference LU as its oracle)',
+        'poisson_solves_per_s': res['poisson_solves_per_s'],
+        'poisson_relres': res['poisson_relres'],
+        'roofline': res['roofline'],
+        'management_in_timed_steps': res['management_in_timed_steps'],
+        'management_event': res['management_event'],
     }
-    if not a.no_f32_compare and a.dtype == 'f64':
-        out['f32_vs_f64'] = compare_f32(a, n, b, b.it - it0 - a.steps)
+    if not a.no_dipole:
+        torch.cuda.synchronize()
+        d = measure_dipole(a, log, a.steps, a.warmup, 2000000, 'f64', a.eager)
+        log('dipole: %.4g particle-steps/s, %.3f ms/step' % (d['value'], d['ms_per_step']))
+        out['plain_dipole'] = d
     if not a.no_cpu_baseline:
-        cb = cpu_baseline(a.cpu_seconds)
-        out['cpu_baseline'] = {k: cb[k] for k in ('value', 'unit', 'cores', 'kind', 'sample')}
-        out['cpu_baseline_poisson_solves_per_s'] = cb['poisson_solves_per_s']
-    print(json.dumps(out))
+        n_avg = sum(counts) / len(counts)
+        cb = cpu_baseline_full(g, bq, n_avg, hits, a.cpu_seconds)
+        out['cpu_baseline'] = cb
+    print(json.dumps(out), flush=True)
+
+
+def main_dipole(a):
+    from openecloud_amd import _lib as L
+    L.require_gpu()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    n = int(a.particles)
+    d = measure_dipole(a, log, a.steps, a.warmup, n, a.dtype, a.eager,
+                       compare=not a.no_f32_compare and a.dtype == 'f64')
+    out = {'metric': 'buildup particle-steps/sec and Poisson solves/sec, learned-PCG batched vs plain',
+           'n_gpus': 1, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+           'data': 'synthetic (seeded uniform load, Philox draws)', 'solver': 'direct'}
+    out.update(d)
+    if not a.no_cpu_baseline:
+        from openecloud_amd.grid import Grid
+        g = Grid(512, 384, 0.09, 0.07, 1)
+        hits = d['config']['absorbed_total'] / max(1, a.warmup + a.steps)
+        out['cpu_baseline'] = cpu_baseline_dipole(g, n, hits, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def main(argv=None):
+    a = parse(argv)
+    if a.workload == 'learned':
+        return main_learned(a)
+    if a.workload == 'scan':
+        return main_scan(a)
+    if a.workload == 'dipole':
+        return main_dipole(a)
+    return main_full(a)
 
 
 if __name__ == '__main__':

@@ -400,6 +400,38 @@ class Buildup:
             if getattr(self, '_args', None) is not None:
                 self._args.data = p._data.data_ptr()
 
+    def manage(self, targetMacroParticleCount, upper=1.05, lower=0.95, method='local', **kw):
+        """Particle management inside a native buildup (example03_kdTree.py:57-59): when the
+        macro-particle count is above upper * target or below lower * target, localRandom
+        (or globalRandom) brings it back to the target.  The particle block keeps its address
+        and capacity (a captured graph stays valid), the device count is updated and the
+        charge of the managed particles is deposited again.  Synchronises; returns the
+        management's operation counts, or None when the count is within the band."""
+        from . import particleManagement
+        self.sync()
+        p = self.p
+        n = p.macroParticleCount
+        target = int(targetMacroParticleCount)
+        if lower * target <= n <= upper * target:
+            return None
+        block, cap = p._data, p.capacity
+        if method == 'local':
+            ops = particleManagement.localRandom(self.g, p, self.pb, target, **kw)
+        elif method == 'global':
+            particleManagement.globalRandom(self.g, p, target)
+            ops = {'method': 'global'}
+        else:
+            raise ValueError('method must be "local" or "global"')
+        if p.capacity != cap:
+            raise L.OeclError('particle management grew the block beyond its capacity %d' % cap)
+        if p._data is not block:
+            m = p.macroParticleCount
+            block[:, :m].copy_(p._data[:, :m])
+            p._data, p._spare = block, p._data
+        self._nDev.fill_(p.macroParticleCount)
+        self.deposit_all()
+        return ops
+
     def counts(self, first, last):
         """Particle counts at the start of steps first .. last-1 (native steps)."""
         return self._hist[first:last].cpu().tolist()
