@@ -157,6 +157,17 @@ uint64_t oecl_scan_work_bytes(uint32_t n);
 int oecl_save_absorbed(int dtype, const void *data, uint64_t stride, uint32_t n,
                        const uint8_t *isInside, void *absorbed, uint32_t absCap,
                        uint32_t *nAbsorbed, void *work, void *stream);
+/* oecl_save_absorbed without a workspace and without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The scan's scratch is a
+ * 128 KiB array inside the library, one per GPU, held until the process ends; n is limited to
+ * oecl_save_absorbed_noalloc_max() (67 088 384) and larger n returns OECL_ERR_CAPACITY before
+ * any device work.  Calls on one GPU share that array: calls that overlap on one GPU (on
+ * different streams or threads) race and must be serialised by the caller. */
+uint32_t oecl_save_absorbed_noalloc_max(void);
+int oecl_save_absorbed_noalloc(int dtype, const void *data, uint64_t stride, uint32_t n,
+                               const uint8_t *isInside, void *absorbed, uint32_t absCap,
+                               uint32_t *nAbsorbed, void *stream);
 
 /* calculateInteractionPoint + calculateNormalVectors of the three walls
  * (Rect :108-221, Ellip :252-297, CutCell :373-481) over nAbs absorbed particles (AoS).

@@ -3,6 +3,8 @@
 // Compaction never scans serially: one pass counts holes per 4096-particle tile, a single
 // block scans the tile counts, and the consuming kernels recompute each hole's rank inside
 // its tile with a block scan.  Only particles that move are read or written.
+#include <mutex>
+
 #include "common.h"
 #include "scan.h"
 #include "wall.h"
@@ -444,6 +446,64 @@ OECL_EXPORT int oecl_save_absorbed(int dtype, const void *data, uint64_t stride,
     }
     OECL_CHECK_LAUNCH("oecl_save_absorbed");
     return OECL_OK;
+}
+
+// ---------------------------------------------------------------- saveAbsorbed without allocation
+// The hole scan's tile counts and offsets (oecl_scan_work_bytes(n) bytes) live in this array
+// instead of a workspace argument.  Every kernel writes the words it later reads (the tile
+// counts, then the offsets), so nothing has to start at zero.  Each GPU holds its own copy at
+// its own address; host code uses that address, never the array's name.
+namespace {
+constexpr uint64_t kSaveAbsorbedScratchBytes = 128u << 10;
+__device__ __align__(256) uint32_t scratch[kSaveAbsorbedScratchBytes / 4];
+constexpr int kMaxDevices = 64;
+void *g_scratchDev[kMaxDevices];
+std::mutex g_scratchLock;
+
+int save_absorbed_scratch(void **out)
+{
+    int dev = 0;
+    OECL_HIP(hipGetDevice(&dev), "save_absorbed_noalloc: current device");
+    if (dev < 0 || dev >= kMaxDevices) {
+        oecl_set_error_msg("oecl_save_absorbed_noalloc: device index beyond 63");
+        return OECL_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratchLock);
+    if (!g_scratchDev[dev]) {
+        void *p = nullptr;
+        OECL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "save_absorbed_noalloc: scratch address");
+        g_scratchDev[dev] = p;
+    }
+    *out = g_scratchDev[dev];
+    return OECL_OK;
+}
+}  // namespace
+
+OECL_EXPORT uint32_t oecl_save_absorbed_noalloc_max(void)
+{
+    // largest n whose scan workspace fits: 4 (2 (tiles + 1) + 8) bytes
+    const uint64_t tiles = kSaveAbsorbedScratchBytes / 8 - 5;
+    const uint64_t n = tiles * OECL_SCAN_TILE;
+    return n > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)n;
+}
+
+OECL_EXPORT int oecl_save_absorbed_noalloc(int dtype, const void *data, uint64_t stride, uint32_t n,
+                                           const uint8_t *isInside, void *absorbed, uint32_t absCap,
+                                           uint32_t *nAbsorbed, void *stream)
+{
+    if (!isInside || !nAbsorbed || (n > 0 && !data)) return OECL_ERR_ARG;
+    if (dtype != OECL_F64 && dtype != OECL_F32) return OECL_ERR_ARG;
+    if (oecl_scan_work_bytes(n) > kSaveAbsorbedScratchBytes) {
+        char m[160];
+        snprintf(m, sizeof(m), "oecl_save_absorbed_noalloc: n = %u exceeds the scratch limit of %u particles", n,
+                 oecl_save_absorbed_noalloc_max());
+        oecl_set_error_msg(m);
+        return OECL_ERR_CAPACITY;
+    }
+    void *work = nullptr;
+    const int rc = save_absorbed_scratch(&work);
+    if (rc != OECL_OK) return rc;
+    return oecl_save_absorbed(dtype, data, stride, n, isInside, absorbed, absCap, nAbsorbed, work, stream);
 }
 
 OECL_EXPORT int oecl_wall_interaction(const oecl_wall_t *wall, int dtype, void *absorbed,

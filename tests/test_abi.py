@@ -66,6 +66,22 @@ def test_invalid_arguments_are_refused_before_any_device_call():
                               ctypes.c_uint32(4), None, None, None, None) == ARG
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason='liboecl.so not built')
+def test_noalloc_size_limit_without_gpu():
+    """The no-allocation saveAbsorbed refuses sizes beyond its scratch before any device call."""
+    lib = ctypes.CDLL(LIB)
+    lib.oecl_save_absorbed_noalloc_max.restype = ctypes.c_uint32
+    lib.oecl_last_error.restype = ctypes.c_char_p
+    mx = lib.oecl_save_absorbed_noalloc_max()
+    assert mx == (128 * 1024 // 8 - 5) * 4096
+    one = ctypes.c_void_p(16)   # never dereferenced: the size check comes first
+    assert lib.oecl_save_absorbed_noalloc(0, one, ctypes.c_uint64(mx + 1), ctypes.c_uint32(mx + 1), one, None,
+                                          ctypes.c_uint32(0), one, None) == 3
+    assert b'scratch limit' in lib.oecl_last_error()
+    assert lib.oecl_save_absorbed_noalloc(0, one, ctypes.c_uint64(8), ctypes.c_uint32(8), None, None,
+                                          ctypes.c_uint32(0), one, None) == 1
+
+
 def test_push_block_size_matches_the_scan_layout():
     """Batched run segments and the per-block run table use the library's push block size; the
     scan's host layout must use the same."""
