@@ -104,6 +104,16 @@ int oecl_calc_grid_weights(const oecl_grid_t *g, int dtype, const void *data, ui
 int oecl_charge_to_grid(const oecl_grid_t *g, int dtype, const void *data, uint64_t stride,
                         uint32_t n, const uint32_t *inCell, const void *w4, uint64_t wstride,
                         double q, double *rho, void *stream);
+/* particles.pyx:70-77 -> _chargeToGrid :336-351 with the reference's summation order: every
+ * node's charge is the sum of its contributions in increasing particle index, starting from
+ * zero, exactly as the sequential loop forms it, so rho is bit-identical to the reference for
+ * the same particles (the atomic deposit above agrees to rounding, 1e-13 relative).  A stable
+ * sort by cell plus one ordered merge per node; for parity runs.  work:
+ * oecl_charge_ordered_work_bytes(n, npExt) bytes. */
+uint64_t oecl_charge_ordered_work_bytes(uint32_t n, uint32_t np);
+int oecl_charge_to_grid_ordered(const oecl_grid_t *g, int dtype, const void *data, uint64_t stride,
+                                uint32_t n, const uint32_t *inCell, const void *w4, uint64_t wstride,
+                                double q, double *rho, void *work, void *stream);
 
 /* particles.pyx:58-68 -> _fieldToParticles :354-372 (dims 2) / _fieldToParticles3Dims
  * :375-397 (dims 3).  out is AoS [n][dims] in the particle dtype. */

@@ -84,8 +84,14 @@ class Buildup:
         return self._work
 
     def deposit_all(self):
-        """rho = charge of all current particles (the reference's chargeToGrid)."""
+        """rho = charge of all current particles (the reference's chargeToGrid; in its
+        summation order when the particles use orderedDeposit)."""
         p = self.p
+        if p.orderedDeposit:
+            p.calcGridWeights(self.g)
+            p._charge(self.g, self.rho)
+            self._rho_valid = True
+            return
         self.rho.zero_()
         L.check(L.lib().oecl_deposit_add(ctypes.byref(self.gd), L.i32(p.code), L.ptr(p._data),
                                          L.u64(p.capacity), L.u32(p.macroParticleCount),
@@ -146,7 +152,7 @@ class Buildup:
     def step(self):
         g, p, pb, ps, em = self.g, self.p, self.pb, self.ps, self.em
         t = self.it * self.dt
-        if not self._rho_valid:
+        if not self._rho_valid or p.orderedDeposit:
             self.deposit_all()
         E = ps.solve(self._rhs(t))
         n = p.macroParticleCount
@@ -326,6 +332,11 @@ class Buildup:
         """One full step in one library call; counts stay on the device (call sync())."""
         if self.it >= self._tabEnd:
             raise L.OeclError('native step: maxSteps exhausted; call native_setup again')
+        if self.p.orderedDeposit:
+            # the charge of the current particles in the reference's order, replacing the
+            # push's deposit (needs the host count: synchronises)
+            self.sync()
+            self.deposit_all()
         L.check(L.lib().oecl_step_run(ctypes.byref(self.gd), ctypes.byref(self.pb.desc),
                                       ctypes.byref(self._prm), self.ps._fp,
                                       ctypes.byref(self._args), L.stream()), 'step')

@@ -222,8 +222,8 @@ __global__ void k_wall_interaction(WallDev w, T *ap, uint32_t na, double dt, T *
         n0 = w.ccn[2 * c2];
         n1 = w.ccn[2 * c2 + 1];
     }
-    p[0] = (T)x;
-    p[1] = (T)y;
+    p[0] = wall_store<T>(x);
+    p[1] = wall_store<T>(y);
     rts[ii] = (T)r;
     nv[2 * (uint64_t)ii] = (T)n0;
     nv[2 * (uint64_t)ii + 1] = (T)n1;

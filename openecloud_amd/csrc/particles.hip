@@ -16,13 +16,14 @@ __global__ void k_calc_weights(oecl_grid_t g, Soa<T> p, uint32_t n, uint32_t *in
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Weights<T> w = grid_weights<T>(p(0, i), p(1, i), (T)(1. / g.dx), (T)(1. / g.dy),
-                                   (T)g.lxHalfExt, (T)g.lyHalfExt, g.nx, g.ny);
+    // FP64 arithmetic (particles.pyx:400-418), weights stored in the particle dtype
+    Weights<double> w = grid_weights<double>((double)p(0, i), (double)p(1, i), 1. / g.dx, 1. / g.dy,
+                                             g.lxHalfExt, g.lyHalfExt, g.nx, g.ny);
     inCell[i] = w.cell;
-    w4[i] = w.wLL;
-    w4[ws + i] = w.wUL;
-    w4[2 * ws + i] = w.wLU;
-    w4[3 * ws + i] = w.wUU;
+    w4[i] = (T)w.wLL;
+    w4[ws + i] = (T)w.wUL;
+    w4[2 * ws + i] = (T)w.wLU;
+    w4[3 * ws + i] = (T)w.wUU;
 }
 
 __global__ void k_zero(double *a, uint32_t n)
@@ -51,8 +52,10 @@ __global__ void k_deposit_add(oecl_grid_t g, Soa<T> p, uint32_t n, double q, dou
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Weights<T> w = grid_weights<T>(p(0, i), p(1, i), (T)(1. / g.dx), (T)(1. / g.dy),
-                                   (T)g.lxHalfExt, (T)g.lyHalfExt, g.nx, g.ny);
+    // FP64 weights of the stored position, as the push deposits (FP32 storage: in-cell
+    // coordinates of a 2048-cell grid keep only ~6e-5 of a cell in FP32)
+    Weights<double> w = grid_weights<double>((double)p(0, i), (double)p(1, i), 1. / g.dx, 1. / g.dy,
+                                             g.lxHalfExt, g.lyHalfExt, g.nx, g.ny);
     double wt = (double)p(5, i);
     unsafeAtomicAdd(rho + w.cell, (double)w.wLL * wt * q);
     unsafeAtomicAdd(rho + w.cell + 1, (double)w.wUL * wt * q);
@@ -68,8 +71,8 @@ __global__ void k_deposit_add_aos(oecl_grid_t g, const T *__restrict__ a, uint32
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const T *r = a + 6 * (uint64_t)i;
-    Weights<T> w = grid_weights<T>(r[0], r[1], (T)(1. / g.dx), (T)(1. / g.dy), (T)g.lxHalfExt,
-                                   (T)g.lyHalfExt, g.nx, g.ny);
+    Weights<double> w = grid_weights<double>((double)r[0], (double)r[1], 1. / g.dx, 1. / g.dy, g.lxHalfExt,
+                                             g.lyHalfExt, g.nx, g.ny);
     double wt = (double)r[5];
     unsafeAtomicAdd(rho + w.cell, (double)w.wLL * wt * q);
     unsafeAtomicAdd(rho + w.cell + 1, (double)w.wUL * wt * q);
@@ -152,6 +155,8 @@ __global__ void __launch_bounds__(256) k_push_fused(oecl_grid_t g, WallDev wall,
         C x = p(0, i), y = p(1, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
         Weights<C> w = grid_weights<C>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
         gather_push<C, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vx, vy, vz);
+        x = (C)(T)x;  // the wall test sees the position as stored (FP32 storage: rounded)
+        y = (C)(T)y;
         uint8_t in = wall_inside<C>(wall, x, y);  // may clip x, y (cut-cell wall)
         p(0, i) = (T)x;
         p(1, i) = (T)y;

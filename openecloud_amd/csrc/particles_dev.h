@@ -47,6 +47,10 @@ __device__ __forceinline__ void deposit_corners(double *rho, const Weights<T> &w
     unsafeAtomicAdd(rho + w.cell + nx + 1, (double)w.wUU * wt * q);
 }
 
+#ifndef OECL_DEPOSIT_RUN_SUMS
+#define OECL_DEPOSIT_RUN_SUMS 1
+#endif
+
 // ---- wave-level reductions through DPP (data-parallel primitives: VALU lane moves within
 // rows of 16 lanes, row_bcast across rows), which cost no LDS bandwidth.  All 64 lanes of
 // the wave must be active when these are called.
@@ -57,19 +61,35 @@ __device__ __forceinline__ uint32_t dpp_u32(uint32_t v, uint32_t ident)
     return (uint32_t)__builtin_amdgcn_update_dpp((int)ident, (int)v, CTRL, ROWMASK, 0xF, false);
 }
 
+// Lane move with every row written: lanes whose source lies outside their row read 0
+// (bound_ctrl), so no identity has to be placed in the destination first.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp0_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_any_u32(uint32_t v)
+{
+    if (ROWMASK == 0xF) return dpp0_u32<CTRL>(v);
+    return dpp_u32<CTRL, ROWMASK>(v, 0u);
+}
+
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ double dpp_f64(double v)
 {
     const uint32_t lo = (uint32_t)__double2loint(v), hi = (uint32_t)__double2hiint(v);
-    return __hiloint2double((int)dpp_u32<CTRL, ROWMASK>(hi, 0u), (int)dpp_u32<CTRL, ROWMASK>(lo, 0u));
+    return __hiloint2double((int)dpp_any_u32<CTRL, ROWMASK>(hi), (int)dpp_any_u32<CTRL, ROWMASK>(lo));
 }
 
 // One step of a segmented inclusive scan: a lane that does not start a segment inside its
-// current range adds its predecessor's partial sums and inherits its start flag.
+// current range adds its predecessor's partial sums and inherits its start flag.  Lanes
+// without a predecessor in range receive 0 for both.
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ void seg_scan_step(uint32_t &f, double &q0, double &q1, double &q2, double &q3)
 {
-    const uint32_t fp = dpp_u32<CTRL, ROWMASK>(f, 0u);
+    const uint32_t fp = dpp_any_u32<CTRL, ROWMASK>(f);
     const double p0 = dpp_f64<CTRL, ROWMASK>(q0), p1 = dpp_f64<CTRL, ROWMASK>(q1);
     const double p2 = dpp_f64<CTRL, ROWMASK>(q2), p3 = dpp_f64<CTRL, ROWMASK>(q3);
     if (!f) {
@@ -188,7 +208,11 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t c = valid ? ix + iy * nx : 0xFFFFFFFFu;
     if (!valid) q00 = q10 = q01 = q11 = 0.;
+#if OECL_DEPOSIT_RUN_SUMS
     const bool head = wave_run_sum(c, q00, q10, q01, q11);
+#else
+    const bool head = valid;
+#endif
     const uint32_t X0 = wave_minmax<true>(head ? ix : 0xFFFFFFFFu), X1 = wave_minmax<false>(head ? ix : 0u);
     const uint32_t Y0 = wave_minmax<true>(head ? iy : 0xFFFFFFFFu), Y1 = wave_minmax<false>(head ? iy : 0u);
     if (threadIdx.x == 0) {

@@ -323,3 +323,130 @@ OECL_EXPORT int oecl_permute_particles(int dtype, const void *src, void *dst, ui
     OECL_CHECK_LAUNCH("oecl_permute_particles");
     return OECL_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// The reference's deposit order (particles.pyx:336-351): rho is zeroed, then each particle in
+// index order adds (w_corner * weight) * q to its four corners.  Every node therefore receives
+// its contributions in increasing particle index, summed from 0.  The ordered deposit
+// reproduces that sum exactly: a stable sort of the particle indices by cell gives, for every
+// cell, its particles in index order; node j's contributions come from four cells (j as the
+// lower-left corner, j - 1 as lower-right, j - nx as upper-left, j - nx - 1 as upper-right),
+// and one thread per node merges those four lists by particle index.
+namespace {
+
+__global__ void k_zero_u32(uint32_t *a, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// inCell from calcGridWeights is at most np - nx - 2 (the cell index is clamped to the grid), so
+// every list below stays inside the node arrays; an index past np is clamped to the last node.
+__global__ void k_ordered_keys(const uint32_t *__restrict__ inCell, uint32_t n, uint32_t np, uint64_t *keys,
+                               uint32_t *vals, uint32_t *counts)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = inCell[i];
+    if (c >= np) c = np - 1;
+    keys[i] = c;
+    vals[i] = i;
+    atomicAdd(counts + c, 1u);
+}
+
+template <typename T>
+__global__ void k_ordered_nodes(Soa<T> p, const T *__restrict__ w4, uint64_t ws, const uint32_t *__restrict__ perm,
+                                const uint32_t *__restrict__ start, uint32_t nx, uint32_t np, double q,
+                                double *__restrict__ rho)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= np) return;
+    // list r: the particles of cell j - off[r], whose corner r is node j
+    uint32_t a[4], e[4];
+    const uint32_t off[4] = {0u, 1u, nx, nx + 1u};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        if (j >= off[r]) {
+            const uint32_t c = j - off[r];
+            a[r] = start[c];
+            e[r] = start[c + 1];
+        } else {
+            a[r] = e[r] = 0;
+        }
+    }
+    double acc = 0.;
+    for (;;) {
+        int h = -1;
+        uint32_t best = 0xFFFFFFFFu;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (a[r] < e[r]) {
+                const uint32_t i = perm[a[r]];
+                if (i < best) {
+                    best = i;
+                    h = r;
+                }
+            }
+        }
+        if (h < 0) break;
+        acc += (double)w4[(uint64_t)h * ws + best] * (double)p(5, best) * q;
+        ++a[h];
+    }
+    rho[j] = acc;
+}
+
+uint64_t align16(uint64_t b) { return (b + 15) & ~15ull; }
+
+}  // namespace
+
+OECL_EXPORT uint64_t oecl_charge_ordered_work_bytes(uint32_t n, uint32_t np)
+{
+    return align16(8ull * n) + align16(4ull * n) + align16(4ull * (np + 2)) + align16(4ull * (np + 2)) +
+           align16(oecl_scan_work_bytes(np + 1)) + align16(oecl_sort_work_bytes(n)) + 64;
+}
+
+OECL_EXPORT int oecl_charge_to_grid_ordered(const oecl_grid_t *g, int dtype, const void *data, uint64_t stride,
+                                            uint32_t n, const uint32_t *inCell, const void *w4, uint64_t wstride,
+                                            double q, double *rho, void *work, void *stream)
+{
+    if (!g || !rho || !work || g->np < 1 || g->nx < 2) return OECL_ERR_ARG;
+    if (n > 0 && (!data || !inCell || !w4)) return OECL_ERR_ARG;
+    if (dtype != OECL_F64 && dtype != OECL_F32) return OECL_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t np = g->np;
+    char *w = (char *)work;
+    uint64_t *keys = (uint64_t *)w;
+    w += align16(8ull * n);
+    uint32_t *perm = (uint32_t *)w;
+    w += align16(4ull * n);
+    uint32_t *counts = (uint32_t *)w;
+    w += align16(4ull * (np + 2));
+    uint32_t *start = (uint32_t *)w;
+    w += align16(4ull * (np + 2));
+    uint32_t *scanw = (uint32_t *)w;
+    w += align16(oecl_scan_work_bytes(np + 1));
+    void *sortw = w;
+    hipLaunchKernelGGL(k_zero_u32, dim3(oecl_blocks(np + 2, 256)), dim3(256), 0, s, counts, np + 2);
+    if (n > 0)
+        hipLaunchKernelGGL(k_ordered_keys, dim3(oecl_blocks(n, 256)), dim3(256), 0, s, inCell, n, np, keys, perm,
+                           counts);
+    int bits = 1;
+    while ((1ull << bits) < np) ++bits;
+    if (n > 1) {
+        int rc = oecl_radix_sort_u64(keys, perm, n, bits, sortw, stream);
+        if (rc) return rc;
+    }
+    // start[c] = particles in cells before c; start[np] = n
+    int rc = oecl_launch_scan_u32(counts, np, start, scanw, s);
+    if (rc) return rc;
+    if (dtype == OECL_F64)
+        hipLaunchKernelGGL(k_ordered_nodes<double>, dim3(oecl_blocks(np, 256)), dim3(256), 0, s,
+                           Soa<double>{(double *)data, stride}, (const double *)w4, wstride, perm, start, g->nx, np,
+                           q, rho);
+    else
+        hipLaunchKernelGGL(k_ordered_nodes<float>, dim3(oecl_blocks(np, 256)), dim3(256), 0, s,
+                           Soa<float>{(float *)data, stride}, (const float *)w4, wstride, perm, start, g->nx, np, q,
+                           rho);
+    OECL_CHECK_LAUNCH("oecl_charge_to_grid_ordered");
+    return OECL_OK;
+}

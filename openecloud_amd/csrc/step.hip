@@ -128,15 +128,14 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
 // cell sort, so that XCD's L2 holds that part of E and rho.
 
 // Holes (absorbed particles) of one wave in index order: the wave's count goes to
-// counts[w], the indices to list[w * 64 + rank] (w = global wave index of logical block blk),
-// and a wave with holes appends w to nzList (in no particular order).  All 64 lanes of the
-// wave call it.
-__device__ __forceinline__ void wave_holes(bool dead, uint32_t i, uint32_t blk, uint32_t *__restrict__ counts,
+// counts[w], the indices to list[w * 64 + rank] (w = global index of the 64-particle group the
+// wave handles), and a wave with holes appends w to nzList (in no particular order).  All 64
+// lanes of the wave call it.
+__device__ __forceinline__ void wave_holes(bool dead, uint32_t i, uint32_t w, uint32_t *__restrict__ counts,
                                            uint32_t *__restrict__ list, uint32_t *__restrict__ misc,
                                            uint32_t *__restrict__ nzList)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t w = blk * (PB / 64) + (threadIdx.x >> 6);
     const uint64_t bal = __ballot(dead);
     if (dead) list[(size_t)w * 64 + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = i;
     if (lane == 0) {
@@ -146,10 +145,19 @@ __device__ __forceinline__ void wave_holes(bool dead, uint32_t i, uint32_t blk, 
     }
 }
 
-// Gather + Boris push + wall test + hole list + deposit of the survivors, one thread per
-// particle slot; the block's run gives its E, rho, counters and live count.
+#ifndef OECL_PUSH_PER_THREAD
+#define OECL_PUSH_PER_THREAD 1
+#endif
+// Particles per push thread: the block's PB particles are handled by PB / KP threads, each
+// taking particles tid, tid + PB / KP, ... (two independent chains of dependent loads and
+// arithmetic per thread when KP = 2).
+constexpr int KP = OECL_PUSH_PER_THREAD;
+constexpr uint32_t PT = PB / KP;   // threads per push block
+
+// Gather + Boris push + wall test + hole list + deposit of the survivors, KP particle slots per
+// thread; the block's run gives its E, rho, counters and live count.
 template <typename T, int TYPEB>
-__global__ void __launch_bounds__(PB)
+__global__ void __launch_bounds__(PT)
     k_push_dev(Runs runs, oecl_grid_t g, WallDev wall, Soa<T> p, const uint32_t *__restrict__ nDev,
                const double *__restrict__ E, const double *__restrict__ Eil, const double *__restrict__ B, double b0, double b1,
                double b2, double dt, double k, double q, uint8_t *__restrict__ isInside,
@@ -164,42 +172,56 @@ __global__ void __launch_bounds__(PB)
     const uint32_t n = nDev[run];
     const uint64_t first = (uint64_t)blk * PB;
     if (first - sBeg >= n) return;  // whole block past the run's live particles
-    const uint32_t i = (uint32_t)first + threadIdx.x;
-    const bool live = i - sBeg < n;
     E += (size_t)run * 2 * g.np;
     if (Eil) Eil += (size_t)run * 2 * g.np;
     rho += (size_t)run * g.np;
     misc += run * M_COUNT;
     nzList += sBeg / 64;
     // FP32 storage is widened on load: weights, gather and the Boris rotation run in FP64
-    // registers (the kernel is bound by memory traffic, not arithmetic).
+    // registers.
     using C = double;
     const C dxi = 1. / g.dx, dyi = 1. / g.dy, lxh = g.lxHalfExt, lyh = g.lyHalfExt;
-    bool dep = false, dead = false;
-    Weights<C> w2{};
-    double wq = 0.;
-    if (live) {
-        C x = p(0, i), y = p(1, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
-        Weights<C> w = grid_weights<C>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
-        gather_push<C, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vx, vy, vz, Eil);
-        uint8_t in = wall_inside<C>(wall, x, y);
-        p(0, i) = (T)x;
-        p(1, i) = (T)y;
-        p(2, i) = (T)vx;
-        p(3, i) = (T)vy;
-        if (TYPEB != 0) p(4, i) = (T)vz;
-        isInside[i] = in;
-        if (in) {
-            dep = true;
-            w2 = grid_weights<C>((C)(T)x, (C)(T)y, dxi, dyi, lxh, lyh, g.nx, g.ny);
-            wq = (double)p(5, i);
-        } else {
-            dead = true;
+    bool dep[KP], dead[KP];
+    Weights<C> w2[KP];
+    double wq[KP];
+#pragma unroll
+    for (int kk = 0; kk < KP; ++kk) {
+        const uint32_t i = (uint32_t)first + kk * PT + threadIdx.x;
+        dep[kk] = dead[kk] = false;
+        w2[kk] = Weights<C>{};
+        wq[kk] = 0.;
+        if (i - sBeg < n) {
+            C x = p(0, i), y = p(1, i), vx = p(2, i), vy = p(3, i), vz = p(4, i);
+            Weights<C> w = grid_weights<C>(x, y, dxi, dyi, lxh, lyh, g.nx, g.ny);
+            gather_push<C, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vx, vy, vz, Eil);
+            // the wall test sees the position as stored (FP32 storage: rounded)
+            x = (C)(T)x;
+            y = (C)(T)y;
+            uint8_t in = wall_inside<C>(wall, x, y);
+            p(0, i) = (T)x;
+            p(1, i) = (T)y;
+            p(2, i) = (T)vx;
+            p(3, i) = (T)vy;
+            if (TYPEB != 0) p(4, i) = (T)vz;
+            isInside[i] = in;
+            if (in) {
+                dep[kk] = true;
+                w2[kk] = grid_weights<C>((C)(T)x, (C)(T)y, dxi, dyi, lxh, lyh, g.nx, g.ny);
+                wq[kk] = (double)p(5, i);
+            } else {
+                dead[kk] = true;
+            }
         }
     }
-    wave_holes(dead, i, blk, counts, list, misc, nzList);
-    block_deposit<PB / 64, 512>(rho, g.nx, dep, w2.ix, w2.iy, w2.wLL * wq * q, w2.wUL * wq * q, w2.wLU * wq * q,
-                                w2.wUU * wq * q, tiles, dbox);
+#pragma unroll
+    for (int kk = 0; kk < KP; ++kk) {
+        const uint32_t i = (uint32_t)first + kk * PT + threadIdx.x;
+        wave_holes(dead[kk], i, blk * (PB / 64) + kk * (PT / 64) + (threadIdx.x >> 6), counts, list, misc, nzList);
+        if (kk) __syncthreads();   // the previous deposit's readers of the tile and box are done
+        block_deposit<PT / 64, 512 * KP>(rho, g.nx, dep[kk], w2[kk].ix, w2[kk].iy, w2[kk].wLL * wq[kk] * q,
+                                         w2[kk].wUL * wq[kk] * q, w2[kk].wLU * wq[kk] * q, w2[kk].wUU * wq[kk] * q,
+                                         tiles, dbox);
+    }
 }
 
 // Exclusive prefix of v over the CT threads of the block (wsum: CT/64 LDS words); *tot the sum.
@@ -391,8 +413,8 @@ __device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const
     bool found;
     hit_geom(w, dt, x, y, a[2], a[3], r, n0, n1, found);
     if (!found) atomicAdd(stats + 3, 1u);
-    a[0] = (T)x;
-    a[1] = (T)y;
+    a[0] = wall_store<T>(x);
+    a[1] = wall_store<T>(y);
     rts[jj] = (T)r;
     T *nn = nv + 2 * (uint64_t)jj;
     nn[0] = (T)n0;
@@ -540,9 +562,10 @@ __global__ void __launch_bounds__(CT)
     __syncthreads();
     // secondary s (or photoelectron s - carry): deposit, copy out, place
     auto place = [&](uint32_t s, const T *r) {
-        Weights<T> w = grid_weights<T>(r[0], r[1], (T)(1. / g.dx), (T)(1. / g.dy), (T)g.lxHalfExt,
-                                       (T)g.lyHalfExt, g.nx, g.ny);
-        deposit_corners<T>(rho, w, g.nx, (double)r[5], q);
+        // FP64 weights of the stored position, as the push deposits the survivors
+        Weights<double> w = grid_weights<double>((double)r[0], (double)r[1], 1. / g.dx, 1. / g.dy, g.lxHalfExt,
+                                                 g.lyHalfExt, g.nx, g.ny);
+        deposit_corners<double>(rho, w, g.nx, (double)r[5], q);
         if (secOut) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) secOut[6 * (uint64_t)s + c] = r[c];
@@ -693,7 +716,7 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
                        a->e2nCol, a->e2nVal, a->phi, np, a->E, a->Eil);
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
-    const dim3 gp((unsigned)nBlk), bp(PB);
+    const dim3 gp((unsigned)nBlk), bp(PT);
     if (a->evPushBegin) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushBegin, s), "event");
     if (a->typeB == 0)
         hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,

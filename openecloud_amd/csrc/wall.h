@@ -73,6 +73,25 @@ __device__ __forceinline__ uint8_t cutcell_inside(const WallDev &w, double x, do
 
 // indexInside for one particle.  The cut-cell wall first clips the position to the grid
 // (particleBoundary.pyx:349-351 via _clipParticleDataToGrid :485-508) and writes it back.
+// A point on the wall (the interaction point of an absorbed particle, from which secondaries
+// start) in the particle dtype.  FP64 keeps it as computed.  FP32 rounds toward zero: the
+// rectangular and elliptical chambers are centred on the origin, so the rounded point stays
+// inside whenever the FP64 point is.  (The wall search insets the point by 1e-6 of a cell, 4e-11 m
+// at 45 mm, far below FP32 resolution there: rounded to nearest, half of the points would lie
+// outside and their secondaries be absorbed again at once.)
+template <typename T>
+__device__ __forceinline__ T wall_store(double v);
+template <>
+__device__ __forceinline__ double wall_store<double>(double v)
+{
+    return v;
+}
+template <>
+__device__ __forceinline__ float wall_store<float>(double v)
+{
+    return __double2float_rz(v);
+}
+
 template <typename T>
 __device__ __forceinline__ uint8_t wall_inside(const WallDev &w, T &x, T &y)
 {

@@ -32,9 +32,14 @@ class Particles:
     ordering: 'reference' reproduces the reference's particle order after
     addAndRemoveParticles exactly (including its capacity-dependent choice between the
     in-place and the new-array rule); 'fast' always uses the hole-filling rule, which moves
-    only the particles that must move and yields the same particle set."""
+    only the particles that must move and yields the same particle set.
 
-    def __init__(self, particleSpecies, particleType, dtype=torch.float64, ordering='reference'):
+    orderedDeposit: chargeToGrid sums every node's contributions in particle index order, as
+    the reference's sequential loop does (particles.pyx:336-351), so the charge on the grid is
+    bit-identical to it; the default atomic deposit agrees to rounding (1e-13 relative)."""
+
+    def __init__(self, particleSpecies, particleType, dtype=torch.float64, ordering='reference',
+                 orderedDeposit=False):
         L.require_gpu()
         L.lib()
         self.macroParticleCount = 0
@@ -55,6 +60,8 @@ class Particles:
         self.dtype = dtype
         self.code = L.dtype_code(dtype)
         self.ordering = ordering
+        self.orderedDeposit = bool(orderedDeposit)
+        self._ordWork = None
         self.dt = 0.
         self.bConst = np.zeros(3)
         self._refCapacity = 0
@@ -192,11 +199,26 @@ class Particles:
         npx = gridObj.getNpExt()
         if self.chargeOnGrid.shape[0] < npx:
             self.chargeOnGrid = torch.empty(npx, dtype=torch.float64, device=L.device())
+        self._charge(gridObj, self.chargeOnGrid)
+
+    def _charge(self, gridObj, rho):
+        """The charge of the particles on the grid into rho, from the weights of the last
+        calcGridWeights, atomic or in the reference's order (orderedDeposit)."""
         g = L.grid_desc(gridObj)
+        n = self.macroParticleCount
+        if self.orderedDeposit:
+            need = int(L.lib().oecl_charge_ordered_work_bytes(L.u32(n), L.u32(gridObj.getNpExt()))) // 8 + 8
+            if self._ordWork is None or self._ordWork.numel() < need:
+                self._ordWork = torch.empty(int(need * 1.25), dtype=torch.float64, device=L.device())
+            L.check(L.lib().oecl_charge_to_grid_ordered(
+                ctypes.byref(g), L.i32(self.code), L.ptr(self._data), L.u64(self.capacity), L.u32(n),
+                L.ptr(self.inCell), L.ptr(self._w4), L.u64(self.capacity), L.f64(self.particleCharge),
+                L.ptr(rho), L.ptr(self._ordWork), L.stream()), 'chargeToGrid (ordered)')
+            return
         L.check(L.lib().oecl_charge_to_grid(
             ctypes.byref(g), L.i32(self.code), L.ptr(self._data), L.u64(self.capacity),
-            L.u32(self.macroParticleCount), L.ptr(self.inCell), L.ptr(self._w4),
-            L.u64(self.capacity), L.f64(self.particleCharge), L.ptr(self.chargeOnGrid),
+            L.u32(n), L.ptr(self.inCell), L.ptr(self._w4),
+            L.u64(self.capacity), L.f64(self.particleCharge), L.ptr(rho),
             L.stream()), 'chargeToGrid')
 
     def _field(self, gridObj, field, dims, out):

@@ -59,7 +59,7 @@ def test_graph_replays_match_eager_steps(photo):
         got = p2.getParticleData().cpu().numpy()
         assert got.shape == ref[r].shape, (r, got.shape, ref[r].shape)
         scale = np.abs(ref[r]).max(axis=0)
-        assert (np.abs(got - ref[r]) / scale).max() < 1e-11, r
+        assert (np.abs(got - ref[r]) / scale).max() < 1e-12, r
         if r == 1:
             b2.sort_particles(keepBuffers=True)   # the graph's pointers stay valid
     assert st['absorbed_total'] > 0

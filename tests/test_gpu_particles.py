@@ -54,6 +54,41 @@ def test_weights_deposit_gather_bitexact(grids, orc, gname):
     assert abs(rho.sum() - (-1.602176565e-19) * pd[:, 5].sum()) <= 1e-12 * abs(rho.sum())
 
 
+@pytest.mark.parametrize('gname,n,dtype', [('rect', 5000, 'f64'), ('ellip', 200000, 'f64'), ('rect', 1, 'f64'),
+                                            ('ellip', 30000, 'f32'), ('rect', 0, 'f64')])
+def test_ordered_deposit_is_the_sequential_sum(grids, orc, gname, n, dtype):
+    """orderedDeposit: rho bit-identical to the reference's sequential loop
+    (particles.pyx:336-351), for many particles per cell, a single particle, none, and FP32
+    storage (weights and products in FP64 from the stored FP32 values, as the oracle does
+    from the same values)."""
+    from openecloud_amd.particles import Particles
+    g = grids[gname]
+    tdt = torch.float64 if dtype == 'f64' else torch.float32
+    pd = random_particles(g, max(n, 1), seed=21, wall='ellip' if gname == 'ellip' else 'rect')[:n]
+    if dtype == 'f32':
+        pd = pd.astype(np.float32).astype(np.float64)
+    p = Particles('electrons', 'variableWeight', dtype=tdt, orderedDeposit=True)
+    p.setParticleData(pd)
+    p.calcGridWeights(g)
+    p.chargeToGrid(g)
+    rho = p.getChargeOnGrid().cpu().numpy()
+    if n == 0:
+        assert np.all(rho == 0.)
+        return
+    w4 = [p._w4[k, :n].to(torch.float64).cpu().numpy() for k in range(4)]
+    ic = p.getInCell().cpu().numpy().astype(np.uint32)
+    if dtype == 'f64':
+        ic0, w0 = orc.calc_grid_weights(pd, g.getDx(), g.getDy(), g.getLxExt(), g.getLyExt(), g.getNxExt())
+        assert np.array_equal(ic, ic0) and all(np.array_equal(a, b) for a, b in zip(w4, w0))
+    rho_o = orc.charge_to_grid(pd, ic, w4, -1.602176565e-19, g.getNxExt(), g.getNpExt())
+    assert np.array_equal(rho, rho_o)
+    # the atomic deposit of the same particles differs only by summation order
+    p.orderedDeposit = False
+    p.chargeToGrid(g)
+    rho_a = p.getChargeOnGrid().cpu().numpy()
+    assert np.allclose(rho_a, rho_o, rtol=1e-13, atol=1e-13 * np.abs(rho_o).max())
+
+
 @pytest.mark.parametrize('typeB', [0, 1, 2])
 def test_boris_push_bitexact(grids, orc, typeB):
     g = grids['rect']

@@ -85,7 +85,7 @@ def test_scan_runs_match_standalone_native_steps():
         st = sc.sync()
         for r in range(3):
             got = sc.run_particles(r).cpu().numpy()
-            assert _close(got, ref[r][it], 1e-11), (it, r)
+            assert _close(got, ref[r][it], 1e-12), (it, r)
     assert all(a > 0 for a in st['absorbed_total']) and sum(st['emitted_total']) > 0
     assert st['counts'] == [ref[r][-1].shape[0] for r in range(3)]
     # the per-run histories hold each step's starting count
@@ -105,7 +105,7 @@ def test_scan_graph_replays_match_eager_steps():
     st = sc.sync()
     assert sc.it == eager.it == 8
     for r in range(3):
-        assert _close(sc.run_particles(r).cpu().numpy(), eager.run_particles(r).cpu().numpy(), 1e-11), r
+        assert _close(sc.run_particles(r).cpu().numpy(), eager.run_particles(r).cpu().numpy(), 1e-12), r
     assert st['counts'] == eager.counts()
 
 
