@@ -314,17 +314,27 @@ int oecl_split_heaviest(const oecl_wall_t *wall, int dtype, void *data, uint64_t
                         uint32_t first, uint32_t cnt, double dx, double dy, double lx, double ly,
                         uint32_t streamIndex, uint32_t seed0, uint32_t seed1, void *add,
                         void *stream);
-/* localRandom's merge (:166-236) over particles [0, nc): Z-order (Morton) sort of the 5-D
- * coordinates scaled by pw[5] (device), groups of G (2..8) Z-neighbours within squared
- * distance disLimit of the group's first member, lightest groups first, until at least
- * toMerge particles are removed.  keep[nc] written; *merged (device) = removed count;
- * *groupsUsed (host) = merged groups.  work: oecl_merge_work_bytes(nc).  This call reads
- * the sorted group list back to the host once (it synchronises the stream). */
+/* localRandom's merge (particleManagement.pyx:166-236, kdTree.pyx:146-255) over particles
+ * [0, nc), sorted by weight ascending (rank = index, lightest first).  The reference's rule:
+ * a particle merges with its nearest neighbour in the 5-D space (x, y, vx, vy, vz) scaled by
+ * pw[5] (device) when the squared distance is below disLimit (:185 compares the squared
+ * distance with factorDisLimit * ds); the survivor is one of the two with probability
+ * proportional to weight and takes the pair's weight (scheme 1: velocity rescaled to keep the
+ * pair's kinetic energy, :209-232); merged particles leave the pool.  Run as rounds of parallel
+ * matching: each pooled particle finds its exact nearest neighbour in the pool through a
+ * Morton-ordered tree (Z-order sort of the scaled points, leaves of 8, an implicit binary tree
+ * of bounding boxes, branch-and-bound queries), a pair holds both of its particles only when
+ * it is the lowest-ranked claim on each, and accepted pairs are taken in rank order until
+ * toMerge merges are made or no particle finds a neighbour.  G (2..8): the survivor stays in
+ * the pool while its group has fewer than G particles (2: the reference's pairs).  keep[nc]
+ * written; *merged (device) and *hostMerged = merges made; *hostRounds = rounds; pairs
+ * (optional, [toMerge][2]) receives (survivor, removed) per merge.  work:
+ * oecl_merge_work_bytes(nc).  Synchronises the stream once per round. */
 uint64_t oecl_merge_work_bytes(uint32_t nc);
-int oecl_morton_merge(int dtype, void *data, uint64_t stride, uint32_t nc, const double *pw,
-                      int G, double disLimit, uint32_t toMerge, int scheme, uint32_t streamIndex,
-                      uint32_t seed0, uint32_t seed1, uint8_t *keep, uint32_t *merged,
-                      uint32_t *groupsUsed, void *work, void *stream);
+int oecl_nn_merge(int dtype, void *data, uint64_t stride, uint32_t nc, const double *pw, int G,
+                  double disLimit, uint32_t toMerge, int scheme, uint32_t streamIndex, uint32_t seed0,
+                  uint32_t seed1, uint8_t *keep, uint32_t *merged, uint32_t *pairs, uint32_t *hostMerged,
+                  uint32_t *hostRounds, void *work, void *stream);
 
 /* ---- poissonSolver.pyx -------------------------------------------------------------- */
 

@@ -2,12 +2,20 @@
 //
 // globalRandom: weight rescale, then a uniformly random kill set (the k smallest of per-
 // particle Philox keys, ties broken by index) or uniformly random clones.
-// localRandom: the reference's weight thresholds and split rule; the merge replaces the
-// serial kd-tree nearest-neighbour search by a Morton (Z-order) sort of the quantised 5-D
-// weighted phase-space coordinates, after which particles that are neighbours in Z order and
-// within the reference's distance limit are merged in groups of up to G, lightest groups
-// first.  Every merge gives the survivor the sum of the group's weights.
+// localRandom: the reference's weight thresholds and split rule; the merge keeps the reference's
+// rule -- each particle, lightest first, is merged with its nearest neighbour in the weighted
+// 5-D phase space (x, y, lambda vx, lambda vy, lambda vz) when the squared distance is below
+// the limit, the survivor chosen with probability proportional to weight and given the pair's
+// total weight -- and replaces the serial kd-tree loop by rounds of parallel matching: every
+// particle still in the pool finds its exact nearest neighbour in the pool through a
+// Morton-ordered tree (the weighted points sorted by their 5-D Z-order code, leaves of 8
+// consecutive points, an implicit binary tree of bounding boxes over the leaves, branch and
+// bound queries), a pair holds both of its particles only when it is the lightest-ranked claim
+// on each (atomic minimum locks), and the accepted pairs are taken in rank order until the
+// requested number of merges is reached.
 #include "common.h"
+#include "particles_dev.h"
+#include "scan.h"
 #include "wall.h"
 
 #define PURPOSE_MANAGE 3u
@@ -112,44 +120,57 @@ __global__ void k_split(Soa<T> p, uint32_t first, uint32_t cnt, WallDev w, doubl
 }
 
 // Per-block min / max of the 5 weighted coordinates over [0, n).
-template <typename T>
-__global__ void __launch_bounds__(256) k_bbox(Soa<T> p, uint32_t n, const double *__restrict__ pw,
-                                              double *__restrict__ part)
+constexpr uint32_t NN_NONE = 0xFFFFFFFFu;    // no proposal (out of the pool)
+constexpr uint32_t NN_RETIRE = 0xFFFFFFFEu;  // no neighbour within the limit: leaves the pool
+constexpr uint32_t NN_MERGED = 0xFFFFFFFEu;  // lock value of a particle merged in this round
+constexpr uint32_t NN_LEAF = 8;              // points per leaf of the tree
+constexpr int NN_STACK = 64;                 // traversal stack (tree depth <= 30)
+
+__global__ void k_zero_u32(uint32_t *a, uint32_t n)
 {
-    __shared__ double lo[256][5], hi[256][5];
-    double l[5], h[5];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+// Bounding box of the weighted 5-D points: per-block partial minima and maxima, then a final pass.
+template <typename T>
+__global__ void __launch_bounds__(256) k_bbox(Soa<T> p, uint32_t n, const double *__restrict__ pw, double *part)
+{
+    __shared__ double sh[10][256];
+    double lo[5], hi[5];
+#pragma unroll
     for (int c = 0; c < 5; ++c) {
-        l[c] = INFINITY;
-        h[c] = -INFINITY;
+        lo[c] = INFINITY;
+        hi[c] = -INFINITY;
     }
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+#pragma unroll
         for (int c = 0; c < 5; ++c) {
-            double v = (double)p(c, i) * pw[c];
-            l[c] = fmin(l[c], v);
-            h[c] = fmax(h[c], v);
+            const double v = (double)p(c, i) * pw[c];
+            lo[c] = fmin(lo[c], v);
+            hi[c] = fmax(hi[c], v);
         }
+#pragma unroll
     for (int c = 0; c < 5; ++c) {
-        lo[threadIdx.x][c] = l[c];
-        hi[threadIdx.x][c] = h[c];
+        sh[c][threadIdx.x] = lo[c];
+        sh[5 + c][threadIdx.x] = hi[c];
     }
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st)
+        if ((int)threadIdx.x < st)
+#pragma unroll
             for (int c = 0; c < 5; ++c) {
-                lo[threadIdx.x][c] = fmin(lo[threadIdx.x][c], lo[threadIdx.x + st][c]);
-                hi[threadIdx.x][c] = fmax(hi[threadIdx.x][c], hi[threadIdx.x + st][c]);
+                sh[c][threadIdx.x] = fmin(sh[c][threadIdx.x], sh[c][threadIdx.x + st]);
+                sh[5 + c][threadIdx.x] = fmax(sh[5 + c][threadIdx.x], sh[5 + c][threadIdx.x + st]);
             }
         __syncthreads();
     }
-    if (threadIdx.x < 5) {
-        part[blockIdx.x * 10 + threadIdx.x] = lo[0][threadIdx.x];
-        part[blockIdx.x * 10 + 5 + threadIdx.x] = hi[0][threadIdx.x];
-    }
+    if (threadIdx.x < 10) part[blockIdx.x * 10 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 __global__ void k_bbox_final(const double *part, uint32_t nb, double *box)
 {
-    int c = threadIdx.x;
+    const int c = threadIdx.x;
     if (c >= 5) return;
     double l = INFINITY, h = -INFINITY;
     for (uint32_t b = 0; b < nb; ++b) {
@@ -162,120 +183,230 @@ __global__ void k_bbox_final(const double *part, uint32_t nb, double *box)
 
 __device__ __forceinline__ uint64_t spread5(uint32_t v)
 {
-    // places bit b of a 12-bit value at bit 5*b
+    // bit b of a 12-bit value goes to bit 5 b
     uint64_t r = 0;
 #pragma unroll
     for (int b = 0; b < 12; ++b) r |= (uint64_t)((v >> b) & 1u) << (5 * b);
     return r;
 }
 
+// 5-D Z-order code of each weighted point (12 bits per coordinate inside the bounding box).
 template <typename T>
 __global__ void k_morton(Soa<T> p, uint32_t n, const double *__restrict__ pw, const double *__restrict__ box,
-                         uint64_t *keys, uint32_t *vals)
+                         uint64_t *keys, uint32_t *vals, uint8_t *gsize)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint64_t key = 0;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        double lo = box[c], hi = box[5 + c];
-        double v = (double)p(c, i) * pw[c];
-        double f = hi > lo ? (v - lo) / (hi - lo) : 0.;
-        uint32_t q = (uint32_t)fmin(4095., fmax(0., f * 4096.));
+        const double lo = box[c], hi = box[5 + c];
+        const double v = (double)p(c, i) * pw[c];
+        const double f = hi > lo ? (v - lo) / (hi - lo) : 0.;
+        const uint32_t q = (uint32_t)fmin(4095., fmax(0., f * 4096.));
         key |= spread5(q) << c;
     }
     keys[i] = key;
     vals[i] = i;
+    gsize[i] = 1;
 }
 
-// Group g = Z-order positions [G g, G g + G).  Members within sqrt(disLimit) (squared
-// distance compared with disLimit, as particleManagement.pyx:185 compares) of the group's
-// first member are eligible; the group's sort key is the weight of its lightest member,
-// or all ones when fewer than two members are eligible.  elig[g] is a member bit mask.
+// Leaf boxes: leaf l holds sorted positions [8 l, 8 l + 8) (those below nc); node L + l of the
+// heap-ordered tree (root 1, children 2k and 2k + 1).  Empty leaves get an empty box.
 template <typename T>
-__global__ void k_group_keys(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ zperm,
-                             const double *__restrict__ pw, int G, double disLimit, uint64_t *gkeys,
-                             uint32_t *gvals, uint8_t *elig)
+__global__ void k_leaf_boxes(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ zperm, const double *__restrict__ pw,
+                             uint32_t L, double *blo, double *bhi)
 {
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t ng = (nc + G - 1) / G;
-    if (g >= ng) return;
-    uint32_t a = zperm[(uint64_t)g * G];
-    double pa[5];
-    for (int c = 0; c < 5; ++c) pa[c] = (double)p(c, a) * pw[c];
-    double wmin = (double)p(5, a);
-    uint8_t mask = 1;
-    int cnt = 1;
-    for (int k = 1; k < G; ++k) {
-        uint64_t pos = (uint64_t)g * G + k;
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L) return;
+    double lo[5], hi[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        lo[c] = INFINITY;
+        hi[c] = -INFINITY;
+    }
+    for (uint32_t k = 0; k < NN_LEAF; ++k) {
+        const uint64_t pos = (uint64_t)l * NN_LEAF + k;
         if (pos >= nc) break;
-        uint32_t b = zperm[pos];
-        double d2 = 0.;
+        const uint32_t b = zperm[pos];
+#pragma unroll
         for (int c = 0; c < 5; ++c) {
-            double d = (double)p(c, b) * pw[c] - pa[c];
-            d2 += d * d;
-        }
-        if (d2 < disLimit) {
-            mask |= (uint8_t)(1u << k);
-            ++cnt;
-            wmin = fmin(wmin, (double)p(5, b));
+            const double v = (double)p(c, b) * pw[c];
+            lo[c] = fmin(lo[c], v);
+            hi[c] = fmax(hi[c], v);
         }
     }
-    elig[g] = mask;
-    gvals[g] = g;
-    gkeys[g] = cnt >= 2 ? (uint64_t)__double_as_longlong(wmin) : ~0ull;
+    const uint64_t node = (uint64_t)L + l;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        blo[node * 5 + c] = lo[c];
+        bhi[node * 5 + c] = hi[c];
+    }
 }
 
-// Merge the first `take` groups of the sorted group order.  Survivor chosen with
-// probability proportional to weight; it receives the group's total weight.  Scheme 1
-// (pairs) rescales the survivor's velocity as particleManagement.pyx:209-232 does.
-template <typename T>
-__global__ void k_group_merge(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ zperm,
-                              const uint32_t *__restrict__ gorder, const uint8_t *__restrict__ elig,
-                              uint32_t take, int G, int scheme, uint32_t stream, uint32_t s0, uint32_t s1,
-                              uint8_t *keep, uint32_t *merged)
+// Inner boxes of one level: nodes [first, first + cnt), each the union of its two children.
+__global__ void k_inner_boxes(uint32_t first, uint32_t cnt, double *blo, double *bhi)
 {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= take) return;
-    uint32_t g = gorder[r];
-    uint8_t mask = elig[g];
-    uint32_t idx[8];
-    double wsum = 0.;
-    int cnt = 0;
-    for (int k = 0; k < G; ++k)
-        if (mask & (1u << k)) {
-            uint64_t pos = (uint64_t)g * G + k;
-            if (pos >= nc) break;
-            idx[cnt] = zperm[pos];
-            wsum += (double)p(5, idx[cnt]);
-            ++cnt;
-        }
-    if (cnt < 2) return;
-    double u = philox_uniform(g, 4u, stream, PURPOSE_MANAGE + 2u, s0, s1) * wsum;
-    int sv = cnt - 1;
-    double acc = 0.;
-    for (int k = 0; k < cnt; ++k) {
-        acc += (double)p(5, idx[k]);
-        if (u < acc) {
-            sv = k;
-            break;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const uint64_t k = (uint64_t)first + t, a = 2 * k, b = 2 * k + 1;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        blo[k * 5 + c] = fmin(blo[a * 5 + c], blo[b * 5 + c]);
+        bhi[k * 5 + c] = fmax(bhi[a * 5 + c], bhi[b * 5 + c]);
+    }
+}
+
+__device__ __forceinline__ double box_dist2(const double *__restrict__ blo, const double *__restrict__ bhi,
+                                            uint64_t k, const double *q)
+{
+    double d2 = 0.;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const double lo = blo[k * 5 + c], hi = bhi[k * 5 + c];
+        const double d = q[c] < lo ? lo - q[c] : (q[c] > hi ? q[c] - hi : 0.);
+        d2 += d * d;
+    }
+    return d2;   // +inf for an empty box
+}
+
+// Proposal of candidate a (rank = index, lightest first): its exact nearest neighbour in the pool
+// (gsize != 0, groups of at most G particles together), ties to the lower index, found by branch
+// and bound over the tree (near child first, boxes farther than the best distance pruned); a
+// box containing particles that left the pool still bounds the pooled ones, so the search stays
+// exact.  NN_RETIRE when none lies within the squared-distance limit (particleManagement.pyx:185
+// compares the squared distance with factorDisLimit * ds).  Resets a's lock for this round.
+template <typename T>
+__global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ zperm, uint32_t L,
+                             const double *__restrict__ blo, const double *__restrict__ bhi,
+                             const double *__restrict__ pw, double disLimit, int G,
+                             const uint8_t *__restrict__ gsize, uint32_t *__restrict__ prop, uint32_t *__restrict__ lock,
+                             uint32_t *__restrict__ cnt)
+{
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nc) return;
+    lock[a] = NN_NONE;
+    const uint32_t sa = gsize[a];
+    if (sa == 0 || (int)sa >= G) {
+        prop[a] = NN_NONE;
+        return;
+    }
+    double q[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) q[c] = (double)p(c, a) * pw[c];
+    double best = INFINITY;
+    uint32_t bi = NN_NONE;
+    uint32_t stack[NN_STACK];
+    int sp = 0;
+    stack[sp++] = 1;
+    while (sp > 0) {
+        const uint32_t k = stack[--sp];
+        if (box_dist2(blo, bhi, k, q) > best) continue;
+        if (k >= L) {
+            const uint64_t base = (uint64_t)(k - L) * NN_LEAF;
+            for (uint32_t j = 0; j < NN_LEAF; ++j) {
+                const uint64_t pos = base + j;
+                if (pos >= nc) break;
+                const uint32_t b = zperm[pos];
+                if (b == a) continue;
+                const uint32_t sb = gsize[b];
+                if (sb == 0 || (int)(sa + sb) > G) continue;
+                double d2 = 0.;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) {
+                    const double d = (double)p(c, b) * pw[c] - q[c];
+                    d2 += d * d;
+                }
+                if (d2 < best || (d2 == best && b < bi)) {
+                    best = d2;
+                    bi = b;
+                }
+            }
+        } else {
+            const uint32_t c0 = 2 * k, c1 = 2 * k + 1;
+            const double d0 = box_dist2(blo, bhi, c0, q), d1 = box_dist2(blo, bhi, c1, q);
+            // push the far child first so that the near one is searched first
+            const uint32_t nearC = d0 <= d1 ? c0 : c1, farC = d0 <= d1 ? c1 : c0;
+            const double dn = d0 <= d1 ? d0 : d1, df = d0 <= d1 ? d1 : d0;
+            if (df <= best && sp < NN_STACK) stack[sp++] = farC;
+            if (dn <= best && sp < NN_STACK) stack[sp++] = nearC;
         }
     }
-    uint32_t s = idx[sv];
-    if (scheme == 1 && cnt == 2) {
-        uint32_t o = idx[1 - sv];
-        double ws = p(5, s), wo = p(5, o), W = ws + wo;
-        double vs2 = (double)p(2, s) * p(2, s) + (double)p(3, s) * p(3, s) + (double)p(4, s) * p(4, s);
-        double vo2 = (double)p(2, o) * p(2, o) + (double)p(3, o) * p(3, o) + (double)p(4, o) * p(4, o);
-        double f = ws / W + vo2 / vs2 * wo / W;
-        p(2, s) = (T)(p(2, s) * f);
-        p(3, s) = (T)(p(3, s) * f);
-        p(4, s) = (T)(p(4, s) * f);
+    if (bi == NN_NONE || !(best < disLimit)) {
+        prop[a] = NN_RETIRE;
+        return;
     }
-    p(5, s) = (T)wsum;
-    for (int k = 0; k < cnt; ++k)
-        if (k != sv) keep[idx[k]] = 0;
-    atomicAdd(merged, (uint32_t)(cnt - 1));
+    prop[a] = bi;
+    atomicAdd(cnt, 1u);
+}
+
+// Locks: each particle keeps the lowest-ranked proposal that involves it.
+__global__ void k_nn_lock(uint32_t nc, const uint32_t *__restrict__ prop, uint32_t *lock)
+{
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nc) return;
+    const uint32_t b = prop[a];
+    if (b >= nc) return;
+    atomicMin(lock + a, a);
+    atomicMin(lock + b, a);
+}
+
+__global__ void k_nn_accept(uint32_t nc, const uint32_t *__restrict__ prop, const uint32_t *__restrict__ lock,
+                            uint32_t *acc)
+{
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nc) return;
+    const uint32_t b = prop[a];
+    acc[a] = (b < nc && lock[a] == a && lock[b] == a) ? 1u : 0u;
+}
+
+// Accepted pair a -> prop[a] of rank order off[a] < need: the survivor is a with probability
+// w_a / (w_a + w_b) (particleManagement.pyx:186-191) and takes the pair's weight; scheme 1 rescales
+// its velocity to conserve the pair's kinetic energy (:209-232).  pairs (optional) records
+// (survivor, removed) at slot base + off[a].  Groups: the survivor stays in the pool while its
+// group has fewer than G particles (G = 2: the reference's pairs, the survivor leaves it).
+template <typename T>
+__global__ void k_nn_merge(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ prop, const uint32_t *__restrict__ acc,
+                           const uint32_t *__restrict__ off, uint32_t need, int G, int scheme, uint32_t round,
+                           uint32_t stream, uint32_t s0, uint32_t s1, uint8_t *gsize, uint32_t *lock, uint8_t *keep,
+                           uint32_t *merged, uint32_t *pairs, uint32_t base)
+{
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nc || !acc[a] || off[a] >= need) return;
+    const uint32_t b = prop[a];
+    const double wa = (double)p(5, a), wb = (double)p(5, b), W = wa + wb;
+    const double u = philox_uniform(a, round, stream, PURPOSE_MANAGE + 2u, s0, s1);
+    const bool keepA = u < wa / W;
+    const uint32_t sv = keepA ? a : b, rm = keepA ? b : a;
+    if (scheme == 1) {
+        const double ws = keepA ? wa : wb, wo = keepA ? wb : wa;
+        const double vs2 = (double)p(2, sv) * p(2, sv) + (double)p(3, sv) * p(3, sv) + (double)p(4, sv) * p(4, sv);
+        const double vo2 = (double)p(2, rm) * p(2, rm) + (double)p(3, rm) * p(3, rm) + (double)p(4, rm) * p(4, rm);
+        const double f = ws / W + vo2 / vs2 * wo / W;
+        p(2, sv) = (T)(p(2, sv) * f);
+        p(3, sv) = (T)(p(3, sv) * f);
+        p(4, sv) = (T)(p(4, sv) * f);
+    }
+    p(5, sv) = (T)W;
+    keep[rm] = 0;
+    const int ss = (int)gsize[a] + (int)gsize[b];
+    gsize[rm] = 0;
+    gsize[sv] = (uint8_t)(ss >= G ? 0 : ss);
+    lock[a] = lock[b] = NN_MERGED;
+    if (pairs) {
+        pairs[2 * (uint64_t)(base + off[a])] = sv;
+        pairs[2 * (uint64_t)(base + off[a]) + 1] = rm;
+    }
+    atomicAdd(merged, 1u);
+}
+
+// Candidates whose neighbour search failed leave the pool, unless a pair took them this round.
+__global__ void k_nn_retire(uint32_t nc, const uint32_t *__restrict__ prop, const uint32_t *__restrict__ lock,
+                            uint8_t *gsize)
+{
+    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nc) return;
+    if (prop[a] == NN_RETIRE && lock[a] != NN_MERGED) gsize[a] = 0;
 }
 
 }  // namespace
@@ -383,86 +514,118 @@ OECL_EXPORT int oecl_split_heaviest(const oecl_wall_t *wall, int dtype, void *da
     return OECL_OK;
 }
 
-// Morton merge over particles [0, nc).  pw[5] phase-space weights (device), G group size
-// (2..8), disLimit the squared-distance limit, target merges toMerge.  keep[nc] is set;
-// *merged (device) receives the number of particles removed.
-OECL_EXPORT uint64_t oecl_merge_work_bytes(uint32_t nc)
+// Nearest-neighbour merge over particles [0, nc) (sorted by weight, lightest first).
+static uint64_t al16(uint64_t b) { return (b + 15) & ~15ull; }
+
+static uint32_t nn_leaves(uint32_t nc)
 {
-    return oecl_sort_work_bytes(nc) + 24ull * nc + 8192 + 512 * 80;
+    const uint64_t buckets = ((uint64_t)nc + NN_LEAF - 1) / NN_LEAF;
+    uint32_t L = 1;
+    while (L < buckets) L <<= 1;
+    return L;
 }
 
-OECL_EXPORT int oecl_morton_merge(int dtype, void *data, uint64_t stride, uint32_t nc, const double *pw,
-                                  int G, double disLimit, uint32_t toMerge, int scheme, uint32_t stream,
-                                  uint32_t s0, uint32_t s1, uint8_t *keep, uint32_t *merged,
-                                  uint32_t *hostGroupsUsed, void *work, void *st)
+OECL_EXPORT uint64_t oecl_merge_work_bytes(uint32_t nc)
+{
+    const uint64_t L = nn_leaves(nc);
+    return al16(8ull * nc) + 5 * al16(4ull * nc + 4) + al16(nc) + al16(8ull * 512 * 10) + al16(8ull * 16) +
+           2 * al16(8ull * 5 * 2 * L) + al16(oecl_scan_work_bytes(nc + 1)) + al16(oecl_sort_work_bytes(nc)) + 256;
+}
+
+OECL_EXPORT int oecl_nn_merge(int dtype, void *data, uint64_t stride, uint32_t nc, const double *pw, int G,
+                              double disLimit, uint32_t toMerge, int scheme, uint32_t stream, uint32_t s0, uint32_t s1,
+                              uint8_t *keep, uint32_t *merged, uint32_t *pairs, uint32_t *hostMerged,
+                              uint32_t *hostRounds, void *work, void *st)
 {
     if (!data || !pw || !keep || !merged || !work || G < 2 || G > 8) return OECL_ERR_ARG;
+    if (dtype != OECL_F64 && dtype != OECL_F32) return OECL_ERR_ARG;
     hipStream_t s = (hipStream_t)st;
+    if (hostMerged) *hostMerged = 0;
+    if (hostRounds) *hostRounds = 0;
     hipLaunchKernelGGL(k_fill_u8, dim3(oecl_blocks(nc ? nc : 1, 256)), dim3(256), 0, s, keep, nc, (uint8_t)1);
-    hipLaunchKernelGGL(k_fill_u8, dim3(1), dim3(64), 0, s, (uint8_t *)merged, 4u, (uint8_t)0);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, merged, 1u);
     if (nc < 2 || toMerge == 0) {
-        if (hostGroupsUsed) *hostGroupsUsed = 0;
+        OECL_CHECK_LAUNCH("oecl_nn_merge");
         return OECL_OK;
     }
+    const uint32_t L = nn_leaves(nc);
     char *w = (char *)work;
     uint64_t *keys = (uint64_t *)w;
-    uint32_t *zperm = (uint32_t *)(keys + nc);
-    uint8_t *elig = (uint8_t *)(zperm + nc);
-    double *part = (double *)(((uintptr_t)(elig + nc) + 15) & ~(uintptr_t)15);
-    double *box = part + 512 * 10;
-    char *sw = (char *)(box + 16);
-    uint32_t ng = (nc + G - 1) / G;
-    // the group keys reuse the tail of the work after the Z sort is done
+    w += al16(8ull * nc);
+    uint32_t *zperm = (uint32_t *)w;
+    w += al16(4ull * nc + 4);
+    uint32_t *prop = (uint32_t *)w;
+    w += al16(4ull * nc + 4);
+    uint32_t *lock = (uint32_t *)w;
+    w += al16(4ull * nc + 4);
+    uint32_t *acc = (uint32_t *)w;
+    w += al16(4ull * nc + 4);
+    uint32_t *off = (uint32_t *)w;
+    w += al16(4ull * nc + 4);
+    uint8_t *gsize = (uint8_t *)w;
+    w += al16(nc);
+    double *part = (double *)w;
+    w += al16(8ull * 512 * 10);
+    double *box = (double *)w;   // box[10], then the round's proposal count as a word
+    uint32_t *cnt = (uint32_t *)(box + 10);
+    w += al16(8ull * 16);
+    double *blo = (double *)w;
+    w += al16(8ull * 5 * 2 * L);
+    double *bhi = (double *)w;
+    w += al16(8ull * 5 * 2 * L);
+    uint32_t *scanw = (uint32_t *)w;
+    w += al16(oecl_scan_work_bytes(nc + 1));
+    void *sortw = w;
+    const dim3 gr(oecl_blocks(nc, 256)), bl(256);
     Soa<double> pd{(double *)data, stride};
     Soa<float> pf{(float *)data, stride};
-    if (dtype == OECL_F64) hipLaunchKernelGGL(k_bbox<double>, dim3(512), dim3(256), 0, s, pd, nc, pw, part);
-    else hipLaunchKernelGGL(k_bbox<float>, dim3(512), dim3(256), 0, s, pf, nc, pw, part);
+    // the tree: Z-order sort of the weighted points, leaf boxes, inner boxes level by level
+    if (dtype == OECL_F64) hipLaunchKernelGGL(k_bbox<double>, dim3(512), bl, 0, s, pd, nc, pw, part);
+    else hipLaunchKernelGGL(k_bbox<float>, dim3(512), bl, 0, s, pf, nc, pw, part);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, part, 512u, box);
-    if (dtype == OECL_F64)
-        hipLaunchKernelGGL(k_morton<double>, dim3(oecl_blocks(nc, 256)), dim3(256), 0, s, pd, nc, pw, box, keys,
-                           zperm);
-    else
-        hipLaunchKernelGGL(k_morton<float>, dim3(oecl_blocks(nc, 256)), dim3(256), 0, s, pf, nc, pw, box, keys,
-                           zperm);
-    int rc = oecl_radix_sort_u64(keys, zperm, nc, 64, sw, st);
+    if (dtype == OECL_F64) hipLaunchKernelGGL(k_morton<double>, gr, bl, 0, s, pd, nc, pw, box, keys, zperm, gsize);
+    else hipLaunchKernelGGL(k_morton<float>, gr, bl, 0, s, pf, nc, pw, box, keys, zperm, gsize);
+    int rc = oecl_radix_sort_u64(keys, zperm, nc, 64, sortw, st);
     if (rc) return rc;
-    uint64_t *gkeys = keys;   // Morton keys no longer needed
-    uint32_t *gvals = (uint32_t *)(sw + oecl_sort_work_bytes(nc));
     if (dtype == OECL_F64)
-        hipLaunchKernelGGL(k_group_keys<double>, dim3(oecl_blocks(ng, 256)), dim3(256), 0, s, pd, nc, zperm, pw,
-                           G, disLimit, gkeys, gvals, elig);
+        hipLaunchKernelGGL(k_leaf_boxes<double>, dim3(oecl_blocks(L, 256)), bl, 0, s, pd, nc, zperm, pw, L, blo, bhi);
     else
-        hipLaunchKernelGGL(k_group_keys<float>, dim3(oecl_blocks(ng, 256)), dim3(256), 0, s, pf, nc, zperm, pw,
-                           G, disLimit, gkeys, gvals, elig);
-    rc = oecl_radix_sort_u64(gkeys, gvals, ng, 64, sw, st);
-    if (rc) return rc;
-    // Each eligible group removes at least one particle: merging the first toMerge sorted
-    // groups (all eligible when their key is not all ones) reaches at least toMerge
-    // removals if enough eligible groups exist; a host pass trims to exactly what is needed.
-    uint64_t *hk = (uint64_t *)malloc(8ull * ng);
-    uint8_t *hel = (uint8_t *)malloc(ng);
-    uint32_t *hv = (uint32_t *)malloc(4ull * ng);
-    OECL_HIP(hipMemcpyAsync(hk, gkeys, 8ull * ng, hipMemcpyDeviceToHost, s), "merge copy");
-    OECL_HIP(hipMemcpyAsync(hv, gvals, 4ull * ng, hipMemcpyDeviceToHost, s), "merge copy");
-    OECL_HIP(hipMemcpyAsync(hel, elig, ng, hipMemcpyDeviceToHost, s), "merge copy");
-    OECL_HIP(hipStreamSynchronize(s), "merge sync");
-    uint32_t take = 0, removed = 0;
-    while (take < ng && removed < toMerge && hk[take] != ~0ull) {
-        removed += (uint32_t)__builtin_popcount(hel[hv[take]]) - 1;
-        ++take;
-    }
-    free(hk);
-    free(hel);
-    free(hv);
-    if (hostGroupsUsed) *hostGroupsUsed = take;
-    if (take) {
+        hipLaunchKernelGGL(k_leaf_boxes<float>, dim3(oecl_blocks(L, 256)), bl, 0, s, pf, nc, zperm, pw, L, blo, bhi);
+    for (uint32_t first = L >> 1; first >= 1; first >>= 1)
+        hipLaunchKernelGGL(k_inner_boxes, dim3(oecl_blocks(first, 256)), bl, 0, s, first, first, blo, bhi);
+    uint32_t done = 0, rounds = 0, h[2] = {0, 0};
+    for (;;) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 1u);
         if (dtype == OECL_F64)
-            hipLaunchKernelGGL(k_group_merge<double>, dim3(oecl_blocks(take, 128)), dim3(128), 0, s, pd, nc,
-                               zperm, gvals, elig, take, G, scheme, stream, s0, s1, keep, merged);
+            hipLaunchKernelGGL(k_nn_propose<double>, gr, bl, 0, s, pd, nc, zperm, L, blo, bhi, pw, disLimit, G, gsize,
+                               prop, lock, cnt);
         else
-            hipLaunchKernelGGL(k_group_merge<float>, dim3(oecl_blocks(take, 128)), dim3(128), 0, s, pf, nc,
-                               zperm, gvals, elig, take, G, scheme, stream, s0, s1, keep, merged);
+            hipLaunchKernelGGL(k_nn_propose<float>, gr, bl, 0, s, pf, nc, zperm, L, blo, bhi, pw, disLimit, G, gsize,
+                               prop, lock, cnt);
+        hipLaunchKernelGGL(k_nn_lock, gr, bl, 0, s, nc, prop, lock);
+        hipLaunchKernelGGL(k_nn_accept, gr, bl, 0, s, nc, prop, lock, acc);
+        rc = oecl_launch_scan_u32(acc, nc, off, scanw, s);
+        if (rc) return rc;
+        if (dtype == OECL_F64)
+            hipLaunchKernelGGL(k_nn_merge<double>, gr, bl, 0, s, pd, nc, prop, acc, off, toMerge - done, G, scheme,
+                               rounds, stream, s0, s1, gsize, lock, keep, merged, pairs, done);
+        else
+            hipLaunchKernelGGL(k_nn_merge<float>, gr, bl, 0, s, pf, nc, prop, acc, off, toMerge - done, G, scheme,
+                               rounds, stream, s0, s1, gsize, lock, keep, merged, pairs, done);
+        hipLaunchKernelGGL(k_nn_retire, gr, bl, 0, s, nc, prop, lock, gsize);
+        OECL_CHECK_LAUNCH("oecl_nn_merge round");
+        OECL_HIP(hipMemcpyAsync(&h[0], merged, 4, hipMemcpyDeviceToHost, s), "merge count");
+        OECL_HIP(hipMemcpyAsync(&h[1], cnt, 4, hipMemcpyDeviceToHost, s), "merge count");
+        OECL_HIP(hipStreamSynchronize(s), "merge round");
+        ++rounds;
+        const bool progress = h[0] > done;
+        done = h[0];
+        // Every round with proposals merges at least the pair of the lowest-ranked proposer
+        // (no lower rank can hold either of its locks), so the loop ends when enough merges
+        // are made or nobody proposes any more; the round cap is a guard.
+        if (done >= toMerge || h[1] == 0 || !progress || rounds >= 4096) break;
     }
-    OECL_CHECK_LAUNCH("oecl_morton_merge");
+    if (hostMerged) *hostMerged = done;
+    if (hostRounds) *hostRounds = rounds;
     return OECL_OK;
 }

@@ -2,10 +2,13 @@
 
 globalRandom keeps the reference's rule exactly (weight rescale by N/target, then a uniformly
 random kill set or uniformly random clones); only the random stream differs.  localRandom
-keeps the reference's thresholds, operation counts, velocity scale, split rule and
-survivor rule; the nearest-neighbour pairing of the serial kd-tree is replaced by merging
-Z-order (Morton) neighbours in the same 5-D weighted phase space, in groups of up to
-`mergeGroup` particles (2 = the reference's pairs), lightest groups first.
+keeps the reference's thresholds, operation counts, velocity scale, split rule, merge rule
+(nearest neighbour in the weighted 5-D phase space, the squared distance below
+factorDisLimit * ds) and survivor rule; the serial kd-tree loop becomes rounds of parallel
+matching on the device (oecl_nn_merge): exact nearest neighbours through a Morton-ordered tree,
+pairs taken lightest first.  mergeGroup > 2 lets a survivor merge again until its group holds that many
+particles (2 = the reference's pairs).  The total weight is conserved exactly: the rounding of
+the merged weights in the particle dtype is measured and given back to one particle.
 """
 import ctypes
 import math
@@ -73,8 +76,10 @@ def globalRandom(gridObj, particlesObj, targetMacroParticleCount):
 
 def localRandom(gridObj, particlesObj, particleBoundaryObj, targetMacroParticleCount,
                 phaseSpaceWeights=None, thresholdFactor=2., lambdavFactor=1., factorDisLimit=1.,
-                mergeScheme=0, mergeGroup=2):
-    """particleManagement.pyx:74-238.  Returns a dict with the operation counts."""
+                mergeScheme=0, mergeGroup=2, pairs=None):
+    """particleManagement.pyx:74-238.  Returns a dict with the operation counts.  pairs: an
+    optional device int32 tensor of at least 2 * toMerge entries that receives (survivor,
+    removed) indices, in the weight-sorted order, for every merge."""
     from .diagnostics import moments
     p = particlesObj
     n = p.macroParticleCount
@@ -130,13 +135,24 @@ def localRandom(gridObj, particlesObj, particleBoundaryObj, targetMacroParticleC
     nc = n - toSplit
     keep = torch.ones(n, dtype=torch.uint8, device=dev)
     merged = torch.zeros(1, dtype=torch.int32, device=dev)
-    used = ctypes.c_uint32(0)
+    nMerged, rounds = ctypes.c_uint32(0), ctypes.c_uint32(0)
+    if pairs is not None and pairs.numel() < 2 * toMerge:
+        raise ValueError('pairs must hold 2 * toMerge = %d entries' % (2 * toMerge))
     if toMerge and nc > 1:
-        L.check(L.lib().oecl_morton_merge(
+        L.check(L.lib().oecl_nn_merge(
             L.i32(p.code), L.ptr(p._data), L.u64(p.capacity), L.u32(nc), L.ptr(pwd),
             L.i32(mergeGroup), L.f64(factorDisLimit * ds), L.u32(toMerge), L.i32(mergeScheme),
-            L.u32(sidx), L.u32(k0), L.u32(k1), L.ptr(keep), L.ptr(merged), ctypes.byref(used),
-            L.ptr(_work(L.lib().oecl_merge_work_bytes(nc))), L.stream()), 'localRandom merge')
+            L.u32(sidx), L.u32(k0), L.u32(k1), L.ptr(keep), L.ptr(merged), L.ptr(pairs),
+            ctypes.byref(nMerged), ctypes.byref(rounds), L.ptr(_work(L.lib().oecl_merge_work_bytes(nc))),
+            L.stream()), 'localRandom merge')
     p.addAndRemoveParticles(add, keep)
-    return {'toSplit': toSplit, 'toMerge': toMerge, 'merged': int(merged.item()),
-            'groups': used.value, 'vTypical': vTypical}
+    # The merged weights were rounded to the particle dtype: the measured difference of the
+    # total (FP64, fixed summation order) goes back to the particle in the first slot, so the
+    # total charge changes by at most that one weight's final rounding.
+    if p.macroParticleCount:
+        lost = m[0] - moments(p)[0]
+        if lost != 0.:
+            w0 = float(p._data[5, 0].item())
+            p._data[5, 0] = w0 + lost
+    return {'toSplit': toSplit, 'toMerge': toMerge, 'merged': nMerged.value, 'rounds': rounds.value,
+            'vTypical': vTypical}

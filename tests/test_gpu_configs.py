@@ -323,3 +323,19 @@ def test_c5_push_sample_matches_oracle(c5, orc):
     scale = np.abs(ref).max(axis=0)
     assert (np.abs(got - ref) / scale).max() < 1e-5
     assert np.mean(ins == ins_ref) > 1 - 1e-4
+
+
+def test_c5_merge_event_reaches_target(c5):
+    """localRandom at the 1.5x trigger on the 2e7-particle buildup: the count falls to the
+    target exactly and the total weight is unchanged."""
+    from openecloud_amd.diagnostics import moments
+    from openecloud_amd.particleManagement import localRandom
+    g, bq, p, pb, em, ps, b = c5
+    print('C5 merge event', flush=True)
+    n = p.getMacroParticleCount()
+    target = int(n / 1.5)
+    w0 = moments(p)[0]
+    info = localRandom(g, p, pb, target)
+    print('C5 merge: %d -> %d in %d rounds' % (n, p.getMacroParticleCount(), info['rounds']), flush=True)
+    assert p.getMacroParticleCount() == target
+    assert abs(moments(p)[0] - w0) <= 1e-12 * w0

@@ -68,28 +68,126 @@ def test_global_random_decrease_and_increase():
     assert key(got2) <= key(got)                    # clones copy existing particles
 
 
-@pytest.mark.parametrize('group', [2, 4, 8])
-def test_local_random_conserves_charge_and_reaches_target(group):
+def _sorted_case(n, seed, dtype):
+    """Particles with non-dyadic weights, and the same set sorted by weight as localRandom's
+    stable device sort orders it."""
+    rng = np.random.default_rng(seed)
+    g, p, pb, pd = _parts(n, seed, w=1e4 * (0.5 + rng.random(n)))
+    if dtype == torch.float32:
+        pd = pd.astype(np.float32).astype(np.float64)
+        from openecloud_amd.particles import Particles
+        from openecloud_amd.particleBoundary import AbsorbElliptical
+        p = Particles('electrons', 'variableWeight', dtype=torch.float32)
+        p.setParticleData(pd)
+        pb = AbsorbElliptical(g, p)
+    order = np.argsort(pd[:, 5], kind='stable')
+    return g, p, pb, pd, pd[order]
+
+
+def _weights(g, info, lambdavFactor=1.):
+    ds = min(g.getDx(), g.getDy())
+    lam = lambdavFactor * ds / info['vTypical']
+    return np.array([1., 1., lam, lam, lam]), ds
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_local_random_reaches_target_with_exact_charge_and_the_distance_rule(dtype):
+    """localRandom to two thirds of the count: the target is reached exactly, the total weight
+    is unchanged (non-dyadic weights, FP64 and FP32 storage), every merged pair satisfies the
+    reference's rule (squared weighted distance below factorDisLimit * ds,
+    particleManagement.pyx:185), no particle is in two pairs, survivors keep an original phase
+    point and the merged weight is the pair's sum."""
     from openecloud_amd import randomGen
     from openecloud_amd.diagnostics import moments
     from openecloud_amd.particleManagement import localRandom
     randomGen.seed(5)
-    rng = np.random.default_rng(9)
-    n = 60000
-    w = 2.0 ** rng.integers(0, 6, n) * 64.        # dyadic weights, spread 1:32
-    g, p, pb, pd = _parts(n, 4, w=w)
+    n, target = 60000, 40000
+    g, p, pb, pd, ps = _sorted_case(n, 4, dtype)
     m0 = moments(p)
-    info = localRandom(g, p, pb, 40000, mergeGroup=group)
+    pairs = torch.full((2 * n,), -1, dtype=torch.int32, device='cuda')
+    info = localRandom(g, p, pb, target, pairs=pairs)
     m1 = moments(p)
-    assert m1[0] == m0[0]                           # sum of weights exactly conserved
-    assert p.getMacroParticleCount() == n + info['toSplit'] - info['merged']
-    assert info['merged'] > 0 and info['toSplit'] > 0
-    assert info['merged'] <= info['toMerge'] + group - 2
-    got = p.getParticleData().cpu().numpy()
-    assert np.all(got[:, 5] > 0)
-    # positions of survivors are original positions (merges keep a member's phase point)
+    assert p.getMacroParticleCount() == target
+    assert info['merged'] == info['toMerge'] > 0
+    tol = 1e-15 if dtype == torch.float64 else 1e-12
+    assert abs(m1[0] - m0[0]) <= tol * m0[0], (m1[0] - m0[0]) / m0[0]
+    pr = pairs[:2 * info['merged']].cpu().numpy().reshape(-1, 2).astype(np.int64)
+    assert np.all(pr >= 0) and np.all(pr < n - info['toSplit'])
+    assert np.unique(pr).size == pr.size                   # pairs are disjoint
+    pw, ds = _weights(g, info)
+    d2 = (((ps[pr[:, 0], :5] - ps[pr[:, 1], :5]) * pw) ** 2).sum(1)
+    assert np.all(d2 < 1. * ds)
+    got = p.getParticleData().cpu().numpy().astype(np.float64)
     orig = set(map(tuple, np.round(pd[:, :2], 15)))
     assert all(tuple(r) in orig for r in np.round(got[:, :2], 15))
+    # the survivors' weights: pair sums (the one particle that took the rounding aside)
+    wsum = ps[pr[:, 0], 5] + ps[pr[:, 1], 5]
+    merged_w = np.sort(wsum.astype(np.float32 if dtype == torch.float32 else np.float64).astype(np.float64))
+    have = np.sort(got[:, 5])
+    assert np.isin(merged_w, have).mean() > 0.999
+
+
+@pytest.mark.parametrize('group', [4, 8])
+def test_local_random_groups_reach_target(group):
+    """mergeGroup > 2: survivors merge again until their group holds `group` particles; the
+    target is reached and the total weight is unchanged."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.diagnostics import moments
+    from openecloud_amd.particleManagement import localRandom
+    randomGen.seed(7)
+    n = 60000
+    g, p, pb, pd, ps = _sorted_case(n, 8, torch.float64)
+    m0 = moments(p)
+    pairs = torch.full((2 * n,), -1, dtype=torch.int32, device='cuda')
+    # 25000 merges: within the reference's cap of 2 toMerge + toSplit < 0.95 n (:106-122)
+    info = localRandom(g, p, pb, 35000, mergeGroup=group, pairs=pairs)
+    assert p.getMacroParticleCount() == 35000 and info['merged'] == info['toMerge']
+    assert abs(moments(p)[0] - m0[0]) <= 1e-15 * m0[0]
+    pr = pairs[:2 * info['merged']].cpu().numpy().reshape(-1, 2)
+    assert np.unique(pr[:, 1]).size == pr.shape[0]         # each particle is removed once
+    assert np.bincount(np.concatenate([pr[:, 0], pr[:, 1]])).max() <= group - 1
+
+
+def test_local_random_against_serial_greedy_merge():
+    """The same weight-sorted particles through the device merge and the oracle's restatement of
+    the reference's serial kd-tree loop: both make the requested merges, with merged-pair
+    distances of the same distribution (the pairing itself differs by design), and the charge
+    density on the grid changes by a like amount."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.particleManagement import localRandom
+    from oracle.merge_oracle import greedy_merge
+    import oracle as orc
+    randomGen.seed(11)
+    n, target = 20000, 14000
+    g, p, pb, pd, ps = _sorted_case(n, 12, torch.float64)
+    pairs = torch.full((2 * n,), -1, dtype=torch.int32, device='cuda')
+    info = localRandom(g, p, pb, target, pairs=pairs)
+    pr = pairs[:2 * info['merged']].cpu().numpy().reshape(-1, 2).astype(np.int64)
+    pw, ds = _weights(g, info)
+    nc = n - info['toSplit']
+    keep, w, ref_pairs = greedy_merge(ps, nc, pw, ds, info['toMerge'], np.random.default_rng(3).random(nc))
+    assert len(ref_pairs) == info['toMerge'] == pr.shape[0]
+    d_gpu = np.sqrt((((ps[pr[:, 0], :5] - ps[pr[:, 1], :5]) * pw) ** 2).sum(1))
+    d_ref = np.sqrt(np.array([x[2] for x in ref_pairs]))
+    for q in (0.25, 0.5, 0.75):
+        r = np.quantile(d_gpu, q) / np.quantile(d_ref, q)
+        assert 0.6 < r < 1.6, (q, r)
+    # charge density change: the merged set against the original set, both ways
+    nx, npx = g.getNxExt(), g.getNpExt()
+    Q = -1.602176565e-19
+
+    def rho(a):
+        ic, ww = orc.calc_grid_weights(a, g.getDx(), g.getDy(), g.getLxExt(), g.getLyExt(), nx)
+        return orc.charge_to_grid(a, ic, ww, Q, nx, npx)
+    r0 = rho(ps[:nc])
+    after_ref = ps[:nc][keep].copy()
+    after_ref[:, 5] = w[keep]
+    got = p.getParticleData().cpu().numpy()
+    # the device set also holds the split particles: compare the merged part only
+    got_m = got[np.isin(np.round(got[:, 0], 15), np.round(ps[:nc, 0], 15))]
+    e_ref = np.linalg.norm(rho(after_ref) - r0) / np.linalg.norm(r0)
+    e_gpu = np.linalg.norm(rho(got_m) - r0) / np.linalg.norm(r0)
+    assert 0.5 < e_gpu / e_ref < 2.0, (e_gpu, e_ref)
 
 
 def test_local_random_split_only_increases_count():
