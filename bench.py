@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-WORKLOADS = ('full', 'dipole', 'scan', 'learned')
+WORKLOADS = ('full', 'dipole', 'scan', 'learned', 'northstar')
 
 
 def profile_traffic(name):
@@ -77,7 +77,9 @@ def parse(argv=None):
     ap.add_argument('--workload', default='full', choices=WORKLOADS,
                     help='full: the 2048^2 quadrupole buildup with particle management (default); '
                          'dipole: the 512x384 dipole buildup; scan: 64 buildups stepped together; '
-                         'learned: the learned-preconditioner Poisson solve on a 1024^2 cut-cell grid')
+                         'learned: the learned-preconditioner Poisson solve on a 1024^2 cut-cell grid; '
+                         'northstar: the 64-run scan with every run solved by the learned flexible CG, against '
+                         'the plain method')
     ap.add_argument('--runs', type=int, default=64, help='scan: number of buildups')
     ap.add_argument('--scan-start', type=int, default=700,
                     help='scan: the timed steps start at this step of the simulation (the first bunch passage, '
@@ -1071,8 +1073,167 @@ def main_dipole(a):
     print(json.dumps(out), flush=True)
 
 
+def _scan_rate(scan, steps, sortEvery):
+    """Particle-steps/s of `steps` eager steps of every run of a scan (a cell sort every
+    sortEvery steps, counted from the first)."""
+    import torch
+    it0 = scan.it
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        if sortEvery and k % sortEvery == 0:
+            scan.sort()
+        scan.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    scan.sync()
+    ps_ = float(sum(sum(scan.history(r, it0, it0 + steps)) for r in range(scan.R)))
+    return ps_ / wall, wall, ps_
+
+
+def main_northstar(a):
+    """The combined form: the 64-run scan (256x256 rectangular 100x50 mm chamber, By 0.3 T,
+    delta_max over [1.1, 2.0], log-uniform intensities and counts, FP32 particles with FP64
+    charge and Poisson arithmetic) with every run's Poisson problem solved by the learned
+    flexible CG to 1e-8 relative residual, warm-started from the run's previous potential; the
+    U-Net is trained on the device on (q, phi) pairs dumped from the scan's own runs, and the
+    direct solve is fallback and judge.  Beside it: the same runs with the batched direct
+    solve, the plain GPU method (the reference-order method-by-method step, a drop-in for the
+    reference's loop) on four of the runs, and the cloud density of the learned FP32 runs
+    against the same runs stepped in FP64 with the direct solve."""
+    import numpy as np
+    import torch
+    from openecloud_amd import _lib as L
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    L.require_gpu()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    R, W, K = a.runs, a.warmup, a.steps
+    dumpEvery, nDumps = 10, 4
+    maxSteps = W + K + dumpEvery * nDumps + 40
+    # training pairs from the scan itself: every run's (q, phi) every dumpEvery steps
+    t0 = time.perf_counter()
+    g, ps, runs, scan, prm = build_scan(R, 'f32', maxSteps)
+    scan.sort()
+    pairs = []
+    for d in range(nDumps):
+        for _ in range(dumpEvery):
+            scan.step()
+        for r in range(R):
+            sc = float(scan._scaleHost[r, scan.it])
+            q = (scan.run_rho(r) + sc * scan._prof).clone()
+            pairs.append((q, ps.solvePhi(q).clone()))
+    scan.sync()
+    net = UNetPreconditioner(g, seed=0)
+    val = [pairs[-1][0], pairs[-R // 2][0]]   # held out: the model with the fewest iterations on them is kept
+    # the scan's own dumps, and as many operator pairs (A phi, phi) with phi random at 0-64 cell
+    # scales: cloud potentials alone leave the network's short reach poorly trained
+    train = net.smoothed_pairs(ps, pairs[:-1] + net.operator_pairs(ps, len(pairs), seed=1))
+    t1 = time.perf_counter()
+    hist = net.train(ps, train, steps=a.train_steps, lr=5e-4, lrEnd=5e-5, batch=4,
+                     logEvery=max(1, a.train_steps // 10), validate=val,
+                     validateEvery=max(1, a.train_steps // 6), validateIter=4000)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    log('northstar: %d pairs, trained in %.1f s (set-up %.1f s)' % (len(pairs), t_train, t1 - t0))
+    del scan, runs, pairs, train
+    torch.cuda.empty_cache()
+    # the learned batched form
+    g, ps, runs, scan, prm = build_scan(R, 'f32', maxSteps)
+    scan.sort()
+    scan.set_solver(net, tol=1e-8, maxIter=4000, judgeEvery=max(1, K // 4))
+    for _ in range(W):
+        scan.step()
+    scan.sync()
+    nIt0 = len(scan.learnedStats['iterations'])
+    rate_l, wall_l, psteps_l = _scan_rate(scan, K, a.sort_every)
+    its = scan.learnedStats['iterations'][nIt0:]
+    lstats = dict(scan.learnedStats)
+    lstats['iterations'] = None
+    rho_l = [scan.run_rho(r).clone() for r in range(R)]
+    counts_l = scan.counts()
+    log('northstar: learned batched %.4g particle-steps/s, %.1f FCG iterations per solve' % (rate_l, np.mean(its)))
+    del scan, runs
+    torch.cuda.empty_cache()
+    # the same runs, batched direct solve
+    g, ps, runs, scan, prm = build_scan(R, 'f32', maxSteps)
+    scan.sort()
+    for _ in range(W):
+        scan.step()
+    scan.sync()
+    rate_d, wall_d, _ = _scan_rate(scan, K, a.sort_every)
+    # the plain GPU method: the reference-order method-by-method step on four of the runs
+    ref_ps, ref_t = 0., 0.
+    for r in (0, R // 3, 2 * R // 3, R - 1):
+        b = runs[r]
+        b.step_reference()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(2):
+            n = b.p.macroParticleCount
+            b.step_reference()
+            ref_ps += n
+        torch.cuda.synchronize()
+        ref_t += time.perf_counter() - t
+    rate_plain = ref_ps / ref_t
+    del scan, runs
+    torch.cuda.empty_cache()
+    # cloud density: the learned FP32 runs against the same runs in FP64 with the direct solve
+    g, ps, runs, scan, prm = build_scan(R, 'f64', maxSteps)
+    scan.sort()
+    for k in range(W + K):
+        if k >= W and a.sort_every and (k - W) % a.sort_every == 0:
+            scan.sort()
+        scan.step()
+    scan.sync()
+    errs = []
+    for r in range(R):
+        r64 = scan.run_rho(r)
+        den = float(torch.linalg.norm(r64))
+        errs.append(float(torch.linalg.norm(rho_l[r] - r64)) / den if den > 0. else 0.)
+    counts64 = scan.counts()
+    out = {
+        'metric': 'buildup particle-steps/sec and Poisson solves/sec, learned-PCG batched vs plain',
+        'value': rate_l,
+        'unit': 'particle-steps/s',
+        'n_gpus': 1,
+        'steps': K,
+        'warmup': W,
+        'ms_per_step': wall_l / K * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 particles, f64 Poisson (learned FCG: bf16 network with f32 accumulation, f64 vectors)',
+        'data': 'synthetic (seeded uniform loads, Philox draws); the network trained on the scan\'s own dumps',
+        'config': {'workload': '%d buildups on a shared 256x256 rectangular 100x50 mm chamber, By 0.3 T, dt 1e-11 s, '
+                               'Gaussian bunches sigma 2 mm x8, delta_max linspace(1.1, 2.0), ppb log-uniform '
+                               '[5e10, 2e11], macro-particles log-uniform [1e4, 2e6]; every run\'s Poisson problem by '
+                               'the learned flexible CG to 1e-8, warm-started, direct solve as fallback and judge' % R,
+                   'runs': R, 'grid': [256, 256], 'train_pairs': '%d scan dumps + %d operator pairs' % (R * nDumps - 1,
+                                                                                            R * nDumps),
+                   'train_steps': a.train_steps,
+                   'train_seconds': t_train, 'loss_first_last': [hist[0][1], hist[-1][1]] if hist else None,
+                   'validation_iterations': net.validation},
+        'poisson_solves_per_s': R * K / wall_l,
+        'learned': {'fcg_iterations_mean': float(np.mean(its)), 'fcg_iterations_max': int(np.max(its)),
+                    'relres_max': lstats['relres_max'], 'fallbacks': lstats['fallbacks'],
+                    'judge_max_rel_err': lstats['judge_max_rel_err'], 'judged': lstats['judged']},
+        'batched_direct': {'value': rate_d, 'unit': 'particle-steps/s', 'ms_per_step': wall_d / K * 1e3,
+                           'poisson_solves_per_s': R * K / wall_d},
+        'plain_reference_order': {'value': rate_plain, 'unit': 'particle-steps/s',
+                                  'note': 'method-by-method GPU step in the reference order, 4 of the runs'},
+        'ratio_learned_batched_vs_plain': rate_l / rate_plain,
+        'ratio_direct_batched_vs_plain': rate_d / rate_plain,
+        'cloud_density_vs_fp64_direct': {'rel_l2_max': max(errs), 'rel_l2_mean': float(np.mean(errs)),
+                                         'macro_particles_f32_f64': [int(sum(counts_l)), int(sum(counts64))],
+                                         'after_steps': W + K},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main(argv=None):
     a = parse(argv)
+    if a.workload == 'northstar':
+        return main_northstar(a)
     if a.workload == 'learned':
         return main_learned(a)
     if a.workload == 'scan':

@@ -497,6 +497,9 @@ typedef struct {
     const oecl_secem_t *secem;  /* NULL: the Furman emitter `prm`; else this simpler emitter */
     double *Eil;                /* [2 npExt] (per run) scratch: E interleaved (Ex, Ey) per node for
                                    the push's gather, or NULL to gather from E */
+    int32_t externalPhi;        /* 0: the step solves for phi with the direct solver; 1: phi (per
+                                   run) already holds the potential of this step's charge
+                                   rho + s(step) * profile, solved by the caller (any method) */
 } oecl_step_args;
 
 /* Particles per push block (256): batched run segments are multiples of it, and the per-block

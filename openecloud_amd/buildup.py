@@ -40,7 +40,7 @@ class StepArgs(ctypes.Structure):
                 ('evPushBegin', _VP), ('evPushEnd', _VP), ('photoCount', _VP), ('photoWeight', _VP),
                 ('nPhoto', ctypes.c_uint32), ('photoE0', ctypes.c_double),
                 ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32),
-                ('secem', _VP), ('Eil', _VP)]
+                ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32)]
 
 
 ST_ABS_OVERFLOW, ST_SEC_OVERFLOW, ST_CAP_OVERFLOW, ST_PHOTO_OVERFLOW = 1, 2, 4, 8

@@ -711,8 +711,10 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
                        a->nScale, a->stepDev, a->profile, box, a->phi, a->nDev, a->countHist, a->histCap, misc, wd,
                        a->photoCount, a->photoWeight, a->nPhoto, a->photoCap, a->seed0, a->seed1, a->photoE0,
                        a->photoSigmaE, P->particleMass, (T *)a->photoSlots, a->stats);
-    rc = fastpoisson_solve_box(fp, a->phi, R, s);
-    if (rc) return rc;
+    if (!a->externalPhi) {
+        rc = fastpoisson_solve_box(fp, a->phi, R, s);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
                        a->e2nCol, a->e2nVal, a->phi, np, a->E, a->Eil);
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;

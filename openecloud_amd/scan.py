@@ -147,9 +147,10 @@ class BuildupScan:
                     raise ValueError('the runs of a scan share the beam profile')
             sc = np.array([[bm.getScale(k * self.dt) for k in range(nTab)] for bm in beams])
             self._scale = L.to_dev(sc.reshape(-1))
+            self._scaleHost = sc
             self._prof = prof0
         else:
-            self._scale, self._prof = None, None
+            self._scale, self._prof, self._scaleHost = None, None, None
         self._bField = b0.bField
         self._desc = RunsDesc(R, self._segOff.data_ptr(), self._blockRun.data_ptr(), self._absOff.data_ptr(),
                               self._furman.data_ptr(), self._keys.data_ptr())
@@ -158,7 +159,53 @@ class BuildupScan:
         self._graph = None
         self._sortkeys = None
         self._it0 = self.it
+        self._learned = None
+        self.learnedStats = {'iterations': [], 'relres_max': 0., 'fallbacks': 0, 'judge_max_rel_err': 0.,
+                             'judged': 0}
         self.deposit_all()
+
+    def set_solver(self, preconditioner=None, tol=1e-8, maxIter=4000, judgeEvery=0):
+        """The Poisson solve of the steps.  None: the direct solve inside the batched step (the
+        default).  A trained UNetPreconditioner: every run's potential by the learned flexible CG
+        to tol, warm-started from the run's previous potential, the direct solve taking over when
+        it does not converge; every judgeEvery-th step (0: never) the direct solve of each run
+        judges the result (learnedStats['judge_max_rel_err']).  The learned solve reads residual
+        norms on the host, so such steps run eagerly (no graph capture)."""
+        self._learned = preconditioner
+        self._ltol, self._lmax, self._ljudge = float(tol), int(maxIter), int(judgeEvery)
+        self._args.externalPhi = 1 if preconditioner is not None else 0
+        if preconditioner is not None:
+            self._graph = None
+
+    def _solve_learned(self):
+        """phi_r for this step's charge rho_r + s_r(step) profile of every run."""
+        net, ps, npx = self._learned, self.ps, self.np
+        st = self.learnedStats
+        judge = self._ljudge > 0 and (self.it - self._it0) % self._ljudge == 0
+        for r in range(self.R):
+            rho_r = self.rho[r * npx:(r + 1) * npx]
+            q_r = self.q[r * npx:(r + 1) * npx]
+            phi_r = self.phi[r * npx:(r + 1) * npx]
+            sc = float(self._scaleHost[r, self.it]) if self._scaleHost is not None else 0.
+            if sc != 0.:
+                L.check(L.lib().oecl_axpy_rhs(L.u32(npx), L.ptr(rho_r), L.f64(sc), L.ptr(self._prof), L.ptr(q_r),
+                                              L.stream()), 'scan rhs')
+            else:
+                q_r.copy_(rho_r)
+            _, it, rr, ok = net.fcg(ps, q_r, phi=phi_r, tol=self._ltol, maxIter=self._lmax)
+            st['iterations'].append(it)
+            if not ok:
+                st['fallbacks'] += 1
+                ps._direct(q_r, phi_r)
+            else:
+                st['relres_max'] = max(st['relres_max'], rr)
+            if judge:
+                ref = torch.empty_like(phi_r)
+                ps._direct(q_r, ref)
+                den = float(torch.linalg.norm(ref))
+                if den > 0.:
+                    st['judge_max_rel_err'] = max(st['judge_max_rel_err'], float(torch.linalg.norm(phi_r - ref)) / den)
+                st['judged'] += 1
 
     def _make_args(self, b0):
         p, ps = b0.p, self.ps
@@ -249,9 +296,11 @@ class BuildupScan:
 
     # ------------------------------------------------------------------ stepping
     def step(self):
-        """One step of every run, one library call."""
+        """One step of every run, one library call (after the runs' learned solves, when set)."""
         if self.it - self._it0 >= self.maxSteps:
             raise L.OeclError('scan: maxSteps exhausted')
+        if self._learned is not None:
+            self._solve_learned()
         L.check(L.lib().oecl_step_run_batch(ctypes.byref(self.runs[0].gd), ctypes.byref(self.pb.desc),
                                             ctypes.byref(self._desc), self.ps._fp, ctypes.byref(self._args),
                                             L.stream()), 'scan step')
@@ -260,6 +309,8 @@ class BuildupScan:
     def capture(self, nSteps):
         """Records nSteps steps of every run into one HIP graph; replay() runs the next
         nSteps.  The scan's buffers keep their addresses (sort() copies back)."""
+        if self._learned is not None:
+            raise L.OeclError('scan: the learned solve runs eagerly (set_solver(None) to capture)')
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         g = torch.cuda.CUDAGraph()

@@ -151,3 +151,26 @@ def test_scan_fp32_runs_match_standalone():
         assert idx.size > 0.95 * m
         scale = np.abs(ref[r][-1]).max(axis=0)
         assert (np.abs(got[idx] - ref[r][-1][idx]) / scale).max() < 1e-5
+
+
+def test_scan_with_learned_solve_follows_the_direct_scan():
+    """The batched step with every run's potential from the learned flexible CG (the network
+    trained briefly on this grid) against the same scan with the direct solve: the particles
+    agree to the solve tolerance's effect, no run falls back, and the judge confirms the
+    potentials."""
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    g, ps = _shared()
+    net = UNetPreconditioner(g, seed=0)
+    net.train(ps, net.smoothed_pairs(ps, net.operator_pairs(ps, 16, seed=1)), steps=300, lr=5e-4, batch=2)
+    ref, lrn = _scan(g, ps), _scan(g, ps)
+    lrn.set_solver(net, tol=1e-12, maxIter=4000, judgeEvery=2)
+    for it in range(6):
+        ref.step()
+        lrn.step()
+    ref.sync()
+    lrn.sync()
+    st = lrn.learnedStats
+    assert st['fallbacks'] == 0 and st['relres_max'] <= 1e-12 and st['judged'] > 0
+    assert st['judge_max_rel_err'] < 1e-9
+    for r in range(3):
+        assert _close(lrn.run_particles(r).cpu().numpy(), ref.run_particles(r).cpu().numpy(), 1e-9), r
