@@ -294,7 +294,10 @@ __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
     double q[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) q[c] = (double)p(c, a) * pw[c];
-    double best = INFINITY;
+    // a neighbour at or beyond the limit is rejected anyway (the reference's test is strict), so
+    // the search starts with the limit as the best distance: only the ball inside it is searched,
+    // and the result is the same
+    double best = disLimit;
     uint32_t bi = NN_NONE;
     uint32_t stack[NN_STACK];
     int sp = 0;
@@ -317,7 +320,7 @@ __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
                     const double d = (double)p(c, b) * pw[c] - q[c];
                     d2 += d * d;
                 }
-                if (d2 < best || (d2 == best && b < bi)) {
+                if (d2 < best || (d2 == best && bi != NN_NONE && b < bi)) {
                     best = d2;
                     bi = b;
                 }
@@ -332,7 +335,7 @@ __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
             if (dn <= best && sp < NN_STACK) stack[sp++] = nearC;
         }
     }
-    if (bi == NN_NONE || !(best < disLimit)) {
+    if (bi == NN_NONE) {
         prop[a] = NN_RETIRE;
         return;
     }
