@@ -369,15 +369,21 @@ int oecl_axpy_rhs(uint32_t np, const double *rho, double scale, const double *pr
                   double *out, void *stream);
 
 /* Direct solve of the same system (poissonSolver.pyx:59,85 -- the SuperLU factorisation
- * and back-substitution) by the capacitance-matrix method: the box of possible inside nodes
- * (bx = nxExt-4 by by = nyExt-4) with the uniform edge coefficients cxu (x) and cyu (y) is
- * inverted exactly by a type-I DST along x (FP64 MFMA products) and tridiagonal solves along
- * y, and the m inside rows next to the wall are corrected through the m x m capacitance
- * matrix, built and inverted on the device at creation.  sIdx [m]: box index
- * (i-2) + (j-2)*bx of each corrected row; W (CSR over box indices): those rows of (M - R).
- * maskDev (inside flags, device) must outlive the handle.  *capMax receives max |C_aa|
- * (a scale check). */
+ * and back-substitution) by the capacitance-matrix method: the box of nodes
+ * 2 <= i < 2 + bx, 2 <= j < nyExt - 2 (bx = oecl_fastpoisson_box_width(nxExt) >= nxExt - 4,
+ * by = nyExt - 4) with the uniform edge coefficients cxu (x) and cyu (y) is inverted exactly
+ * by a type-I DST along x (an FFT of length bx + 1 per line) and tridiagonal solves along y,
+ * and the m inside rows whose operator row differs from the box's are corrected through the
+ * m x m capacitance matrix, built and inverted on the device at creation.  sIdx [m]: box
+ * index (i-2) + (j-2)*bx of each corrected row; W (CSR over box indices): those rows of
+ * (M - R) (box columns from nxExt - 2 on hold no inside node, so an inside node next to
+ * them is a corrected row).  maskDev (inside flags, device) must outlive the handle.
+ * *capMax receives max |C_aa| (a scale check).  nxExt - 4 <= 2047, nyExt - 4 <= 2048. */
 typedef struct oecl_fastpoisson oecl_fastpoisson;
+/* The solver's box width for an extended grid of nxExt columns: the least 2^p - 1 that is at
+ * least nxExt - 4 (0 for nxExt < 5).  No counterpart in the reference (its SuperLU solve has
+ * no box). */
+uint32_t oecl_fastpoisson_box_width(uint32_t nxExt);
 int oecl_fastpoisson_create(uint32_t nxExt, uint32_t nyExt, double cxu, double cyu, uint32_t m,
                             const int32_t *sIdx, const int32_t *wPtr, const int32_t *wCol,
                             const double *wVal, const uint8_t *maskDev, oecl_fastpoisson **out,

@@ -1,8 +1,8 @@
 // Direct FIT Poisson solve by the capacitance-matrix method.
 //
 // The operator A (poissonSolver.pyx:44-59) acts on the inside nodes, all of which lie in
-// the box B of nodes 2 <= i <= nxExt-3, 2 <= j <= nyExt-3 (bx x by nodes, column-major with
-// x fastest).  Let R be the box's uniform 5-point operator (x-edge coefficient cxu, y-edge
+// the box B of nodes 2 <= i < 2 + bx, 2 <= j <= nyExt-3 (bx x by nodes, column-major with
+// x fastest; bx = 2^p - 1 >= nxExt - 4, the columns beyond nxExt - 3 are padding).  Let R be the box's uniform 5-point operator (x-edge coefficient cxu, y-edge
 // cyu, zero Dirichlet data outside B) and M the box operator whose rows are A's rows on
 // inside nodes and R's rows elsewhere.  M x = b with b = q on inside nodes reproduces
 // A phi = q exactly on inside nodes (the inside rows never reference a non-inside node).  M
@@ -16,12 +16,12 @@
 //     (cxu lx_q I + cyu Ty) hat u[q, :] = hat b[q, :],   lx_q = 2 - 2 cos(pi (q+1)/(bx+1)),
 // solved exactly by Thomas elimination with precomputed pivots.  The whole solve stays in
 // the transformed space:
-//     hat b = Sx b                                        (FP64 MFMA product)
+//     hat b = Sx b                                        (FFT of length bx + 1 per line)
 //     hat y = T^-1 hat b                                  (Thomas, per x mode)
 //     (W y)_a = sum_k W[a,k] (Sx hat y)[k]               (k near the wall: short dot products)
 //     z = C^-1 W y ;  hat b -= Sx[:, i_a] z_a  for each a (column j_a of hat b)
-//     hat x = T^-1 hat b ;  x = Sx hat x                  (product fused with the scatter)
-// Two matrix products per solve.  C is built once on the device with the same kernels
+//     hat x = T^-1 hat b ;  x = Sx hat x                  (transform fused with the scatter)
+// Two transforms per solve.  C is built once on the device with the same kernels
 // (batched unit right-hand sides), so that the correction matches the operator applied,
 // and inverted with rocSOLVER.
 #include <math.h>
@@ -37,13 +37,14 @@
 
 struct oecl_fastpoisson {
     uint32_t nx, ny, np;      // extended grid
-    uint32_t bx, by, nb;      // box sizes, nb = bx * by
+    uint32_t bx, by, nb;      // box sizes, nb = bx * by; bx + 1 a power of two
+    uint32_t bxg;             // interior grid columns nx - 4 (box columns beyond are padding)
     uint32_t m;               // irregular rows
     uint32_t nJ;              // distinct box columns j among the irregular nodes
     double cyu;
     rocblas_handle blas;
     double *Sx;               // bx * bx (symmetric)
-    double *Fx;               // [2][kx][kx] even/odd-row halves of Sx, kx = ceil(bx / 2)
+    double2 *tab;             // 2 (bx + 1): e^{-i pi k / (bx + 1)}, the FFT's twiddles
     double *invw;             // nb: Thomas pivots 1 / w[q, j] of mode q's y system
     int32_t *sIdx;            // [m] box index of row a
     int32_t *wPtr, *wCol;     // CSR of W over box columns, m rows
@@ -57,163 +58,140 @@ struct oecl_fastpoisson {
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
     uint32_t rCap;            // right-hand sides the vectors above hold
     char *batchMem;           // their own allocation once rCap > 1
-    double *gP, *gT;          // library-product path (large boxes): folded operands and products,
-    char *gemmMem;            //   2 kx x (by rCap) each, their own allocation
 };
 
 namespace {
 
 // ------------------------------------------------------------------ per-solve kernels
-// The DST-I matrix S (order n, symmetric) satisfies S[i][n-1-j] = (-1)^i S[i][j], so
-//     (S X)[2i+z, :] = sum_{j < K_z} F_z[i][j] P_z[j, :],   F_z[i][j] = S[2i+z][j],
-//     P_z[j, :] = X[j, :] + (-1)^z X[n-1-j, :]   (the middle row of odd n enters P_0 once),
-// with K_0 = ceil(n/2), K_1 = floor(n/2): one transform is two half-size products, half
-// the flops of S X.  k_dst_gemm computes parity z = blockIdx.z on FP64 MFMA
-// (v_mfma_f64_16x16x4f64): 32 x 32 output tiles, GNW waves splitting each GKC-deep chunk of
-// the contraction, 2 x 2 MFMA tiles per wave, a cross-wave sum through LDS at the end.  The
-// fold P_z is formed while X is staged into LDS; the next chunk's operands are loaded into
-// registers while the current one is multiplied.
-//   A = F_z (col-major K_z x K_z, lda = K_0, F_z at z * K_0 * K_0), X = n x N (ldx)
-// EPI 0 stores C (ldc), EPI 2 scatters into the extended-grid potential (masked; the rest
-// of the grid is written by the caller).  Several right-hand sides are simply more columns:
-// for EPI 2, column gn is line gn % by of the potential gn / by (np nodes apart).
-typedef double v4d __attribute__((ext_vector_type(4)));
-constexpr int GT = 32, GNW = 8, GKC = 64, GNT = 64 * GNW, GPER = GT * GKC / GNT;
+// The orthonormal DST-I of order n = M - 1 (M = bx + 1, a power of two: the box is padded to
+// that width), S_k = sqrt(2/M) sum_{j=1..n} v_j sin(pi j k / M), k = 1..n, through one complex
+// FFT of length M per line:
+//   y (length 2M) = odd extension of v: y_0 = y_M = 0, y_j = v_j, y_{2M-j} = -v_j, whose DFT
+//   is Y_k = -2 i S_k;  z_m = y_{2m} + i y_{2m+1} (m < M), Z = FFT_M(z);
+//   Y_k = (Z_k + conj Z_{M-k}) / 2 + w^k (Z_k - conj Z_{M-k}) / (2 i),  w = e^{-i pi / M},
+// so that
+//   S_k = -Im(Y_k) / 2 = -(1/4) [(Im Z_k - Im Z_{M-k}) - cos(pi k/M) (Re Z_k - Re Z_{M-k})
+//                                - sin(pi k/M) (Im Z_k + Im Z_{M-k})].
+// The FFT is a Stockham autosort in LDS (radix-4 passes, one radix-2 pass for odd log2 M), each
+// pass reading all of a thread's inputs into registers before a barrier and writing after it,
+// so one buffer of M complex values per line suffices.  Twiddles come from a table of
+// e^{-i pi k / M}, k < 2M, built on the host in extended precision.  Error O(eps log M)
+// against O(eps sqrt(M)) for the dense product.  T threads per line, 256 / T lines per block.
+// EPI 0 stores the transform of line l at C + l * ldc; EPI 2 scatters it into the
+// extended-grid potential: line l is row l % by of potential l / by (np nodes apart), only the
+// box columns that are interior grid columns (k < bxg) are written, masked.
+template <int M>
+struct FftShape {
+    static constexpr int T = M >= 1024 ? 256 : (M >= 4 ? M / 4 : 1);
+    static constexpr int LB = 256 / T;
+};
 
-template <int EPI>
-__global__ void __launch_bounds__(GNT)
-    k_dst_gemm(int N, int n, const double *__restrict__ A, const double *__restrict__ X, int ldx,
-               double *__restrict__ C, int ldc, uint32_t nx, const uint8_t *__restrict__ mask, uint32_t by,
-               uint32_t np)
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
-    constexpr int LDSN = (2 * GKC * 33 > GNW * GT * 33) ? 2 * GKC * 33 : GNW * GT * 33;
-    __shared__ double sm[LDSN];
-    double(*As)[33] = reinterpret_cast<double(*)[33]>(sm);
-    double(*Bs)[33] = reinterpret_cast<double(*)[33]>(sm + GKC * 33);
-    const int z = blockIdx.z;
-    const int K0 = (n + 1) / 2, Kz = z ? n / 2 : K0;
-    const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
-    if (m0 >= Kz || n0 >= N) return;  // whole block: no barrier is skipped by part of it
-    const double *Az = A + (size_t)z * K0 * K0;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    v4d acc[2][2];
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// z value m of the packed odd extension of v (n = M - 1 values staged in LDS)
+template <int M>
+__device__ __forceinline__ double2 packed_odd(const double *v, uint32_t m)
+{
+    const uint32_t t0 = 2 * m, t1 = 2 * m + 1;
+    const double y0 = t0 == 0 ? 0. : (t0 < (uint32_t)M ? v[t0 - 1] : (t0 == (uint32_t)M ? 0. : -v[2 * M - t0 - 1]));
+    const double y1 = t1 < (uint32_t)M ? v[t1 - 1] : -v[2 * M - t1 - 1];
+    return make_double2(y0, y1);
+}
+
+// One Stockham pass of radix R at stride NS: out[(j / NS) NS R + j % NS + r NS] =
+// DFT_R(in[j + r M / R] w^{r (j % NS)}), w = e^{-2 pi i / (NS R)}; the first pass (NS = 1)
+// reads the packed odd extension of v instead of z.
+template <int M, int R, int NS>
+__device__ __forceinline__ void fft_pass(double2 *z, const double *v, uint32_t t, const double2 *__restrict__ tab)
+{
+    constexpr int T = FftShape<M>::T, J = M / (R * T);
+    double2 x[J][R];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int q = 0; q < J; ++q) {
+        const uint32_t j = t + q * T;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0., 0., 0., 0.};
-    double ra[GPER], rb[GPER];
-    auto load = [&](int k0) {
+        for (int r = 0; r < R; ++r) x[q][r] = NS == 1 ? packed_odd<M>(v, j + r * (M / R)) : z[j + r * (M / R)];
+        if (NS > 1) {
+            const uint32_t jm = j & (NS - 1);
 #pragma unroll
-        for (int r = 0; r < GPER; ++r) {
-            const int idx = tid + r * GNT;
-            const int row = idx & (GT - 1), kk = idx >> 5;
-            const int gm = m0 + row, gk = k0 + kk;
-            ra[r] = (gm < Kz && gk < Kz) ? Az[gm + (size_t)gk * K0] : 0.;
-            const int kb = idx % GKC, col = idx / GKC;
-            const int gkb = k0 + kb, gn = n0 + col;
-            double v = 0.;
-            if (gkb < Kz && gn < N) {
-                const double *xc = X + (size_t)gn * ldx;
-                const int h = n - 1 - gkb;
-                v = xc[gkb];
-                if (h != gkb) v = z ? v - xc[h] : v + xc[h];
-            }
-            rb[r] = v;
+            for (int r = 1; r < R; ++r) x[q][r] = cmul(x[q][r], tab[2 * r * jm * (M / (NS * R))]);
         }
-    };
-    load(0);
-    constexpr int KW = GKC / GNW;
-    for (int k0 = 0; k0 < Kz; k0 += GKC) {
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < GPER; ++r) {
-            const int idx = tid + r * GNT;
-            As[idx >> 5][idx & (GT - 1)] = ra[r];
-            Bs[idx % GKC][idx / GKC] = rb[r];
-        }
-        __syncthreads();
-        if (k0 + GKC < Kz) load(k0 + GKC);
-#pragma unroll
-        for (int kk = wave * KW; kk < wave * KW + KW; kk += 4) {
-            const int k = kk + (lane >> 4), c = lane & 15;
-            const double a0 = As[k][c], a1 = As[k][16 + c];
-            const double b0 = Bs[k][c], b1 = Bs[k][16 + c];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-    // MFMA 16x16 result: register r of lane l holds row (l >> 4) + 4 r, column l & 15.
-    double(*Red)[GT][33] = reinterpret_cast<double(*)[GT][33]>(sm);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                Red[wave][16 * i + (lane >> 4) + 4 * r][16 * j + (lane & 15)] = acc[i][j][r];
-    __syncthreads();
-    for (int e = tid; e < GT * GT; e += GNT) {
-        const int mm = e & (GT - 1), nn = e >> 5;
-        const int gm = m0 + mm, gn = n0 + nn;
-        if (gm >= Kz || gn >= N) continue;
-        double s = Red[0][mm][nn];
-#pragma unroll
-        for (int w = 1; w < GNW; ++w) s += Red[w][mm][nn];
-        const int om = 2 * gm + z;
-        if (EPI == 0) {
-            C[om + (size_t)gn * ldc] = s;
+        if (R == 4) {
+            const double2 a = x[q][0], b = x[q][1], c = x[q][2], d = x[q][3];
+            const double2 t0 = make_double2(a.x + c.x, a.y + c.y), t1 = make_double2(a.x - c.x, a.y - c.y);
+            const double2 t2 = make_double2(b.x + d.x, b.y + d.y), t3 = make_double2(b.x - d.x, b.y - d.y);
+            x[q][0] = make_double2(t0.x + t2.x, t0.y + t2.y);
+            x[q][1] = make_double2(t1.x + t3.y, t1.y - t3.x);
+            x[q][2] = make_double2(t0.x - t2.x, t0.y - t2.y);
+            x[q][3] = make_double2(t1.x - t3.y, t1.y + t3.x);
         } else {
-            // column gn is line gn % by of right-hand side gn / by
-            const uint32_t p = (uint32_t)(om + 2) + ((uint32_t)gn % by + 2) * nx;
-            C[p + (size_t)((uint32_t)gn / by) * np] = mask[p] ? s : 0.;
+            const double2 a = x[q][0], b = x[q][1];
+            x[q][0] = make_double2(a.x + b.x, a.y + b.y);
+            x[q][1] = make_double2(a.x - b.x, a.y - b.y);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < J; ++q) {
+        const uint32_t j = t + q * T;
+        const uint32_t o = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+        for (int r = 0; r < R; ++r) z[o + r * NS] = x[q][r];
+    }
+    __syncthreads();
+}
+
+template <int M, int NS>
+__device__ __forceinline__ void fft_passes(double2 *z, const double *v, uint32_t t, const double2 *__restrict__ tab)
+{
+    if constexpr (NS < M) {
+        constexpr int R = (M / NS) % 4 == 0 ? 4 : 2;
+        fft_pass<M, R, NS>(z, v, t, tab);
+        fft_passes<M, NS * R>(z, v, t, tab);
+    }
+}
+
+template <int M, int EPI>
+__global__ void __launch_bounds__(256)
+    k_dst_fft(uint32_t nLines, const double *__restrict__ X, uint32_t ldx, double *__restrict__ C, uint32_t ldc,
+              const double2 *__restrict__ tab, uint32_t nx, const uint8_t *__restrict__ mask, uint32_t by,
+              uint32_t np, uint32_t bxg)
+{
+    constexpr int T = FftShape<M>::T, LB = FftShape<M>::LB, n = M - 1;
+    __shared__ double2 sh[LB * M];
+    const uint32_t t = threadIdx.x % T, sub = threadIdx.x / T;
+    const uint32_t line = blockIdx.x * LB + sub;
+    const bool live = line < nLines;
+    double2 *z = sh + (size_t)sub * M;
+    double *v = reinterpret_cast<double *>(z) + M;   // the upper half of the line's buffer
+    const double *xl = X + (size_t)(live ? line : 0) * ldx;
+    for (uint32_t k = t; k < (uint32_t)n; k += T) v[k] = live ? xl[k] : 0.;
+    __syncthreads();
+    fft_passes<M, 1>(z, v, t, tab);
+    constexpr double f = -0.25;
+    const double sc = f * sqrt(2. / M);
+    if (!live) return;
+    uint32_t run = 0, jj = 0;
+    if (EPI == 2) {
+        run = line / by;
+        jj = line - run * by;
+    }
+    for (uint32_t k = t; k < (uint32_t)n; k += T) {
+        const double2 a = z[k + 1], b = z[M - 1 - k], w = tab[k + 1];
+        // w = (cos(pi (k+1)/M), -sin(pi (k+1)/M))
+        const double s = sc * (((a.y - b.y) - w.x * (a.x - b.x)) + w.y * (a.y + b.y));
+        if (EPI == 0) {
+            C[(size_t)line * ldc + k] = s;
+        } else if (k < bxg) {
+            const uint32_t p = (k + 2) + (jj + 2) * nx;
+            C[(size_t)run * np + p] = mask[p] ? s : 0.;
         }
     }
 }
 
-// The same transform for large boxes through library GEMMs: k_dst_fold forms P_z (col-major
-// kx x N each, ld kx; row kx - 1 of P_1 is zero padding for odd n), rocBLAS multiplies
-// T_z = F_z P_z, and k_dst_unfold interleaves the rows of T_0, T_1 into C (EPI 0) or scatters
-// them into the potential (EPI 2) exactly as k_dst_gemm's epilogues do.  From kx >= kBlasMinK
-// the library's large tiles beat the 32 x 32 tiles above (at kx = 1022, N = 2044: 81 us a
-// product against about 200); below it the launch and the extra passes cost more.
-constexpr int kBlasMinK = 384;
-
-__global__ void k_dst_fold(int n, int N, const double *__restrict__ X, int ldx, int kx, double *__restrict__ P)
-{
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)kx * N) return;
-    const int j = (int)(t % kx), c = (int)(t / kx);
-    const double *xc = X + (size_t)c * ldx;
-    const int h = n - 1 - j;
-    const double a = xc[j];
-    double p0 = a, p1 = 0.;  // the middle row of odd n enters P_0 once
-    if (h != j) {
-        const double b = xc[h];
-        p0 = a + b;
-        p1 = a - b;
-    }
-    P[j + (size_t)c * kx] = p0;
-    P[(size_t)kx * N + j + (size_t)c * kx] = p1;
-}
-
-template <int EPI>
-__global__ void k_dst_unfold(int N, int n, int kx, const double *__restrict__ T, double *__restrict__ C, int ldc,
-                             uint32_t nx, const uint8_t *__restrict__ mask, uint32_t by, uint32_t np)
-{
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)n * N) return;
-    const int om = (int)(t % n), gn = (int)(t / n);
-    const double s = T[(size_t)(om & 1) * kx * N + (om >> 1) + (size_t)gn * kx];
-    if (EPI == 0) {
-        C[om + (size_t)gn * ldc] = s;
-    } else {
-        const uint32_t p = (uint32_t)(om + 2) + ((uint32_t)gn % by + 2) * nx;
-        C[p + (size_t)((uint32_t)gn / by) * np] = mask[p] ? s : 0.;
-    }
-}
 
 // b = masked charge on the box; the potential outside the box is zero.
 __global__ void k_gather_box_zero(uint32_t nx, uint32_t ny, uint32_t bx, const uint8_t *__restrict__ mask,
@@ -620,7 +598,19 @@ int launch_cap_wy(const oecl_fastpoisson *h, bool cap, const double *yh, double 
 
 }  // namespace
 
-static int reserve_gemm(oecl_fastpoisson *h, uint32_t R);
+
+uint32_t fastpoisson_box_width(uint32_t nx)
+{
+    if (nx < 5) return 0;
+    uint32_t bx = 1;
+    while (bx < nx - 4) bx = 2 * bx + 1;
+    return bx;
+}
+
+OECL_EXPORT uint32_t oecl_fastpoisson_box_width(uint32_t nx)
+{
+    return fastpoisson_box_width(nx);
+}
 
 OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, double cyu, uint32_t m,
                                         const int32_t *sIdxHost, const int32_t *wPtrHost,
@@ -628,14 +618,19 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
                                         const uint8_t *maskDev, oecl_fastpoisson **out, double *capDiag)
 {
     if (!out || !maskDev || nx < 5 || ny < 5 || !(cxu > 0.) || !(cyu > 0.)) return OECL_ERR_ARG;
-    if (ny - 4 > 64 * 32 || nx - 4 > 64 * 32) {
-        oecl_set_error_msg("fastpoisson: nxExt - 4 and nyExt - 4 must not exceed 2048");
+    if (ny - 4 > 64 * 32 || nx - 4 > 64 * 32 - 1) {
+        oecl_set_error_msg("fastpoisson: nxExt - 4 must not exceed 2047 and nyExt - 4 not 2048");
         return OECL_ERR_ARG;
     }
     if (m > 0 && (!sIdxHost || !wPtrHost || !wColHost || !wValHost)) return OECL_ERR_ARG;
-    const uint32_t bx = nx - 4, by = ny - 4, nb = bx * by;
+    // The box is padded in x to bx = 2^p - 1 >= nx - 4 columns (the transform is an FFT of
+    // length bx + 1); the padding columns lie on the grid's boundary ring or beyond the grid and
+    // hold no inside node.  The caller's rows S and W are in this box's indices
+    // (i - 2) + (j - 2) bx, so an inside node next to the first padding column is an
+    // irregular row (R couples it to that column, A does not).
+    const uint32_t bxg = nx - 4, by = ny - 4, bx = fastpoisson_box_width(nx), nb = bx * by;
     for (uint32_t a = 0; a < m; ++a)
-        if (sIdxHost[a] < 0 || (uint32_t)sIdxHost[a] >= nb) return OECL_ERR_ARG;
+        if (sIdxHost[a] < 0 || (uint32_t)sIdxHost[a] >= nb || (uint32_t)sIdxHost[a] % bx >= bxg) return OECL_ERR_ARG;
     for (uint32_t a = 0; a < m; ++a)
         if (wPtrHost[a + 1] < wPtrHost[a] || wPtrHost[a + 1] - wPtrHost[a] > WMAX) return OECL_ERR_ARG;
     const uint32_t nnz = m ? (uint32_t)wPtrHost[m] : 0;
@@ -646,6 +641,7 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->ny = ny;
     h->np = nx * ny;
     h->bx = bx;
+    h->bxg = bxg;
     h->by = by;
     h->nb = nb;
     h->m = m;
@@ -653,8 +649,6 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->mask = maskDev;
     h->rCap = 1;
     h->batchMem = nullptr;
-    h->gemmMem = nullptr;
-    h->gP = h->gT = nullptr;
     // rows a grouped by their box column j (stable), for the per-column update
     std::vector<int32_t> jA(m), jCol, jStart;
     for (uint32_t a = 0; a < m; ++a) jA[a] = (int32_t)a;
@@ -676,14 +670,14 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
         oecl_set_error_msg("rocblas_create_handle failed");
         return OECL_ERR_HIP;
     }
-    const uint64_t kx = (bx + 1) / 2;
-    const uint64_t bytes = 8ull * ((uint64_t)bx * bx + 2ull * kx * kx + 4ull * nb + 2ull * m + (uint64_t)m * m + nnz) +
+    const uint64_t M = bx + 1ull;
+    const uint64_t bytes = 8ull * ((uint64_t)bx * bx + 4ull * M + 4ull * nb + 2ull * m + (uint64_t)m * m + nnz) +
                            4ull * (m + (m + 1) + nnz + h->nJ + (h->nJ + 1) + 2ull * m) + 1024;
     char *mem = nullptr;
     OECL_HIP(hipMalloc(&mem, bytes), "fastpoisson alloc");
     char *cur = mem;
     h->Sx = (double *)cur; cur += 8ull * bx * bx;
-    h->Fx = (double *)cur; cur += 16ull * kx * kx;
+    h->tab = (double2 *)cur; cur += 32ull * M;
     h->invw = (double *)cur; cur += 8ull * nb;
     h->b = (double *)cur; cur += 8ull * nb;
     h->t1 = (double *)cur; cur += 8ull * nb;
@@ -704,11 +698,12 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     std::vector<double> lx(bx);
     for (uint32_t q = 0; q < bx; ++q) lx[q] = 2. - 2. * cos(M_PI * (q + 1.) / (bx + 1.));
     {
-        // F_z[i][j] = Sx[2i + z][j], column-major kx x kx (odd bx: F_1's last row/column zero)
-        std::vector<double> F(2 * kx * kx, 0.);
-        for (uint32_t z = 0; z < 2; ++z)
-            for (uint32_t i = 0; 2 * i + z < bx; ++i)
-                for (uint32_t j = 0; j < kx; ++j) F[z * kx * kx + i + j * kx] = Sx[(2 * i + z) + (uint64_t)j * bx];
+        // twiddles e^{-i pi k / M}, k < 2M, from the angle pi k / M < 2 pi in extended precision
+        std::vector<double2> tw(2 * M);
+        for (uint64_t k = 0; k < 2 * M; ++k) {
+            const long double a = 3.141592653589793238462643383279502884L * (long double)k / (long double)M;
+            tw[k] = make_double2((double)cosl(a), -(double)sinl(a));
+        }
         // Thomas pivots of (cxu lx_q + 2 cyu) on the diagonal, -cyu off it
         std::vector<double> iw(nb);
         for (uint32_t q = 0; q < bx; ++q) {
@@ -720,7 +715,9 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
             }
         }
         OECL_HIP(hipMemcpy(h->Sx, Sx.data(), 8ull * bx * bx, hipMemcpyHostToDevice), "copy Sx");
-        OECL_HIP(hipMemcpy(h->Fx, F.data(), 16ull * kx * kx, hipMemcpyHostToDevice), "copy Fx");
+        OECL_HIP(hipMemcpy(h->tab, tw.data(), 32ull * M, hipMemcpyHostToDevice), "copy twiddles");
+        // the padding columns of b are never written by the gathers: zero once
+        OECL_HIP(hipMemset(h->b, 0, 8ull * nb), "zero box");
         OECL_HIP(hipMemcpy(h->invw, iw.data(), 8ull * nb, hipMemcpyHostToDevice), "copy pivots");
     }
     if (m) {
@@ -783,8 +780,6 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     } else if (capDiag) {
         *capDiag = 0.;
     }
-    const int rcg = reserve_gemm(h, 1);
-    if (rcg != OECL_OK) return rcg;
     OECL_HIP(hipDeviceSynchronize(), "fastpoisson create");
     *out = h;
     return OECL_OK;
@@ -795,32 +790,16 @@ FpBox fastpoisson_box(const oecl_fastpoisson *h)
     return FpBox{h->b, h->mask, h->nx, h->ny, h->bx};
 }
 
-// Buffers of the library-product path for R right-hand sides (large boxes only).
-static int reserve_gemm(oecl_fastpoisson *h, uint32_t R)
-{
-    const uint64_t kx = (h->bx + 1) / 2;
-    if ((int)kx < kBlasMinK) return OECL_OK;
-    const uint64_t e = 2ull * kx * h->by * R;
-    char *mem = nullptr;
-    OECL_HIP(hipMalloc(&mem, 16ull * e + 512), "fastpoisson product buffers");
-    if (h->gemmMem) (void)hipFree(h->gemmMem);
-    h->gemmMem = mem;
-    h->gP = (double *)mem;
-    h->gT = h->gP + e;
-    return OECL_OK;
-}
-
 int fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R)
 {
     if (R <= h->rCap) return OECL_OK;
     const uint64_t nb = h->nb, m = h->m;
-    int rcg = reserve_gemm(h, R);
-    if (rcg != OECL_OK) return rcg;
     char *mem = nullptr;
     OECL_HIP(hipMalloc(&mem, 8ull * R * (3 * nb + 2 * m) + 1024), "fastpoisson batch alloc");
     if (h->batchMem) (void)hipFree(h->batchMem);
     h->batchMem = mem;
     h->b = (double *)mem;
+    OECL_HIP(hipMemset(h->b, 0, 8ull * R * nb), "zero box");
     h->t1 = h->b + R * nb;
     h->t2 = h->t1 + R * nb;
     h->wy = h->t2 + R * nb;
@@ -844,38 +823,25 @@ OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, dou
     return fastpoisson_solve_box(h, phi, 1, s);
 }
 
-// One DST transform of the N columns of X (bx rows, ld bx): EPI 0 into C (ld ldc), EPI 2 into the
-// potential C = phi.  Library products for large boxes, except while the stream is being
-// captured into a graph (the library's calls are kept out of graphs).
+// One DST transform of the N lines of X (bx values each, ld bx): EPI 0 into C (ld ldc), EPI 2
+// into the potential C = phi.
 template <int EPI>
 static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C, int ldc, hipStream_t s)
 {
-    const int bx = h->bx, kx = (bx + 1) / 2;
-    bool lib = kx >= kBlasMinK && h->gemmMem;
-    if (lib) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) lib = false;
-    }
-    if (!lib) {
-        const dim3 gg((kx + GT - 1) / GT, (N + GT - 1) / GT, 2), gb(GNT);
-        hipLaunchKernelGGL(k_dst_gemm<EPI>, gg, gb, 0, s, N, bx, h->Fx, X, bx, C, ldc, h->nx, h->mask,
-                           (uint32_t)h->by, h->np);
+#define OECL_FFT(MM)                                                                                           \
+    case MM:                                                                                                 \
+        hipLaunchKernelGGL((k_dst_fft<MM, EPI>), dim3((N + FftShape<MM>::LB - 1) / FftShape<MM>::LB), dim3(256), \
+                           0, s, (uint32_t)N, X, h->bx, C, (uint32_t)ldc, h->tab, h->nx, h->mask, h->by, h->np,  \
+                           h->bxg);                                                                          \
         return OECL_OK;
+    switch (h->bx + 1) {
+        OECL_FFT(2) OECL_FFT(4) OECL_FFT(8) OECL_FFT(16) OECL_FFT(32) OECL_FFT(64) OECL_FFT(128) OECL_FFT(256)
+        OECL_FFT(512) OECL_FFT(1024) OECL_FFT(2048) OECL_FFT(4096)
+    default:
+        oecl_set_error_msg("fastpoisson: padded box width out of range");
+        return OECL_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_dst_fold, dim3((unsigned)(((size_t)kx * N + 255) / 256)), dim3(256), 0, s, bx, N, X, bx, kx,
-                       h->gP);
-    RB(rocblas_set_stream(h->blas, s), "fastpoisson set stream");
-    const double one = 1., zero = 0.;
-    for (int z = 0; z < 2; ++z) {
-        const int kz = z ? bx / 2 : kx;
-        RB(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, kz, N, kz, &one,
-                         h->Fx + (size_t)z * kx * kx, kx, h->gP + (size_t)z * kx * N, kx, &zero,
-                         h->gT + (size_t)z * kx * N, kx),
-           "fastpoisson dst product");
-    }
-    hipLaunchKernelGGL(k_dst_unfold<EPI>, dim3((unsigned)(((size_t)bx * N + 255) / 256)), dim3(256), 0, s, N, bx, kx,
-                       h->gT, C, ldc, h->nx, h->mask, (uint32_t)h->by, h->np);
-    return OECL_OK;
+#undef OECL_FFT
 }
 
 int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStream_t s)
@@ -909,7 +875,6 @@ OECL_EXPORT int oecl_fastpoisson_destroy(oecl_fastpoisson *h)
     if (!h) return OECL_OK;
     (void)hipFree(h->Sx);
     if (h->batchMem) (void)hipFree(h->batchMem);
-    if (h->gemmMem) (void)hipFree(h->gemmMem);
     rocblas_destroy_handle(h->blas);
     delete h;
     return OECL_OK;

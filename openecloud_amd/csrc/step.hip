@@ -16,8 +16,6 @@
 // whole blocks past the count return at once; the per-hit and per-secondary kernels loop
 // over the live counts with small fixed grids.  Overflows set status bits instead of writing
 // past a buffer; the host checks the status word.
-#include <rocblas/rocblas.h>
-
 #include "common.h"
 #include "fastpoisson_internal.h"
 #include "furman_dev.h"

@@ -32,11 +32,16 @@ from . import _lib as L
 from .constants import epsilon_0
 
 
-def capacitance_rows(nx, ny, mask, cx, cy):
+def capacitance_rows(nx, ny, mask, cx, cy, bx=None):
     """Rows of (M - R) for the capacitance method (see csrc/fastpoisson.hip).
 
-    Returns (cxu, cyu, sIdx, wPtr, wCol, wVal) with box indices t = (i-2) + (j-2)*(nx-4)."""
-    bx, by = nx - 4, ny - 4
+    The box holds the nodes 2 <= i < 2 + bx, 2 <= j < ny - 2 (bx >= nx - 4, default nx - 4;
+    columns from nx - 2 on hold no inside node).  Returns (cxu, cyu, sIdx, wPtr, wCol, wVal)
+    with box indices t = (i-2) + (j-2)*bx."""
+    bx = nx - 4 if bx is None else int(bx)
+    if bx < nx - 4:
+        raise ValueError('the box must hold every interior column')
+    by = ny - 4
     m2 = mask.reshape(ny, nx).astype(bool)
     cx2 = cx.reshape(ny, nx)
     cy2 = cy.reshape(ny, nx)
@@ -54,7 +59,7 @@ def capacitance_rows(nx, ny, mask, cx, cy):
     okn[:2, :] = True
     okn[ny - 2:, :] = True
     okn[:, :2] = True
-    okn[:, nx - 2:] = True
+    okn[:, 2 + bx:] = True
     reg = np.zeros_like(m2)
     c = (slice(1, ny - 1), slice(1, nx - 1))
     reg[c] = (m2[c] & okn[1:-1, 2:] & okn[1:-1, :-2] & okn[2:, 1:-1] & okn[:-2, 1:-1] &
@@ -69,7 +74,7 @@ def capacitance_rows(nx, ny, mask, cx, cy):
         s_idx.append(t)
         row = [(t, d - (2 * cxu + 2 * cyu))]
         for ii, jj, c, u in nb:
-            if 2 <= ii < nx - 2 and 2 <= jj < ny - 2:
+            if 2 <= ii < 2 + bx and 2 <= jj < ny - 2:
                 diff = (-c if m2[jj, ii] else 0.) + u
                 if diff != 0.:
                     row.append(((ii - 2) + (jj - 2) * bx, diff))
@@ -135,8 +140,11 @@ class PoissonSolver:
 
     def _setup_direct(self):
         nx, ny = self.nx, self.ny
+        bx = int(L.lib().oecl_fastpoisson_box_width(L.u32(nx)))
+        if bx == 0:
+            raise L.OeclError('direct solver: grid too small')
         cxu, cyu, sIdx, wPtr, wCol, wVal = capacitance_rows(
-            nx, ny, self.mask.cpu().numpy(), self.cx_host, self.cy_host)
+            nx, ny, self.mask.cpu().numpy(), self.cx_host, self.cy_host, bx)
         self.capacitanceRows = int(sIdx.size)
         self._fp_keep = (sIdx, wPtr, wCol, wVal)
         h = ctypes.c_void_p(0)

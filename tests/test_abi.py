@@ -38,6 +38,10 @@ def test_size_queries_without_gpu():
     lib.oecl_pcg_work_bytes.restype = ctypes.c_uint64
     assert lib.oecl_scan_work_bytes(ctypes.c_uint32(10 ** 6)) > 0
     assert lib.oecl_pcg_work_bytes(ctypes.c_uint32(8580)) >= 5 * 8 * 8580
+    # the direct solver's box width: the least 2^p - 1 holding the nxExt - 4 interior columns
+    lib.oecl_fastpoisson_box_width.restype = ctypes.c_uint32
+    widths = {nx: lib.oecl_fastpoisson_box_width(ctypes.c_uint32(nx)) for nx in (4, 5, 6, 8, 11, 12, 259, 260, 2048, 2051)}
+    assert widths == {4: 0, 5: 1, 6: 3, 8: 7, 11: 7, 12: 15, 259: 255, 260: 511, 2048: 2047, 2051: 2047}
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason='liboecl.so not built')

@@ -70,3 +70,37 @@ def test_point_charge_symmetry():
     assert np.allclose(P, P[:, ::-1], rtol=1e-10, atol=1e-14 * np.abs(P).max())
     assert np.allclose(P, P.T, rtol=1e-10, atol=1e-14 * np.abs(P).max())
     assert P.max() == P.flat[c] > 0
+
+
+@pytest.mark.parametrize('args', [(40, 24, 0.1, 0.05, 0), (37, 30, 0.09, 0.07, 1), (64, 48, 0.09, 0.07, 1)])
+@pytest.mark.parametrize('pad', [0, 1, 9])
+def test_capacitance_rows_rebuild_the_operator_on_padded_boxes(args, pad):
+    """M = R + U W over the box of bx = nxExt - 4 + pad columns (R: the uniform 5-point
+    operator with zero Dirichlet data outside the box) has exactly the oracle operator's rows on
+    inside nodes, including the nodes next to the first padding column."""
+    import scipy.sparse as spsp
+    from openecloud_amd.poissonSolver import capacitance_rows
+    g = Grid(*args)
+    ps = OraclePoisson(g)
+    nx, ny, npx = g.nxExt, g.nyExt, g.npExt
+    ip = np.asarray(g.insidePoints)
+    meps = EPS0 * np.asarray(g.dsi, np.float64) * np.roll(np.asarray(g.dst, np.float64), npx)
+    bx, by = nx - 4 + pad, ny - 4
+    cxu, cyu, sIdx, wPtr, wCol, wVal = capacitance_rows(nx, ny, ip.astype(np.uint8), meps[:npx], meps[npx:], bx)
+    T = lambda n: spsp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])
+    R = spsp.kron(spsp.identity(by), cxu * T(bx)) + spsp.kron(cyu * T(by), spsp.identity(bx))
+    W = spsp.csr_matrix((wVal, wCol, wPtr), shape=(sIdx.size, bx * by))
+    U = spsp.csr_matrix((np.ones(sIdx.size), (sIdx, np.arange(sIdx.size))), shape=(bx * by, sIdx.size))
+    M = (R + U @ W).tocsr()
+    # box index of each inside node, in the oracle's inside ordering
+    J, I = np.divmod(ps.insidePointsInd, nx)
+    t = (I - 2) + (J - 2) * bx
+    assert (I >= 2).all() and (I < nx - 2).all()
+    rows = M[t, :].toarray()
+    ref = np.zeros_like(rows)
+    ref[:, t] = ps.a.toarray() if spsp.issparse(ps.a) else ps.a
+    assert np.abs(rows - ref).max() <= 1e-12 * np.abs(ref).max()
+    if pad:
+        # the last interior column's inside nodes now see a padding column
+        last = t[I == nx - 3]
+        assert set(last.tolist()) <= set(sIdx.tolist())
