@@ -51,7 +51,7 @@ struct oecl_fastpoisson {
     double *wVal;
     int32_t *jCol, *jStart;   // distinct columns j of the irregular nodes, their entry ranges
     int32_t *ePos;            // [m] entry of row a in the by-column order
-    uint32_t *colOff;         // [m] i_a * bx of entry e (by-column order)
+    uint32_t *iPos;           // [m] box column i_a of entry e (by-column order)
     double *CinvT;            // m*m, C^-1 row-major (row a contiguous)
     double *b, *t1, *t2;      // nb each, per right-hand side
     double *wy, *zE;          // m each (zE in the by-column order), per right-hand side
@@ -154,40 +154,73 @@ __device__ __forceinline__ void fft_passes(double2 *z, const double *v, uint32_t
     }
 }
 
+struct FftArgs {
+    uint32_t nLines;
+    const double *X;            // EPI 0, 2: line l at X + l * ldx; EPI 3: zE (m per right-hand side)
+    uint32_t ldx;
+    double *C;                  // EPI 0, 3: line l at C + l * ldc (EPI 3: line jCol[c] of side r); EPI 2: phi
+    uint32_t ldc;
+    const double2 *tab;
+    uint32_t nx, by, np, bxg;   // EPI 2: the extended grid and its interior columns
+    const uint8_t *mask;
+    const int32_t *jCol, *jStart;   // EPI 3: distinct box rows of the corrected nodes, entry ranges
+    const uint32_t *iPos;           // EPI 3: box column of entry e
+    uint32_t nJ, m;
+    uint64_t nb;
+};
+
+// EPI 3 transforms the sparse lines u_c (z values of the entries of box row jCol[c] at their
+// columns, zero elsewhere; line l = r nJ + c for right-hand side r) and subtracts the result
+// from row jCol[c] of C: hat b -= Sx U z in the transformed space.  X and C may be the same
+// buffer (EPI 0 in place): a line is read whole into LDS before any of it is written.
 template <int M, int EPI>
-__global__ void __launch_bounds__(256)
-    k_dst_fft(uint32_t nLines, const double *__restrict__ X, uint32_t ldx, double *__restrict__ C, uint32_t ldc,
-              const double2 *__restrict__ tab, uint32_t nx, const uint8_t *__restrict__ mask, uint32_t by,
-              uint32_t np, uint32_t bxg)
+__global__ void __launch_bounds__(256) k_dst_fft(FftArgs g)
 {
     constexpr int T = FftShape<M>::T, LB = FftShape<M>::LB, n = M - 1;
     __shared__ double2 sh[LB * M];
     const uint32_t t = threadIdx.x % T, sub = threadIdx.x / T;
     const uint32_t line = blockIdx.x * LB + sub;
-    const bool live = line < nLines;
+    const bool live = line < g.nLines;
     double2 *z = sh + (size_t)sub * M;
     double *v = reinterpret_cast<double *>(z) + M;   // the upper half of the line's buffer
-    const double *xl = X + (size_t)(live ? line : 0) * ldx;
-    for (uint32_t k = t; k < (uint32_t)n; k += T) v[k] = live ? xl[k] : 0.;
-    __syncthreads();
-    fft_passes<M, 1>(z, v, t, tab);
-    constexpr double f = -0.25;
-    const double sc = f * sqrt(2. / M);
-    if (!live) return;
-    uint32_t run = 0, jj = 0;
-    if (EPI == 2) {
-        run = line / by;
-        jj = line - run * by;
+    uint32_t r = 0, c = 0;
+    if (EPI == 3) {
+        r = live ? line / g.nJ : 0;
+        c = live ? line - r * g.nJ : 0;
+        for (uint32_t k = t; k < (uint32_t)n; k += T) v[k] = 0.;
+        __syncthreads();
+        if (live) {
+            const double *ze = g.X + (size_t)r * g.m;
+            for (int32_t e = g.jStart[c] + (int32_t)t; e < g.jStart[c + 1]; e += T) v[g.iPos[e]] = ze[e];
+        }
+    } else {
+        const double *xl = g.X + (size_t)(live ? line : 0) * g.ldx;
+        for (uint32_t k = t; k < (uint32_t)n; k += T) v[k] = live ? xl[k] : 0.;
     }
+    __syncthreads();
+    fft_passes<M, 1>(z, v, t, g.tab);
+    if (!live) return;
+    const double sc = -0.25 * sqrt(2. / M);
+    double *out = g.C;
+    uint32_t run = 0, jj = 0;
+    if (EPI == 0) out += (size_t)line * g.ldc;
+    if (EPI == 2) {
+        run = line / g.by;
+        jj = line - run * g.by;
+        out += (size_t)run * g.np;
+    }
+    if (EPI == 3) out += (size_t)r * g.nb + (size_t)g.jCol[c] * g.ldc;
     for (uint32_t k = t; k < (uint32_t)n; k += T) {
-        const double2 a = z[k + 1], b = z[M - 1 - k], w = tab[k + 1];
+        const double2 a = z[k + 1], b = z[M - 1 - k], w = g.tab[k + 1];
         // w = (cos(pi (k+1)/M), -sin(pi (k+1)/M))
         const double s = sc * (((a.y - b.y) - w.x * (a.x - b.x)) + w.y * (a.y + b.y));
         if (EPI == 0) {
-            C[(size_t)line * ldc + k] = s;
-        } else if (k < bxg) {
-            const uint32_t p = (k + 2) + (jj + 2) * nx;
-            C[(size_t)run * np + p] = mask[p] ? s : 0.;
+            out[k] = s;
+        } else if (EPI == 3) {
+            out[k] -= s;
+        } else if (k < g.bxg) {
+            const uint32_t p = (k + 2) + (jj + 2) * g.nx;
+            out[p] = g.mask[p] ? s : 0.;
         }
     }
 }
@@ -477,29 +510,18 @@ __global__ void __launch_bounds__(256)
     if (lane == 0) zE[ePos[a]] = s;
 }
 
-// hat b[q, j] -= sum over rows a with j_a = j of z_a Sx[q, i_a]: one thread per (q, column
-// of the distinct-column list), rows taken in order (deterministic).  Entry e of the column
-// order carries zE[e] and the offset colOff[e] = i_a * bx of its Sx column, so the loads of
-// different entries are independent.  blockIdx.z: the right-hand side (m apart in zE, nb in bh).
+// wy[a] = sum_k W[a, k] y[k] with y = R^-1 b in the box (physical) layout: one thread per row a;
+// blockIdx.y: the right-hand side (m apart in wy, nb in y).
 __global__ void __launch_bounds__(256)
-    k_cap_update(uint32_t nJ, uint32_t bx, const int32_t *__restrict__ jCol, const int32_t *__restrict__ jStart,
-                 const uint32_t *__restrict__ colOff, const double *__restrict__ zE, const double *__restrict__ Sx,
-                 double *__restrict__ bh, uint32_t m, size_t nb)
+    k_wy(uint32_t m, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col, const double *__restrict__ val,
+         const double *__restrict__ y, double *__restrict__ wy, size_t nb)
 {
-    const uint32_t q = blockIdx.x * 64 + (threadIdx.x & 63);
-    const uint32_t c = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (q >= bx || c >= nJ) return;
-    zE += (size_t)blockIdx.z * m;
-    bh += (size_t)blockIdx.z * nb;
-    const int32_t e0 = jStart[c], e1 = jStart[c + 1];
-    double s[4] = {0., 0., 0., 0.};
-    int32_t e = e0;
-    for (; e + 3 < e1; e += 4) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) s[u] = fma(zE[e + u], Sx[q + colOff[e + u]], s[u]);
-    }
-    for (; e < e1; ++e) s[0] = fma(zE[e], Sx[q + colOff[e]], s[0]);
-    bh[q + (size_t)jCol[c] * bx] -= (s[0] + s[1]) + (s[2] + s[3]);
+    const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= m) return;
+    y += (size_t)blockIdx.y * nb;
+    double s = 0.;
+    for (int32_t k = ptr[a]; k < ptr[a + 1]; ++k) s = fma(val[k], y[col[k]], s);
+    wy[(size_t)blockIdx.y * m + a] = s;
 }
 
 #define RB(call, where)                                         \
@@ -692,7 +714,7 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->jCol = (int32_t *)cur; cur += 4ull * h->nJ;
     h->jStart = (int32_t *)cur; cur += 4ull * (h->nJ + 1);
     h->ePos = (int32_t *)cur; cur += 4ull * m;
-    h->colOff = (uint32_t *)cur; cur += 4ull * m;
+    h->iPos = (uint32_t *)cur; cur += 4ull * m;
 
     const std::vector<double> Sx = dst1(bx);
     std::vector<double> lx(bx);
@@ -728,13 +750,13 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
         OECL_HIP(hipMemcpy(h->jCol, jCol.data(), 4ull * h->nJ, hipMemcpyHostToDevice), "copy columns");
         OECL_HIP(hipMemcpy(h->jStart, jStart.data(), 4ull * (h->nJ + 1), hipMemcpyHostToDevice), "copy columns");
         std::vector<int32_t> ePos(m);
-        std::vector<uint32_t> colOff(m);
+        std::vector<uint32_t> iPos(m);
         for (uint32_t e = 0; e < m; ++e) {
             ePos[jA[e]] = (int32_t)e;
-            colOff[e] = ((uint32_t)sIdxHost[jA[e]] % bx) * bx;
+            iPos[e] = (uint32_t)sIdxHost[jA[e]] % bx;
         }
         OECL_HIP(hipMemcpy(h->ePos, ePos.data(), 4ull * m, hipMemcpyHostToDevice), "copy columns");
-        OECL_HIP(hipMemcpy(h->colOff, colOff.data(), 4ull * m, hipMemcpyHostToDevice), "copy columns");
+        OECL_HIP(hipMemcpy(h->iPos, iPos.data(), 4ull * m, hipMemcpyHostToDevice), "copy columns");
         // Capacitance matrix C = I + W R^-1 U with exactly the operator the solve applies
         // (Sx, the Thomas scan, the W dot products), in chunks of `chunk` unit columns.
         uint32_t chunk = (uint32_t)((256ull << 20) / (8ull * nb));
@@ -823,16 +845,33 @@ OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, dou
     return fastpoisson_solve_box(h, phi, 1, s);
 }
 
-// One DST transform of the N lines of X (bx values each, ld bx): EPI 0 into C (ld ldc), EPI 2
-// into the potential C = phi.
+// One DST transform of N lines: EPI 0 of X (ld bx) into C (ld ldc; C may be X), EPI 2 of X into
+// the potential C = phi, EPI 3 of the sparse lines of zE (N = R nJ) subtracted from C (ld bx).
 template <int EPI>
 static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C, int ldc, hipStream_t s)
 {
-#define OECL_FFT(MM)                                                                                           \
-    case MM:                                                                                                 \
-        hipLaunchKernelGGL((k_dst_fft<MM, EPI>), dim3((N + FftShape<MM>::LB - 1) / FftShape<MM>::LB), dim3(256), \
-                           0, s, (uint32_t)N, X, h->bx, C, (uint32_t)ldc, h->tab, h->nx, h->mask, h->by, h->np,  \
-                           h->bxg);                                                                          \
+    FftArgs g;
+    g.nLines = (uint32_t)N;
+    g.X = X;
+    g.ldx = h->bx;
+    g.C = C;
+    g.ldc = (uint32_t)ldc;
+    g.tab = h->tab;
+    g.nx = h->nx;
+    g.by = h->by;
+    g.np = h->np;
+    g.bxg = h->bxg;
+    g.mask = h->mask;
+    g.jCol = h->jCol;
+    g.jStart = h->jStart;
+    g.iPos = h->iPos;
+    g.nJ = h->nJ;
+    g.m = h->m;
+    g.nb = h->nb;
+#define OECL_FFT(MM)                                                                                    \
+    case MM:                                                                                          \
+        hipLaunchKernelGGL((k_dst_fft<MM, EPI>), dim3((N + FftShape<MM>::LB - 1) / FftShape<MM>::LB), \
+                           dim3(256), 0, s, g);                                                       \
         return OECL_OK;
     switch (h->bx + 1) {
         OECL_FFT(2) OECL_FFT(4) OECL_FFT(8) OECL_FFT(16) OECL_FFT(32) OECL_FFT(64) OECL_FFT(128) OECL_FFT(256)
@@ -849,20 +888,23 @@ int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStrea
     if (R < 1 || R > h->rCap || (uint64_t)h->by * R > 0x7FFFFFFFull) return OECL_ERR_ARG;
     const int bx = h->bx, by = h->by, N = by * (int)R;
     // hat b = Sx b
-    int rct = dst_transform<0>(h, N, h->b, h->t1, bx, s);
-    if (rct != OECL_OK) return rct;
+    int rc = dst_transform<0>(h, N, h->b, h->t1, bx, s);
+    if (rc != OECL_OK) return rc;
     if (h->m) {
-        // hat y = T^-1 hat b ; wy = W Sx hat y ; hat b -= (U C^-1 wy) in x-transformed form
-        int rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
+        // hat y = T^-1 hat b ; y = Sx hat y (in place) ; wy = W y ; z = C^-1 wy ;
+        // hat b -= Sx U z, line by line over the box rows holding corrected nodes
+        rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
         if (rc != OECL_OK) return rc;
-        rc = launch_cap_wy(h, false, h->t2, h->wy, 0, R, s);
+        rc = dst_transform<0>(h, N, h->t2, h->t2, bx, s);
         if (rc != OECL_OK) return rc;
+        hipLaunchKernelGGL(k_wy, dim3((h->m + 255) / 256, R), dim3(256), 0, s, h->m, h->wPtr, h->wCol, h->wVal, h->t2,
+                           h->wy, (size_t)h->nb);
         hipLaunchKernelGGL(k_cap_z, dim3((h->m + 3) / 4, R), dim3(256), 0, s, h->m, h->CinvT, h->wy, h->ePos, h->zE);
-        hipLaunchKernelGGL(k_cap_update, dim3((h->bx + 63) / 64, (h->nJ + 3) / 4, R), dim3(256), 0, s, h->nJ, h->bx,
-                           h->jCol, h->jStart, h->colOff, h->zE, h->Sx, h->t1, h->m, (size_t)h->nb);
+        rc = dst_transform<3>(h, (int)(h->nJ * R), h->zE, h->t1, bx, s);
+        if (rc != OECL_OK) return rc;
     }
     // hat x = T^-1 hat b ; phi = Sx hat x scattered into the extended grid (np apart per side)
-    int rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
+    rc = launch_thomas(h, false, h->t1, h->t2, 0, R, s);
     if (rc != OECL_OK) return rc;
     rc = dst_transform<2>(h, N, h->t2, phi, 0, s);
     if (rc != OECL_OK) return rc;
