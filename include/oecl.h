@@ -506,6 +506,12 @@ typedef struct {
     int32_t externalPhi;        /* 0: the step solves for phi with the direct solver; 1: phi (per
                                    run) already holds the potential of this step's charge
                                    rho + s(step) * profile, solved by the caller (any method) */
+    double *EBil;               /* [4 npExt] (per run), typeB 2 with a grid field whose Bz is zero
+                                   everywhere: (Ex, Ey, Bx, By) per node, Bx and By filled by the
+                                   caller from bField, Ex and Ey written by the step; the push
+                                   gathers E and B from it (two 16-byte loads per node, the same
+                                   products in the same order).  NULL: E from Eil or E, B from
+                                   bField.  No reference counterpart (a layout of the same data). */
 } oecl_step_args;
 
 /* Particles per push block (256): batched run segments are multiples of it, and the per-block

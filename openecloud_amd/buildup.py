@@ -40,7 +40,18 @@ class StepArgs(ctypes.Structure):
                 ('evPushBegin', _VP), ('evPushEnd', _VP), ('photoCount', _VP), ('photoWeight', _VP),
                 ('nPhoto', ctypes.c_uint32), ('photoE0', ctypes.c_double),
                 ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32),
-                ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32)]
+                ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32), ('EBil', _VP)]
+
+
+def eb_layout(bField, npx, runs=1):
+    """The (Ex, Ey, Bx, By)-per-node buffer of a grid field whose Bz is zero everywhere (Bx, By
+    filled, E left for the step), runs copies; None when Bz is not zero."""
+    if bField is None or bool((bField[2 * npx:3 * npx] != 0).any()):
+        return None
+    eb = torch.zeros((runs, npx, 4), dtype=torch.float64, device=bField.device)
+    eb[:, :, 2] = bField[:npx]
+    eb[:, :, 3] = bField[npx:2 * npx]
+    return eb
 
 
 ST_ABS_OVERFLOW, ST_SEC_OVERFLOW, ST_CAP_OVERFLOW, ST_PHOTO_OVERFLOW = 1, 2, 4, 8
@@ -323,6 +334,8 @@ class Buildup:
             self._prm = self.em.desc
         self._eil = torch.empty(2 * self.g.getNpExt(), dtype=torch.float64, device=dev)
         a.Eil = self._eil.data_ptr()
+        self._ebil = eb_layout(self.bField, self.g.getNpExt()) if self.typeB == 2 else None
+        a.EBil = self._ebil.data_ptr() if self._ebil is not None else None
         a.work = self._stepwork.data_ptr()
         self._args = a
         if not self._rho_valid:

@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(PT)
     const uint64_t first = (uint64_t)blk * PB;
     if (first - sBeg >= n) return;  // whole block past the run's live particles
     E += (size_t)run * 2 * g.np;
-    if (Eil) Eil += (size_t)run * 2 * g.np;
+    if (Eil) Eil += (size_t)run * (TYPEB == 3 ? 4 : 2) * g.np;
     rho += (size_t)run * g.np;
     misc += run * M_COUNT;
     nzList += sBeg / 64;
@@ -670,9 +670,11 @@ namespace {
 
 // E_r = phiToE phi_r for every run r = blockIdx.y (rows in the CSR's own column order, the
 // arithmetic of oecl_csr_spmv); with yil, also the interleaved copy yil[2 (r % xStride) + r / xStride].
+// E = phiToE phi per run (rows [Ex | Ey], nRows = 2 np); also into the interleaved copy yil when
+// given: ilw doubles per node (2: (Ex, Ey); 4: (Ex, Ey, Bx, By), B left as it is).
 __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, const int32_t *__restrict__ col,
                             const double *__restrict__ val, const double *__restrict__ x, uint32_t xStride,
-                            double *__restrict__ y, double *__restrict__ yil)
+                            double *__restrict__ y, double *__restrict__ yil, uint32_t ilw)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nRows) return;
@@ -681,7 +683,7 @@ __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, 
     double sum = 0.;
     for (int32_t k = rowPtr[r]; k < rowPtr[r + 1]; ++k) sum += val[k] * x[col[k]];
     y[r] = sum;
-    if (yil) yil[(size_t)blockIdx.y * nRows + 2 * (r % xStride) + r / xStride] = sum;
+    if (yil) yil[(size_t)blockIdx.y * ilw * xStride + ilw * (r % xStride) + r / xStride] = sum;
 }
 
 }  // namespace
@@ -713,8 +715,10 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
         rc = fastpoisson_solve_box(fp, a->phi, R, s);
         if (rc) return rc;
     }
+    // grid field with Bz = 0: E and B gathered from the (Ex, Ey, Bx, By) layout
+    const bool eb = a->typeB == 2 && a->EBil;
     hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
-                       a->e2nCol, a->e2nVal, a->phi, np, a->E, a->Eil);
+                       a->e2nCol, a->e2nVal, a->phi, np, a->E, eb ? a->EBil : a->Eil, eb ? 4u : 2u);
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
     const dim3 gp((unsigned)nBlk), bp(PT);
     if (a->evPushBegin) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushBegin, s), "event");
@@ -724,6 +728,9 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     else if (a->typeB == 1)
         hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, a->b0,
                            a->b1, a->b2, a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
+    else if (eb)
+        hipLaunchKernelGGL((k_push_dev<T, 3>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->EBil, a->bField, 0., 0.,
+                           0., a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
     else
         hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
                            a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);

@@ -389,10 +389,13 @@ def test_native_step_deposit_paths_match_oracle(orc):
     assert np.allclose(rho, rho_o, rtol=1e-13, atol=1e-13 * np.abs(rho_o).max())
 
 
-def test_native_step_grid_b_matches_oracle(orc):
+@pytest.mark.parametrize('layout', ['eb', 'separate'])
+def test_native_step_grid_b_matches_oracle(orc, layout):
     """B on the grid (quadrupole plus a skew term, typeBField 2, gathered at each particle like
     E): the one-call native step against the oracle loop, FP64 to 1e-12 of each column's
-    scale (a swapped or dropped B component, with fields of ~2 T, would differ at order one)."""
+    scale (a swapped or dropped B component, with fields of ~2 T, would differ at order one).
+    Bz is zero, so the step gathers E and B from the interleaved (Ex, Ey, Bx, By) layout
+    ('eb'); 'separate' takes that layout away and gathers from E and the three B arrays."""
     from openecloud_amd import randomGen
     from openecloud_amd.beam import GaussianBunchTrain
     from openecloud_amd.buildup import Buildup
@@ -420,6 +423,9 @@ def test_native_step_grid_b_matches_oracle(orc):
                               nBunches=1, bunchLengthSigma=0.003)
     b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=2, bAtGridPoints=bgrid)
     b.native_setup(absCap=8192, secCap=16384, maxSteps=64)
+    assert b._ebil is not None
+    if layout == 'separate':
+        b._args.EBil = None
     nsteps = 10
     ref = _oracle_loop(orc, g, pd0.copy(), nsteps, dt, 2, bgrid, beam.getTransversalProfile(),
                        beam.getScale, material('copper', 1.6, eHat0=300.), key, n, rule='fast')

@@ -51,3 +51,20 @@ def test_beam_line_charge_per_bunch():
     tot = sum(b.getScale(k * dt) for k in range(int(25e-9 / dt)))
     assert math.isclose(tot * dt * c, 1.15e11 * elementary_charge, rel_tol=1e-6)
     assert b.getScale(25e-9 + 3 * dt) == 0.   # one bunch only
+
+
+def test_eb_layout_interleaves_a_transverse_grid_field():
+    """The step's (Ex, Ey, Bx, By) layout: Bx and By of each node in slots 2 and 3 for every run,
+    E slots left zero; a field with any nonzero Bz keeps the separate arrays (None)."""
+    import torch
+    from openecloud_amd.buildup import eb_layout
+    npx = 7
+    b = torch.arange(3 * npx, dtype=torch.float64) + 1.
+    assert eb_layout(b, npx) is None
+    b[2 * npx:] = 0.
+    eb = eb_layout(b, npx, runs=3)
+    assert eb.shape == (3, npx, 4)
+    assert torch.equal(eb[:, :, 2], b[:npx].expand(3, npx))
+    assert torch.equal(eb[:, :, 3], b[npx:2 * npx].expand(3, npx))
+    assert not eb[:, :, :2].any()
+    assert eb_layout(None, npx) is None
