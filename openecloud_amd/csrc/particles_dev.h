@@ -140,6 +140,26 @@ __device__ __forceinline__ uint32_t wave_minmax(uint32_t v)
     return MIN ? min(min(a, b), min(c, d)) : max(max(a, b), max(c, d));
 }
 
+// Minimum (MIN) or maximum over the wave of two 16-bit fields at once (v = lo | hi << 16;
+// packed 16-bit integer min/max per field), uniform result.
+template <bool MIN>
+__device__ __forceinline__ uint32_t wave_minmax2x16(uint32_t v)
+{
+    typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+    const uint32_t id = MIN ? 0xFFFFFFFFu : 0u;
+    auto op = [](uint32_t a, uint32_t b) {
+        const u2 x = __builtin_bit_cast(u2, a), y = __builtin_bit_cast(u2, b);
+        return __builtin_bit_cast(uint32_t, MIN ? __builtin_elementwise_min(x, y) : __builtin_elementwise_max(x, y));
+    };
+    v = op(v, dpp_u32<0x111, 0xF>(v, id));
+    v = op(v, dpp_u32<0x112, 0xF>(v, id));
+    v = op(v, dpp_u32<0x114, 0xF>(v, id));
+    v = op(v, dpp_u32<0x118, 0xF>(v, id));
+    const uint32_t a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
+    const uint32_t c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
+    return op(op(a, b), op(c, d));
+}
+
 // Wave-cooperative deposit (all 64 lanes call it; valid = false when the lane has no
 // particle), with the particle's cell (ix, iy) and its four corner charges.  Runs of equal
 // cells are summed first (wave_run_sum).  The run totals are then summed in the wave's own
@@ -213,8 +233,11 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
 #else
     const bool head = valid;
 #endif
-    const uint32_t X0 = wave_minmax<true>(head ? ix : 0xFFFFFFFFu), X1 = wave_minmax<false>(head ? ix : 0u);
-    const uint32_t Y0 = wave_minmax<true>(head ? iy : 0xFFFFFFFFu), Y1 = wave_minmax<false>(head ? iy : 0u);
+    // the heads' bounding box in cells (ix, iy < 0xFFFF: both fit 16 bits; no head: X0 = 0xFFFFFFFF)
+    const uint32_t xy = ix | iy << 16;
+    const uint32_t mn = wave_minmax2x16<true>(head ? xy : 0xFFFFFFFFu), mx = wave_minmax2x16<false>(head ? xy : 0u);
+    const uint32_t X0 = mn == 0xFFFFFFFFu ? 0xFFFFFFFFu : mn & 0xFFFFu, X1 = mx & 0xFFFFu;
+    const uint32_t Y0 = mn >> 16, Y1 = mx >> 16;
     if (threadIdx.x == 0) {
         box[0] = 0xFFFFFFFFu;
         box[1] = 0u;

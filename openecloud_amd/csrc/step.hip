@@ -158,7 +158,7 @@ template <typename T, int TYPEB>
 __global__ void __launch_bounds__(PT)
     k_push_dev(Runs runs, oecl_grid_t g, WallDev wall, Soa<T> p, const uint32_t *__restrict__ nDev,
                const double *__restrict__ E, const double *__restrict__ Eil, const double *__restrict__ B, double b0, double b1,
-               double b2, double dt, double k, double q, uint8_t *__restrict__ isInside,
+               double b2, double dt, double k, double q, double dxi, double dyi, uint8_t *__restrict__ isInside,
                double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
                uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
 {
@@ -178,7 +178,8 @@ __global__ void __launch_bounds__(PT)
     // FP32 storage is widened on load: weights, gather and the Boris rotation run in FP64
     // registers.
     using C = double;
-    const C dxi = 1. / g.dx, dyi = 1. / g.dy, lxh = g.lxHalfExt, lyh = g.lyHalfExt;
+    // dxi = 1 / dx and dyi = 1 / dy come from the host (the same correctly rounded quotients)
+    const C lxh = g.lxHalfExt, lyh = g.lyHalfExt;
     bool dep[KP], dead[KP];
     Weights<C> w2[KP];
     double wq[KP];
@@ -720,20 +721,21 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
                        a->e2nCol, a->e2nVal, a->phi, np, a->E, eb ? a->EBil : a->Eil, eb ? 4u : 2u);
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
+    const double dxi = 1. / g->dx, dyi = 1. / g->dy;
     const dim3 gp((unsigned)nBlk), bp(PT);
     if (a->evPushBegin) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushBegin, s), "event");
     if (a->typeB == 0)
         hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
-                           a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
+                           a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
     else if (a->typeB == 1)
         hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, a->b0,
-                           a->b1, a->b2, a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
+                           a->b1, a->b2, a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
     else if (eb)
         hipLaunchKernelGGL((k_push_dev<T, 3>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->EBil, a->bField, 0., 0.,
-                           0., a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
+                           0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
     else
         hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
-                           a->dt, kk, a->qe, a->isInside, a->rho, counts, list, misc, nzList);
+                           a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
     if (a->evPushEnd) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushEnd, s), "event");
     hipLaunchKernelGGL(k_holes_collect<T>, dim3(R), dim3(CT), 0, s, runs, p, a->nDev, counts, list, nzList, holePos,
                        (T *)a->absorbed, misc);
@@ -774,6 +776,10 @@ static int step_check(const oecl_grid_t *g, const oecl_wall_t *wall, oecl_fastpo
         return OECL_ERR_ARG;
     }
     if (a->dtype != OECL_F64 && a->dtype != OECL_F32) return OECL_ERR_ARG;
+    if (g->nx >= 0xFFFFu || g->ny >= 0xFFFFu) {   // the push's deposit packs cell indices in 16 bits
+        oecl_set_error_msg("step: nxExt and nyExt must be below 65535");
+        return OECL_ERR_ARG;
+    }
     return OECL_OK;
 }
 
