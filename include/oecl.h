@@ -512,6 +512,15 @@ typedef struct {
                                    gathers E and B from it (two 16-byte loads per node, the same
                                    products in the same order).  NULL: E from Eil or E, B from
                                    bField.  No reference counterpart (a layout of the same data). */
+    const uint8_t *e2nKind;     /* [npExt] or NULL: row classes of phiToEAtGridPoints per node,
+                                   bits 0-1 the Ex row, bits 2-3 the Ey row: 0 empty, 1 exactly
+                                   (c phi[node - o], -c phi[node + o]) with c = e2nCx, o = 1 (Ex)
+                                   or c = e2nCy, o = nxExt (Ey), 2 listed in e2nIrr and summed
+                                   from the CSR.  The step then evaluates E without reading the
+                                   CSR of the interior rows (the same products and sums). */
+    const int32_t *e2nIrr;      /* [nIrr] rows (of 2 npExt) of class 2 */
+    uint32_t nIrr;
+    double e2nCx, e2nCy;
 } oecl_step_args;
 
 /* Particles per push block (256): batched run segments are multiples of it, and the per-block

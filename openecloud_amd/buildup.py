@@ -40,7 +40,17 @@ class StepArgs(ctypes.Structure):
                 ('evPushBegin', _VP), ('evPushEnd', _VP), ('photoCount', _VP), ('photoWeight', _VP),
                 ('nPhoto', ctypes.c_uint32), ('photoE0', ctypes.c_double),
                 ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32),
-                ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32), ('EBil', _VP)]
+                ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32), ('EBil', _VP),
+                ('e2nKind', _VP), ('e2nIrr', _VP), ('nIrr', ctypes.c_uint32), ('e2nCx', ctypes.c_double),
+                ('e2nCy', ctypes.c_double)]
+
+
+def set_e2n_classes(a, ps):
+    """Step arguments for the row-class evaluation of E = phiToE phi (PoissonSolver.e2nKind)."""
+    a.e2nKind = ps.e2nKind.data_ptr()
+    a.e2nIrr = ps.e2nIrr.data_ptr() if ps.e2nIrr.numel() else None
+    a.nIrr = int(ps.e2nIrr.numel())
+    a.e2nCx, a.e2nCy = ps.e2nCx, ps.e2nCy
 
 
 def eb_layout(bField, npx, runs=1):
@@ -291,6 +301,7 @@ class Buildup:
         a.e2nRowPtr = ps.rowPtr.data_ptr()
         a.e2nCol = ps.col.data_ptr()
         a.e2nVal = ps.val.data_ptr()
+        set_e2n_classes(a, ps)
         a.bField = self.bField.data_ptr() if self.bField is not None else None
         a.b0, a.b1, a.b2 = [float(v) for v in p.bConst]
         a.typeB = self.typeB

@@ -687,6 +687,55 @@ __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, 
     if (yil) yil[(size_t)blockIdx.y * ilw * xStride + ilw * (r % xStride) + r / xStride] = sum;
 }
 
+// E from phi by row class (e2nKind): the interior pattern (c phi[node - o], -c phi[node + o]),
+// summed as the CSR rows are (0 + first product, then the second); empty rows give 0; listed
+// rows are left to k_e_listed.  One thread per node, blockIdx.y the run; also into the
+// interleaved copy yil (ilw doubles per node) when given.
+__global__ void k_e_struct(uint32_t np, uint32_t nx, const uint8_t *__restrict__ kind, double cx, double cy,
+                           const double *__restrict__ phi, double *__restrict__ E, double *__restrict__ yil, uint32_t ilw)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= np) return;
+    phi += (size_t)blockIdx.y * np;
+    E += (size_t)blockIdx.y * 2 * np;
+    if (yil) yil += (size_t)blockIdx.y * ilw * np + (size_t)ilw * r;
+    const uint32_t k = kind[r], kx = k & 3u, ky = k >> 2;
+    if (kx != 2u) {
+        double s = 0.;
+        if (kx) {
+            s += cx * phi[r - 1];
+            s += (-cx) * phi[r + 1];
+        }
+        E[r] = s;
+        if (yil) yil[0] = s;
+    }
+    if (ky != 2u) {
+        double s = 0.;
+        if (ky) {
+            s += cy * phi[r - nx];
+            s += (-cy) * phi[r + nx];
+        }
+        E[np + r] = s;
+        if (yil) yil[1] = s;
+    }
+}
+
+// The listed rows of phiToE from the CSR (one thread per listed row, blockIdx.y the run).
+__global__ void k_e_listed(uint32_t nList, const int32_t *__restrict__ rows, const int32_t *__restrict__ rowPtr,
+                           const int32_t *__restrict__ col, const double *__restrict__ val,
+                           const double *__restrict__ phi, uint32_t np, double *__restrict__ E,
+                           double *__restrict__ yil, uint32_t ilw)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nList) return;
+    const uint32_t r = (uint32_t)rows[t];
+    phi += (size_t)blockIdx.y * np;
+    double sum = 0.;
+    for (int32_t k = rowPtr[r]; k < rowPtr[r + 1]; ++k) sum += val[k] * phi[col[k]];
+    E[(size_t)blockIdx.y * 2 * np + r] = sum;
+    if (yil) yil[(size_t)blockIdx.y * ilw * np + ilw * (r % np) + r / np] = sum;
+}
+
 }  // namespace
 
 // Queues one step of every run.  R = runs.R; absTotal = rows of the per-hit arrays.
@@ -718,8 +767,18 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     }
     // grid field with Bz = 0: E and B gathered from the (Ex, Ey, Bx, By) layout
     const bool eb = a->typeB == 2 && a->EBil;
-    hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
-                       a->e2nCol, a->e2nVal, a->phi, np, a->E, eb ? a->EBil : a->Eil, eb ? 4u : 2u);
+    double *il = eb ? a->EBil : a->Eil;
+    const uint32_t ilw = eb ? 4u : 2u;
+    if (a->e2nKind) {
+        hipLaunchKernelGGL(k_e_struct, dim3(oecl_blocks(np, 256), R), dim3(256), 0, s, np, g->nx, a->e2nKind,
+                           a->e2nCx, a->e2nCy, a->phi, a->E, il, ilw);
+        if (a->nIrr)
+            hipLaunchKernelGGL(k_e_listed, dim3(oecl_blocks(a->nIrr, 256), R), dim3(256), 0, s, a->nIrr, a->e2nIrr,
+                               a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw);
+    } else {
+        hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
+                           a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw);
+    }
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
     const double dxi = 1. / g->dx, dyi = 1. / g->dy;
     const dim3 gp((unsigned)nBlk), bp(PT);
@@ -768,6 +827,7 @@ static int step_check(const oecl_grid_t *g, const oecl_wall_t *wall, oecl_fastpo
         !a->secOff || !a->impact || !a->stepDev || !a->stats || !a->work || a->absCap == 0)
         return OECL_ERR_ARG;
     if (a->typeB == 2 && !a->bField) return OECL_ERR_ARG;
+    if (a->e2nKind && a->nIrr && !a->e2nIrr) return OECL_ERR_ARG;
     if (a->secem && a->secem->model != OECL_SECEM_STAINLESS && a->secem->model != OECL_SECEM_SIMPLE)
         return OECL_ERR_ARG;
     if (a->photoCount && (!a->photoWeight || !a->photoSlots || a->photoCap == 0)) return OECL_ERR_ARG;

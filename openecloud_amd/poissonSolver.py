@@ -86,6 +86,41 @@ def capacitance_rows(nx, ny, mask, cx, cy, bx=None):
             np.asarray(cols, np.int32), np.asarray(vals, np.float64))
 
 
+def e2n_structure(csr, npx, nx):
+    """Row classes of phiToEAtGridPoints (CSR, rows [Ex | Ey], columns sorted) for the step's
+    E kernel: an interior row is exactly (c phi[node - o], -c phi[node + o]) with o = 1 (Ex) or
+    nx (Ey) and one c per component; other non-empty rows are listed and summed from the CSR.
+
+    Returns (kind [npx] uint8: bits 0-1 the Ex row, bits 2-3 the Ey row, 0 empty, 1 interior
+    pattern, 2 listed; listed rows [int32]; cx; cy)."""
+    ptr, ind, val = csr.indptr, csr.indices, csr.data
+    rows = np.arange(2 * npx)
+    nnz = np.diff(ptr)
+    node = rows % npx
+    off = np.where(rows < npx, 1, nx)
+    two = nnz == 2
+    k0 = ptr[:-1]
+    c0 = np.where(two, ind[np.minimum(k0, ind.size - 1)], -1)
+    c1 = np.where(two, ind[np.minimum(k0 + 1, ind.size - 1)], -1)
+    v0 = np.where(two, val[np.minimum(k0, val.size - 1)], 0.)
+    v1 = np.where(two, val[np.minimum(k0 + 1, val.size - 1)], 0.)
+    shape = two & (c0 == node - off) & (c1 == node + off) & (v0 == -v1) & (v0 != 0.)
+    cs = []
+    for comp in (0, 1):
+        sel = shape & ((rows < npx) if comp == 0 else (rows >= npx))
+        if sel.any():
+            vals, cnt = np.unique(v0[sel], return_counts=True)
+            cs.append(float(vals[np.argmax(cnt)]))
+        else:
+            cs.append(0.)
+    cuni = np.where(rows < npx, cs[0], cs[1])
+    reg = shape & (v0 == cuni)
+    kind = np.where(nnz == 0, 0, np.where(reg, 1, 2)).astype(np.uint8)
+    packed = (kind[:npx] | (kind[npx:] << 2)).astype(np.uint8)
+    listed = np.nonzero(kind == 2)[0].astype(np.int32)
+    return packed, listed, cs[0], cs[1]
+
+
 class PoissonSolver:
 
     def __init__(self, gridObj, tol=1e-12, maxIter=200000, method='direct', checkEvery=1, preconditioner=None):
@@ -118,6 +153,9 @@ class PoissonSolver:
         csr = spsp.csr_matrix(phiToE)
         csr.sort_indices()
         self.phiToE_host = csr
+        self.e2nKind, self.e2nIrr, self.e2nCx, self.e2nCy = e2n_structure(csr, npx, nx)
+        self.e2nKind = torch.as_tensor(self.e2nKind).to(dev)
+        self.e2nIrr = torch.as_tensor(self.e2nIrr).to(dev)
         self.rowPtr = torch.as_tensor(csr.indptr.astype(np.int32)).to(dev)
         self.col = torch.as_tensor(csr.indices.astype(np.int32)).to(dev)
         self.val = torch.as_tensor(csr.data.astype(np.float64)).to(dev)

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .buildup import StepArgs, ST_ABS_OVERFLOW, ST_CAP_OVERFLOW, ST_SEC_OVERFLOW, eb_layout
+from .buildup import StepArgs, ST_ABS_OVERFLOW, ST_CAP_OVERFLOW, ST_SEC_OVERFLOW, eb_layout, set_e2n_classes
 
 PB = 256
 
@@ -226,6 +226,7 @@ class BuildupScan:
         a.e2nRowPtr = ps.rowPtr.data_ptr()
         a.e2nCol = ps.col.data_ptr()
         a.e2nVal = ps.val.data_ptr()
+        set_e2n_classes(a, ps)
         a.bField = self._bField.data_ptr() if self._bField is not None else None
         a.b0, a.b1, a.b2 = [float(v) for v in p.bConst]
         a.typeB = self.typeB

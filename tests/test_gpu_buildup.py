@@ -438,3 +438,52 @@ def test_native_step_grid_b_matches_oracle(orc, layout):
         err = (np.abs(got - ref[it]) / scale).max()
         assert err < 1e-12, (it, err)
     assert st['absorbed_total'] > 0 and st['emitted_total'] > 0
+
+
+@pytest.mark.parametrize('typeB,path', [(0, 'classes'), (2, 'classes'), (2, 'csr')])
+def test_native_step_field_equals_the_csr_product(typeB, path):
+    """E written by the step (row classes of phiToE, or the plain CSR product when the classes are
+    taken away) equals the library's CSR product of the step's phi bit for bit, and the
+    interleaved copy the push reads ((Ex, Ey) or (Ex, Ey, Bx, By) per node) holds the same values
+    (with Bx, By untouched)."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.magneticField import multipoleExpansion
+    from openecloud_amd.particleBoundary import AbsorbElliptical
+    from openecloud_amd.particleEmitter import FurmanEmitter
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = Grid(96, 72, 0.09, 0.07, 1)
+    npx = g.getNpExt()
+    bgrid = np.asarray(multipoleExpansion(g, [0., 50.]), np.float64) if typeB == 2 else None
+    pd0 = random_particles(g, 30000, seed=4, wall='ellip', vscale=3e6)
+    pd0[:, 5] = 5e4
+    randomGen.seed(7)
+    p = Particles('electrons', 'variableWeight', ordering='fast')
+    p.setParticleData(pd0)
+    pb = AbsorbElliptical(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.6)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    assert 0 < int(ps.e2nIrr.numel()) < 2 * npx // 10
+    beam = GaussianBunchTrain(g, 2e-11, nParticles=1e11, tBunchSpacing=25e-9, radiusSigma=2e-3, nBunches=1,
+                              bunchLengthSigma=0.003)
+    b = Buildup(g, p, pb, em, ps, beam, dt=2e-11, typeBField=typeB, bAtGridPoints=bgrid)
+    b.native_setup(absCap=8192, secCap=16384, maxSteps=8)
+    if path == 'csr':
+        b._args.e2nKind = None
+    for _ in range(3):
+        b.step_native()
+    b.sync()
+    E = ps.eAtGridPoints.clone()
+    ref = ps.phiToE().clone()
+    assert torch.equal(E, ref)
+    if typeB == 2:
+        eb = b._ebil.view(npx, 4)
+        assert torch.equal(eb[:, 0], E[:npx]) and torch.equal(eb[:, 1], E[npx:])
+        B = torch.as_tensor(bgrid).cuda()
+        assert torch.equal(eb[:, 2], B[:npx]) and torch.equal(eb[:, 3], B[npx:2 * npx])
+    else:
+        il = b._eil.view(npx, 2)
+        assert torch.equal(il[:, 0], E[:npx]) and torch.equal(il[:, 1], E[npx:])

@@ -104,3 +104,43 @@ def test_capacitance_rows_rebuild_the_operator_on_padded_boxes(args, pad):
         # the last interior column's inside nodes now see a padding column
         last = t[I == nx - 3]
         assert set(last.tolist()) <= set(sIdx.tolist())
+
+
+@pytest.mark.parametrize('args', [(64, 48, 0.09, 0.07, 1), (40, 24, 0.1, 0.05, 0), (70, 70, 0.05, 0.05, 4)])
+def test_e2n_row_classes_reproduce_the_csr_product(args):
+    """The step's row-class evaluation of E = phiToE phi (interior pattern with one coefficient
+    per component, empty rows, listed rows from the CSR), emulated in numpy with the kernel's
+    order of operations, equals scipy's CSR product bit for bit."""
+    import scipy.sparse as spsp
+    from openecloud_amd.poissonSolver import e2n_structure
+    a = list(args)
+    if a[4] == 4:
+        g = Grid(*a, boundFunc=np.array([[0.024, 0.], [0.012, 0.0208], [0., 0.0208]]))
+    else:
+        g = Grid(*a)
+    npx, nx = g.nxExt * g.nyExt, g.nxExt
+    st = spsp.hstack([spsp.identity(npx) - spsp.eye(npx, npx, -1), spsp.identity(npx) - spsp.eye(npx, npx, -nx)])
+    csr = spsp.csr_matrix(g.edgeToNode.dot(spsp.diags(np.asarray(g.dsi, np.float64), 0, (2 * npx, 2 * npx)).dot(st.T)))
+    csr.sort_indices()
+    kind, listed, cx, cy = e2n_structure(csr, npx, nx)
+    k = np.concatenate([kind & 3, kind >> 2])
+    assert set(np.nonzero(k == 2)[0].tolist()) == set(listed.tolist())
+    assert (k == 1).sum() > 0.5 * k.size
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal(npx)
+    ref = csr.dot(phi)
+    E = np.zeros(2 * npx)
+    node = np.arange(npx)
+    for comp, c, o in ((0, cx, 1), (1, cy, nx)):
+        kc = (kind >> (2 * comp)) & 3
+        sel = node[kc == 1]
+        s = np.zeros(sel.size)
+        s = s + c * phi[sel - o]
+        s = s + (-c) * phi[sel + o]
+        E[comp * npx + sel] = s
+    for r in listed:
+        acc = 0.
+        for kk in range(csr.indptr[r], csr.indptr[r + 1]):
+            acc += csr.data[kk] * phi[csr.indices[kk]]
+        E[r] = acc
+    assert np.array_equal(E, ref)
