@@ -506,12 +506,11 @@ typedef struct {
     int32_t externalPhi;        /* 0: the step solves for phi with the direct solver; 1: phi (per
                                    run) already holds the potential of this step's charge
                                    rho + s(step) * profile, solved by the caller (any method) */
-    double *EBil;               /* [4 npExt] (per run), typeB 2 with a grid field whose Bz is zero
-                                   everywhere: (Ex, Ey, Bx, By) per node, Bx and By filled by the
-                                   caller from bField, Ex and Ey written by the step; the push
-                                   gathers E and B from it (two 16-byte loads per node, the same
-                                   products in the same order).  NULL: E from Eil or E, B from
-                                   bField.  No reference counterpart (a layout of the same data). */
+    const double *Bil;          /* [2 npExt] or NULL, typeB 2 with a grid field whose Bz is zero
+                                   everywhere: (Bx, By) per node, shared by all runs; the push then
+                                   gathers B from it (one 16-byte load per node, the same products
+                                   in the same order) and E from Eil, which must be given.  No
+                                   reference counterpart (a layout of the same data). */
     const uint8_t *e2nKind;     /* [npExt] or NULL: row classes of phiToEAtGridPoints per node,
                                    bits 0-1 the Ex row, bits 2-3 the Ey row: 0 empty, 1 exactly
                                    (c phi[node - o], -c phi[node + o]) with c = e2nCx, o = 1 (Ex)

@@ -40,7 +40,7 @@ class StepArgs(ctypes.Structure):
                 ('evPushBegin', _VP), ('evPushEnd', _VP), ('photoCount', _VP), ('photoWeight', _VP),
                 ('nPhoto', ctypes.c_uint32), ('photoE0', ctypes.c_double),
                 ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32),
-                ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32), ('EBil', _VP),
+                ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32), ('Bil', _VP),
                 ('e2nKind', _VP), ('e2nIrr', _VP), ('nIrr', ctypes.c_uint32), ('e2nCx', ctypes.c_double),
                 ('e2nCy', ctypes.c_double)]
 
@@ -53,15 +53,12 @@ def set_e2n_classes(a, ps):
     a.e2nCx, a.e2nCy = ps.e2nCx, ps.e2nCy
 
 
-def eb_layout(bField, npx, runs=1):
-    """The (Ex, Ey, Bx, By)-per-node buffer of a grid field whose Bz is zero everywhere (Bx, By
-    filled, E left for the step), runs copies; None when Bz is not zero."""
+def b_layout(bField, npx):
+    """The (Bx, By)-per-node copy of a grid field whose Bz is zero everywhere (the step's Bil);
+    None when Bz is not zero."""
     if bField is None or bool((bField[2 * npx:3 * npx] != 0).any()):
         return None
-    eb = torch.zeros((runs, npx, 4), dtype=torch.float64, device=bField.device)
-    eb[:, :, 2] = bField[:npx]
-    eb[:, :, 3] = bField[npx:2 * npx]
-    return eb
+    return torch.stack([bField[:npx], bField[npx:2 * npx]], dim=1).contiguous()
 
 
 ST_ABS_OVERFLOW, ST_SEC_OVERFLOW, ST_CAP_OVERFLOW, ST_PHOTO_OVERFLOW = 1, 2, 4, 8
@@ -345,8 +342,8 @@ class Buildup:
             self._prm = self.em.desc
         self._eil = torch.empty(2 * self.g.getNpExt(), dtype=torch.float64, device=dev)
         a.Eil = self._eil.data_ptr()
-        self._ebil = eb_layout(self.bField, self.g.getNpExt()) if self.typeB == 2 else None
-        a.EBil = self._ebil.data_ptr() if self._ebil is not None else None
+        self._bil = b_layout(self.bField, self.g.getNpExt()) if self.typeB == 2 else None
+        a.Bil = self._bil.data_ptr() if self._bil is not None else None
         a.work = self._stepwork.data_ptr()
         self._args = a
         if not self._rho_valid:

@@ -377,8 +377,8 @@ __device__ __forceinline__ void gather_e_il(const Weights<T> &w, uint32_t nx, co
 // Gather E (and B for TYPEB 2) at the particle and apply the Boris push (particles.pyx:
 // 272-333 with _fieldToParticles's accumulation order).  Positions advance in x and y only.
 // With Eil (not null) E is read from the interleaved copy.  TYPEB 3 is TYPEB 2 for a field with
-// Bz = 0, E and B read from Eil in the (Ex, Ey, Bx, By) layout: the same products and sums
-// (the Bz sum of TYPEB 2 is +0 there).
+// Bz = 0: E from Eil, B from the (Bx, By) layout B; the same products and sums (the Bz sum of
+// TYPEB 2 is +0 there).
 template <typename T, int TYPEB>
 __device__ __forceinline__ void gather_push(const Weights<T> &w, uint32_t nx, uint32_t np,
                                             const double *__restrict__ E,
@@ -389,21 +389,13 @@ __device__ __forceinline__ void gather_push(const Weights<T> &w, uint32_t nx, ui
     const uint32_t c = w.cell;
     T ex, ey;
     if (TYPEB == 3) {
-        // Eil is the (Ex, Ey, Bx, By) layout: node c at 4 c; Bz is zero
+        // E from Eil, B from B in the (Bx, By) layout; Bz is zero
         typedef double d2 __attribute__((ext_vector_type(2)));
-        const d2 *P = reinterpret_cast<const d2 *>(Eil);
-        const d2 ea = P[2 * c], eb = P[2 * (c + 1)], ec = P[2 * (c + nx)], ed = P[2 * (c + nx + 1)];
-        const d2 ba = P[2 * c + 1], bb = P[2 * (c + 1) + 1], bc = P[2 * (c + nx) + 1], bd = P[2 * (c + nx + 1) + 1];
-        ex = w.wLL * (T)ea.x;
-        ey = w.wLL * (T)ea.y;
-        ex += w.wUL * (T)eb.x;
-        ey += w.wUL * (T)eb.y;
-        ex += w.wLU * (T)ec.x;
-        ey += w.wLU * (T)ec.y;
-        ex += w.wUU * (T)ed.x;
-        ey += w.wUU * (T)ed.y;
+        gather_e_il<T>(w, nx, Eil, ex, ey);
         vx += k * ex;
         vy += k * ey;
+        const d2 *B2 = reinterpret_cast<const d2 *>(B);
+        const d2 ba = B2[c], bb = B2[c + 1], bc = B2[c + nx], bd = B2[c + nx + 1];
         T bx = w.wLL * (T)ba.x, by = w.wLL * (T)ba.y;
         bx += w.wUL * (T)bb.x;
         by += w.wUL * (T)bb.y;

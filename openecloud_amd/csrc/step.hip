@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(PT)
     const uint64_t first = (uint64_t)blk * PB;
     if (first - sBeg >= n) return;  // whole block past the run's live particles
     E += (size_t)run * 2 * g.np;
-    if (Eil) Eil += (size_t)run * (TYPEB == 3 ? 4 : 2) * g.np;
+    if (Eil) Eil += (size_t)run * 2 * g.np;
     rho += (size_t)run * g.np;
     misc += run * M_COUNT;
     nzList += sBeg / 64;
@@ -690,33 +690,35 @@ __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, 
 // E from phi by row class (e2nKind): the interior pattern (c phi[node - o], -c phi[node + o]),
 // summed as the CSR rows are (0 + first product, then the second); empty rows give 0; listed
 // rows are left to k_e_listed.  One thread per node, blockIdx.y the run; also into the
-// interleaved copy yil (ilw doubles per node) when given.
+// interleaved copy yil ((Ex, Ey) per node) when given.
 __global__ void k_e_struct(uint32_t np, uint32_t nx, const uint8_t *__restrict__ kind, double cx, double cy,
-                           const double *__restrict__ phi, double *__restrict__ E, double *__restrict__ yil, uint32_t ilw)
+                           const double *__restrict__ phi, double *__restrict__ E, double *__restrict__ yil)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= np) return;
     phi += (size_t)blockIdx.y * np;
     E += (size_t)blockIdx.y * 2 * np;
-    if (yil) yil += (size_t)blockIdx.y * ilw * np + (size_t)ilw * r;
+    if (yil) yil += (size_t)blockIdx.y * 2 * np + (size_t)2 * r;
     const uint32_t k = kind[r], kx = k & 3u, ky = k >> 2;
-    if (kx != 2u) {
-        double s = 0.;
-        if (kx) {
-            s += cx * phi[r - 1];
-            s += (-cx) * phi[r + 1];
-        }
-        E[r] = s;
-        if (yil) yil[0] = s;
+    double ex = 0., ey = 0.;
+    if (kx == 1u) {
+        ex += cx * phi[r - 1];
+        ex += (-cx) * phi[r + 1];
     }
-    if (ky != 2u) {
-        double s = 0.;
-        if (ky) {
-            s += cy * phi[r - nx];
-            s += (-cy) * phi[r + nx];
+    if (ky == 1u) {
+        ey += cy * phi[r - nx];
+        ey += (-cy) * phi[r + nx];
+    }
+    if (kx != 2u) E[r] = ex;
+    if (ky != 2u) E[np + r] = ey;
+    if (yil) {
+        if (kx != 2u && ky != 2u) {
+            typedef double d2 __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<d2 *>(yil) = d2{ex, ey};   // one 16-byte store
+        } else {
+            if (kx != 2u) yil[0] = ex;
+            if (ky != 2u) yil[1] = ey;
         }
-        E[np + r] = s;
-        if (yil) yil[1] = s;
     }
 }
 
@@ -765,13 +767,13 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
         rc = fastpoisson_solve_box(fp, a->phi, R, s);
         if (rc) return rc;
     }
-    // grid field with Bz = 0: E and B gathered from the (Ex, Ey, Bx, By) layout
-    const bool eb = a->typeB == 2 && a->EBil;
-    double *il = eb ? a->EBil : a->Eil;
-    const uint32_t ilw = eb ? 4u : 2u;
+    // grid field with Bz = 0: B gathered from the (Bx, By) layout, E from Eil
+    const bool eb = a->typeB == 2 && a->Bil && a->Eil;
+    double *il = a->Eil;
+    const uint32_t ilw = 2u;
     if (a->e2nKind) {
         hipLaunchKernelGGL(k_e_struct, dim3(oecl_blocks(np, 256), R), dim3(256), 0, s, np, g->nx, a->e2nKind,
-                           a->e2nCx, a->e2nCy, a->phi, a->E, il, ilw);
+                           a->e2nCx, a->e2nCy, a->phi, a->E, il);
         if (a->nIrr)
             hipLaunchKernelGGL(k_e_listed, dim3(oecl_blocks(a->nIrr, 256), R), dim3(256), 0, s, a->nIrr, a->e2nIrr,
                                a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw);
@@ -790,7 +792,7 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
         hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, a->b0,
                            a->b1, a->b2, a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
     else if (eb)
-        hipLaunchKernelGGL((k_push_dev<T, 3>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->EBil, a->bField, 0., 0.,
+        hipLaunchKernelGGL((k_push_dev<T, 3>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->Bil, 0., 0.,
                            0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
     else
         hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .buildup import StepArgs, ST_ABS_OVERFLOW, ST_CAP_OVERFLOW, ST_SEC_OVERFLOW, eb_layout, set_e2n_classes
+from .buildup import StepArgs, ST_ABS_OVERFLOW, ST_CAP_OVERFLOW, ST_SEC_OVERFLOW, b_layout, set_e2n_classes
 
 PB = 256
 
@@ -152,7 +152,7 @@ class BuildupScan:
         else:
             self._scale, self._prof, self._scaleHost = None, None, None
         self._bField = b0.bField
-        self._ebil = eb_layout(self._bField, npx, R) if self.typeB == 2 else None
+        self._bil = b_layout(self._bField, npx) if self.typeB == 2 else None
         self._desc = RunsDesc(R, self._segOff.data_ptr(), self._blockRun.data_ptr(), self._absOff.data_ptr(),
                               self._furman.data_ptr(), self._keys.data_ptr())
         self._args = self._make_args(b0)
@@ -249,7 +249,7 @@ class BuildupScan:
         a.histCap = self._nTab
         a.work = self._work.data_ptr()
         a.Eil = self._eil.data_ptr()
-        a.EBil = self._ebil.data_ptr() if self._ebil is not None else None
+        a.Bil = self._bil.data_ptr() if self._bil is not None else None
         return a
 
     # ------------------------------------------------------------------ state

@@ -394,8 +394,8 @@ def test_native_step_grid_b_matches_oracle(orc, layout):
     """B on the grid (quadrupole plus a skew term, typeBField 2, gathered at each particle like
     E): the one-call native step against the oracle loop, FP64 to 1e-12 of each column's
     scale (a swapped or dropped B component, with fields of ~2 T, would differ at order one).
-    Bz is zero, so the step gathers E and B from the interleaved (Ex, Ey, Bx, By) layout
-    ('eb'); 'separate' takes that layout away and gathers from E and the three B arrays."""
+    Bz is zero, so the step gathers B from the (Bx, By) layout and E from its interleaved copy
+    ('eb'); 'separate' takes that layout away and gathers B from the three B arrays."""
     from openecloud_amd import randomGen
     from openecloud_amd.beam import GaussianBunchTrain
     from openecloud_amd.buildup import Buildup
@@ -423,9 +423,9 @@ def test_native_step_grid_b_matches_oracle(orc, layout):
                               nBunches=1, bunchLengthSigma=0.003)
     b = Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=2, bAtGridPoints=bgrid)
     b.native_setup(absCap=8192, secCap=16384, maxSteps=64)
-    assert b._ebil is not None
+    assert b._bil is not None
     if layout == 'separate':
-        b._args.EBil = None
+        b._args.Bil = None
     nsteps = 10
     ref = _oracle_loop(orc, g, pd0.copy(), nsteps, dt, 2, bgrid, beam.getTransversalProfile(),
                        beam.getScale, material('copper', 1.6, eHat0=300.), key, n, rule='fast')
@@ -444,8 +444,8 @@ def test_native_step_grid_b_matches_oracle(orc, layout):
 def test_native_step_field_equals_the_csr_product(typeB, path):
     """E written by the step (row classes of phiToE, or the plain CSR product when the classes are
     taken away) equals the library's CSR product of the step's phi bit for bit, and the
-    interleaved copy the push reads ((Ex, Ey) or (Ex, Ey, Bx, By) per node) holds the same values
-    (with Bx, By untouched)."""
+    interleaved (Ex, Ey) copy the push reads holds the same values (the (Bx, By) copy of the grid
+    field untouched)."""
     from openecloud_amd import randomGen
     from openecloud_amd.beam import GaussianBunchTrain
     from openecloud_amd.buildup import Buildup
@@ -479,11 +479,8 @@ def test_native_step_field_equals_the_csr_product(typeB, path):
     E = ps.eAtGridPoints.clone()
     ref = ps.phiToE().clone()
     assert torch.equal(E, ref)
+    il = b._eil.view(npx, 2)
+    assert torch.equal(il[:, 0], E[:npx]) and torch.equal(il[:, 1], E[npx:])
     if typeB == 2:
-        eb = b._ebil.view(npx, 4)
-        assert torch.equal(eb[:, 0], E[:npx]) and torch.equal(eb[:, 1], E[npx:])
         B = torch.as_tensor(bgrid).cuda()
-        assert torch.equal(eb[:, 2], B[:npx]) and torch.equal(eb[:, 3], B[npx:2 * npx])
-    else:
-        il = b._eil.view(npx, 2)
-        assert torch.equal(il[:, 0], E[:npx]) and torch.equal(il[:, 1], E[npx:])
+        assert torch.equal(b._bil[:, 0], B[:npx]) and torch.equal(b._bil[:, 1], B[npx:2 * npx])

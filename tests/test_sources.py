@@ -53,18 +53,16 @@ def test_beam_line_charge_per_bunch():
     assert b.getScale(25e-9 + 3 * dt) == 0.   # one bunch only
 
 
-def test_eb_layout_interleaves_a_transverse_grid_field():
-    """The step's (Ex, Ey, Bx, By) layout: Bx and By of each node in slots 2 and 3 for every run,
-    E slots left zero; a field with any nonzero Bz keeps the separate arrays (None)."""
+def test_b_layout_interleaves_a_transverse_grid_field():
+    """The step's (Bx, By) layout of a grid field: Bx and By of each node side by side; a field
+    with any nonzero Bz keeps the separate arrays (None)."""
     import torch
-    from openecloud_amd.buildup import eb_layout
+    from openecloud_amd.buildup import b_layout
     npx = 7
     b = torch.arange(3 * npx, dtype=torch.float64) + 1.
-    assert eb_layout(b, npx) is None
+    assert b_layout(b, npx) is None
     b[2 * npx:] = 0.
-    eb = eb_layout(b, npx, runs=3)
-    assert eb.shape == (3, npx, 4)
-    assert torch.equal(eb[:, :, 2], b[:npx].expand(3, npx))
-    assert torch.equal(eb[:, :, 3], b[npx:2 * npx].expand(3, npx))
-    assert not eb[:, :, :2].any()
-    assert eb_layout(None, npx) is None
+    bl = b_layout(b, npx)
+    assert bl.shape == (npx, 2) and bl.is_contiguous()
+    assert torch.equal(bl[:, 0], b[:npx]) and torch.equal(bl[:, 1], b[npx:2 * npx])
+    assert b_layout(None, npx) is None
