@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-WORKLOADS = ('full', 'dipole', 'scan', 'learned', 'northstar')
+WORKLOADS = ('full', 'dipole', 'scan', 'learned', 'northstar', 'push')
 
 
 def profile_traffic(name):
@@ -43,7 +43,8 @@ def profile_traffic(name):
 
 
 TRAFFIC_PROFILES = {('dipole', 'f64', 2000000): 'push_hbm_traffic_ellip512x384_2e6_f64.md',
-                    ('scan', 'f32', 64): 'push_hbm_traffic_scan_64runs_256sq_f32.md'}
+                    ('scan', 'f32', 64): 'push_hbm_traffic_scan_64runs_256sq_f32.md',
+                    ('push', 'f32', 20000000): 'push_hbm_traffic_rect2049sq_2e7_f32_quadrupole.md'}
 
 
 def parse(argv=None):
@@ -79,7 +80,9 @@ def parse(argv=None):
                          'dipole: the 512x384 dipole buildup; scan: 64 buildups stepped together; '
                          'learned: the learned-preconditioner Poisson solve on a 1024^2 cut-cell grid; '
                          'northstar: the 64-run scan with every run solved by the learned flexible CG, against '
-                         'the plain method')
+                         'the plain method; push: the full workload\'s push at its size (2e7 FP32 particles, '
+                         'quadrupole on a 2048^2-cell grid) on a rectangular chamber whose Poisson box needs no '
+                         'capacitance rows, for counter passes on the push alone')
     ap.add_argument('--runs', type=int, default=64, help='scan: number of buildups')
     ap.add_argument('--scan-start', type=int, default=700,
                     help='scan: the timed steps start at this step of the simulation (the first bunch passage, '
@@ -1241,8 +1244,88 @@ def main_northstar(a):
     print(json.dumps(out), flush=True)
 
 
+def main_push(a):
+    """The push of the full workload at its size: 2e7 FP32 macro-electrons (uniform load, 1 eV
+    Maxwellian), the 50 T/m quadrupole on the grid, a 2049 x 2049-cell rectangular 90 x 70 mm
+    chamber (2051 extended nodes: the Poisson box is 2047 = 2^11 - 1 columns wide with no
+    capacitance rows, so the set-up has no dense factorisation), cell-sorted once, then
+    warm-up and timed one-call steps; HIP events around every push kernel.  For counter passes
+    on the push alone (rocprofv3 --kernel-include-regex k_push_dev)."""
+    import torch
+    from openecloud_amd import _lib as L
+    from openecloud_amd import randomGen
+    from openecloud_amd.beam import GaussianBunchTrain
+    from openecloud_amd.buildup import Buildup
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.magneticField import multipoleExpansion
+    from openecloud_amd.particleBoundary import AbsorbRectangular
+    from openecloud_amd.particleEmitter import FurmanEmitter, homoLoader
+    from openecloud_amd.particles import Particles
+    from openecloud_amd.poissonSolver import PoissonSolver
+    L.require_gpu()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    n = int(a.particles)
+    randomGen.seed(0)
+    g = Grid(2049, 2049, 0.09, 0.07, 0)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    log('push: grid %d^2 extended, %d capacitance rows' % (g.getNxExt(), ps.capacitanceRows))
+    bq = multipoleExpansion(g, [0., 50.])
+    p = Particles('electrons', 'variableWeight', dtype=torch.float32, ordering='fast')
+    pb = AbsorbRectangular(g, p)
+    em = FurmanEmitter(pb, p, 'copper', seyMax=1.5)
+    beam = GaussianBunchTrain(g, 2e-12, nParticles=1.15e11, tBunchSpacing=25e-9, radiusSigma=1e-3, nBunches=20)
+    homoLoader(g, p, pb, n, 1e9 / n, 4.194e5)
+    p.reserve(int(1.6 * n) + 4096)
+    b = Buildup(g, p, pb, em, ps, beam, dt=2e-12, typeBField=2, bAtGridPoints=bq)
+    b.native_setup(absCap=max(65536, n // 32), secCap=max(131072, n // 16), maxSteps=a.warmup + a.steps + 4)
+    b.sort_particles()
+    run_plan(b, [('step',)] * a.warmup)
+    b.sync()
+    evs = make_events(a.steps)
+    it0 = b.it
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_plan(b, [('step',)] * a.steps, events=evs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    b.sync()
+    counts = b.counts(it0, it0 + a.steps)
+    push = event_us(evs)
+    avg_us = sum(push) / len(push)
+    npx = g.getNpExt()
+    bytes_per = sum(11 * 4 * c + 48 * npx for c in counts) / len(counts)
+    achieved = bytes_per / (avg_us * 1e-6) / 1e9
+    tb = profile_traffic(TRAFFIC_PROFILES[('push', 'f32', 20000000)]) if n == 20000000 else None
+    out = {
+        'metric': 'push kernel of the full workload: particle-steps/sec',
+        'value': float(sum(counts)) / wall,
+        'unit': 'particle-steps/s',
+        'n_gpus': 1,
+        'steps': a.steps,
+        'warmup': a.warmup,
+        'ms_per_step': wall / a.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (seeded uniform load, Philox draws)',
+        'config': {'workload': '2049x2049-cell rectangular 90x70 mm chamber (no capacitance rows), quadrupole '
+                               '50 T/m on the grid, %d FP32 macro-electrons, dt 2e-12 s, direct Poisson solve' % n,
+                   'grid': [2049, 2049], 'particles': n, 'push_us': push},
+        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': tb,
+                     'traffic_source': ('profiles/' + TRAFFIC_PROFILES[('push', 'f32', 20000000)]) if tb else None,
+                     'avg_launch_us': avg_us, 'bytes_per_launch': bytes_per,
+                     'bytes_model': '44 B per FP32 macro-particle + 48 B per grid node (E, rho, B)'},
+    }
+    log('push: %.1f us per launch, %.3f of 8 TB/s' % (avg_us, achieved / HBM_PEAK_GBS))
+    print(json.dumps(out), flush=True)
+
+
 def main(argv=None):
     a = parse(argv)
+    if a.workload == 'push':
+        return main_push(a)
     if a.workload == 'northstar':
         return main_northstar(a)
     if a.workload == 'learned':
