@@ -346,6 +346,15 @@ def measure_full(a, log):
     q1 = weight_sum(p)
     solves = poisson_rate(ps, b.rho.clone(), 20)
     res = ps.residual(b.rho.clone())
+    # counters are taken on the same push kernel at this size through --workload push (a
+    # rectangular chamber of 2049^2 cells: the same particle count, grid size and field)
+    traffic, traffic_src = None, 'not measured for this workload'
+    if target == 20000000:
+        name = TRAFFIC_PROFILES[('push', 'f32', 20000000)]
+        traffic = profile_traffic(name)
+        if traffic is not None:
+            traffic_src = ('profiles/%s: the same k_push_dev at this size (2e7 FP32, quadrupole on the grid) on the '
+                           'rectangular 2049^2-cell chamber of --workload push' % name)
     out = {
         'value': float(sum(counts)) / wall,
         'ms_per_step': wall / a.steps * 1e3,
@@ -362,8 +371,8 @@ def measure_full(a, log):
         'poisson_solves_per_s': solves,
         'poisson_relres': res,
         'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'traffic_source': 'not measured for this workload', 'avg_launch_us': avg_us,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
+                     'traffic_source': traffic_src, 'avg_launch_us': avg_us,
                      'bytes_per_launch': bytes_per,
                      'bytes_model': '44 B per FP32 macro-particle + 48 B per grid node (E, rho, B)'},
         'management_events': mgmt,
