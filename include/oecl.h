@@ -450,6 +450,19 @@ uint64_t oecl_fcg_work_bytes(uint32_t np, uint32_t mDirs);
 int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy, const uint8_t *mask,
                    const double *q, double *phi, double tol, uint32_t maxIter, uint32_t check, double omega,
                    uint32_t mDirs, void *work, uint32_t *iters, double *relres, void *stream);
+/* The same flexible CG for R problems on one grid in one set of launches (each kernel takes the
+ * run as its second grid dimension): q and phi of run b at b nx ny; iters[R] and relres[R]
+ * (host) as oecl_fcg_solve gives them.  A run stops at the first check point where it has
+ * converged, so each run's phi, iters and relres equal oecl_fcg_solve's on its own problem bit
+ * for bit.  Needs oecl_unet_reserve(u, R') with R' >= R first (R <= 4096), which sizes the
+ * network's activation and reduction buffers for R images.  work:
+ * oecl_fcg_batch_work_bytes(np, mDirs, R) bytes. */
+int oecl_unet_reserve(oecl_unet *u, uint32_t R);
+uint64_t oecl_fcg_batch_work_bytes(uint32_t np, uint32_t mDirs, uint32_t R);
+int oecl_fcg_solve_batch(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                         const uint8_t *mask, uint32_t R, const double *q, double *phi, double tol, uint32_t maxIter,
+                         uint32_t check, double omega, uint32_t mDirs, void *work, uint32_t *iters, double *relres,
+                         void *stream);
 
 /* ---- the whole plain step (example01_basic.py:52-97, example02/04 variants) ---------- */
 

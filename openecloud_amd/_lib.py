@@ -72,11 +72,13 @@ def lib():
         l.oecl_last_error.restype = ctypes.c_char_p
         for name in ('oecl_scan_work_bytes', 'oecl_compact_work_bytes', 'oecl_pcg_work_bytes',
                      'oecl_sort_work_bytes', 'oecl_merge_work_bytes', 'oecl_step_work_bytes',
-                     'oecl_step_batch_work_bytes', 'oecl_fcg_work_bytes', 'oecl_charge_ordered_work_bytes'):
+                     'oecl_step_batch_work_bytes', 'oecl_fcg_work_bytes', 'oecl_charge_ordered_work_bytes',
+                     'oecl_fcg_batch_work_bytes'):
             getattr(l, name).restype = ctypes.c_uint64
         for name in ('oecl_scan_work_bytes', 'oecl_pcg_work_bytes', 'oecl_sort_work_bytes', 'oecl_merge_work_bytes'):
             getattr(l, name).argtypes = [ctypes.c_uint32]
         l.oecl_fcg_work_bytes.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        l.oecl_fcg_batch_work_bytes.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         l.oecl_fastpoisson_box_width.restype = ctypes.c_uint32
         l.oecl_fastpoisson_box_width.argtypes = [ctypes.c_uint32]
         l.oecl_compact_work_bytes.argtypes = [ctypes.c_uint32, ctypes.c_uint32]

@@ -44,6 +44,8 @@ struct oecl_unet {
     char *mem;
     uint32_t adamT;
     int convRows;             // output rows per convolution wave: 0 = chosen per image size
+    uint32_t batch;           // images the activation, grid-vector and reduction buffers hold
+    char *bmem;               // their allocation once batch > 1
 };
 
 namespace {
@@ -86,6 +88,16 @@ struct GridIO {
     const uint8_t *mask;  // inside nodes
     const double *sc;
     uint32_t nx, ny;
+    size_t gs;            // batch: image b's r and z at b gs (grid vectors), its sc at b ss
+    uint32_t ss;
+};
+
+// Strides of a batch of independent problems (image or run b = blockIdx.y): grid vectors at
+// b v, partial sums at b p, scalar blocks at b s; w is the stride between the slots of the
+// flexible CG's direction rings (R v for a batch of R, v for one).
+struct Bat {
+    size_t v, w, p;
+    uint32_t s;
 };
 
 __device__ __forceinline__ double grid_rms(const double *sc) { return sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.; }
@@ -110,6 +122,17 @@ __global__ void __launch_bounds__(kThreads)
            uint16_t *__restrict__ pooled)
 {
     __shared__ __align__(16) uint16_t tile[RW + 2][CONV_SPAN][C];
+    if (blockIdx.y) {  // image blockIdx.y of a batch: its images and grid vectors
+        const size_t b = blockIdx.y, is = (size_t)H * W * C, isc = (size_t)(H >> 1) * (W >> 1) * C;
+        if (in) in += b * is;
+        if (up) up += b * isc;
+        if (pooled) pooled += b * isc;
+        if (out) out = MODE == 2 ? (void *)(static_cast<float *>(out) + b * (size_t)H * W)
+                                 : (void *)(static_cast<uint16_t *>(out) + b * is);
+        if (g.r) g.r += b * g.gs;
+        if (g.z) g.z += b * g.gs;
+        if (g.sc) g.sc += b * g.ss;
+    }
     const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4, wv = threadIdx.x >> 6;
     const int quads = (W + 63) >> 6;
     const int blk = (int)xcd_block(blockIdx.x, gridDim.x);
@@ -337,6 +360,8 @@ __global__ void k_relu_back(size_t n4, uint16_t *__restrict__ g, const uint16_t 
 // out (H x W) = 2x2 average of in (2H x 2W); one thread per 4 channels of an output pixel.
 __global__ void k_pool(int H, int W, const uint16_t *__restrict__ in, uint16_t *__restrict__ out)
 {
+    in += (size_t)blockIdx.y * 4 * H * W * C;   // image blockIdx.y of a batch
+    out += (size_t)blockIdx.y * H * W * C;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)H * W * 4) return;
     const int q = t & 3;
@@ -359,6 +384,9 @@ template <bool BACK>
 __global__ void k_up_add(int H, int W, const uint16_t *__restrict__ u, const uint16_t *__restrict__ s,
                          uint16_t *__restrict__ out)
 {
+    u += (size_t)blockIdx.y * (H >> 1) * (W >> 1) * C;   // image blockIdx.y of a batch
+    s += (size_t)blockIdx.y * H * W * C;
+    out += (size_t)blockIdx.y * H * W * C;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)H * W * 4) return;
     const int q = t & 3;
@@ -447,8 +475,11 @@ __device__ __forceinline__ void block_sum(double (&v)[K])
 
 // Final sum of K partials per block into s[off .. off + K), in a fixed order.
 template <int K>
-__global__ void __launch_bounds__(kThreads) k_finish(const double *__restrict__ part, uint32_t nb, double *__restrict__ s, int off)
+__global__ void __launch_bounds__(kThreads) k_finish(const double *__restrict__ part, uint32_t nb, double *__restrict__ s, int off,
+                                                     Bat bt)
 {
+    part += blockIdx.y * bt.p;
+    s += blockIdx.y * bt.s;
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.;
@@ -466,8 +497,11 @@ __global__ void __launch_bounds__(kThreads) k_finish(const double *__restrict__ 
 constexpr uint32_t kRedBlocks = 1024;
 
 // Partial sums of r^2 and of the inside-node count (for the RMS scale of the network input).
-__global__ void __launch_bounds__(kThreads) k_rms_part(uint32_t np, const uint8_t *__restrict__ mask, const double *__restrict__ r, double *__restrict__ part)
+__global__ void __launch_bounds__(kThreads) k_rms_part(uint32_t np, const uint8_t *__restrict__ mask, const double *__restrict__ r, double *__restrict__ part,
+                                                       Bat bt)
 {
+    r += blockIdx.y * bt.v;
+    part += blockIdx.y * bt.p;
     double v[2] = {0., 0.};
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads)
         if (mask[i]) {
@@ -597,8 +631,12 @@ __global__ void k_resid_az(uint32_t nx, uint32_t np, const double *__restrict__ 
 __global__ void __launch_bounds__(kThreads)
     k_smooth1(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
               const uint8_t *__restrict__ mask, const double *__restrict__ r, double omega, double *__restrict__ z1,
-              double *__restrict__ r1, double *__restrict__ part)
+              double *__restrict__ r1, double *__restrict__ part, Bat bt)
 {
+    r += blockIdx.y * bt.v;
+    z1 += blockIdx.y * bt.v;
+    r1 += blockIdx.y * bt.v;
+    part += blockIdx.y * bt.p;
     double v[2] = {0., 0.};
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         double zi = 0., ri = 0.;
@@ -624,8 +662,12 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     k_an_dots(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
               const uint8_t *__restrict__ mask, const double *__restrict__ n, const double *__restrict__ r1,
-              double *__restrict__ an, double *__restrict__ part)
+              double *__restrict__ an, double *__restrict__ part, Bat bt)
 {
+    n += blockIdx.y * bt.v;
+    r1 += blockIdx.y * bt.v;
+    an += blockIdx.y * bt.v;
+    part += blockIdx.y * bt.p;
     double v[2] = {0., 0.};
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         double a = 0.;
@@ -650,10 +692,17 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void k_combine(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
                           const uint8_t *__restrict__ mask, const double *__restrict__ z1, const double *__restrict__ n,
                           const double *__restrict__ r1, const double *__restrict__ an, const double *__restrict__ sc,
-                          double omega, double *__restrict__ z)
+                          double omega, double *__restrict__ z, Bat bt)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
+    const size_t o = blockIdx.y * bt.v;
+    z1 += o;
+    n += o;
+    r1 += o;
+    an += o;
+    z += o;
+    sc += blockIdx.y * bt.s;
     const double c = sc[9] > 0. ? sc[8] / sc[9] : 0.;
     z[i] = mask[i] ? z1[i] + c * n[i] + omega * (r1[i] - c * an[i]) / diag_a(cx, cy, i, nx) : 0.;
 }
@@ -665,14 +714,20 @@ __global__ void k_combine(uint32_t nx, uint32_t np, const double *__restrict__ c
 // (beta = -z.Ap / p.Ap).  Keeping more directions restores what a preconditioner that varies
 // from one application to the next (a ReLU network and its line search) takes away from the
 // CG recurrence.
-// scalars: s[2] q.q, s[3] alpha, s[6] p.Ap, s[7] p.r, s[8] r.r (after the update), s[9] q.q (init)
+// scalars per run: s[3] alpha, s[6] p.Ap, s[7] p.r, s[8] r.r (after the update), s[9] q.q (init;
+// then the update's node count), s[12] stopped, s[13] its iteration, s[14] q.q, s[16..23] z.Ap_j,
+// s[24..32] p.Ap per ring slot
 constexpr uint32_t FCG_MAXM = 8;
 
 __global__ void __launch_bounds__(kThreads)
     k_fcg_init(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
                const uint8_t *__restrict__ mask, const double *__restrict__ q, double *__restrict__ phi,
-               double *__restrict__ r, double *__restrict__ part)
+               double *__restrict__ r, double *__restrict__ part, Bat bt)
 {
+    q += blockIdx.y * bt.v;
+    phi += blockIdx.y * bt.v;
+    r += blockIdx.y * bt.v;
+    part += blockIdx.y * bt.p;
     double v[2] = {0., 0.};
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         double ri = 0.;
@@ -690,8 +745,9 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
-__global__ void k_mask_zero(uint32_t np, const uint8_t *__restrict__ mask, double *__restrict__ x)
+__global__ void k_mask_zero(uint32_t np, const uint8_t *__restrict__ mask, double *__restrict__ x, Bat bt)
 {
+    x += blockIdx.y * bt.v;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < np && !mask[i]) x[i] = 0.;
 }
@@ -699,8 +755,11 @@ __global__ void k_mask_zero(uint32_t np, const uint8_t *__restrict__ mask, doubl
 // Partial z.Ap_j for the window's directions j = k - 1 - t, t < nUsed (FCG_MAXM per block).
 __global__ void __launch_bounds__(kThreads)
     k_fcg_zdots(uint32_t np, const double *__restrict__ z, const double *__restrict__ AP, uint32_t k, uint32_t m1,
-                uint32_t nUsed, double *__restrict__ part)
+                uint32_t nUsed, double *__restrict__ part, Bat bt)
 {
+    z += blockIdx.y * bt.v;
+    AP += blockIdx.y * bt.v;
+    part += blockIdx.y * bt.p;
     double v[FCG_MAXM];
 #pragma unroll
     for (uint32_t t = 0; t < FCG_MAXM; ++t) v[t] = 0.;
@@ -708,7 +767,7 @@ __global__ void __launch_bounds__(kThreads)
         const double zi = z[i];
 #pragma unroll
         for (uint32_t t = 0; t < FCG_MAXM; ++t)
-            if (t < nUsed) v[t] += zi * AP[(size_t)((k - 1 - t) % m1) * np + i];
+            if (t < nUsed) v[t] += zi * AP[(size_t)((k - 1 - t) % m1) * bt.w + i];
     }
     block_sum<FCG_MAXM>(v);
     if (threadIdx.x == 0)
@@ -720,25 +779,33 @@ __global__ void __launch_bounds__(kThreads)
 // direction with p.Ap <= 0 is skipped).
 __global__ void k_fcg_newdir(uint32_t np, const double *__restrict__ z, double *__restrict__ P,
                              const double *__restrict__ d, const double *__restrict__ pAp, uint32_t k, uint32_t m1,
-                             uint32_t nUsed)
+                             uint32_t nUsed, Bat bt)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
+    z += blockIdx.y * bt.v;
+    P += blockIdx.y * bt.v;
+    d += blockIdx.y * bt.s;
+    pAp += blockIdx.y * bt.s;
     double v = z[i];
     for (uint32_t t = 0; t < nUsed; ++t) {
         const uint32_t sl = (k - 1 - t) % m1;
         const double den = pAp[sl];
-        if (den > 0.) v -= (d[t] / den) * P[(size_t)sl * np + i];
+        if (den > 0.) v -= (d[t] / den) * P[(size_t)sl * bt.w + i];
     }
-    P[(size_t)(k % m1) * np + i] = v;
+    P[(size_t)(k % m1) * bt.w + i] = v;
 }
 
 // Ap; partial (p.Ap, p.r)
 __global__ void __launch_bounds__(kThreads)
     k_fcg_ap(uint32_t nx, uint32_t np, const double *__restrict__ cx, const double *__restrict__ cy,
              const uint8_t *__restrict__ mask, const double *__restrict__ p, const double *__restrict__ r,
-             double *__restrict__ ap, double *__restrict__ part)
+             double *__restrict__ ap, double *__restrict__ part, Bat bt)
 {
+    p += blockIdx.y * bt.v;
+    r += blockIdx.y * bt.v;
+    ap += blockIdx.y * bt.v;
+    part += blockIdx.y * bt.p;
     double v[2] = {0., 0.};
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
         double a = 0.;
@@ -756,24 +823,70 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
-// alpha = p.r / p.Ap (0 when p.Ap is not positive); p.Ap kept for the direction's slot.
-__global__ void k_alpha(double *s, double *pAp, uint32_t slot)
+// alpha = p.r / p.Ap (0 when p.Ap is not positive); p.Ap kept for the direction's slot.  One
+// thread per run; a run that has stopped (s[12] != 0) takes alpha = 0, so its phi and r stay as
+// they are, bit for bit.
+__global__ void k_alpha(double *s, double *pAp, uint32_t slot, uint32_t R, Bat bt)
 {
-    s[3] = s[6] > 0. ? s[7] / s[6] : 0.;
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= R) return;
+    s += b * bt.s;
+    pAp += b * bt.s;
+    s[3] = s[12] != 0. ? 0. : (s[6] > 0. ? s[7] / s[6] : 0.);
     pAp[slot] = s[6];
 }
 
-// phi += alpha p, r -= alpha Ap; partial r.r
+// The convergence check every `check` iterations, per run (one thread each): a run stops (s[12]
+// = 1, s[13] = it) once sqrt(r.r) <= tol |q| (|q|^2 in s[14], taken as 1 when q = 0) or r.r is
+// not finite
+// -- the test the single-problem loop applies at the same iterations.
+__global__ void k_fcg_check(double *s, uint32_t R, Bat bt, double tol, uint32_t it)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= R) return;
+    s += b * bt.s;
+    if (s[12] != 0.) return;
+    const double rr = s[8], qn = sqrt(s[14]) > 0. ? sqrt(s[14]) : 1.;
+    if (!(sqrt(rr) > tol * qn) || !(rr == rr)) {
+        s[12] = 1.;
+        s[13] = (double)it;
+    }
+}
+
+// Clears the stop flags (s[12], s[13]) of R runs and keeps q.q (s[9] after k_fcg_init's
+// reduction; later reductions reuse s[9]) in s[14].
+__global__ void k_fcg_start(double *s, uint32_t R, Bat bt)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= R) return;
+    s += b * bt.s;
+    s[12] = 0.;
+    s[13] = 0.;
+    s[14] = s[9];
+}
+
+// phi += alpha p, r -= alpha Ap; partial r.r.  A stopped run keeps phi and r and recomputes r.r
+// from the stored r in the same order, so its s[8] is unchanged bit for bit.
 __global__ void __launch_bounds__(kThreads)
     k_fcg_update(uint32_t np, const double *__restrict__ s, double *__restrict__ phi, double *__restrict__ r,
-                 const double *__restrict__ p, const double *__restrict__ ap, double *__restrict__ part)
+                 const double *__restrict__ p, const double *__restrict__ ap, double *__restrict__ part, Bat bt)
 {
+    s += blockIdx.y * bt.s;
+    phi += blockIdx.y * bt.v;
+    r += blockIdx.y * bt.v;
+    p += blockIdx.y * bt.v;
+    ap += blockIdx.y * bt.v;
+    part += blockIdx.y * bt.p;
     double v[2] = {0., 0.};
     const double a = s[3];
+    const bool live = s[12] == 0.;
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
-        const double ri = r[i] - a * ap[i];
-        phi[i] += a * p[i];
-        r[i] = ri;
+        double ri = r[i];
+        if (live) {
+            ri -= a * ap[i];
+            phi[i] += a * p[i];
+            r[i] = ri;
+        }
         v[0] += ri * ri;
         v[1] += 1.;
     }
@@ -785,6 +898,13 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 uint32_t nblk(size_t n, int b = kThreads) { return (uint32_t)((n + b - 1) / b); }
+
+// Per-image strides of the preconditioner's own buffers (sv: grid vectors np apart, part:
+// kPartImg partial sums, scal: kScalImg scalars) for a batch of problems on one nx x ny grid.
+constexpr size_t kPartImg = 4 * (size_t)kRedBlocks + 64;
+constexpr uint32_t kScalImg = 64;
+Bat unet_bat(uint32_t np) { return Bat{np, np, kPartImg, kScalImg}; }
+
 
 // Activation images per level l (H_l = H >> l): x0 (input), a (first conv), s (second conv /
 // skip), p (pooled input of the level), t (up + skip input of an up level), u (up level's
@@ -819,11 +939,12 @@ void launch_conv_mode(int rw, dim3 g, hipStream_t s, int H, int W, const uint16_
 // pooled (modes 0, 1): also write the 2x2 average of the output there when the tile has two or
 // more rows; returns 1 when it did (the caller pools otherwise), 0 when not, < 0 on error.
 int launch_conv(const oecl_unet *u, int l, int mode, const uint16_t *in, const uint16_t *w, const float *b, void *out,
-                hipStream_t s, const GridIO &io = GridIO{}, const uint16_t *up = nullptr, uint16_t *pooled = nullptr)
+                hipStream_t s, const GridIO &io = GridIO{}, const uint16_t *up = nullptr, uint16_t *pooled = nullptr,
+                uint32_t nimg = 1)
 {
     const int H = u->H >> l, W = u->W >> l;
     const int rw = conv_rows(u, H, W);
-    const dim3 g((uint32_t)((H + rw - 1) / rw) * ((W + 63) / 64));  // one block per RW x 64 tile
+    const dim3 g((uint32_t)((H + rw - 1) / rw) * ((W + 63) / 64), nimg);  // one block per RW x 64 tile and image
     uint16_t *pl = (mode == 0 || mode == 1) && rw >= 2 ? pooled : nullptr;
     if (mode == 0) launch_conv_mode<0>(rw, g, s, H, W, in, w, b, out, io, up, pl);
     else if (mode == 1) launch_conv_mode<1>(rw, g, s, H, W, in, w, b, out, io, up, pl);
@@ -842,47 +963,52 @@ const float *bc(const oecl_unet *u, int c) { return u->bm + (size_t)c * C; }
 // io != null: a pass from the grid vector io->r to io->z (no FP32 output image).  Without tape
 // it stages the input and the up levels' sums inside the convolutions (inference); with tape
 // (training) it writes the input image and those sums, which the backward pass reads.
-void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr, bool tape = false)
+void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr, bool tape = false, uint32_t nimg = 1)
 {
     uint16_t **A = u->act;
     auto pool = [&](int l, const uint16_t *in, uint16_t *out) {  // level l-1 -> l
         const int H = u->H >> l, W = u->W >> l;
-        hipLaunchKernelGGL(k_pool, dim3(nblk((size_t)H * W * 4)), dim3(kThreads), 0, s, H, W, in, out);
+        hipLaunchKernelGGL(k_pool, dim3(nblk((size_t)H * W * 4), nimg), dim3(kThreads), 0, s, H, W, in, out);
     };
     auto upadd = [&](int l, const uint16_t *uu, const uint16_t *sk, uint16_t *out) {  // level l+1 -> l
         const int H = u->H >> l, W = u->W >> l;
-        hipLaunchKernelGGL(k_up_add<false>, dim3(nblk((size_t)H * W * 4)), dim3(kThreads), 0, s, H, W, uu, sk, out);
+        hipLaunchKernelGGL(k_up_add<false>, dim3(nblk((size_t)H * W * 4), nimg), dim3(kThreads), 0, s, H, W, uu, sk,
+                           out);
+    };
+    auto conv = [&](int l, int mode, const uint16_t *in, int c, void *out, const GridIO &g = GridIO{},
+                    const uint16_t *up = nullptr, uint16_t *pooled = nullptr) {
+        return launch_conv(u, l, mode, in, Wc(u, c), bc(u, c), out, s, g, up, pooled, nimg);
     };
     const bool fused = io && !tape;
     if (fused) {
-        launch_conv(u, 0, 4, nullptr, Wc(u, 0), bc(u, 0), A[IA], s, *io);
+        conv(0, 4, nullptr, 0, A[IA], *io);
     } else {
         if (io)
             hipLaunchKernelGGL(k_net_input, dim3(nblk((size_t)u->H * u->W * 4)), dim3(kThreads), 0, s, io->nx, io->ny,
                                (int)u->H, (int)u->W, io->mask, io->r, io->sc, A[IX]);
-        launch_conv(u, 0, 0, A[IX], Wc(u, 0), bc(u, 0), A[IA], s);
+        conv(0, 0, A[IX], 0, A[IA]);
     }
     // each level's second convolution also pools its output for the next level when it can
-    int pooledNext = launch_conv(u, 0, 0, A[IA], Wc(u, 1), bc(u, 1), A[IS], s, GridIO{}, nullptr, A[7 + IP]);
+    int pooledNext = conv(0, 0, A[IA], 1, A[IS], GridIO{}, nullptr, A[7 + IP]);
     for (int l = 1; l <= 3; ++l) {
         if (!pooledNext) pool(l, A[7 * (l - 1) + IS], A[7 * l + IP]);
-        launch_conv(u, l, 0, A[7 * l + IP], Wc(u, 2 * l), bc(u, 2 * l), A[7 * l + IA], s);
-        pooledNext = launch_conv(u, l, 0, A[7 * l + IA], Wc(u, 2 * l + 1), bc(u, 2 * l + 1), A[7 * l + IS], s,
-                                 GridIO{}, nullptr, l < 3 ? A[7 * (l + 1) + IP] : nullptr);
+        conv(l, 0, A[7 * l + IP], 2 * l, A[7 * l + IA]);
+        pooledNext = conv(l, 0, A[7 * l + IA], 2 * l + 1, A[7 * l + IS], GridIO{}, nullptr,
+                          l < 3 ? A[7 * (l + 1) + IP] : nullptr);
     }
     // level 3's output is u3 = A[7*3 + IS]
     for (int l = 2; l >= 0; --l) {
         const uint16_t *below = l == 2 ? A[7 * 3 + IS] : A[7 * (l + 1) + IV];
         const int c0 = 8 + 2 * (2 - l);
         if (fused) {  // inference: the up level's sum is staged by its first convolution
-            launch_conv(u, l, 5, A[7 * l + IS], Wc(u, c0), bc(u, c0), A[7 * l + IU], s, *io, below);
+            conv(l, 5, A[7 * l + IS], c0, A[7 * l + IU], *io, below);
         } else {
             upadd(l, below, A[7 * l + IS], A[7 * l + IT]);
-            launch_conv(u, l, 0, A[7 * l + IT], Wc(u, c0), bc(u, c0), A[7 * l + IU], s);
+            conv(l, 0, A[7 * l + IT], c0, A[7 * l + IU]);
         }
-        if (l > 0) launch_conv(u, l, 0, A[7 * l + IU], Wc(u, c0 + 1), bc(u, c0 + 1), A[7 * l + IV], s);
-        else if (io) launch_conv(u, 0, 3, A[IU], Wc(u, 13), bc(u, 13), nullptr, s, *io);
-        else launch_conv(u, 0, 2, A[IU], Wc(u, 13), bc(u, 13), u->out, s);
+        if (l > 0) conv(l, 0, A[7 * l + IU], c0 + 1, A[7 * l + IV]);
+        else if (io) conv(0, 3, A[IU], 13, nullptr, *io);
+        else conv(0, 2, A[IU], 13, u->out);
     }
 }
 
@@ -972,6 +1098,7 @@ OECL_EXPORT int oecl_unet_create(uint32_t H, uint32_t W, uint32_t seed, oecl_une
     oecl_unet *u = new oecl_unet();
     u->H = H;
     u->W = W;
+    u->batch = 1;
     const size_t nW = (size_t)NCONV * WSZ, nb = (size_t)(NCONV + 1) * C;  // + one zero bias row
     // carve the buffers out of one allocation: a first pass sizes it, the second assigns
     char *base = nullptr;
@@ -1002,8 +1129,8 @@ OECL_EXPORT int oecl_unet_create(uint32_t H, uint32_t W, uint32_t seed, oecl_une
         }
         u->out = (float *)take(4ull * H * W);
         for (int k = 0; k < 4; ++k) u->sv[k] = (double *)take(8ull * H * W);
-        u->scal = (double *)take(8 * 64);
-        u->part = (double *)take(8ull * 4 * (nblk((size_t)H * W) + 16));
+        u->scal = (double *)take(8 * kScalImg);
+        u->part = (double *)take(8ull * std::max(4 * ((size_t)nblk((size_t)H * W) + 16), kPartImg));
         if (pass == 0) {
             if (hipMalloc(&base, used) != hipSuccess) {
                 delete u;
@@ -1040,7 +1167,49 @@ OECL_EXPORT int oecl_unet_destroy(oecl_unet *u)
 {
     if (!u) return OECL_OK;
     (void)hipFree(u->mem);
+    if (u->bmem) (void)hipFree(u->bmem);
     delete u;
+    return OECL_OK;
+}
+
+// Room for batches of up to R inference images (oecl_fcg_solve_batch): the activation images,
+// the smoothed preconditioner's grid vectors and the reduction buffers are reallocated R times
+// over (contents are not kept; the weights and the training state are untouched).
+OECL_EXPORT int oecl_unet_reserve(oecl_unet *u, uint32_t R)
+{
+    if (!u || R < 1 || R > 4096) return OECL_ERR_ARG;
+    if (R <= u->batch) return OECL_OK;
+    const size_t hw = (size_t)u->H * u->W;
+    char *base = nullptr;
+    size_t used = 0;
+    auto take = [&](size_t b) {
+        char *r = base ? base + used : nullptr;
+        used += (b + 255) / 256 * 256;
+        return r;
+    };
+    uint16_t *act[32];
+    double *sv[4], *scal = nullptr, *part = nullptr;
+    for (int pass = 0; pass < 2; ++pass) {
+        used = 0;
+        for (int l = 0; l < 4; ++l)
+            for (int k = 0; k < 7; ++k) act[7 * l + k] = (uint16_t *)take(2 * img_elems(u, l) * R);
+        for (int k = 0; k < 4; ++k) sv[k] = (double *)take(8 * hw * R);
+        scal = (double *)take(8ull * kScalImg * R);
+        part = (double *)take(8 * (std::max(4 * ((size_t)nblk(hw) + 16), kPartImg) + kPartImg * (R - 1)));
+        if (pass == 0 && hipMalloc(&base, used) != hipSuccess) {
+            oecl_set_error_msg("unet batch alloc failed");
+            return OECL_ERR_HIP;
+        }
+    }
+    OECL_HIP(hipMemset(base, 0, used), "unet batch clear");
+    OECL_HIP(hipDeviceSynchronize(), "unet batch");
+    if (u->bmem) (void)hipFree(u->bmem);
+    u->bmem = base;
+    for (int k = 0; k < 28; ++k) u->act[k] = act[k];
+    for (int k = 0; k < 4; ++k) u->sv[k] = sv[k];
+    u->scal = scal;
+    u->part = part;
+    u->batch = R;
     return OECL_OK;
 }
 
@@ -1120,14 +1289,17 @@ OECL_EXPORT int oecl_unet_adam(oecl_unet *u, float lr, float beta1, float beta2,
 }
 
 // z = rms(r) * Net(r / rms(r)) on the inside nodes of an nx x ny grid (row-major, mask = inside).
+// z = rms(r) * Net(r / rms(r)) on the inside nodes of an nx x ny grid (row-major, mask = inside);
+// nimg problems at once (r and z np apart, np = nx ny).
 static int precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r, double *z,
-                        hipStream_t s, bool tape = false)
+                        hipStream_t s, bool tape = false, uint32_t nimg = 1)
 {
     const uint32_t np = nx * ny, nbr = std::min(nblk(np), kRedBlocks);
-    hipLaunchKernelGGL(k_rms_part, dim3(nbr), dim3(kThreads), 0, s, np, mask, r, u->part);
-    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0);
-    const GridIO io{r, z, mask, u->scal, nx, ny};
-    unet_forward(u, s, &io, tape);
+    const Bat bt = unet_bat(np);
+    hipLaunchKernelGGL(k_rms_part, dim3(nbr, nimg), dim3(kThreads), 0, s, np, mask, r, u->part, bt);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1, nimg), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0, bt);
+    const GridIO io{r, z, mask, u->scal, nx, ny, np, kScalImg};
+    unet_forward(u, s, &io, tape, nimg);
     return OECL_OK;
 }
 
@@ -1135,18 +1307,23 @@ static int precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *m
 // n for r1 = r - A z1, scaled by the A-norm line search c = n.r1 / n.A n; z = z1 + c n +
 // omega D^-1 (r1 - c A n).  omega = 0: the network alone.
 static int precondition_smoothed(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
-                                 const uint8_t *mask, const double *r, double *z, double omega, hipStream_t s)
+                                 const uint8_t *mask, const double *r, double *z, double omega, hipStream_t s,
+                                 uint32_t nimg = 1)
 {
-    if (!(omega > 0.)) return precondition(u, nx, ny, mask, r, z, s);
+    if (nimg > u->batch) return OECL_ERR_ARG;
+    if (!(omega > 0.)) return precondition(u, nx, ny, mask, r, z, s, false, nimg);
     const uint32_t np = nx * ny, nb = nblk(np), nbr = std::min(nb, kRedBlocks);
+    const Bat bt = unet_bat(np);
     double *z1 = u->sv[0], *r1 = u->sv[1], *n = u->sv[2], *an = u->sv[3];
-    hipLaunchKernelGGL(k_smooth1, dim3(nbr), dim3(kThreads), 0, s, nx, np, cx, cy, mask, r, omega, z1, r1, u->part);
-    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0);
-    const GridIO io{r1, n, mask, u->scal, nx, ny};
-    unet_forward(u, s, &io);
-    hipLaunchKernelGGL(k_an_dots, dim3(nbr), dim3(kThreads), 0, s, nx, np, cx, cy, mask, n, r1, an, u->part);
-    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, s, u->part, nbr, u->scal, 8);
-    hipLaunchKernelGGL(k_combine, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, z1, n, r1, an, u->scal, omega, z);
+    hipLaunchKernelGGL(k_smooth1, dim3(nbr, nimg), dim3(kThreads), 0, s, nx, np, cx, cy, mask, r, omega, z1, r1, u->part,
+                       bt);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1, nimg), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0, bt);
+    const GridIO io{r1, n, mask, u->scal, nx, ny, np, kScalImg};
+    unet_forward(u, s, &io, false, nimg);
+    hipLaunchKernelGGL(k_an_dots, dim3(nbr, nimg), dim3(kThreads), 0, s, nx, np, cx, cy, mask, n, r1, an, u->part, bt);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1, nimg), dim3(kThreads), 0, s, u->part, nbr, u->scal, 8, bt);
+    hipLaunchKernelGGL(k_combine, dim3(nb, nimg), dim3(kThreads), 0, s, nx, np, cx, cy, mask, z1, n, r1, an, u->scal,
+                       omega, z, bt);
     return OECL_OK;
 }
 
@@ -1189,7 +1366,7 @@ OECL_EXPORT int oecl_unet_train_sample(oecl_unet *u, uint32_t nx, uint32_t ny, c
     int rc = precondition(u, nx, ny, mask, q, z, s, true);  // z, input RMS in scal[0..1], the tape
     if (rc) return rc;
     hipLaunchKernelGGL(k_loss_az, dim3(nb), dim3(kThreads), 0, s, nx, np, cx, cy, mask, z, phi, q, az, u->part);
-    hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(kThreads), 0, s, u->part, nb, u->scal, 4);
+    hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(kThreads), 0, s, u->part, nb, u->scal, 4, unet_bat(np));
     // the output gradient goes into Q_0, which the backward first reads and only later reuses
     hipLaunchKernelGGL(k_loss_grad, dim3(nblk((size_t)u->H * u->W * 4)), dim3(kThreads), 0, s, nx, ny, (int)u->H,
                        (int)u->W, mask, az, q, u->scal, u->grad[1]);
@@ -1214,6 +1391,56 @@ OECL_EXPORT uint64_t oecl_fcg_work_bytes(uint32_t np, uint32_t mDirs)
     return 8ull * ((2 + 2 * m1) * (uint64_t)np + FCG_MAXM * (nblk(np) + 1) + 2 * FCG_MAXM + 64) + 256;
 }
 
+namespace {
+
+// One block of `check` flexible-CG iterations for R problems on one grid (R = 1: a single
+// solve).  Vectors of run b at b np; the direction rings' slots R np apart; the scalar block of
+// run b at S + b kScalImg (D = S + 16: z.Ap_j, pAp = S + 24: p.Ap per slot).
+int fcg_iterations(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy, const uint8_t *mask,
+                   uint32_t R, double *phi, double omega, uint32_t mDirs, double *r, double *z, double *P, double *AP,
+                   double *part, size_t pst, double *S, uint32_t &it, uint32_t check, uint32_t maxIter,
+                   hipStream_t st)
+{
+    const uint32_t np = nx * ny, nbr = std::min(nblk(np), kRedBlocks), m1 = mDirs + 1;
+    const Bat bt{np, (size_t)R * np, pst, kScalImg};
+    double *D = S + 16, *pAp = S + 24;
+    const dim3 gr(nbr, R), g1(1, R);
+    for (uint32_t k = 0; k < check && it < maxIter; ++k, ++it) {
+        int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, st, R);
+        if (rc) return rc;
+        const uint32_t nUsed = it < mDirs ? it : mDirs;
+        if (nUsed) {
+            hipLaunchKernelGGL(k_fcg_zdots, gr, dim3(kThreads), 0, st, np, z, AP, it, m1, nUsed, part, bt);
+            hipLaunchKernelGGL(k_finish<FCG_MAXM>, g1, dim3(kThreads), 0, st, part, nbr, D, 0, bt);
+        }
+        hipLaunchKernelGGL(k_fcg_newdir, dim3(nblk(np, 256), R), dim3(256), 0, st, np, z, P, D, pAp, it, m1, nUsed,
+                           bt);
+        double *p = P + (size_t)(it % m1) * bt.w, *ap = AP + (size_t)(it % m1) * bt.w;
+        hipLaunchKernelGGL(k_fcg_ap, gr, dim3(kThreads), 0, st, nx, np, cx, cy, mask, p, r, ap, part, bt);
+        hipLaunchKernelGGL(k_finish<2>, g1, dim3(kThreads), 0, st, part, nbr, S, 6, bt);
+        hipLaunchKernelGGL(k_alpha, dim3((R + 63) / 64), dim3(64), 0, st, S, pAp, it % m1, R, bt);
+        hipLaunchKernelGGL(k_fcg_update, gr, dim3(kThreads), 0, st, np, S, phi, r, p, ap, part, bt);
+        hipLaunchKernelGGL(k_finish<2>, g1, dim3(kThreads), 0, st, part, nbr, S, 8, bt);
+    }
+    return OECL_OK;
+}
+
+// r = q - A phi (phi zeroed off the inside nodes), s[8] = r.r, s[9] = q.q, stop flags cleared.
+void fcg_begin(uint32_t nx, uint32_t ny, const double *cx, const double *cy, const uint8_t *mask, uint32_t R,
+               const double *q, double *phi, double *r, double *part, size_t pst, double *S, hipStream_t st)
+{
+    const uint32_t np = nx * ny, nb = nblk(np), nbr = std::min(nb, kRedBlocks);
+    const Bat bt{np, (size_t)R * np, pst, kScalImg};
+    hipLaunchKernelGGL(k_mask_zero, dim3(nb, R), dim3(kThreads), 0, st, np, mask, phi, bt);
+    hipLaunchKernelGGL(k_fcg_init, dim3(nbr, R), dim3(kThreads), 0, st, nx, np, cx, cy, mask, q, phi, r, part, bt);
+    hipLaunchKernelGGL(k_finish<2>, dim3(1, R), dim3(kThreads), 0, st, part, nbr, S, 8, bt);
+    hipLaunchKernelGGL(k_fcg_start, dim3((R + 63) / 64), dim3(64), 0, st, S, R, bt);
+}
+
+size_t fcg_part_stride(uint32_t np) { return FCG_MAXM * ((size_t)std::min(nblk(np), kRedBlocks) + 1); }
+
+}  // namespace
+
 // Flexible CG on the FIT system with the network as preconditioner, keeping mDirs directions
 // (1 .. 8).  phi: initial guess in, solution out (zero off the inside nodes).  Converged when
 // ||q - A phi|| <= tol ||q||; the residual norm is read on the host every `check` iterations.
@@ -1227,12 +1454,11 @@ OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const dou
         return OECL_ERR_ARG;
     if (check == 0) check = 1;
     hipStream_t st = (hipStream_t)stream;
-    const uint32_t np = nx * ny, nb = nblk(np), nbr = std::min(nb, kRedBlocks), m1 = mDirs + 1;
+    const uint32_t np = nx * ny, m1 = mDirs + 1;
+    const size_t pst = fcg_part_stride(np);
     double *r = (double *)work, *z = r + np, *P = z + np, *AP = P + (size_t)m1 * np;
-    double *part = AP + (size_t)m1 * np, *S = part + FCG_MAXM * (nb + 1), *D = S + 16, *pAp = D + FCG_MAXM;
-    hipLaunchKernelGGL(k_mask_zero, dim3(nb), dim3(kThreads), 0, st, np, mask, phi);
-    hipLaunchKernelGGL(k_fcg_init, dim3(nbr), dim3(kThreads), 0, st, nx, np, cx, cy, mask, q, phi, r, part);
-    hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nbr, S, 8);  // S[8] r.r, S[9] q.q
+    double *part = AP + (size_t)m1 * np, *S = part + pst;
+    fcg_begin(nx, ny, cx, cy, mask, 1, q, phi, r, part, pst, S, st);
     double h[2];
     OECL_HIP(hipMemcpyAsync(h, S + 8, 16, hipMemcpyDeviceToHost, st), "fcg copy");
     OECL_HIP(hipStreamSynchronize(st), "fcg sync");
@@ -1240,22 +1466,9 @@ OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const dou
     double rr = h[0];
     uint32_t it = 0;
     while (sqrt(rr) > tol * qn && it < maxIter) {
-        for (uint32_t k = 0; k < check && it < maxIter; ++k, ++it) {
-            int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, st);
-            if (rc) return rc;
-            const uint32_t nUsed = it < mDirs ? it : mDirs;
-            if (nUsed) {
-                hipLaunchKernelGGL(k_fcg_zdots, dim3(nbr), dim3(kThreads), 0, st, np, z, AP, it, m1, nUsed, part);
-                hipLaunchKernelGGL(k_finish<FCG_MAXM>, dim3(1), dim3(kThreads), 0, st, part, nbr, D, 0);
-            }
-            hipLaunchKernelGGL(k_fcg_newdir, dim3(nblk(np, 256)), dim3(256), 0, st, np, z, P, D, pAp, it, m1, nUsed);
-            double *p = P + (size_t)(it % m1) * np, *ap = AP + (size_t)(it % m1) * np;
-            hipLaunchKernelGGL(k_fcg_ap, dim3(nbr), dim3(kThreads), 0, st, nx, np, cx, cy, mask, p, r, ap, part);
-            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nbr, S, 6);
-            hipLaunchKernelGGL(k_alpha, dim3(1), dim3(1), 0, st, S, pAp, it % m1);
-            hipLaunchKernelGGL(k_fcg_update, dim3(nbr), dim3(kThreads), 0, st, np, S, phi, r, p, ap, part);
-            hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(kThreads), 0, st, part, nbr, S, 8);
-        }
+        int rc = fcg_iterations(u, nx, ny, cx, cy, mask, 1, phi, omega, mDirs, r, z, P, AP, part, pst, S, it, check,
+                                maxIter, st);
+        if (rc) return rc;
         OECL_CHECK_LAUNCH("oecl_fcg_solve");
         OECL_HIP(hipMemcpyAsync(h, S + 8, 8, hipMemcpyDeviceToHost, st), "fcg copy");
         OECL_HIP(hipStreamSynchronize(st), "fcg sync");
@@ -1266,4 +1479,55 @@ OECL_EXPORT int oecl_fcg_solve(oecl_unet *u, uint32_t nx, uint32_t ny, const dou
     const double rel = sqrt(rr) / qn;
     if (relres) *relres = rel;
     return rel <= tol ? OECL_OK : OECL_ERR_NOCONV;
+}
+
+OECL_EXPORT uint64_t oecl_fcg_batch_work_bytes(uint32_t np, uint32_t mDirs, uint32_t R)
+{
+    const uint64_t m1 = (mDirs < 1 ? 1 : mDirs > FCG_MAXM ? FCG_MAXM : mDirs) + 1;
+    return 8ull * R * ((2 + 2 * m1) * (uint64_t)np + fcg_part_stride(np) + kScalImg) + 256;
+}
+
+// R flexible CGs on one grid in the same launches (oecl_unet_reserve(u, R) first): q and phi of
+// run b at b np; iters[b], relres[b] as oecl_fcg_solve gives them.  Each run stops at the first
+// check point where it has converged (its phi and r are then left as they are), so every run's
+// result is bit for bit that of oecl_fcg_solve on its own problem.
+OECL_EXPORT int oecl_fcg_solve_batch(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
+                                     const uint8_t *mask, uint32_t R, const double *q, double *phi, double tol,
+                                     uint32_t maxIter, uint32_t check, double omega, uint32_t mDirs, void *work,
+                                     uint32_t *iters, double *relres, void *stream)
+{
+    if (!u || !cx || !cy || !mask || !q || !phi || !work || ny > u->H || nx > u->W || nx < 3 || ny < 3 ||
+        mDirs < 1 || mDirs > FCG_MAXM || R < 1 || R > u->batch)
+        return OECL_ERR_ARG;
+    if (check == 0) check = 1;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t np = nx * ny, m1 = mDirs + 1;
+    const size_t pst = fcg_part_stride(np), Rn = (size_t)R * np;
+    double *r = (double *)work, *z = r + Rn, *P = z + Rn, *AP = P + m1 * Rn;
+    double *part = AP + m1 * Rn, *S = part + pst * R;
+    const Bat bt{np, Rn, pst, kScalImg};
+    fcg_begin(nx, ny, cx, cy, mask, R, q, phi, r, part, pst, S, st);
+    std::vector<double> h((size_t)R * kScalImg);
+    uint32_t it = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_fcg_check, dim3((R + 63) / 64), dim3(64), 0, st, S, R, bt, tol, it);
+        OECL_CHECK_LAUNCH("oecl_fcg_solve_batch");
+        OECL_HIP(hipMemcpyAsync(h.data(), S, 8 * h.size(), hipMemcpyDeviceToHost, st), "fcg copy");
+        OECL_HIP(hipStreamSynchronize(st), "fcg sync");
+        bool live = false;
+        for (uint32_t b = 0; b < R; ++b) live |= h[(size_t)b * kScalImg + 12] == 0.;
+        if (!live || it >= maxIter) break;
+        int rc = fcg_iterations(u, nx, ny, cx, cy, mask, R, phi, omega, mDirs, r, z, P, AP, part, pst, S, it, check,
+                                maxIter, st);
+        if (rc) return rc;
+    }
+    bool ok = true;
+    for (uint32_t b = 0; b < R; ++b) {
+        const double *sb = h.data() + (size_t)b * kScalImg;
+        const double qn = sqrt(sb[14]) > 0. ? sqrt(sb[14]) : 1., rel = sqrt(sb[8]) / qn;
+        if (iters) iters[b] = sb[12] != 0. ? (uint32_t)sb[13] : it;
+        if (relres) relres[b] = rel;
+        ok &= rel <= tol;
+    }
+    return ok ? OECL_OK : OECL_ERR_NOCONV;
 }

@@ -250,6 +250,34 @@ class UNetPreconditioner:
         self.lastIterations, self.lastRelres = it.value, rr.value
         return phi, it.value, rr.value, rc == 0
 
+    def reserve(self, R):
+        """Sizes the network's inference buffers for batches of R problems (fcg_batch)."""
+        L.check(L.lib().oecl_unet_reserve(self._h, L.u32(R)), 'unet reserve')
+
+    def fcg_batch(self, ps, q, phi, R, tol=1e-8, maxIter=2000, check=4, omega=None, mDirs=None):
+        """fcg for R problems at once: q and phi hold R grid vectors back to back (phi: initial
+        guesses in, solutions out).  Each run's result is bit for bit fcg's on its own problem.
+        -> (iterations[R], relative residuals[R]) as numpy arrays; a run converged when its
+        relative residual is <= tol."""
+        npx = self.nx * self.ny
+        if q.numel() < R * npx or phi.numel() < R * npx:
+            raise L.OeclError('fcg_batch: q and phi need %d x %d values' % (R, npx))
+        om = self.omega if omega is None else float(omega)
+        m = self.mDirs if mDirs is None else int(mDirs)
+        lib = L.lib()
+        self.reserve(R)
+        need = int(lib.oecl_fcg_batch_work_bytes(npx, max(1, min(m, 8)), R)) // 8 + 8
+        if self._fcgwork is None or self._fcgwork.numel() < need:
+            self._fcgwork = torch.empty(need, dtype=torch.float64, device=L.device())
+        its = (ctypes.c_uint32 * R)()
+        rel = (ctypes.c_double * R)()
+        rc = lib.oecl_fcg_solve_batch(self._h, L.u32(self.nx), L.u32(self.ny), L.ptr(ps.cx), L.ptr(ps.cy),
+                                      L.ptr(ps.mask), L.u32(R), L.ptr(q), L.ptr(phi), L.f64(tol), L.u32(maxIter),
+                                      L.u32(check), L.f64(om), L.u32(m), L.ptr(self._fcgwork), its, rel, L.stream())
+        if rc not in (0, 4):
+            L.check(rc, 'fcg batch solve')
+        return np.frombuffer(its, dtype=np.uint32).copy(), np.frombuffer(rel, dtype=np.float64).copy()
+
     def solve(self, ps, q, tol=1e-8, maxIter=2000):
         """phi = A^-1 q by the learned flexible CG; the direct solve when it does not converge."""
         phi, it, rr, ok = self.fcg(ps, q, tol=tol, maxIter=maxIter)

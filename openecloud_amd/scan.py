@@ -186,16 +186,20 @@ class BuildupScan:
         for r in range(self.R):
             rho_r = self.rho[r * npx:(r + 1) * npx]
             q_r = self.q[r * npx:(r + 1) * npx]
-            phi_r = self.phi[r * npx:(r + 1) * npx]
             sc = float(self._scaleHost[r, self.it]) if self._scaleHost is not None else 0.
             if sc != 0.:
                 L.check(L.lib().oecl_axpy_rhs(L.u32(npx), L.ptr(rho_r), L.f64(sc), L.ptr(self._prof), L.ptr(q_r),
                                               L.stream()), 'scan rhs')
             else:
                 q_r.copy_(rho_r)
-            _, it, rr, ok = net.fcg(ps, q_r, phi=phi_r, tol=self._ltol, maxIter=self._lmax)
+        # every run's flexible CG in one set of launches (each run stops at its own check point)
+        its, rels = net.fcg_batch(ps, self.q, self.phi, self.R, tol=self._ltol, maxIter=self._lmax)
+        for r in range(self.R):
+            q_r = self.q[r * npx:(r + 1) * npx]
+            phi_r = self.phi[r * npx:(r + 1) * npx]
+            it, rr = int(its[r]), float(rels[r])
             st['iterations'].append(it)
-            if not ok:
+            if not rr <= self._ltol:
                 st['fallbacks'] += 1
                 ps._direct(q_r, phi_r)
             else:

@@ -261,3 +261,47 @@ def test_train_sample_gradients_match_the_image_path():
         assert np.linalg.norm(gW1[k]) > 0 and np.linalg.norm(gb1[k]) > 0, k
         assert _rel(gW1[k].astype(np.float64), gW2[k].astype(np.float64)) < 1e-2, (k, _rel(gW1[k], gW2[k]))
         assert _rel(gb1[k].astype(np.float64), gb2[k].astype(np.float64)) < 1e-2, (k, _rel(gb1[k], gb2[k]))
+
+
+@pytest.mark.parametrize('omega,mDirs', [(2. / 3., 8), (0., 1), (2. / 3., 3)])
+def test_batched_flexible_cg_equals_each_run_alone(omega, mDirs):
+    """oecl_fcg_solve_batch advances R flexible CGs in the same launches; each run stops at its
+    own check point.  Every run's solution, iteration count and relative residual must equal
+    oecl_fcg_solve's on its own problem bit for bit: runs that converge early (a zero right-hand
+    side at iteration 0, a scaled copy), runs that reach maxIter, and non-zero initial guesses.
+    The training path must give the same loss before and after the buffers grow for the batch."""
+    from openecloud_amd.learnedPreconditioner import UNetPreconditioner
+    from openecloud_amd.poissonSolver import PoissonSolver
+    g = _chamber(45)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    net = UNetPreconditioner(g, seed=4)
+    npx = g.getNxExt() * g.getNyExt()
+    pairs = _pairs(g, ps, 4, seed=9)
+    loss0 = net.loss(ps, *pairs[0])
+    rng = np.random.default_rng(1)
+    qs = [pairs[0][0], torch.zeros_like(pairs[0][0]), pairs[1][0], 1e3 * pairs[0][0], pairs[2][0], pairs[3][0]]
+    noise = lambda: 1. + 1e-4 * torch.as_tensor(rng.standard_normal(npx)).cuda()
+    z = torch.zeros_like(qs[0])
+    # near-solutions as guesses for runs 2 and 4 (run 2 converges early)
+    guesses = [z, z, pairs[1][1] * noise(), z, 0.5 * pairs[2][1] * noise(), z]
+    R, tol, maxIter, check = len(qs), 1e-8, 48, 4
+    alone = []
+    for q, p0 in zip(qs, guesses):
+        phi, it, rr, ok = net.fcg(ps, q, phi=p0.clone(), tol=tol, maxIter=maxIter, check=check, omega=omega,
+                                  mDirs=mDirs)
+        alone.append((phi.cpu().numpy(), it, rr))
+    Q = torch.cat(qs)
+    PHI = torch.cat([p.clone() for p in guesses])
+    its, rels = net.fcg_batch(ps, Q, PHI, R, tol=tol, maxIter=maxIter, check=check, omega=omega, mDirs=mDirs)
+    got = PHI.cpu().numpy().reshape(R, npx)
+    print('batched flexible CG: iterations', list(its), 'alone', [a[1] for a in alone])
+    assert any(a[1] == maxIter for a in alone) and any(a[1] < maxIter for a in alone)
+    assert alone[1][1] == 0
+    for b in range(R):
+        assert np.array_equal(got[b], alone[b][0]), b
+        assert its[b] == alone[b][1] and rels[b] == alone[b][2], (b, its[b], alone[b][1], rels[b], alone[b][2])
+    # a second, smaller batch reuses the grown buffers
+    its2, rels2 = net.fcg_batch(ps, Q[:2 * npx].clone(), torch.cat([p.clone() for p in guesses[:2]]), 2, tol=tol,
+                                maxIter=maxIter, check=check, omega=omega, mDirs=mDirs)
+    assert list(its2) == list(its[:2]) and list(rels2) == list(rels[:2])
+    assert net.loss(ps, *pairs[0]) == loss0
