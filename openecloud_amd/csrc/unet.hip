@@ -90,6 +90,7 @@ struct GridIO {
     uint32_t nx, ny;
     size_t gs;            // batch: image b's r and z at b gs (grid vectors), its sc at b ss
     uint32_t ss;
+    const double *st;     // optional: image b is skipped when st[b ss] != 0 (a stopped run)
 };
 
 // Strides of a batch of independent problems (image or run b = blockIdx.y): grid vectors at
@@ -98,7 +99,11 @@ struct GridIO {
 struct Bat {
     size_t v, w, p;
     uint32_t s;
+    const double *st;  // optional: run b's kernels do nothing when st[b s] != 0 (it has stopped)
 };
+
+// Whether this block's run has stopped (uniform over the block: it depends on blockIdx.y only).
+__device__ __forceinline__ bool stopped(const Bat &bt) { return bt.st && bt.st[blockIdx.y * bt.s] != 0.; }
 
 __device__ __forceinline__ double grid_rms(const double *sc) { return sc[1] > 0. && sc[0] > 0. ? sqrt(sc[0] / sc[1]) : 1.; }
 
@@ -122,6 +127,7 @@ __global__ void __launch_bounds__(kThreads)
            uint16_t *__restrict__ pooled)
 {
     __shared__ __align__(16) uint16_t tile[RW + 2][CONV_SPAN][C];
+    if (g.st && g.st[(size_t)blockIdx.y * g.ss] != 0.) return;  // a stopped run: whole block
     if (blockIdx.y) {  // image blockIdx.y of a batch: its images and grid vectors
         const size_t b = blockIdx.y, is = (size_t)H * W * C, isc = (size_t)(H >> 1) * (W >> 1) * C;
         if (in) in += b * is;
@@ -478,6 +484,7 @@ template <int K>
 __global__ void __launch_bounds__(kThreads) k_finish(const double *__restrict__ part, uint32_t nb, double *__restrict__ s, int off,
                                                      Bat bt)
 {
+    if (stopped(bt)) return;
     part += blockIdx.y * bt.p;
     s += blockIdx.y * bt.s;
     double v[K];
@@ -500,6 +507,7 @@ constexpr uint32_t kRedBlocks = 1024;
 __global__ void __launch_bounds__(kThreads) k_rms_part(uint32_t np, const uint8_t *__restrict__ mask, const double *__restrict__ r, double *__restrict__ part,
                                                        Bat bt)
 {
+    if (stopped(bt)) return;
     r += blockIdx.y * bt.v;
     part += blockIdx.y * bt.p;
     double v[2] = {0., 0.};
@@ -633,6 +641,7 @@ __global__ void __launch_bounds__(kThreads)
               const uint8_t *__restrict__ mask, const double *__restrict__ r, double omega, double *__restrict__ z1,
               double *__restrict__ r1, double *__restrict__ part, Bat bt)
 {
+    if (stopped(bt)) return;
     r += blockIdx.y * bt.v;
     z1 += blockIdx.y * bt.v;
     r1 += blockIdx.y * bt.v;
@@ -664,6 +673,7 @@ __global__ void __launch_bounds__(kThreads)
               const uint8_t *__restrict__ mask, const double *__restrict__ n, const double *__restrict__ r1,
               double *__restrict__ an, double *__restrict__ part, Bat bt)
 {
+    if (stopped(bt)) return;
     n += blockIdx.y * bt.v;
     r1 += blockIdx.y * bt.v;
     an += blockIdx.y * bt.v;
@@ -694,6 +704,7 @@ __global__ void k_combine(uint32_t nx, uint32_t np, const double *__restrict__ c
                           const double *__restrict__ r1, const double *__restrict__ an, const double *__restrict__ sc,
                           double omega, double *__restrict__ z, Bat bt)
 {
+    if (stopped(bt)) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const size_t o = blockIdx.y * bt.v;
@@ -757,6 +768,7 @@ __global__ void __launch_bounds__(kThreads)
     k_fcg_zdots(uint32_t np, const double *__restrict__ z, const double *__restrict__ AP, uint32_t k, uint32_t m1,
                 uint32_t nUsed, double *__restrict__ part, Bat bt)
 {
+    if (stopped(bt)) return;
     z += blockIdx.y * bt.v;
     AP += blockIdx.y * bt.v;
     part += blockIdx.y * bt.p;
@@ -781,6 +793,7 @@ __global__ void k_fcg_newdir(uint32_t np, const double *__restrict__ z, double *
                              const double *__restrict__ d, const double *__restrict__ pAp, uint32_t k, uint32_t m1,
                              uint32_t nUsed, Bat bt)
 {
+    if (stopped(bt)) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     z += blockIdx.y * bt.v;
@@ -802,6 +815,7 @@ __global__ void __launch_bounds__(kThreads)
              const uint8_t *__restrict__ mask, const double *__restrict__ p, const double *__restrict__ r,
              double *__restrict__ ap, double *__restrict__ part, Bat bt)
 {
+    if (stopped(bt)) return;
     p += blockIdx.y * bt.v;
     r += blockIdx.y * bt.v;
     ap += blockIdx.y * bt.v;
@@ -824,15 +838,16 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // alpha = p.r / p.Ap (0 when p.Ap is not positive); p.Ap kept for the direction's slot.  One
-// thread per run; a run that has stopped (s[12] != 0) takes alpha = 0, so its phi and r stay as
-// they are, bit for bit.
+// thread per run; a run that has stopped (s[12] != 0) is left as it is (its other kernels skip
+// it too, so its phi, r and s[8] keep their values bit for bit).
 __global__ void k_alpha(double *s, double *pAp, uint32_t slot, uint32_t R, Bat bt)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= R) return;
     s += b * bt.s;
     pAp += b * bt.s;
-    s[3] = s[12] != 0. ? 0. : (s[6] > 0. ? s[7] / s[6] : 0.);
+    if (s[12] != 0.) return;
+    s[3] = s[6] > 0. ? s[7] / s[6] : 0.;
     pAp[slot] = s[6];
 }
 
@@ -865,12 +880,12 @@ __global__ void k_fcg_start(double *s, uint32_t R, Bat bt)
     s[14] = s[9];
 }
 
-// phi += alpha p, r -= alpha Ap; partial r.r.  A stopped run keeps phi and r and recomputes r.r
-// from the stored r in the same order, so its s[8] is unchanged bit for bit.
+// phi += alpha p, r -= alpha Ap; partial r.r.
 __global__ void __launch_bounds__(kThreads)
     k_fcg_update(uint32_t np, const double *__restrict__ s, double *__restrict__ phi, double *__restrict__ r,
                  const double *__restrict__ p, const double *__restrict__ ap, double *__restrict__ part, Bat bt)
 {
+    if (stopped(bt)) return;
     s += blockIdx.y * bt.s;
     phi += blockIdx.y * bt.v;
     r += blockIdx.y * bt.v;
@@ -879,14 +894,10 @@ __global__ void __launch_bounds__(kThreads)
     part += blockIdx.y * bt.p;
     double v[2] = {0., 0.};
     const double a = s[3];
-    const bool live = s[12] == 0.;
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < np; i += gridDim.x * kThreads) {
-        double ri = r[i];
-        if (live) {
-            ri -= a * ap[i];
-            phi[i] += a * p[i];
-            r[i] = ri;
-        }
+        const double ri = r[i] - a * ap[i];
+        phi[i] += a * p[i];
+        r[i] = ri;
         v[0] += ri * ri;
         v[1] += 1.;
     }
@@ -913,13 +924,14 @@ enum { IX = 0, IA = 1, IS = 2, IP = 3, IT = 4, IU = 5, IV = 6 };
 
 size_t img_elems(const oecl_unet *u, int l) { return (size_t)(u->H >> l) * (u->W >> l) * C; }
 
-// Rows per convolution block: the tallest tile that still leaves >= 1024 blocks (4 per CU).
-int conv_rows(const oecl_unet *u, int H, int W)
+// Rows per convolution block: the tallest tile that still leaves >= 1024 blocks (4 per CU) over
+// the nimg images of a launch.
+int conv_rows(const oecl_unet *u, int H, int W, uint32_t nimg = 1)
 {
     if (u->convRows) return u->convRows;
     const long quads = (W + 63) / 64;
     for (int rw = 8; rw > 1; rw >>= 1)
-        if ((H + rw - 1) / rw * quads >= 1024) return rw;
+        if ((H + rw - 1) / rw * quads * (long)nimg >= 1024) return rw;
     return 1;
 }
 
@@ -943,7 +955,7 @@ int launch_conv(const oecl_unet *u, int l, int mode, const uint16_t *in, const u
                 uint32_t nimg = 1)
 {
     const int H = u->H >> l, W = u->W >> l;
-    const int rw = conv_rows(u, H, W);
+    const int rw = conv_rows(u, H, W, nimg);
     const dim3 g((uint32_t)((H + rw - 1) / rw) * ((W + 63) / 64), nimg);  // one block per RW x 64 tile and image
     uint16_t *pl = (mode == 0 || mode == 1) && rw >= 2 ? pooled : nullptr;
     if (mode == 0) launch_conv_mode<0>(rw, g, s, H, W, in, w, b, out, io, up, pl);
@@ -975,7 +987,12 @@ void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr, bool 
         hipLaunchKernelGGL(k_up_add<false>, dim3(nblk((size_t)H * W * 4), nimg), dim3(kThreads), 0, s, H, W, uu, sk,
                            out);
     };
-    auto conv = [&](int l, int mode, const uint16_t *in, int c, void *out, const GridIO &g = GridIO{},
+    GridIO none{};  // the convolutions without grid input or output: only the stop flags
+    if (io) {
+        none.st = io->st;
+        none.ss = io->ss;
+    }
+    auto conv = [&](int l, int mode, const uint16_t *in, int c, void *out, const GridIO &g,
                     const uint16_t *up = nullptr, uint16_t *pooled = nullptr) {
         return launch_conv(u, l, mode, in, Wc(u, c), bc(u, c), out, s, g, up, pooled, nimg);
     };
@@ -986,14 +1003,14 @@ void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr, bool 
         if (io)
             hipLaunchKernelGGL(k_net_input, dim3(nblk((size_t)u->H * u->W * 4)), dim3(kThreads), 0, s, io->nx, io->ny,
                                (int)u->H, (int)u->W, io->mask, io->r, io->sc, A[IX]);
-        conv(0, 0, A[IX], 0, A[IA]);
+        conv(0, 0, A[IX], 0, A[IA], none);
     }
     // each level's second convolution also pools its output for the next level when it can
-    int pooledNext = conv(0, 0, A[IA], 1, A[IS], GridIO{}, nullptr, A[7 + IP]);
+    int pooledNext = conv(0, 0, A[IA], 1, A[IS], none, nullptr, A[7 + IP]);
     for (int l = 1; l <= 3; ++l) {
         if (!pooledNext) pool(l, A[7 * (l - 1) + IS], A[7 * l + IP]);
-        conv(l, 0, A[7 * l + IP], 2 * l, A[7 * l + IA]);
-        pooledNext = conv(l, 0, A[7 * l + IA], 2 * l + 1, A[7 * l + IS], GridIO{}, nullptr,
+        conv(l, 0, A[7 * l + IP], 2 * l, A[7 * l + IA], none);
+        pooledNext = conv(l, 0, A[7 * l + IA], 2 * l + 1, A[7 * l + IS], none, nullptr,
                           l < 3 ? A[7 * (l + 1) + IP] : nullptr);
     }
     // level 3's output is u3 = A[7*3 + IS]
@@ -1004,11 +1021,11 @@ void unet_forward(oecl_unet *u, hipStream_t s, const GridIO *io = nullptr, bool 
             conv(l, 5, A[7 * l + IS], c0, A[7 * l + IU], *io, below);
         } else {
             upadd(l, below, A[7 * l + IS], A[7 * l + IT]);
-            conv(l, 0, A[7 * l + IT], c0, A[7 * l + IU]);
+            conv(l, 0, A[7 * l + IT], c0, A[7 * l + IU], none);
         }
-        if (l > 0) conv(l, 0, A[7 * l + IU], c0 + 1, A[7 * l + IV]);
+        if (l > 0) conv(l, 0, A[7 * l + IU], c0 + 1, A[7 * l + IV], none);
         else if (io) conv(0, 3, A[IU], 13, nullptr, *io);
-        else conv(0, 2, A[IU], 13, u->out);
+        else conv(0, 2, A[IU], 13, u->out, none);
     }
 }
 
@@ -1292,13 +1309,14 @@ OECL_EXPORT int oecl_unet_adam(oecl_unet *u, float lr, float beta1, float beta2,
 // z = rms(r) * Net(r / rms(r)) on the inside nodes of an nx x ny grid (row-major, mask = inside);
 // nimg problems at once (r and z np apart, np = nx ny).
 static int precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *mask, const double *r, double *z,
-                        hipStream_t s, bool tape = false, uint32_t nimg = 1)
+                        hipStream_t s, bool tape = false, uint32_t nimg = 1, const double *st = nullptr)
 {
     const uint32_t np = nx * ny, nbr = std::min(nblk(np), kRedBlocks);
-    const Bat bt = unet_bat(np);
+    Bat bt = unet_bat(np);
+    bt.st = st;
     hipLaunchKernelGGL(k_rms_part, dim3(nbr, nimg), dim3(kThreads), 0, s, np, mask, r, u->part, bt);
     hipLaunchKernelGGL(k_finish<2>, dim3(1, nimg), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0, bt);
-    const GridIO io{r, z, mask, u->scal, nx, ny, np, kScalImg};
+    const GridIO io{r, z, mask, u->scal, nx, ny, np, kScalImg, st};
     unet_forward(u, s, &io, tape, nimg);
     return OECL_OK;
 }
@@ -1308,17 +1326,18 @@ static int precondition(oecl_unet *u, uint32_t nx, uint32_t ny, const uint8_t *m
 // omega D^-1 (r1 - c A n).  omega = 0: the network alone.
 static int precondition_smoothed(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, const double *cy,
                                  const uint8_t *mask, const double *r, double *z, double omega, hipStream_t s,
-                                 uint32_t nimg = 1)
+                                 uint32_t nimg = 1, const double *st = nullptr)
 {
     if (nimg > u->batch) return OECL_ERR_ARG;
-    if (!(omega > 0.)) return precondition(u, nx, ny, mask, r, z, s, false, nimg);
+    if (!(omega > 0.)) return precondition(u, nx, ny, mask, r, z, s, false, nimg, st);
     const uint32_t np = nx * ny, nb = nblk(np), nbr = std::min(nb, kRedBlocks);
-    const Bat bt = unet_bat(np);
+    Bat bt = unet_bat(np);
+    bt.st = st;
     double *z1 = u->sv[0], *r1 = u->sv[1], *n = u->sv[2], *an = u->sv[3];
     hipLaunchKernelGGL(k_smooth1, dim3(nbr, nimg), dim3(kThreads), 0, s, nx, np, cx, cy, mask, r, omega, z1, r1, u->part,
                        bt);
     hipLaunchKernelGGL(k_finish<2>, dim3(1, nimg), dim3(kThreads), 0, s, u->part, nbr, u->scal, 0, bt);
-    const GridIO io{r1, n, mask, u->scal, nx, ny, np, kScalImg};
+    const GridIO io{r1, n, mask, u->scal, nx, ny, np, kScalImg, st};
     unet_forward(u, s, &io, false, nimg);
     hipLaunchKernelGGL(k_an_dots, dim3(nbr, nimg), dim3(kThreads), 0, s, nx, np, cx, cy, mask, n, r1, an, u->part, bt);
     hipLaunchKernelGGL(k_finish<2>, dim3(1, nimg), dim3(kThreads), 0, s, u->part, nbr, u->scal, 8, bt);
@@ -1402,11 +1421,11 @@ int fcg_iterations(oecl_unet *u, uint32_t nx, uint32_t ny, const double *cx, con
                    hipStream_t st)
 {
     const uint32_t np = nx * ny, nbr = std::min(nblk(np), kRedBlocks), m1 = mDirs + 1;
-    const Bat bt{np, (size_t)R * np, pst, kScalImg};
+    const Bat bt{np, (size_t)R * np, pst, kScalImg, S + 12};  // stopped runs skip every kernel
     double *D = S + 16, *pAp = S + 24;
     const dim3 gr(nbr, R), g1(1, R);
     for (uint32_t k = 0; k < check && it < maxIter; ++k, ++it) {
-        int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, st, R);
+        int rc = precondition_smoothed(u, nx, ny, cx, cy, mask, r, z, omega, st, R, S + 12);
         if (rc) return rc;
         const uint32_t nUsed = it < mDirs ? it : mDirs;
         if (nUsed) {

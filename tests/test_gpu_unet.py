@@ -304,4 +304,10 @@ def test_batched_flexible_cg_equals_each_run_alone(omega, mDirs):
     its2, rels2 = net.fcg_batch(ps, Q[:2 * npx].clone(), torch.cat([p.clone() for p in guesses[:2]]), 2, tol=tol,
                                 maxIter=maxIter, check=check, omega=omega, mDirs=mDirs)
     assert list(its2) == list(its[:2]) and list(rels2) == list(rels[:2])
+    # taller convolution tiles (with the pooling fused into them) give the same bits
+    net.set_conv_rows(8)
+    PHI8 = torch.cat([p.clone() for p in guesses])
+    its8, rels8 = net.fcg_batch(ps, Q, PHI8, R, tol=tol, maxIter=maxIter, check=check, omega=omega, mDirs=mDirs)
+    assert torch.equal(PHI8, PHI) and list(its8) == list(its) and list(rels8) == list(rels)
+    net.set_conv_rows(0)
     assert net.loss(ps, *pairs[0]) == loss0
