@@ -69,12 +69,15 @@ def parse(argv=None):
                     help='re-order particles by cell every this many steps (0: never)')
     ap.add_argument('--target', type=float, default=None,
                     help='full: particle-management target count (default: the loaded count)')
-    ap.add_argument('--train-steps', type=int, default=8000, help='learned: Adam steps')
+    ap.add_argument('--train-steps', type=int, default=None,
+                    help='learned, northstar: Adam steps (default 8000 for learned, 16000 for northstar)')
     ap.add_argument('--pairs', type=int, default=192, help='learned: training pairs')
     ap.add_argument('--train-pairs', choices=('operator', 'buildup'), default='operator',
                     help='learned: train on (A phi, phi) with phi random at 0-64 cell scales, or on the '
                          'buildup dumps (half) and white-noise charges (half) with phi from the direct solve')
     ap.add_argument('--grid', type=int, default=1024, help='learned: grid cells per side')
+    ap.add_argument('--fcg-dirs', type=int, default=8,
+                    help='learned, northstar: directions the flexible CG keeps (1-8)')
     ap.add_argument('--workload', default='full', choices=WORKLOADS,
                     help='full: the 2048^2 quadrupole buildup with particle management (default); '
                          'dipole: the 512x384 dipole buildup; scan: 64 buildups stepped together; '
@@ -113,6 +116,10 @@ def parse(argv=None):
         a.dtype = 'f64' if a.workload == 'dipole' else 'f32'
     if a.particles is None:
         a.particles = 2e6 if a.workload == 'dipole' else 2e7
+    if not 1 <= a.fcg_dirs <= 8:
+        ap.error('--fcg-dirs is 1-8')
+    if a.train_steps is None:
+        a.train_steps = 16000 if a.workload == 'northstar' else 8000
     return a
 
 
@@ -762,7 +769,7 @@ def main_learned(a):
         noise.append((q, ps.solvePhi(q).clone()))
     test = clouds[-ntest:] + noise[-ntest:]
     val = [clouds[-ntest - 1][0], noise[-ntest - 1][0]]
-    net = UNetPreconditioner(g, seed=0)
+    net = UNetPreconditioner(g, seed=0, mDirs=a.fcg_dirs)
     raw = (clouds[:-ntest - 1] + noise[:-ntest - 1]) if fromBuildup else net.operator_pairs(ps, a.pairs, seed=1)
     pairs = raw
     log('%d training pairs (%s), %d test pairs: %.1f s' % (len(raw), a.train_pairs, len(test),
@@ -1146,7 +1153,7 @@ def main_northstar(a):
             q = (scan.run_rho(r) + sc * scan._prof).clone()
             pairs.append((q, ps.solvePhi(q).clone()))
     scan.sync()
-    net = UNetPreconditioner(g, seed=0)
+    net = UNetPreconditioner(g, seed=0, mDirs=a.fcg_dirs)
     val = [pairs[-1][0], pairs[-R // 2][0]]   # held out: the model with the fewest iterations on them is kept
     # the scan's own dumps, and as many operator pairs (A phi, phi) with phi random at 0-64 cell
     # scales: cloud potentials alone leave the network's short reach poorly trained
@@ -1233,7 +1240,7 @@ def main_northstar(a):
                                'the learned flexible CG to 1e-8, warm-started, direct solve as fallback and judge' % R,
                    'runs': R, 'grid': [256, 256], 'train_pairs': '%d scan dumps + %d operator pairs' % (R * nDumps - 1,
                                                                                             R * nDumps),
-                   'train_steps': a.train_steps,
+                   'train_steps': a.train_steps, 'fcg_dirs': a.fcg_dirs,
                    'train_seconds': t_train, 'loss_first_last': [hist[0][1], hist[-1][1]] if hist else None,
                    'validation_iterations': net.validation},
         'poisson_solves_per_s': R * K / wall_l,

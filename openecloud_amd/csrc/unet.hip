@@ -57,13 +57,15 @@ constexpr int kThreads = 256;
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// float -> bfloat16, round to nearest even (NaN kept quiet)
-__device__ __forceinline__ uint16_t f2bf(float f)
+// float -> bfloat16, round to nearest even with subnormals kept (gfx950's v_cvt_pk_bf16_f32;
+// equal to the integer rounding u + 0x7FFF + ((u >> 16) & 1) for every non-NaN float, a NaN
+// stays a NaN).  f2bf2 converts a pair into one 32-bit word (a in the low half).
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t f2bf2(float a, float b)
 {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v2f{a, b}, v2bf));
 }
 
 __device__ __forceinline__ v4s load4(const uint16_t *p) { return *reinterpret_cast<const v4s *>(p); }
@@ -115,9 +117,8 @@ __device__ __forceinline__ double grid_rms(const double *sc) { return sc[1] > 0.
 constexpr int CONV_SPAN = 64 + 2;  // staged pixels per row
 __device__ __forceinline__ uint32_t add_bf16x2(uint32_t a, uint32_t b)
 {
-    const uint32_t lo = f2bf(bf2f((uint16_t)(a & 0xFFFFu)) + bf2f((uint16_t)(b & 0xFFFFu)));
-    const uint32_t hi = f2bf(bf2f((uint16_t)(a >> 16)) + bf2f((uint16_t)(b >> 16)));
-    return lo | (hi << 16);
+    return f2bf2(__uint_as_float(a << 16) + __uint_as_float(b << 16),
+                 __uint_as_float(a & 0xFFFF0000u) + __uint_as_float(b & 0xFFFF0000u));
 }
 
 template <int MODE, int RW, int SRC = 0>
@@ -127,9 +128,14 @@ __global__ void __launch_bounds__(kThreads)
            uint16_t *__restrict__ pooled)
 {
     __shared__ __align__(16) uint16_t tile[RW + 2][CONV_SPAN][C];
-    if (g.st && g.st[(size_t)blockIdx.y * g.ss] != 0.) return;  // a stopped run: whole block
-    if (blockIdx.y) {  // image blockIdx.y of a batch: its images and grid vectors
-        const size_t b = blockIdx.y, is = (size_t)H * W * C, isc = (size_t)(H >> 1) * (W >> 1) * C;
+    // tile and image: the (tile, image) grid is flattened and ordered XCD-aware as a whole, so
+    // that the tiles of one image sharing halo rows run on one XCD
+    const uint32_t lin = xcd_block(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const uint32_t img = lin / gridDim.x;
+    const int blk = (int)(lin - img * gridDim.x);
+    if (g.st && g.st[(size_t)img * g.ss] != 0.) return;  // a stopped run: whole block
+    if (img) {  // image img of a batch: its images and grid vectors
+        const size_t b = img, is = (size_t)H * W * C, isc = (size_t)(H >> 1) * (W >> 1) * C;
         if (in) in += b * is;
         if (up) up += b * isc;
         if (pooled) pooled += b * isc;
@@ -141,8 +147,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     const int lane = threadIdx.x & 63, px = lane & 15, cq = lane >> 4, wv = threadIdx.x >> 6;
     const int quads = (W + 63) >> 6;
-    const int blk = (int)xcd_block(blockIdx.x, gridDim.x);
-    const int y0 = (blk / quads) * RW, xq = (blk % quads) << 6;  // grid = groups * quads exactly
+    const int y0 = (blk / quads) * RW, xq = (blk % quads) << 6;  // grid.x = groups * quads exactly
     // stage rows y0 - 1 .. y0 + RW, pixels xq - 1 .. xq + 64 (two 16-byte halves per pixel):
     // every load of the thread is issued before the first LDS store
     constexpr int NE = (RW + 2) * CONV_SPAN * 2, NL = (NE + kThreads - 1) / kThreads;
@@ -233,13 +238,14 @@ __global__ void __launch_bounds__(kThreads)
             if (cq == 0) static_cast<float *>(out)[p] = acc[r][0] + bias[0];
             continue;
         }
-        v4s o;
+        float f[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            float v = acc[r][j] + bias[4 * cq + j];
-            if (MODE == 0) v = v > 0.f ? v : 0.f;
-            o[j] = (short)f2bf(v);
+            f[j] = acc[r][j] + bias[4 * cq + j];
+            if (MODE == 0) f[j] = f[j] > 0.f ? f[j] : 0.f;
         }
+        const uint32_t w01 = f2bf2(f[0], f[1]), w23 = f2bf2(f[2], f[3]);
+        const v4s o{(short)(w01 & 0xFFFFu), (short)(w01 >> 16), (short)(w23 & 0xFFFFu), (short)(w23 >> 16)};
         *reinterpret_cast<v4s *>(static_cast<uint16_t *>(out) + p * C + 4 * cq) = o;
         if constexpr (RW >= 2) {
             // pooled != null: also the 2x2 average of the stored values into the next level's
