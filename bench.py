@@ -42,9 +42,25 @@ def profile_traffic(name):
     return None
 
 
+def push_model(dtype, arith, gridB):
+    """(kernel, algorithmic bytes per particle, per grid node, description) of one push launch:
+    the particle reads x, y, vx, vy, vz, w and writes x, y, vx, vy, vz (11 values); per node
+    the push reads E (FP64 (Ex, Ey), or its FP32 copy for the mixed push), a grid B field's
+    (Bx, By) likewise, and adds into rho (an FP64 read and write)."""
+    s = 8 if dtype == 'f64' else 4
+    mixed = dtype == 'f32' and arith != 'fp64'
+    fe = 8 if mixed else 16
+    node = fe + (fe if gridB else 0) + 16
+    kern = 'k_push_mixed' if mixed else 'k_push_dev'
+    desc = ('%d B per %s macro-particle + %d B per grid node (E%s %s, rho FP64 read and write)'
+            % (11 * s, dtype.upper(), node, ', B' if gridB else '', 'FP32' if mixed else 'FP64'))
+    return kern, 11 * s, node, desc
+
+
 TRAFFIC_PROFILES = {('dipole', 'f64', 2000000): 'push_hbm_traffic_ellip512x384_2e6_f64.md',
-                    ('scan', 'f32', 64): 'push_hbm_traffic_scan_64runs_256sq_f32.md',
-                    ('push', 'f32', 20000000): 'push_hbm_traffic_rect2049sq_2e7_f32_quadrupole.md'}
+                    ('scan', 'fp64', 64): 'push_hbm_traffic_scan_64runs_256sq_f32.md',
+                    ('push', 'fp64', 20000000): 'push_hbm_traffic_rect2049sq_2e7_f32_quadrupole.md',
+                    ('push', 'mixed', 20000000): 'push_mixed_hbm_traffic_rect2049sq_2e7_f32_quadrupole.md'}
 
 
 def parse(argv=None):
@@ -95,6 +111,9 @@ def parse(argv=None):
                     help='scan: sortEvery,order,binShift to use instead of autotuning')
     ap.add_argument('--no-f32-compare', action='store_true',
                     help='dipole: skip the FP32-storage run compared against the FP64 one')
+    ap.add_argument('--push-arith', default=None, choices=('mixed', 'fp64'),
+                    help='push arithmetic of FP32 particles: mixed (FP32 gather and Boris push, FP64 charge '
+                         'sums and the FP64 push within 2 cells of the wall; the default) or fp64')
     a = ap.parse_args(argv)
     if a.gpus != 1:
         ap.exit(2, 'bench.py runs on one GPU only: this build has no multi-GPU path (the sharding '
@@ -114,6 +133,8 @@ def parse(argv=None):
             ap.error('--scan-config is sortEvery,order,binShift (sortEvery >= 1, order 0 or 1, binShift 0-8)')
     if a.dtype is None:
         a.dtype = 'f64' if a.workload == 'dipole' else 'f32'
+    if a.push_arith == 'mixed' and a.dtype == 'f64':
+        ap.error('--push-arith mixed needs FP32 particles')
     if a.particles is None:
         a.particles = 2e6 if a.workload == 'dipole' else 2e7
     if not 1 <= a.fcg_dirs <= 8:
@@ -306,7 +327,8 @@ def measure_full(a, log):
     n = int(1.525 * target) if not a.target else int(a.particles)
     g, bq, p, pb, em, ps, b = build_full(n, log)
     b.native_setup(absCap=max(65536, n // 32), secCap=max(131072, n // 16),
-                   maxSteps=a.warmup + a.steps + 20)
+                   maxSteps=a.warmup + a.steps + 20, pushArithmetic=a.push_arith)
+    kern, bpp, bpn, bdesc = push_model('f32', b.pushArithmetic, True)
     mgmt = []
 
     def manage():
@@ -339,7 +361,7 @@ def measure_full(a, log):
     push = event_us(evs)
     avg_us = sum(push) / len(push)
     npx = g.getNpExt()
-    bytes_per = sum(11 * 4 * c + 48 * npx for c in counts) / len(counts)
+    bytes_per = sum(bpp * c + bpn * npx for c in counts) / len(counts)
     achieved = bytes_per / (avg_us * 1e-6) / 1e9
     hits = (st['absorbed_total'] - st0['absorbed_total']) / a.steps
     # one forced management event (the 1.5x trigger's reduction from the current count), timed
@@ -357,11 +379,11 @@ def measure_full(a, log):
     # rectangular chamber of 2049^2 cells: the same particle count, grid size and field)
     traffic, traffic_src = None, 'not measured for this workload'
     if target == 20000000:
-        name = TRAFFIC_PROFILES[('push', 'f32', 20000000)]
+        name = TRAFFIC_PROFILES.get(('push', b.pushArithmetic, 20000000), '')
         traffic = profile_traffic(name)
         if traffic is not None:
-            traffic_src = ('profiles/%s: the same k_push_dev at this size (2e7 FP32, quadrupole on the grid) on the '
-                           'rectangular 2049^2-cell chamber of --workload push' % name)
+            traffic_src = ('profiles/%s: the same %s at this size (2e7 FP32, quadrupole on the grid) on the '
+                           'rectangular 2049^2-cell chamber of --workload push' % (name, kern))
     out = {
         'value': float(sum(counts)) / wall,
         'ms_per_step': wall / a.steps * 1e3,
@@ -377,11 +399,11 @@ def measure_full(a, log):
                    'emitted_total': st['emitted_total']},
         'poisson_solves_per_s': solves,
         'poisson_relres': res,
-        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+        'push_arithmetic': b.pushArithmetic,
+        'roofline': {'bound': 'hbm', 'kernel': kern, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
                      'traffic_source': traffic_src, 'avg_launch_us': avg_us,
-                     'bytes_per_launch': bytes_per,
-                     'bytes_model': '44 B per FP32 macro-particle + 48 B per grid node (E, rho, B)'},
+                     'bytes_per_launch': bytes_per, 'bytes_model': bdesc},
         'management_events': mgmt,
         'management_event': {'count_before': nBefore, 'count_after': p.macroParticleCount, 'seconds': t_merge,
                              'charge_rel_change': abs(q1 - q0) / abs(q0), 'ops': ops},
@@ -395,7 +417,8 @@ def measure_dipole(a, log, steps, warmup, n, dtype, eager, compare=False):
     import torch
     g, p, pb, em, ps, b = build_dipole(n, dtype)
     b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4),
-                   maxSteps=warmup + steps + 40)
+                   maxSteps=warmup + steps + 40, pushArithmetic=a.push_arith if dtype == 'f32' else None)
+    kern, bpp, bpn, bdesc = push_model(dtype, b.pushArithmetic, False)
     b.sort_particles()
     if a.calib_copy:
         from openecloud_amd import _lib as L
@@ -432,9 +455,8 @@ def measure_dipole(a, log, steps, warmup, n, dtype, eager, compare=False):
     counts = b.counts(it0, it0 + steps)
     push_us = event_us(evs_used)
     event_us(evs_left)
-    s = 8 if dtype == 'f64' else 4
     npx = g.getNpExt()
-    avg_bytes = sum(11 * s * c + 24 * npx for c in counts) / len(counts)
+    avg_bytes = sum(bpp * c + bpn * npx for c in counts) / len(counts)
     avg_us = sum(push_us) / len(push_us)
     achieved = avg_bytes / (avg_us * 1e-6) / 1e9
     q = b.rho.clone()
@@ -457,12 +479,13 @@ def measure_dipole(a, log, steps, warmup, n, dtype, eager, compare=False):
         'poisson_solves_per_s': poisson_rate(ps, q, 100),
         'poisson_relres': ps.residual(q),
         'poisson_capacitance_rows': ps.capacitanceRows,
-        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved,
+        'push_arithmetic': b.pushArithmetic,
+        'roofline': {'bound': 'hbm', 'kernel': kern, 'achieved': achieved,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                      'traffic': tr,
                      'traffic_source': ('profiles/' + TRAFFIC_PROFILES[('dipole', dtype, n)]) if tr else
                      'not measured for this workload',
-                     'avg_launch_us': avg_us, 'bytes_per_launch': avg_bytes,
+                     'avg_launch_us': avg_us, 'bytes_per_launch': avg_bytes, 'bytes_model': bdesc,
                      'note': 'the 96 MB FP64 particle state fits the 256 MB last-level cache'},
     }
     if compare:
@@ -477,7 +500,8 @@ def compare_f32(a, n, b64, warmup, steps, eager, extra):
     and macro-particle count."""
     import torch
     g, p, pb, em, ps, b = build_dipole(n, 'f32')
-    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4), maxSteps=warmup + steps + 40)
+    b.native_setup(absCap=max(65536, n // 8), secCap=max(131072, n // 4), maxSteps=warmup + steps + 40,
+                   pushArithmetic=a.push_arith)
     b.sort_particles()
     run_plan(b, step_plan(warmup, a.sort_every, False))
     b.sync()
@@ -504,6 +528,7 @@ def compare_f32(a, n, b64, warmup, steps, eager, extra):
             'cloud_density_rel_l2': float(torch.linalg.norm(r32 - r64) / torch.linalg.norm(r64)),
             'total_charge_rel': abs(q32 - q64) / abs(q64),
             'macro_particles': [b64.p.macroParticleCount, b.p.macroParticleCount],
+            'push_arithmetic': b.pushArithmetic,
             'note': 'FP32 particle storage, same seeds and steps; differences from the FP64 run after '
                     'the timed steps'}
 
@@ -660,7 +685,7 @@ def cpu_baseline_dipole(g, n, hits, seconds):
             'nproc': info['nproc'], 'cpu_model': info['cpu_model'], 'poisson_solves_per_s': 1. / t_lu}
 
 
-def build_scan(R, dtype, maxSteps):
+def build_scan(R, dtype, maxSteps, pushArithmetic=None):
     """The scan workload: R buildups on a shared 256 x 256 rectangular 100 x 50 mm chamber,
     By = 0.3 T, dt = 1e-11 s, Gaussian bunches sigma 2 mm at 25 ns, 8 bunches, Furman copper
     with E_max 300 eV.  delta_max runs over linspace(1.1, 2.0, R); bunch intensity and
@@ -697,7 +722,7 @@ def build_scan(R, dtype, maxSteps):
                                   nBunches=8)
         homoLoader(g, p, pb, int(nmp[r]), 1e9 / float(nmp[r]), 4.194e5)
         runs.append(Buildup(g, p, pb, em, ps, beam, dt=dt, typeBField=1))
-    scan = BuildupScan(runs, seeds, growth=1.5, maxSteps=maxSteps)
+    scan = BuildupScan(runs, seeds, growth=1.5, maxSteps=maxSteps, pushArithmetic=pushArithmetic)
     return g, ps, runs, scan, dict(sey=sey, ppb=ppb, n=nmp)
 
 
@@ -900,7 +925,7 @@ def main_scan(a):
     log = lambda m: print(m, file=sys.stderr, flush=True)
     start = max(int(a.scan_start), a.warmup)
     maxSteps = start + a.steps + 2000
-    g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps)
+    g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps, a.push_arith)
     scan.sort()
     for _ in range(a.warmup):
         scan.step()
@@ -917,7 +942,7 @@ def main_scan(a):
     del scan, runs
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
-    g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps)
+    g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps, a.push_arith)
     scan.sort(best['order'], best['binShift'])
     for _ in range(a.warmup):
         scan.step()
@@ -966,8 +991,8 @@ def main_scan(a):
         L.lib().oecl_event_destroy(e0)
         L.lib().oecl_event_destroy(e1)
     tot = [sum(scan.history(r, i1 + k, i1 + k + 1)[0] for r in range(R)) for k in range(se)]
-    s_b = 8 if a.dtype == 'f64' else 4
-    by = [11 * s_b * tot[k] + 24 * g.getNpExt() * R for k in range(se)]
+    kern, bpp, bpn, bdesc = push_model(a.dtype, scan.pushArithmetic, False)
+    by = [bpp * tot[k] + bpn * g.getNpExt() * R for k in range(se)]
     avg_us = sum(push_us) / len(push_us)
     achieved = (sum(by) / len(by)) / (avg_us * 1e-6) / 1e9
     print('scan push time (us) by steps since the last sort: ' +
@@ -1001,7 +1026,7 @@ def main_scan(a):
         ref_t += time.perf_counter() - t
     seq_rate, ref_rate = seq_ps / seq_t, ref_ps / ref_t
     n0 = [int(x) for x in prm['n']]
-    tr = profile_traffic(TRAFFIC_PROFILES.get(('scan', a.dtype, R), ''))
+    tr = profile_traffic(TRAFFIC_PROFILES.get(('scan', scan.pushArithmetic, R), ''))
     out = {
         'metric': 'batched scan buildup particle-steps/sec (%d buildups per step call)' % R,
         'value': rate,
@@ -1025,11 +1050,12 @@ def main_scan(a):
                    'absorbed_in_timed_steps': int(sum(st['absorbed_total']) - sum(st0['absorbed_total'])),
                    'emitted_in_timed_steps': int(sum(st['emitted_total']) - sum(st0['emitted_total']))},
         'poisson_solves_per_s': R * steps / wall,
-        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+        'push_arithmetic': scan.pushArithmetic,
+        'roofline': {'bound': 'hbm', 'kernel': kern, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': tr,
-                     'traffic_source': ('profiles/' + TRAFFIC_PROFILES[('scan', a.dtype, R)]) if tr else
+                     'traffic_source': ('profiles/' + TRAFFIC_PROFILES[('scan', scan.pushArithmetic, R)]) if tr else
                      'not measured for this workload',
-                     'avg_launch_us': avg_us, 'bytes_per_launch': sum(by) / len(by)},
+                     'avg_launch_us': avg_us, 'bytes_per_launch': sum(by) / len(by), 'bytes_model': bdesc},
         'autotune': {'best': best, 'candidates_us_per_step': [[c, 1e6 * t] for c, t in results]},
         'plain_one_run_per_call': {'value': seq_rate, 'unit': 'particle-steps/s',
                                    'note': 'the same runs stepped one at a time with the single-run one-call step'},
@@ -1293,7 +1319,9 @@ def main_push(a):
     homoLoader(g, p, pb, n, 1e9 / n, 4.194e5)
     p.reserve(int(1.6 * n) + 4096)
     b = Buildup(g, p, pb, em, ps, beam, dt=2e-12, typeBField=2, bAtGridPoints=bq)
-    b.native_setup(absCap=max(65536, n // 32), secCap=max(131072, n // 16), maxSteps=a.warmup + a.steps + 4)
+    b.native_setup(absCap=max(65536, n // 32), secCap=max(131072, n // 16), maxSteps=a.warmup + a.steps + 4,
+                   pushArithmetic=a.push_arith)
+    kern, bpp, bpn, bdesc = push_model('f32', b.pushArithmetic, True)
     b.sort_particles()
     run_plan(b, [('step',)] * a.warmup)
     b.sync()
@@ -1309,9 +1337,10 @@ def main_push(a):
     push = event_us(evs)
     avg_us = sum(push) / len(push)
     npx = g.getNpExt()
-    bytes_per = sum(11 * 4 * c + 48 * npx for c in counts) / len(counts)
+    bytes_per = sum(bpp * c + bpn * npx for c in counts) / len(counts)
     achieved = bytes_per / (avg_us * 1e-6) / 1e9
-    tb = profile_traffic(TRAFFIC_PROFILES[('push', 'f32', 20000000)]) if n == 20000000 else None
+    tname = TRAFFIC_PROFILES.get(('push', b.pushArithmetic, 20000000), '')
+    tb = profile_traffic(tname) if n == 20000000 else None
     out = {
         'metric': 'push kernel of the full workload: particle-steps/sec',
         'value': float(sum(counts)) / wall,
@@ -1328,11 +1357,11 @@ def main_push(a):
         'config': {'workload': '2049x2049-cell rectangular 90x70 mm chamber (no capacitance rows), quadrupole '
                                '50 T/m on the grid, %d FP32 macro-electrons, dt 2e-12 s, direct Poisson solve' % n,
                    'grid': [2049, 2049], 'particles': n, 'push_us': push},
-        'roofline': {'bound': 'hbm', 'kernel': 'k_push_dev', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+        'push_arithmetic': b.pushArithmetic,
+        'roofline': {'bound': 'hbm', 'kernel': kern, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': tb,
-                     'traffic_source': ('profiles/' + TRAFFIC_PROFILES[('push', 'f32', 20000000)]) if tb else None,
-                     'avg_launch_us': avg_us, 'bytes_per_launch': bytes_per,
-                     'bytes_model': '44 B per FP32 macro-particle + 48 B per grid node (E, rho, B)'},
+                     'traffic_source': ('profiles/' + tname) if tb else None,
+                     'avg_launch_us': avg_us, 'bytes_per_launch': bytes_per, 'bytes_model': bdesc},
     }
     log('push: %.1f us per launch, %.3f of 8 TB/s' % (avg_us, achieved / HBM_PEAK_GBS))
     print(json.dumps(out), flush=True)

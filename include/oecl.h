@@ -481,7 +481,8 @@ typedef struct {
     void *data;                 /* particle SoA block, 6 rows of `stride` (= capacity) */
     uint64_t stride;
     uint32_t *nDev;
-    uint8_t *isInside;          /* [stride] */
+    uint8_t *isInside;          /* [stride] or NULL: the push's inside flag per slot (before the
+                                   step's compaction); not written when NULL */
     double *rho, *q;            /* [npExt]: charge of the current particles (in / out), scratch */
     const double *profile;      /* [npExt] beam profile, or NULL */
     const double *beamScale;    /* [nScale] beam scale per step, or NULL */
@@ -533,6 +534,19 @@ typedef struct {
     const int32_t *e2nIrr;      /* [nIrr] rows (of 2 npExt) of class 2 */
     uint32_t nIrr;
     double e2nCx, e2nCy;
+    /* Arithmetic of the push for FP32 particles (dtype OECL_F32; ignored for FP64), no
+     * reference counterpart (the reference computes in FP64 only, particles.pyx:272-418):
+     * 0  FP64 registers for the weights, gathers, Boris rotation, wall test and deposit;
+     * 1  mixed: weights, gathers and the Boris push in FP32 from FP32 copies of the fields,
+     *    the charge split into the four corners, the run sums and rho in FP64; a particle
+     *    within nearCells cells of the wall before or after the push (emitted electrons,
+     *    grazing orbits) is pushed and tested exactly as in mode 0 (FP64 fields, FP64
+     *    rotation, FP64 wall test), so every absorption decision is the FP64 one. */
+    int32_t pushArith;
+    float *Eil32;               /* [2 npExt] (per run) scratch for pushArith 1: (Ex, Ey) per node in
+                                   FP32 (E itself stays FP64) */
+    const float *Bil32;         /* [2 npExt] or NULL: Bil in FP32, needed with Bil for pushArith 1 */
+    double nearCells;           /* pushArith 1: the wall margin in cells (>= 1) */
 } oecl_step_args;
 
 /* Particles per push block (256): batched run segments are multiples of it, and the per-block

@@ -42,7 +42,55 @@ class StepArgs(ctypes.Structure):
                 ('photoSigmaE', ctypes.c_double), ('photoSlots', _VP), ('photoCap', ctypes.c_uint32),
                 ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32), ('Bil', _VP),
                 ('e2nKind', _VP), ('e2nIrr', _VP), ('nIrr', ctypes.c_uint32), ('e2nCx', ctypes.c_double),
-                ('e2nCy', ctypes.c_double)]
+                ('e2nCy', ctypes.c_double), ('pushArith', ctypes.c_int32), ('Eil32', _VP), ('Bil32', _VP),
+                ('nearCells', ctypes.c_double)]
+
+
+PUSH_ARITHMETIC = ('fp64', 'mixed')
+
+
+def push_arithmetic(dtype, mode):
+    """The push arithmetic of the one-call step for particles of `dtype`:
+      'fp64'   weights, gathers, Boris rotation, wall test and deposit in FP64 registers (the
+               only form for FP64 particles);
+      'mixed'  FP32 particles: gather and Boris push in FP32 from FP32 copies of E and B, the
+               charge split, run sums and rho in FP64, and the FP64 form for every particle
+               within `nearCells` cells of the wall before or after its push (emitted electrons
+               and grazing orbits), so absorption decisions are the FP64 ones.
+    None gives 'mixed' for FP32 particles and 'fp64' for FP64 ones."""
+    fp32 = dtype == torch.float32
+    if mode is None:
+        return 'mixed' if fp32 else 'fp64'
+    if mode not in PUSH_ARITHMETIC:
+        raise ValueError('pushArithmetic must be one of %s' % (PUSH_ARITHMETIC,))
+    if mode == 'mixed' and not fp32:
+        raise ValueError('the mixed push needs FP32 particles')
+    return mode
+
+
+def set_push_fields(obj, a, nNodes, mode, nearCells):
+    """Allocates on obj the field copies the push of `mode` reads (nNodes: grid nodes of all
+    runs) and points the step arguments a at them; obj._bil is the (Bx, By) copy or None."""
+    dev = L.device()
+    if mode == 'mixed':
+        if not nearCells >= 1.:
+            raise ValueError('nearCells must be at least 1')
+        obj._eil = None
+        obj._eil32 = torch.zeros(2 * nNodes, dtype=torch.float32, device=dev)
+        obj._bil32 = obj._bil.float().contiguous() if obj._bil is not None else None
+        a.Eil = None
+        a.Eil32 = obj._eil32.data_ptr()
+        a.Bil32 = obj._bil32.data_ptr() if obj._bil32 is not None else None
+        a.pushArith = 1
+        a.nearCells = float(nearCells)
+    else:
+        obj._eil = torch.zeros(2 * nNodes, dtype=torch.float64, device=dev)
+        obj._eil32 = obj._bil32 = None
+        a.Eil = obj._eil.data_ptr()
+        a.Eil32 = None
+        a.Bil32 = None
+        a.pushArith = 0
+        a.nearCells = 0.
 
 
 def set_e2n_classes(a, ps):
@@ -247,9 +295,12 @@ class Buildup:
         self.it += 1
 
     # ------------------------------------------------------------------ native step
-    def native_setup(self, absCap=None, secCap=None, maxSteps=100000):
+    def native_setup(self, absCap=None, secCap=None, maxSteps=100000, pushArithmetic=None, nearCells=2.):
         """Allocates the device state of the one-call step (oecl_step_run).  The beam scale of
-        every step up to maxSteps is tabulated now (it depends on the step index only)."""
+        every step up to maxSteps is tabulated now (it depends on the step index only).
+        pushArithmetic: 'fp64' or 'mixed' (push_arithmetic; None: 'mixed' for FP32 particles),
+        nearCells the wall margin of the mixed push."""
+        self.pushArithmetic = push_arithmetic(self.p.dtype, pushArithmetic)
         p, pb, ps = self.p, self.pb, self.ps
         if ps.method != 'direct':
             raise ValueError('the native step needs PoissonSolver(method="direct"); method "%s" runs through '
@@ -287,7 +338,7 @@ class Buildup:
         a.data = p._data.data_ptr()
         a.stride = cap
         a.nDev = self._nDev.data_ptr()
-        a.isInside = p.isInside.data_ptr()
+        a.isInside = None          # the one-call step keeps its holes in lists; no per-slot flags
         a.rho = self.rho.data_ptr()
         a.q = self.q.data_ptr()
         a.profile = prof.data_ptr() if prof is not None else None
@@ -340,10 +391,9 @@ class Buildup:
             self._prm.particleMass = p.particleMass
         else:
             self._prm = self.em.desc
-        self._eil = torch.empty(2 * self.g.getNpExt(), dtype=torch.float64, device=dev)
-        a.Eil = self._eil.data_ptr()
         self._bil = b_layout(self.bField, self.g.getNpExt()) if self.typeB == 2 else None
         a.Bil = self._bil.data_ptr() if self._bil is not None else None
+        set_push_fields(self, a, self.g.getNpExt(), self.pushArithmetic, nearCells)
         a.work = self._stepwork.data_ptr()
         self._args = a
         if not self._rho_valid:

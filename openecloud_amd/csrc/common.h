@@ -126,3 +126,36 @@ __device__ __forceinline__ uint32_t xcd_block_chunk(uint32_t b, uint32_t n, uint
 // per-block run table (blockRun) has one entry per kPushBlock particle slots: every kernel that
 // reads that table divides slot indices by this constant.
 constexpr uint32_t kPushBlock = 256;
+
+// ---- raw buffer access: a 128-bit buffer resource in SGPRs plus a 32-bit byte offset per lane
+// (one VALU offset serves several loads: the constant part goes to the instruction's immediate
+// offset, a uniform part to the scalar offset).  Loads past `bytes` read 0 and stores past it are
+// dropped by the hardware.  The resource's third dword is the gfx9 raw-buffer format word.
+typedef __amdgpu_buffer_rsrc_t BufRes;
+
+__device__ __forceinline__ BufRes buf_res(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float buf_ld_f32(BufRes r, uint32_t off, uint32_t soff = 0)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
+}
+
+// Two adjacent floats: two dword loads, which the backend merges into one dwordx2 load.  (The b64
+// form of the load builtin is not used: with this compiler it loads the low dword only.)
+__device__ __forceinline__ float2 buf_ld_f32x2(BufRes r, uint32_t off, uint32_t soff = 0)
+{
+    return make_float2(buf_ld_f32(r, off, soff), buf_ld_f32(r, off + 4u, soff));
+}
+
+__device__ __forceinline__ void buf_st_f32(float v, BufRes r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+}
+
+__device__ __forceinline__ void buf_st_u8(uint8_t v, BufRes r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b8(v, r, (int)off, 0, 0);
+}

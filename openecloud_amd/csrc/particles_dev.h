@@ -119,7 +119,17 @@ __device__ __forceinline__ bool wave_run_sum(uint32_t c, double &q0, double &q1,
     // runs are cut at aligned groups of S lanes (S = 64: the whole wave)
     constexpr int S = OECL_RUNSUM_SPAN;
     const int lane = threadIdx.x & 63;
-    const uint32_t cp = __shfl_up(c, 1, 64), cn = __shfl_down(c, 1, 64);
+    // the neighbours' cells: within a row of 16 lanes by DPP row shifts when the groups do not
+    // cross rows (a row's first lane starts a group, its last lane ends one, so the 0 that
+    // bound_ctrl gives them is never used), else by cross-lane shuffles
+    uint32_t cp, cn;
+    if (S <= 16) {
+        cp = dpp0_u32<0x111>(c);   // row_shr:1
+        cn = dpp0_u32<0x101>(c);   // row_shl:1
+    } else {
+        cp = __shfl_up(c, 1, 64);
+        cn = __shfl_down(c, 1, 64);
+    }
     uint32_t f = (lane % S == 0 || cp != c) ? 1u : 0u;
     const bool last = lane % S == S - 1 || cn != c;
     seg_scan_step<0x111, 0xF>(f, q0, q1, q2, q3);                  // row_shr:1
@@ -134,18 +144,19 @@ __device__ __forceinline__ bool wave_run_sum(uint32_t c, double &q0, double &q1,
 }
 
 // Minimum (MIN) or maximum over the wave, uniform result.
+// Rows of 16 lanes are reduced by rotations (row_ror 1, 2, 4, 8: after them every lane holds its
+// row's result), which read no lane outside the row and so need no identity value.
 template <bool MIN>
 __device__ __forceinline__ uint32_t wave_minmax(uint32_t v)
 {
-    const uint32_t id = MIN ? 0xFFFFFFFFu : 0u;
     uint32_t o;
-    o = dpp_u32<0x111, 0xF>(v, id);
+    o = dpp0_u32<0x121>(v);
     v = MIN ? min(v, o) : max(v, o);
-    o = dpp_u32<0x112, 0xF>(v, id);
+    o = dpp0_u32<0x122>(v);
     v = MIN ? min(v, o) : max(v, o);
-    o = dpp_u32<0x114, 0xF>(v, id);
+    o = dpp0_u32<0x124>(v);
     v = MIN ? min(v, o) : max(v, o);
-    o = dpp_u32<0x118, 0xF>(v, id);
+    o = dpp0_u32<0x128>(v);
     v = MIN ? min(v, o) : max(v, o);
     const uint32_t a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
     const uint32_t c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
@@ -158,18 +169,35 @@ template <bool MIN>
 __device__ __forceinline__ uint32_t wave_minmax2x16(uint32_t v)
 {
     typedef unsigned short u2 __attribute__((ext_vector_type(2)));
-    const uint32_t id = MIN ? 0xFFFFFFFFu : 0u;
     auto op = [](uint32_t a, uint32_t b) {
         const u2 x = __builtin_bit_cast(u2, a), y = __builtin_bit_cast(u2, b);
         return __builtin_bit_cast(uint32_t, MIN ? __builtin_elementwise_min(x, y) : __builtin_elementwise_max(x, y));
     };
-    v = op(v, dpp_u32<0x111, 0xF>(v, id));
-    v = op(v, dpp_u32<0x112, 0xF>(v, id));
-    v = op(v, dpp_u32<0x114, 0xF>(v, id));
-    v = op(v, dpp_u32<0x118, 0xF>(v, id));
+    v = op(v, dpp0_u32<0x121>(v));   // row_ror:1
+    v = op(v, dpp0_u32<0x122>(v));   // row_ror:2
+    v = op(v, dpp0_u32<0x124>(v));   // row_ror:4
+    v = op(v, dpp0_u32<0x128>(v));   // row_ror:8
     const uint32_t a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
     const uint32_t c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
     return op(op(a, b), op(c, d));
+}
+
+// Adds the non-zero nodes of an LDS tile of nt <= 2048 nodes (tw nodes per row, rows of nodes
+// from (X0, Y0)) to rho, one global FP64 atomic each; thread t of `step` takes nodes t, t + step,
+// ...  The row of tile node k is floor((k + 0.5) / tw) in FP32 with the hardware reciprocal:
+// (k + 0.5) / tw lies at least 0.5 / tw from an integer, the computed quotient (below
+// 2048 / tw) within 1.5 * 2^-23 of its value, so the floor is exact for k < 2048.
+__device__ __forceinline__ void tile_to_rho(double *__restrict__ rho, uint32_t nx, const double *tile, uint32_t nt,
+                                            uint32_t tw, uint32_t X0, uint32_t Y0, uint32_t t, uint32_t step)
+{
+    const float rtw = __builtin_amdgcn_rcpf((float)tw);
+    for (uint32_t k = t; k < nt; k += step) {
+        const double v = tile[k];
+        if (v != 0.) {
+            const uint32_t r = (uint32_t)(((float)k + 0.5f) * rtw);
+            unsafeAtomicAdd(rho + ((X0 + (k - r * tw)) + (Y0 + r) * nx), v);
+        }
+    }
 }
 
 // Wave-cooperative deposit (all 64 lanes call it; valid = false when the lane has no
@@ -185,6 +213,7 @@ __device__ __forceinline__ void wave_deposit(double *__restrict__ rho, uint32_t 
                                              uint32_t iy, double q00, double q10, double q01, double q11,
                                              double *win)
 {
+    static_assert(WN <= 2048, "tile_to_rho handles tiles of at most 2048 nodes");
     const int lane = threadIdx.x & 63;
     const uint32_t c = valid ? ix + iy * nx : 0xFFFFFFFFu;
     if (!valid) q00 = q10 = q01 = q11 = 0.;
@@ -216,13 +245,7 @@ __device__ __forceinline__ void wave_deposit(double *__restrict__ rho, uint32_t 
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t k = lane; k < nw; k += 64) {
-        const double v = win[k];
-        if (v != 0.) {
-            const uint32_t r = k / wx;
-            unsafeAtomicAdd(rho + (X0 + (k - r * wx)) + (size_t)(Y0 + r) * nx, v);
-        }
-    }
+    tile_to_rho(rho, nx, win, nw, wx, X0, Y0, lane, 64);
 }
 
 // Block-cooperative deposit: every thread of the block calls it, once.  As wave_deposit, but the
@@ -237,6 +260,7 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
                                               uint32_t iy, double q00, double q10, double q01, double q11,
                                               double *tile, uint32_t *box)
 {
+    static_assert(NW * WN <= 2048, "tile_to_rho handles tiles of at most 2048 nodes");
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t c = valid ? ix + iy * nx : 0xFFFFFFFFu;
     if (!valid) q00 = q10 = q01 = q11 = 0.;
@@ -279,13 +303,7 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
             atomicAdd(&tile[o + bw + 1], q11);
         }
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < nb; k += NW * 64) {
-            const double v = tile[k];
-            if (v != 0.) {
-                const uint32_t r = k / bw;
-                unsafeAtomicAdd(rho + (B0 + (k - r * bw)) + (size_t)(C0 + r) * nx, v);
-            }
-        }
+        tile_to_rho(rho, nx, tile, nb, bw, B0, C0, threadIdx.x, NW * 64);
         return;
     }
     if (X0 == 0xFFFFFFFFu) return;  // wave-uniform
@@ -313,13 +331,7 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t k = lane; k < nw; k += 64) {
-        const double v = win[k];
-        if (v != 0.) {
-            const uint32_t r = k / wx;
-            unsafeAtomicAdd(rho + (X0 + (k - r * wx)) + (size_t)(Y0 + r) * nx, v);
-        }
-    }
+    tile_to_rho(rho, nx, win, nw, wx, X0, Y0, lane, 64);
 }
 
 // Boris rotation + second half kick of particles.pyx:272-321, from the velocity after the
@@ -389,9 +401,10 @@ __device__ __forceinline__ void gather_push(const Weights<T> &w, uint32_t nx, ui
     const uint32_t c = w.cell;
     T ex, ey;
     if (TYPEB == 3) {
-        // E from Eil, B from B in the (Bx, By) layout; Bz is zero
+        // E from Eil (or E when Eil is null), B from B in the (Bx, By) layout; Bz is zero
         typedef double d2 __attribute__((ext_vector_type(2)));
-        gather_e_il<T>(w, nx, Eil, ex, ey);
+        if (Eil) gather_e_il<T>(w, nx, Eil, ex, ey);
+        else gather_e<T>(w, nx, np, E, ex, ey);
         vx += k * ex;
         vy += k * ey;
         const d2 *B2 = reinterpret_cast<const d2 *>(B);
@@ -438,4 +451,83 @@ __device__ __forceinline__ void gather_push(const Weights<T> &w, uint32_t nx, ui
     }
     x += dt * vx;
     y += dt * vy;
+}
+
+// ---- FP32 arithmetic for FP32 particles (the step's mixed push) ------------------------------
+// Bilinear location of (x, y) in FP32: fx = x / dx + ox with ox = lxHalfExt / dx (one fused
+// multiply-add), the cell clamped into the grid as grid_weights does, the fractions left in
+// (fx, fy).
+struct Loc32 {
+    uint32_t ix, iy;
+    float fx, fy;
+};
+
+__device__ __forceinline__ Loc32 locate32(float x, float y, float dxi, float dyi, float ox, float oy, uint32_t nx,
+                                          uint32_t ny)
+{
+    float fx = fmaf(x, dxi, ox), fy = fmaf(y, dyi, oy);
+    uint32_t ix = (uint32_t)fmaxf(fx, 0.f), iy = (uint32_t)fmaxf(fy, 0.f);
+    ix = min(ix, nx - 2);
+    iy = min(iy, ny - 2);
+    Loc32 l;
+    l.ix = ix;
+    l.iy = iy;
+    l.fx = fx - (float)ix;
+    l.fy = fy - (float)iy;
+    return l;
+}
+
+// Bilinear interpolation of a two-component node field stored as float2 per node, at cell
+// (row pointers r0 = field + row iy, r1 = the next row) with corner weights wLL..wUU.
+__device__ __forceinline__ float2 bilinear2(const float2 *__restrict__ r0, const float2 *__restrict__ r1, uint32_t ix,
+                                           float wLL, float wUL, float wLU, float wUU)
+{
+    const float2 a = r0[ix], b = r0[ix + 1], c = r1[ix], d = r1[ix + 1];
+    float2 v;
+    v.x = fmaf(wUU, d.x, fmaf(wLU, c.x, fmaf(wUL, b.x, wLL * a.x)));
+    v.y = fmaf(wUU, d.y, fmaf(wLU, c.y, fmaf(wUL, b.y, wLL * a.y)));
+    return v;
+}
+
+// The same from a buffer resource: the corners at byte offsets oc, oc + 8 (row iy) and oc + nx8,
+// oc + nx8 + 8 (row iy + 1).
+__device__ __forceinline__ float2 bilinear2_buf(BufRes r, uint32_t oc, uint32_t nx8, float wLL, float wUL, float wLU,
+                                               float wUU)
+{
+    const float2 a = buf_ld_f32x2(r, oc), b = buf_ld_f32x2(r, oc + 8u);
+    const float2 c = buf_ld_f32x2(r, oc, nx8), d = buf_ld_f32x2(r, oc + 8u, nx8);
+    float2 v;
+    v.x = fmaf(wUU, d.x, fmaf(wLU, c.x, fmaf(wUL, b.x, wLL * a.x)));
+    v.y = fmaf(wUU, d.y, fmaf(wLU, c.y, fmaf(wUL, b.y, wLL * a.y)));
+    return v;
+}
+
+// Boris rotation + second half kick (particles.pyx:272-321) in FP32 with fused multiply-adds,
+// t = (t0, t1, t2), from the velocity after the first half kick; 1 / (1 + |t|^2) from the
+// hardware reciprocal (1 ulp).
+__device__ __forceinline__ void boris_rotate32(float &vx, float &vy, float &vz, float t0, float t1, float t2,
+                                               float kex, float key)
+{
+    const float vs0 = fmaf(-vz, t1, fmaf(vy, t2, vx));
+    const float vs1 = fmaf(-vx, t2, fmaf(vz, t0, vy));
+    const float vs2 = fmaf(-vy, t0, fmaf(vx, t1, vz));
+    const float f = 2.f * __builtin_amdgcn_rcpf(fmaf(t2, t2, fmaf(t1, t1, fmaf(t0, t0, 1.f))));
+    const float s0 = t0 * f, s1 = t1 * f, s2 = t2 * f;
+    vx = fmaf(-vs2, s1, fmaf(vs1, s2, vx)) + kex;
+    vy = fmaf(-vs0, s2, fmaf(vs2, s0, vy)) + key;
+    vz = fmaf(-vs1, s0, fmaf(vs0, s1, vz));
+}
+
+// Boris rotation with Bz = 0 (t2 = 0): the same as boris_rotate32 without the t2 terms.
+__device__ __forceinline__ void boris_rotate32_bz0(float &vx, float &vy, float &vz, float t0, float t1, float kex,
+                                                   float key)
+{
+    const float vs0 = fmaf(-vz, t1, vx);
+    const float vs1 = fmaf(vz, t0, vy);
+    const float vs2 = fmaf(-vy, t0, fmaf(vx, t1, vz));
+    const float f = 2.f * __builtin_amdgcn_rcpf(fmaf(t1, t1, fmaf(t0, t0, 1.f)));
+    const float s0 = t0 * f, s1 = t1 * f;
+    vx = fmaf(-vs2, s1, vx) + kex;
+    vy = fmaf(vs2, s0, vy) + key;
+    vz = fmaf(-vs1, s0, fmaf(vs0, s1, vz));
 }

@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(PT)
             p(2, i) = (T)vx;
             p(3, i) = (T)vy;
             if (TYPEB != 0) p(4, i) = (T)vz;
-            isInside[i] = in;
+            if (isInside) isInside[i] = in;
             if (in) {
                 dep[kk] = true;
                 w2[kk] = grid_weights<C>((C)(T)x, (C)(T)y, dxi, dyi, lxh, lyh, g.nx, g.ny);
@@ -220,6 +220,185 @@ __global__ void __launch_bounds__(PT)
         block_deposit<PT / 64, 512 * KP>(rho, g.nx, dep[kk], w2[kk].ix, w2[kk].iy, w2[kk].wLL * wq[kk] * q,
                                          w2[kk].wUL * wq[kk] * q, w2[kk].wLU * wq[kk] * q, w2[kk].wUU * wq[kk] * q,
                                          tiles, dbox);
+    }
+}
+
+// The wall margin of the mixed push, in FP32: a position with
+//   rectangle  |x| <= xlim and |y| <= ylim,
+//   ellipse    x^2 ax + y^2 ay <= gmax
+// lies at least the margin inside the wall.  (For an ellipse of semi-axes a, b, every point of
+// the scaled ellipse rho E lies at least (1 - rho) min(a, b) inside E: the distance to the
+// boundary is the minimum over directions n of h_E(n) - P.n >= (1 - rho) h_E(n), and the support
+// function h_E of an ellipse centred on the origin is at least min(a, b).)  The limits are
+// rounded toward the chamber's centre.  The cut-cell wall uses the grid's cut cells instead.
+struct NearWall {
+    float xlim, ylim;
+    float ax, ay, gmax;
+};
+
+// FP32 constants of the mixed push: 1/dx, 1/dy, lxHalfExt/dx, lyHalfExt/dy, dt, k (q/m dt for
+// typeB 0, q/m dt / 2 otherwise) and k b for a constant field (typeB 1).
+struct Push32 {
+    float dxi, dyi, ox, oy, dt, k, t0, t1, t2;
+};
+
+// The mixed push (pushArith 1, FP32 particles): gather and Boris push in FP32 from the FP32 field
+// copies E32 ((Ex, Ey) per node, per run) and B32 ((Bx, By) per node, typeB 3); the deposit as
+// k_push_dev's (FP64 weights of the stored position, FP64 charges, run sums and rho).  A
+// particle within the margin of the wall before or after the FP32 push takes the FP64 path of
+// k_push_dev<float, TYPEB> instead, from its stored state, with the FP64 fields E ([Ex | Ey]) and
+// B (typeB 3: (Bx, By) per node; typeB 2: [Bx | By | Bz]): its result, wall test included, is
+// that of k_push_dev bit for bit.  A particle outside the margin is inside the chamber as stored,
+// with the margin far above FP32 rounding.
+#ifndef OECL_MIXED_PER_THREAD
+#define OECL_MIXED_PER_THREAD 1
+#endif
+// Particles per thread of the mixed push (MKP particle slots per thread, MPT threads per block of
+// PB particles), as KP for k_push_dev.
+constexpr int MKP = OECL_MIXED_PER_THREAD;
+constexpr uint32_t MPT = PB / MKP;
+
+// One particle's result in the mixed push: absorbed (dead) or deposited (dep) at cell (cx, cy)
+// with the four corner charges.
+struct MixedOut {
+    bool dep, dead;
+    uint32_t cx, cy;
+    double q00, q10, q01, q11;
+};
+
+template <int TYPEB>
+__global__ void __launch_bounds__(MPT)
+    k_push_mixed(Runs runs, oecl_grid_t g, WallDev wall, NearWall nw, Push32 P, Soa<float> p,
+                 const uint32_t *__restrict__ nDev, const double *__restrict__ E, const float *__restrict__ E32,
+                 const double *__restrict__ B, const float *__restrict__ B32, double b0, double b1, double b2,
+                 double dt, double k, double q, double dxi, double dyi, uint8_t *__restrict__ isInside,
+                 double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
+                 uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
+{
+    __shared__ double tiles[PB / 64 * 512];
+    __shared__ uint32_t dbox[4];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t run = runs.block_run(blk);
+    const uint64_t sBeg = runs.seg0(run);
+    const uint32_t n = nDev[run];
+    const uint64_t first = (uint64_t)blk * PB;
+    if (first - sBeg >= n) return;  // whole block past the run's live particles
+    E += (size_t)run * 2 * g.np;
+    rho += (size_t)run * g.np;
+    misc += run * M_COUNT;
+    nzList += sBeg / 64;
+    // buffer resources (the host checks that every size fits 32 bits): one per particle component,
+    // the run's FP32 E copy and the FP32 B copy, 8 bytes per node
+    const uint32_t sb = (uint32_t)(p.stride * 4u), nb8 = g.np * 8u, nx8 = g.nx * 8u;
+    const BufRes rX = buf_res(p.base, sb), rY = buf_res(p.base + p.stride, sb);
+    const BufRes rVx = buf_res(p.base + 2 * p.stride, sb), rVy = buf_res(p.base + 3 * p.stride, sb);
+    const BufRes rVz = buf_res(p.base + 4 * p.stride, sb), rW = buf_res(p.base + 5 * p.stride, sb);
+    const BufRes rE = buf_res(E32 + (size_t)run * 2 * g.np, nb8);
+    auto one = [&](uint32_t i, MixedOut &o) {
+        const uint32_t oi = i * 4u;
+        const float x0 = buf_ld_f32(rX, oi), y0 = buf_ld_f32(rY, oi), vx0 = buf_ld_f32(rVx, oi);
+        const float vy0 = buf_ld_f32(rVy, oi), vz0 = buf_ld_f32(rVz, oi);
+        // ---- FP32 gather and push: the four corners of a field at byte offsets oc, oc + 8 and
+        // the same one row up (scalar offset nx8)
+        const Loc32 l = locate32(x0, y0, P.dxi, P.dyi, P.ox, P.oy, g.nx, g.ny);
+        const float gx = 1.f - l.fx, gy = 1.f - l.fy;
+        const float wLL = gx * gy, wUL = l.fx * gy, wLU = gx * l.fy, wUU = l.fx * l.fy;
+        const uint32_t row = l.iy * g.nx;
+        const uint32_t oc = (row + l.ix) * 8u;
+        const float2 ev = bilinear2_buf(rE, oc, nx8, wLL, wUL, wLU, wUU);
+        float vx = vx0, vy = vy0, vz = vz0;
+        const float kex = P.k * ev.x, key = P.k * ev.y;
+        vx += kex;
+        vy += key;
+        if (TYPEB == 1) {
+            boris_rotate32(vx, vy, vz, P.t0, P.t1, P.t2, kex, key);
+        } else if (TYPEB == 3) {
+            const float2 bv = bilinear2_buf(buf_res(B32, nb8), oc, nx8, wLL, wUL, wLU, wUU);
+            boris_rotate32_bz0(vx, vy, vz, P.k * bv.x, P.k * bv.y, kex, key);
+        } else if (TYPEB == 2) {
+            const uint32_t c = row + l.ix;
+            float bx = wLL * (float)B[c], by = wLL * (float)B[c + g.np], bz = wLL * (float)B[c + 2 * g.np];
+            bx = fmaf(wUL, (float)B[c + 1], bx);
+            by = fmaf(wUL, (float)B[c + 1 + g.np], by);
+            bz = fmaf(wUL, (float)B[c + 1 + 2 * g.np], bz);
+            bx = fmaf(wLU, (float)B[c + g.nx], bx);
+            by = fmaf(wLU, (float)B[c + g.nx + g.np], by);
+            bz = fmaf(wLU, (float)B[c + g.nx + 2 * g.np], bz);
+            bx = fmaf(wUU, (float)B[c + g.nx + 1], bx);
+            by = fmaf(wUU, (float)B[c + g.nx + 1 + g.np], by);
+            bz = fmaf(wUU, (float)B[c + g.nx + 1 + 2 * g.np], bz);
+            boris_rotate32(vx, vy, vz, P.k * bx, P.k * by, P.k * bz, kex, key);
+        }
+        const float x1 = fmaf(P.dt, vx, x0), y1 = fmaf(P.dt, vy, y0);
+        // ---- the wall margin, before and after the push (NaN positions count as near)
+        bool near;
+        if (wall.kind == OECL_WALL_RECT) {
+            near = !(fabsf(x0) <= nw.xlim && fabsf(y0) <= nw.ylim && fabsf(x1) <= nw.xlim && fabsf(y1) <= nw.ylim);
+        } else if (wall.kind == OECL_WALL_ELLIP) {
+            near = !(fmaf(x0 * x0, nw.ax, y0 * y0 * nw.ay) <= nw.gmax && fmaf(x1 * x1, nw.ax, y1 * y1 * nw.ay) <= nw.gmax);
+        } else {
+            // a cut or outside cell under the old or the new position; the new one located as the
+            // FP64 wall test does (clipped to the grid: a clipped point lies in an edge cell,
+            // which is outside the chamber)
+            double xd = (double)x1, yd = (double)y1;
+            if (!(xd <= wall.clipX)) xd = wall.clipX;
+            if (!(xd >= -wall.clipX)) xd = -wall.clipX;
+            if (!(yd <= wall.clipY)) yd = wall.clipY;
+            if (!(yd >= -wall.clipY)) yd = -wall.clipY;
+            near = wall.insideFaces[row + l.ix] != 8 || wall.insideFaces[wall_cell(wall, xd, yd)] != 8;
+        }
+        float xs = x1, ys = y1;
+        uint8_t in = 1;
+        if (near) {
+            // ---- FP64 path: k_push_dev<float, TYPEB> for this particle
+            double x = x0, y = y0, vxd = vx0, vyd = vy0, vzd = vz0;
+            const Weights<double> w = grid_weights<double>(x, y, dxi, dyi, g.lxHalfExt, g.lyHalfExt, g.nx, g.ny);
+            gather_push<double, TYPEB>(w, g.nx, g.np, E, B, b0, b1, b2, dt, k, x, y, vxd, vyd, vzd, nullptr);
+            x = (double)(float)x;
+            y = (double)(float)y;
+            in = wall_inside<double>(wall, x, y);
+            xs = (float)x;
+            ys = (float)y;
+            vx = (float)vxd;
+            vy = (float)vyd;
+            vz = (float)vzd;
+        }
+        buf_st_f32(xs, rX, oi);
+        buf_st_f32(ys, rY, oi);
+        buf_st_f32(vx, rVx, oi);
+        buf_st_f32(vy, rVy, oi);
+        if (TYPEB != 0) buf_st_f32(vz, rVz, oi);
+        if (isInside) buf_st_u8(in, buf_res(isInside, (uint32_t)p.stride), i);
+        if (in) {
+            // the deposit of k_push_dev: FP64 weights of the stored position, FP64 charges
+            o.dep = true;
+            const Weights<double> w2 = grid_weights<double>((double)xs, (double)ys, dxi, dyi, g.lxHalfExt,
+                                                            g.lyHalfExt, g.nx, g.ny);
+            const double wq = (double)buf_ld_f32(rW, oi);
+            o.cx = w2.ix;
+            o.cy = w2.iy;
+            o.q00 = w2.wLL * wq * q;
+            o.q10 = w2.wUL * wq * q;
+            o.q01 = w2.wLU * wq * q;
+            o.q11 = w2.wUU * wq * q;
+        } else {
+            o.dead = true;
+        }
+    };
+    MixedOut o[MKP];
+#pragma unroll
+    for (int kk = 0; kk < MKP; ++kk) {
+        const uint32_t i = (uint32_t)first + kk * MPT + threadIdx.x;
+        o[kk] = MixedOut{false, false, 0u, 0u, 0., 0., 0., 0.};
+        if (i - sBeg < n) one(i, o[kk]);
+    }
+#pragma unroll
+    for (int kk = 0; kk < MKP; ++kk) {
+        const uint32_t i = (uint32_t)first + kk * MPT + threadIdx.x;
+        wave_holes(o[kk].dead, i, blk * (PB / 64) + kk * (MPT / 64) + (threadIdx.x >> 6), counts, list, misc, nzList);
+        if (kk) __syncthreads();   // the previous deposit's readers of the tile and box are done
+        block_deposit<MPT / 64, 512 * MKP>(rho, g.nx, o[kk].dep, o[kk].cx, o[kk].cy, o[kk].q00, o[kk].q10, o[kk].q01,
+                                           o[kk].q11, tiles, dbox);
     }
 }
 
@@ -675,7 +854,8 @@ namespace {
 // given: ilw doubles per node (2: (Ex, Ey); 4: (Ex, Ey, Bx, By), B left as it is).
 __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, const int32_t *__restrict__ col,
                             const double *__restrict__ val, const double *__restrict__ x, uint32_t xStride,
-                            double *__restrict__ y, double *__restrict__ yil, uint32_t ilw)
+                            double *__restrict__ y, double *__restrict__ yil, uint32_t ilw,
+                            float *__restrict__ yil32)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nRows) return;
@@ -684,21 +864,25 @@ __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, 
     double sum = 0.;
     for (int32_t k = rowPtr[r]; k < rowPtr[r + 1]; ++k) sum += val[k] * x[col[k]];
     y[r] = sum;
-    if (yil) yil[(size_t)blockIdx.y * ilw * xStride + ilw * (r % xStride) + r / xStride] = sum;
+    const size_t o = (size_t)blockIdx.y * ilw * xStride + ilw * (r % xStride) + r / xStride;
+    if (yil) yil[o] = sum;
+    if (yil32) yil32[o] = (float)sum;
 }
 
 // E from phi by row class (e2nKind): the interior pattern (c phi[node - o], -c phi[node + o]),
 // summed as the CSR rows are (0 + first product, then the second); empty rows give 0; listed
 // rows are left to k_e_listed.  One thread per node, blockIdx.y the run; also into the
-// interleaved copy yil ((Ex, Ey) per node) when given.
+// interleaved copy yil ((Ex, Ey) per node) when given, and into its FP32 form yil32 when given.
 __global__ void k_e_struct(uint32_t np, uint32_t nx, const uint8_t *__restrict__ kind, double cx, double cy,
-                           const double *__restrict__ phi, double *__restrict__ E, double *__restrict__ yil)
+                           const double *__restrict__ phi, double *__restrict__ E, double *__restrict__ yil,
+                           float *__restrict__ yil32)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= np) return;
     phi += (size_t)blockIdx.y * np;
     E += (size_t)blockIdx.y * 2 * np;
     if (yil) yil += (size_t)blockIdx.y * 2 * np + (size_t)2 * r;
+    if (yil32) yil32 += (size_t)blockIdx.y * 2 * np + (size_t)2 * r;
     const uint32_t k = kind[r], kx = k & 3u, ky = k >> 2;
     double ex = 0., ey = 0.;
     if (kx == 1u) {
@@ -720,13 +904,21 @@ __global__ void k_e_struct(uint32_t np, uint32_t nx, const uint8_t *__restrict__
             if (ky != 2u) yil[1] = ey;
         }
     }
+    if (yil32) {
+        if (kx != 2u && ky != 2u) {
+            *reinterpret_cast<float2 *>(yil32) = make_float2((float)ex, (float)ey);   // one 8-byte store
+        } else {
+            if (kx != 2u) yil32[0] = (float)ex;
+            if (ky != 2u) yil32[1] = (float)ey;
+        }
+    }
 }
 
 // The listed rows of phiToE from the CSR (one thread per listed row, blockIdx.y the run).
 __global__ void k_e_listed(uint32_t nList, const int32_t *__restrict__ rows, const int32_t *__restrict__ rowPtr,
                            const int32_t *__restrict__ col, const double *__restrict__ val,
                            const double *__restrict__ phi, uint32_t np, double *__restrict__ E,
-                           double *__restrict__ yil, uint32_t ilw)
+                           double *__restrict__ yil, uint32_t ilw, float *__restrict__ yil32)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nList) return;
@@ -735,10 +927,43 @@ __global__ void k_e_listed(uint32_t nList, const int32_t *__restrict__ rows, con
     double sum = 0.;
     for (int32_t k = rowPtr[r]; k < rowPtr[r + 1]; ++k) sum += val[k] * phi[col[k]];
     E[(size_t)blockIdx.y * 2 * np + r] = sum;
-    if (yil) yil[(size_t)blockIdx.y * ilw * np + ilw * (r % np) + r / np] = sum;
+    const size_t o = (size_t)blockIdx.y * ilw * np + ilw * (r % np) + r / np;
+    if (yil) yil[o] = sum;
+    if (yil32) yil32[o] = (float)sum;
 }
 
 }  // namespace
+
+// A float no larger than v (v >= 0 here: the limits of the wall margin).
+static float float_down(double v)
+{
+    float f = (float)v;
+    if ((double)f > v) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+// The mixed push's wall margin of m cells (NearWall); a margin that leaves no room gives limits
+// that no position meets, so every particle takes the FP64 path.
+static NearWall near_wall(const WallDev &w, double m)
+{
+    NearWall nw{};
+    nw.xlim = nw.ylim = -1.f;
+    nw.gmax = -1.f;
+    if (w.kind == OECL_WALL_RECT) {
+        const double xl = w.lxHalf - m * w.dx, yl = w.lyHalf - m * w.dy;
+        if (xl > 0. && yl > 0.) {
+            nw.xlim = float_down(xl);
+            nw.ylim = float_down(yl);
+        }
+    } else if (w.kind == OECL_WALL_ELLIP) {
+        const double r = 1. - m * fmax(w.dx, w.dy) / fmin(0.5 * w.lx, 0.5 * w.ly);
+        nw.ax = (float)w.ax;
+        nw.ay = (float)w.ay;
+        // FP32 evaluation of x^2 ax + y^2 ay is within a few 1e-7 of its value: keep 1e-5 clear
+        if (r > 0.) nw.gmax = float_down(r * r * (1. - 1e-5));
+    }
+    return nw;
+}
 
 // Queues one step of every run.  R = runs.R; absTotal = rows of the per-hit arrays.
 template <typename T>
@@ -767,25 +992,51 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
         rc = fastpoisson_solve_box(fp, a->phi, R, s);
         if (rc) return rc;
     }
-    // grid field with Bz = 0: B gathered from the (Bx, By) layout, E from Eil
-    const bool eb = a->typeB == 2 && a->Bil && a->Eil;
-    double *il = a->Eil;
+    // mixed push (FP32 particles): E into its FP32 interleaved copy, the FP64 interleaved copy
+    // is not used
+    const bool mixed = sizeof(T) == 4 && a->pushArith == 1;
+    // grid field with Bz = 0: B gathered from the (Bx, By) layout, E from Eil (or Eil32)
+    const bool eb = a->typeB == 2 && a->Bil && (mixed ? a->Bil32 != nullptr : a->Eil != nullptr);
+    double *il = mixed ? nullptr : a->Eil;
+    float *il32 = mixed ? a->Eil32 : nullptr;
     const uint32_t ilw = 2u;
     if (a->e2nKind) {
         hipLaunchKernelGGL(k_e_struct, dim3(oecl_blocks(np, 256), R), dim3(256), 0, s, np, g->nx, a->e2nKind,
-                           a->e2nCx, a->e2nCy, a->phi, a->E, il);
+                           a->e2nCx, a->e2nCy, a->phi, a->E, il, il32);
         if (a->nIrr)
             hipLaunchKernelGGL(k_e_listed, dim3(oecl_blocks(a->nIrr, 256), R), dim3(256), 0, s, a->nIrr, a->e2nIrr,
-                               a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw);
+                               a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw, il32);
     } else {
         hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
-                           a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw);
+                           a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw, il32);
     }
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
     const double dxi = 1. / g->dx, dyi = 1. / g->dy;
     const dim3 gp((unsigned)nBlk), bp(PT);
     if (a->evPushBegin) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushBegin, s), "event");
-    if (a->typeB == 0)
+    if (mixed) {
+        const NearWall nw = near_wall(wd, a->nearCells);
+        const Push32 P32{(float)dxi, (float)dyi, (float)(g->lxHalfExt * dxi), (float)(g->lyHalfExt * dyi),
+                         (float)a->dt, (float)kk, (float)(kk * a->b0), (float)(kk * a->b1), (float)(kk * a->b2)};
+        Soa<float> pf{(float *)a->data, a->stride};
+        const dim3 gm((unsigned)nBlk), bm(MPT);
+        if (a->typeB == 0)
+            hipLaunchKernelGGL((k_push_mixed<0>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
+                               a->bField, nullptr, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
+                               list, misc, nzList);
+        else if (a->typeB == 1)
+            hipLaunchKernelGGL((k_push_mixed<1>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
+                               a->bField, nullptr, a->b0, a->b1, a->b2, a->dt, kk, a->qe, dxi, dyi, a->isInside,
+                               a->rho, counts, list, misc, nzList);
+        else if (eb)
+            hipLaunchKernelGGL((k_push_mixed<3>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
+                               a->Bil, a->Bil32, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
+                               list, misc, nzList);
+        else
+            hipLaunchKernelGGL((k_push_mixed<2>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
+                               a->bField, nullptr, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
+                               list, misc, nzList);
+    } else if (a->typeB == 0)
         hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
                            a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
     else if (a->typeB == 1)
@@ -824,7 +1075,7 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
 
 static int step_check(const oecl_grid_t *g, const oecl_wall_t *wall, oecl_fastpoisson *fp, const oecl_step_args *a)
 {
-    if (!g || !wall || !fp || !a || !a->data || !a->nDev || !a->isInside || !a->rho || !a->q || !a->phi || !a->E ||
+    if (!g || !wall || !fp || !a || !a->data || !a->nDev || !a->rho || !a->q || !a->phi || !a->E ||
         !a->e2nRowPtr || !a->e2nCol || !a->e2nVal || !a->absorbed || !a->normals || !a->remaining || !a->nSec ||
         !a->secOff || !a->impact || !a->stepDev || !a->stats || !a->work || a->absCap == 0)
         return OECL_ERR_ARG;
@@ -838,6 +1089,22 @@ static int step_check(const oecl_grid_t *g, const oecl_wall_t *wall, oecl_fastpo
         return OECL_ERR_ARG;
     }
     if (a->dtype != OECL_F64 && a->dtype != OECL_F32) return OECL_ERR_ARG;
+    if (a->pushArith != 0 && a->pushArith != 1) return OECL_ERR_ARG;
+    if (a->dtype == OECL_F32 && a->pushArith == 1) {
+        if (!a->Eil32 || !(a->nearCells >= 1.)) {
+            oecl_set_error_msg("step: the mixed push needs Eil32 and nearCells >= 1");
+            return OECL_ERR_ARG;
+        }
+        if (a->typeB == 2 && a->Bil && !a->Bil32) {
+            oecl_set_error_msg("step: the mixed push needs Bil32 with Bil");
+            return OECL_ERR_ARG;
+        }
+        // the mixed push addresses a particle component and a grid field copy by 32-bit byte offsets
+        if (a->stride * 4u > 0xFFFFFFFFull || (uint64_t)g->np * 8u > 0xFFFFFFFFull) {
+            oecl_set_error_msg("step: the mixed push needs capacity * 4 and npExt * 8 below 2^32 bytes");
+            return OECL_ERR_ARG;
+        }
+    }
     if (g->nx >= 0xFFFFu || g->ny >= 0xFFFFu) {   // the push's deposit packs cell indices in 16 bits
         oecl_set_error_msg("step: nxExt and nyExt must be below 65535");
         return OECL_ERR_ARG;

@@ -22,7 +22,8 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .buildup import StepArgs, ST_ABS_OVERFLOW, ST_CAP_OVERFLOW, ST_SEC_OVERFLOW, b_layout, set_e2n_classes
+from .buildup import (StepArgs, ST_ABS_OVERFLOW, ST_CAP_OVERFLOW, ST_SEC_OVERFLOW, b_layout, push_arithmetic,
+                      set_e2n_classes, set_push_fields)
 
 PB = 256
 
@@ -63,7 +64,8 @@ class BuildupScan:
     are copied into the scan's block at construction; read them back with run_particles(r).
     """
 
-    def __init__(self, buildups, seeds, growth=2.0, absFrac=1. / 16, maxSteps=100000, slack=1024):
+    def __init__(self, buildups, seeds, growth=2.0, absFrac=1. / 16, maxSteps=100000, slack=1024,
+                 pushArithmetic=None, nearCells=2.):
         if len(buildups) != len(seeds) or not buildups:
             raise ValueError('one seed per buildup')
         b0 = buildups[0]
@@ -85,6 +87,8 @@ class BuildupScan:
         self.g, self.ps, self.pb = b0.g, b0.ps, b0.pb
         self.dt, self.typeB = b0.dt, b0.typeB
         self.dtype = b0.p.dtype
+        self.pushArithmetic = push_arithmetic(self.dtype, pushArithmetic)
+        self._nearCells = float(nearCells)
         self.code = b0.p.code
         self.qe = b0.p.particleCharge
         self.qm = b0.p.particleCharge / b0.p.particleMass
@@ -127,8 +131,6 @@ class BuildupScan:
         self.q = torch.zeros(R * npx, dtype=torch.float64, device=dev)
         self.phi = torch.zeros(R * npx, dtype=torch.float64, device=dev)
         self.E = torch.zeros(R * 2 * npx, dtype=torch.float64, device=dev)
-        self._eil = torch.zeros(R * 2 * npx, dtype=torch.float64, device=dev)
-        self.isInside = torch.zeros(cap, dtype=torch.uint8, device=dev)
         self.absorbed = torch.zeros((absTot, 6), dtype=self.dtype, device=dev)
         self.normals = torch.zeros((absTot, 2), dtype=self.dtype, device=dev)
         self.remaining = torch.zeros(absTot, dtype=self.dtype, device=dev)
@@ -219,7 +221,7 @@ class BuildupScan:
         a.data = self.data.data_ptr()
         a.stride = self.capacity
         a.nDev = self._nDev.data_ptr()
-        a.isInside = self.isInside.data_ptr()
+        a.isInside = None          # the batched step keeps its holes in lists; no per-slot flags
         a.rho = self.rho.data_ptr()
         a.q = self.q.data_ptr()
         a.profile = self._prof.data_ptr() if self._prof is not None else None
@@ -252,8 +254,8 @@ class BuildupScan:
         a.countHist = self._hist.data_ptr()
         a.histCap = self._nTab
         a.work = self._work.data_ptr()
-        a.Eil = self._eil.data_ptr()
         a.Bil = self._bil.data_ptr() if self._bil is not None else None
+        set_push_fields(self, a, self.R * self.np, self.pushArithmetic, self._nearCells)
         return a
 
     # ------------------------------------------------------------------ state

@@ -94,3 +94,38 @@ def test_push_block_size_matches_the_scan_layout():
     lib = L.lib()
     lib.oecl_push_block_size.restype = ctypes.c_uint32
     assert lib.oecl_push_block_size() == PB
+
+
+def _c_offsets(struct, fields, tmp_path):
+    """sizeof and offsetof(field) of a struct of include/oecl.h, from a C program built by gcc."""
+    src = tmp_path / 'off.c'
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "oecl.h"', 'int main(void) {',
+             '    printf("%%zu\\n", sizeof(%s));' % struct]
+    lines += ['    printf("%%zu\\n", offsetof(%s, %s));' % (struct, f) for f in fields]
+    lines += ['    return 0;', '}']
+    src.write_text('\n'.join(lines) + '\n')
+    exe = tmp_path / 'off'
+    subprocess.check_call(['gcc', '-I', os.path.join(ROOT, 'include'), '-o', str(exe), str(src)])
+    return [int(v) for v in subprocess.check_output([str(exe)]).split()]
+
+
+def test_step_args_mirror_matches_the_header(tmp_path):
+    """The ctypes mirror of oecl_step_args has the C struct's size and field offsets."""
+    from openecloud_amd.buildup import StepArgs
+    names = [f[0] for f in StepArgs._fields_]
+    got = _c_offsets('oecl_step_args', names, tmp_path)
+    assert got[0] == ctypes.sizeof(StepArgs)
+    assert got[1:] == [getattr(StepArgs, n).offset for n in names]
+
+
+def test_push_arithmetic_choices():
+    """'mixed' is the default for FP32 particles and refused for FP64 ones."""
+    import torch
+    from openecloud_amd.buildup import push_arithmetic
+    with pytest.raises(ValueError):
+        push_arithmetic(torch.float64, 'mixed')
+    with pytest.raises(ValueError):
+        push_arithmetic(torch.float32, 'fp16')
+    assert push_arithmetic(torch.float32, None) == 'mixed'
+    assert push_arithmetic(torch.float32, 'fp64') == 'fp64'
+    assert push_arithmetic(torch.float64, None) == 'fp64'
