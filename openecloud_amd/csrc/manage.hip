@@ -124,7 +124,6 @@ constexpr uint32_t NN_NONE = 0xFFFFFFFFu;    // no proposal (out of the pool)
 constexpr uint32_t NN_RETIRE = 0xFFFFFFFEu;  // no neighbour within the limit: leaves the pool
 constexpr uint32_t NN_MERGED = 0xFFFFFFFEu;  // lock value of a particle merged in this round
 constexpr uint32_t NN_LEAF = 8;              // points per leaf of the tree
-constexpr int NN_STACK = 64;                 // traversal stack (tree depth <= 30)
 
 __global__ void k_zero_u32(uint32_t *a, uint32_t n)
 {
@@ -272,9 +271,10 @@ __device__ __forceinline__ double box_dist2(const double *__restrict__ blo, cons
 
 // Proposal of candidate a (rank = index, lightest first): its exact nearest neighbour in the pool
 // (gsize != 0, groups of at most G particles together), ties to the lower index, found by branch
-// and bound over the tree (near child first, boxes farther than the best distance pruned); a
-// box containing particles that left the pool still bounds the pooled ones, so the search stays
-// exact.  NN_RETIRE when none lies within the squared-distance limit (particleManagement.pyx:185
+// and bound over the tree (boxes farther than the best distance pruned); a box containing
+// particles that left the pool still bounds the pooled ones, and the boxes are refitted after
+// every round in which a survivor that stays in the pool has moved (groups with scheme 1), so
+// the search stays exact.  NN_RETIRE when none lies within the squared-distance limit (particleManagement.pyx:185
 // compares the squared distance with factorDisLimit * ds).  Resets a's lock for this round.
 template <typename T>
 __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ zperm, uint32_t L,
@@ -283,8 +283,11 @@ __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
                              const uint8_t *__restrict__ gsize, uint32_t *__restrict__ prop, uint32_t *__restrict__ lock,
                              uint32_t *__restrict__ cnt)
 {
-    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nc) return;
+    // thread t searches for the t-th point in Z-order, so the lanes of a wave are neighbours in
+    // space and walk nearly the same part of the tree
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    const uint32_t a = zperm[t];
     lock[a] = NN_NONE;
     const uint32_t sa = gsize[a];
     if (sa == 0 || (int)sa >= G) {
@@ -299,41 +302,51 @@ __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
     // and the result is the same
     double best = disLimit;
     uint32_t bi = NN_NONE;
-    uint32_t stack[NN_STACK];
-    int sp = 0;
-    stack[sp++] = 1;
-    while (sp > 0) {
-        const uint32_t k = stack[--sp];
-        if (box_dist2(blo, bhi, k, q) > best) continue;
-        if (k >= L) {
-            const uint64_t base = (uint64_t)(k - L) * NN_LEAF;
-            for (uint32_t j = 0; j < NN_LEAF; ++j) {
-                const uint64_t pos = base + j;
-                if (pos >= nc) break;
-                const uint32_t b = zperm[pos];
-                if (b == a) continue;
-                const uint32_t sb = gsize[b];
-                if (sb == 0 || (int)(sa + sb) > G) continue;
-                double d2 = 0.;
+    auto scan_leaf = [&](uint32_t leaf) {
+        const uint64_t base = (uint64_t)leaf * NN_LEAF;
+        for (uint32_t j = 0; j < NN_LEAF; ++j) {
+            const uint64_t pos = base + j;
+            if (pos >= nc) break;
+            const uint32_t b = zperm[pos];
+            if (b == a) continue;
+            const uint32_t sb = gsize[b];
+            if (sb == 0 || (int)(sa + sb) > G) continue;
+            double d2 = 0.;
 #pragma unroll
-                for (int c = 0; c < 5; ++c) {
-                    const double d = (double)p(c, b) * pw[c] - q[c];
-                    d2 += d * d;
-                }
-                if (d2 < best || (d2 == best && bi != NN_NONE && b < bi)) {
-                    best = d2;
-                    bi = b;
-                }
+            for (int c = 0; c < 5; ++c) {
+                const double d = (double)p(c, b) * pw[c] - q[c];
+                d2 += d * d;
             }
-        } else {
-            const uint32_t c0 = 2 * k, c1 = 2 * k + 1;
-            const double d0 = box_dist2(blo, bhi, c0, q), d1 = box_dist2(blo, bhi, c1, q);
-            // push the far child first so that the near one is searched first
-            const uint32_t nearC = d0 <= d1 ? c0 : c1, farC = d0 <= d1 ? c1 : c0;
-            const double dn = d0 <= d1 ? d0 : d1, df = d0 <= d1 ? d1 : d0;
-            if (df <= best && sp < NN_STACK) stack[sp++] = farC;
-            if (dn <= best && sp < NN_STACK) stack[sp++] = nearC;
+            if (d2 < best || (d2 == best && bi != NN_NONE && b < bi)) {
+                best = d2;
+                bi = b;
+            }
         }
+    };
+    // the candidate's own leaf and its neighbours in Z-order first: they hold the points nearest in
+    // the ordering, which bounds the walk below tightly from the start
+    const uint32_t own = t / NN_LEAF;
+    scan_leaf(own);
+    if (own > 0) scan_leaf(own - 1);
+    if (own + 1 < L) scan_leaf(own + 1);
+    // stackless depth-first walk of the heap-ordered tree (root 1, children 2k and 2k + 1, leaves
+    // L..2L-1): descend into the left child of a box within reach; after a leaf or a box out of
+    // reach, go to the right sibling of the nearest left child on the way up (done at the root).
+    // The result, the least (distance, index) within the limit, does not depend on the order in
+    // which points are met (a point met twice changes nothing).
+    uint32_t k = 1;
+    for (;;) {
+        if (box_dist2(blo, bhi, k, q) <= best) {
+            if (k < L) {
+                k = 2 * k;
+                continue;
+            }
+            const uint32_t leaf = k - L;
+            if (leaf + 1 < own || leaf > own + 1) scan_leaf(leaf);
+        }
+        while (k & 1u) k >>= 1;
+        if (k == 0) break;
+        ++k;
     }
     if (bi == NN_NONE) {
         prop[a] = NN_RETIRE;
@@ -616,6 +629,17 @@ OECL_EXPORT int oecl_nn_merge(int dtype, void *data, uint64_t stride, uint32_t n
             hipLaunchKernelGGL(k_nn_merge<float>, gr, bl, 0, s, pf, nc, prop, acc, off, toMerge - done, G, scheme,
                                rounds, stream, s0, s1, gsize, lock, keep, merged, pairs, done);
         hipLaunchKernelGGL(k_nn_retire, gr, bl, 0, s, nc, prop, lock, gsize);
+        if (G > 2 && scheme == 1) {
+            // survivors that stay in the pool had their velocity rescaled: refit the boxes
+            if (dtype == OECL_F64)
+                hipLaunchKernelGGL(k_leaf_boxes<double>, dim3(oecl_blocks(L, 256)), bl, 0, s, pd, nc, zperm, pw, L,
+                                   blo, bhi);
+            else
+                hipLaunchKernelGGL(k_leaf_boxes<float>, dim3(oecl_blocks(L, 256)), bl, 0, s, pf, nc, zperm, pw, L,
+                                   blo, bhi);
+            for (uint32_t first = L >> 1; first >= 1; first >>= 1)
+                hipLaunchKernelGGL(k_inner_boxes, dim3(oecl_blocks(first, 256)), bl, 0, s, first, first, blo, bhi);
+        }
         OECL_CHECK_LAUNCH("oecl_nn_merge round");
         OECL_HIP(hipMemcpyAsync(&h[0], merged, 4, hipMemcpyDeviceToHost, s), "merge count");
         OECL_HIP(hipMemcpyAsync(&h[1], cnt, 4, hipMemcpyDeviceToHost, s), "merge count");

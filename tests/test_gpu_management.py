@@ -148,6 +148,30 @@ def test_local_random_groups_reach_target(group):
     assert np.bincount(np.concatenate([pr[:, 0], pr[:, 1]])).max() <= group - 1
 
 
+@pytest.mark.parametrize('seed', [21, 22, 23])
+def test_local_random_first_merge_is_the_exact_nearest_neighbour(seed):
+    """One merge: the lightest candidate's proposal is always accepted, so the merged pair is
+    that particle and its exact nearest neighbour among the other candidates in the reference's
+    weighted 5-D metric (brute force over all of them, ties to the lower index)."""
+    from openecloud_amd import randomGen
+    from openecloud_amd.particleManagement import localRandom
+    randomGen.seed(seed)
+    n = 20000
+    g, p, pb, pd, ps = _sorted_case(n, seed, torch.float64)
+    pairs = torch.full((2 * n,), -1, dtype=torch.int32, device='cuda')
+    info = localRandom(g, p, pb, n - 1, pairs=pairs)
+    if info['toMerge'] < 1:
+        pytest.skip('no merge requested for this set')
+    nc = n - info['toSplit']
+    pw, ds = _weights(g, info)
+    pr = pairs[:2 * info['merged']].cpu().numpy().reshape(-1, 2).astype(np.int64)
+    assert info['merged'] == info['toMerge']
+    d2 = (((ps[1:nc, :5] - ps[0, :5]) * pw) ** 2).sum(1)
+    j = int(np.argmin(d2)) + 1
+    assert d2[j - 1] < ds
+    assert sorted(pr[0].tolist()) == [0, j], (pr[0], j)
+
+
 def test_local_random_against_serial_greedy_merge():
     """The same weight-sorted particles through the device merge and the oracle's restatement of
     the reference's serial kd-tree loop: both make the requested merges, with merged-pair
