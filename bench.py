@@ -33,8 +33,10 @@ WORKLOADS = ('full', 'dipole', 'scan', 'learned', 'northstar', 'push')
 def profile_traffic(name):
     """HBM bytes per push launch from a committed counter summary under profiles/ (the line
     'traffic_bytes_per_launch: <bytes>'), or None when there is none for this workload."""
+    if not name:
+        return None
     path = os.path.join(ROOT, 'profiles', name)
-    if not os.path.exists(path):
+    if not os.path.isfile(path):
         return None
     for line in open(path):
         if line.startswith('traffic_bytes_per_launch:'):
@@ -53,7 +55,7 @@ def push_model(dtype, arith, gridB):
     node = fe + (fe if gridB else 0) + 16
     kern = 'k_push_mixed' if mixed else 'k_push_dev'
     desc = ('%d B per %s macro-particle + %d B per grid node (E%s %s, rho FP64 read and write)'
-            % (11 * s, dtype.upper(), node, ', B' if gridB else '', 'FP32' if mixed else 'FP64'))
+            % (11 * s, 'FP64' if dtype == 'f64' else 'FP32', node, ', B' if gridB else '', 'FP32' if mixed else 'FP64'))
     return kern, 11 * s, node, desc
 
 

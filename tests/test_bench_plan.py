@@ -70,3 +70,19 @@ def test_more_than_one_gpu_exits_before_any_gpu_work():
     r = subprocess.run([sys.executable, '-c', code], capture_output=True, text=True, timeout=120)
     assert 'rc 2 False' in r.stdout, r.stdout + r.stderr
     assert 'one GPU only' in r.stderr
+
+
+def test_traffic_profiles_resolve():
+    """Every traffic profile the bench names exists and holds its per-launch bytes; a workload
+    without one gets None (not an error)."""
+    assert bench.profile_traffic('') is None
+    assert bench.profile_traffic('no_such_profile.md') is None
+    for key, name in bench.TRAFFIC_PROFILES.items():
+        v = bench.profile_traffic(name)
+        assert v is not None and v > 1e6, (key, name)
+    k, bpp, bpn, _ = bench.push_model('f32', 'mixed', True)
+    assert (k, bpp, bpn) == ('k_push_mixed', 44, 32)
+    k, bpp, bpn, _ = bench.push_model('f32', 'fp64', True)
+    assert (k, bpp, bpn) == ('k_push_dev', 44, 48)
+    k, bpp, bpn, _ = bench.push_model('f64', None, False)
+    assert (k, bpp, bpn) == ('k_push_dev', 88, 32)

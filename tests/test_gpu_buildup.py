@@ -112,7 +112,8 @@ def test_buildup_matches_oracle_fp64(orc, fused, ordered):
 @pytest.mark.parametrize('dtype', ['f64', 'f32'])
 def test_native_step_matches_oracle(orc, dtype):
     """The one-call native step (device-resident counts) against the oracle loop with the
-    same hole-filling order; FP64 to 1e-12 of each column's scale, FP32 to 1e-4."""
+    same hole-filling order; FP64 to 1e-12 of each column's scale; FP32 particles (the mixed
+    push) to 1e-5 after 10 steps on the particles that no compaction has moved."""
     from openecloud_amd import randomGen
     from openecloud_amd.beam import GaussianBunchTrain
     from openecloud_amd.buildup import Buildup
@@ -163,9 +164,12 @@ def test_native_step_matches_oracle(orc, dtype):
         m = min(got.shape[0], ref[-1].shape[0])
         # compare the first particles, which no compaction has moved in either run
         idx = np.nonzero(np.all(np.abs(got[:2000, :2] - ref[-1][:2000, :2]) < 1e-6, axis=1))[0]
-        assert idx.size > 1500
+        assert idx.size > 1900
         scale = np.abs(ref[-1]).max(axis=0)
-        assert (np.abs(got[idx] - ref[-1][idx]) / scale).max() < 1e-4
+        err = (np.abs(got[idx] - ref[-1][idx]) / scale).max(axis=0)
+        print('FP32 particles, %s push, after %d steps: %d of the first 2000 in place, max deviation per column %s'
+              % (b.pushArithmetic, nsteps, idx.size, err), flush=True)
+        assert err.max() < 1e-5
 
 
 @pytest.mark.parametrize('native', [False, True])
