@@ -932,6 +932,16 @@ def main_scan(a):
     for _ in range(a.warmup):
         scan.step()
     scan.sync()
+    if not a.scan_config:
+        # tune where the timed window runs: the state at step `start` (multipacting under way),
+        # reached with a sort every 25 steps
+        scan.capture(25)
+        while scan.it + 25 <= start:
+            scan.sort()
+            scan.replay()
+        while scan.it < start:
+            scan.step()
+        scan.sync()
     if a.scan_config:
         se_, o_, b_ = [int(v) for v in a.scan_config.split(',')]
         best, results = dict(sortEvery=se_, order=o_, binShift=b_, graphSteps=se_), []
@@ -945,6 +955,7 @@ def main_scan(a):
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     g, ps, runs, scan, prm = build_scan(R, a.dtype, maxSteps, a.push_arith)
+    scan.set_deposit(int(best.get('span', 0)), bool(best.get('waveTiles', False)))
     scan.sort(best['order'], best['binShift'])
     for _ in range(a.warmup):
         scan.step()

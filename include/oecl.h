@@ -547,6 +547,12 @@ typedef struct {
                                    FP32 (E itself stays FP64) */
     const float *Bil32;         /* [2 npExt] or NULL: Bil in FP32, needed with Bil for pushArith 1 */
     double nearCells;           /* pushArith 1: the wall margin in cells (>= 1) */
+    /* The push's deposit tiles (no reference counterpart; a tuning choice that changes only the
+     * summation order of rho): depSpan, lanes over which runs of equal cells are summed before
+     * the tile (0: 4; else a power of two, 2-64); depTile 0 one LDS tile per push block (per-wave
+     * tiles when the block's cells do not fit it), 1 per-wave tiles only. */
+    uint32_t depSpan;
+    int32_t depTile;
 } oecl_step_args;
 
 /* Particles per push block (256): batched run segments are multiples of it, and the per-block

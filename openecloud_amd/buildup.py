@@ -43,7 +43,7 @@ class StepArgs(ctypes.Structure):
                 ('secem', _VP), ('Eil', _VP), ('externalPhi', ctypes.c_int32), ('Bil', _VP),
                 ('e2nKind', _VP), ('e2nIrr', _VP), ('nIrr', ctypes.c_uint32), ('e2nCx', ctypes.c_double),
                 ('e2nCy', ctypes.c_double), ('pushArith', ctypes.c_int32), ('Eil32', _VP), ('Bil32', _VP),
-                ('nearCells', ctypes.c_double)]
+                ('nearCells', ctypes.c_double), ('depSpan', ctypes.c_uint32), ('depTile', ctypes.c_int32)]
 
 
 PUSH_ARITHMETIC = ('fp64', 'mixed')

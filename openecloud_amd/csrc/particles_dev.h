@@ -114,11 +114,13 @@ __device__ __forceinline__ void seg_scan_step(uint32_t &f, double &q0, double &q
 #ifndef OECL_RUNSUM_SPAN
 #define OECL_RUNSUM_SPAN 4
 #endif
-__device__ __forceinline__ bool wave_run_sum(uint32_t c, double &q0, double &q1, double &q2, double &q3)
+// S (a power of two, 2-64, the same in all lanes of the wave) may be a run-time value: the scan
+// steps it needs are then taken by wave-uniform branches.
+__device__ __forceinline__ bool wave_run_sum(uint32_t c, double &q0, double &q1, double &q2, double &q3,
+                                             uint32_t S = OECL_RUNSUM_SPAN)
 {
     // runs are cut at aligned groups of S lanes (S = 64: the whole wave)
-    constexpr int S = OECL_RUNSUM_SPAN;
-    const int lane = threadIdx.x & 63;
+    const uint32_t lane = threadIdx.x & 63;
     // the neighbours' cells: within a row of 16 lanes by DPP row shifts when the groups do not
     // cross rows (a row's first lane starts a group, its last lane ends one, so the 0 that
     // bound_ctrl gives them is never used), else by cross-lane shuffles
@@ -130,8 +132,8 @@ __device__ __forceinline__ bool wave_run_sum(uint32_t c, double &q0, double &q1,
         cp = __shfl_up(c, 1, 64);
         cn = __shfl_down(c, 1, 64);
     }
-    uint32_t f = (lane % S == 0 || cp != c) ? 1u : 0u;
-    const bool last = lane % S == S - 1 || cn != c;
+    uint32_t f = ((lane & (S - 1)) == 0 || cp != c) ? 1u : 0u;
+    const bool last = (lane & (S - 1)) == S - 1 || cn != c;
     seg_scan_step<0x111, 0xF>(f, q0, q1, q2, q3);                  // row_shr:1
     if (S > 2) seg_scan_step<0x112, 0xF>(f, q0, q1, q2, q3);       // row_shr:2
     if (S > 4) seg_scan_step<0x114, 0xF>(f, q0, q1, q2, q3);       // row_shr:4
@@ -255,17 +257,25 @@ __device__ __forceinline__ void wave_deposit(double *__restrict__ rho, uint32_t 
 // bounds the push (adding the run totals to rho directly made it 3.4x slower).  A block whose box
 // is larger uses per-wave tiles in the same LDS (WN nodes each), as wave_deposit does.  tile:
 // NW * WN doubles of LDS; box: 4 words of LDS.
+// The deposit's tile choices, the same for the whole launch: span, the run-sum span of
+// wave_run_sum; waveTiles, per-wave tiles only (no block tile, no block barrier).
+struct DepCfg {
+    uint32_t span;
+    uint32_t waveTiles;
+};
+
 template <int NW, int WN>
 __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t nx, bool valid, uint32_t ix,
                                               uint32_t iy, double q00, double q10, double q01, double q11,
-                                              double *tile, uint32_t *box)
+                                              double *tile, uint32_t *box,
+                                              DepCfg dc = DepCfg{OECL_RUNSUM_SPAN, 0u})
 {
     static_assert(NW * WN <= 2048, "tile_to_rho handles tiles of at most 2048 nodes");
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint32_t c = valid ? ix + iy * nx : 0xFFFFFFFFu;
     if (!valid) q00 = q10 = q01 = q11 = 0.;
 #if OECL_DEPOSIT_RUN_SUMS
-    const bool head = wave_run_sum(c, q00, q10, q01, q11);
+    const bool head = wave_run_sum(c, q00, q10, q01, q11, dc.span);
 #else
     const bool head = valid;
 #endif
@@ -274,37 +284,39 @@ __device__ __forceinline__ void block_deposit(double *__restrict__ rho, uint32_t
     const uint32_t mn = wave_minmax2x16<true>(head ? xy : 0xFFFFFFFFu), mx = wave_minmax2x16<false>(head ? xy : 0u);
     const uint32_t X0 = mn == 0xFFFFFFFFu ? 0xFFFFFFFFu : mn & 0xFFFFu, X1 = mx & 0xFFFFu;
     const uint32_t Y0 = mn >> 16, Y1 = mx >> 16;
-    if (threadIdx.x == 0) {
-        box[0] = 0xFFFFFFFFu;
-        box[1] = 0u;
-        box[2] = 0xFFFFFFFFu;
-        box[3] = 0u;
-    }
-    __syncthreads();
-    if (lane == 0 && X0 != 0xFFFFFFFFu) {
-        atomicMin(&box[0], X0);
-        atomicMax(&box[1], X1);
-        atomicMin(&box[2], Y0);
-        atomicMax(&box[3], Y1);
-    }
-    __syncthreads();
-    const uint32_t B0 = box[0], B1 = box[1], C0 = box[2], C1 = box[3];
-    if (B0 == 0xFFFFFFFFu) return;  // nothing to deposit in this block (block-uniform)
-    const uint32_t bw = B1 - B0 + 2, bh = C1 - C0 + 2;
-    if ((uint64_t)bw * bh <= (uint64_t)NW * WN) {  // block-uniform
-        const uint32_t nb = bw * bh;
-        for (uint32_t k = threadIdx.x; k < nb; k += NW * 64) tile[k] = 0.;
-        __syncthreads();
-        if (head) {
-            const uint32_t o = (ix - B0) + (iy - C0) * bw;
-            atomicAdd(&tile[o], q00);
-            atomicAdd(&tile[o + 1], q10);
-            atomicAdd(&tile[o + bw], q01);
-            atomicAdd(&tile[o + bw + 1], q11);
+    if (!dc.waveTiles) {   // launch-uniform
+        if (threadIdx.x == 0) {
+            box[0] = 0xFFFFFFFFu;
+            box[1] = 0u;
+            box[2] = 0xFFFFFFFFu;
+            box[3] = 0u;
         }
         __syncthreads();
-        tile_to_rho(rho, nx, tile, nb, bw, B0, C0, threadIdx.x, NW * 64);
-        return;
+        if (lane == 0 && X0 != 0xFFFFFFFFu) {
+            atomicMin(&box[0], X0);
+            atomicMax(&box[1], X1);
+            atomicMin(&box[2], Y0);
+            atomicMax(&box[3], Y1);
+        }
+        __syncthreads();
+        const uint32_t B0 = box[0], B1 = box[1], C0 = box[2], C1 = box[3];
+        if (B0 == 0xFFFFFFFFu) return;  // nothing to deposit in this block (block-uniform)
+        const uint32_t bw = B1 - B0 + 2, bh = C1 - C0 + 2;
+        if ((uint64_t)bw * bh <= (uint64_t)NW * WN) {  // block-uniform
+            const uint32_t nb = bw * bh;
+            for (uint32_t k = threadIdx.x; k < nb; k += NW * 64) tile[k] = 0.;
+            __syncthreads();
+            if (head) {
+                const uint32_t o = (ix - B0) + (iy - C0) * bw;
+                atomicAdd(&tile[o], q00);
+                atomicAdd(&tile[o + 1], q10);
+                atomicAdd(&tile[o + bw], q01);
+                atomicAdd(&tile[o + bw + 1], q11);
+            }
+            __syncthreads();
+            tile_to_rho(rho, nx, tile, nb, bw, B0, C0, threadIdx.x, NW * 64);
+            return;
+        }
     }
     if (X0 == 0xFFFFFFFFu) return;  // wave-uniform
     double *win = tile + wv * WN;

@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(PT)
                const double *__restrict__ E, const double *__restrict__ Eil, const double *__restrict__ B, double b0, double b1,
                double b2, double dt, double k, double q, double dxi, double dyi, uint8_t *__restrict__ isInside,
                double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
-               uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
+               uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList, DepCfg dc)
 {
     __shared__ double tiles[PB / 64 * 512];
     __shared__ uint32_t dbox[4];
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(PT)
         if (kk) __syncthreads();   // the previous deposit's readers of the tile and box are done
         block_deposit<PT / 64, 512 * KP>(rho, g.nx, dep[kk], w2[kk].ix, w2[kk].iy, w2[kk].wLL * wq[kk] * q,
                                          w2[kk].wUL * wq[kk] * q, w2[kk].wLU * wq[kk] * q, w2[kk].wUU * wq[kk] * q,
-                                         tiles, dbox);
+                                         tiles, dbox, dc);
     }
 }
 
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(MPT)
                  const double *__restrict__ B, const float *__restrict__ B32, double b0, double b1, double b2,
                  double dt, double k, double q, double dxi, double dyi, uint8_t *__restrict__ isInside,
                  double *__restrict__ rho, uint32_t *__restrict__ counts, uint32_t *__restrict__ list,
-                 uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList)
+                 uint32_t *__restrict__ misc, uint32_t *__restrict__ nzList, DepCfg dc)
 {
     __shared__ double tiles[PB / 64 * 512];
     __shared__ uint32_t dbox[4];
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(MPT)
         wave_holes(o[kk].dead, i, blk * (PB / 64) + kk * (MPT / 64) + (threadIdx.x >> 6), counts, list, misc, nzList);
         if (kk) __syncthreads();   // the previous deposit's readers of the tile and box are done
         block_deposit<MPT / 64, 512 * MKP>(rho, g.nx, o[kk].dep, o[kk].cx, o[kk].cy, o[kk].q00, o[kk].q10, o[kk].q01,
-                                           o[kk].q11, tiles, dbox);
+                                           o[kk].q11, tiles, dbox, dc);
     }
 }
 
@@ -1013,6 +1013,7 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     double kk = a->typeB == 0 ? a->qm * a->dt : 0.5 * a->qm * a->dt;
     const double dxi = 1. / g->dx, dyi = 1. / g->dy;
     const dim3 gp((unsigned)nBlk), bp(PT);
+    const DepCfg dc{a->depSpan ? a->depSpan : (uint32_t)OECL_RUNSUM_SPAN, a->depTile ? 1u : 0u};
     if (a->evPushBegin) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushBegin, s), "event");
     if (mixed) {
         const NearWall nw = near_wall(wd, a->nearCells);
@@ -1023,31 +1024,31 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
         if (a->typeB == 0)
             hipLaunchKernelGGL((k_push_mixed<0>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
                                a->bField, nullptr, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
-                               list, misc, nzList);
+                               list, misc, nzList, dc);
         else if (a->typeB == 1)
             hipLaunchKernelGGL((k_push_mixed<1>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
                                a->bField, nullptr, a->b0, a->b1, a->b2, a->dt, kk, a->qe, dxi, dyi, a->isInside,
-                               a->rho, counts, list, misc, nzList);
+                               a->rho, counts, list, misc, nzList, dc);
         else if (eb)
             hipLaunchKernelGGL((k_push_mixed<3>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
                                a->Bil, a->Bil32, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
-                               list, misc, nzList);
+                               list, misc, nzList, dc);
         else
             hipLaunchKernelGGL((k_push_mixed<2>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
                                a->bField, nullptr, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
-                               list, misc, nzList);
+                               list, misc, nzList, dc);
     } else if (a->typeB == 0)
         hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
-                           a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
+                           a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList, dc);
     else if (a->typeB == 1)
         hipLaunchKernelGGL((k_push_dev<T, 1>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, a->b0,
-                           a->b1, a->b2, a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
+                           a->b1, a->b2, a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList, dc);
     else if (eb)
         hipLaunchKernelGGL((k_push_dev<T, 3>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->Bil, 0., 0.,
-                           0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
+                           0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList, dc);
     else
         hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
-                           a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList);
+                           a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList, dc);
     if (a->evPushEnd) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushEnd, s), "event");
     hipLaunchKernelGGL(k_holes_collect<T>, dim3(R), dim3(CT), 0, s, runs, p, a->nDev, counts, list, nzList, holePos,
                        (T *)a->absorbed, misc);
@@ -1090,6 +1091,11 @@ static int step_check(const oecl_grid_t *g, const oecl_wall_t *wall, oecl_fastpo
     }
     if (a->dtype != OECL_F64 && a->dtype != OECL_F32) return OECL_ERR_ARG;
     if (a->pushArith != 0 && a->pushArith != 1) return OECL_ERR_ARG;
+    if ((a->depSpan != 0 && (a->depSpan < 2 || a->depSpan > 64 || (a->depSpan & (a->depSpan - 1)))) ||
+        (a->depTile != 0 && a->depTile != 1)) {
+        oecl_set_error_msg("step: depSpan must be 0 or a power of two in 2..64, depTile 0 or 1");
+        return OECL_ERR_ARG;
+    }
     if (a->dtype == OECL_F32 && a->pushArith == 1) {
         if (!a->Eil32 || !(a->nearCells >= 1.)) {
             oecl_set_error_msg("step: the mixed push needs Eil32 and nearCells >= 1");

@@ -315,6 +315,19 @@ class BuildupScan:
                                             L.stream()), 'scan step')
         self.it += 1
 
+    def set_deposit(self, span=0, waveTiles=False):
+        """The push's deposit tiles (oecl_step_args.depSpan, depTile): span, the lanes over which
+        runs of equal cells are summed (0: the library's default of 4; else a power of two,
+        2-64); waveTiles, per-wave LDS tiles only instead of one tile per push block.  Changes
+        only the summation order of rho.  A captured graph is dropped (it holds the old
+        arguments)."""
+        span = int(span)
+        if span and (span < 2 or span > 64 or span & (span - 1)):
+            raise ValueError('span must be 0 or a power of two in 2..64')
+        self._args.depSpan = span
+        self._args.depTile = 1 if waveTiles else 0
+        self._graph = None
+
     def capture(self, nSteps):
         """Records nSteps steps of every run into one HIP graph; replay() runs the next
         nSteps.  The scan's buffers keep their addresses (sort() copies back)."""

@@ -174,3 +174,45 @@ def test_scan_with_learned_solve_follows_the_direct_scan():
     assert st['judge_max_rel_err'] < 1e-9
     for r in range(3):
         assert _close(lrn.run_particles(r).cpu().numpy(), ref.run_particles(r).cpu().numpy(), 1e-9), r
+
+
+@pytest.mark.parametrize('span,waveTiles', [(16, False), (64, False), (4, True), (2, True)])
+def test_scan_deposit_tiles_change_only_the_summation_order(span, waveTiles):
+    """The autotuner's deposit tile choices (run-sum span, per-wave tiles): every run of the scan
+    follows the default-tile scan to 1e-12 (FP64) over 8 steps, with the same counts."""
+    g, ps = _shared()
+    ref = _scan(g, ps)
+    sc = _scan(g, ps)
+    sc.set_deposit(span, waveTiles)
+    for it in range(8):
+        ref.step()
+        sc.step()
+    st, st0 = sc.sync(), ref.sync()
+    assert st['counts'] == st0['counts']
+    for r in range(3):
+        assert _close(sc.run_particles(r).cpu().numpy(), ref.run_particles(r).cpu().numpy(), 1e-12), r
+    with pytest.raises(ValueError):
+        sc.set_deposit(3)
+
+
+def test_autotuner_picks_a_sort_interval_and_tiles():
+    """tune_scan on a small scan: both stages are timed, the choice is one of the candidates and
+    the scan is left set to its tiles; the cache returns the same choice without timing."""
+    import os
+    import tempfile
+    from openecloud_amd.autotune import tune_scan
+    g, ps = _shared(torch.float32)
+    sc = _scan(g, ps, torch.float32)
+    cands = [dict(sortEvery=2, order=0, binShift=0), dict(sortEvery=3, order=1, binShift=1)]
+    tiles = [dict(span=4, waveTiles=False), dict(span=16, waveTiles=True)]
+    with tempfile.TemporaryDirectory() as d:
+        cache = os.path.join(d, 'tune.json')
+        best, results = tune_scan(sc, candidates=cands, rounds=1, cacheFile=cache, tiles=tiles)
+        assert len(results) == 4
+        assert best['sortEvery'] in (2, 3) and best['graphSteps'] == best['sortEvery']
+        assert (best['span'], best['waveTiles']) in [(4, False), (16, True)]
+        assert (int(sc._args.depSpan), bool(sc._args.depTile)) == (best['span'], best['waveTiles'])
+        sc.set_deposit(0, False)
+        again, res2 = tune_scan(sc, candidates=cands, rounds=1, cacheFile=cache, tiles=tiles)
+        assert res2 == [] and again == best
+        assert (int(sc._args.depSpan), bool(sc._args.depTile)) == (best['span'], best['waveTiles'])
