@@ -121,9 +121,13 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
     else box.b[(ix - 2) + (iy - 2) * box.bx] = box.mask[i] ? qv : 0.;
 }
 
-// The push maps hardware block b to logical block xcd_block(b, n) (common.h): every XCD works
-// through whole chunks of consecutive blocks, whose particles share a few rows of cells after a
-// cell sort, so that XCD's L2 holds that part of E and rho.
+// The push maps hardware block b to logical block xcd_block_chunk(b, n, PUSH_XCD_CHUNK)
+// (common.h): every XCD works through whole chunks of consecutive blocks, whose particles share a
+// few rows of cells after a cell sort, so that XCD's L2 holds that part of E and rho.
+#ifndef OECL_PUSH_XCD_CHUNK
+#define OECL_PUSH_XCD_CHUNK 32
+#endif
+constexpr uint32_t PUSH_XCD_CHUNK = OECL_PUSH_XCD_CHUNK;
 
 // Holes (absorbed particles) of one wave in index order: the wave's count goes to
 // counts[w], the indices to list[w * 64 + rank] (w = global index of the 64-particle group the
@@ -164,7 +168,7 @@ __global__ void __launch_bounds__(PT)
 {
     __shared__ double tiles[PB / 64 * 512];
     __shared__ uint32_t dbox[4];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t blk = xcd_block_chunk(blockIdx.x, gridDim.x, PUSH_XCD_CHUNK);
     const uint32_t run = runs.block_run(blk);
     const uint64_t sBeg = runs.seg0(run);
     const uint32_t n = nDev[run];
@@ -277,7 +281,7 @@ __global__ void __launch_bounds__(MPT)
 {
     __shared__ double tiles[PB / 64 * 512];
     __shared__ uint32_t dbox[4];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t blk = xcd_block_chunk(blockIdx.x, gridDim.x, PUSH_XCD_CHUNK);
     const uint32_t run = runs.block_run(blk);
     const uint64_t sBeg = runs.seg0(run);
     const uint32_t n = nDev[run];
