@@ -7,3 +7,7 @@ kernels of liboecl.so.
 """
 __all__ = ['grid', 'particles', 'poissonSolver', 'particleBoundary', 'particleEmitter',
            'particleManagement', 'magneticField', 'beam', 'randomGen', 'constants']
+
+# set_host_arrays(True): the reference API's array getters return NumPy copies, as a reference
+# driver script expects (numpy.sum / numpy.asarray / numpy.save on them).
+from ._lib import host_arrays, set_host_arrays  # noqa: E402,F401

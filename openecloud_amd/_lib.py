@@ -149,3 +149,35 @@ def to_dev(a, dtype=None):
     if dtype is not None:
         t = t.to(dtype)
     return t.to(device())
+
+
+# ---- host arrays: reference drivers index the getters' results with numpy (numpy.sum,
+# numpy.asarray, numpy.save on getParticleData() and getChargeOnGrid()).  With host arrays on, the
+# array getters and the array-returning methods return NumPy copies of the device data instead
+# of device tensors (a synchronising copy each call); their inputs take NumPy arrays either way.
+_HOST_ARRAYS = False
+
+
+def set_host_arrays(on=True):
+    """Array getters return NumPy copies (on) or aliasing device tensors (off, the default)."""
+    global _HOST_ARRAYS
+    _HOST_ARRAYS = bool(on)
+
+
+def host_arrays():
+    return _HOST_ARRAYS
+
+
+def host_out(fn):
+    """Decorator of a public array-returning method: a device tensor result becomes a NumPy copy
+    while host arrays are on."""
+    import functools
+
+    @functools.wraps(fn)
+    def wrapped(*args, **kwargs):
+        r = fn(*args, **kwargs)
+        if _HOST_ARRAYS and isinstance(r, torch.Tensor):
+            return r.detach().cpu().numpy()
+        return r
+    return wrapped
+

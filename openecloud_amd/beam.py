@@ -86,6 +86,7 @@ class _BeamBase:
             return 0.
         return s * self.dt * self.beamVelocity / (self.nParticles * self.charge)
 
+    @L.host_out
     def getCharge(self, t):
         s = self.getScale(t)
         if s == 0.:

@@ -186,13 +186,13 @@ class Buildup:
         t = self.it * self.dt
         p.calcGridWeights(g)
         p.chargeToGrid(g)
-        q = p.getChargeOnGrid()
+        q = p.chargeOnGrid
         if self.beam is not None:
             s = self.beam.getScale(t)
             if s != 0.:
                 q = q + s * self.beam.getProfile()
-        ps.solve(q)
-        p.eFieldToParticles(g, ps.getEAtGridPoints())
+        ps._solve(q)
+        p.eFieldToParticles(g, ps.eAtGridPoints)
         if self.typeB == 2:
             p.bFieldToParticles(g, self.bField)
         p.borisPush(self.dt, typeBField=self.typeB)
@@ -205,11 +205,11 @@ class Buildup:
             pb.calculateInteractionPoint()
             pb.calculateNormalVectors()
             if self.secondaryEmission:
-                sec = self.em.generateSecondaries(step=self.it)
+                sec = self.em._generate(step=self.it)
             nsec = sec.shape[0]
         ph = self.photo.generate(self.it) if self.photo is not None else sec[:0]
         if na > 0 or ph.shape[0] > 0:
-            p.addAndRemoveParticles(torch.cat([sec, ph]) if ph.shape[0] else sec, p.getIsInside())
+            p.addAndRemoveParticles(torch.cat([sec, ph]) if ph.shape[0] else sec, p.isInside[:p.macroParticleCount])
         self._rho_valid = False
         self._record(na, nsec, ph.shape[0])
         self.it += 1
@@ -220,7 +220,7 @@ class Buildup:
         t = self.it * self.dt
         if not self._rho_valid or p.orderedDeposit:
             self.deposit_all()
-        E = ps.solve(self._rhs(t))
+        E = ps._solve(self._rhs(t))
         n = p.macroParticleCount
         s = L.stream()
         self.rho.zero_()
@@ -251,7 +251,7 @@ class Buildup:
         if na > 0:
             pb._interaction()
             if self.secondaryEmission:
-                sec = em.generateSecondaries(step=self.it)
+                sec = em._generate(step=self.it)
                 nsec = sec.shape[0]
         nph = 0
         if self.photo is not None:

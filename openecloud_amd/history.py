@@ -78,7 +78,7 @@ class BuildupHistory:
         q = b._rhs(b.it * b.dt).clone()
         b.ps.solvePhi(q)
         shape = (b.g.getNyExt(), b.g.getNxExt())
-        return (q.cpu().numpy().reshape(shape).copy(), b.ps.getPhi().cpu().numpy().reshape(shape).copy())
+        return (q.cpu().numpy().reshape(shape).copy(), b.ps.phi.cpu().numpy().reshape(shape).copy())
 
     def arrays(self):
         return {'step': np.asarray(self.step, np.int64),
@@ -100,7 +100,7 @@ class BuildupHistory:
         np.save(os.path.join(d, 'physicalParticleCount.npy'), a['physicalParticleCount'])
         np.save(os.path.join(d, 'macroParticleCount.npy'), a['macroParticleCount'])
         np.save(os.path.join(d, 'particleData.npy'),
-                self.b.p.getParticleData().to(torch.float64).cpu().numpy())
+                self.b.p._data[:, :self.b.p.macroParticleCount].t().to(torch.float64).cpu().numpy())
         np.savez(os.path.join(d, 'history.npz'), **a)
         if self.pairs:
             np.savez(os.path.join(d, 'poissonPairs.npz'), q=np.stack([q for q, _ in self.pairs]),

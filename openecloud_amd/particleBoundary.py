@@ -132,15 +132,18 @@ class ParticleBoundary:
             self._interaction()
         self._normals_done = False
 
+    @L.host_out
     def getAbsorbedParticles(self):
         return self.absorbedParticles[:self.absorbedMacroParticleCount]
 
     def getAbsorbedMacroParticleCount(self):
         return self.absorbedMacroParticleCount
 
+    @L.host_out
     def getNormalVectors(self):
         return self.normalVectors[:self.absorbedMacroParticleCount]
 
+    @L.host_out
     def getRemainingTimeStep(self):
         return self.remainingTimeStep[:self.absorbedMacroParticleCount]
 

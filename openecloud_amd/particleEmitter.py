@@ -84,7 +84,7 @@ class FurmanEmitter:
             self._off = torch.empty(cap + 1, dtype=torch.int32, device=dev)
             self._impact = torch.empty(4 * cap, dtype=torch.float64, device=dev)
 
-    def generateSecondaries(self, uniforms=None, step=None):
+    def _generate(self, uniforms=None, step=None):
         """particleEmitter.pyx:154-308 -> (nSecSum, 6) device tensor.
 
         uniforms: optional injected table (nAbs, 35) of FP64 uniforms; step: the Philox
@@ -123,9 +123,16 @@ class FurmanEmitter:
                 'furman pass 2')
         return self.secondaries
 
+    @L.host_out
+    def generateSecondaries(self, uniforms=None, step=None):
+        """The secondaries of the absorbed particles (see _generate)."""
+        return self._generate(uniforms, step)
+
+    @L.host_out
     def getEmittedParticles(self):
         return self.secondaries
 
+    @L.host_out
     def getNSec(self):
         na = self.particleBoundaryObj.getAbsorbedMacroParticleCount()
         return self._nsec[:na]
@@ -154,7 +161,7 @@ class _SimpleModelEmitter:
         d.particleMass = self.particleMass
         return d
 
-    def generateSecondaries(self, uniforms=None, step=None):
+    def _generate(self, uniforms=None, step=None):
         """-> (nSecSum, 6) device tensor, the secondaries of each impact in the reference's
         order (elastic, rediffused, true ones).  uniforms: optional injected table (nAbs, 44)
         of FP64 uniforms; step: the Philox stream index (defaults to a call counter)."""
@@ -194,9 +201,16 @@ class _SimpleModelEmitter:
                 L.ptr(self.secondaries), L.u32(total), L.stream()), 'secondary emission pass 2')
         return self.secondaries
 
+    @L.host_out
+    def generateSecondaries(self, uniforms=None, step=None):
+        """The secondaries of the absorbed particles (see _generate)."""
+        return self._generate(uniforms, step)
+
+    @L.host_out
     def getEmittedParticles(self):
         return self.secondaries
 
+    @L.host_out
     def getNSec(self):
         na = self.particleBoundaryObj.getAbsorbedMacroParticleCount()
         return self._nsec[:na]

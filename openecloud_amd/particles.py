@@ -122,10 +122,12 @@ class Particles:
         self._refCapacity = n
         self._alloc_scratch(cap)
 
+    @L.host_out
     def getParticleData(self):
         """(N, 6) view aliasing the device storage (particles.pyx:142-143)."""
         return self._data[:, :self.macroParticleCount].t()
 
+    @L.host_out
     def getFullParticleData(self):
         return self._data.t()
 
@@ -138,6 +140,7 @@ class Particles:
     def setMacroParticleCount(self, macroParticleCount):
         self.macroParticleCount = int(macroParticleCount)
 
+    @L.host_out
     def getChargeOnGrid(self):
         return self.chargeOnGrid
 
@@ -150,30 +153,39 @@ class Particles:
     def getParticleMass(self):
         return self.particleMass
 
+    @L.host_out
     def getInCell(self):
         return self.inCell[:self.macroParticleCount]
 
+    @L.host_out
     def getFullIsInside(self):
         return self.isInside
 
+    @L.host_out
     def getIsInside(self):
         return self.isInside[:self.macroParticleCount]
 
+    @L.host_out
     def getWeightLowXLowY(self):
         return self._w4[0, :self.macroParticleCount]
 
+    @L.host_out
     def getWeightUpXLowY(self):
         return self._w4[1, :self.macroParticleCount]
 
+    @L.host_out
     def getWeightLowXUpY(self):
         return self._w4[2, :self.macroParticleCount]
 
+    @L.host_out
     def getWeightUpXUpY(self):
         return self._w4[3, :self.macroParticleCount]
 
+    @L.host_out
     def getEAtParticles(self):
         return self.eAtParticles[:self.macroParticleCount]
 
+    @L.host_out
     def getBAtParticles(self):
         return self.bAtParticles
 

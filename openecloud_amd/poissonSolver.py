@@ -291,13 +291,20 @@ class PoissonSolver:
             L.ptr(self.phi), L.ptr(self.eAtGridPoints), L.stream()), 'phiToE')
         return self.eAtGridPoints
 
-    def solve(self, q):
-        """poissonSolver.pyx:83-88: returns E at the nodes, [Ex | Ey]."""
+    def _solve(self, q):
+        """solve() on the device: E at the nodes as the device tensor eAtGridPoints."""
         self.solvePhi(q)
         return self.phiToE()
 
+    @L.host_out
+    def solve(self, q):
+        """poissonSolver.pyx:83-88: returns E at the nodes, [Ex | Ey]."""
+        return self._solve(q)
+
+    @L.host_out
     def getPhi(self):
         return self.phi
 
+    @L.host_out
     def getEAtGridPoints(self):
         return self.eAtGridPoints
