@@ -338,7 +338,7 @@ def measure_full(a, log):
         torch.cuda.synchronize()
         t = time.perf_counter()
         nb = p.macroParticleCount
-        ops = b.manage(target, upper=1.5, lower=0.95)
+        ops = b.manage(target, upper=1.5, lower=0.95, mergeGroup=8)
         torch.cuda.synchronize()
         if ops is not None:
             mgmt.append(dict(ops, step=b.it, count_before=nb, count_after=p.macroParticleCount,
@@ -371,7 +371,7 @@ def measure_full(a, log):
     q0 = weight_sum(p)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ops = b.manage(int(nBefore / 1.5), upper=1.0, lower=0.)
+    ops = b.manage(int(nBefore / 1.5), upper=1.0, lower=0., mergeGroup=8)
     torch.cuda.synchronize()
     t_merge = time.perf_counter() - t1
     q1 = weight_sum(p)
@@ -391,7 +391,7 @@ def measure_full(a, log):
         'ms_per_step': wall / a.steps * 1e3,
         'config': {'workload': '2048x2048 elliptical 90x70 mm chamber, quadrupole 50 T/m on the grid, '
                                '%d FP32 macro-electrons, Gaussian bunches sigma 1 mm 1.15e11 ppb x20, dt 2e-12 s, '
-                               'Furman copper delta_max 1.5, direct Poisson solve, localRandom management '
+                               'Furman copper delta_max 1.5, direct Poisson solve, localRandom management (groups of up to 8) '
                                'outside [0.95, 1.5] x %d (loaded at %d, merged at the step-0 check)'
                                % (target, target, n),
                    'grid': [2048, 2048], 'particles': target, 'loaded': n, 'management_target': target,

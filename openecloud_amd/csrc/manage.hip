@@ -123,7 +123,10 @@ __global__ void k_split(Soa<T> p, uint32_t first, uint32_t cnt, WallDev w, doubl
 constexpr uint32_t NN_NONE = 0xFFFFFFFFu;    // no proposal (out of the pool)
 constexpr uint32_t NN_RETIRE = 0xFFFFFFFEu;  // no neighbour within the limit: leaves the pool
 constexpr uint32_t NN_MERGED = 0xFFFFFFFEu;  // lock value of a particle merged in this round
-constexpr uint32_t NN_LEAF = 8;              // points per leaf of the tree
+#ifndef OECL_NN_LEAF
+#define OECL_NN_LEAF 8
+#endif
+constexpr uint32_t NN_LEAF = OECL_NN_LEAF;   // points per leaf of the tree
 
 __global__ void k_zero_u32(uint32_t *a, uint32_t n)
 {
@@ -189,19 +192,26 @@ __device__ __forceinline__ uint64_t spread5(uint32_t v)
     return r;
 }
 
-// 5-D Z-order code of each weighted point (12 bits per coordinate inside the bounding box).
+// 5-D Z-order code of each weighted point: 12 bits per coordinate, one cell size for all five
+// (the largest extent of the bounding box over 4096), so that the cells of every level are cubes
+// in the metric the search uses.  (Scaled per coordinate instead, the cells were some ten times
+// longer along the velocities than along the positions for a thermal load, and every search ball
+// met many more of those slab-shaped leaf boxes.)
 template <typename T>
 __global__ void k_morton(Soa<T> p, uint32_t n, const double *__restrict__ pw, const double *__restrict__ box,
                          uint64_t *keys, uint32_t *vals, uint8_t *gsize)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    double span = 0.;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) span = fmax(span, box[5 + c] - box[c]);
     uint64_t key = 0;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        const double lo = box[c], hi = box[5 + c];
+        const double lo = box[c];
         const double v = (double)p(c, i) * pw[c];
-        const double f = hi > lo ? (v - lo) / (hi - lo) : 0.;
+        const double f = span > 0. ? (v - lo) / span : 0.;
         const uint32_t q = (uint32_t)fmin(4095., fmax(0., f * 4096.));
         key |= spread5(q) << c;
     }
@@ -210,11 +220,26 @@ __global__ void k_morton(Soa<T> p, uint32_t n, const double *__restrict__ pw, co
     gsize[i] = 1;
 }
 
+// The weighted points in Z-order: pz[c * ldz + pos] = p(c, zperm[pos]) * pw[c] (a leaf's points
+// are then 8 consecutive values per coordinate), zpos[zperm[pos]] = pos, and the pool sizes in
+// Z-order, gz[pos] = gsize[zperm[pos]] (= 1 here).
+template <typename T>
+__global__ void k_zorder_points(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ zperm,
+                                const double *__restrict__ pw, double *pz, uint64_t ldz, uint32_t *zpos, uint8_t *gz)
+{
+    const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= nc) return;
+    const uint32_t b = zperm[pos];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) pz[c * ldz + pos] = (double)p(c, b) * pw[c];
+    zpos[b] = pos;
+    gz[pos] = 1;
+}
+
 // Leaf boxes: leaf l holds sorted positions [8 l, 8 l + 8) (those below nc); node L + l of the
 // heap-ordered tree (root 1, children 2k and 2k + 1).  Empty leaves get an empty box.
-template <typename T>
-__global__ void k_leaf_boxes(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ zperm, const double *__restrict__ pw,
-                             uint32_t L, double *blo, double *bhi)
+__global__ void k_leaf_boxes(uint32_t nc, const double *__restrict__ pz, uint64_t ldz, uint32_t L, double *blo,
+                             double *bhi)
 {
     const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= L) return;
@@ -227,10 +252,9 @@ __global__ void k_leaf_boxes(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
     for (uint32_t k = 0; k < NN_LEAF; ++k) {
         const uint64_t pos = (uint64_t)l * NN_LEAF + k;
         if (pos >= nc) break;
-        const uint32_t b = zperm[pos];
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
-            const double v = (double)p(c, b) * pw[c];
+            const double v = pz[c * ldz + pos];
             lo[c] = fmin(lo[c], v);
             hi[c] = fmax(hi[c], v);
         }
@@ -274,13 +298,16 @@ __device__ __forceinline__ double box_dist2(const double *__restrict__ blo, cons
 // and bound over the tree (boxes farther than the best distance pruned); a box containing
 // particles that left the pool still bounds the pooled ones, and the boxes are refitted after
 // every round in which a survivor that stays in the pool has moved (groups with scheme 1), so
-// the search stays exact.  NN_RETIRE when none lies within the squared-distance limit (particleManagement.pyx:185
-// compares the squared distance with factorDisLimit * ds).  Resets a's lock for this round.
-template <typename T>
-__global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ zperm, uint32_t L,
+// the search stays exact.  NN_RETIRE when none lies within the squared-distance limit
+// (particleManagement.pyx:185 compares the squared distance with factorDisLimit * ds).  From the
+// second round on, a proposal a -> b of the previous round stands without a new search when
+// neither a nor b took part in a merge since (mrk): the pool has only lost particles, b is still
+// in it and still the nearest.  Resets a's lock for this round.
+__global__ void k_nn_propose(uint32_t nc, const uint32_t *__restrict__ zperm, uint32_t L,
                              const double *__restrict__ blo, const double *__restrict__ bhi,
-                             const double *__restrict__ pw, double disLimit, int G,
-                             const uint8_t *__restrict__ gsize, uint32_t *__restrict__ prop, uint32_t *__restrict__ lock,
+                             const double *__restrict__ pz, uint64_t ldz, const uint8_t *__restrict__ gz,
+                             const uint8_t *__restrict__ gsize, const uint8_t *__restrict__ mrk, double disLimit,
+                             int G, uint32_t round, uint32_t *__restrict__ prop, uint32_t *__restrict__ lock,
                              uint32_t *__restrict__ cnt)
 {
     // thread t searches for the t-th point in Z-order, so the lanes of a wave are neighbours in
@@ -289,14 +316,21 @@ __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
     if (t >= nc) return;
     const uint32_t a = zperm[t];
     lock[a] = NN_NONE;
-    const uint32_t sa = gsize[a];
+    const uint32_t sa = gz[t];
     if (sa == 0 || (int)sa >= G) {
         prop[a] = NN_NONE;
         return;
     }
+    if (round > 0) {
+        const uint32_t prev = prop[a];
+        if (prev < nc && !mrk[a] && !mrk[prev] && gsize[prev] != 0 && (int)(sa + gsize[prev]) <= G) {
+            atomicAdd(cnt, 1u);
+            return;   // prop[a] stands
+        }
+    }
     double q[5];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) q[c] = (double)p(c, a) * pw[c];
+    for (int c = 0; c < 5; ++c) q[c] = pz[c * ldz + t];
     // a neighbour at or beyond the limit is rejected anyway (the reference's test is strict), so
     // the search starts with the limit as the best distance: only the ball inside it is searched,
     // and the result is the same
@@ -307,19 +341,20 @@ __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
         for (uint32_t j = 0; j < NN_LEAF; ++j) {
             const uint64_t pos = base + j;
             if (pos >= nc) break;
-            const uint32_t b = zperm[pos];
-            if (b == a) continue;
-            const uint32_t sb = gsize[b];
-            if (sb == 0 || (int)(sa + sb) > G) continue;
+            const uint32_t sb = gz[pos];
+            if (sb == 0 || (int)(sa + sb) > G || pos == t) continue;
             double d2 = 0.;
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
-                const double d = (double)p(c, b) * pw[c] - q[c];
+                const double d = pz[c * ldz + pos] - q[c];
                 d2 += d * d;
             }
-            if (d2 < best || (d2 == best && bi != NN_NONE && b < bi)) {
-                best = d2;
-                bi = b;
+            if (d2 <= best) {
+                const uint32_t b = zperm[pos];
+                if (d2 < best || (bi != NN_NONE && b < bi)) {
+                    best = d2;
+                    bi = b;
+                }
             }
         }
     };
@@ -357,10 +392,11 @@ __global__ void k_nn_propose(Soa<T> p, uint32_t nc, const uint32_t *__restrict__
 }
 
 // Locks: each particle keeps the lowest-ranked proposal that involves it.
-__global__ void k_nn_lock(uint32_t nc, const uint32_t *__restrict__ prop, uint32_t *lock)
+__global__ void k_nn_lock(uint32_t nc, const uint32_t *__restrict__ prop, uint32_t *lock, uint8_t *mrk)
 {
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nc) return;
+    mrk[a] = 0;   // the proposals have read the previous round's marks
     const uint32_t b = prop[a];
     if (b >= nc) return;
     atomicMin(lock + a, a);
@@ -385,7 +421,8 @@ template <typename T>
 __global__ void k_nn_merge(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ prop, const uint32_t *__restrict__ acc,
                            const uint32_t *__restrict__ off, uint32_t need, int G, int scheme, uint32_t round,
                            uint32_t stream, uint32_t s0, uint32_t s1, uint8_t *gsize, uint32_t *lock, uint8_t *keep,
-                           uint32_t *merged, uint32_t *pairs, uint32_t base)
+                           uint32_t *merged, uint32_t *pairs, uint32_t base, const uint32_t *__restrict__ zpos,
+                           uint8_t *gz, uint8_t *mrk, double *pz, uint64_t ldz, const double *__restrict__ pw)
 {
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nc || !acc[a] || off[a] >= need) return;
@@ -402,12 +439,18 @@ __global__ void k_nn_merge(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ p
         p(2, sv) = (T)(p(2, sv) * f);
         p(3, sv) = (T)(p(3, sv) * f);
         p(4, sv) = (T)(p(4, sv) * f);
+        // the survivor's point in the Z-ordered copy follows
+        const uint32_t zs = zpos[sv];
+        for (int c = 2; c < 5; ++c) pz[c * ldz + zs] = (double)p(c, sv) * pw[c];
     }
     p(5, sv) = (T)W;
     keep[rm] = 0;
     const int ss = (int)gsize[a] + (int)gsize[b];
     gsize[rm] = 0;
     gsize[sv] = (uint8_t)(ss >= G ? 0 : ss);
+    gz[zpos[rm]] = 0;
+    gz[zpos[sv]] = gsize[sv];
+    mrk[a] = mrk[b] = 1;
     lock[a] = lock[b] = NN_MERGED;
     if (pairs) {
         pairs[2 * (uint64_t)(base + off[a])] = sv;
@@ -418,11 +461,14 @@ __global__ void k_nn_merge(Soa<T> p, uint32_t nc, const uint32_t *__restrict__ p
 
 // Candidates whose neighbour search failed leave the pool, unless a pair took them this round.
 __global__ void k_nn_retire(uint32_t nc, const uint32_t *__restrict__ prop, const uint32_t *__restrict__ lock,
-                            uint8_t *gsize)
+                            uint8_t *gsize, const uint32_t *__restrict__ zpos, uint8_t *gz)
 {
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nc) return;
-    if (prop[a] == NN_RETIRE && lock[a] != NN_MERGED) gsize[a] = 0;
+    if (prop[a] == NN_RETIRE && lock[a] != NN_MERGED) {
+        gsize[a] = 0;
+        gz[zpos[a]] = 0;
+    }
 }
 
 }  // namespace
@@ -545,7 +591,8 @@ OECL_EXPORT uint64_t oecl_merge_work_bytes(uint32_t nc)
 {
     const uint64_t L = nn_leaves(nc);
     return al16(8ull * nc) + 5 * al16(4ull * nc + 4) + al16(nc) + al16(8ull * 512 * 10) + al16(8ull * 16) +
-           2 * al16(8ull * 5 * 2 * L) + al16(oecl_scan_work_bytes(nc + 1)) + al16(oecl_sort_work_bytes(nc)) + 256;
+           2 * al16(8ull * 5 * 2 * L) + al16(oecl_scan_work_bytes(nc + 1)) + al16(oecl_sort_work_bytes(nc)) +
+           al16(40ull * nc) + al16(4ull * nc) + 2 * al16(nc) + 256;
 }
 
 OECL_EXPORT int oecl_nn_merge(int dtype, void *data, uint64_t stride, uint32_t nc, const double *pw, int G,
@@ -591,6 +638,15 @@ OECL_EXPORT int oecl_nn_merge(int dtype, void *data, uint64_t stride, uint32_t n
     w += al16(8ull * 5 * 2 * L);
     uint32_t *scanw = (uint32_t *)w;
     w += al16(oecl_scan_work_bytes(nc + 1));
+    double *pz = (double *)w;     // the weighted points in Z-order, 5 rows of ldz
+    const uint64_t ldz = nc;
+    w += al16(40ull * nc);
+    uint32_t *zpos = (uint32_t *)w;
+    w += al16(4ull * nc);
+    uint8_t *gz = (uint8_t *)w;
+    w += al16(nc);
+    uint8_t *mrk = (uint8_t *)w;
+    w += al16(nc);
     void *sortw = w;
     const dim3 gr(oecl_blocks(nc, 256)), bl(256);
     Soa<double> pd{(double *)data, stride};
@@ -604,39 +660,34 @@ OECL_EXPORT int oecl_nn_merge(int dtype, void *data, uint64_t stride, uint32_t n
     int rc = oecl_radix_sort_u64(keys, zperm, nc, 64, sortw, st);
     if (rc) return rc;
     if (dtype == OECL_F64)
-        hipLaunchKernelGGL(k_leaf_boxes<double>, dim3(oecl_blocks(L, 256)), bl, 0, s, pd, nc, zperm, pw, L, blo, bhi);
+        hipLaunchKernelGGL(k_zorder_points<double>, gr, bl, 0, s, pd, nc, zperm, pw, pz, ldz, zpos, gz);
     else
-        hipLaunchKernelGGL(k_leaf_boxes<float>, dim3(oecl_blocks(L, 256)), bl, 0, s, pf, nc, zperm, pw, L, blo, bhi);
+        hipLaunchKernelGGL(k_zorder_points<float>, gr, bl, 0, s, pf, nc, zperm, pw, pz, ldz, zpos, gz);
+    hipLaunchKernelGGL(k_fill_u8, gr, bl, 0, s, mrk, nc, (uint8_t)0);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(oecl_blocks(L, 256)), bl, 0, s, nc, pz, ldz, L, blo, bhi);
     for (uint32_t first = L >> 1; first >= 1; first >>= 1)
         hipLaunchKernelGGL(k_inner_boxes, dim3(oecl_blocks(first, 256)), bl, 0, s, first, first, blo, bhi);
     uint32_t done = 0, rounds = 0, h[2] = {0, 0};
     for (;;) {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 1u);
-        if (dtype == OECL_F64)
-            hipLaunchKernelGGL(k_nn_propose<double>, gr, bl, 0, s, pd, nc, zperm, L, blo, bhi, pw, disLimit, G, gsize,
-                               prop, lock, cnt);
-        else
-            hipLaunchKernelGGL(k_nn_propose<float>, gr, bl, 0, s, pf, nc, zperm, L, blo, bhi, pw, disLimit, G, gsize,
-                               prop, lock, cnt);
-        hipLaunchKernelGGL(k_nn_lock, gr, bl, 0, s, nc, prop, lock);
+        hipLaunchKernelGGL(k_nn_propose, gr, bl, 0, s, nc, zperm, L, blo, bhi, pz, ldz, gz, gsize, mrk, disLimit, G,
+                           rounds, prop, lock, cnt);
+        hipLaunchKernelGGL(k_nn_lock, gr, bl, 0, s, nc, prop, lock, mrk);
         hipLaunchKernelGGL(k_nn_accept, gr, bl, 0, s, nc, prop, lock, acc);
         rc = oecl_launch_scan_u32(acc, nc, off, scanw, s);
         if (rc) return rc;
         if (dtype == OECL_F64)
             hipLaunchKernelGGL(k_nn_merge<double>, gr, bl, 0, s, pd, nc, prop, acc, off, toMerge - done, G, scheme,
-                               rounds, stream, s0, s1, gsize, lock, keep, merged, pairs, done);
+                               rounds, stream, s0, s1, gsize, lock, keep, merged, pairs, done, zpos, gz, mrk, pz, ldz,
+                               pw);
         else
             hipLaunchKernelGGL(k_nn_merge<float>, gr, bl, 0, s, pf, nc, prop, acc, off, toMerge - done, G, scheme,
-                               rounds, stream, s0, s1, gsize, lock, keep, merged, pairs, done);
-        hipLaunchKernelGGL(k_nn_retire, gr, bl, 0, s, nc, prop, lock, gsize);
+                               rounds, stream, s0, s1, gsize, lock, keep, merged, pairs, done, zpos, gz, mrk, pz, ldz,
+                               pw);
+        hipLaunchKernelGGL(k_nn_retire, gr, bl, 0, s, nc, prop, lock, gsize, zpos, gz);
         if (G > 2 && scheme == 1) {
             // survivors that stay in the pool had their velocity rescaled: refit the boxes
-            if (dtype == OECL_F64)
-                hipLaunchKernelGGL(k_leaf_boxes<double>, dim3(oecl_blocks(L, 256)), bl, 0, s, pd, nc, zperm, pw, L,
-                                   blo, bhi);
-            else
-                hipLaunchKernelGGL(k_leaf_boxes<float>, dim3(oecl_blocks(L, 256)), bl, 0, s, pf, nc, zperm, pw, L,
-                                   blo, bhi);
+            hipLaunchKernelGGL(k_leaf_boxes, dim3(oecl_blocks(L, 256)), bl, 0, s, nc, pz, ldz, L, blo, bhi);
             for (uint32_t first = L >> 1; first >= 1; first >>= 1)
                 hipLaunchKernelGGL(k_inner_boxes, dim3(oecl_blocks(first, 256)), bl, 0, s, first, first, blo, bhi);
         }

@@ -326,8 +326,10 @@ def test_c5_push_sample_matches_oracle(c5, orc):
 
 
 def test_c5_merge_event_reaches_target(c5):
-    """localRandom at the 1.5x trigger on the 2e7-particle buildup: the count falls to the
-    target exactly and the total weight is unchanged."""
+    """localRandom at the 1.5x trigger on the 2e7-particle buildup, merging groups of up to 8
+    particles as the full workload does: the count falls to the target exactly and the total
+    weight is unchanged."""
+    group = 8
     from openecloud_amd.diagnostics import moments
     from openecloud_amd.particleManagement import localRandom
     g, bq, p, pb, em, ps, b = c5
@@ -335,7 +337,8 @@ def test_c5_merge_event_reaches_target(c5):
     n = p.getMacroParticleCount()
     target = int(n / 1.5)
     w0 = moments(p)[0]
-    info = localRandom(g, p, pb, target)
-    print('C5 merge: %d -> %d in %d rounds' % (n, p.getMacroParticleCount(), info['rounds']), flush=True)
+    info = localRandom(g, p, pb, target, mergeGroup=group)
+    print('C5 merge, groups of up to %d: %d -> %d in %d rounds' % (group, n, p.getMacroParticleCount(), info['rounds']),
+          flush=True)
     assert p.getMacroParticleCount() == target
     assert abs(moments(p)[0] - w0) <= 1e-12 * w0
