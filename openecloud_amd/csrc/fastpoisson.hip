@@ -167,6 +167,12 @@ struct FftArgs {
     const uint32_t *iPos;           // EPI 3: box column of entry e
     uint32_t nJ, m;
     uint64_t nb;
+    // EPI 4: the right-hand side gathered from the grid, q = rho + s(step) * prof on the inside
+    // nodes (rho per run, np apart; s from the per-run table at the device step counter)
+    double *rho;
+    const double *prof, *table;
+    uint32_t nScale;
+    const uint32_t *stepDev;
 };
 
 // EPI 3 transforms the sparse lines u_c (z values of the entries of box row jCol[c] at their
@@ -193,6 +199,27 @@ __global__ void __launch_bounds__(256) k_dst_fft(FftArgs g)
             const double *ze = g.X + (size_t)r * g.m;
             for (int32_t e = g.jStart[c] + (int32_t)t; e < g.jStart[c + 1]; e += T) v[g.iPos[e]] = ze[e];
         }
+    } else if (EPI == 4) {
+        // line = box row jj of run r: grid row jj + 2; box column k is grid column k + 2.  The
+        // row's rho is read into the line (masked, plus the beam term) and zeroed for the
+        // step's deposit, every node of the grid row (each thread zeroes what it read).
+        const uint32_t rr = live ? line / g.by : 0, jj = live ? line - rr * g.by : 0;
+        const uint32_t s = *g.stepDev;
+        const double sc = (g.prof && g.table && s < g.nScale) ? g.table[(size_t)rr * g.nScale + s] : 0.;
+        double *rrow = g.rho + (size_t)rr * g.np + (size_t)(jj + 2) * g.nx;
+        const double *prow = g.prof ? g.prof + (size_t)(jj + 2) * g.nx : nullptr;
+        const uint8_t *mrow = g.mask + (size_t)(jj + 2) * g.nx;
+        for (uint32_t k = t; k < (uint32_t)n; k += T) {
+            double q = 0.;
+            if (live && k < g.bxg) {
+                const double r0 = rrow[k + 2];
+                q = mrow[k + 2] ? (sc != 0. ? r0 + sc * prow[k + 2] : r0) : 0.;
+                rrow[k + 2] = 0.;
+            }
+            v[k] = q;
+        }
+        if (live)
+            for (uint32_t c = t; c < 4; c += T) rrow[c < 2 ? c : g.nx - 4 + c] = 0.;   // grid columns 0, 1, nx-2, nx-1
     } else {
         const double *xl = g.X + (size_t)(live ? line : 0) * g.ldx;
         for (uint32_t k = t; k < (uint32_t)n; k += T) v[k] = live ? xl[k] : 0.;
@@ -203,7 +230,7 @@ __global__ void __launch_bounds__(256) k_dst_fft(FftArgs g)
     const double sc = -0.25 * sqrt(2. / M);
     double *out = g.C;
     uint32_t run = 0, jj = 0;
-    if (EPI == 0) out += (size_t)line * g.ldc;
+    if (EPI == 0 || EPI == 4) out += (size_t)line * g.ldc;
     if (EPI == 2) {
         run = line / g.by;
         jj = line - run * g.by;
@@ -214,7 +241,7 @@ __global__ void __launch_bounds__(256) k_dst_fft(FftArgs g)
         const double2 a = z[k + 1], b = z[M - 1 - k], w = g.tab[k + 1];
         // w = (cos(pi (k+1)/M), -sin(pi (k+1)/M))
         const double s = sc * (((a.y - b.y) - w.x * (a.x - b.x)) + w.y * (a.y + b.y));
-        if (EPI == 0) {
+        if (EPI == 0 || EPI == 4) {
             out[k] = s;
         } else if (EPI == 3) {
             out[k] -= s;
@@ -847,8 +874,17 @@ OECL_EXPORT int oecl_fastpoisson_solve(oecl_fastpoisson *h, const double *q, dou
 
 // One DST transform of N lines: EPI 0 of X (ld bx) into C (ld ldc; C may be X), EPI 2 of X into
 // the potential C = phi, EPI 3 of the sparse lines of zE (N = R nJ) subtracted from C (ld bx).
+// The grid-side arguments of EPI 4.
+struct FftGrid {
+    double *rho;
+    const double *prof, *table;
+    uint32_t nScale;
+    const uint32_t *stepDev;
+};
+
 template <int EPI>
-static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C, int ldc, hipStream_t s)
+static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C, int ldc, hipStream_t s,
+                         const FftGrid *gg = nullptr)
 {
     FftArgs g;
     g.nLines = (uint32_t)N;
@@ -868,6 +904,18 @@ static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C,
     g.nJ = h->nJ;
     g.m = h->m;
     g.nb = h->nb;
+    g.rho = nullptr;
+    g.prof = g.table = nullptr;
+    g.nScale = 0;
+    g.stepDev = nullptr;
+    if (EPI == 4) {
+        if (!gg) return OECL_ERR_ARG;
+        g.rho = gg->rho;
+        g.prof = gg->prof;
+        g.table = gg->table;
+        g.nScale = gg->nScale;
+        g.stepDev = gg->stepDev;
+    }
 #define OECL_FFT(MM)                                                                                    \
     case MM:                                                                                          \
         hipLaunchKernelGGL((k_dst_fft<MM, EPI>), dim3((N + FftShape<MM>::LB - 1) / FftShape<MM>::LB), \
@@ -883,13 +931,32 @@ static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C,
 #undef OECL_FFT
 }
 
+static int solve_after_first(oecl_fastpoisson *h, double *phi, uint32_t R, hipStream_t s);
+
 int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStream_t s)
 {
     if (R < 1 || R > h->rCap || (uint64_t)h->by * R > 0x7FFFFFFFull) return OECL_ERR_ARG;
-    const int bx = h->bx, by = h->by, N = by * (int)R;
     // hat b = Sx b
-    int rc = dst_transform<0>(h, N, h->b, h->t1, bx, s);
+    int rc = dst_transform<0>(h, (int)(h->by * R), h->b, h->t1, (int)h->bx, s);
     if (rc != OECL_OK) return rc;
+    return solve_after_first(h, phi, R, s);
+}
+
+int fastpoisson_solve_grid(oecl_fastpoisson *h, double *rho, const double *prof, const double *table, uint32_t nScale,
+                           const uint32_t *stepDev, double *phi, uint32_t R, hipStream_t s)
+{
+    if (R < 1 || R > h->rCap || (uint64_t)h->by * R > 0x7FFFFFFFull || !rho || !stepDev) return OECL_ERR_ARG;
+    const FftGrid gg{rho, prof, table, nScale, stepDev};
+    // hat b = Sx q, q gathered from the grid (rho zeroed on the way)
+    int rc = dst_transform<4>(h, (int)(h->by * R), nullptr, h->t1, (int)h->bx, s, &gg);
+    if (rc != OECL_OK) return rc;
+    return solve_after_first(h, phi, R, s);
+}
+
+static int solve_after_first(oecl_fastpoisson *h, double *phi, uint32_t R, hipStream_t s)
+{
+    const int bx = h->bx, by = h->by, N = by * (int)R;
+    int rc = OECL_OK;
     if (h->m) {
         // hat y = T^-1 hat b ; y = Sx hat y (in place) ; wy = W y ; z = C^-1 wy ;
         // hat b -= Sx U z, line by line over the box rows holding corrected nodes

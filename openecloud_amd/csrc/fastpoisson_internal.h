@@ -20,3 +20,9 @@ int fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R);
 // phi (R potentials np apart) is written on the box nodes only (nodes outside the box must be
 // zero already).
 int fastpoisson_solve_box(oecl_fastpoisson *h, double *phi, uint32_t R, hipStream_t s);
+// The same with the right-hand side gathered from the grid inside the first transform:
+// q = rho + table[r nScale + *stepDev] * prof on the inside nodes of run r (prof and table may
+// be null: q = rho), and every node of rho zeroed except grid rows 0, 1, ny-2 and ny-1.
+int fastpoisson_solve_grid(oecl_fastpoisson *h, double *rho, const double *prof, const double *table, uint32_t nScale,
+                           const uint32_t *stepDev, double *phi, uint32_t R, hipStream_t s);
+

@@ -77,22 +77,18 @@ struct Runs {
 //   rho = 0 for this step's deposit; thread 0 records the particle count of the step;
 //   the step's photoelectrons (a per-step table, single run only) are generated into their
 //   slots.
+// The per-run bookkeeping at the start of a step, thread i of run `run` (gridDim.x * blockDim.x
+// threads per run): the step's particle count into the history, the counters, and the step's
+// photoelectrons (a per-step table, single run only) generated into their slots.
 template <typename T>
-__global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double *__restrict__ table,
-                            uint32_t nScale, const uint32_t *__restrict__ stepDev,
-                            const double *__restrict__ prof, FpBox box, double *__restrict__ phi,
-                            const uint32_t *__restrict__ nDev, uint32_t *__restrict__ hist, uint32_t histCap,
-                            uint32_t *__restrict__ misc, WallDev w, const uint32_t *__restrict__ photoCount,
-                            const double *__restrict__ photoWeight, uint32_t nPhoto, uint32_t photoCap,
-                            uint32_t s0, uint32_t s1, double e0, double sigmaE, double mass,
-                            T *__restrict__ photoSlots, uint32_t *stats)
+__device__ __forceinline__ void begin_step_run(uint32_t i, uint32_t run, uint32_t s, const uint32_t *__restrict__ nDev,
+                                               uint32_t *__restrict__ hist, uint32_t histCap,
+                                               uint32_t *__restrict__ misc, WallDev w,
+                                               const uint32_t *__restrict__ photoCount,
+                                               const double *__restrict__ photoWeight, uint32_t nPhoto,
+                                               uint32_t photoCap, uint32_t s0, uint32_t s1, double e0, double sigmaE,
+                                               double mass, T *__restrict__ photoSlots, uint32_t *stats)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t s = *stepDev, run = blockIdx.y;
-    rho += (size_t)run * np;
-    phi += (size_t)run * np;
-    box.b += (size_t)run * box.bx * (box.ny - 4);
-    if (table) table += (size_t)run * nScale;
     if (hist) hist += (size_t)run * histCap;
     misc += run * M_COUNT;
     nDev += run;
@@ -111,6 +107,33 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
     }
     for (uint32_t k = i; k < nph; k += gridDim.x * blockDim.x)
         photo::emit_one<T>(w, k, s, s0, s1, e0, sigmaE, photoWeight[s], mass, photoSlots + 6 * (uint64_t)k);
+}
+
+// Start of a step, one thread per node, blockIdx.y = run:
+//   the Poisson right-hand side q = rho + s(step) * beam profile (example01_basic.py:81),
+//   masked and written straight into the direct solver's box vector (inside nodes keep q,
+//   other box nodes 0; the potential outside the box is set to 0);
+//   rho = 0 for this step's deposit; the run's bookkeeping (begin_step_run).
+// Used when the caller solves (externalPhi); the step's own direct solve gathers q in its first
+// pass instead (k_begin_edges).
+template <typename T>
+__global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double *__restrict__ table,
+                            uint32_t nScale, const uint32_t *__restrict__ stepDev,
+                            const double *__restrict__ prof, FpBox box, double *__restrict__ phi,
+                            const uint32_t *__restrict__ nDev, uint32_t *__restrict__ hist, uint32_t histCap,
+                            uint32_t *__restrict__ misc, WallDev w, const uint32_t *__restrict__ photoCount,
+                            const double *__restrict__ photoWeight, uint32_t nPhoto, uint32_t photoCap,
+                            uint32_t s0, uint32_t s1, double e0, double sigmaE, double mass,
+                            T *__restrict__ photoSlots, uint32_t *stats)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = *stepDev, run = blockIdx.y;
+    rho += (size_t)run * np;
+    phi += (size_t)run * np;
+    box.b += (size_t)run * box.bx * (box.ny - 4);
+    if (table) table += (size_t)run * nScale;
+    begin_step_run<T>(i, run, s, nDev, hist, histCap, misc, w, photoCount, photoWeight, nPhoto, photoCap, s0, s1, e0,
+                      sigmaE, mass, photoSlots, stats);
     if (i >= np) return;
     const double sc = (prof && table && s < nScale) ? table[s] : 0.;
     const double r = rho[i];
@@ -119,6 +142,41 @@ __global__ void k_begin_dev(uint32_t np, double *__restrict__ rho, const double 
     const uint32_t ix = i % box.nx, iy = i / box.nx;
     if (ix < 2 || iy < 2 || ix >= box.nx - 2 || iy >= box.ny - 2) phi[i] = 0.;
     else box.b[(ix - 2) + (iy - 2) * box.bx] = box.mask[i] ? qv : 0.;
+}
+
+// Start of a step whose direct solve gathers q = rho + s(step) * profile from the grid in its
+// first pass (fastpoisson_solve_grid, which also zeroes rho on grid rows 2 .. ny-3): the run's
+// bookkeeping (begin_step_run), rho = 0 on grid rows 0, 1, ny-2 and ny-1, and phi = 0 outside the
+// box (those rows and grid columns 0, 1, nx-2, nx-1), which the solve never writes.  blockIdx.y
+// = run.
+template <typename T>
+__global__ void k_begin_edges(uint32_t nx, uint32_t ny, double *__restrict__ rho, const uint32_t *__restrict__ stepDev,
+                              double *__restrict__ phi, const uint32_t *__restrict__ nDev,
+                              uint32_t *__restrict__ hist, uint32_t histCap, uint32_t *__restrict__ misc, WallDev w,
+                              const uint32_t *__restrict__ photoCount, const double *__restrict__ photoWeight,
+                              uint32_t nPhoto, uint32_t photoCap, uint32_t s0, uint32_t s1, double e0,
+                              double sigmaE, double mass, T *__restrict__ photoSlots, uint32_t *stats)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = *stepDev, run = blockIdx.y;
+    const size_t np = (size_t)nx * ny;
+    rho += (size_t)run * np;
+    phi += (size_t)run * np;
+    begin_step_run<T>(i, run, s, nDev, hist, histCap, misc, w, photoCount, photoWeight, nPhoto, photoCap, s0, s1, e0,
+                      sigmaE, mass, photoSlots, stats);
+    const uint32_t nRows = 4 * nx, nEdge = nRows + 4 * (ny - 4);
+    for (uint32_t k = i; k < nEdge; k += gridDim.x * blockDim.x) {
+        size_t node;
+        if (k < nRows) {
+            const uint32_t rr = k / nx, c = k - rr * nx;
+            node = (size_t)(rr < 2 ? rr : ny - 4 + rr) * nx + c;
+            rho[node] = 0.;
+        } else {
+            const uint32_t kk = k - nRows, rr = 2 + kk / 4, c = kk % 4;
+            node = (size_t)rr * nx + (c < 2 ? c : nx - 4 + c);
+        }
+        phi[node] = 0.;
+    }
 }
 
 // The push maps hardware block b to logical block xcd_block_chunk(b, n, PUSH_XCD_CHUNK)
@@ -988,12 +1046,20 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     int rc = fastpoisson_reserve(fp, R);
     if (rc) return rc;
     const FpBox box = fastpoisson_box(fp);
-    hipLaunchKernelGGL(k_begin_dev<T>, dim3(oecl_blocks(np, 256), R), dim3(256), 0, s, np, a->rho, a->beamScale,
-                       a->nScale, a->stepDev, a->profile, box, a->phi, a->nDev, a->countHist, a->histCap, misc, wd,
-                       a->photoCount, a->photoWeight, a->nPhoto, a->photoCap, a->seed0, a->seed1, a->photoE0,
-                       a->photoSigmaE, P->particleMass, (T *)a->photoSlots, a->stats);
-    if (!a->externalPhi) {
-        rc = fastpoisson_solve_box(fp, a->phi, R, s);
+    if (a->externalPhi) {
+        hipLaunchKernelGGL(k_begin_dev<T>, dim3(oecl_blocks(np, 256), R), dim3(256), 0, s, np, a->rho, a->beamScale,
+                           a->nScale, a->stepDev, a->profile, box, a->phi, a->nDev, a->countHist, a->histCap, misc,
+                           wd, a->photoCount, a->photoWeight, a->nPhoto, a->photoCap, a->seed0, a->seed1, a->photoE0,
+                           a->photoSigmaE, P->particleMass, (T *)a->photoSlots, a->stats);
+    } else {
+        const uint64_t nEdge = 4ull * g->nx + 4ull * (g->ny - 4);
+        const uint64_t nThr = nEdge > a->photoCap ? nEdge : a->photoCap;
+        hipLaunchKernelGGL(k_begin_edges<T>, dim3(oecl_blocks(nThr, 256), R), dim3(256), 0, s, g->nx, g->ny, a->rho,
+                           a->stepDev, a->phi, a->nDev, a->countHist, a->histCap, misc, wd, a->photoCount,
+                           a->photoWeight, a->nPhoto, a->photoCap, a->seed0, a->seed1, a->photoE0, a->photoSigmaE,
+                           P->particleMass, (T *)a->photoSlots, a->stats);
+        rc = fastpoisson_solve_grid(fp, a->rho, a->profile, a->beamScale, a->beamScale ? a->nScale : 0, a->stepDev,
+                                    a->phi, R, s);
         if (rc) return rc;
     }
     // mixed push (FP32 particles): E into its FP32 interleaved copy, the FP64 interleaved copy
