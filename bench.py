@@ -226,6 +226,14 @@ def event_us(evs):
     return out
 
 
+def event_free(evs):
+    """Releases event pairs that were never recorded."""
+    from openecloud_amd import _lib as L
+    for e0, e1 in evs:
+        L.lib().oecl_event_destroy(e0)
+        L.lib().oecl_event_destroy(e1)
+
+
 def host_info():
     model = platform.processor() or ''
     try:
@@ -456,7 +464,7 @@ def measure_dipole(a, log, steps, warmup, n, dtype, eager, compare=False):
     st = b.sync()
     counts = b.counts(it0, it0 + steps)
     push_us = event_us(evs_used)
-    event_us(evs_left)
+    event_free(evs_left)
     npx = g.getNpExt()
     avg_bytes = sum(bpp * c + bpn * npx for c in counts) / len(counts)
     avg_us = sum(push_us) / len(push_us)

@@ -138,9 +138,11 @@ __device__ __forceinline__ BufRes buf_res(const void *base, uint32_t bytes)
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
 }
 
+// CPOL: the instruction's cache-policy bits (bit 1: non-temporal, for data streamed once).
+template <int CPOL = 0>
 __device__ __forceinline__ float buf_ld_f32(BufRes r, uint32_t off, uint32_t soff = 0)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, CPOL));
 }
 
 // Two adjacent floats: two dword loads, which the backend merges into one dwordx2 load.  (The b64
@@ -150,9 +152,10 @@ __device__ __forceinline__ float2 buf_ld_f32x2(BufRes r, uint32_t off, uint32_t 
     return make_float2(buf_ld_f32(r, off, soff), buf_ld_f32(r, off + 4u, soff));
 }
 
+template <int CPOL = 0>
 __device__ __forceinline__ void buf_st_f32(float v, BufRes r, uint32_t off)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, CPOL);
 }
 
 __device__ __forceinline__ void buf_st_u8(uint8_t v, BufRes r, uint32_t off)

@@ -40,6 +40,7 @@ struct oecl_fastpoisson {
     uint32_t bx, by, nb;      // box sizes, nb = bx * by; bx + 1 a power of two
     uint32_t bxg;             // interior grid columns nx - 4 (box columns beyond are padding)
     uint32_t m;               // irregular rows
+    uint32_t ldc;             // m rounded up to even: row stride of C^-1 and of wy
     uint32_t nJ;              // distinct box columns j among the irregular nodes
     double cyu;
     rocblas_handle blas;
@@ -52,9 +53,9 @@ struct oecl_fastpoisson {
     int32_t *jCol, *jStart;   // distinct columns j of the irregular nodes, their entry ranges
     int32_t *ePos;            // [m] entry of row a in the by-column order
     uint32_t *iPos;           // [m] box column i_a of entry e (by-column order)
-    double *CinvT;            // m*m, C^-1 row-major (row a contiguous)
+    double *CinvT;            // m*ldc, C^-1 row-major (row a contiguous, 16-byte aligned, padding zero)
     double *b, *t1, *t2;      // nb each, per right-hand side
-    double *wy, *zE;          // m each (zE in the by-column order), per right-hand side
+    double *wy, *zE;          // ldc (padding entry zero), m (zE in the by-column order), per right-hand side
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
     uint32_t rCap;            // right-hand sides the vectors above hold
     char *batchMem;           // their own allocation once rCap > 1
@@ -468,13 +469,13 @@ __device__ __forceinline__ double wave_sum(double s)
 // a row has at most WMAX entries (the node and its four neighbours), all of whose dot
 // products are accumulated together so that every load of the row is in flight at once.
 // CAP: batch element c = blockIdx.y holds R^-1 of the unit vector of row a0 + c, and the
-// result is column a0 + c of C = I + W R^-1 U, stored row-major (C^T column-major).  Else
-// blockIdx.y is the right-hand side of a batched solve: wy[blockIdx.y * m + a].
+// result is column a0 + c of C = I + W R^-1 U, stored row-major with row stride ld (C^T
+// column-major).  Else blockIdx.y is the right-hand side of a batched solve: wy[blockIdx.y * ld + a].
 constexpr int WMAX = 5;
 
 template <int CQ, bool CAP>
 __global__ void __launch_bounds__(256)
-    k_cap_wy(uint32_t m, uint32_t bx, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+    k_cap_wy(uint32_t m, uint32_t ld, uint32_t bx, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
              const double *__restrict__ val, const double *__restrict__ Sx, const double *__restrict__ yh,
              double *__restrict__ wy, size_t nb, uint32_t a0)
 {
@@ -506,49 +507,64 @@ __global__ void __launch_bounds__(256)
     if (lane == 0) {
         if (CAP) {
             const uint32_t c = a0 + blockIdx.y;
-            wy[(size_t)a * m + c] = s + (a == c ? 1. : 0.);
+            wy[(size_t)a * ld + c] = s + (a == c ? 1. : 0.);
         } else {
-            wy[(size_t)blockIdx.y * m + a] = s;
+            wy[(size_t)blockIdx.y * ld + a] = s;
         }
     }
 }
 
-// z = C^-1 wy, C^-1 row-major: one wave per row a, 8 independent partial sums per lane;
-// z_a is stored at zE[ePos[a]], its place in the by-column order of the update.  blockIdx.y:
-// the right-hand side (m apart in wy and zE).
+// z = C^-1 wy, C^-1 row-major with rows padded to an even length ldc (16-byte aligned): one wave
+// per row a, 16-byte loads (two entries per lane and load, eight loads of the row in flight),
+// 8 partial sums per lane; z_a is stored at zE[ePos[a]], its place in the by-column order of
+// the update.
+// blockIdx.y: the right-hand side (m apart in wy and zE; wy 16-byte aligned, its padding
+// entry zero).
 __global__ void __launch_bounds__(256)
-    k_cap_z(uint32_t m, const double *__restrict__ CinvT, const double *__restrict__ wy,
+    k_cap_z(uint32_t m, uint32_t ldc, const double *__restrict__ CinvT, const double *__restrict__ wy,
             const int32_t *__restrict__ ePos, double *__restrict__ zE)
 {
     const uint32_t a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (a >= m) return;
-    wy += (size_t)blockIdx.y * m;
+    const double2 *w2 = reinterpret_cast<const double2 *>(wy + (size_t)blockIdx.y * ldc);
     zE += (size_t)blockIdx.y * m;
-    const double *row = CinvT + (size_t)a * m;
+    const double2 *row = reinterpret_cast<const double2 *>(CinvT + (size_t)a * ldc);
+    const uint32_t n2 = ldc / 2;
     double p[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
     uint32_t k = lane;
-    for (; k + 7 * 64 < m; k += 8 * 64) {
+    for (; k + 7 * 64 < n2; k += 8 * 64) {
+        double2 c[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) p[u] = fma(row[k + 64 * u], wy[k + 64 * u], p[u]);
+        for (int u = 0; u < 8; ++u) c[u] = row[k + 64 * u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double2 y = w2[k + 64 * u];
+            p[u] = fma(c[u].y, y.y, fma(c[u].x, y.x, p[u]));
+        }
     }
-    for (; k < m; k += 64) p[0] = fma(row[k], wy[k], p[0]);
+    for (; k < n2; k += 64) {
+        const double2 c = row[k], y = w2[k];
+        p[0] = fma(c.x, y.x, p[0]);
+        p[1] = fma(c.y, y.y, p[1]);
+    }
     double s = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
     s = wave_sum(s);
     if (lane == 0) zE[ePos[a]] = s;
 }
 
-// wy[a] = sum_k W[a, k] y[k] with y = R^-1 b in the box (physical) layout: one thread per row a;
-// blockIdx.y: the right-hand side (m apart in wy, nb in y).
+// wy[a] = sum_k W[a, k] y[k] with y = R^-1 b in the box (physical) layout: one thread per row a,
+// rows m..ld-1 (the padding) written zero; blockIdx.y: the right-hand side (ld apart in wy, nb in y).
 __global__ void __launch_bounds__(256)
-    k_wy(uint32_t m, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col, const double *__restrict__ val,
-         const double *__restrict__ y, double *__restrict__ wy, size_t nb)
+    k_wy(uint32_t m, uint32_t ld, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+         const double *__restrict__ val, const double *__restrict__ y, double *__restrict__ wy, size_t nb)
 {
     const uint32_t a = blockIdx.x * 256 + threadIdx.x;
-    if (a >= m) return;
+    if (a >= ld) return;
     y += (size_t)blockIdx.y * nb;
     double s = 0.;
-    for (int32_t k = ptr[a]; k < ptr[a + 1]; ++k) s = fma(val[k], y[col[k]], s);
-    wy[(size_t)blockIdx.y * m + a] = s;
+    if (a < m)
+        for (int32_t k = ptr[a]; k < ptr[a + 1]; ++k) s = fma(val[k], y[col[k]], s);
+    wy[(size_t)blockIdx.y * ld + a] = s;
 }
 
 #define RB(call, where)                                         \
@@ -630,11 +646,11 @@ int launch_cap_wy(const oecl_fastpoisson *h, bool cap, const double *yh, double 
 #define OECL_WY(CQ)                                                                                        \
     do {                                                                                                   \
         if (cap)                                                                                           \
-            hipLaunchKernelGGL((k_cap_wy<CQ, true>), g, b, 0, s, h->m, h->bx, h->wPtr, h->wCol, h->wVal,  \
-                               h->Sx, yh, out, (size_t)h->nb, a0);                                         \
+            hipLaunchKernelGGL((k_cap_wy<CQ, true>), g, b, 0, s, h->m, h->ldc, h->bx, h->wPtr, h->wCol,    \
+                               h->wVal, h->Sx, yh, out, (size_t)h->nb, a0);                                \
         else                                                                                               \
-            hipLaunchKernelGGL((k_cap_wy<CQ, false>), g, b, 0, s, h->m, h->bx, h->wPtr, h->wCol, h->wVal, \
-                               h->Sx, yh, out, (size_t)h->nb, a0);                                         \
+            hipLaunchKernelGGL((k_cap_wy<CQ, false>), g, b, 0, s, h->m, h->ldc, h->bx, h->wPtr, h->wCol,   \
+                               h->wVal, h->Sx, yh, out, (size_t)h->nb, a0);                                \
     } while (0)
     if (c <= 4) OECL_WY(4);
     else if (c <= 8) OECL_WY(8);
@@ -694,6 +710,7 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->by = by;
     h->nb = nb;
     h->m = m;
+    h->ldc = m + (m & 1u);
     h->cyu = cyu;
     h->mask = maskDev;
     h->rCap = 1;
@@ -720,7 +737,8 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
         return OECL_ERR_HIP;
     }
     const uint64_t M = bx + 1ull;
-    const uint64_t bytes = 8ull * ((uint64_t)bx * bx + 4ull * M + 4ull * nb + 2ull * m + (uint64_t)m * m + nnz) +
+    const uint64_t ldc = h->ldc;
+    const uint64_t bytes = 8ull * ((uint64_t)bx * bx + 4ull * M + 4ull * nb + ldc + m + (uint64_t)m * ldc + nnz) +
                            4ull * (m + (m + 1) + nnz + h->nJ + (h->nJ + 1) + 2ull * m) + 1024;
     char *mem = nullptr;
     OECL_HIP(hipMalloc(&mem, bytes), "fastpoisson alloc");
@@ -731,9 +749,10 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->b = (double *)cur; cur += 8ull * nb;
     h->t1 = (double *)cur; cur += 8ull * nb;
     h->t2 = (double *)cur; cur += 8ull * nb;
-    h->wy = (double *)cur; cur += 8ull * m;
+    cur = mem + ((cur - mem + 15) & ~(ptrdiff_t)15);  // wy and C^-1 rows 16-byte aligned
+    h->wy = (double *)cur; cur += 8ull * ldc;
+    h->CinvT = (double *)cur; cur += 8ull * m * ldc;
     h->zE = (double *)cur; cur += 8ull * m;
-    h->CinvT = (double *)cur; cur += 8ull * m * m;
     h->wVal = (double *)cur; cur += 8ull * nnz;
     h->sIdx = (int32_t *)cur; cur += 4ull * m;
     h->wPtr = (int32_t *)cur; cur += 4ull * (m + 1);
@@ -791,6 +810,7 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
         if (chunk > m) chunk = m;
         if (chunk > 65535) chunk = 65535;
         double *G = nullptr;
+        OECL_HIP(hipMemset(h->CinvT, 0, 8ull * m * ldc), "zero C");  // the row padding stays zero
         OECL_HIP(hipMalloc(&G, 8ull * nb * chunk), "cap alloc");
         for (uint32_t a0 = 0; a0 < m; a0 += chunk) {
             const uint32_t cnt = m - a0 < chunk ? m - a0 : chunk;
@@ -806,7 +826,7 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
         if (capDiag) {
             // report max |C_aa| for a scale check
             std::vector<double> hc(m);
-            OECL_HIP(hipMemcpy2D(hc.data(), 8, h->CinvT, 8ull * (m + 1), 8, m, hipMemcpyDeviceToHost), "diag");
+            OECL_HIP(hipMemcpy2D(hc.data(), 8, h->CinvT, 8ull * (ldc + 1), 8, m, hipMemcpyDeviceToHost), "diag");
             double mx = 0.;
             for (uint32_t a = 0; a < m; ++a) mx = fabs(hc[a]) > mx ? fabs(hc[a]) : mx;
             *capDiag = mx;
@@ -816,8 +836,8 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
         rocblas_int *ipiv = nullptr, *info = nullptr;
         OECL_HIP(hipMalloc(&ipiv, 4ull * m), "ipiv");
         OECL_HIP(hipMalloc(&info, 4), "info");
-        RB(rocsolver_dgetrf(h->blas, m, m, h->CinvT, m, ipiv, info), "dgetrf");
-        RB(rocsolver_dgetri(h->blas, m, h->CinvT, m, ipiv, info), "dgetri");
+        RB(rocsolver_dgetrf(h->blas, m, m, h->CinvT, (rocblas_int)ldc, ipiv, info), "dgetrf");
+        RB(rocsolver_dgetri(h->blas, m, h->CinvT, (rocblas_int)ldc, ipiv, info), "dgetri");
         int hinfo = 0;
         OECL_HIP(hipMemcpy(&hinfo, info, 4, hipMemcpyDeviceToHost), "info");
         (void)hipFree(ipiv);
@@ -842,17 +862,17 @@ FpBox fastpoisson_box(const oecl_fastpoisson *h)
 int fastpoisson_reserve(oecl_fastpoisson *h, uint32_t R)
 {
     if (R <= h->rCap) return OECL_OK;
-    const uint64_t nb = h->nb, m = h->m;
+    const uint64_t nb = h->nb, m = h->m, ldc = h->ldc;
     char *mem = nullptr;
-    OECL_HIP(hipMalloc(&mem, 8ull * R * (3 * nb + 2 * m) + 1024), "fastpoisson batch alloc");
+    OECL_HIP(hipMalloc(&mem, 8ull * R * (3 * nb + ldc + m) + 1024), "fastpoisson batch alloc");
     if (h->batchMem) (void)hipFree(h->batchMem);
     h->batchMem = mem;
     h->b = (double *)mem;
     OECL_HIP(hipMemset(h->b, 0, 8ull * R * nb), "zero box");
     h->t1 = h->b + R * nb;
     h->t2 = h->t1 + R * nb;
-    h->wy = h->t2 + R * nb;
-    h->zE = h->wy + R * m;
+    h->wy = h->t2 + R * nb + ((R * nb) & 1u);  // 16-byte aligned (the allocation holds the slack)
+    h->zE = h->wy + R * ldc;
     h->rCap = R;
     return OECL_OK;
 }
@@ -964,9 +984,10 @@ static int solve_after_first(oecl_fastpoisson *h, double *phi, uint32_t R, hipSt
         if (rc != OECL_OK) return rc;
         rc = dst_transform<0>(h, N, h->t2, h->t2, bx, s);
         if (rc != OECL_OK) return rc;
-        hipLaunchKernelGGL(k_wy, dim3((h->m + 255) / 256, R), dim3(256), 0, s, h->m, h->wPtr, h->wCol, h->wVal, h->t2,
-                           h->wy, (size_t)h->nb);
-        hipLaunchKernelGGL(k_cap_z, dim3((h->m + 3) / 4, R), dim3(256), 0, s, h->m, h->CinvT, h->wy, h->ePos, h->zE);
+        hipLaunchKernelGGL(k_wy, dim3((h->ldc + 255) / 256, R), dim3(256), 0, s, h->m, h->ldc, h->wPtr, h->wCol,
+                           h->wVal, h->t2, h->wy, (size_t)h->nb);
+        hipLaunchKernelGGL(k_cap_z, dim3((h->m + 3) / 4, R), dim3(256), 0, s, h->m, h->ldc, h->CinvT, h->wy,
+                           h->ePos, h->zE);
         rc = dst_transform<3>(h, (int)(h->nJ * R), h->zE, h->t1, bx, s);
         if (rc != OECL_OK) return rc;
     }

@@ -328,7 +328,14 @@ struct MixedOut {
     double q00, q10, q01, q11;
 };
 
-template <int TYPEB>
+// CP: the cache-policy bits of the particle stream's loads and stores.  Non-temporal (2) when
+// the particle state is larger than the last-level cache can usefully hold (kStreamBytes): each
+// component is then read and written once a step and only evicts the field copies and rho that
+// the gather and deposit reuse (2e7 particles: 274 -> 265 us a push).  A state that fits stays
+// cached from one step to the next instead (0; 2e6 particles: non-temporal is 4 % slower).
+constexpr uint64_t kStreamBytes = 128ull << 20;
+
+template <int TYPEB, int CP>
 __global__ void __launch_bounds__(MPT)
     k_push_mixed(Runs runs, oecl_grid_t g, WallDev wall, NearWall nw, Push32 P, Soa<float> p,
                  const uint32_t *__restrict__ nDev, const double *__restrict__ E, const float *__restrict__ E32,
@@ -358,8 +365,9 @@ __global__ void __launch_bounds__(MPT)
     const BufRes rE = buf_res(E32 + (size_t)run * 2 * g.np, nb8);
     auto one = [&](uint32_t i, MixedOut &o) {
         const uint32_t oi = i * 4u;
-        const float x0 = buf_ld_f32(rX, oi), y0 = buf_ld_f32(rY, oi), vx0 = buf_ld_f32(rVx, oi);
-        const float vy0 = buf_ld_f32(rVy, oi), vz0 = buf_ld_f32(rVz, oi);
+        constexpr int LC = CP, SC = CP;
+        const float x0 = buf_ld_f32<LC>(rX, oi), y0 = buf_ld_f32<LC>(rY, oi), vx0 = buf_ld_f32<LC>(rVx, oi);
+        const float vy0 = buf_ld_f32<LC>(rVy, oi), vz0 = buf_ld_f32<LC>(rVz, oi);
         // ---- FP32 gather and push: the four corners of a field at byte offsets oc, oc + 8 and
         // the same one row up (scalar offset nx8)
         const Loc32 l = locate32(x0, y0, P.dxi, P.dyi, P.ox, P.oy, g.nx, g.ny);
@@ -425,18 +433,18 @@ __global__ void __launch_bounds__(MPT)
             vy = (float)vyd;
             vz = (float)vzd;
         }
-        buf_st_f32(xs, rX, oi);
-        buf_st_f32(ys, rY, oi);
-        buf_st_f32(vx, rVx, oi);
-        buf_st_f32(vy, rVy, oi);
-        if (TYPEB != 0) buf_st_f32(vz, rVz, oi);
+        buf_st_f32<SC>(xs, rX, oi);
+        buf_st_f32<SC>(ys, rY, oi);
+        buf_st_f32<SC>(vx, rVx, oi);
+        buf_st_f32<SC>(vy, rVy, oi);
+        if (TYPEB != 0) buf_st_f32<SC>(vz, rVz, oi);
         if (isInside) buf_st_u8(in, buf_res(isInside, (uint32_t)p.stride), i);
         if (in) {
             // the deposit of k_push_dev: FP64 weights of the stored position, FP64 charges
             o.dep = true;
             const Weights<double> w2 = grid_weights<double>((double)xs, (double)ys, dxi, dyi, g.lxHalfExt,
                                                             g.lyHalfExt, g.nx, g.ny);
-            const double wq = (double)buf_ld_f32(rW, oi);
+            const double wq = (double)buf_ld_f32<LC>(rW, oi);
             o.cx = w2.ix;
             o.cy = w2.iy;
             o.q00 = w2.wLL * wq * q;
@@ -489,14 +497,14 @@ __device__ __forceinline__ uint32_t block_scan_ct(uint32_t v, uint32_t *wsum, ui
 }
 
 // One block per run: all holes of the run in ascending index order (holePos, the first absCap
-// of them), their count misc[M_HOLES], and the absorbed rows (hole r -> row r).  The push
-// waves that have holes (nzList, usually a few dozen) are sorted in LDS and their lists
-// concatenated in wave order; beyond NZMAX such waves every wave's count is scanned instead.
-template <typename T>
+// of them) and their count misc[M_HOLES] (k_hits_dev copies the absorbed rows, hole r -> row
+// r, over the whole grid).  The push waves that have holes (nzList, usually a few dozen) are
+// sorted in LDS and their lists concatenated in wave order; beyond NZMAX such waves every
+// wave's count is scanned instead.
 __global__ void __launch_bounds__(CT)
-    k_holes_collect(Runs runs, Soa<T> p, const uint32_t *__restrict__ nDev, const uint32_t *__restrict__ counts,
+    k_holes_collect(Runs runs, const uint32_t *__restrict__ nDev, const uint32_t *__restrict__ counts,
                     const uint32_t *__restrict__ list, const uint32_t *__restrict__ nzList,
-                    uint32_t *__restrict__ holePos, T *__restrict__ absorbed, uint32_t *__restrict__ misc)
+                    uint32_t *__restrict__ holePos, uint32_t *__restrict__ misc)
 {
     __shared__ uint32_t keys[NZMAX];
     __shared__ uint32_t wsum[CT / 64];
@@ -505,12 +513,22 @@ __global__ void __launch_bounds__(CT)
     misc += run * M_COUNT;
     nzList += w0;
     holePos += a0;
-    absorbed += 6 * (uint64_t)a0;
     const uint32_t n = nDev[run], nb = (n + 63) / 64, nz = misc[M_NZ];
     const uint32_t tid = threadIdx.x;
     const bool sorted = nz <= NZMAX;
     uint32_t N = 1;
-    if (sorted) {
+    if (nz <= CT) {
+        // one wave index per thread: its place is the number of smaller ones (the indices are
+        // distinct), read from LDS by all threads at once
+        const uint32_t mine = tid < nz ? nzList[tid] : 0u;
+        if (tid < nz) keys[tid] = mine;
+        __syncthreads();
+        uint32_t rank = 0;
+        for (uint32_t k = 0; k < nz; ++k) rank += keys[k] < mine ? 1u : 0u;
+        __syncthreads();
+        if (tid < nz) keys[rank] = mine;
+        __syncthreads();
+    } else if (sorted) {
         while (N < nz) N <<= 1;
         for (uint32_t k = tid; k < N; k += CT) keys[k] = k < nz ? nzList[k] : 0xFFFFFFFFu;
         __syncthreads();
@@ -545,13 +563,6 @@ __global__ void __launch_bounds__(CT)
         carry += tot;
     }
     if (tid == 0) misc[M_HOLES] = carry;
-    __syncthreads();
-    const uint32_t na = carry < absCap ? carry : absCap;
-    for (uint32_t r = tid; r < na; r += CT) {
-        const uint32_t i = holePos[r];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) absorbed[6 * (uint64_t)r + c] = p(c, i);
-    }
 }
 
 __device__ __forceinline__ uint32_t n_hits(const uint32_t *misc, uint32_t absCap)
@@ -676,7 +687,8 @@ __device__ __forceinline__ uint32_t hit_one(uint32_t jj, const WallDev &w, const
 template <typename T>
 __global__ void __launch_bounds__(128)
     k_hits_dev(Runs runs, const uint32_t *__restrict__ hitOff, WallDev w, oecl_furman_t P, oecl_secem_t S,
-               int useSecem, T *ap, T *nv, T *rts, double dt, int emit, uint32_t s0, uint32_t s1,
+               int useSecem, Soa<T> p, const uint32_t *__restrict__ holePos, T *ap, T *nv, T *rts, double dt,
+               int emit, uint32_t s0, uint32_t s1,
                const uint32_t *__restrict__ stepDev, uint32_t *__restrict__ nSec, double *__restrict__ impact,
                const uint32_t *misc, uint32_t *stats)
 {
@@ -685,6 +697,13 @@ __global__ void __launch_bounds__(128)
         const uint32_t run = hitOff ? hit_run(hitOff, runs.R, t) : 0;
         const uint32_t jj = hitOff ? t - hitOff[run] : t, a0 = runs.abs0(run);
         const oecl_furman_t &Pr = runs.furman ? runs.furman[run] : P;
+        {
+            // the absorbed row of hole jj: the particle as the push left it
+            const uint32_t i = holePos[a0 + jj];
+            T *row = ap + 6 * (uint64_t)(a0 + jj);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) row[c] = p(c, i);
+        }
         hit_one<T>(jj, w, Pr, S, useSecem, ap + 6 * (uint64_t)a0, nv + 2 * (uint64_t)a0, rts + a0, dt, emit,
                    runs.key0(run, s0), runs.key1(run, s1), step, nSec + a0, impact + 4 * (uint64_t)a0,
                    stats + 8 * run);
@@ -1091,22 +1110,30 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
                          (float)a->dt, (float)kk, (float)(kk * a->b0), (float)(kk * a->b1), (float)(kk * a->b2)};
         Soa<float> pf{(float *)a->data, a->stride};
         const dim3 gm((unsigned)nBlk), bm(MPT);
+        // 24 bytes of FP32 state per particle slot
+        const bool nt = 24ull * a->stride > kStreamBytes;
+#define OECL_MIXED(TB, ...)                                                       \
+    do {                                                                          \
+        if (nt) hipLaunchKernelGGL((k_push_mixed<TB, 2>), gm, bm, 0, s, __VA_ARGS__); \
+        else hipLaunchKernelGGL((k_push_mixed<TB, 0>), gm, bm, 0, s, __VA_ARGS__);    \
+    } while (0)
         if (a->typeB == 0)
-            hipLaunchKernelGGL((k_push_mixed<0>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
+            OECL_MIXED(0, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
                                a->bField, nullptr, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
                                list, misc, nzList, dc);
         else if (a->typeB == 1)
-            hipLaunchKernelGGL((k_push_mixed<1>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
+            OECL_MIXED(1, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
                                a->bField, nullptr, a->b0, a->b1, a->b2, a->dt, kk, a->qe, dxi, dyi, a->isInside,
                                a->rho, counts, list, misc, nzList, dc);
         else if (eb)
-            hipLaunchKernelGGL((k_push_mixed<3>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
+            OECL_MIXED(3, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
                                a->Bil, a->Bil32, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
                                list, misc, nzList, dc);
         else
-            hipLaunchKernelGGL((k_push_mixed<2>), gm, bm, 0, s, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
+            OECL_MIXED(2, runs, *g, wd, nw, P32, pf, a->nDev, a->E, a->Eil32,
                                a->bField, nullptr, 0., 0., 0., a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts,
                                list, misc, nzList, dc);
+#undef OECL_MIXED
     } else if (a->typeB == 0)
         hipLaunchKernelGGL((k_push_dev<T, 0>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
                            a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList, dc);
@@ -1120,14 +1147,13 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
         hipLaunchKernelGGL((k_push_dev<T, 2>), gp, bp, 0, s, runs, *g, wd, p, a->nDev, a->E, a->Eil, a->bField, 0., 0., 0.,
                            a->dt, kk, a->qe, dxi, dyi, a->isInside, a->rho, counts, list, misc, nzList, dc);
     if (a->evPushEnd) OECL_HIP(hipEventRecord((hipEvent_t)a->evPushEnd, s), "event");
-    hipLaunchKernelGGL(k_holes_collect<T>, dim3(R), dim3(CT), 0, s, runs, p, a->nDev, counts, list, nzList, holePos,
-                       (T *)a->absorbed, misc);
+    hipLaunchKernelGGL(k_holes_collect, dim3(R), dim3(CT), 0, s, runs, a->nDev, counts, list, nzList, holePos, misc);
     const oecl_secem_t S = a->secem ? *a->secem : oecl_secem_t{};
     const int useSecem = a->secem ? 1 : 0;
     if (hitOff) hipLaunchKernelGGL(k_hit_offsets, dim3(1), dim3(CT), 0, s, runs, misc, hitOff);
     const unsigned hitBlocks = grid_cap(absTotal, 128, R > 1 ? 2048 : 512);
-    hipLaunchKernelGGL(k_hits_dev<T>, dim3(hitBlocks), dim3(128), 0, s, runs, hitOff, wd, *P, S, useSecem,
-                       (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, a->emit, a->seed0, a->seed1,
+    hipLaunchKernelGGL(k_hits_dev<T>, dim3(hitBlocks), dim3(128), 0, s, runs, hitOff, wd, *P, S, useSecem, p,
+                       holePos, (T *)a->absorbed, (T *)a->normals, (T *)a->remaining, a->dt, a->emit, a->seed0, a->seed1,
                        a->stepDev, a->nSec, a->impact, misc, a->stats);
     if (a->emit && useSecem)
         hipLaunchKernelGGL(k_secem_emit_dev<T>, dim3(hitBlocks), dim3(128), 0, s, runs, hitOff, S,
