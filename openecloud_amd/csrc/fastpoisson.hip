@@ -77,14 +77,19 @@ namespace {
 // pass reading all of a thread's inputs into registers before a barrier and writing after it,
 // so one buffer of M complex values per line suffices.  Twiddles come from a table of
 // e^{-i pi k / M}, k < 2M, built on the host in extended precision.  Error O(eps log M)
-// against O(eps sqrt(M)) for the dense product.  T threads per line, 256 / T lines per block.
+// against O(eps sqrt(M)) for the dense product.  T threads per line, LB lines per block of NT
+// threads.
 // EPI 0 stores the transform of line l at C + l * ldc; EPI 2 scatters it into the
 // extended-grid potential: line l is row l % by of potential l / by (np nodes apart), only the
 // box columns that are interior grid columns (k < bxg) are written, masked.
+#ifndef OECL_FFT_BIG_T
+#define OECL_FFT_BIG_T 256
+#endif
 template <int M>
 struct FftShape {
-    static constexpr int T = M >= 1024 ? 256 : (M >= 4 ? M / 4 : 1);
-    static constexpr int LB = 256 / T;
+    static constexpr int T = M >= 2048 ? OECL_FFT_BIG_T : (M >= 1024 ? 256 : (M >= 4 ? M / 4 : 1));
+    static constexpr int LB = T >= 256 ? 1 : 256 / T;
+    static constexpr int NT = T * LB;
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
@@ -109,6 +114,7 @@ template <int M, int R, int NS>
 __device__ __forceinline__ void fft_pass(double2 *z, const double *v, uint32_t t, const double2 *__restrict__ tab)
 {
     constexpr int T = FftShape<M>::T, J = M / (R * T);
+    static_assert(J >= 1 && J * R * T == M, "every thread takes whole butterflies of each pass");
     double2 x[J][R];
 #pragma unroll
     for (int q = 0; q < J; ++q) {
@@ -181,7 +187,7 @@ struct FftArgs {
 // from row jCol[c] of C: hat b -= Sx U z in the transformed space.  X and C may be the same
 // buffer (EPI 0 in place): a line is read whole into LDS before any of it is written.
 template <int M, int EPI>
-__global__ void __launch_bounds__(256) k_dst_fft(FftArgs g)
+__global__ void __launch_bounds__(FftShape<M>::NT) k_dst_fft(FftArgs g)
 {
     constexpr int T = FftShape<M>::T, LB = FftShape<M>::LB, n = M - 1;
     __shared__ double2 sh[LB * M];
@@ -939,7 +945,7 @@ static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C,
 #define OECL_FFT(MM)                                                                                    \
     case MM:                                                                                          \
         hipLaunchKernelGGL((k_dst_fft<MM, EPI>), dim3((N + FftShape<MM>::LB - 1) / FftShape<MM>::LB), \
-                           dim3(256), 0, s, g);                                                       \
+                           dim3(FftShape<MM>::NT), 0, s, g);                                          \
         return OECL_OK;
     switch (h->bx + 1) {
         OECL_FFT(2) OECL_FFT(4) OECL_FFT(8) OECL_FFT(16) OECL_FFT(32) OECL_FFT(64) OECL_FFT(128) OECL_FFT(256)
