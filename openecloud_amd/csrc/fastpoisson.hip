@@ -59,6 +59,7 @@ struct oecl_fastpoisson {
     const uint8_t *mask;      // inside flags of the extended grid (caller owned)
     uint32_t rCap;            // right-hand sides the vectors above hold
     char *batchMem;           // their own allocation once rCap > 1
+    uint32_t pipeGrid[5];     // blocks of k_dst_fft_pipe<EPI> that are resident at once (0: not used)
 };
 
 namespace {
@@ -107,6 +108,64 @@ __device__ __forceinline__ double2 packed_odd(const double *v, uint32_t m)
     return make_double2(y0, y1);
 }
 
+// LDS place of complex entry e of a line: the low 3 bits XOR-ed with bits 3-5, a permutation
+// within each block of 64 entries that spreads the strided writes of a pass (entries R or
+// 8 R apart in consecutive lanes) over all 8 16-byte bank groups, while the contiguous reads
+// stay spread too.
+#ifndef OECL_FFT_SWIZZLE
+#define OECL_FFT_SWIZZLE 1
+#endif
+__device__ __forceinline__ uint32_t zsw(uint32_t e)
+{
+    return OECL_FFT_SWIZZLE ? e ^ ((e >> 3) & 7u) : e;
+}
+
+// 4-point DFT (e^{-2 pi i / 4} = -i) of a, b, c, d into X[0..3].
+__device__ __forceinline__ void dft4(double2 a, double2 b, double2 c, double2 d, double2 *X)
+{
+    const double2 t0 = make_double2(a.x + c.x, a.y + c.y), t1 = make_double2(a.x - c.x, a.y - c.y);
+    const double2 t2 = make_double2(b.x + d.x, b.y + d.y), t3 = make_double2(b.x - d.x, b.y - d.y);
+    X[0] = make_double2(t0.x + t2.x, t0.y + t2.y);
+    X[1] = make_double2(t1.x + t3.y, t1.y - t3.x);
+    X[2] = make_double2(t0.x - t2.x, t0.y - t2.y);
+    X[3] = make_double2(t1.x - t3.y, t1.y + t3.x);
+}
+
+// In-place R-point DFT (R = 2, 4, 8) of x[0..R-1].
+template <int R>
+__device__ __forceinline__ void bfly(double2 *x)
+{
+    if constexpr (R == 8) {
+        // two 4-point DFTs of the even and odd inputs, combined with w8^k = e^{-i pi k / 4}
+        double2 e[4], o[4];
+        dft4(x[0], x[2], x[4], x[6], e);
+        dft4(x[1], x[3], x[5], x[7], o);
+        constexpr double h = 0.70710678118654752440;   // sqrt(1/2)
+        const double2 o1 = make_double2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));   // (1 - i) h o1
+        const double2 o2 = make_double2(o[2].y, -o[2].x);                                  // -i o2
+        const double2 o3 = make_double2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));  // (-1 - i) h o3
+        x[0] = make_double2(e[0].x + o[0].x, e[0].y + o[0].y);
+        x[4] = make_double2(e[0].x - o[0].x, e[0].y - o[0].y);
+        x[1] = make_double2(e[1].x + o1.x, e[1].y + o1.y);
+        x[5] = make_double2(e[1].x - o1.x, e[1].y - o1.y);
+        x[2] = make_double2(e[2].x + o2.x, e[2].y + o2.y);
+        x[6] = make_double2(e[2].x - o2.x, e[2].y - o2.y);
+        x[3] = make_double2(e[3].x + o3.x, e[3].y + o3.y);
+        x[7] = make_double2(e[3].x - o3.x, e[3].y - o3.y);
+    } else if constexpr (R == 4) {
+        double2 X[4];
+        dft4(x[0], x[1], x[2], x[3], X);
+        x[0] = X[0];
+        x[1] = X[1];
+        x[2] = X[2];
+        x[3] = X[3];
+    } else {
+        const double2 a = x[0], b = x[1];
+        x[0] = make_double2(a.x + b.x, a.y + b.y);
+        x[1] = make_double2(a.x - b.x, a.y - b.y);
+    }
+}
+
 // One Stockham pass of radix R at stride NS: out[(j / NS) NS R + j % NS + r NS] =
 // DFT_R(in[j + r M / R] w^{r (j % NS)}), w = e^{-2 pi i / (NS R)}; the first pass (NS = 1)
 // reads the packed odd extension of v instead of z.
@@ -120,25 +179,13 @@ __device__ __forceinline__ void fft_pass(double2 *z, const double *v, uint32_t t
     for (int q = 0; q < J; ++q) {
         const uint32_t j = t + q * T;
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[q][r] = NS == 1 ? packed_odd<M>(v, j + r * (M / R)) : z[j + r * (M / R)];
+        for (int r = 0; r < R; ++r) x[q][r] = NS == 1 ? packed_odd<M>(v, j + r * (M / R)) : z[zsw(j + r * (M / R))];
         if (NS > 1) {
             const uint32_t jm = j & (NS - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) x[q][r] = cmul(x[q][r], tab[2 * r * jm * (M / (NS * R))]);
         }
-        if (R == 4) {
-            const double2 a = x[q][0], b = x[q][1], c = x[q][2], d = x[q][3];
-            const double2 t0 = make_double2(a.x + c.x, a.y + c.y), t1 = make_double2(a.x - c.x, a.y - c.y);
-            const double2 t2 = make_double2(b.x + d.x, b.y + d.y), t3 = make_double2(b.x - d.x, b.y - d.y);
-            x[q][0] = make_double2(t0.x + t2.x, t0.y + t2.y);
-            x[q][1] = make_double2(t1.x + t3.y, t1.y - t3.x);
-            x[q][2] = make_double2(t0.x - t2.x, t0.y - t2.y);
-            x[q][3] = make_double2(t1.x - t3.y, t1.y + t3.x);
-        } else {
-            const double2 a = x[q][0], b = x[q][1];
-            x[q][0] = make_double2(a.x + b.x, a.y + b.y);
-            x[q][1] = make_double2(a.x - b.x, a.y - b.y);
-        }
+        bfly<R>(x[q]);
     }
     __syncthreads();
 #pragma unroll
@@ -146,16 +193,24 @@ __device__ __forceinline__ void fft_pass(double2 *z, const double *v, uint32_t t
         const uint32_t j = t + q * T;
         const uint32_t o = (j / NS) * NS * R + (j & (NS - 1));
 #pragma unroll
-        for (int r = 0; r < R; ++r) z[o + r * NS] = x[q][r];
+        for (int r = 0; r < R; ++r) z[zsw(o + r * NS)] = x[q][r];
     }
     __syncthreads();
 }
 
+#ifndef OECL_FFT_RADIX8
+#define OECL_FFT_RADIX8 0
+#endif
+// Radix 4 passes, then one radix 2 (with OECL_FFT_RADIX8, radix 8 first while 8 divides what is
+// left and every thread takes whole 8-point butterflies: fewer passes, but measured 5 % slower
+// for M = 2048 in this one-line-per-block form).
 template <int M, int NS>
 __device__ __forceinline__ void fft_passes(double2 *z, const double *v, uint32_t t, const double2 *__restrict__ tab)
 {
     if constexpr (NS < M) {
-        constexpr int R = (M / NS) % 4 == 0 ? 4 : 2;
+        constexpr int T = FftShape<M>::T;
+        constexpr int R = (OECL_FFT_RADIX8 && (M / NS) % 8 == 0 && M % (8 * T) == 0) ? 8
+                          : ((M / NS) % 4 == 0 ? 4 : 2);
         fft_pass<M, R, NS>(z, v, t, tab);
         fft_passes<M, NS * R>(z, v, t, tab);
     }
@@ -245,7 +300,7 @@ __global__ void __launch_bounds__(FftShape<M>::NT) k_dst_fft(FftArgs g)
     }
     if (EPI == 3) out += (size_t)r * g.nb + (size_t)g.jCol[c] * g.ldc;
     for (uint32_t k = t; k < (uint32_t)n; k += T) {
-        const double2 a = z[k + 1], b = z[M - 1 - k], w = g.tab[k + 1];
+        const double2 a = z[zsw(k + 1)], b = z[zsw(M - 1 - k)], w = g.tab[k + 1];
         // w = (cos(pi (k+1)/M), -sin(pi (k+1)/M))
         const double s = sc * (((a.y - b.y) - w.x * (a.x - b.x)) + w.y * (a.y + b.y));
         if (EPI == 0 || EPI == 4) {
@@ -259,6 +314,215 @@ __global__ void __launch_bounds__(FftShape<M>::NT) k_dst_fft(FftArgs g)
     }
 }
 
+
+// ---- the persistent, pipelined form for M = 2048 (256 threads a line, 8 entries a thread)
+// Block b transforms lines b, b + G, b + 2G, ... (G = gridDim.x).  The next line's entries are
+// loaded into registers while the current line is transformed, so that a line's HBM latency
+// hides behind the previous line's passes.  For that the passes issue no global load: their
+// twiddles are powers of one base per pass and butterfly (w^2 = w w, ...: a few ulp from the
+// table values), and the epilogue's e^{-i pi (k+1)/M} for k = t + i T are e^{-i pi (t+1)/M}
+// times i factors e^{-i pi T/M} = e^{-i pi / 8}, all formed from table entries loaded before the
+// first line.  Radix 8, 8, 8, 4.
+#ifndef OECL_FFT_PIPE
+#define OECL_FFT_PIPE 1
+#endif
+
+template <int R>
+__device__ __forceinline__ void apply_powers(double2 *x, double2 w)
+{
+    if constexpr (R == 2) {
+        x[1] = cmul(x[1], w);
+    } else if constexpr (R == 4) {
+        const double2 w2 = cmul(w, w), w3 = cmul(w2, w);
+        x[1] = cmul(x[1], w);
+        x[2] = cmul(x[2], w2);
+        x[3] = cmul(x[3], w3);
+    } else {
+        const double2 w2 = cmul(w, w), w3 = cmul(w2, w), w4 = cmul(w2, w2);
+        x[1] = cmul(x[1], w);
+        x[2] = cmul(x[2], w2);
+        x[3] = cmul(x[3], w3);
+        x[4] = cmul(x[4], w4);
+        x[5] = cmul(x[5], cmul(w4, w));
+        x[6] = cmul(x[6], cmul(w4, w2));
+        x[7] = cmul(x[7], cmul(w4, w3));
+    }
+}
+
+// A pass as fft_pass, its twiddle bases bw[q] (q < J) given.
+template <int M, int T, int R, int NS>
+__device__ __forceinline__ void fft_pass_reg(double2 *z, const double *v, uint32_t t, const double2 *bw)
+{
+    constexpr int J = M / (R * T);
+    static_assert(J >= 1 && J * R * T == M, "every thread takes whole butterflies of each pass");
+    double2 x[J][R];
+#pragma unroll
+    for (int q = 0; q < J; ++q) {
+        const uint32_t j = t + q * T;
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[q][r] = NS == 1 ? packed_odd<M>(v, j + r * (M / R)) : z[zsw(j + r * (M / R))];
+        if (NS > 1) apply_powers<R>(x[q], bw[q]);
+        bfly<R>(x[q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < J; ++q) {
+        const uint32_t j = t + q * T;
+        const uint32_t o = (j / NS) * NS * R + (j & (NS - 1));
+#pragma unroll
+        for (int r = 0; r < R; ++r) z[zsw(o + r * NS)] = x[q][r];
+    }
+    __syncthreads();
+}
+
+// EPI 0, 2, 3 and 4 as k_dst_fft.
+template <int EPI>
+__global__ void __launch_bounds__(256) k_dst_fft_pipe(FftArgs g)
+{
+    constexpr int M = 2048, T = 256, P = M / T, n = M - 1;
+    __shared__ double2 z[M];
+    double *v = reinterpret_cast<double *>(z) + M;
+    const uint32_t t = threadIdx.x;
+    const uint32_t step = EPI == 4 ? *g.stepDev : 0u;
+    const double sc0 = -0.25 * sqrt(2. / M);
+    // a line's inputs as loaded, one line ahead: pv the entries (EPI 0, 2: X; EPI 4: rho; EPI 3:
+    // the row of C it updates), pp the beam profile (EPI 4), pm the inside flags (EPI 2, 4);
+    // EPI 3: this thread's sparse entry (column pi, value pz) among the line's ne
+    double pv[P], pp[P];
+    uint32_t pm[P];
+    double pscale = 0., pz = 0.;
+    uint32_t pi = 0, ne = 0;
+    auto fetch = [&](uint32_t line) {
+        if (EPI == 4) {
+            const uint32_t rr = line / g.by, jj = line - rr * g.by;
+            pscale = (g.prof && g.table && step < g.nScale) ? g.table[(size_t)rr * g.nScale + step] : 0.;
+            double *rrow = g.rho + (size_t)rr * g.np + (size_t)(jj + 2) * g.nx;
+            const double *prow = g.prof ? g.prof + (size_t)(jj + 2) * g.nx : nullptr;
+            const uint8_t *mrow = g.mask + (size_t)(jj + 2) * g.nx;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const uint32_t k = t + i * T;
+                pv[i] = 0.;
+                pp[i] = 0.;
+                pm[i] = 0u;
+                if (k < g.bxg) {
+                    pv[i] = rrow[k + 2];
+                    pm[i] = mrow[k + 2];
+                    if (prow && pscale != 0.) pp[i] = prow[k + 2];
+                    rrow[k + 2] = 0.;
+                }
+            }
+            if (t < 4) rrow[t < 2 ? t : g.nx - 4 + t] = 0.;   // grid columns 0, 1, nx-2, nx-1
+        } else if (EPI == 3) {
+            const uint32_t r = line / g.nJ, c = line - r * g.nJ;
+            const int32_t e0 = g.jStart[c], e1 = g.jStart[c + 1];
+            ne = (uint32_t)(e1 - e0);
+            if (t < ne) {
+                pi = g.iPos[e0 + t];
+                pz = g.X[(size_t)r * g.m + e0 + t];
+            }
+            const double *orow = g.C + (size_t)r * g.nb + (size_t)g.jCol[c] * g.ldc;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const uint32_t k = t + i * T;
+                pv[i] = k < (uint32_t)n ? orow[k] : 0.;
+            }
+        } else {
+            const double *xl = g.X + (size_t)line * g.ldx;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const uint32_t k = t + i * T;
+                pv[i] = k < (uint32_t)n ? xl[k] : 0.;
+            }
+            if (EPI == 2) {
+                const uint32_t jj = line - (line / g.by) * g.by;
+                const uint8_t *mrow = g.mask + (size_t)(jj + 2) * g.nx + 2;
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const uint32_t k = t + i * T;
+                    pm[i] = k < g.bxg ? mrow[k] : 0u;
+                }
+            }
+        }
+    };
+    const uint32_t G = gridDim.x;
+    if (blockIdx.x < g.nLines) fetch(blockIdx.x);
+    for (uint32_t line = blockIdx.x; line < g.nLines; line += G) {
+        // the thread index as an opaque value in each iteration: the LDS addresses derived from
+        // it are formed again in each line rather than held in registers across the loop
+        uint32_t t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        // the pass twiddle bases e^{-2 pi i jm / (NS R)} = tab[2 jm M / (NS R)] (jm = j % NS)
+        // and the epilogue's e^{-i pi (t+1)/M}, loaded in each line before its prefetch (L2 hits;
+        // the passes wait for them, not for the prefetch behind them)
+        const double2 b8[1] = {g.tab[64 * (t & 7u)]}, b64[1] = {g.tab[8 * (t & 63u)]};
+        const double2 b512[2] = {g.tab[2 * t], g.tab[2 * (t + 256u)]}, eb = g.tab[t + 1];
+        // this line's values the epilogue needs (the prefetch below overwrites the registers)
+        double co[EPI == 3 ? P : 1];
+        uint32_t cm[EPI == 2 ? P : 1];
+        if (EPI == 3) {
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const uint32_t k = t + i * T;
+                co[i] = pv[i];
+                if (k < (uint32_t)n) v[k] = 0.;
+            }
+            __syncthreads();
+            if (t < ne) v[pi] = pz;
+            if (ne > (uint32_t)T) {   // block-uniform: the entries beyond one per thread
+                const uint32_t r = line / g.nJ, c = line - r * g.nJ;
+                const double *ze = g.X + (size_t)r * g.m;
+                for (int32_t e = g.jStart[c] + T + (int32_t)t; e < g.jStart[c + 1]; e += T) v[g.iPos[e]] = ze[e];
+            }
+        } else {
+            const double scl = pscale;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const uint32_t k = t + i * T;
+                double q = pv[i];
+                if (EPI == 4) q = pm[i] ? (scl != 0. ? pv[i] + scl * pp[i] : pv[i]) : 0.;
+                if (k < (uint32_t)n) v[k] = q;
+                if (EPI == 2) cm[i] = pm[i];
+            }
+        }
+        __syncthreads();
+        if (line + G < g.nLines) fetch(line + G);
+        fft_pass_reg<M, T, 8, 1>(z, v, t, nullptr);
+        fft_pass_reg<M, T, 8, 8>(z, v, t, b8);
+        fft_pass_reg<M, T, 8, 64>(z, v, t, b64);
+        fft_pass_reg<M, T, 4, 512>(z, v, t, b512);
+        double *out = g.C;
+        uint32_t jj = 0;
+        if (EPI == 0 || EPI == 4) {
+            out += (size_t)line * g.ldc;
+        } else if (EPI == 3) {
+            const uint32_t r = line / g.nJ, c = line - r * g.nJ;
+            out += (size_t)r * g.nb + (size_t)g.jCol[c] * g.ldc;
+        } else {
+            const uint32_t run = line / g.by;
+            jj = line - run * g.by;
+            out += (size_t)run * g.np;
+        }
+        double2 w = eb;   // e^{-i pi (k+1)/M}, advanced by e^{-i pi / 8} from one k to the next
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const uint32_t k = t + i * T;
+            if (i) w = cmul(w, make_double2(0.92387953251128675613, -0.38268343236508977173));
+            if (k < (uint32_t)n) {
+                const double2 a = z[zsw(k + 1)], b = z[zsw(M - 1 - k)];
+                const double s = sc0 * (((a.y - b.y) - w.x * (a.x - b.x)) + w.y * (a.y + b.y));
+                if (EPI == 0 || EPI == 4) {
+                    out[k] = s;
+                } else if (EPI == 3) {
+                    out[k] = co[i] - s;
+                } else if (k < g.bxg) {
+                    out[(k + 2) + (jj + 2) * g.nx] = cm[i] ? s : 0.;
+                }
+            }
+        }
+        __syncthreads();   // the line's LDS reads are done before the next line is staged
+    }
+}
 
 // b = masked charge on the box; the potential outside the box is zero.
 __global__ void k_gather_box_zero(uint32_t nx, uint32_t ny, uint32_t bx, const uint8_t *__restrict__ mask,
@@ -721,6 +985,20 @@ OECL_EXPORT int oecl_fastpoisson_create(uint32_t nx, uint32_t ny, double cxu, do
     h->mask = maskDev;
     h->rCap = 1;
     h->batchMem = nullptr;
+    for (int e = 0; e < 5; ++e) h->pipeGrid[e] = 0;
+    if (OECL_FFT_PIPE && bx + 1 == 2048) {
+        int dev = 0, cus = 0, b0 = 0, b2 = 0, b3 = 0, b4 = 0;
+        OECL_HIP(hipGetDevice(&dev), "device");
+        OECL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+        OECL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, k_dst_fft_pipe<0>, 256, 0), "occupancy");
+        OECL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_dst_fft_pipe<2>, 256, 0), "occupancy");
+        OECL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b3, k_dst_fft_pipe<3>, 256, 0), "occupancy");
+        OECL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b4, k_dst_fft_pipe<4>, 256, 0), "occupancy");
+        h->pipeGrid[0] = (uint32_t)(cus * b0);
+        h->pipeGrid[2] = (uint32_t)(cus * b2);
+        h->pipeGrid[3] = (uint32_t)(cus * b3);
+        h->pipeGrid[4] = (uint32_t)(cus * b4);
+    }
     // rows a grouped by their box column j (stable), for the per-column update
     std::vector<int32_t> jA(m), jCol, jStart;
     for (uint32_t a = 0; a < m; ++a) jA[a] = (int32_t)a;
@@ -941,6 +1219,12 @@ static int dst_transform(oecl_fastpoisson *h, int N, const double *X, double *C,
         g.table = gg->table;
         g.nScale = gg->nScale;
         g.stepDev = gg->stepDev;
+    }
+    if (h->bx + 1 == 2048 && h->pipeGrid[EPI] && N > 0) {
+        // as many blocks as are resident at once, each taking every G-th line
+        const uint32_t G = (uint32_t)N < h->pipeGrid[EPI] ? (uint32_t)N : h->pipeGrid[EPI];
+        hipLaunchKernelGGL((k_dst_fft_pipe<EPI>), dim3(G), dim3(256), 0, s, g);
+        return OECL_OK;
     }
 #define OECL_FFT(MM)                                                                                    \
     case MM:                                                                                          \
