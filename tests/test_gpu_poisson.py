@@ -64,6 +64,29 @@ def test_direct_alone_matches_lu(args):
     assert ps.residual(q) <= 1e-9
 
 
+def test_direct_2048_wide_box_rows_beyond_one_per_thread():
+    """A flat 2048 x 64 ellipse: the box is 2047 columns wide (the persistent 2048-point DST
+    passes), and its top and bottom rows hold more corrected nodes than the sparse-update pass
+    has threads (256), so that pass also scatters the entries beyond one per thread."""
+    from openecloud_amd.grid import Grid
+    from openecloud_amd.poissonSolver import PoissonSolver
+    from oracle.grid_oracle import OraclePoisson
+    import openecloud_amd._lib as L
+    g = Grid(2048, 64, 0.09, 0.003, 1)
+    ps = PoissonSolver(g, method='direct', checkEvery=0)
+    bx = int(L.lib().oecl_fastpoisson_box_width(L.u32(ps.nx)))
+    assert bx + 1 == 2048
+    sIdx = ps._fp_keep[0]
+    assert np.bincount(sIdx // bx).max() > 256
+    q = _rhs(g, 5)
+    ps.solvePhi(torch.as_tensor(q).cuda())
+    phi = ps.getPhi().cpu().numpy()
+    ref = OraclePoisson(g)
+    ref.solve(q)
+    assert np.linalg.norm(phi - ref.phi) <= 1e-10 * np.linalg.norm(ref.phi)
+    assert ps.residual(q) <= 1e-9
+
+
 def test_pcg_warm_start_and_reproducible():
     from openecloud_amd.grid import Grid
     from openecloud_amd.poissonSolver import PoissonSolver
