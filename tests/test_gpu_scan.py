@@ -14,10 +14,10 @@ SEY = (1.3, 1.6, 2.0)
 PPB = (5e10, 1e11, 2e11)
 
 
-def _shared(dtype=torch.float64):
+def _shared(dtype=torch.float64, grid=(64, 48, 0.09, 0.07, 1)):
     from openecloud_amd.grid import Grid
     from openecloud_amd.poissonSolver import PoissonSolver
-    g = Grid(64, 48, 0.09, 0.07, 1)
+    g = Grid(*grid)
     ps = PoissonSolver(g, method='direct', checkEvery=0)
     return g, ps
 
@@ -90,6 +90,21 @@ def test_scan_runs_match_standalone_native_steps():
     assert st['counts'] == [ref[r][-1].shape[0] for r in range(3)]
     # the per-run histories hold each step's starting count
     assert sc.history(2, 1, nsteps) == [ref[2][k].shape[0] for k in range(nsteps - 1)]
+
+
+def test_scan_runs_match_standalone_on_a_2048_wide_box():
+    """The same on a flat 2048 x 32 ellipse, whose 2047-column Poisson box takes the persistent
+    2048-point DST passes: the three runs' lines in one launch of each pass."""
+    g, ps = _shared(grid=(2048, 32, 0.09, 0.0015, 1))
+    nsteps = 3
+    ref = _alone(g, ps, nsteps)
+    sc = _scan(g, ps)
+    for it in range(nsteps):
+        sc.step()
+        sc.sync()
+        for r in range(3):
+            got = sc.run_particles(r).cpu().numpy()
+            assert _close(got, ref[r][it], 1e-12), (it, r)
 
 
 def test_scan_graph_replays_match_eager_steps():
