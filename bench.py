@@ -60,7 +60,7 @@ def push_model(dtype, arith, gridB):
 
 
 TRAFFIC_PROFILES = {('dipole', 'f64', 2000000): 'push_hbm_traffic_ellip512x384_2e6_f64.md',
-                    ('scan', 'fp64', 64): 'push_hbm_traffic_scan_64runs_256sq_f32.md',
+                    ('scan', 'mixed', 64): 'push_hbm_traffic_scan_64runs_256sq_f32.md',
                     ('push', 'fp64', 20000000): 'push_hbm_traffic_rect2049sq_2e7_f32_quadrupole.md',
                     ('push', 'mixed', 20000000): 'push_mixed_hbm_traffic_rect2049sq_2e7_f32_quadrupole.md'}
 
