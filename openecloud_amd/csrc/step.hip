@@ -962,11 +962,11 @@ __global__ void k_spmv_runs(uint32_t nRows, const int32_t *__restrict__ rowPtr, 
 // summed as the CSR rows are (0 + first product, then the second); empty rows give 0; listed
 // rows are left to k_e_listed.  One thread per node, blockIdx.y the run; also into the
 // interleaved copy yil ((Ex, Ey) per node) when given, and into its FP32 form yil32 when given.
-__global__ void k_e_struct(uint32_t np, uint32_t nx, const uint8_t *__restrict__ kind, double cx, double cy,
-                           const double *__restrict__ phi, double *__restrict__ E, double *__restrict__ yil,
-                           float *__restrict__ yil32)
+__device__ __forceinline__ void e_struct_node(uint32_t r, uint32_t np, uint32_t nx, const uint8_t *__restrict__ kind,
+                                              double cx, double cy, const double *__restrict__ phi,
+                                              double *__restrict__ E, double *__restrict__ yil,
+                                              float *__restrict__ yil32)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= np) return;
     phi += (size_t)blockIdx.y * np;
     E += (size_t)blockIdx.y * 2 * np;
@@ -1004,12 +1004,12 @@ __global__ void k_e_struct(uint32_t np, uint32_t nx, const uint8_t *__restrict__
 }
 
 // The listed rows of phiToE from the CSR (one thread per listed row, blockIdx.y the run).
-__global__ void k_e_listed(uint32_t nList, const int32_t *__restrict__ rows, const int32_t *__restrict__ rowPtr,
-                           const int32_t *__restrict__ col, const double *__restrict__ val,
-                           const double *__restrict__ phi, uint32_t np, double *__restrict__ E,
-                           double *__restrict__ yil, uint32_t ilw, float *__restrict__ yil32)
+__device__ __forceinline__ void e_listed_row(uint32_t t, uint32_t nList, const int32_t *__restrict__ rows,
+                                             const int32_t *__restrict__ rowPtr, const int32_t *__restrict__ col,
+                                             const double *__restrict__ val, const double *__restrict__ phi,
+                                             uint32_t np, double *__restrict__ E, double *__restrict__ yil,
+                                             uint32_t ilw, float *__restrict__ yil32)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nList) return;
     const uint32_t r = (uint32_t)rows[t];
     phi += (size_t)blockIdx.y * np;
@@ -1019,6 +1019,21 @@ __global__ void k_e_listed(uint32_t nList, const int32_t *__restrict__ rows, con
     const size_t o = (size_t)blockIdx.y * ilw * np + ilw * (r % np) + r / np;
     if (yil) yil[o] = sum;
     if (yil32) yil32[o] = (float)sum;
+}
+
+// E = phiToE phi in one launch: blocks below sBlocks take the nodes by row class, the others
+// the listed rows (the two write disjoint components, so their order does not matter).
+__global__ void __launch_bounds__(256)
+    k_e_rows(uint32_t sBlocks, uint32_t np, uint32_t nx, const uint8_t *__restrict__ kind, double cx, double cy,
+             uint32_t nList, const int32_t *__restrict__ rows, const int32_t *__restrict__ rowPtr,
+             const int32_t *__restrict__ col, const double *__restrict__ val, const double *__restrict__ phi,
+             double *__restrict__ E, double *__restrict__ yil, uint32_t ilw, float *__restrict__ yil32)
+{
+    if (blockIdx.x < sBlocks)
+        e_struct_node(blockIdx.x * 256 + threadIdx.x, np, nx, kind, cx, cy, phi, E, yil, yil32);
+    else
+        e_listed_row((blockIdx.x - sBlocks) * 256 + threadIdx.x, nList, rows, rowPtr, col, val, phi, np, E, yil, ilw,
+                     yil32);
 }
 
 }  // namespace
@@ -1098,11 +1113,10 @@ static int step_typed(const oecl_grid_t *g, const WallDev &wd, const oecl_furman
     float *il32 = mixed ? a->Eil32 : nullptr;
     const uint32_t ilw = 2u;
     if (a->e2nKind) {
-        hipLaunchKernelGGL(k_e_struct, dim3(oecl_blocks(np, 256), R), dim3(256), 0, s, np, g->nx, a->e2nKind,
-                           a->e2nCx, a->e2nCy, a->phi, a->E, il, il32);
-        if (a->nIrr)
-            hipLaunchKernelGGL(k_e_listed, dim3(oecl_blocks(a->nIrr, 256), R), dim3(256), 0, s, a->nIrr, a->e2nIrr,
-                               a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw, il32);
+        const uint32_t sBlocks = oecl_blocks(np, 256), lBlocks = a->nIrr ? oecl_blocks(a->nIrr, 256) : 0u;
+        hipLaunchKernelGGL(k_e_rows, dim3(sBlocks + lBlocks, R), dim3(256), 0, s, sBlocks, np, g->nx, a->e2nKind,
+                           a->e2nCx, a->e2nCy, a->nIrr, a->e2nIrr, a->e2nRowPtr, a->e2nCol, a->e2nVal, a->phi, a->E,
+                           il, ilw, il32);
     } else {
         hipLaunchKernelGGL(k_spmv_runs, dim3(oecl_blocks(2 * np, 256), R), dim3(256), 0, s, 2 * np, a->e2nRowPtr,
                            a->e2nCol, a->e2nVal, a->phi, np, a->E, il, ilw, il32);
